@@ -1,0 +1,244 @@
+"""Initial conditions and point sets of the benchmark problems (host side, numpy).
+
+1D problems restate luna-cpp's src/hydro/problems_1d.hpp (LUNA's setups): Sod (Toro 1, :212), Shu-Osher (:234-237,
+shock_into_wave :46-51), Woodward-Colella (:242, :55-59), the linear sound wave (:271, eigenmode :69-74), as exact
+cell averages of the conserved state per unit length. 2D/3D setups are the benchmark configurations of this project:
+jittered lattices (the convention of src/hydro/problems_3d.hpp:36-54, uniform jitter in [-j, j] spacings per axis),
+the Sedov blast, the Kelvin-Helmholtz shear layer and the smooth acoustic wave.
+"""
+import math
+from dataclasses import dataclass, field
+from typing import Callable, Tuple
+
+import numpy as np
+
+GRID_BITS = 30  # positions snap to a grid of 2^-30 of the longest side
+
+
+@dataclass
+class Problem1D:
+    L: float
+    origin: float
+    t_end: float
+    gamma: float
+    walls: Tuple[str, str]
+    average: Callable  # (a, b) -> conserved state per unit length, averaged over [a, b]
+    periodic: bool = False
+
+
+def _conserved1(W, gamma):
+    # problems_1d.hpp:31-33 (its own operation order)
+    return (W[0], W[0] * W[1], W[2] / (gamma - 1) + 0.5 * W[0] * W[1] * W[1])
+
+
+def _fraction_below(x, a, b):
+    return min(max((x - a) / (b - a), 0.0), 1.0)
+
+
+def _tube(L, origin, t_end, gamma, left, right, x0, walls):
+    l, r = _conserved1(left, gamma), _conserved1(right, gamma)
+
+    def average(a, b):
+        f = _fraction_below(x0, a, b)
+        return tuple(f * l[k] + (1 - f) * r[k] for k in range(3))
+
+    return Problem1D(L, origin, t_end, gamma, walls, average)
+
+
+def _shock_into_wave(a, b, left, x0, amp, k, gamma):
+    h = b - a
+    f = _fraction_below(x0, a, b)
+    lo = max(a, x0)
+    rho = f * left[0] + (1 - f)
+    if b > lo:
+        rho += amp * (math.cos(k * lo) - math.cos(k * b)) / k / h
+    return (rho, f * left[0] * left[1],
+            f * (left[2] / (gamma - 1) + 0.5 * left[0] * left[1] * left[1]) + (1 - f) / (gamma - 1))
+
+
+def _woodward_colella(a, b):
+    f1, f2 = _fraction_below(0.1, a, b), _fraction_below(0.9, a, b)
+    p = f1 * 1000 + (f2 - f1) * 0.01 + (1 - f2) * 100
+    return (1.0, 0.0, p / (1.4 - 1))
+
+
+def _eigenmode(a, b, eps, v0, drho, dv, dp):
+    k, G = 2 * math.pi, 5.0 / 3.0
+    P0 = 1 / G
+    h = b - a
+    half = 0.5 * k * h
+    s = math.sin(k * 0.5 * (a + b)) * math.sin(half) / half
+    return (1 + eps * drho * s, v0 + eps * (dv + v0 * drho) * s,
+            P0 / (G - 1) + 0.5 * v0 * v0 + eps * (dp / (G - 1) + 0.5 * v0 * v0 * drho + v0 * dv) * s)
+
+
+def problem_1d(name):
+    outflow, closed = ("outflow", "outflow"), ("reflective", "reflective")
+    if name in ("sod", "toro-1"):
+        return _tube(1.0, 0.0, 0.25, 1.4, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 0.5, outflow)
+    if name == "shu-osher":
+        left = (3.857143, 2.629369, 10.33333)
+        return Problem1D(10.0, -5.0, 1.8, 1.4, outflow,
+                         lambda a, b: _shock_into_wave(a, b, left, -4, 0.2, 5, 1.4))
+    if name == "woodward-colella":
+        return Problem1D(1.0, 0.0, 0.038, 1.4, closed, _woodward_colella)
+    if name == "sound-right":
+        return Problem1D(1.0, 0.0, 1.0, 5.0 / 3.0, outflow,
+                         lambda a, b: _eigenmode(a, b, 1e-6, 0.0, 1.0, 1.0, 1.0), periodic=True)
+    raise ValueError("unknown 1D problem " + name)
+
+
+def line_points(n, L, jitter=0.0, seed=1):
+    """n generators on [0, L]: L (i + 1/2 + jitter u_i) / n, u_i uniform in [-1/2, 1/2) from mt19937_64(seed)
+    (hydro_1d.cpp:136-139); the jitter draws use numpy's MT19937 and are not the reference's bits."""
+    i = np.arange(n, dtype=np.float64)
+    if jitter == 0.0:
+        return L * (i + 0.5) / n
+    u = np.random.Generator(np.random.MT19937(seed)).uniform(-0.5, 0.5, n)
+    return L * (i + 0.5 + jitter * u) / n
+
+
+def line_grid(L):
+    """The snapping grid of a line of length L: (step, period)."""
+    period = 1 << GRID_BITS
+    return L / period, period
+
+
+def line_keys(x, L):
+    step, period = line_grid(L)
+    keys = np.rint(np.asarray(x) / step).astype(np.int64) % period
+    return np.sort(keys), step, period
+
+
+def line_initial_state(problem, keys, step, period):
+    """Q (extensive, n x 3) from the problem's exact cell averages over each cell [c - V/2, c + V/2]."""
+    n = len(keys)
+    gaps = np.empty(n + 1, dtype=np.int64)  # gaps[s] = gap(s - 1)
+    inner = np.diff(keys)
+    if problem.periodic:
+        wrap = keys[0] + period - keys[-1]
+        gaps[0], gaps[n] = wrap, wrap
+    else:
+        gaps[0], gaps[n] = 2 * keys[0], 2 * (period - keys[-1])
+    gaps[1:n] = inner
+    q = np.empty((n, 3))
+    for s in range(n):
+        V = (float(gaps[s]) + float(gaps[s + 1])) * (0.5 * step)
+        c = (float(gaps[s + 1]) - float(gaps[s])) * (0.25 * step)
+        x = float(keys[s]) * step
+        a = x + c - 0.5 * V
+        b = a + V
+        u = problem.average(a + problem.origin, b + problem.origin)
+        q[s] = [u[k] * V for k in range(3)]
+    return q
+
+
+# ----------------------------------------------------------------------------------------------------------------- 2D / 3D
+
+
+def morton_key(idx, bits):
+    """Interleaved bits of integer lattice indices idx (N x D), axis 0 lowest."""
+    idx = np.asarray(idx, dtype=np.uint64)
+    D = idx.shape[1]
+    key = np.zeros(idx.shape[0], dtype=np.uint64)
+    for b in range(bits):
+        for a in range(D):
+            key |= ((idx[:, a] >> np.uint64(b)) & np.uint64(1)) << np.uint64(b * D + a)
+    return key
+
+
+def jittered_lattice(n, dim, jitter, seed, side=1.0, morton=True):
+    """n^dim generators on the periodic box [0, side)^dim: lattice centres moved by up to `jitter` spacings per axis
+    (uniform draws, numpy MT19937 seeded), snapped to the 2^-30 grid, in Morton order of their lattice cells."""
+    h = side / n
+    axes = np.meshgrid(*[np.arange(n)] * dim, indexing="ij")
+    idx = np.stack([a.ravel() for a in axes], axis=1)  # lattice index, last axis fastest
+    if morton:
+        bits = max(1, int(math.ceil(math.log2(n))))
+        order = np.argsort(morton_key(idx, bits), kind="stable")
+        idx = idx[order]
+    rng = np.random.Generator(np.random.MT19937(seed))
+    u = rng.uniform(-1.0, 1.0, size=idx.shape)
+    x = (idx + 0.5 + jitter * u) * h
+    return snap_positions(x, side)
+
+
+def grid_of(side):
+    """The snapping grid of a box of sides `side`: step 2^-30 of the longest side, integer periods per axis."""
+    side = np.asarray(side, dtype=np.float64)
+    step = float(side.max()) / (1 << GRID_BITS)
+    period = np.rint(side / step).astype(np.int64)
+    return step, period
+
+
+def snap_positions(x, side=1.0):
+    x = np.asarray(x, dtype=np.float64)
+    D = x.shape[1]
+    sides = np.full(D, side, dtype=np.float64) if np.isscalar(side) else np.asarray(side, dtype=np.float64)
+    step, period = grid_of(sides)
+    k = np.rint(x * (1.0 / step)).astype(np.int64)
+    k = np.mod(k, period[None, :])
+    return k.astype(np.float64) * step
+
+
+def conserved_rows(W, gamma):
+    """W (N x V primitives) -> U per volume (euler.hpp:165-172 order: E = P/(gamma-1) + (0.5 rho) v.v)."""
+    W = np.asarray(W, dtype=np.float64)
+    D = W.shape[1] - 2
+    rho = W[:, 0]
+    vv = W[:, 1] * W[:, 1]
+    for a in range(1, D):
+        vv = vv + W[:, 1 + a] * W[:, 1 + a]
+    U = np.empty_like(W)
+    U[:, 0] = rho
+    U[:, 1:D + 1] = rho[:, None] * W[:, 1:D + 1]
+    U[:, D + 1] = W[:, D + 1] / (gamma - 1.0) + 0.5 * rho * vv
+    return U
+
+
+def sedov_state(mesh, gamma=5.0 / 3.0, rho=1.0, p=1e-5, energy=1.0, radius_cells=2.0, n=None):
+    """The Sedov blast: uniform (rho, 0, p); energy E added as energy density E / sum V over the cells whose generator lies
+    within radius_cells lattice spacings of the box centre (periodic distance), i.e. volume-weighted. Returns Q (extensive)."""
+    D, N = mesh["dim"], mesh["cells"]
+    side = np.asarray(mesh["side"][:D])
+    n = n if n is not None else round(N ** (1.0 / D))
+    h = side[0] / n
+    x = np.asarray(mesh["positions"]).reshape(N, D)
+    d = x - 0.5 * side[None, :]
+    d = d - side[None, :] * np.round(d / side[None, :])
+    inside = np.sqrt((d * d).sum(axis=1)) < radius_cells * h
+    V = np.asarray(mesh["volume"])
+    W = np.zeros((N, D + 2))
+    W[:, 0] = rho
+    W[:, D + 1] = p
+    U = conserved_rows(W, gamma)
+    deposit = V[inside].sum()
+    U[inside, D + 1] += energy / deposit
+    return U * V[:, None], int(inside.sum())
+
+
+def kelvin_helmholtz_state(mesh, gamma=5.0 / 3.0):
+    """2D shear layer on the periodic unit square: p = 2.5; rho = 2, vx = 0.5 for |y - 1/2| < 1/4, else rho = 1, vx = -0.5;
+    vy = 0.01 sin(4 pi x) [exp(-(y - 1/4)^2 / (2 0.05^2)) + exp(-(y - 3/4)^2 / (2 0.05^2))], at the cell centroids."""
+    N = mesh["cells"]
+    x = np.asarray(mesh["positions"]).reshape(N, 2) + np.asarray(mesh["centroid"]).reshape(N, 2)
+    X, Y = x[:, 0] % 1.0, x[:, 1] % 1.0
+    inner = np.abs(Y - 0.5) < 0.25
+    W = np.zeros((N, 4))
+    W[:, 0] = np.where(inner, 2.0, 1.0)
+    W[:, 1] = np.where(inner, 0.5, -0.5)
+    s = 0.05
+    W[:, 2] = 0.01 * np.sin(4 * np.pi * X) * (np.exp(-(Y - 0.25) ** 2 / (2 * s * s)) + np.exp(-(Y - 0.75) ** 2 / (2 * s * s)))
+    W[:, 3] = 2.5
+    return conserved_rows(W, gamma) * np.asarray(mesh["volume"])[:, None]
+
+
+def acoustic_state(mesh, gamma=5.0 / 3.0, amplitude=1e-3, k=(1, 1, 0)):
+    """rho = 1 + amplitude sin(2 pi k.x) at the cell centroids, v = 0, p = 3/5 (c = 1 for gamma = 5/3)."""
+    D, N = mesh["dim"], mesh["cells"]
+    x = np.asarray(mesh["positions"]).reshape(N, D) + np.asarray(mesh["centroid"]).reshape(N, D)
+    phase = 2 * np.pi * (x @ np.asarray(k[:D], dtype=np.float64))
+    W = np.zeros((N, D + 2))
+    W[:, 0] = 1 + amplitude * np.sin(phase)
+    W[:, D + 1] = 3.0 / 5.0
+    return conserved_rows(W, gamma) * np.asarray(mesh["volume"])[:, None]

@@ -1,0 +1,837 @@
+/* CPU oracle of luna-cpp's finite-volume hydro step (TEST INFRASTRUCTURE ONLY; see luna_oracle.h).
+ *
+ * Build: gcc -O2 -std=c99 -fopenmp -ffp-contract=off -fPIC -shared (oracle/Makefile). No contraction: every product and
+ * sum rounds on its own, as the reference's host build does on x86-64 without FMA.
+ * Reference paths are relative to the luna-cpp checkout. */
+#include "luna_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAXV 5
+#define MAXD 3
+
+/* euler.hpp:69-73: left to right, written out per D. */
+static double dot(int D, const double* a, const double* b) {
+    if (D == 1) return a[0] * b[0];
+    if (D == 2) return a[0] * b[0] + a[1] * b[1];
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* euler.hpp:91-93 */
+void orc_gas_init(orc_gas* gas, double gamma) {
+    gas->gamma = gamma;
+    gas->smallr = 1e-10;
+    gas->smallc = 1e-10;
+    gas->inverse_gm1 = 1.0 / (gamma - 1.0);
+}
+
+/* euler.hpp:104-113 primitive(): U -> W with the RAMSES floors. */
+void orc_primitive(int D, const orc_gas* gas, const double* U, double* W) {
+    const double rho = fmax(U[0], gas->smallr);
+    const double inv_rho = 1.0 / rho;
+    double v[MAXD];
+    for (int k = 0; k < D; ++k) v[k] = U[1 + k] * inv_rho;
+    const double smalle = gas->smallc * gas->smallc / gas->gamma / (gas->gamma - 1.0);
+    const double kinetic = 0.5 * rho * dot(D, v, v);
+    const double e = fmax(U[D + 1] - kinetic, smalle * rho);
+    W[0] = rho;
+    for (int k = 0; k < D; ++k) W[1 + k] = v[k];
+    W[D + 1] = (gas->gamma - 1.0) * e;
+}
+
+/* euler.hpp:165-172 conserved(): W -> U without floors. */
+void orc_conserved(int D, const orc_gas* gas, const double* W, double* U) {
+    const double rho = W[0];
+    const double kinetic = 0.5 * rho * dot(D, W + 1, W + 1);
+    U[0] = rho;
+    for (int k = 0; k < D; ++k) U[1 + k] = rho * W[1 + k];
+    U[D + 1] = W[D + 1] / (gas->gamma - 1.0) + kinetic;
+}
+
+/* euler.hpp:175-178 */
+double orc_sound_speed(int D, const orc_gas* gas, const double* W) { return sqrt(gas->gamma * W[D + 1] / W[0]); }
+
+/* euler.hpp:186-204 half_step_rate(): sum_d A_d(W) dW/dx_d, axes in order. g: [V][D]. */
+static void half_step_rate(int D, const orc_gas* gas, const double* W, const double* g, double* A) {
+    const int P = D + 1;
+    const double rho = W[0], inv_rho = 1.0 / W[0], rho_c2 = gas->gamma * W[P];
+    for (int v = 0; v < D + 2; ++v) A[v] = 0;
+    for (int d = 0; d < D; ++d) {
+        const double vn = W[1 + d];
+        A[0] += vn * g[0 * D + d] + rho * g[(1 + d) * D + d];
+        for (int k = 0; k < D; ++k) {
+            double row = vn * g[(1 + k) * D + d];
+            if (k == d) row = row + inv_rho * g[P * D + d];
+            A[1 + k] += row;
+        }
+        A[P] += rho_c2 * g[(1 + d) * D + d] + vn * g[P * D + d];
+    }
+}
+
+/* euler.hpp:207-214 half_step(). */
+void orc_half_step(int D, const orc_gas* gas, const double* W, const double* g, double dt, double* out) {
+    double A[MAXV];
+    half_step_rate(D, gas, W, g, A);
+    for (int v = 0; v < D + 2; ++v) out[v] = W[v] - 0.5 * dt * A[v];
+}
+
+/* euler.hpp:316-320 */
+typedef struct {
+    double v[MAXD];
+    double vn, c, E;
+    double F[MAXV], U[MAXV];
+} side_t;
+
+/* euler.hpp:324-336 side_with() */
+static void side_with(int D, const orc_gas* gas, const double* W, const double* n, const double* w, double vn, side_t* s) {
+    const int P = D + 1;
+    for (int k = 0; k < D; ++k) s->v[k] = W[1 + k] - w[k];
+    s->vn = vn;
+    s->c = 0;
+    s->E = W[P] * gas->inverse_gm1 + W[0] * (0.5 * dot(D, s->v, s->v));
+    const double m = W[0] * s->vn;
+    s->F[0] = m;
+    for (int k = 0; k < D; ++k) s->F[1 + k] = m * s->v[k] + W[P] * n[k];
+    s->F[P] = (s->E + W[P]) * s->vn;
+    s->U[0] = W[0];
+    for (int k = 0; k < D; ++k) s->U[1 + k] = W[0] * s->v[k];
+    s->U[P] = s->E;
+}
+
+/* euler.hpp:340-346 side() */
+static void side_full(int D, const orc_gas* gas, const double* W, const double* n, const double* w, side_t* s) {
+    const int P = D + 1;
+    double rel[MAXD];
+    for (int k = 0; k < D; ++k) rel[k] = W[1 + k] - w[k];
+    side_with(D, gas, W, n, w, dot(D, rel, n), s);
+    const double c2 = gas->gamma * W[P] / W[0];
+    s->c = c2 > 0 ? sqrt(c2) : 0.0;
+}
+
+/* euler.hpp:350-356 to_lab() */
+static void to_lab(int D, double* F, const double* w) {
+    if (D == 1) F[2] += w[0] * F[1] + 0.5 * dot(1, w, w) * F[0];
+    if (D == 2) F[3] += w[0] * F[1] + w[1] * F[2] + 0.5 * dot(2, w, w) * F[0];
+    if (D == 3) F[4] += w[0] * F[1] + w[1] * F[2] + w[2] * F[3] + 0.5 * dot(3, w, w) * F[0];
+    for (int k = 0; k < D; ++k) F[1 + k] += w[k] * F[0];
+}
+
+/* euler.hpp:444-461 HLL::flux() */
+void orc_hll(int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, double* F) {
+    side_t l, r;
+    side_full(D, gas, L, n, w, &l);
+    side_full(D, gas, R, n, w, &r);
+    const double SL = fmin(l.vn - l.c, r.vn - r.c);
+    const double SR = fmax(l.vn + l.c, r.vn + r.c);
+    if (SL >= 0.0) {
+        for (int v = 0; v < D + 2; ++v) F[v] = l.F[v];
+    } else if (SR <= 0.0) {
+        for (int v = 0; v < D + 2; ++v) F[v] = r.F[v];
+    } else {
+        const double inv_dS = 1.0 / (SR - SL);
+        for (int v = 0; v < D + 2; ++v) F[v] = (SR * l.F[v] - SL * r.F[v] + SL * SR * (r.U[v] - l.U[v])) * inv_dS;
+    }
+    to_lab(D, F, w);
+}
+
+/* euler.hpp:547-550 HLLC::nonzero() */
+static double nonzero(double d) {
+    const double eps = 1e-20;
+    return fabs(d) < eps ? (d < 0 ? -eps : eps) : d;
+}
+
+/* euler.hpp:609-642 HLLC::flux() (LUNA_FLUX_LEVER 0) */
+void orc_hllc(int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, double* F) {
+    const int P = D + 1;
+    double rl[MAXD], rr[MAXD];
+    for (int k = 0; k < D; ++k) {
+        rl[k] = L[1 + k] - w[k];
+        rr[k] = R[1 + k] - w[k];
+    }
+    const double vnL = dot(D, rl, n), vnR = dot(D, rr, n);
+    const double inv_rhoL = 1.0 / L[0], inv_rhoR = 1.0 / R[0];
+    const double c2L = gas->gamma * L[P] * inv_rhoL, c2R = gas->gamma * R[P] * inv_rhoR;
+    const int ok = c2L >= 0 && c2R >= 0;
+    const double cL = c2L > 0 ? sqrt(c2L) : 0.0, cR = c2R > 0 ? sqrt(c2R) : 0.0;
+    const double SL = fmin(vnL - cL, vnR - cR);
+    const double SR = fmax(vnL + cL, vnR + cR);
+    const double denomL = L[0] * (SL - vnL), denomR = R[0] * (SR - vnR);
+    const double SM = (R[P] - L[P] + denomL * vnL - denomR * vnR) / nonzero(denomL - denomR);
+    const int left = ok && (SL >= 0 || SM >= 0);
+    const int star = ok && SL < 0 && (SM >= 0 || SR > 0);
+    double W[MAXV];
+    for (int v = 0; v < D + 2; ++v) W[v] = left ? L[v] : R[v];
+    side_t s;
+    side_with(D, gas, W, n, w, left ? vnL : vnR, &s);
+    if (!star) {
+        for (int v = 0; v < D + 2; ++v) F[v] = s.F[v];
+        to_lab(D, F, w);
+        return;
+    }
+    const double S = left ? SL : SR, denom = left ? denomL : denomR;
+    const double rho_star = denom / nonzero(S - SM);
+    const double E_star = s.E * (left ? inv_rhoL : inv_rhoR) + (SM - s.vn) * (SM + W[P] * (1.0 / nonzero(denom)));
+    double U_star[MAXV];
+    U_star[0] = rho_star;
+    for (int k = 0; k < D; ++k) U_star[1 + k] = rho_star * (s.v[k] + n[k] * (SM - s.vn));
+    U_star[P] = rho_star * E_star;
+    for (int v = 0; v < D + 2; ++v) F[v] = s.F[v] + S * (U_star[v] - s.U[v]);
+    to_lab(D, F, w);
+}
+
+static void riemann(int kind, int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, double* F) {
+    if (kind == 1)
+        orc_hll(D, gas, L, R, n, w, F);
+    else
+        orc_hllc(D, gas, L, R, n, w, F);
+}
+
+/* ------------------------------------------------------------------ D = 2, 3 on a Voronoi mesh ----------------------------- */
+
+/* The nearest periodic image of a separation (the role of tet_mesh::wrap, finite_volume.hpp:279-282). */
+static double wrap(double d, double side) { return d - side * round(d / side); }
+
+/* finite_volume.hpp:279-282 Geometry::separation(i, j): generator j seen from generator i. */
+static void separation(const orc_geometry* g, int i, int j, double* d) {
+    for (int a = 0; a < g->D; ++a) d[a] = wrap(g->positions[j * g->D + a] - g->positions[i * g->D + a], g->side[a]);
+}
+
+/* finite_volume.hpp:322-326 Geometry::face_offset(f, i): face centroid relative to cell i's centroid. */
+static void face_offset(const orc_geometry* g, int f, int i, double* r) {
+    const int D = g->D, o = g->owner[f];
+    double rel[MAXD];
+    if (o == i) {
+        for (int a = 0; a < D; ++a) rel[a] = g->face_centroid[f * D + a];
+    } else {
+        double s[MAXD];
+        separation(g, i, o, s);
+        for (int a = 0; a < D; ++a) rel[a] = g->face_centroid[f * D + a] + s[a];
+    }
+    for (int a = 0; a < D; ++a) r[a] = rel[a] - g->centroid[i * D + a];
+}
+
+static int list_face(int e) { return e >= 0 ? e : ~e; }
+
+static double magnitude_of(const orc_geometry* g, int f) { return sqrt(dot(g->D, g->area + f * g->D, g->area + f * g->D)); }
+
+/* finite_volume.hpp:241-246 radius() */
+static double radius(int D, double v) {
+    if (D == 3) return cbrt(v * (3.0 / (4.0 * M_PI)));
+    return sqrt(v * (1.0 / M_PI));
+}
+/* finite_volume.hpp:248-253 face_angle() */
+static double face_angle(int D, double area) {
+    if (D == 3) return sqrt(area / M_PI);
+    return 0.5 * area;
+}
+
+/* finite_volume.hpp:403-415 primitives(): prims row = (W, c_s, r) from Q / V. */
+void orc_primitives(orc_fv* fv, const orc_geometry* geo) {
+    const int D = fv->D, V = D + 2, PW = V + 2;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < fv->N; ++i) {
+        const double inv = 1.0 / geo->volume[i];
+        double U[MAXV], W[MAXV];
+        for (int v = 0; v < V; ++v) U[v] = fv->q[i * V + v] * inv;
+        orc_primitive(D, &fv->gas, U, W);
+        double* row = fv->prims + (size_t)i * PW;
+        for (int v = 0; v < V; ++v) row[v] = W[v];
+        row[V] = orc_sound_speed(D, &fv->gas, W);
+        row[V + 1] = radius(D, geo->volume[i]);
+    }
+}
+
+/* finite_volume.hpp:590-615 Sweep::solve(): Cholesky of M (upper triangle), then M x_v = b_v. */
+static void solve_fit(int D, double M[MAXD + 1][MAXD + 1], double b[MAXV][MAXD + 1], double* grad) {
+    const int U = D + 1, V = D + 2;
+    double L[MAXD + 1][MAXD + 1];
+    memset(L, 0, sizeof L);
+    for (int p = 0; p < U; ++p)
+        for (int s = 0; s <= p; ++s) {
+            double sum = M[s][p];
+            for (int m = 0; m < s; ++m) sum -= L[p][m] * L[s][m];
+            L[p][s] = p == s ? sqrt(sum) : sum / L[s][s];
+        }
+    for (int v = 0; v < V; ++v) {
+        double z[MAXD + 1], x[MAXD + 1];
+        for (int p = 0; p < U; ++p) {
+            double sum = b[v][p];
+            for (int m = 0; m < p; ++m) sum -= L[p][m] * z[m];
+            z[p] = sum / L[p][p];
+        }
+        for (int p = U - 1; p >= 0; --p) {
+            double sum = z[p];
+            for (int m = p + 1; m < U; ++m) sum -= L[m][p] * x[m];
+            x[p] = sum / L[p][p];
+        }
+        for (int a = 0; a < D; ++a) grad[v * D + a] = x[1 + a];
+    }
+}
+
+/* finite_volume.hpp:463-569 Sweep::operator() (double policy, the face-centroid limiter or none), with the moving mesh's
+ * point velocity (:526-537, constants :74) and neighbour_range (:572-588). */
+static void sweep_cell(orc_fv* fv, const orc_geometry* geo, int i) {
+    const int D = fv->D, V = D + 2, U = D + 1, PW = V + 2;
+    const double* Wi = fv->prims + (size_t)i * PW;
+    double surface = 0, smallest = INFINITY, max_angle = 0;
+    double lo[MAXV], hi[MAXV];
+    for (int v = 0; v < V; ++v) {
+        lo[v] = DBL_MAX;
+        hi[v] = -DBL_MAX;
+    }
+    double M[MAXD + 1][MAXD + 1], b[MAXV][MAXD + 1];
+    memset(M, 0, sizeof M);
+    memset(b, 0, sizeof b);
+    const double* centroid = geo->centroid + (size_t)i * D;
+    const int first = geo->start[i], end = geo->start[i + 1];
+    const int grad_on = fv->order == 1;
+    for (int k = first; k < end; ++k) {
+        const int e = geo->list[k], f = list_face(e);
+        const int j = e >= 0 ? geo->neighbour[f] : geo->owner[f];
+        const double* Wj = fv->prims + (size_t)j * PW;
+        double d[MAXD];
+        separation(geo, i, j, d);
+        const double magnitude = magnitude_of(geo, f);
+        surface += magnitude;
+        smallest = fmin(smallest, magnitude);
+        if (fv->moving) max_angle = fmax(max_angle, face_angle(D, magnitude) / (0.5 * sqrt(dot(D, d, d))));
+        if (grad_on) {
+            double X[MAXD + 1];
+            X[0] = 1;
+            for (int p = 1; p < U; ++p) X[p] = d[p - 1] + geo->centroid[(size_t)j * D + p - 1] - centroid[p - 1];
+            const double w = magnitude;
+            for (int p = 0; p < U; ++p)
+                for (int s = p; s < U; ++s) M[p][s] += w * X[p] * X[s];
+            for (int v = 0; v < V; ++v) {
+                const double dW = Wj[v] - Wi[v];
+                for (int p = 0; p < U; ++p) b[v][p] += w * X[p] * dW;
+                lo[v] = fmin(lo[v], Wj[v]);
+                hi[v] = fmax(hi[v], Wj[v]);
+            }
+        }
+    }
+    const double tiny = 1e-6 * surface;
+    const int any_tiny = smallest < tiny;
+    if (grad_on && any_tiny) {
+        for (int v = 0; v < V; ++v) {
+            lo[v] = DBL_MAX;
+            hi[v] = -DBL_MAX;
+        }
+        for (int k = first; k < end; ++k) {
+            const int e = geo->list[k], f = list_face(e);
+            const int j = e >= 0 ? geo->neighbour[f] : geo->owner[f];
+            if (magnitude_of(geo, f) < tiny) continue;
+            const double* Wj = fv->prims + (size_t)j * PW;
+            for (int v = 0; v < V; ++v) {
+                lo[v] = fmin(lo[v], Wj[v]);
+                hi[v] = fmax(hi[v], Wj[v]);
+            }
+        }
+    }
+    double grad[MAXV * MAXD];
+    if (grad_on) solve_fit(D, M, b, grad);
+    if (fv->moving) {
+        const double shaping = 0.5, factor = 2.25;
+        double fraction = 0;
+        if (max_angle > 0.75 * factor)
+            fraction = max_angle > factor ? shaping : shaping * (max_angle - 0.75 * factor) / (0.25 * factor);
+        double wi[MAXD];
+        for (int a = 0; a < D; ++a) wi[a] = Wi[1 + a];
+        const double cd = sqrt(dot(D, centroid, centroid));
+        if (cd > 0 && fraction > 0) {
+            const double s = fraction * Wi[V] / cd;
+            for (int a = 0; a < D; ++a) wi[a] = wi[a] + centroid[a] * s;
+        }
+        for (int a = 0; a < D; ++a) fv->w[(size_t)i * D + a] = wi[a];
+    }
+    if (!grad_on) return;
+    if (fv->limiter == 0) {
+        for (int k = first; k < end; ++k) {
+            const int f = list_face(geo->list[k]);
+            if (any_tiny && magnitude_of(geo, f) < tiny) continue;
+            double offset[MAXD];
+            face_offset(geo, f, i, offset);
+            for (int v = 0; v < V; ++v) {
+                double* g = grad + v * D;
+                const double dp = dot(D, g, offset);
+                if (dp > 0 && Wi[v] + dp > hi[v]) {
+                    const double s = hi[v] > Wi[v] ? (hi[v] - Wi[v]) / dp : 0.0;
+                    for (int a = 0; a < D; ++a) g[a] = g[a] * s;
+                } else if (dp < 0 && Wi[v] + dp < lo[v]) {
+                    const double s = lo[v] < Wi[v] ? (lo[v] - Wi[v]) / dp : 0.0;
+                    for (int a = 0; a < D; ++a) g[a] = g[a] * s;
+                }
+            }
+        }
+    }
+    memcpy(fv->grad + (size_t)i * V * D, grad, sizeof(double) * V * D);
+}
+
+/* finite_volume.hpp:1040-1052 CellStep, :1054-1071 LunaStep, :1074-1090 FaceStep, :1107-1149 sweep(). */
+static double cell_term(const orc_fv* fv, int i, double courant) {
+    const int D = fv->D, V = D + 2, PW = V + 2;
+    const double* row = fv->prims + (size_t)i * PW;
+    double rel[MAXD];
+    for (int a = 0; a < D; ++a) rel[a] = row[1 + a];
+    if (fv->moving)
+        for (int a = 0; a < D; ++a) rel[a] = rel[a] - fv->w[(size_t)i * D + a];
+    const double step = row[V + 1] / (row[V] + sqrt(dot(D, rel, rel)));
+    return courant * step;
+}
+
+static double luna_term(const orc_fv* fv, int i, double courant) {
+    const int D = fv->D, V = D + 2, PW = V + 2;
+    const double* row = fv->prims + (size_t)i * PW;
+    const double c = row[V], dx = row[V + 1] * sqrt(M_PI);
+    double rate = 0;
+    for (int a = 0; a < D; ++a) {
+        double relative = row[1 + a];
+        if (fv->moving) relative -= fv->w[(size_t)i * D + a];
+        rate += (fabs(relative) + c) / dx;
+    }
+    return courant / rate;
+}
+
+static double face_term(const orc_fv* fv, const orc_geometry* geo, int f, double courant) {
+    const int D = fv->D, V = D + 2, PW = V + 2;
+    const int o = geo->owner[f], j = geo->neighbour[f];
+    double d[MAXD], dv[MAXD];
+    separation(geo, o, j, d);
+    const double dist = sqrt(dot(D, d, d));
+    const double* po = fv->prims + (size_t)o * PW;
+    const double* pj = fv->prims + (size_t)j * PW;
+    for (int a = 0; a < D; ++a) dv[a] = pj[1 + a] - po[1 + a];
+    const double vsig = po[V] + pj[V] - dot(D, dv, d) / dist;
+    double smallest = INFINITY;
+    if (vsig > 0) {
+        smallest = fmin(smallest, courant * ((dist + po[V + 1]) / vsig));
+        smallest = fmin(smallest, courant * ((dist + pj[V + 1]) / vsig));
+    }
+    return smallest;
+}
+
+/* finite_volume.hpp:190-201 time_step(): primitives (unless the moving step's second pass wrote them), the sweep, dt. */
+double orc_time_step(orc_fv* fv, const orc_geometry* geo, double courant) {
+    if (!fv->prims_ready) orc_primitives(fv, geo);
+    fv->prims_ready = 0;
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int i = 0; i < fv->N; ++i) sweep_cell(fv, geo, i);
+    double smallest = INFINITY;
+    if (fv->D == 2 && fv->step_rule == 1) {
+#pragma omp parallel for reduction(min : smallest)
+        for (int i = 0; i < fv->N; ++i) smallest = fmin(smallest, luna_term(fv, i, courant));
+        return smallest;
+    }
+    const int total = geo->F > fv->N ? geo->F : fv->N;
+#pragma omp parallel for reduction(min : smallest)
+    for (int k = 0; k < total; ++k) {
+        if (k < geo->F) smallest = fmin(smallest, face_term(fv, geo, k, courant));
+        if (k < fv->N) smallest = fmin(smallest, cell_term(fv, k, courant));
+    }
+    return smallest;
+}
+
+/* finite_volume.hpp:1306-1329 FaceFlux::side() at order 1 (double): half step + g.r, positivity fallback.
+ * step_dt: the FaceFlux's dt (dt of the step; 2 dt on a moving step's second pass). */
+static void face_side(const orc_fv* fv, int i, const double* r, double step_dt, double* Wf, int* fallbacks) {
+    const int D = fv->D, V = D + 2, PW = V + 2;
+    const double* W = fv->prims + (size_t)i * PW;
+    const double* g = fv->grad + (size_t)i * V * D;
+    if (fv->order == 0) {
+        for (int v = 0; v < V; ++v) Wf[v] = W[v];
+        return;
+    }
+    orc_half_step(D, &fv->gas, W, g, step_dt, Wf);
+    for (int v = 0; v < V; ++v) Wf[v] += dot(D, g + v * D, r);
+    if (Wf[0] > 0 && Wf[D + 1] > 0) return;
+    ++*fallbacks;
+    for (int v = 0; v < V; ++v) Wf[v] = W[v];
+}
+
+/* finite_volume.hpp:1451-1465 MovingFlux<false>::pass_side(): W + g.r on the mesh of t^n, no half step. */
+static void pass_side_first(const orc_fv* fv, int i, const double* r, double* Wf, int* fallbacks) {
+    const int D = fv->D, V = D + 2, PW = V + 2;
+    const double* W = fv->prims + (size_t)i * PW;
+    const double* g = fv->grad + (size_t)i * V * D;
+    for (int v = 0; v < V; ++v) Wf[v] = W[v] + dot(D, g + v * D, r);
+    if (Wf[0] > 0 && Wf[D + 1] > 0) return;
+    ++*fallbacks;
+    for (int v = 0; v < V; ++v) Wf[v] = W[v];
+}
+
+/* finite_volume.hpp:1380-1397 FaceFlux::face_velocity() */
+static void face_velocity(const orc_geometry* geo, int f, int o, int j, const double* wL, const double* wR, double* wf) {
+    const int D = geo->D;
+    double d[MAXD], c[MAXD], dw[MAXD];
+    separation(geo, o, j, d);
+    const double dist = sqrt(dot(D, d, d));
+    const double inv_dist = 1.0 / dist;
+    for (int a = 0; a < D; ++a) c[a] = geo->face_centroid[(size_t)f * D + a] - d[a] * 0.5;
+    for (int a = 0; a < D; ++a) dw[a] = wL[a] - wR[a];
+    double facv = dot(D, c, dw) * inv_dist;
+    const double limit = 0.9 * dist, c2 = dot(D, c, c);
+    if (c2 > limit * limit) facv *= limit / sqrt(c2);
+    for (int a = 0; a < D; ++a) wf[a] = (wL[a] + wR[a]) * 0.5 + d[a] * (facv * inv_dist);
+}
+
+/* The moving step's point velocity from its grid steps (finite_volume.hpp:1446-1448). */
+static void grid_velocity(const orc_fv* fv, int i, double unit, double* w) {
+    for (int a = 0; a < fv->D; ++a) w[a] = (double)fv->steps[(size_t)i * fv->D + a] * unit;
+}
+
+/* One face's flux times |A| (finite_volume.hpp:1415-1434 FaceFlux::operator(), :1466-1490 MovingFlux::operator()).
+ * pass: -1 static, 0 a moving step's first pass (mesh t^n), 1 its second (mesh t^n+1). */
+static void face_flux(const orc_fv* fv, const orc_geometry* geo, int f, double dt, int pass, double unit, double* F, int* fallbacks) {
+    const int D = fv->D, V = D + 2;
+    const int o = geo->owner[f], j = geo->neighbour[f];
+    const double* a = geo->area + (size_t)f * D;
+    const double magnitude = sqrt(dot(D, a, a));
+    for (int v = 0; v < V; ++v) F[v] = 0;
+    if (magnitude == 0) return;
+    double L[MAXV], R[MAXV], n[MAXD], wf[MAXD] = {0, 0, 0}, ro[MAXD], rj[MAXD];
+    if (pass < 0) {
+        const double inv = 1.0 / magnitude;
+        for (int k = 0; k < D; ++k) n[k] = a[k] * inv;
+        face_offset(geo, f, o, ro);
+        face_offset(geo, f, j, rj);
+        const double half = 0.5 * dt;
+        for (int k = 0; k < D; ++k) {
+            ro[k] = ro[k] + wf[k] * half;
+            rj[k] = rj[k] + wf[k] * half;
+        }
+        face_side(fv, o, ro, dt, L, fallbacks);
+        face_side(fv, j, rj, dt, R, fallbacks);
+    } else {
+        const double inv = 1.0 / magnitude;
+        for (int k = 0; k < D; ++k) n[k] = a[k] * inv;
+        double wo[MAXD], wj[MAXD];
+        grid_velocity(fv, o, unit, wo);
+        grid_velocity(fv, j, unit, wj);
+        face_velocity(geo, f, o, j, wo, wj, wf);
+        face_offset(geo, f, o, ro);
+        face_offset(geo, f, j, rj);
+        if (pass == 0) {
+            pass_side_first(fv, o, ro, L, fallbacks);
+            pass_side_first(fv, j, rj, R, fallbacks);
+        } else {
+            /* Second pass: side(i, r + c^n+1 - c^n + w dt) with the FaceFlux dt = 2 dt (its half step spans dt). */
+            const double step_dt = 2 * dt;
+            double r2[MAXD];
+            const int cells[2] = {o, j};
+            const double* rs[2] = {ro, rj};
+            double* out[2] = {L, R};
+            for (int s = 0; s < 2; ++s) {
+                const int i = cells[s];
+                double wi[MAXD];
+                grid_velocity(fv, i, unit, wi);
+                for (int k = 0; k < D; ++k) {
+                    const double moved = wi[k] * (0.5 * step_dt);
+                    r2[k] = rs[s][k] + geo->centroid[(size_t)i * D + k] - fv->old_centroid[(size_t)i * D + k] + moved;
+                }
+                face_side(fv, i, r2, step_dt, out[s], fallbacks);
+            }
+        }
+    }
+    riemann(fv->riemann, D, &fv->gas, L, R, n, wf, F);
+    for (int v = 0; v < V; ++v) F[v] *= magnitude;
+}
+
+/* finite_volume.hpp:1539-1631 apply(), per-face loop: the flux rows, then Q -= dt sum_k sign_k F in list order.
+ * pass 1 (a moving step's second) also writes the next primitive rows (:1596-1602). */
+static void apply(orc_fv* fv, const orc_geometry* geo, double flux_dt, double apply_dt, int pass, double unit) {
+    const int D = fv->D, V = D + 2, PW = V + 2;
+    long long count = 0;
+#pragma omp parallel for reduction(+ : count) schedule(static)
+    for (int f = 0; f < geo->F; ++f) {
+        int fallbacks = 0;
+        face_flux(fv, geo, f, flux_dt, pass, unit, fv->flux + (size_t)f * V, &fallbacks);
+        count += fallbacks;
+    }
+    fv->fallbacks[pass == 1 ? 1 : 0] = count;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < fv->N; ++i) {
+        double sum[MAXV] = {0, 0, 0, 0, 0};
+        for (int k = geo->start[i]; k < geo->start[i + 1]; ++k) {
+            const int e = geo->list[k], f = list_face(e);
+            const double sign = e >= 0 ? 1.0 : -1.0;
+            for (int v = 0; v < V; ++v) sum[v] += sign * fv->flux[(size_t)f * V + v];
+        }
+        double next[MAXV];
+        for (int v = 0; v < V; ++v) fv->q[(size_t)i * V + v] = next[v] = fv->q[(size_t)i * V + v] - apply_dt * sum[v];
+        if (pass == 1) {
+            const double inv = 1.0 / geo->volume[i];
+            double U[MAXV], W[MAXV];
+            for (int v = 0; v < V; ++v) U[v] = next[v] * inv;
+            orc_primitive(D, &fv->gas, U, W);
+            double* row = fv->prims + (size_t)i * PW;
+            for (int v = 0; v < V; ++v) row[v] = W[v];
+            row[V] = orc_sound_speed(D, &fv->gas, W);
+            row[V + 1] = radius(D, geo->volume[i]);
+        }
+    }
+}
+
+/* finite_volume.hpp:205-226 step() on a static mesh, order 0 or 1, per-face loop (advance -> apply). */
+void orc_step_static(orc_fv* fv, const orc_geometry* geo, double dt) {
+    fv->fallbacks[1] = 0;
+    apply(fv, geo, dt, dt, -1, 0.0);
+}
+
+/* The integer grid key of x (the role of exact_predicates::snap): the nearest grid point, wrapped into [0, period). */
+long long orc_snap(double x, double step, long long period) {
+    long long k = (long long)round(x * (1.0 / step));
+    k %= period;
+    if (k < 0) k += period;
+    return k;
+}
+/* The move k0 -> k1 in grid steps to the nearest periodic image (the role of exact_predicates::delta). */
+int orc_delta(long long k0, long long k1, long long period) {
+    long long d = k1 - k0;
+    if (d > period / 2) d -= period;
+    if (d < -(period / 2)) d += period;
+    return (int)d;
+}
+
+/* finite_volume.hpp:1497-1516 step_moving() up to the mesh update: keep c^n, snap x, whole grid steps; pass 1 (half dt);
+ * drift (:1219-1231) onto the grid point the steps reach. The caller then builds the mesh of the new positions. */
+void orc_moving_begin(orc_fv* fv, const orc_geometry* geo, double dt) {
+    const int D = fv->D;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < fv->N; ++i)
+        for (int a = 0; a < D; ++a) {
+            const size_t at = (size_t)i * D + a;
+            fv->old_centroid[at] = geo->centroid[at];
+            const long long k0 = orc_snap(fv->x[at], geo->step, geo->period[a]);
+            const long long k1 = orc_snap(fv->x[at] + dt * fv->w[at], geo->step, geo->period[a]);
+            fv->x[at] = (double)k0 * geo->step;
+            fv->steps[at] = orc_delta(k0, k1, geo->period[a]);
+        }
+    fv->fallbacks[1] = 0;
+    apply(fv, geo, 0.0, 0.5 * dt, 0, geo->step / dt);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < fv->N; ++i)
+        for (int a = 0; a < D; ++a) {
+            const size_t at = (size_t)i * D + a;
+            const long long k = (long long)round(fv->x[at] * (1.0 / geo->step)) + fv->steps[at];
+            const long long p = geo->period[a];
+            fv->x[at] = (double)(((k % p) + p) % p) * geo->step;
+        }
+}
+
+/* finite_volume.hpp:1517-1518: the second pass on the mesh of t^n+1 (half dt); its update writes the next primitives. */
+void orc_moving_finish(orc_fv* fv, const orc_geometry* geo, double dt) {
+    apply(fv, geo, dt, 0.5 * dt, 1, geo->step / dt);
+    fv->prims_ready = 1;
+}
+
+/* ------------------------------------------------------------------ D = 1 on a sorted line ---------------------------------- */
+
+/* sorted_mesh::Line as finite_volume_1d.hpp reads it: gaps in grid steps between neighbouring generators, a wall's gap
+ * twice the distance to it (the mirror ghost, :152-155, :636-653). */
+static long long line_gap(const orc_line* l, int s) {
+    const int n = l->n;
+    if (s >= 0 && s < n - 1) return l->key[s + 1] - l->key[s];
+    if (l->periodic) return l->key[0] + l->period - l->key[n - 1];
+    if (s < 0) return 2 * l->key[0];
+    return 2 * (l->period - l->key[n - 1]);
+}
+static double line_length(const orc_line* l, int s) { return (double)line_gap(l, s) * l->step; }
+/* finite_volume_1d.hpp:173, 176 (load_inner): V and the centroid relative to the generator from the two gaps. */
+double orc_line_volume(const orc_line* l, int s) { return ((double)line_gap(l, s - 1) + (double)line_gap(l, s)) * (0.5 * l->step); }
+double orc_line_centroid(const orc_line* l, int s) { return ((double)line_gap(l, s) - (double)line_gap(l, s - 1)) * (0.25 * l->step); }
+static int line_slot(const orc_line* l, int s, int k) {
+    const int t = s + k;
+    if (l->periodic) return ((t % l->n) + l->n) % l->n;
+    return t >= 0 && t < l->n ? t : -1;
+}
+
+/* finite_volume_1d.hpp:107-111 primitive_of() */
+static void line_primitive(const orc_line* l, int s, double* W) {
+    const double inv = 1.0 / orc_line_volume(l, s);
+    const double U[3] = {l->q[s * 3] * inv, l->q[s * 3 + 1] * inv, l->q[s * 3 + 2] * inv};
+    orc_primitive(1, &l->gas, U, W);
+}
+
+/* finite_volume_1d.hpp:117-128 ghost(), ghost_face() */
+static void line_ghost(const orc_line* l, const double* W, int left, double* out) {
+    const int wall = left ? l->wall_left : l->wall_right;
+    for (int v = 0; v < 3; ++v) out[v] = W[v];
+    if (wall == 0) out[1] = -out[1];
+}
+static void line_ghost_face(const orc_line* l, const double* Wf, const double* W, int left, double* out) {
+    const int wall = left ? l->wall_left : l->wall_right;
+    if (wall == 0) {
+        for (int v = 0; v < 3; ++v) out[v] = Wf[v];
+        out[1] = -out[1];
+        return;
+    }
+    line_ghost(l, W, left, out);
+}
+
+typedef struct {
+    double W[3], c, V;
+    int real;
+} line_cell;
+
+/* finite_volume_1d.hpp:132-156 load<R>() with R = 2. */
+static void line_load(const orc_line* l, int s, line_cell cell[5], double gap[4]) {
+    int slot[5];
+    for (int k = 0; k < 5; ++k) {
+        slot[k] = line_slot(l, s, k - 2);
+        cell[k].W[0] = cell[k].W[1] = cell[k].W[2] = 0;
+        cell[k].c = cell[k].V = 0;
+        cell[k].real = 1;
+    }
+    for (int k = 0; k < 5; ++k)
+        if (slot[k] >= 0) {
+            line_primitive(l, slot[k], cell[k].W);
+            cell[k].c = orc_line_centroid(l, slot[k]);
+            cell[k].V = orc_line_volume(l, slot[k]);
+        }
+    for (int k = 0; k < 5; ++k) {
+        if (slot[k] >= 0) continue;
+        cell[k].real = 0;
+        const int end = k < 2 ? k + 1 : k - 1;
+        if (slot[end] < 0) continue;
+        line_ghost(l, cell[end].W, k < 2, cell[k].W);
+        cell[k].c = -cell[end].c;
+    }
+    for (int k = 0; k < 4; ++k) {
+        const int a = slot[k], b = slot[k + 1];
+        gap[k] = a >= 0 ? line_length(l, a) : b >= 0 ? line_length(l, -1) : 0.0;
+    }
+}
+
+/* finite_volume_1d.hpp:183-205 Stencil::gradient(): the limited gradient of variable v (MC face limiter, minmod, van Leer, none). */
+static double line_gradient(const orc_line* l, const line_cell* cell, double gapL, double gapR, int v) {
+    const line_cell* me = cell;
+    const double dL = -gapL + cell[-1].c - me->c, dR = gapR + cell[1].c - me->c;
+    const double Wi = me->W[v], left = cell[-1].W[v], right = cell[1].W[v];
+    if (l->limiter == 2 || l->limiter == 3) {
+        const double sL = (left - Wi) / dL, sR = (right - Wi) / dR;
+        if (!(sL * sR > 0)) return 0.0;
+        if (l->limiter == 3) return 2 * sL * sR / (sL + sR);
+        return fabs(sL) < fabs(sR) ? sL : sR;
+    }
+    double g = ((right - Wi) - (left - Wi)) / (dR - dL);
+    if (l->limiter == 1) return g;
+    const double r[2] = {-0.5 * gapL - me->c, 0.5 * gapR - me->c};
+    const double lo = fmin(left, right), hi = fmax(left, right);
+    for (int f = 0; f < 2; ++f) {
+        const double dp = g * r[f];
+        if (dp > 0 && Wi + dp > hi)
+            g = g * (hi > Wi ? (hi - Wi) / dp : 0.0);
+        else if (dp < 0 && Wi + dp < lo)
+            g = g * (lo < Wi ? (lo - Wi) / dp : 0.0);
+    }
+    return g;
+}
+
+/* finite_volume_1d.hpp:492-505 Kernel::face_state() (static). */
+static void line_face_state(const orc_line* l, const double* W, const double* g, double r, double dt, double* out) {
+    if (l->order == 0) {
+        for (int v = 0; v < 3; ++v) out[v] = W[v];
+        return;
+    }
+    orc_half_step(1, &l->gas, W, g, dt, out);
+    for (int v = 0; v < 3; ++v) out[v] += g[v] * r;
+    if (out[0] > 0 && out[2] > 0) return;
+    for (int v = 0; v < 3; ++v) out[v] = W[v];
+}
+
+/* finite_volume_1d.hpp:555-661 Kernel::body() (static): the slot's two face fluxes and its new state; fused: the next dt by
+ * LUNA's rule from the new state. */
+static double line_body(const orc_line* l, int s, double dt, double courant, int fused) {
+    line_cell cell[5];
+    double gap[4];
+    line_load(l, s, cell, gap);
+    double grad[5][3];
+    memset(grad, 0, sizeof grad);
+    for (int k = 1; k <= 3; ++k) {
+        if (!cell[k].real) continue;
+        if (l->order == 1)
+            for (int v = 0; v < 3; ++v) grad[k][v] = line_gradient(l, cell + k, gap[k - 1], gap[k], v);
+    }
+    double F[2][3];
+    const double one = 1.0, zero = 0.0;
+    for (int f = 0; f < 2; ++f) {
+        const int a = 1 + f, b = 2 + f;
+        const double w = 0, shift = 0.5 * dt * w;
+        double L[3] = {0, 0, 0}, R[3] = {0, 0, 0};
+        if (cell[a].real) line_face_state(l, cell[a].W, grad[a], 0.5 * gap[a] - cell[a].c + shift, dt, L);
+        if (cell[b].real) line_face_state(l, cell[b].W, grad[b], -0.5 * gap[a] - cell[b].c + shift, dt, R);
+        if (!cell[a].real) line_ghost_face(l, R, cell[b].W, 1, L);
+        if (!cell[b].real) line_ghost_face(l, L, cell[a].W, 0, R);
+        riemann(l->riemann, 1, &l->gas, L, R, &one, &zero, F[f]);
+    }
+    double Unew[3];
+    for (int v = 0; v < 3; ++v) {
+        Unew[v] = l->q[s * 3 + v] - dt * (F[1][v] - F[0][v]);
+        l->next[s * 3 + v] = Unew[v];
+    }
+    if (!fused) return INFINITY;
+    const double volume = cell[2].V;
+    const double inv = 1.0 / volume;
+    const double U[3] = {Unew[0] * inv, Unew[1] * inv, Unew[2] * inv};
+    double Wnew[3];
+    orc_primitive(1, &l->gas, U, Wnew);
+    const double relative = fabs(Wnew[1]);
+    return courant * volume / (relative + orc_sound_speed(1, &l->gas, Wnew));
+}
+
+/* finite_volume_1d.hpp:332-386 time_step() on a static line: LUNA's rule or AREPO's (with the wall ghosts' pair terms). */
+double orc_line_time_step(orc_line* l, double courant, int luna_rule) {
+    double smallest = INFINITY;
+#pragma omp parallel for reduction(min : smallest)
+    for (int s = 0; s < l->n; ++s) {
+        double Wi[3];
+        line_primitive(l, s, Wi);
+        const double ci = orc_sound_speed(1, &l->gas, Wi);
+        const double volume = orc_line_volume(l, s);
+        const double vi = fabs(Wi[1]);
+        if (luna_rule) {
+            smallest = fmin(smallest, courant * volume / (vi + ci));
+            continue;
+        }
+        const double r = 0.5 * volume;
+        double step = r / (ci + vi);
+        for (int side = -1; side <= 1; side += 2) {
+            const int t = line_slot(l, s, side);
+            double Wj[3];
+            if (t >= 0)
+                line_primitive(l, t, Wj);
+            else
+                line_ghost(l, Wi, side < 0, Wj);
+            const double dist = line_length(l, side < 0 ? s - 1 : s);
+            const double vsig = ci + orc_sound_speed(1, &l->gas, Wj) - (Wj[1] - Wi[1]) * side;
+            if (vsig > 0) step = fmin(step, (dist + r) / vsig);
+        }
+        smallest = fmin(smallest, courant * step);
+    }
+    return smallest;
+}
+
+static void line_swap(orc_line* l) {
+    double* t = l->q;
+    l->q = l->next;
+    l->next = t;
+}
+
+/* finite_volume_1d.hpp:389-407 step() (static): one kernel over the slots, then Q and next swap. */
+void orc_line_step(orc_line* l, double dt) {
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < l->n; ++s) line_body(l, s, dt, 0.0, 0);
+    line_swap(l);
+}
+
+/* finite_volume_1d.hpp:412-417, 679-708 step_and_time_step() (static): the step and the next dt by LUNA's rule. */
+double orc_line_step_fused(orc_line* l, double dt, double courant) {
+    double smallest = INFINITY;
+#pragma omp parallel for reduction(min : smallest) schedule(static)
+    for (int s = 0; s < l->n; ++s) smallest = fmin(smallest, line_body(l, s, dt, courant, 1));
+    line_swap(l);
+    return smallest;
+}

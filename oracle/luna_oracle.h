@@ -1,0 +1,102 @@
+/* CPU oracle of luna-cpp's finite-volume hydro step (TEST INFRASTRUCTURE ONLY).
+ *
+ * A plain C99 + OpenMP restatement of the reference's FP64 arithmetic, in the reference's operation order, compiled with
+ * -ffp-contract=off. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only as the checker
+ * or as the timed CPU baseline; the product (libluna_hydro.so) never calls it.
+ *
+ * Every function cites the reference file:line it follows (paths relative to the luna-cpp checkout). Parity pins:
+ *   orc_hll / orc_hllc ............ LUNA golden fluxes (tests/golden/riemann.json), closed forms, symmetry, rotation
+ *   orc_primitive / orc_conserved . round trips, floors (closed forms)
+ *   orc_line_* .................... LUNA's N=16 final states (MC) and 5-step forward states (minmod, MC, van Leer),
+ *                                   exact Riemann L1 bounds
+ *   orc_time_step / orc_step_static (D = 2, 3): closed forms (linear field reproduced by the unlimited fit, uniform state
+ *                                   stays uniform, conservation), brute-force lstsq of the fit, limiter range invariant,
+ *                                   dt closed form on a lattice
+ *   orc_moving_begin / orc_moving_finish: uniform bulk flow stays uniform, conservation; the whole moving step against
+ *                                   the reference's own build is "parity unpinned" (voronoi-cpp and Kokkos are absent).
+ */
+#ifndef LUNA_ORACLE_H
+#define LUNA_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* euler.hpp:82-101 */
+typedef struct {
+    double gamma, smallr, smallc, inverse_gm1;
+} orc_gas;
+
+void orc_gas_init(orc_gas* gas, double gamma);
+void orc_primitive(int D, const orc_gas* gas, const double* U, double* W);
+void orc_conserved(int D, const orc_gas* gas, const double* W, double* U);
+double orc_sound_speed(int D, const orc_gas* gas, const double* W);
+void orc_half_step(int D, const orc_gas* gas, const double* W, const double* g, double dt, double* out);
+void orc_hll(int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, double* F);
+void orc_hllc(int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, double* F);
+
+/* The mesh arrays the step reads (the voronoi-cpp output contract, finite_volume.hpp:257-327).
+ * list entries: f for a face the cell owns, ~f (negative) for one it does not. */
+typedef struct {
+    int D, N, F;
+    const int* owner;
+    const int* neighbour;
+    const int* start;
+    const int* list;
+    const double* area;          /* F x D, |A| n, owner to neighbour */
+    const double* face_centroid; /* F x D, relative to the owner's generator */
+    const double* centroid;      /* N x D, relative to the cell's generator */
+    const double* volume;        /* N */
+    const double* positions;     /* N x D, in [0, side) */
+    double side[3];
+    double step;                 /* snapping grid step */
+    long long period[3];         /* grid steps per side */
+} orc_geometry;
+
+/* Solver state (arrays owned by the caller). limiter: 0 face centroid, 1 none. riemann: 0 HLLC, 1 HLL.
+ * step_rule: 0 AREPO, 1 LUNA (2D only). */
+typedef struct {
+    int D, N, order, limiter, riemann, moving, step_rule;
+    orc_gas gas;
+    double* q;            /* N x V, extensive */
+    double* prims;        /* N x (V + 2): W, sound speed, radius */
+    double* grad;         /* N x (V D) */
+    double* w;            /* N x D point velocities (moving) */
+    double* x;            /* N x D positions (moving) */
+    double* old_centroid; /* N x D (moving) */
+    int* steps;           /* N x D whole grid steps (moving) */
+    double* flux;         /* F x V scratch (at least the largest mesh's F) */
+    long long fallbacks[2];
+    int prims_ready;
+} orc_fv;
+
+void orc_primitives(orc_fv* fv, const orc_geometry* geo);
+double orc_time_step(orc_fv* fv, const orc_geometry* geo, double courant);
+void orc_step_static(orc_fv* fv, const orc_geometry* geo, double dt);
+void orc_moving_begin(orc_fv* fv, const orc_geometry* geo, double dt);
+void orc_moving_finish(orc_fv* fv, const orc_geometry* geo, double dt);
+long long orc_snap(double x, double step, long long period);
+int orc_delta(long long k0, long long k1, long long period);
+
+/* 1D sorted line (finite_volume_1d.hpp), static; slots are cells (order = identity).
+ * wall kinds: 0 reflective, 1 outflow. limiter: 0 face centroid (MC on a uniform line), 1 none, 2 minmod, 3 van Leer. */
+typedef struct {
+    int n, periodic, wall_left, wall_right, order, limiter, riemann;
+    orc_gas gas;
+    const long long* key; /* ascending grid keys */
+    long long period;
+    double step;
+    double* q;    /* n x 3 */
+    double* next; /* n x 3 */
+} orc_line;
+
+double orc_line_time_step(orc_line* line, double courant, int luna_rule);
+void orc_line_step(orc_line* line, double dt);
+double orc_line_step_fused(orc_line* line, double dt, double courant);
+double orc_line_volume(const orc_line* line, int s);
+double orc_line_centroid(const orc_line* line, int s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

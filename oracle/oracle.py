@@ -1,0 +1,234 @@
+"""ctypes front of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads oracle/liboracle.so (built by oracle/Makefile) and mirrors the solver's calls on numpy arrays. Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module; the product never does.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_longlong)
+
+
+class Gas(ctypes.Structure):
+    _fields_ = [("gamma", ctypes.c_double), ("smallr", ctypes.c_double), ("smallc", ctypes.c_double),
+                ("inverse_gm1", ctypes.c_double)]
+
+
+class _Geometry(ctypes.Structure):
+    _fields_ = [("D", ctypes.c_int), ("N", ctypes.c_int), ("F", ctypes.c_int),
+                ("owner", _ip), ("neighbour", _ip), ("start", _ip), ("list", _ip),
+                ("area", _dp), ("face_centroid", _dp), ("centroid", _dp), ("volume", _dp), ("positions", _dp),
+                ("side", ctypes.c_double * 3), ("step", ctypes.c_double), ("period", ctypes.c_longlong * 3)]
+
+
+class _FV(ctypes.Structure):
+    _fields_ = [("D", ctypes.c_int), ("N", ctypes.c_int), ("order", ctypes.c_int), ("limiter", ctypes.c_int),
+                ("riemann", ctypes.c_int), ("moving", ctypes.c_int), ("step_rule", ctypes.c_int), ("gas", Gas),
+                ("q", _dp), ("prims", _dp), ("grad", _dp), ("w", _dp), ("x", _dp), ("old_centroid", _dp),
+                ("steps", _ip), ("flux", _dp), ("fallbacks", ctypes.c_longlong * 2), ("prims_ready", ctypes.c_int)]
+
+
+class _Line(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("periodic", ctypes.c_int), ("wall_left", ctypes.c_int),
+                ("wall_right", ctypes.c_int), ("order", ctypes.c_int), ("limiter", ctypes.c_int),
+                ("riemann", ctypes.c_int), ("gas", Gas), ("key", _lp), ("period", ctypes.c_longlong),
+                ("step", ctypes.c_double), ("q", _dp), ("next", _dp)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = ctypes.CDLL(path)
+        L.orc_time_step.restype = ctypes.c_double
+        L.orc_line_time_step.restype = ctypes.c_double
+        L.orc_line_step_fused.restype = ctypes.c_double
+        L.orc_line_volume.restype = ctypes.c_double
+        L.orc_line_centroid.restype = ctypes.c_double
+        L.orc_sound_speed.restype = ctypes.c_double
+        L.orc_snap.restype = ctypes.c_longlong
+        L.orc_snap.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_longlong]
+        L.orc_delta.argtypes = [ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong]
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a, kind=_dp):
+    return a.ctypes.data_as(kind) if a is not None else None
+
+
+def gas(gamma):
+    g = Gas()
+    lib().orc_gas_init(ctypes.byref(g), ctypes.c_double(gamma))
+    return g
+
+
+def _vec(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def primitive(U, gamma):
+    U = _vec(U)
+    W = np.empty_like(U)
+    g = gas(gamma)
+    lib().orc_primitive(len(U) - 2, ctypes.byref(g), _ptr(U), _ptr(W))
+    return W
+
+
+def conserved(W, gamma):
+    W = _vec(W)
+    U = np.empty_like(W)
+    g = gas(gamma)
+    lib().orc_conserved(len(W) - 2, ctypes.byref(g), _ptr(W), _ptr(U))
+    return U
+
+
+def flux(L, R, n, w, gamma, riemann="hllc"):
+    L, R, n, w = _vec(L), _vec(R), _vec(n), _vec(w)
+    F = np.empty_like(L)
+    g = gas(gamma)
+    fn = lib().orc_hll if riemann == "hll" else lib().orc_hllc
+    fn(len(L) - 2, ctypes.byref(g), _ptr(L), _ptr(R), _ptr(n), _ptr(w), _ptr(F))
+    return F
+
+
+def snap(x, step, period):
+    return lib().orc_snap(ctypes.c_double(x), ctypes.c_double(step), ctypes.c_longlong(period))
+
+
+class Geometry:
+    """Holds a mesh record (dict of numpy arrays, the voronoi-cpp output contract) as the oracle reads it."""
+
+    def __init__(self, mesh):
+        self.mesh = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in mesh.items()}
+        m = self.mesh
+        g = _Geometry()
+        g.D, g.N, g.F = m["dim"], m["cells"], m["faces"]
+        g.owner, g.neighbour = _ptr(m["owner"], _ip), _ptr(m["neighbour"], _ip)
+        g.start, g.list = _ptr(m["start"], _ip), _ptr(m["list"], _ip)
+        g.area, g.face_centroid = _ptr(m["area"]), _ptr(m["face_centroid"])
+        g.centroid, g.volume, g.positions = _ptr(m["centroid"]), _ptr(m["volume"]), _ptr(m["positions"])
+        for a in range(3):
+            g.side[a] = m["side"][a] if a < len(m["side"]) else 0.0
+            g.period[a] = int(m["period"][a]) if a < len(m["period"]) else 0
+        g.step = m["step"]
+        self.c = g
+
+
+class FiniteVolume:
+    """The oracle's FiniteVolume<D> (D = 2, 3): state, time_step, step, and the moving step split at the mesh update."""
+
+    def __init__(self, mesh, gamma, order=1, limiter="face", riemann="hllc", moving=False, step_rule="arepo"):
+        D, N = mesh["dim"], mesh["cells"]
+        V = D + 2
+        self.D, self.N, self.V = D, N, V
+        self.q = np.zeros((N, V))
+        self.prims = np.zeros((N, V + 2))
+        self.grad = np.zeros((N, V * D))
+        self.w = np.zeros((N, D))
+        self.x = np.array(mesh["positions"], dtype=np.float64).reshape(N, D).copy()
+        self.old_centroid = np.zeros((N, D))
+        self.steps = np.zeros((N, D), dtype=np.int32)
+        self.flux = np.zeros((max(1, mesh["faces"]), V))
+        c = _FV()
+        c.D, c.N, c.order = D, N, order
+        c.limiter = {"face": 0, "none": 1}[limiter]
+        c.riemann = {"hllc": 0, "hll": 1}[riemann]
+        c.moving = int(moving)
+        c.step_rule = {"arepo": 0, "luna": 1}[step_rule]
+        c.gas = gas(gamma)
+        c.q, c.prims, c.grad, c.w = _ptr(self.q), _ptr(self.prims), _ptr(self.grad), _ptr(self.w)
+        c.x, c.old_centroid, c.steps, c.flux = _ptr(self.x), _ptr(self.old_centroid), _ptr(self.steps, _ip), _ptr(self.flux)
+        c.prims_ready = 0
+        self.c = c
+        self.set_mesh(mesh)
+
+    def set_mesh(self, mesh):
+        self.geo = Geometry(mesh)
+        if mesh["faces"] > self.flux.shape[0]:
+            self.flux = np.zeros((mesh["faces"], self.V))
+            self.c.flux = _ptr(self.flux)
+
+    def set_state(self, q):
+        self.q[...] = q
+        self.c.prims_ready = 0
+
+    def time_step(self, courant):
+        return lib().orc_time_step(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(courant))
+
+    def step(self, dt):
+        lib().orc_step_static(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(dt))
+
+    def moving_begin(self, dt):
+        lib().orc_moving_begin(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(dt))
+        return self.x.copy()
+
+    def moving_finish(self, mesh, dt):
+        self.set_mesh(mesh)
+        lib().orc_moving_finish(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(dt))
+
+    def fallbacks(self):
+        return int(self.c.fallbacks[0] + self.c.fallbacks[1])
+
+
+WALLS = {"reflective": 0, "outflow": 1}
+LIMITERS = {"face": 0, "none": 1, "minmod": 2, "vanleer": 3}
+
+
+class Line:
+    """The oracle's static FiniteVolume<1> on a sorted line of grid keys (walls or periodic)."""
+
+    def __init__(self, keys, period, step, gamma, periodic=False, walls=("outflow", "outflow"), order=1,
+                 limiter="face", riemann="hllc"):
+        self.keys = np.ascontiguousarray(np.asarray(keys, dtype=np.int64))
+        n = len(self.keys)
+        self.n = n
+        self.q = np.zeros((n, 3))
+        self.next = np.zeros((n, 3))
+        c = _Line()
+        c.n, c.periodic = n, int(periodic)
+        c.wall_left, c.wall_right = WALLS[walls[0]], WALLS[walls[1]]
+        c.order, c.limiter, c.riemann = order, LIMITERS[limiter], {"hllc": 0, "hll": 1}[riemann]
+        c.gas = gas(gamma)
+        c.key, c.period, c.step = _ptr(self.keys, _lp), period, step
+        self._bufs = [self.q, self.next]
+        c.q, c.next = _ptr(self.q), _ptr(self.next)
+        self.c = c
+
+    def _sync(self):
+        # The C side swaps q and next after each step; follow its pointers.
+        addr = ctypes.cast(self.c.q, ctypes.c_void_p).value
+        if addr != self._bufs[0].ctypes.data:
+            self._bufs.reverse()
+
+    @property
+    def state(self):
+        self._sync()
+        return self._bufs[0]
+
+    def set_state(self, q):
+        self.state[...] = q
+
+    def volume(self, s):
+        return lib().orc_line_volume(ctypes.byref(self.c), s)
+
+    def centroid(self, s):
+        return lib().orc_line_centroid(ctypes.byref(self.c), s)
+
+    def time_step(self, courant, rule="luna"):
+        return lib().orc_line_time_step(ctypes.byref(self.c), ctypes.c_double(courant), int(rule == "luna"))
+
+    def step(self, dt):
+        lib().orc_line_step(ctypes.byref(self.c), ctypes.c_double(dt))
+
+    def step_and_time_step(self, dt, courant):
+        return lib().orc_line_step_fused(ctypes.byref(self.c), ctypes.c_double(dt), ctypes.c_double(courant))
