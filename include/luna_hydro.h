@@ -1,0 +1,156 @@
+/* luna_hydro: the finite-volume hydro step of luna-cpp on MI355X (gfx950), as a C-ABI shared library (libluna_hydro.so).
+ *
+ * This is the drop-in boundary of luna-cpp's luna_hydro_library for the hydro step: each entry point replaces a member of
+ * hydro::FiniteVolume<D> (src/hydro/finite_volume.hpp) or FiniteVolume<1> (src/hydro/finite_volume_1d.hpp), cited
+ * beside it. The reference is a C++ template class on Kokkos; its C++ facade over this C-ABI is include/luna/finite_volume.hpp.
+ * Every kernel is hand-written HIP for CDNA4, FP64 throughout (the reference's Precision<> policy). One HIP stream per
+ * handle; a handle is not thread-safe (as the reference). Errors: every entry point returns LUNA_OK (0) or an error code,
+ * with the message in luna_last_error() (the reference throws std::invalid_argument / std::logic_error instead).
+ */
+#ifndef LUNA_HYDRO_H
+#define LUNA_HYDRO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    LUNA_OK = 0,
+    LUNA_ERR_ARGUMENT = 1, /* std::invalid_argument in the reference (order not 0/1, bad sizes) */
+    LUNA_ERR_LOGIC = 2,    /* std::logic_error in the reference (call not valid for this motion / state) */
+    LUNA_ERR_DEVICE = 3,   /* a HIP runtime error */
+    LUNA_ERR_MEMORY = 4    /* device allocation failed */
+};
+
+/* finite_volume.hpp:25-34 */
+enum { LUNA_LIMITER_FACE_CENTROID = 0, LUNA_LIMITER_NONE = 1, LUNA_LIMITER_MINMOD = 2, LUNA_LIMITER_VAN_LEER = 3 };
+enum { LUNA_RIEMANN_HLLC = 0, LUNA_RIEMANN_HLL = 1 };
+enum { LUNA_MOTION_STATIC = 0, LUNA_MOTION_MOVING = 1 };
+enum { LUNA_STEP_AREPO = 0, LUNA_STEP_LUNA = 1 };
+enum { LUNA_WALL_REFLECTIVE = 0, LUNA_WALL_OUTFLOW = 1 };
+
+/* hydro::Gas (euler.hpp:82-101) and the constructor's options (finite_volume.hpp:79-80). The update loop is the
+ * reference's per-face one (Update::per_face): flux rows, then a deterministic per-cell gather in list order. */
+typedef struct {
+    int dim;       /* 2 or 3 */
+    int order;     /* 0 or 1 */
+    int limiter;   /* LUNA_LIMITER_FACE_CENTROID or LUNA_LIMITER_NONE */
+    int riemann;   /* LUNA_RIEMANN_HLLC or LUNA_RIEMANN_HLL */
+    int motion;    /* LUNA_MOTION_STATIC or LUNA_MOTION_MOVING (order 1) */
+    int step_rule; /* LUNA_STEP_AREPO, or LUNA_STEP_LUNA in 2D (set_step_rule, finite_volume.hpp:181) */
+    double gamma, smallr, smallc;
+} luna_fv_options;
+
+/* The mesh arrays the step reads, host pointers: voronoi-cpp's output contract as finite_volume.hpp:257-327 and :353-364
+ * read it (luna_mesh_record of include/luna_mesh.h starts with the same fields). Row-major [rows][dim] FP64, int32
+ * indices; list entries f (the cell owns face f) or ~f (it does not); area A = |A| n points owner -> neighbour; face
+ * centroids relative to the owner's generator, cell centroids relative to their own. */
+typedef struct {
+    int dim, cells, faces;
+    const int* owner;
+    const int* neighbour;
+    const int* start;
+    const int* list;
+    const double* area;
+    const double* face_centroid;
+    const double* centroid;
+    const double* volume;
+    const double* positions;
+    double side[3];
+    double step;         /* the mesh's snapping grid step (exact_predicates::BoxN::step) */
+    long long period[3]; /* grid steps per box side (BoxN::period) */
+} luna_geometry;
+
+typedef struct luna_fv luna_fv;
+
+/* FiniteVolume(exec, mesh, gas, order, Update::per_face, limiter, motion) (finite_volume.hpp:79-107): allocates the
+ * solver on the current HIP device, uploads the mesh; Q starts at zero. */
+int luna_fv_create(const luna_fv_options* options, const luna_geometry* mesh, luna_fv** out);
+int luna_fv_destroy(luna_fv* fv);
+
+/* state() (finite_volume.hpp:145-148): Q, N x (dim + 2) extensive (mass, momentum, energy), row-major FP64.
+ * set/get copy through the host; state_device returns the device array (and, as state(), drops the cached primitives). */
+int luna_fv_set_state(luna_fv* fv, const double* q);
+int luna_fv_get_state(luna_fv* fv, double* q);
+int luna_fv_state_device(luna_fv* fv, double** q);
+
+/* time_step(courant) (finite_volume.hpp:190-201): primitives, the sweep (limited gradients; moving: point velocities),
+ * and the global dt, read back to the host (blocking). */
+int luna_fv_time_step(luna_fv* fv, double courant, double* dt);
+
+/* step(dt) on a static mesh (finite_volume.hpp:205-226, per-face loop). */
+int luna_fv_step(luna_fv* fv, double dt);
+
+/* step(dt) on a moving mesh at order 1 (step_moving, finite_volume.hpp:1497-1519), split where the reference calls the
+ * mesh library (move(), :1212-1250): begin keeps the centroids of t^n, puts the points on the grid, applies the fluxes of
+ * the mesh of t^n over dt/2 and drifts the points by their whole grid steps; it returns the new positions (N x dim, host).
+ * The caller builds the mesh of those positions and passes it to finish, which applies the fluxes of the new mesh over
+ * dt/2 and writes the next primitives. */
+int luna_fv_moving_begin(luna_fv* fv, double dt, double* positions);
+int luna_fv_moving_finish(luna_fv* fv, const luna_geometry* mesh, double dt);
+
+/* fallbacks() (finite_volume.hpp:230-238): face sides of the last step that took the cell's state. Fences. */
+int luna_fv_fallbacks(luna_fv* fv, long long* count);
+
+/* positions(), point_velocities() (finite_volume.hpp:150, 157), host copies N x dim. */
+int luna_fv_positions(luna_fv* fv, double* x);
+int luna_fv_point_velocities(luna_fv* fv, double* w);
+
+/* Host copy of the limited gradients of the last time_step(), N x (dim + 2) x dim (density_gradients(),
+ * finite_volume.hpp:165-180, gives the first dim of each row). */
+int luna_fv_gradients(luna_fv* fv, double* g);
+
+/* faces() (finite_volume.hpp:141). */
+int luna_fv_faces(luna_fv* fv, int* faces);
+
+/* Kernel timing with HIP events on the handle's stream (on: 1, off: 0; turning it on clears the sums). read: up to max
+ * entries of (name, launches, total milliseconds), for the launches since it was turned on. */
+typedef struct {
+    char name[32];
+    long long launches;
+    double total_ms;
+} luna_kernel_time;
+int luna_fv_profile(luna_fv* fv, int on);
+int luna_fv_profile_read(luna_fv* fv, luna_kernel_time* out, int max, int* count);
+
+/* ---------------------------------------------------------------- D = 1: FiniteVolume<1> on a static sorted line */
+
+/* The line as finite_volume_1d.hpp reads voronoi-cpp's sorted_mesh::Line: cells in ascending order of their integer grid
+ * keys, grid step `step`, `period` keys per side; walls at 0 and period * step, or periodic. */
+typedef struct {
+    int cells;
+    const long long* keys;
+    long long period;
+    double step;
+    int periodic;
+    int wall_left, wall_right; /* LUNA_WALL_* (finite_volume_1d.hpp:9-14) */
+} luna_line;
+
+typedef struct {
+    int order;   /* 0 or 1 */
+    int limiter; /* any LUNA_LIMITER_* */
+    int riemann;
+    double gamma, smallr, smallc;
+} luna_fv1_options;
+
+typedef struct luna_fv1 luna_fv1;
+
+/* FiniteVolume<1>(exec, mesh, gas, order, limiter, static_mesh, walls) (finite_volume_1d.hpp:41-53). */
+int luna_fv1_create(const luna_fv1_options* options, const luna_line* line, luna_fv1** out);
+int luna_fv1_destroy(luna_fv1* fv);
+int luna_fv1_set_state(luna_fv1* fv, const double* q);
+int luna_fv1_get_state(luna_fv1* fv, double* q);
+/* time_step(courant, rule) (finite_volume_1d.hpp:332-386); rule LUNA_STEP_AREPO or LUNA_STEP_LUNA. */
+int luna_fv1_time_step(luna_fv1* fv, double courant, int rule, double* dt);
+/* step(dt) (finite_volume_1d.hpp:389-407). */
+int luna_fv1_step(luna_fv1* fv, double dt);
+/* step_and_time_step(dt, courant) (finite_volume_1d.hpp:412-417): the step and the next dt by LUNA's rule, one launch. */
+int luna_fv1_step_and_time_step(luna_fv1* fv, double dt, double courant, double* next);
+
+/* The last error message of this thread. */
+const char* luna_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

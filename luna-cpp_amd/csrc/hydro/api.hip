@@ -1,0 +1,658 @@
+// The C-ABI of libluna_hydro.so (include/luna_hydro.h): solver handles, device memory, the launch sequence of a step,
+// HIP-event kernel timing and error reporting. One HIP stream per handle (default priority).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/luna_hydro.h"
+#include "kernels.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& what) {
+    g_error = what;
+    return code;
+}
+
+#define LUNA_HIP(call)                                                                                       \
+    do {                                                                                                     \
+        const hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) return fail(LUNA_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// A device array that grows by reallocation (moving meshes change their face count every step).
+template <class T>
+struct Buf {
+    T* p = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        const size_t want = n + n / 8;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * (want > 0 ? want : 1));
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Kernel timing: events around each launch while on; read at the next synchronisation.
+struct Profiler {
+    bool on = false;
+    struct Pending {
+        int name;
+        hipEvent_t start, stop;
+    };
+    std::vector<std::string> names;
+    std::vector<long long> launches;
+    std::vector<double> total_ms;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+
+    int id(const char* name) {
+        for (size_t k = 0; k < names.size(); ++k)
+            if (names[k] == name) return static_cast<int>(k);
+        names.emplace_back(name);
+        launches.push_back(0);
+        total_ms.push_back(0);
+        return static_cast<int>(names.size() - 1);
+    }
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    hipError_t collect() {
+        for (const Pending& p : pending) {
+            hipError_t e = hipEventSynchronize(p.stop);
+            if (e != hipSuccess) return e;
+            float ms = 0;
+            e = hipEventElapsedTime(&ms, p.start, p.stop);
+            if (e != hipSuccess) return e;
+            launches[p.name] += 1;
+            total_ms[p.name] += ms;
+            pool.push_back(p.start);
+            pool.push_back(p.stop);
+        }
+        pending.clear();
+        return hipSuccess;
+    }
+    void clear() {
+        (void)collect();
+        for (auto& t : total_ms) t = 0;
+        for (auto& l : launches) l = 0;
+    }
+    void destroy() {
+        (void)collect();
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+        pool.clear();
+    }
+};
+
+// Runs launch() between two events when profiling.
+template <class F>
+hipError_t timed(Profiler& prof, hipStream_t s, const char* name, F&& launch) {
+    if (!prof.on) return launch();
+    Profiler::Pending p{prof.id(name), prof.take(), prof.take()};
+    hipError_t e = hipEventRecord(p.start, s);
+    if (e != hipSuccess) return e;
+    e = launch();
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(p.stop, s);
+    if (e != hipSuccess) return e;
+    prof.pending.push_back(p);
+    return hipSuccess;
+}
+
+double bits_to_double(unsigned long long b) {
+    double d;
+    std::memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+}  // namespace
+
+struct luna_fv {
+    luna_fv_options opt{};
+    int D = 3, V = 5, N = 0, F = 0, E = 0;
+    hipStream_t stream = nullptr;
+    Buf<int> owner, neighbour, start, list, steps;
+    Buf<double> area, fc, magnitude, centroid, volume, positions;
+    Buf<double> q, prims, grad, w, x, old_centroid, flux;
+    Buf<unsigned long long> counters;  // [0, 1] dt slots, [2, 3] fallbacks of the step's two flux passes
+    unsigned long long* host = nullptr;  // pinned: read-backs
+    double side[3] = {0, 0, 0};
+    luna::Grid grid{};
+    int parity = 0;
+    bool prims_ready = false, swept = false, begun = false;
+    Profiler prof;
+
+    luna::Geo geo() const {
+        luna::Geo g{};
+        g.N = N;
+        g.F = F;
+        g.owner = owner.p;
+        g.neighbour = neighbour.p;
+        g.start = start.p;
+        g.list = list.p;
+        g.area = area.p;
+        g.fc = fc.p;
+        g.magnitude = magnitude.p;
+        g.centroid = centroid.p;
+        g.volume = volume.p;
+        g.positions = positions.p;
+        for (int a = 0; a < 3; ++a) g.side[a] = side[a];
+        return g;
+    }
+    luna::Rows rows() const {
+        return luna::Rows{q.p, prims.p, grad.p, w.p, x.p, old_centroid.p, steps.p, flux.p};
+    }
+    luna::Gas gas() const { return luna::Gas{opt.gamma, opt.smallr, opt.smallc, 1.0 / (opt.gamma - 1.0)}; }
+    luna::Counters counters_now() const {
+        return luna::Counters{counters.p + parity, counters.p + (parity ^ 1), counters.p + 2};
+    }
+};
+
+namespace {
+
+int check_geometry(const luna_geometry* m, int D, int N) {
+    if (!m) return fail(LUNA_ERR_ARGUMENT, "luna: null mesh");
+    if (m->dim != D) return fail(LUNA_ERR_ARGUMENT, "luna: mesh dimension differs from the solver's");
+    if (m->cells != N) return fail(LUNA_ERR_ARGUMENT, "luna: mesh cell count differs from the solver's");
+    if (m->faces < 0 || !m->owner || !m->neighbour || !m->start || !m->list || !m->area || !m->face_centroid || !m->centroid ||
+        !m->volume || !m->positions)
+        return fail(LUNA_ERR_ARGUMENT, "luna: mesh arrays missing");
+    if (m->start[0] != 0) return fail(LUNA_ERR_ARGUMENT, "luna: CSR start[0] must be 0");
+    for (int a = 0; a < D; ++a)
+        if (!(m->side[a] > 0)) return fail(LUNA_ERR_ARGUMENT, "luna: box sides must be positive");
+    return LUNA_OK;
+}
+
+// Uploads a mesh: the face and cell arrays, |A| per face. Host checks: the index ranges the kernels rely on.
+int upload(luna_fv* fv, const luna_geometry* m) {
+    const int D = fv->D, N = fv->N;
+    if (int rc = check_geometry(m, D, N)) return rc;
+    if (fv->opt.motion == LUNA_MOTION_MOVING) {
+        if (!(m->step > 0)) return fail(LUNA_ERR_ARGUMENT, "luna: a moving mesh needs its snapping grid step");
+        for (int a = 0; a < D; ++a)
+            if (m->period[a] <= 0) return fail(LUNA_ERR_ARGUMENT, "luna: a moving mesh needs the grid's periods");
+    }
+    const int F = m->faces, E = m->start[N];
+    if (E != 2 * F) return fail(LUNA_ERR_ARGUMENT, "luna: every face must appear in two lists");
+    for (int i = 0; i < N; ++i)
+        if (m->start[i + 1] < m->start[i]) return fail(LUNA_ERR_ARGUMENT, "luna: CSR start must not decrease");
+    for (int f = 0; f < F; ++f)
+        if (m->owner[f] < 0 || m->owner[f] >= N || m->neighbour[f] < 0 || m->neighbour[f] >= N)
+            return fail(LUNA_ERR_ARGUMENT, "luna: face cell index out of range");
+    for (int k = 0; k < E; ++k) {
+        const int e = m->list[k], f = e >= 0 ? e : ~e;
+        if (f >= F) return fail(LUNA_ERR_ARGUMENT, "luna: list entry out of range");
+    }
+    LUNA_HIP(fv->owner.reserve(F));
+    LUNA_HIP(fv->neighbour.reserve(F));
+    LUNA_HIP(fv->list.reserve(E));
+    LUNA_HIP(fv->area.reserve(static_cast<size_t>(F) * D));
+    LUNA_HIP(fv->fc.reserve(static_cast<size_t>(F) * D));
+    LUNA_HIP(fv->magnitude.reserve(F));
+    LUNA_HIP(fv->flux.reserve(static_cast<size_t>(F) * fv->V));
+    hipStream_t s = fv->stream;
+    LUNA_HIP(hipMemcpyAsync(fv->owner.p, m->owner, sizeof(int) * F, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->neighbour.p, m->neighbour, sizeof(int) * F, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->start.p, m->start, sizeof(int) * (N + 1), hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->list.p, m->list, sizeof(int) * E, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->area.p, m->area, sizeof(double) * F * D, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->fc.p, m->face_centroid, sizeof(double) * F * D, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->centroid.p, m->centroid, sizeof(double) * N * D, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->volume.p, m->volume, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->positions.p, m->positions, sizeof(double) * N * D, hipMemcpyHostToDevice, s));
+    fv->F = F;
+    fv->E = E;
+    for (int a = 0; a < 3; ++a) fv->side[a] = a < D ? m->side[a] : 0.0;
+    fv->grid.step = m->step;
+    fv->grid.per_step = m->step > 0 ? 1.0 / m->step : 0.0;
+    for (int a = 0; a < 3; ++a) fv->grid.period[a] = a < D ? m->period[a] : 1;
+    const luna::Geo g = fv->geo();
+    LUNA_HIP(timed(fv->prof, s, "magnitudes", [&] { return D == 3 ? luna::launch_magnitudes<3>(s, g, fv->magnitude.p)
+                                                                  : luna::launch_magnitudes<2>(s, g, fv->magnitude.p); }));
+    LUNA_HIP(hipStreamSynchronize(s));  // the host arrays may go away after this call
+    return LUNA_OK;
+}
+
+int primitives(luna_fv* fv) {
+    const luna::Geo g = fv->geo();
+    const luna::Rows r = fv->rows();
+    const luna::Gas gas = fv->gas();
+    hipStream_t s = fv->stream;
+    LUNA_HIP(timed(fv->prof, s, "primitives", [&] {
+        return fv->D == 3 ? luna::launch_primitives<3>(s, g, r, gas) : luna::launch_primitives<2>(s, g, r, gas);
+    }));
+    return LUNA_OK;
+}
+
+int sweep(luna_fv* fv, double courant) {
+    const luna::Geo g = fv->geo();
+    const luna::Rows r = fv->rows();
+    const luna::Gas gas = fv->gas();
+    const luna::Counters c = fv->counters_now();
+    const bool moving = fv->opt.motion == LUNA_MOTION_MOVING, grad = fv->opt.order == 1;
+    const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID;
+    hipStream_t s = fv->stream;
+    LUNA_HIP(timed(fv->prof, s, "sweep", [&] {
+        return fv->D == 3 ? luna::launch_sweep<3>(s, g, r, gas, courant, moving, grad, limit, fv->opt.step_rule, c)
+                          : luna::launch_sweep<2>(s, g, r, gas, courant, moving, grad, limit, fv->opt.step_rule, c);
+    }));
+    fv->parity ^= 1;
+    return LUNA_OK;
+}
+
+int ensure_swept(luna_fv* fv) {
+    if (fv->swept) return LUNA_OK;
+    if (!fv->prims_ready)
+        if (int rc = primitives(fv)) return rc;
+    fv->prims_ready = false;
+    return sweep(fv, 1.0);
+}
+
+int flux_and_update(luna_fv* fv, int pass, double flux_dt, double apply_dt, double unit, bool next_prims) {
+    const luna::Geo g = fv->geo();
+    const luna::Rows r = fv->rows();
+    const luna::Gas gas = fv->gas();
+    hipStream_t s = fv->stream;
+    unsigned long long* count = fv->counters.p + (pass == 1 ? 3 : 2);
+    const int riemann = fv->opt.riemann, order = fv->opt.order;
+    LUNA_HIP(timed(fv->prof, s, pass < 0 ? "flux" : pass == 0 ? "flux_pass1" : "flux_pass2", [&] {
+        return fv->D == 3 ? luna::launch_flux<3>(s, g, r, gas, riemann, order, pass, flux_dt, unit, count)
+                          : luna::launch_flux<2>(s, g, r, gas, riemann, order, pass, flux_dt, unit, count);
+    }));
+    LUNA_HIP(timed(fv->prof, s, pass < 0 ? "update" : pass == 0 ? "update_pass1" : "update_pass2", [&] {
+        return fv->D == 3 ? luna::launch_update<3>(s, g, r, gas, apply_dt, next_prims)
+                          : luna::launch_update<2>(s, g, r, gas, apply_dt, next_prims);
+    }));
+    return LUNA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* luna_last_error(void) { return g_error.c_str(); }
+
+int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_fv** out) {
+    if (!opt || !mesh || !out) return fail(LUNA_ERR_ARGUMENT, "luna_fv_create: null argument");
+    *out = nullptr;
+    if (opt->dim != 2 && opt->dim != 3) return fail(LUNA_ERR_ARGUMENT, "hydro::FiniteVolume: D = 2 or 3 (1D: luna_fv1_create)");
+    if (opt->order != 0 && opt->order != 1) return fail(LUNA_ERR_ARGUMENT, "hydro::FiniteVolume: order 0 or 1");
+    if (opt->limiter != LUNA_LIMITER_FACE_CENTROID && opt->limiter != LUNA_LIMITER_NONE)
+        return fail(LUNA_ERR_ARGUMENT, "hydro::FiniteVolume: limiter face_centroid or none in 2D/3D");
+    if (opt->riemann != LUNA_RIEMANN_HLLC && opt->riemann != LUNA_RIEMANN_HLL) return fail(LUNA_ERR_ARGUMENT, "luna: unknown Riemann solver");
+    if (opt->motion != LUNA_MOTION_STATIC && opt->motion != LUNA_MOTION_MOVING) return fail(LUNA_ERR_ARGUMENT, "luna: unknown motion");
+    if (opt->motion == LUNA_MOTION_MOVING && opt->order != 1)
+        return fail(LUNA_ERR_ARGUMENT, "luna: a moving mesh needs order 1 (the two-pass MUSCL-Hancock step)");
+    if (opt->step_rule == LUNA_STEP_LUNA && opt->dim != 2) return fail(LUNA_ERR_ARGUMENT, "StepRule::luna: 2D");
+    if (!(opt->gamma > 1)) return fail(LUNA_ERR_ARGUMENT, "luna: gamma must exceed 1");
+    if (mesh->cells <= 0) return fail(LUNA_ERR_ARGUMENT, "luna: no cells");
+    luna_fv* fv = new luna_fv;
+    fv->opt = *opt;
+    fv->D = opt->dim;
+    fv->V = opt->dim + 2;
+    fv->N = mesh->cells;
+    auto cleanup = [&](int rc) {
+        luna_fv_destroy(fv);
+        return rc;
+    };
+    if (hipStreamCreate(&fv->stream) != hipSuccess) return cleanup(fail(LUNA_ERR_DEVICE, "luna: hipStreamCreate failed"));
+    const size_t N = fv->N, D = fv->D, V = fv->V;
+    hipError_t e = hipSuccess;
+    if ((e = fv->q.reserve(N * V)) != hipSuccess || (e = fv->prims.reserve(N * (V + 2))) != hipSuccess ||
+        (e = fv->grad.reserve(N * V * D)) != hipSuccess || (e = fv->w.reserve(N * D)) != hipSuccess ||
+        (e = fv->centroid.reserve(N * D)) != hipSuccess || (e = fv->volume.reserve(N)) != hipSuccess ||
+        (e = fv->positions.reserve(N * D)) != hipSuccess || (e = fv->start.reserve(N + 1)) != hipSuccess ||
+        (e = fv->counters.reserve(4)) != hipSuccess)
+        return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
+    if (opt->motion == LUNA_MOTION_MOVING) {
+        if ((e = fv->x.reserve(N * D)) != hipSuccess || (e = fv->old_centroid.reserve(N * D)) != hipSuccess ||
+            (e = fv->steps.reserve(N * D)) != hipSuccess)
+            return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
+    }
+    if (hipHostMalloc(reinterpret_cast<void**>(&fv->host), 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(LUNA_ERR_MEMORY, "luna: pinned host allocation failed"));
+    const unsigned long long init[4] = {luna::INF_BITS, luna::INF_BITS, 0, 0};
+    if (hipMemcpy(fv->counters.p, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(LUNA_ERR_DEVICE, "luna: counter initialisation failed"));
+    std::vector<double> zero(N * V, 0.0);
+    if (hipMemcpy(fv->q.p, zero.data(), sizeof(double) * N * V, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(LUNA_ERR_DEVICE, "luna: state initialisation failed"));
+    if (int rc = upload(fv, mesh)) return cleanup(rc);
+    if (opt->motion == LUNA_MOTION_MOVING) {  // copy_positions (finite_volume.hpp:342-349)
+        if (hipMemcpy(fv->x.p, mesh->positions, sizeof(double) * N * D, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(LUNA_ERR_DEVICE, "luna: position upload failed"));
+        std::vector<double> zw(N * D, 0.0);
+        if (hipMemcpy(fv->w.p, zw.data(), sizeof(double) * N * D, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(LUNA_ERR_DEVICE, "luna: velocity initialisation failed"));
+    }
+    *out = fv;
+    return LUNA_OK;
+}
+
+int luna_fv_destroy(luna_fv* fv) {
+    if (!fv) return LUNA_OK;
+    if (fv->stream) (void)hipStreamSynchronize(fv->stream);
+    fv->prof.destroy();
+    for (auto* b : {&fv->area, &fv->fc, &fv->magnitude, &fv->centroid, &fv->volume, &fv->positions, &fv->q, &fv->prims, &fv->grad,
+                    &fv->w, &fv->x, &fv->old_centroid, &fv->flux})
+        b->release();
+    for (auto* b : {&fv->owner, &fv->neighbour, &fv->start, &fv->list, &fv->steps}) b->release();
+    fv->counters.release();
+    if (fv->host) (void)hipHostFree(fv->host);
+    if (fv->stream) (void)hipStreamDestroy(fv->stream);
+    delete fv;
+    return LUNA_OK;
+}
+
+int luna_fv_set_state(luna_fv* fv, const double* q) {
+    if (!fv || !q) return fail(LUNA_ERR_ARGUMENT, "luna_fv_set_state: null argument");
+    LUNA_HIP(hipMemcpyAsync(fv->q.p, q, sizeof(double) * fv->N * fv->V, hipMemcpyHostToDevice, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    fv->prims_ready = false;
+    fv->swept = false;
+    return LUNA_OK;
+}
+
+int luna_fv_get_state(luna_fv* fv, double* q) {
+    if (!fv || !q) return fail(LUNA_ERR_ARGUMENT, "luna_fv_get_state: null argument");
+    LUNA_HIP(hipMemcpyAsync(q, fv->q.p, sizeof(double) * fv->N * fv->V, hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    return LUNA_OK;
+}
+
+int luna_fv_state_device(luna_fv* fv, double** q) {
+    if (!fv || !q) return fail(LUNA_ERR_ARGUMENT, "luna_fv_state_device: null argument");
+    *q = fv->q.p;
+    fv->prims_ready = false;
+    fv->swept = false;
+    return LUNA_OK;
+}
+
+int luna_fv_time_step(luna_fv* fv, double courant, double* dt) {
+    if (!fv || !dt) return fail(LUNA_ERR_ARGUMENT, "luna_fv_time_step: null argument");
+    if (fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_time_step: a moving step is between begin and finish");
+    if (!fv->prims_ready)
+        if (int rc = primitives(fv)) return rc;
+    fv->prims_ready = false;
+    const int slot = fv->parity;
+    if (int rc = sweep(fv, courant)) return rc;
+    LUNA_HIP(hipMemcpyAsync(fv->host, fv->counters.p + slot, sizeof(unsigned long long), hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    LUNA_HIP(fv->prof.collect());
+    *dt = bits_to_double(fv->host[0]);
+    fv->swept = true;
+    return LUNA_OK;
+}
+
+int luna_fv_step(luna_fv* fv, double dt) {
+    if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv_step: null argument");
+    if (fv->opt.motion != LUNA_MOTION_STATIC)
+        return fail(LUNA_ERR_LOGIC, "luna_fv_step: a moving mesh steps with luna_fv_moving_begin / luna_fv_moving_finish");
+    if (int rc = ensure_swept(fv)) return rc;
+    fv->swept = false;
+    if (int rc = flux_and_update(fv, -1, dt, dt, 0.0, true)) return rc;
+    fv->prims_ready = true;
+    return LUNA_OK;
+}
+
+int luna_fv_moving_begin(luna_fv* fv, double dt, double* positions) {
+    if (!fv || !positions) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_begin: null argument");
+    if (fv->opt.motion != LUNA_MOTION_MOVING) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_begin: a static mesh steps with luna_fv_step");
+    if (fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_begin: the last step was not finished");
+    if (!(dt > 0)) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_begin: dt must be positive");
+    if (int rc = ensure_swept(fv)) return rc;
+    fv->swept = false;
+    const luna::Geo g = fv->geo();
+    const luna::Rows r = fv->rows();
+    hipStream_t s = fv->stream;
+    const luna::Grid grid = fv->grid;
+    LUNA_HIP(timed(fv->prof, s, "keep_centroids", [&] {
+        return fv->D == 3 ? luna::launch_keep_centroids<3>(s, g, r, grid, dt) : luna::launch_keep_centroids<2>(s, g, r, grid, dt);
+    }));
+    const double unit = fv->grid.step / dt;
+    if (int rc = flux_and_update(fv, 0, 0.0, 0.5 * dt, unit, false)) return rc;
+    LUNA_HIP(timed(fv->prof, s, "drift", [&] {
+        return fv->D == 3 ? luna::launch_drift<3>(s, fv->N, r, grid) : luna::launch_drift<2>(s, fv->N, r, grid);
+    }));
+    LUNA_HIP(hipMemcpyAsync(positions, fv->x.p, sizeof(double) * fv->N * fv->D, hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipStreamSynchronize(s));
+    fv->begun = true;
+    return LUNA_OK;
+}
+
+int luna_fv_moving_finish(luna_fv* fv, const luna_geometry* mesh, double dt) {
+    if (!fv || !mesh) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_finish: null argument");
+    if (!fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_finish: no step was begun");
+    if (int rc = upload(fv, mesh)) return rc;
+    const double unit = fv->grid.step / dt;
+    if (int rc = flux_and_update(fv, 1, dt, 0.5 * dt, unit, true)) return rc;
+    fv->prims_ready = true;
+    fv->begun = false;
+    return LUNA_OK;
+}
+
+int luna_fv_fallbacks(luna_fv* fv, long long* count) {
+    if (!fv || !count) return fail(LUNA_ERR_ARGUMENT, "luna_fv_fallbacks: null argument");
+    LUNA_HIP(hipMemcpyAsync(fv->host + 2, fv->counters.p + 2, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    LUNA_HIP(fv->prof.collect());
+    *count = static_cast<long long>(fv->host[2] + fv->host[3]);
+    return LUNA_OK;
+}
+
+int luna_fv_positions(luna_fv* fv, double* x) {
+    if (!fv || !x) return fail(LUNA_ERR_ARGUMENT, "luna_fv_positions: null argument");
+    const double* src = fv->opt.motion == LUNA_MOTION_MOVING ? fv->x.p : fv->positions.p;
+    LUNA_HIP(hipMemcpyAsync(x, src, sizeof(double) * fv->N * fv->D, hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    return LUNA_OK;
+}
+
+int luna_fv_point_velocities(luna_fv* fv, double* w) {
+    if (!fv || !w) return fail(LUNA_ERR_ARGUMENT, "luna_fv_point_velocities: null argument");
+    if (fv->opt.motion != LUNA_MOTION_MOVING) return fail(LUNA_ERR_LOGIC, "luna_fv_point_velocities: static mesh");
+    LUNA_HIP(hipMemcpyAsync(w, fv->w.p, sizeof(double) * fv->N * fv->D, hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    return LUNA_OK;
+}
+
+int luna_fv_gradients(luna_fv* fv, double* g) {
+    if (!fv || !g) return fail(LUNA_ERR_ARGUMENT, "luna_fv_gradients: null argument");
+    if (fv->opt.order != 1) return fail(LUNA_ERR_LOGIC, "luna_fv_gradients: order 0 keeps no gradients");
+    LUNA_HIP(hipMemcpyAsync(g, fv->grad.p, sizeof(double) * fv->N * fv->V * fv->D, hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    return LUNA_OK;
+}
+
+int luna_fv_faces(luna_fv* fv, int* faces) {
+    if (!fv || !faces) return fail(LUNA_ERR_ARGUMENT, "luna_fv_faces: null argument");
+    *faces = fv->F;
+    return LUNA_OK;
+}
+
+int luna_fv_profile(luna_fv* fv, int on) {
+    if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv_profile: null argument");
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    fv->prof.clear();
+    fv->prof.on = on != 0;
+    return LUNA_OK;
+}
+
+int luna_fv_profile_read(luna_fv* fv, luna_kernel_time* out, int max, int* count) {
+    if (!fv || !count) return fail(LUNA_ERR_ARGUMENT, "luna_fv_profile_read: null argument");
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    LUNA_HIP(fv->prof.collect());
+    const int n = static_cast<int>(fv->prof.names.size());
+    *count = n;
+    for (int k = 0; k < n && k < max && out; ++k) {
+        std::snprintf(out[k].name, sizeof out[k].name, "%s", fv->prof.names[k].c_str());
+        out[k].launches = fv->prof.launches[k];
+        out[k].total_ms = fv->prof.total_ms[k];
+    }
+    return LUNA_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------------------- D = 1
+
+struct luna_fv1 {
+    luna_fv1_options opt{};
+    int n = 0, periodic = 0, wall_left = 1, wall_right = 1;
+    long long period = 0;
+    double step = 0;
+    hipStream_t stream = nullptr;
+    Buf<long long> keys;
+    Buf<double> q, next;
+    Buf<unsigned long long> counters;
+    unsigned long long* host = nullptr;
+    int parity = 0;
+
+    luna::LineArgs args() const {
+        luna::LineArgs a{};
+        a.n = n;
+        a.periodic = periodic;
+        a.wall_left = wall_left;
+        a.wall_right = wall_right;
+        a.order = opt.order;
+        a.limiter = opt.limiter;
+        a.riemann = opt.riemann;
+        a.gas = luna::Gas{opt.gamma, opt.smallr, opt.smallc, 1.0 / (opt.gamma - 1.0)};
+        a.key = keys.p;
+        a.period = period;
+        a.step = step;
+        a.q = q.p;
+        a.next = next.p;
+        return a;
+    }
+    luna::Counters counters_now() const { return luna::Counters{counters.p + parity, counters.p + (parity ^ 1), counters.p + 2}; }
+};
+
+extern "C" {
+
+int luna_fv1_create(const luna_fv1_options* opt, const luna_line* line, luna_fv1** out) {
+    if (!opt || !line || !out || !line->keys) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_create: null argument");
+    *out = nullptr;
+    if (opt->order != 0 && opt->order != 1) return fail(LUNA_ERR_ARGUMENT, "hydro::FiniteVolume<1>: order 0 or 1");
+    if (opt->limiter < 0 || opt->limiter > 3) return fail(LUNA_ERR_ARGUMENT, "luna: unknown limiter");
+    if (opt->riemann != LUNA_RIEMANN_HLLC && opt->riemann != LUNA_RIEMANN_HLL) return fail(LUNA_ERR_ARGUMENT, "luna: unknown Riemann solver");
+    if (line->cells < 3) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_create: at least 3 cells");
+    if (line->period <= 0 || !(line->step > 0)) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_create: bad grid");
+    for (int s = 0; s < line->cells; ++s) {
+        if (line->keys[s] < 0 || line->keys[s] >= line->period) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_create: key outside the line");
+        if (s > 0 && line->keys[s] <= line->keys[s - 1]) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_create: keys must ascend");
+    }
+    for (int wall : {line->wall_left, line->wall_right})
+        if (!line->periodic && wall != LUNA_WALL_REFLECTIVE && wall != LUNA_WALL_OUTFLOW)
+            return fail(LUNA_ERR_ARGUMENT, "luna_fv1_create: walls reflective or outflow (inflow supplies cells on a moving line only)");
+    luna_fv1* fv = new luna_fv1;
+    fv->opt = *opt;
+    fv->n = line->cells;
+    fv->periodic = line->periodic;
+    fv->wall_left = line->wall_left;
+    fv->wall_right = line->wall_right;
+    fv->period = line->period;
+    fv->step = line->step;
+    auto cleanup = [&](int rc) {
+        luna_fv1_destroy(fv);
+        return rc;
+    };
+    const size_t n = fv->n;
+    if (hipStreamCreate(&fv->stream) != hipSuccess) return cleanup(fail(LUNA_ERR_DEVICE, "luna: hipStreamCreate failed"));
+    if (fv->keys.reserve(n) != hipSuccess || fv->q.reserve(3 * n) != hipSuccess || fv->next.reserve(3 * n) != hipSuccess ||
+        fv->counters.reserve(4) != hipSuccess)
+        return cleanup(fail(LUNA_ERR_MEMORY, "luna: device allocation failed"));
+    if (hipHostMalloc(reinterpret_cast<void**>(&fv->host), 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(LUNA_ERR_MEMORY, "luna: pinned host allocation failed"));
+    const unsigned long long init[4] = {luna::INF_BITS, luna::INF_BITS, 0, 0};
+    std::vector<double> zero(3 * n, 0.0);
+    if (hipMemcpy(fv->counters.p, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(fv->keys.p, line->keys, sizeof(long long) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(fv->q.p, zero.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(LUNA_ERR_DEVICE, "luna: initial upload failed"));
+    *out = fv;
+    return LUNA_OK;
+}
+
+int luna_fv1_destroy(luna_fv1* fv) {
+    if (!fv) return LUNA_OK;
+    if (fv->stream) (void)hipStreamSynchronize(fv->stream);
+    fv->keys.release();
+    fv->q.release();
+    fv->next.release();
+    fv->counters.release();
+    if (fv->host) (void)hipHostFree(fv->host);
+    if (fv->stream) (void)hipStreamDestroy(fv->stream);
+    delete fv;
+    return LUNA_OK;
+}
+
+int luna_fv1_set_state(luna_fv1* fv, const double* q) {
+    if (!fv || !q) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_set_state: null argument");
+    LUNA_HIP(hipMemcpyAsync(fv->q.p, q, sizeof(double) * 3 * fv->n, hipMemcpyHostToDevice, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    return LUNA_OK;
+}
+
+int luna_fv1_get_state(luna_fv1* fv, double* q) {
+    if (!fv || !q) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_get_state: null argument");
+    LUNA_HIP(hipMemcpyAsync(q, fv->q.p, sizeof(double) * 3 * fv->n, hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    return LUNA_OK;
+}
+
+int luna_fv1_time_step(luna_fv1* fv, double courant, int rule, double* dt) {
+    if (!fv || !dt) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_time_step: null argument");
+    const int slot = fv->parity;
+    LUNA_HIP(luna::launch_line_time_step(fv->stream, fv->args(), courant, rule == LUNA_STEP_LUNA, fv->counters_now()));
+    fv->parity ^= 1;
+    LUNA_HIP(hipMemcpyAsync(fv->host, fv->counters.p + slot, sizeof(unsigned long long), hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    *dt = bits_to_double(fv->host[0]);
+    return LUNA_OK;
+}
+
+int luna_fv1_step(luna_fv1* fv, double dt) {
+    if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_step: null argument");
+    LUNA_HIP(luna::launch_line_step(fv->stream, fv->args(), dt, 0.0, false, fv->counters_now()));
+    std::swap(fv->q, fv->next);
+    return LUNA_OK;
+}
+
+int luna_fv1_step_and_time_step(luna_fv1* fv, double dt, double courant, double* next) {
+    if (!fv || !next) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_step_and_time_step: null argument");
+    const int slot = fv->parity;
+    LUNA_HIP(luna::launch_line_step(fv->stream, fv->args(), dt, courant, true, fv->counters_now()));
+    fv->parity ^= 1;
+    std::swap(fv->q, fv->next);
+    LUNA_HIP(hipMemcpyAsync(fv->host, fv->counters.p + slot, sizeof(unsigned long long), hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    *next = bits_to_double(fv->host[0]);
+    return LUNA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+"""Build recipe of the native libraries (in-tree, so they travel with the source tree):
+
+  libluna_hydro.so  hand-written HIP for gfx950 (csrc/hydro/*.hip), hipcc, FP64 with -ffp-contract=off
+  libluna_mesh.so   the periodic Voronoi geometry producer (csrc/mesh/voronoi.cpp), g++ with OpenMP
+
+Run `python luna-cpp_amd/luna_hydro/build.py`, or __graft_entry__.build().
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+
+HIP_SOURCES = ["hydro/mesh_kernels.hip", "hydro/line_kernels.hip", "hydro/api.hip"]
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    # hipcc's host pass does not know the packed-fp32 feature and says so; that line is harmless.
+    out = "\n".join(l for l in res.stdout.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    if out:
+        print(out)
+    if res.returncode != 0:
+        raise RuntimeError("build failed: " + " ".join(cmd))
+
+
+def build_hydro():
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    out = os.path.join(HERE, "libluna_hydro.so")
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    _run([hipcc] + HIPCC_FLAGS + ["-I" + INCLUDE, "-o", out + ".tmp"] + srcs)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_mesh():
+    out = os.path.join(HERE, "libluna_mesh.so")
+    _run(["g++", "-O3", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-Wall", "-Wextra", "-I" + INCLUDE,
+          "-o", out + ".tmp", os.path.join(CSRC, "mesh", "voronoi.cpp")])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_all():
+    return [build_mesh(), build_hydro()]
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,247 @@
+"""Python mirror of hydro::FiniteVolume<D> over the C-ABI of libluna_hydro.so (include/luna_hydro.h).
+
+The names and argument meanings follow luna-cpp's src/hydro/finite_volume.hpp (D = 2, 3) and finite_volume_1d.hpp (D = 1):
+state(), time_step(courant), step(dt), fallbacks(), positions(), point_velocities(); a moving step is
+moving_begin(dt) -> (the caller builds the mesh of the returned positions) -> moving_finish(mesh, dt), the split at the
+reference's call into the mesh library (move(), finite_volume.hpp:1212-1250). Errors raise ValueError for the
+reference's std::invalid_argument, RuntimeError otherwise. There is no CPU fallback: without the HIP library or a GPU
+every call fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_longlong)
+
+OK, ERR_ARGUMENT, ERR_LOGIC, ERR_DEVICE, ERR_MEMORY = range(5)
+LIMITERS = {"face": 0, "none": 1, "minmod": 2, "vanleer": 3}
+RIEMANN = {"hllc": 0, "hll": 1}
+MOTION = {"static": 0, "moving": 1}
+RULES = {"arepo": 0, "luna": 1}
+WALLS = {"reflective": 0, "outflow": 1}
+
+
+class FVOptions(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("limiter", ctypes.c_int), ("riemann", ctypes.c_int),
+                ("motion", ctypes.c_int), ("step_rule", ctypes.c_int), ("gamma", ctypes.c_double),
+                ("smallr", ctypes.c_double), ("smallc", ctypes.c_double)]
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("cells", ctypes.c_int), ("faces", ctypes.c_int),
+                ("owner", _ip), ("neighbour", _ip), ("start", _ip), ("list", _ip),
+                ("area", _dp), ("face_centroid", _dp), ("centroid", _dp), ("volume", _dp), ("positions", _dp),
+                ("side", ctypes.c_double * 3), ("step", ctypes.c_double), ("period", ctypes.c_longlong * 3)]
+
+
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_longlong), ("total_ms", ctypes.c_double)]
+
+
+class LineRecord(ctypes.Structure):
+    _fields_ = [("cells", ctypes.c_int), ("keys", _lp), ("period", ctypes.c_longlong), ("step", ctypes.c_double),
+                ("periodic", ctypes.c_int), ("wall_left", ctypes.c_int), ("wall_right", ctypes.c_int)]
+
+
+class FV1Options(ctypes.Structure):
+    _fields_ = [("order", ctypes.c_int), ("limiter", ctypes.c_int), ("riemann", ctypes.c_int),
+                ("gamma", ctypes.c_double), ("smallr", ctypes.c_double), ("smallc", ctypes.c_double)]
+
+
+def library_path():
+    return os.path.join(HERE, "libluna_hydro.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = library_path()
+        if not os.path.exists(path):
+            raise RuntimeError("libluna_hydro.so is not built (run __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(path)
+        L.luna_last_error.restype = ctypes.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def _check(rc, what):
+    if rc == OK:
+        return
+    msg = "%s: %s" % (what, lib().luna_last_error().decode(errors="replace"))
+    if rc == ERR_ARGUMENT:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def exported_symbols():
+    """The entry points include/luna_hydro.h declares, as the loaded library exports them."""
+    names = ["luna_fv_create", "luna_fv_destroy", "luna_fv_set_state", "luna_fv_get_state", "luna_fv_state_device",
+             "luna_fv_time_step", "luna_fv_step", "luna_fv_moving_begin", "luna_fv_moving_finish", "luna_fv_fallbacks",
+             "luna_fv_positions", "luna_fv_point_velocities", "luna_fv_gradients", "luna_fv_faces", "luna_fv_profile",
+             "luna_fv_profile_read", "luna_fv1_create", "luna_fv1_destroy", "luna_fv1_set_state", "luna_fv1_get_state",
+             "luna_fv1_time_step", "luna_fv1_step", "luna_fv1_step_and_time_step", "luna_last_error"]
+    return {n: hasattr(lib(), n) for n in names}
+
+
+def geometry_record(mesh):
+    """A luna_geometry over the arrays of a mesh dict (kept alive by the returned holder)."""
+    m = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in mesh.items()}
+    for k in ("owner", "neighbour", "start", "list"):
+        m[k] = np.ascontiguousarray(m[k], dtype=np.int32)
+    for k in ("area", "face_centroid", "centroid", "volume", "positions"):
+        m[k] = np.ascontiguousarray(m[k], dtype=np.float64)
+    g = Geometry()
+    g.dim, g.cells, g.faces = m["dim"], m["cells"], m["faces"]
+    g.owner, g.neighbour = m["owner"].ctypes.data_as(_ip), m["neighbour"].ctypes.data_as(_ip)
+    g.start, g.list = m["start"].ctypes.data_as(_ip), m["list"].ctypes.data_as(_ip)
+    for k in ("area", "face_centroid", "centroid", "volume", "positions"):
+        setattr(g, k, m[k].ctypes.data_as(_dp))
+    for a in range(3):
+        g.side[a] = m["side"][a] if a < len(m["side"]) else 0.0
+        g.period[a] = int(m["period"][a]) if a < len(m["period"]) else 0
+    g.step = m["step"]
+    return g, m
+
+
+class FiniteVolume:
+    """hydro::FiniteVolume<D> (D = 2, 3) on the GPU: FiniteVolume(exec, mesh, gas, order, per_face, limiter, motion)."""
+
+    def __init__(self, mesh, gamma, order=1, limiter="face", riemann="hllc", motion="static", step_rule="arepo",
+                 smallr=1e-10, smallc=1e-10):
+        self.D, self.N = mesh["dim"], mesh["cells"]
+        self.V = self.D + 2
+        opt = FVOptions(self.D, order, LIMITERS[limiter], RIEMANN[riemann], MOTION[motion], RULES[step_rule],
+                        gamma, smallr, smallc)
+        g, self._mesh_arrays = geometry_record(mesh)
+        h = ctypes.c_void_p()
+        _check(lib().luna_fv_create(ctypes.byref(opt), ctypes.byref(g), ctypes.byref(h)), "FiniteVolume")
+        self._h = h
+        self.motion = motion
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().luna_fv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.shape == (self.N, self.V)
+        _check(lib().luna_fv_set_state(self._h, q.ctypes.data_as(_dp)), "set_state")
+
+    def state(self):
+        q = np.empty((self.N, self.V))
+        _check(lib().luna_fv_get_state(self._h, q.ctypes.data_as(_dp)), "state")
+        return q
+
+    def time_step(self, courant):
+        dt = ctypes.c_double()
+        _check(lib().luna_fv_time_step(self._h, ctypes.c_double(courant), ctypes.byref(dt)), "time_step")
+        return dt.value
+
+    def step(self, dt):
+        _check(lib().luna_fv_step(self._h, ctypes.c_double(dt)), "step")
+
+    def moving_begin(self, dt):
+        x = np.empty((self.N, self.D))
+        _check(lib().luna_fv_moving_begin(self._h, ctypes.c_double(dt), x.ctypes.data_as(_dp)), "moving_begin")
+        return x
+
+    def moving_finish(self, mesh, dt):
+        g, arrays = geometry_record(mesh)
+        _check(lib().luna_fv_moving_finish(self._h, ctypes.byref(g), ctypes.c_double(dt)), "moving_finish")
+        self._mesh_arrays = arrays
+
+    def fallbacks(self):
+        c = ctypes.c_longlong()
+        _check(lib().luna_fv_fallbacks(self._h, ctypes.byref(c)), "fallbacks")
+        return c.value
+
+    def positions(self):
+        x = np.empty((self.N, self.D))
+        _check(lib().luna_fv_positions(self._h, x.ctypes.data_as(_dp)), "positions")
+        return x
+
+    def point_velocities(self):
+        w = np.empty((self.N, self.D))
+        _check(lib().luna_fv_point_velocities(self._h, w.ctypes.data_as(_dp)), "point_velocities")
+        return w
+
+    def gradients(self):
+        g = np.empty((self.N, self.V * self.D))
+        _check(lib().luna_fv_gradients(self._h, g.ctypes.data_as(_dp)), "gradients")
+        return g
+
+    def faces(self):
+        f = ctypes.c_int()
+        _check(lib().luna_fv_faces(self._h, ctypes.byref(f)), "faces")
+        return f.value
+
+    def profile(self, on=True):
+        _check(lib().luna_fv_profile(self._h, int(on)), "profile")
+
+    def kernel_times(self):
+        out = (KernelTime * 32)()
+        n = ctypes.c_int()
+        _check(lib().luna_fv_profile_read(self._h, out, 32, ctypes.byref(n)), "profile_read")
+        return {out[k].name.decode(): (out[k].launches, out[k].total_ms) for k in range(min(n.value, 32))}
+
+
+class Line:
+    """hydro::FiniteVolume<1> on a static sorted line of grid keys (walls or periodic)."""
+
+    def __init__(self, keys, period, step, gamma, periodic=False, walls=("outflow", "outflow"), order=1,
+                 limiter="face", riemann="hllc", smallr=1e-10, smallc=1e-10):
+        self.keys = np.ascontiguousarray(np.asarray(keys, dtype=np.int64))
+        self.n = len(self.keys)
+        rec = LineRecord(self.n, self.keys.ctypes.data_as(_lp), int(period), float(step), int(periodic),
+                         WALLS[walls[0]], WALLS[walls[1]])
+        opt = FV1Options(order, LIMITERS[limiter], RIEMANN[riemann], gamma, smallr, smallc)
+        h = ctypes.c_void_p()
+        _check(lib().luna_fv1_create(ctypes.byref(opt), ctypes.byref(rec), ctypes.byref(h)), "FiniteVolume<1>")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().luna_fv1_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.shape == (self.n, 3)
+        _check(lib().luna_fv1_set_state(self._h, q.ctypes.data_as(_dp)), "set_state")
+
+    def state(self):
+        q = np.empty((self.n, 3))
+        _check(lib().luna_fv1_get_state(self._h, q.ctypes.data_as(_dp)), "state")
+        return q
+
+    def time_step(self, courant, rule="luna"):
+        dt = ctypes.c_double()
+        _check(lib().luna_fv1_time_step(self._h, ctypes.c_double(courant), RULES[rule], ctypes.byref(dt)), "time_step")
+        return dt.value
+
+    def step(self, dt):
+        _check(lib().luna_fv1_step(self._h, ctypes.c_double(dt)), "step")
+
+    def step_and_time_step(self, dt, courant):
+        nxt = ctypes.c_double()
+        _check(lib().luna_fv1_step_and_time_step(self._h, ctypes.c_double(dt), ctypes.c_double(courant),
+                                                  ctypes.byref(nxt)), "step_and_time_step")
+        return nxt.value
