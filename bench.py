@@ -1,0 +1,223 @@
+"""Benchmark of the hydro step on one MI355X: cell updates/s of the 3D Sedov blast on a 128^3 jittered Voronoi mesh,
+moving (the headline: the hydro part of the two-pass moving step, the mesh producer excluded and reported apart) and
+static, FP64, with the dominant kernel's HBM roofline fraction from HIP events and the CPU restatement timed beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--static-steps S] [--no-cpu]
+
+One JSON line on stdout (the last line); progress on stderr.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "luna-cpp_amd"))
+
+import numpy as np  # noqa: E402
+
+from luna_hydro import hydro, mesh, problems  # noqa: E402
+
+GAMMA = 5.0 / 3.0
+COURANT = 0.3
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per cell of each kernel: every array a phase needs read once and each output written once
+# (unique bytes; 3D, F/N = 7.95 and E/N = 15.9 as luna-cpp measures them; FP64 values, int32 indices).
+#   sweep+dt: Q-derived rows W (40), cell geometry (V 8, c 24, x 24, CSR 4), list entries 63.6, face records 445.2,
+#             writes g 120 and W 40 -> 768.8 static; moving also writes w (24) -> 792.8
+#   flux:     faces 445.2, W + g + c + x of the cells 208, writes the flux rows 318 -> 971.2; moving pass 1 also reads
+#             the grid steps (w, 24) -> 995.2; pass 2 also the old centroids (24) -> 1019.2
+#   update:   CSR 4, list 63.6, flux rows 318, Q read and written 80 -> 465.6; moving pass 1 with the drift (x r/w 48,
+#             w 24) -> 537.6
+ALGO_BYTES = {
+    "static": {"sweep": 768.8, "flux": 971.2, "update": 465.6},
+    "moving": {"sweep": 792.8, "flux_pass1": 995.2, "update_pass1": 537.6, "flux_pass2": 1019.2, "update_pass2": 465.6},
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def dominant(times, table):
+    best = max((k for k in times if k in table), key=lambda k: times[k][1])
+    launches, total = times[best]
+    return best, launches, total / launches
+
+
+def run_static(m, q, steps, warmup):
+    fv = hydro.FiniteVolume(m, GAMMA)
+    fv.set_state(q)
+    for _ in range(warmup):
+        fv.step(fv.time_step(COURANT))
+    fv.fallbacks()
+    hydro.synchronize()
+    fv.profile(True)
+    walls = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        dt = fv.time_step(COURANT)
+        fv.step(dt)
+        fv.fallbacks()  # fences: the step is done (as hydro_sedov's step wall)
+        walls.append(time.perf_counter() - a)
+    hydro.synchronize()
+    total = time.perf_counter() - t0
+    times = fv.kernel_times()
+    fv.close()
+    return total, walls, times
+
+
+def run_moving(m, q, steps, warmup):
+    fv = hydro.FiniteVolume(m, GAMMA, motion="moving")
+    fv.set_state(q)
+    mesh_s, hydro_s, walls = [], [], []
+    current = m
+
+    def one(timed_kernels):
+        nonlocal current
+        a = time.perf_counter()
+        dt = fv.time_step(COURANT)
+        fv.moving_begin(dt, positions=False)
+        b = time.perf_counter()
+        x = fv.positions()
+        nm = mesh.build(x, 1.0)
+        fv.set_geometry(nm)
+        c = time.perf_counter()
+        fv.moving_finish(None, dt)
+        fv.fallbacks()  # fences
+        d = time.perf_counter()
+        current = nm
+        return (b - a) + (d - c), c - b
+
+    for _ in range(warmup):
+        one(False)
+    hydro.synchronize()
+    fv.profile(True)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        h, mm = one(True)
+        hydro_s.append(h)
+        mesh_s.append(mm)
+        log("  moving step %d: hydro %.3f ms, mesh build + upload %.2f s" % (k, 1e3 * h, mm))
+    hydro.synchronize()
+    total = time.perf_counter() - t0
+    times = fv.kernel_times()
+    fv.close()
+    return total, hydro_s, mesh_s, times
+
+
+def cpu_baseline(m, q, steps=2):
+    """The oracle (CPU restatement of luna-cpp's FP64 algorithm, C + OpenMP) on the same mesh and state: the moving step's
+    hydro part (mesh producer excluded), on this host's cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    threads = os.cpu_count() or 1
+    fv = oracle.FiniteVolume(m, GAMMA, moving=True)
+    fv.set_state(q)
+    hydro_s = []
+    for _ in range(steps):
+        a = time.perf_counter()
+        dt = fv.time_step(COURANT)
+        x = fv.moving_begin(dt)
+        b = time.perf_counter()
+        nm = mesh.build(x, 1.0)
+        c = time.perf_counter()
+        fv.moving_finish(nm, dt)
+        d = time.perf_counter()
+        hydro_s.append((b - a) + (d - c))
+    return {"value": m["cells"] / float(np.median(hydro_s)), "unit": "cell updates/s", "cores": threads,
+            "kind": "port",
+            "sample": "%d moving steps of the same 128^3 Sedov mesh and state (hydro part, median), oracle/ C+OpenMP "
+                      "restatement of luna-cpp's algorithm, OMP threads = cores" % steps}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--static-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; the step's sharding is described in DESIGN.md (no multi-GPU path)",
+              file=sys.stderr)
+        sys.exit(2)
+    n = args.n
+    log("building the %d^3 jittered Voronoi mesh (jitter 0.25, seed 1)" % n)
+    t = time.perf_counter()
+    m = mesh.build(problems.jittered_lattice(n, 3, 0.25, 1), 1.0)
+    build_s = time.perf_counter() - t
+    N, F = m["cells"], m["faces"]
+    q, deposited = problems.sedov_state(m)
+    log("mesh: %d cells, %d faces (F/N %.3f), built in %.1f s; blast in %d cells" % (N, F, F / N, build_s, deposited))
+
+    log("static: %d warm-up + %d timed steps" % (args.warmup, args.static_steps))
+    s_total, s_walls, s_times = run_static(m, q, args.static_steps, args.warmup)
+    static_rate = N * args.static_steps / s_total
+    log("static: %.3f ms/step, %.3g cell updates/s; kernels %s" % (1e3 * s_total / args.static_steps, static_rate,
+                                                                    {k: round(v[1] / v[0], 4) for k, v in s_times.items()}))
+
+    log("moving: %d warm-up + %d timed steps" % (args.warmup, args.steps))
+    m_total, hydro_s, mesh_s, m_times = run_moving(m, q, args.steps, args.warmup)
+    hydro_total = float(np.sum(hydro_s))
+    value = N * args.steps / hydro_total
+    log("moving: hydro %.3f ms/step, %.3g cell updates/s; kernels %s" % (1e3 * hydro_total / args.steps, value,
+                                                                          {k: round(v[1] / v[0], 4) for k, v in m_times.items()}))
+
+    name, launches, avg_ms = dominant(m_times, ALGO_BYTES["moving"])
+    per_cell = ALGO_BYTES["moving"][name]
+    achieved = per_cell * N / (avg_ms * 1e-3) / 1e9
+    sname, _, savg = dominant(s_times, ALGO_BYTES["static"])
+    s_ach = ALGO_BYTES["static"][sname] * N / (savg * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle, %d threads)" % (os.cpu_count() or 1))
+        cpu = cpu_baseline(m, q)
+        log("cpu: %.3g cell updates/s" % cpu["value"])
+
+    step_bytes = sum(ALGO_BYTES["moving"].values())
+    out = {
+        "metric": "cell updates/s at 128^3 moving mesh (hydro part, mesh producer excluded)",
+        "value": value,
+        "unit": "cell updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * hydro_total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded 3D Sedov blast on a jittered lattice (jitter 0.25, seed 1), E = 1 in r < 2h",
+        "config": {"workload": "3D Sedov blast, moving Voronoi mesh, %d^3 = %d cells, %d faces, HLLC, MUSCL-Hancock "
+                               "two-pass moving step, CFL 0.3, FP64" % (n, N, F), "cells": N, "faces": F,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_cell": per_cell, "avg_ms": avg_ms, "launches": launches},
+        "cpu_baseline": cpu,
+        "step_roofline": {"algorithmic_bytes_per_cell": step_bytes,
+                          "achieved_GBs": step_bytes * N * args.steps / hydro_total / 1e9,
+                          "frac": step_bytes * N * args.steps / hydro_total / 1e9 / HBM_PEAK_GBS},
+        "mesh": {"build_s_first": build_s, "build_and_upload_s_per_step": float(np.mean(mesh_s)),
+                 "note": "the Voronoi producer (C++/OpenMP, host) and the geometry upload, excluded from value"},
+        "wall_s_moving_loop": m_total,
+        "kernels_ms_avg": {k: v[1] / v[0] for k, v in m_times.items()},
+        "static": {"value": static_rate, "unit": "cell updates/s", "ms_per_step": 1e3 * s_total / args.static_steps,
+                   "step_median_ms": 1e3 * float(np.median(s_walls)), "kernel": sname, "achieved_GBs": s_ach,
+                   "frac": s_ach / HBM_PEAK_GBS, "step_frac": 2205.6 * static_rate / 1e9 / HBM_PEAK_GBS,
+                   "kernels_ms_avg": {k: v[1] / v[0] for k, v in s_times.items()}},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -82,12 +82,17 @@ int luna_fv_time_step(luna_fv* fv, double courant, double* dt);
 int luna_fv_step(luna_fv* fv, double dt);
 
 /* step(dt) on a moving mesh at order 1 (step_moving, finite_volume.hpp:1497-1519), split where the reference calls the
- * mesh library (move(), :1212-1250): begin keeps the centroids of t^n, puts the points on the grid, applies the fluxes of
- * the mesh of t^n over dt/2 and drifts the points by their whole grid steps; it returns the new positions (N x dim, host).
- * The caller builds the mesh of those positions and passes it to finish, which applies the fluxes of the new mesh over
- * dt/2 and writes the next primitives. */
+ * mesh library (move(), :1212-1250). begin keeps the centroids of t^n, puts the points on the grid, applies the fluxes of
+ * the mesh of t^n over dt/2 and drifts the points by their whole grid steps; positions (N x dim, host) receives the new
+ * positions unless NULL (then luna_fv_positions reads them). The caller builds the mesh of those positions and passes it
+ * to finish, or to luna_fv_set_geometry first and NULL to finish; finish applies the fluxes of the new mesh over dt/2 and
+ * writes the next primitives. */
 int luna_fv_moving_begin(luna_fv* fv, double dt, double* positions);
 int luna_fv_moving_finish(luna_fv* fv, const luna_geometry* mesh, double dt);
+
+/* Uploads a mesh. Between moving_begin and moving_finish: the mesh of the new positions. Otherwise: a caller moved the
+ * mesh itself (mesh_moved(), finite_volume.hpp:152-156), and the cached rows are dropped. */
+int luna_fv_set_geometry(luna_fv* fv, const luna_geometry* mesh);
 
 /* fallbacks() (finite_volume.hpp:230-238): face sides of the last step that took the cell's state. Fences. */
 int luna_fv_fallbacks(luna_fv* fv, long long* count);
@@ -99,6 +104,10 @@ int luna_fv_point_velocities(luna_fv* fv, double* w);
 /* Host copy of the limited gradients of the last time_step(), N x (dim + 2) x dim (density_gradients(),
  * finite_volume.hpp:165-180, gives the first dim of each row). */
 int luna_fv_gradients(luna_fv* fv, double* g);
+
+/* Host copy of the primitive rows the last sweep read (or the next one will), N x (dim + 4): W, sound speed, radius
+ * (finite_volume.hpp:67-68 Prims, written by primitives() :403-415). */
+int luna_fv_primitives(luna_fv* fv, double* rows);
 
 /* faces() (finite_volume.hpp:141). */
 int luna_fv_faces(luna_fv* fv, int* faces);
@@ -149,6 +158,9 @@ int luna_fv1_step_and_time_step(luna_fv1* fv, double dt, double courant, double*
 
 /* The last error message of this thread. */
 const char* luna_last_error(void);
+
+/* Waits for all work on the current device (hipDeviceSynchronize). */
+int luna_synchronize(void);
 
 #ifdef __cplusplus
 }

@@ -2,6 +2,7 @@
 // HIP-event kernel timing and error reporting. One HIP stream per handle (default priority).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -136,15 +137,16 @@ struct luna_fv {
     int D = 3, V = 5, N = 0, F = 0, E = 0;
     hipStream_t stream = nullptr;
     Buf<int> owner, neighbour, start, list, steps;
-    Buf<double> area, fc, magnitude, centroid, volume, positions;
+    Buf<double> area, fc, magnitude, centroid, volume, radius, positions;
     Buf<double> q, prims, grad, w, x, old_centroid, flux;
     Buf<unsigned long long> counters;  // [0, 1] dt slots, [2, 3] fallbacks of the step's two flux passes
     unsigned long long* host = nullptr;  // pinned: read-backs
     double side[3] = {0, 0, 0};
     luna::Grid grid{};
     int parity = 0;
-    bool prims_ready = false, swept = false, begun = false;
+    bool prims_ready = false, swept = false, begun = false, new_mesh = false;
     Profiler prof;
+    std::vector<double> host_radius;
 
     luna::Geo geo() const {
         luna::Geo g{};
@@ -157,8 +159,10 @@ struct luna_fv {
         g.area = area.p;
         g.fc = fc.p;
         g.magnitude = magnitude.p;
+        g.magnitude_out = magnitude.p;
         g.centroid = centroid.p;
         g.volume = volume.p;
+        g.radius = radius.p;
         g.positions = positions.p;
         for (int a = 0; a < 3; ++a) g.side[a] = side[a];
         return g;
@@ -187,8 +191,10 @@ int check_geometry(const luna_geometry* m, int D, int N) {
     return LUNA_OK;
 }
 
-// Uploads a mesh: the face and cell arrays, |A| per face. Host checks: the index ranges the kernels rely on.
-int upload(luna_fv* fv, const luna_geometry* m) {
+// Uploads a mesh: the face and cell arrays, the cell radii, and (magnitudes) |A| per face; on a moving step the second
+// flux pass writes |A| itself, as the reference's keeps them (finite_volume.hpp:1572-1585). Host checks: the index ranges
+// the kernels rely on.
+int upload(luna_fv* fv, const luna_geometry* m, bool magnitudes) {
     const int D = fv->D, N = fv->N;
     if (int rc = check_geometry(m, D, N)) return rc;
     if (fv->opt.motion == LUNA_MOTION_MOVING) {
@@ -224,6 +230,12 @@ int upload(luna_fv* fv, const luna_geometry* m) {
     LUNA_HIP(hipMemcpyAsync(fv->centroid.p, m->centroid, sizeof(double) * N * D, hipMemcpyHostToDevice, s));
     LUNA_HIP(hipMemcpyAsync(fv->volume.p, m->volume, sizeof(double) * N, hipMemcpyHostToDevice, s));
     LUNA_HIP(hipMemcpyAsync(fv->positions.p, m->positions, sizeof(double) * N * D, hipMemcpyHostToDevice, s));
+    // finite_volume.hpp:241-246 radius(): on the host, with the C library's cbrt (sqrt in 2D) that the reference's host
+    // build calls, so the time step has its bits (the device's cbrt rounds some volumes differently).
+    fv->host_radius.resize(N);
+    for (int i = 0; i < N; ++i)
+        fv->host_radius[i] = D == 3 ? std::cbrt(m->volume[i] * (3.0 / (4.0 * M_PI))) : std::sqrt(m->volume[i] * (1.0 / M_PI));
+    LUNA_HIP(hipMemcpyAsync(fv->radius.p, fv->host_radius.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
     fv->F = F;
     fv->E = E;
     for (int a = 0; a < 3; ++a) fv->side[a] = a < D ? m->side[a] : 0.0;
@@ -231,8 +243,9 @@ int upload(luna_fv* fv, const luna_geometry* m) {
     fv->grid.per_step = m->step > 0 ? 1.0 / m->step : 0.0;
     for (int a = 0; a < 3; ++a) fv->grid.period[a] = a < D ? m->period[a] : 1;
     const luna::Geo g = fv->geo();
-    LUNA_HIP(timed(fv->prof, s, "magnitudes", [&] { return D == 3 ? luna::launch_magnitudes<3>(s, g, fv->magnitude.p)
-                                                                  : luna::launch_magnitudes<2>(s, g, fv->magnitude.p); }));
+    if (magnitudes)
+        LUNA_HIP(timed(fv->prof, s, "magnitudes", [&] { return D == 3 ? luna::launch_magnitudes<3>(s, g, fv->magnitude.p)
+                                                                      : luna::launch_magnitudes<2>(s, g, fv->magnitude.p); }));
     LUNA_HIP(hipStreamSynchronize(s));  // the host arrays may go away after this call
     return LUNA_OK;
 }
@@ -325,6 +338,7 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
     if ((e = fv->q.reserve(N * V)) != hipSuccess || (e = fv->prims.reserve(N * (V + 2))) != hipSuccess ||
         (e = fv->grad.reserve(N * V * D)) != hipSuccess || (e = fv->w.reserve(N * D)) != hipSuccess ||
         (e = fv->centroid.reserve(N * D)) != hipSuccess || (e = fv->volume.reserve(N)) != hipSuccess ||
+        (e = fv->radius.reserve(N)) != hipSuccess ||
         (e = fv->positions.reserve(N * D)) != hipSuccess || (e = fv->start.reserve(N + 1)) != hipSuccess ||
         (e = fv->counters.reserve(4)) != hipSuccess)
         return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
@@ -341,7 +355,7 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
     std::vector<double> zero(N * V, 0.0);
     if (hipMemcpy(fv->q.p, zero.data(), sizeof(double) * N * V, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(LUNA_ERR_DEVICE, "luna: state initialisation failed"));
-    if (int rc = upload(fv, mesh)) return cleanup(rc);
+    if (int rc = upload(fv, mesh, true)) return cleanup(rc);
     if (opt->motion == LUNA_MOTION_MOVING) {  // copy_positions (finite_volume.hpp:342-349)
         if (hipMemcpy(fv->x.p, mesh->positions, sizeof(double) * N * D, hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(LUNA_ERR_DEVICE, "luna: position upload failed"));
@@ -357,7 +371,7 @@ int luna_fv_destroy(luna_fv* fv) {
     if (!fv) return LUNA_OK;
     if (fv->stream) (void)hipStreamSynchronize(fv->stream);
     fv->prof.destroy();
-    for (auto* b : {&fv->area, &fv->fc, &fv->magnitude, &fv->centroid, &fv->volume, &fv->positions, &fv->q, &fv->prims, &fv->grad,
+    for (auto* b : {&fv->area, &fv->fc, &fv->magnitude, &fv->centroid, &fv->volume, &fv->radius, &fv->positions, &fv->q, &fv->prims, &fv->grad,
                     &fv->w, &fv->x, &fv->old_centroid, &fv->flux})
         b->release();
     for (auto* b : {&fv->owner, &fv->neighbour, &fv->start, &fv->list, &fv->steps}) b->release();
@@ -420,7 +434,7 @@ int luna_fv_step(luna_fv* fv, double dt) {
 }
 
 int luna_fv_moving_begin(luna_fv* fv, double dt, double* positions) {
-    if (!fv || !positions) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_begin: null argument");
+    if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_begin: null argument");
     if (fv->opt.motion != LUNA_MOTION_MOVING) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_begin: a static mesh steps with luna_fv_step");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_begin: the last step was not finished");
     if (!(dt > 0)) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_begin: dt must be positive");
@@ -438,20 +452,43 @@ int luna_fv_moving_begin(luna_fv* fv, double dt, double* positions) {
     LUNA_HIP(timed(fv->prof, s, "drift", [&] {
         return fv->D == 3 ? luna::launch_drift<3>(s, fv->N, r, grid) : luna::launch_drift<2>(s, fv->N, r, grid);
     }));
-    LUNA_HIP(hipMemcpyAsync(positions, fv->x.p, sizeof(double) * fv->N * fv->D, hipMemcpyDeviceToHost, s));
+    if (positions) LUNA_HIP(hipMemcpyAsync(positions, fv->x.p, sizeof(double) * fv->N * fv->D, hipMemcpyDeviceToHost, s));
     LUNA_HIP(hipStreamSynchronize(s));
     fv->begun = true;
+    fv->new_mesh = false;
+    return LUNA_OK;
+}
+
+int luna_fv_set_geometry(luna_fv* fv, const luna_geometry* mesh) {
+    if (!fv || !mesh) return fail(LUNA_ERR_ARGUMENT, "luna_fv_set_geometry: null argument");
+    if (int rc = upload(fv, mesh, !fv->begun)) return rc;
+    if (fv->begun) {
+        fv->new_mesh = true;
+    } else {  // mesh_moved() (finite_volume.hpp:152-156): the cached rows go
+        fv->prims_ready = false;
+        fv->swept = false;
+    }
     return LUNA_OK;
 }
 
 int luna_fv_moving_finish(luna_fv* fv, const luna_geometry* mesh, double dt) {
-    if (!fv || !mesh) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_finish: null argument");
+    if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_finish: null argument");
     if (!fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_finish: no step was begun");
-    if (int rc = upload(fv, mesh)) return rc;
+    if (mesh) {
+        if (int rc = upload(fv, mesh, false)) return rc;
+    } else if (!fv->new_mesh) {
+        return fail(LUNA_ERR_LOGIC, "luna_fv_moving_finish: no mesh of the new positions (pass one or call luna_fv_set_geometry)");
+    }
     const double unit = fv->grid.step / dt;
     if (int rc = flux_and_update(fv, 1, dt, 0.5 * dt, unit, true)) return rc;
     fv->prims_ready = true;
     fv->begun = false;
+    fv->new_mesh = false;
+    return LUNA_OK;
+}
+
+int luna_synchronize(void) {
+    LUNA_HIP(hipDeviceSynchronize());
     return LUNA_OK;
 }
 
@@ -484,6 +521,13 @@ int luna_fv_gradients(luna_fv* fv, double* g) {
     if (!fv || !g) return fail(LUNA_ERR_ARGUMENT, "luna_fv_gradients: null argument");
     if (fv->opt.order != 1) return fail(LUNA_ERR_LOGIC, "luna_fv_gradients: order 0 keeps no gradients");
     LUNA_HIP(hipMemcpyAsync(g, fv->grad.p, sizeof(double) * fv->N * fv->V * fv->D, hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    return LUNA_OK;
+}
+
+int luna_fv_primitives(luna_fv* fv, double* rows) {
+    if (!fv || !rows) return fail(LUNA_ERR_ARGUMENT, "luna_fv_primitives: null argument");
+    LUNA_HIP(hipMemcpyAsync(rows, fv->prims.p, sizeof(double) * fv->N * (fv->V + 2), hipMemcpyDeviceToHost, fv->stream));
     LUNA_HIP(hipStreamSynchronize(fv->stream));
     return LUNA_OK;
 }

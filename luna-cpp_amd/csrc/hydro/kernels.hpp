@@ -18,8 +18,10 @@ struct Geo {
     const double* area;          // F x D
     const double* fc;            // F x D, relative to the owner's generator
     const double* magnitude;     // F: |A|, computed once per mesh
+    double* magnitude_out;       // the same array, written by a moving step's second flux pass
     const double* centroid;      // N x D, relative to the generator
     const double* volume;        // N
+    const double* radius;        // N: the radius of a sphere (circle) of the cell's volume, from the host's cbrt (sqrt)
     const double* positions;     // N x D
     double side[3];
 };

@@ -85,7 +85,7 @@ __global__ __launch_bounds__(BLOCK) void k_primitives(Geo g, Rows r, Gas gas) {
 #pragma unroll
     for (int v = 0; v < V; ++v) row[v] = W[v];
     row[V] = sound_speed<D>(gas, W);
-    row[V + 1] = radius<D>(g.volume[i]);
+    row[V + 1] = g.radius[i];
 }
 
 // |A| of every face (the magnitudes the reference caches, finite_volume.hpp:1582-1585).
@@ -377,6 +377,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux(Geo g, Rows r, Gas gas, int orde
 #pragma unroll
         for (int k = 0; k < D; ++k) a[k] = g.area[(size_t)f * D + k];
         const double magnitude = sqrt(dot<D>(a, a));
+        if constexpr (Pass == 1) g.magnitude_out[f] = magnitude;  // for the next sweep on this mesh
         double F[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) F[v] = 0;
@@ -471,7 +472,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(Geo g, Rows r, Gas gas, double
 #pragma unroll
         for (int v = 0; v < V; ++v) row[v] = W[v];
         row[V] = sound_speed<D>(gas, W);
-        row[V + 1] = radius<D>(g.volume[i]);
+        row[V + 1] = g.radius[i];
     }
 }
 

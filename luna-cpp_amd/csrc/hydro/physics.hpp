@@ -43,15 +43,6 @@ __device__ __forceinline__ double sound_speed(const Gas& gas, const double* W) {
     return sqrt(gas.gamma * W[D + 1] / W[0]);
 }
 
-// finite_volume.hpp:241-246 radius(): a sphere (3D) or circle (2D) of volume v.
-template <int D>
-__device__ __forceinline__ double radius(double v) {
-    if constexpr (D == 3)
-        return cbrt(v * (3.0 / (4.0 * M_PI)));
-    else
-        return sqrt(v * (1.0 / M_PI));
-}
-
 // finite_volume.hpp:248-253 face_angle()
 template <int D>
 __device__ __forceinline__ double face_angle(double area) {

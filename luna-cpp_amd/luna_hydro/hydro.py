@@ -78,13 +78,18 @@ def _check(rc, what):
     raise RuntimeError(msg)
 
 
+def synchronize():
+    _check(lib().luna_synchronize(), "synchronize")
+
+
 def exported_symbols():
     """The entry points include/luna_hydro.h declares, as the loaded library exports them."""
     names = ["luna_fv_create", "luna_fv_destroy", "luna_fv_set_state", "luna_fv_get_state", "luna_fv_state_device",
              "luna_fv_time_step", "luna_fv_step", "luna_fv_moving_begin", "luna_fv_moving_finish", "luna_fv_fallbacks",
-             "luna_fv_positions", "luna_fv_point_velocities", "luna_fv_gradients", "luna_fv_faces", "luna_fv_profile",
+             "luna_fv_positions", "luna_fv_point_velocities", "luna_fv_gradients", "luna_fv_primitives", "luna_fv_faces", "luna_fv_profile",
              "luna_fv_profile_read", "luna_fv1_create", "luna_fv1_destroy", "luna_fv1_set_state", "luna_fv1_get_state",
-             "luna_fv1_time_step", "luna_fv1_step", "luna_fv1_step_and_time_step", "luna_last_error"]
+             "luna_fv1_time_step", "luna_fv1_step", "luna_fv1_step_and_time_step", "luna_last_error",
+             "luna_fv_set_geometry", "luna_synchronize"]
     return {n: hasattr(lib(), n) for n in names}
 
 
@@ -152,12 +157,23 @@ class FiniteVolume:
     def step(self, dt):
         _check(lib().luna_fv_step(self._h, ctypes.c_double(dt)), "step")
 
-    def moving_begin(self, dt):
-        x = np.empty((self.N, self.D))
-        _check(lib().luna_fv_moving_begin(self._h, ctypes.c_double(dt), x.ctypes.data_as(_dp)), "moving_begin")
+    def moving_begin(self, dt, positions=True):
+        """The first half of a moving step; returns the new positions (None with positions=False)."""
+        x = np.empty((self.N, self.D)) if positions else None
+        ptr = x.ctypes.data_as(_dp) if positions else None
+        _check(lib().luna_fv_moving_begin(self._h, ctypes.c_double(dt), ptr), "moving_begin")
         return x
 
+    def set_geometry(self, mesh):
+        g, arrays = geometry_record(mesh)
+        _check(lib().luna_fv_set_geometry(self._h, ctypes.byref(g)), "set_geometry")
+        self._mesh_arrays = arrays
+
     def moving_finish(self, mesh, dt):
+        """The second half: mesh is the mesh of the new positions, or None after set_geometry()."""
+        if mesh is None:
+            _check(lib().luna_fv_moving_finish(self._h, None, ctypes.c_double(dt)), "moving_finish")
+            return
         g, arrays = geometry_record(mesh)
         _check(lib().luna_fv_moving_finish(self._h, ctypes.byref(g), ctypes.c_double(dt)), "moving_finish")
         self._mesh_arrays = arrays
@@ -181,6 +197,11 @@ class FiniteVolume:
         g = np.empty((self.N, self.V * self.D))
         _check(lib().luna_fv_gradients(self._h, g.ctypes.data_as(_dp)), "gradients")
         return g
+
+    def primitives(self):
+        p = np.empty((self.N, self.V + 2))
+        _check(lib().luna_fv_primitives(self._h, p.ctypes.data_as(_dp)), "primitives")
+        return p
 
     def faces(self):
         f = ctypes.c_int()
