@@ -1,0 +1,172 @@
+// C++ facade of libluna_hydro.so: the member functions of luna-cpp's hydro::FiniteVolume<D> (src/hydro/finite_volume.hpp)
+// and FiniteVolume<1> (src/hydro/finite_volume_1d.hpp), with the same names and argument meanings, over the C-ABI of
+// luna_hydro.h. Errors throw what the reference throws: std::invalid_argument for LUNA_ERR_ARGUMENT, std::logic_error for
+// LUNA_ERR_LOGIC, std::runtime_error otherwise. Views are host std::vectors here (the reference returns Kokkos views);
+// state_device() gives the device array. Header-only; link with -lluna_hydro.
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../luna_hydro.h"
+
+namespace luna {
+
+inline void check(int rc, const char* what) {
+    if (rc == LUNA_OK) return;
+    const std::string msg = std::string(what) + ": " + luna_last_error();
+    if (rc == LUNA_ERR_ARGUMENT) throw std::invalid_argument(msg);
+    if (rc == LUNA_ERR_LOGIC) throw std::logic_error(msg);
+    throw std::runtime_error(msg);
+}
+
+// hydro::Gas (euler.hpp:82-101).
+struct Gas {
+    double gamma = 5.0 / 3.0, smallr = 1e-10, smallc = 1e-10;
+};
+
+enum class Limiter { face_centroid = LUNA_LIMITER_FACE_CENTROID, none = LUNA_LIMITER_NONE, minmod = LUNA_LIMITER_MINMOD,
+                     van_leer = LUNA_LIMITER_VAN_LEER };
+enum class Motion { static_mesh = LUNA_MOTION_STATIC, moving = LUNA_MOTION_MOVING };
+enum class Riemann { hllc = LUNA_RIEMANN_HLLC, hll = LUNA_RIEMANN_HLL };
+enum class StepRule { arepo = LUNA_STEP_AREPO, luna = LUNA_STEP_LUNA };
+
+// hydro::FiniteVolume<D> for D = 2, 3 (Update::per_face, Precision<> = FP64 throughout).
+template <int D>
+class FiniteVolume {
+public:
+    static_assert(D == 2 || D == 3, "luna::FiniteVolume: D = 1 is luna::Line");
+    static constexpr int V = D + 2;
+
+    // FiniteVolume(exec, mesh, gas, order, update, limiter, motion) (finite_volume.hpp:79-107); mesh: the voronoi-cpp
+    // output contract as luna_geometry.
+    FiniteVolume(const luna_geometry& mesh, const Gas& gas, int order, Limiter limiter = Limiter::face_centroid,
+                 Motion motion = Motion::static_mesh, Riemann riemann = Riemann::hllc)
+        : n_(mesh.cells) {
+        luna_fv_options o{};
+        o.dim = D;
+        o.order = order;
+        o.limiter = static_cast<int>(limiter);
+        o.riemann = static_cast<int>(riemann);
+        o.motion = static_cast<int>(motion);
+        o.step_rule = LUNA_STEP_AREPO;
+        o.gamma = gas.gamma;
+        o.smallr = gas.smallr;
+        o.smallc = gas.smallc;
+        options_ = o;
+        check(luna_fv_create(&o, &mesh, &fv_), "hydro::FiniteVolume");
+    }
+    FiniteVolume(const FiniteVolume&) = delete;
+    FiniteVolume& operator=(const FiniteVolume&) = delete;
+    ~FiniteVolume() { luna_fv_destroy(fv_); }
+
+    int cells() const { return n_; }
+    int faces() const {
+        int f = 0;
+        check(luna_fv_faces(fv_, &f), "faces");
+        return f;
+    }
+    // state() (:145-148): here a host copy, and set_state() to write it.
+    std::vector<double> state() const {
+        std::vector<double> q(static_cast<size_t>(n_) * V);
+        check(luna_fv_get_state(fv_, q.data()), "state");
+        return q;
+    }
+    void set_state(const std::vector<double>& q) { check(luna_fv_set_state(fv_, q.data()), "set_state"); }
+    double* state_device() {
+        double* q = nullptr;
+        check(luna_fv_state_device(fv_, &q), "state_device");
+        return q;
+    }
+    // time_step(courant) (:190-201).
+    double time_step(double courant) {
+        double dt = 0;
+        check(luna_fv_time_step(fv_, courant, &dt), "time_step");
+        return dt;
+    }
+    // step(dt) (:205-226) on a static mesh. On a moving mesh the reference's step() calls the mesh library itself; here the
+    // caller passes a producer: std::vector<double> positions -> luna_geometry (kept alive by the caller until return).
+    void step(double dt) { check(luna_fv_step(fv_, dt), "step"); }
+    template <class MeshOf>
+    void step(double dt, MeshOf&& mesh_of) {
+        std::vector<double> x(static_cast<size_t>(n_) * D);
+        check(luna_fv_moving_begin(fv_, dt, x.data()), "step (moving: begin)");
+        const luna_geometry& next = mesh_of(x);
+        check(luna_fv_moving_finish(fv_, &next, dt), "step (moving: finish)");
+    }
+    long long fallbacks() const {
+        long long c = 0;
+        check(luna_fv_fallbacks(fv_, &c), "fallbacks");
+        return c;
+    }
+    std::vector<double> positions() const {
+        std::vector<double> x(static_cast<size_t>(n_) * D);
+        check(luna_fv_positions(fv_, x.data()), "positions");
+        return x;
+    }
+    std::vector<double> point_velocities() const {
+        std::vector<double> w(static_cast<size_t>(n_) * D);
+        check(luna_fv_point_velocities(fv_, w.data()), "point_velocities");
+        return w;
+    }
+    // density_gradients() (:165-180): the first D entries of each cell's gradient row.
+    std::vector<double> density_gradients() const {
+        std::vector<double> g(static_cast<size_t>(n_) * V * D), out(static_cast<size_t>(n_) * D);
+        check(luna_fv_gradients(fv_, g.data()), "density_gradients");
+        for (int i = 0; i < n_; ++i)
+            for (int a = 0; a < D; ++a) out[static_cast<size_t>(i) * D + a] = g[static_cast<size_t>(i) * V * D + a];
+        return out;
+    }
+    // mesh_moved() (:152-156): the caller changed the mesh itself.
+    void mesh_moved(const luna_geometry& mesh) { check(luna_fv_set_geometry(fv_, &mesh), "mesh_moved"); }
+    luna_fv* handle() { return fv_; }
+
+private:
+    luna_fv* fv_ = nullptr;
+    luna_fv_options options_{};
+    int n_ = 0;
+};
+
+// FiniteVolume<1> (finite_volume_1d.hpp:41-53) on a static sorted line.
+class Line {
+public:
+    Line(const luna_line& line, const Gas& gas, int order, Limiter limiter = Limiter::face_centroid, Riemann riemann = Riemann::hllc)
+        : n_(line.cells) {
+        luna_fv1_options o{};
+        o.order = order;
+        o.limiter = static_cast<int>(limiter);
+        o.riemann = static_cast<int>(riemann);
+        o.gamma = gas.gamma;
+        o.smallr = gas.smallr;
+        o.smallc = gas.smallc;
+        check(luna_fv1_create(&o, &line, &fv_), "hydro::FiniteVolume<1>");
+    }
+    Line(const Line&) = delete;
+    Line& operator=(const Line&) = delete;
+    ~Line() { luna_fv1_destroy(fv_); }
+    int cells() const { return n_; }
+    std::vector<double> state() const {
+        std::vector<double> q(static_cast<size_t>(n_) * 3);
+        check(luna_fv1_get_state(fv_, q.data()), "state");
+        return q;
+    }
+    void set_state(const std::vector<double>& q) { check(luna_fv1_set_state(fv_, q.data()), "set_state"); }
+    double time_step(double courant, StepRule rule = StepRule::arepo) {
+        double dt = 0;
+        check(luna_fv1_time_step(fv_, courant, static_cast<int>(rule), &dt), "time_step");
+        return dt;
+    }
+    void step(double dt) { check(luna_fv1_step(fv_, dt), "step"); }
+    double step_and_time_step(double dt, double courant) {
+        double next = 0;
+        check(luna_fv1_step_and_time_step(fv_, dt, courant, &next), "step_and_time_step");
+        return next;
+    }
+
+private:
+    luna_fv1* fv_ = nullptr;
+    int n_ = 0;
+};
+
+}  // namespace luna
