@@ -136,6 +136,27 @@ def cpu_baseline(m, q, steps=2):
                       "restatement of luna-cpp's algorithm, OMP threads = cores" % steps}
 
 
+def calibrate_copy(mib):
+    """A plain device-to-device copy of `mib` MiB (the runtime's blit kernel), three times: under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE per dispatch against the known bytes give the counters' scale."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibration_copy_bytes": n, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,7 +165,12 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--static-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device (scale of the rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the step's sharding is described in DESIGN.md (no multi-GPU path)",
               file=sys.stderr)

@@ -2,8 +2,11 @@
 // HIP-event kernel timing and error reporting. One HIP stream per handle (default priority).
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
+#include <algorithm>
 #include <cstdio>
+#include <thread>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -134,11 +137,11 @@ double bits_to_double(unsigned long long b) {
 
 struct luna_fv {
     luna_fv_options opt{};
-    int D = 3, V = 5, N = 0, F = 0, E = 0;
+    int D = 3, V = 5, N = 0, F = 0, E = 0, slots = 0;
     hipStream_t stream = nullptr;
-    Buf<int> owner, neighbour, start, list, steps;
-    Buf<double> area, fc, magnitude, centroid, volume, radius, positions;
-    Buf<double> q, prims, grad, w, x, old_centroid, flux;
+    Buf<int> count, slice, list, other, steps;
+    Buf<double> cell, facerec;
+    Buf<double> q, grad, w, x, old_centroid, flux;
     Buf<unsigned long long> counters;  // [0, 1] dt slots, [2, 3] fallbacks of the step's two flux passes
     unsigned long long* host = nullptr;  // pinned: read-backs
     double side[3] = {0, 0, 0};
@@ -146,29 +149,33 @@ struct luna_fv {
     int parity = 0;
     bool prims_ready = false, swept = false, begun = false, new_mesh = false;
     Profiler prof;
-    std::vector<double> host_radius;
+    // pinned staging of the packed mesh records (reused across uploads)
+    double* stage_cell = nullptr;
+    double* stage_face = nullptr;
+    int* stage_list = nullptr;
+    int* stage_other = nullptr;
+    int* stage_count = nullptr;
+    size_t stage_cell_cap = 0, stage_face_cap = 0, stage_list_cap = 0, stage_other_cap = 0, stage_count_cap = 0;
 
     luna::Geo geo() const {
         luna::Geo g{};
         g.N = N;
         g.F = F;
-        g.owner = owner.p;
-        g.neighbour = neighbour.p;
-        g.start = start.p;
+        g.count = count.p;
+        g.slice = slice.p;
         g.list = list.p;
-        g.area = area.p;
-        g.fc = fc.p;
-        g.magnitude = magnitude.p;
-        g.magnitude_out = magnitude.p;
-        g.centroid = centroid.p;
-        g.volume = volume.p;
-        g.radius = radius.p;
-        g.positions = positions.p;
-        for (int a = 0; a < 3; ++a) g.side[a] = side[a];
+        g.other = other.p;
+        g.cell = cell.p;
+        g.face = facerec.p;
+        for (int a = 0; a < 3; ++a) {
+            g.side[a] = side[a];
+            g.inverse[a] = side[a] > 0 ? 1.0 / side[a] : 0.0;
+        }
         return g;
     }
+    int grad_row() const { return D == 3 ? luna::grad_row<3>() : luna::grad_row<2>(); }
     luna::Rows rows() const {
-        return luna::Rows{q.p, prims.p, grad.p, w.p, x.p, old_centroid.p, steps.p, flux.p};
+        return luna::Rows{q.p, grad.p, w.p, x.p, old_centroid.p, steps.p, flux.p};
     }
     luna::Gas gas() const { return luna::Gas{opt.gamma, opt.smallr, opt.smallc, 1.0 / (opt.gamma - 1.0)}; }
     luna::Counters counters_now() const {
@@ -191,10 +198,42 @@ int check_geometry(const luna_geometry* m, int D, int N) {
     return LUNA_OK;
 }
 
-// Uploads a mesh: the face and cell arrays, the cell radii, and (magnitudes) |A| per face; on a moving step the second
-// flux pass writes |A| itself, as the reference's keeps them (finite_volume.hpp:1572-1585). Host checks: the index ranges
-// the kernels rely on.
-int upload(luna_fv* fv, const luna_geometry* m, bool magnitudes) {
+// Runs body(lo, hi) over [0, n) on the host's cores.
+template <class Body>
+void parallel_for(long long n, Body&& body) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const long long threads = std::max(1LL, std::min<long long>(hw ? hw : 1, n / 65536 + 1));
+    if (threads == 1) {
+        body(0LL, n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (long long t = 0; t < threads; ++t) pool.emplace_back([&, t] { body(n * t / threads, n * (t + 1) / threads); });
+    for (auto& th : pool) th.join();
+}
+
+template <class T>
+hipError_t grow_pinned(T*& p, size_t& cap, size_t n) {
+    if (n <= cap) return hipSuccess;
+    if (p) {
+        hipError_t e = hipHostFree(p);
+        if (e != hipSuccess) return e;
+        p = nullptr;
+        cap = 0;
+    }
+    const size_t want = n + n / 8 + 1;
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(T) * want, hipHostMallocDefault);
+    if (e == hipSuccess) cap = want;
+    return e;
+}
+
+// Uploads a mesh as the kernels read it (kernels.hpp): the geometry half of the cell rows [x, c, V, r], face records
+// [A, fc, |A|, (o, j)], and the lists in 64-cell slices with, per entry, the cell on the other side. The packing runs on
+// the host, beside the mesh producer, as voronoi-cpp's Layout lets the mesh write the rows the hydro reads fastest
+// (finite_volume.hpp:119-128). The radius (finite_volume.hpp:241-246) is taken with the C library's cbrt (sqrt in 2D),
+// the function the reference's host build calls (the device's cbrt rounds some volumes one ulp apart); |A| is the
+// correctly rounded sqrt of A.A as on the device. Host checks: the index ranges the kernels rely on.
+int upload(luna_fv* fv, const luna_geometry* m) {
     const int D = fv->D, N = fv->N;
     if (int rc = check_geometry(m, D, N)) return rc;
     if (fv->opt.motion == LUNA_MOTION_MOVING) {
@@ -213,40 +252,97 @@ int upload(luna_fv* fv, const luna_geometry* m, bool magnitudes) {
         const int e = m->list[k], f = e >= 0 ? e : ~e;
         if (f >= F) return fail(LUNA_ERR_ARGUMENT, "luna: list entry out of range");
     }
-    LUNA_HIP(fv->owner.reserve(F));
-    LUNA_HIP(fv->neighbour.reserve(F));
-    LUNA_HIP(fv->list.reserve(E));
-    LUNA_HIP(fv->area.reserve(static_cast<size_t>(F) * D));
-    LUNA_HIP(fv->fc.reserve(static_cast<size_t>(F) * D));
-    LUNA_HIP(fv->magnitude.reserve(F));
+    // slices of 64 cells, each as long as its longest list
+    const int S = (N + luna::SLICE - 1) / luna::SLICE;
+    std::vector<int> slice(S);
+    long long slots = 0;
+    for (int g = 0; g < S; ++g) {
+        int longest = 0;
+        for (int i = g * luna::SLICE; i < std::min(N, (g + 1) * luna::SLICE); ++i) longest = std::max(longest, m->start[i + 1] - m->start[i]);
+        if (slots + static_cast<long long>(longest) * luna::SLICE > INT_MAX) return fail(LUNA_ERR_ARGUMENT, "luna: mesh lists exceed 2^31 slots");
+        slice[g] = static_cast<int>(slots);
+        slots += static_cast<long long>(longest) * luna::SLICE;
+    }
+    LUNA_HIP(fv->slice.reserve(S));
+    LUNA_HIP(fv->list.reserve(slots > 0 ? slots : 1));
+    LUNA_HIP(fv->other.reserve(slots > 0 ? slots : 1));
+    LUNA_HIP(fv->facerec.reserve(static_cast<size_t>(F) * luna::FACE_ROW));
     LUNA_HIP(fv->flux.reserve(static_cast<size_t>(F) * fv->V));
+    LUNA_HIP(grow_pinned(fv->stage_cell, fv->stage_cell_cap, static_cast<size_t>(N) * luna::GEO_ROW));
+    LUNA_HIP(grow_pinned(fv->stage_face, fv->stage_face_cap, static_cast<size_t>(F) * luna::FACE_ROW));
+    LUNA_HIP(grow_pinned(fv->stage_list, fv->stage_list_cap, static_cast<size_t>(slots)));
+    LUNA_HIP(grow_pinned(fv->stage_other, fv->stage_other_cap, static_cast<size_t>(slots)));
+    LUNA_HIP(grow_pinned(fv->stage_count, fv->stage_count_cap, static_cast<size_t>(N)));
     hipStream_t s = fv->stream;
-    LUNA_HIP(hipMemcpyAsync(fv->owner.p, m->owner, sizeof(int) * F, hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpyAsync(fv->neighbour.p, m->neighbour, sizeof(int) * F, hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpyAsync(fv->start.p, m->start, sizeof(int) * (N + 1), hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpyAsync(fv->list.p, m->list, sizeof(int) * E, hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpyAsync(fv->area.p, m->area, sizeof(double) * F * D, hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpyAsync(fv->fc.p, m->face_centroid, sizeof(double) * F * D, hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpyAsync(fv->centroid.p, m->centroid, sizeof(double) * N * D, hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpyAsync(fv->volume.p, m->volume, sizeof(double) * N, hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpyAsync(fv->positions.p, m->positions, sizeof(double) * N * D, hipMemcpyHostToDevice, s));
-    // finite_volume.hpp:241-246 radius(): on the host, with the C library's cbrt (sqrt in 2D) that the reference's host
-    // build calls, so the time step has its bits (the device's cbrt rounds some volumes differently).
-    fv->host_radius.resize(N);
-    for (int i = 0; i < N; ++i)
-        fv->host_radius[i] = D == 3 ? std::cbrt(m->volume[i] * (3.0 / (4.0 * M_PI))) : std::sqrt(m->volume[i] * (1.0 / M_PI));
-    LUNA_HIP(hipMemcpyAsync(fv->radius.p, fv->host_radius.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    double* cell = fv->stage_cell;
+    double* face = fv->stage_face;
+    int* list = fv->stage_list;
+    int* other = fv->stage_other;
+    int* count = fv->stage_count;
+    parallel_for(N, [&](long long lo, long long hi) {
+        for (long long i = lo; i < hi; ++i) {
+            double* row = cell + i * luna::GEO_ROW;
+            for (int k = 0; k < luna::GEO_ROW; ++k) row[k] = 0;
+            for (int a = 0; a < D; ++a) {
+                row[a] = m->positions[i * D + a];
+                row[D + a] = m->centroid[i * D + a];
+            }
+            row[2 * D] = m->volume[i];
+            row[2 * D + 1] = D == 3 ? std::cbrt(m->volume[i] * (3.0 / (4.0 * M_PI))) : std::sqrt(m->volume[i] * (1.0 / M_PI));
+            const int first = m->start[i], n = m->start[i + 1] - first;
+            count[i] = n;
+            const long long g = i / luna::SLICE, lane = i % luna::SLICE;
+            const long long longest = (g + 1 < S ? slice[g + 1] : slots) - slice[g];
+            for (long long t = 0; t < longest / luna::SLICE; ++t) {
+                const long long slot = slice[g] + t * luna::SLICE + lane;
+                if (t < n) {
+                    const int e = m->list[first + t], f = e >= 0 ? e : ~e;
+                    list[slot] = e;
+                    other[slot] = e >= 0 ? m->neighbour[f] : m->owner[f];
+                } else {  // padding: a valid index that no kernel reads
+                    list[slot] = 0;
+                    other[slot] = 0;
+                }
+            }
+        }
+    });
+    // padding lanes of the last slice (cells past N)
+    for (long long i = N; i < static_cast<long long>(S) * luna::SLICE; ++i) {
+        const long long g = i / luna::SLICE, lane = i % luna::SLICE;
+        for (long long slot = slice[g] + lane; slot < slots; slot += luna::SLICE) {
+            list[slot] = 0;
+            other[slot] = 0;
+        }
+    }
+    parallel_for(F, [&](long long lo, long long hi) {
+        for (long long f = lo; f < hi; ++f) {
+            double* row = face + f * luna::FACE_ROW;
+            for (int k = 0; k < luna::FACE_ROW; ++k) row[k] = 0;
+            const double* a = m->area + f * D;
+            for (int k = 0; k < D; ++k) {
+                row[k] = a[k];
+                row[D + k] = m->face_centroid[f * D + k];
+            }
+            row[2 * D] = std::sqrt(D == 3 ? a[0] * a[0] + a[1] * a[1] + a[2] * a[2] : a[0] * a[0] + a[1] * a[1]);
+            const int pair[2] = {m->owner[f], m->neighbour[f]};
+            std::memcpy(row + 7, pair, sizeof pair);
+        }
+    });
+    LUNA_HIP(hipMemcpyAsync(fv->count.p, count, sizeof(int) * N, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->slice.p, slice.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->list.p, list, sizeof(int) * slots, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->other.p, other, sizeof(int) * slots, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpy2DAsync(fv->cell.p, sizeof(double) * luna::CELL_ROW, cell, sizeof(double) * luna::GEO_ROW, sizeof(double) * luna::GEO_ROW,
+                              N, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->facerec.p, face, sizeof(double) * static_cast<size_t>(F) * luna::FACE_ROW, hipMemcpyHostToDevice, s));
     fv->F = F;
     fv->E = E;
+    fv->slots = static_cast<int>(slots);
     for (int a = 0; a < 3; ++a) fv->side[a] = a < D ? m->side[a] : 0.0;
     fv->grid.step = m->step;
     fv->grid.per_step = m->step > 0 ? 1.0 / m->step : 0.0;
     for (int a = 0; a < 3; ++a) fv->grid.period[a] = a < D ? m->period[a] : 1;
-    const luna::Geo g = fv->geo();
-    if (magnitudes)
-        LUNA_HIP(timed(fv->prof, s, "magnitudes", [&] { return D == 3 ? luna::launch_magnitudes<3>(s, g, fv->magnitude.p)
-                                                                      : luna::launch_magnitudes<2>(s, g, fv->magnitude.p); }));
-    LUNA_HIP(hipStreamSynchronize(s));  // the host arrays may go away after this call
+    LUNA_HIP(hipStreamSynchronize(s));  // the caller's arrays (and the staging) may be reused after this call
     return LUNA_OK;
 }
 
@@ -335,11 +431,9 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
     if (hipStreamCreate(&fv->stream) != hipSuccess) return cleanup(fail(LUNA_ERR_DEVICE, "luna: hipStreamCreate failed"));
     const size_t N = fv->N, D = fv->D, V = fv->V;
     hipError_t e = hipSuccess;
-    if ((e = fv->q.reserve(N * V)) != hipSuccess || (e = fv->prims.reserve(N * (V + 2))) != hipSuccess ||
-        (e = fv->grad.reserve(N * V * D)) != hipSuccess || (e = fv->w.reserve(N * D)) != hipSuccess ||
-        (e = fv->centroid.reserve(N * D)) != hipSuccess || (e = fv->volume.reserve(N)) != hipSuccess ||
-        (e = fv->radius.reserve(N)) != hipSuccess ||
-        (e = fv->positions.reserve(N * D)) != hipSuccess || (e = fv->start.reserve(N + 1)) != hipSuccess ||
+    if ((e = fv->q.reserve(N * V)) != hipSuccess || (e = fv->count.reserve(N)) != hipSuccess ||
+        (e = fv->grad.reserve(N * fv->grad_row())) != hipSuccess || (e = fv->w.reserve(N * D)) != hipSuccess ||
+        (e = fv->cell.reserve(N * luna::CELL_ROW)) != hipSuccess ||
         (e = fv->counters.reserve(4)) != hipSuccess)
         return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
     if (opt->motion == LUNA_MOTION_MOVING) {
@@ -355,7 +449,7 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
     std::vector<double> zero(N * V, 0.0);
     if (hipMemcpy(fv->q.p, zero.data(), sizeof(double) * N * V, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(LUNA_ERR_DEVICE, "luna: state initialisation failed"));
-    if (int rc = upload(fv, mesh, true)) return cleanup(rc);
+    if (int rc = upload(fv, mesh)) return cleanup(rc);
     if (opt->motion == LUNA_MOTION_MOVING) {  // copy_positions (finite_volume.hpp:342-349)
         if (hipMemcpy(fv->x.p, mesh->positions, sizeof(double) * N * D, hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(LUNA_ERR_DEVICE, "luna: position upload failed"));
@@ -371,10 +465,11 @@ int luna_fv_destroy(luna_fv* fv) {
     if (!fv) return LUNA_OK;
     if (fv->stream) (void)hipStreamSynchronize(fv->stream);
     fv->prof.destroy();
-    for (auto* b : {&fv->area, &fv->fc, &fv->magnitude, &fv->centroid, &fv->volume, &fv->radius, &fv->positions, &fv->q, &fv->prims, &fv->grad,
-                    &fv->w, &fv->x, &fv->old_centroid, &fv->flux})
-        b->release();
-    for (auto* b : {&fv->owner, &fv->neighbour, &fv->start, &fv->list, &fv->steps}) b->release();
+    for (auto* b : {&fv->cell, &fv->facerec, &fv->q, &fv->grad, &fv->w, &fv->x, &fv->old_centroid, &fv->flux}) b->release();
+    for (auto* b : {&fv->count, &fv->slice, &fv->list, &fv->other, &fv->steps}) b->release();
+    for (void* p : {static_cast<void*>(fv->stage_cell), static_cast<void*>(fv->stage_face), static_cast<void*>(fv->stage_list),
+                    static_cast<void*>(fv->stage_other), static_cast<void*>(fv->stage_count)})
+        if (p) (void)hipHostFree(p);
     fv->counters.release();
     if (fv->host) (void)hipHostFree(fv->host);
     if (fv->stream) (void)hipStreamDestroy(fv->stream);
@@ -461,7 +556,7 @@ int luna_fv_moving_begin(luna_fv* fv, double dt, double* positions) {
 
 int luna_fv_set_geometry(luna_fv* fv, const luna_geometry* mesh) {
     if (!fv || !mesh) return fail(LUNA_ERR_ARGUMENT, "luna_fv_set_geometry: null argument");
-    if (int rc = upload(fv, mesh, !fv->begun)) return rc;
+    if (int rc = upload(fv, mesh)) return rc;
     if (fv->begun) {
         fv->new_mesh = true;
     } else {  // mesh_moved() (finite_volume.hpp:152-156): the cached rows go
@@ -475,7 +570,7 @@ int luna_fv_moving_finish(luna_fv* fv, const luna_geometry* mesh, double dt) {
     if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_finish: null argument");
     if (!fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_finish: no step was begun");
     if (mesh) {
-        if (int rc = upload(fv, mesh, false)) return rc;
+        if (int rc = upload(fv, mesh)) return rc;
     } else if (!fv->new_mesh) {
         return fail(LUNA_ERR_LOGIC, "luna_fv_moving_finish: no mesh of the new positions (pass one or call luna_fv_set_geometry)");
     }
@@ -503,8 +598,11 @@ int luna_fv_fallbacks(luna_fv* fv, long long* count) {
 
 int luna_fv_positions(luna_fv* fv, double* x) {
     if (!fv || !x) return fail(LUNA_ERR_ARGUMENT, "luna_fv_positions: null argument");
-    const double* src = fv->opt.motion == LUNA_MOTION_MOVING ? fv->x.p : fv->positions.p;
-    LUNA_HIP(hipMemcpyAsync(x, src, sizeof(double) * fv->N * fv->D, hipMemcpyDeviceToHost, fv->stream));
+    if (fv->opt.motion == LUNA_MOTION_MOVING)
+        LUNA_HIP(hipMemcpyAsync(x, fv->x.p, sizeof(double) * fv->N * fv->D, hipMemcpyDeviceToHost, fv->stream));
+    else  // the mesh's positions, from the cell rows
+        LUNA_HIP(hipMemcpy2DAsync(x, sizeof(double) * fv->D, fv->cell.p, sizeof(double) * luna::CELL_ROW, sizeof(double) * fv->D, fv->N,
+                                  hipMemcpyDeviceToHost, fv->stream));
     LUNA_HIP(hipStreamSynchronize(fv->stream));
     return LUNA_OK;
 }
@@ -520,14 +618,20 @@ int luna_fv_point_velocities(luna_fv* fv, double* w) {
 int luna_fv_gradients(luna_fv* fv, double* g) {
     if (!fv || !g) return fail(LUNA_ERR_ARGUMENT, "luna_fv_gradients: null argument");
     if (fv->opt.order != 1) return fail(LUNA_ERR_LOGIC, "luna_fv_gradients: order 0 keeps no gradients");
-    LUNA_HIP(hipMemcpyAsync(g, fv->grad.p, sizeof(double) * fv->N * fv->V * fv->D, hipMemcpyDeviceToHost, fv->stream));
+    const size_t width = sizeof(double) * fv->V * fv->D;
+    LUNA_HIP(hipMemcpy2DAsync(g, width, fv->grad.p, sizeof(double) * fv->grad_row(), width, fv->N, hipMemcpyDeviceToHost, fv->stream));
     LUNA_HIP(hipStreamSynchronize(fv->stream));
     return LUNA_OK;
 }
 
 int luna_fv_primitives(luna_fv* fv, double* rows) {
     if (!fv || !rows) return fail(LUNA_ERR_ARGUMENT, "luna_fv_primitives: null argument");
-    LUNA_HIP(hipMemcpyAsync(rows, fv->prims.p, sizeof(double) * fv->N * (fv->V + 2), hipMemcpyDeviceToHost, fv->stream));
+    const size_t pitch = sizeof(double) * (fv->V + 2), row = sizeof(double) * luna::CELL_ROW;
+    // W and c_s from the primitive half of the cell rows, r from the geometry half
+    LUNA_HIP(hipMemcpy2DAsync(rows, pitch, fv->cell.p + luna::GEO_ROW, row, sizeof(double) * (fv->V + 1), fv->N, hipMemcpyDeviceToHost,
+                              fv->stream));
+    LUNA_HIP(hipMemcpy2DAsync(rows + fv->V + 1, pitch, fv->cell.p + 2 * fv->D + 1, row, sizeof(double), fv->N, hipMemcpyDeviceToHost,
+                              fv->stream));
     LUNA_HIP(hipStreamSynchronize(fv->stream));
     return LUNA_OK;
 }

@@ -1,5 +1,5 @@
-// Device kernels of the hydro step on a Voronoi mesh (D = 2, 3) and on a sorted line (D = 1), and their argument records.
-// Launched by api.hip on the handle's stream. FP64 throughout, compiled with -ffp-contract=off (physics.hpp).
+// Device kernels of the hydro step on a Voronoi mesh (D = 2, 3) and on a sorted line (D = 1), their argument records and
+// the row layouts in HBM. Launched by api.hip on the handle's stream. FP64 throughout, -ffp-contract=off (physics.hpp).
 #pragma once
 
 #include "physics.hpp"
@@ -8,34 +8,53 @@ namespace luna {
 
 constexpr unsigned long long INF_BITS = 0x7FF0000000000000ull;  // +inf as the bits a min-reduction starts from
 
-// The mesh arrays as the kernels read them (device pointers), finite_volume.hpp:257-327 Geometry.
+// Row layouts (doubles per row), each row aligned to its size:
+//   cell row     [x (D), c (D), V, r | W (D + 2), c_s], 16 (128 B, one L2 line). c: centroid relative to the generator;
+//                r: cbrt(3V/4pi) / sqrt(V/pi). The geometry half (first 8) comes with the mesh; the primitive half is
+//                written by the primitives and by the update. A neighbour's position, centroid and state cost one line.
+//   face record  [A (D), fc (D), |A|, (owner, neighbour) as two int32 in the last slot], 8 (64 B)
+//   gradient row [g (D + 2) x D], padded to 16 (3D) or 8 (2D)
+//   flux row     [F (D + 2)] per face, in face order
+constexpr int CELL_ROW = 16, GEO_ROW = 8, FACE_ROW = 8;
+template <int D>
+constexpr int grad_row() {
+    return D == 3 ? 16 : 8;
+}
+template <int D>
+struct CellAt {
+    static constexpr int X = 0, C = D, VOL = 2 * D, RAD = 2 * D + 1, W = GEO_ROW, CS = GEO_ROW + D + 2;
+};
+template <int D>
+struct FaceAt {
+    static constexpr int A = 0, FC = D, MAG = 2 * D, PAIR = 7;
+};
+
+// Face lists in slices of 64 cells (one wavefront): entry t of cell i sits in slot slice[i / 64] + 64 t + i % 64, so the
+// t-th entries of a wavefront's cells are one contiguous run; a slice is as long as its longest list (the rest is padding
+// that no kernel reads). Within a cell the entries keep the mesh's list order.
+constexpr int SLICE = 64;
+
+// The mesh as the kernels read it (device pointers): finite_volume.hpp:257-327 Geometry, packed.
 struct Geo {
     int N, F;
-    const int* owner;
-    const int* neighbour;
-    const int* start;
-    const int* list;
-    const double* area;          // F x D
-    const double* fc;            // F x D, relative to the owner's generator
-    const double* magnitude;     // F: |A|, computed once per mesh
-    double* magnitude_out;       // the same array, written by a moving step's second flux pass
-    const double* centroid;      // N x D, relative to the generator
-    const double* volume;        // N
-    const double* radius;        // N: the radius of a sphere (circle) of the cell's volume, from the host's cbrt (sqrt)
-    const double* positions;     // N x D
-    double side[3];
+    const int* count;   // N: list length of each cell
+    const int* slice;   // ceil(N / 64): first slot of each slice
+    const int* list;    // slots: f if the cell owns face f, ~f otherwise
+    const int* other;   // slots: the cell on the other side of the entry's face
+    double* cell;       // N x CELL_ROW (the kernels write only the primitive half)
+    const double* face; // F x FACE_ROW
+    double side[3], inverse[3];  // the periodic box and 1 / side
 };
 
 // Per-cell rows of the step (device pointers).
 struct Rows {
     double* q;             // N x V, extensive
-    double* prims;         // N x (V + 2): W, sound speed, radius
-    double* grad;          // N x (V D)
+    double* grad;          // N x grad_row<D>()
     double* w;             // N x D point velocities (moving)
     double* x;             // N x D solver positions (moving)
     double* old_centroid;  // N x D centroids of t^n (moving)
     int* steps;            // N x D whole grid steps over the step (moving)
-    double* flux;          // F x V
+    double* flux;          // F x V, face order
 };
 
 struct Grid {
@@ -52,8 +71,6 @@ struct Counters {
 // Kernel launchers (api.hip calls these; each returns the HIP launch status).
 template <int D>
 hipError_t launch_primitives(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas);
-template <int D>
-hipError_t launch_magnitudes(hipStream_t s, const Geo& g, double* magnitude);
 template <int D>
 hipError_t launch_sweep(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double courant, bool moving, bool grad, bool limit,
                         int rule, const Counters& c);

@@ -1,12 +1,44 @@
 // The hydro step on a Voronoi mesh (D = 2, 3), hand-written for CDNA4 (gfx950): one work-item per cell for the sweep
 // and the update, one per face for the fluxes. FP64, -ffp-contract=off; each cell sums over its face list in list order,
 // so every run gives the same bits, and those of the CPU restatement of the reference's arithmetic.
+//
+// Memory (kernels.hpp): a neighbour is one 128 B cell row (position, centroid, state) and a face one 64 B record; the
+// lists are stored in 64-cell slices, so a wavefront reads the t-th entries of its cells as one contiguous run and the
+// entries of the next group are in flight while the current group is summed; work-groups are mapped so that each XCD
+// takes a contiguous run of cells (neighbours in Morton order share its L2).
 #include "kernels.hpp"
+
+// Tuning knobs (compile-time; the defaults are the measured best): entries of the sweep's first walk and of its limiter
+// walk loaded together, flux rows of the update loaded together, and the occupancy floor (waves per SIMD) of the sweep
+// and flux kernels.
+#ifndef LUNA_SWEEP_UNROLL
+#define LUNA_SWEEP_UNROLL 2
+#endif
+#ifndef LUNA_LIMIT_UNROLL
+#define LUNA_LIMIT_UNROLL 2
+#endif
+#ifndef LUNA_UPDATE_UNROLL
+#define LUNA_UPDATE_UNROLL 4
+#endif
+#ifndef LUNA_SWEEP_WAVES
+#define LUNA_SWEEP_WAVES 1
+#endif
+#ifndef LUNA_FLUX_WAVES
+#define LUNA_FLUX_WAVES 1
+#endif
 
 namespace luna {
 namespace {
 
 constexpr int BLOCK = 256;
+
+// The logical block of this work-group. Work-groups go to the 8 XCDs in turn (block b to XCD b mod 8), so XCD x gets the
+// contiguous logical run [x q + min(x, r), ...) of q + (x < r) blocks (q = n / 8, r = n mod 8): a bijection.
+__device__ __forceinline__ int xcd_block() {
+    const int b = blockIdx.x, n = gridDim.x;
+    const int q = n >> 3, rem = n & 7, x = b & 7, k = b >> 3;
+    return x * q + min(x, rem) + k;
+}
 
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
@@ -48,80 +80,64 @@ __device__ __forceinline__ void block_count(int v, unsigned long long* out) {
 
 // finite_volume.hpp:279-282 Geometry::separation(i, j): generator j seen from generator i, nearest image.
 template <int D>
-__device__ __forceinline__ void separation(const Geo& g, int i, int j, double* d) {
+__device__ __forceinline__ void separation(const Geo& g, const double* xi, const double* xj, double* d) {
 #pragma unroll
-    for (int a = 0; a < D; ++a) d[a] = wrap(g.positions[(size_t)j * D + a] - g.positions[(size_t)i * D + a], g.side[a]);
+    for (int a = 0; a < D; ++a) d[a] = wrap(xj[a] - xi[a], g.side[a], g.inverse[a]);
 }
 
-// finite_volume.hpp:322-326 Geometry::face_offset(f, i); owned: whether i is the owner of f (the list's owner bit).
 template <int D>
-__device__ __forceinline__ void face_offset(const Geo& g, int f, int i, bool owned, double* r) {
-    double rel[D];
-    if (owned) {
+__device__ __forceinline__ void load_vec(const double* p, double* v) {
 #pragma unroll
-        for (int a = 0; a < D; ++a) rel[a] = g.fc[(size_t)f * D + a];
-    } else {
-        double s[D];
-        separation<D>(g, i, g.owner[f], s);
-#pragma unroll
-        for (int a = 0; a < D; ++a) rel[a] = g.fc[(size_t)f * D + a] + s[a];
-    }
-#pragma unroll
-    for (int a = 0; a < D; ++a) r[a] = rel[a] - g.centroid[(size_t)i * D + a];
+    for (int a = 0; a < D; ++a) v[a] = p[a];
 }
 
-// finite_volume.hpp:403-415 primitives()
+// The slot of cell i's first list entry; entry t is 64 t further.
+__device__ __forceinline__ int first_slot(const Geo& g, int i) { return g.slice[i >> 6] + (i & (SLICE - 1)); }
+
+// finite_volume.hpp:403-415 primitives(): the primitive half of the cell row.
 template <int D>
 __global__ __launch_bounds__(BLOCK) void k_primitives(Geo g, Rows r, Gas gas) {
-    constexpr int V = D + 2, PW = V + 2;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    constexpr int V = D + 2;
+    using C = CellAt<D>;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
     if (i >= g.N) return;
-    const double inv = 1.0 / g.volume[i];
+    double* row = g.cell + (size_t)i * CELL_ROW;
+    const double inv = 1.0 / row[C::VOL];
     double U[V], W[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) U[v] = r.q[(size_t)i * V + v] * inv;
     primitive<D>(gas, U, W);
-    double* row = r.prims + (size_t)i * PW;
 #pragma unroll
-    for (int v = 0; v < V; ++v) row[v] = W[v];
-    row[V] = sound_speed<D>(gas, W);
-    row[V + 1] = g.radius[i];
+    for (int v = 0; v < V; ++v) row[C::W + v] = W[v];
+    row[C::CS] = sound_speed<D>(gas, W);
 }
 
-// |A| of every face (the magnitudes the reference caches, finite_volume.hpp:1582-1585).
+// One list entry as the sweep's first walk reads it: the face's |A| and the other cell's row.
 template <int D>
-__global__ __launch_bounds__(BLOCK) void k_magnitudes(Geo g, double* magnitude) {
-    const int f = blockIdx.x * BLOCK + threadIdx.x;
-    if (f >= g.F) return;
-    const double* a = g.area + (size_t)f * D;
-    magnitude[f] = sqrt(dot<D>(a, a));
+struct Near {
+    double mag, x[D], c[D], W[D + 2], cs;
+};
+template <int D>
+__device__ __forceinline__ void load_near(const Geo& g, int e, int j, Near<D>& n) {
+    using C = CellAt<D>;
+    const int f = e >= 0 ? e : ~e;
+    n.mag = g.face[(size_t)f * FACE_ROW + FaceAt<D>::MAG];
+    const double* row = g.cell + (size_t)j * CELL_ROW;
+    load_vec<D>(row + C::X, n.x);
+    load_vec<D>(row + C::C, n.c);
+#pragma unroll
+    for (int v = 0; v < D + 2; ++v) n.W[v] = row[C::W + v];
+    n.cs = row[C::CS];
 }
 
-// The sweep (finite_volume.hpp:463-569 Sweep::operator(), double policy) with the time step's terms (:1040-1104) fused:
-// each cell takes its own term and, for every face of its list, the face term with its own radius. The pair term vsig and
-// |d| have the same bits from either side, so the minimum is the reference's (the face kernel's terms are the same set).
-// The first block also resets the other dt slot and the step's fallback counters.
-template <int D, bool Moving, bool Grad, bool Limit, int Rule>
-__global__ __launch_bounds__(BLOCK) void k_sweep(Geo g, Rows r, Gas gas, double courant, Counters c) {
-    constexpr int V = D + 2, U = D + 1, PW = V + 2;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *c.dt_next = INF_BITS;
-        c.fallbacks[0] = 0;
-        c.fallbacks[1] = 0;
-    }
-    double smallest = INFINITY;
-    if (i < g.N) {
-        const double* Wrow = r.prims + (size_t)i * PW;
-        double Wi[V];
-#pragma unroll
-        for (int v = 0; v < V; ++v) Wi[v] = Wrow[v];
-        const double ci = Wrow[V], ri = Wrow[V + 1];
-        double centroid[D];
-#pragma unroll
-        for (int a = 0; a < D; ++a) centroid[a] = g.centroid[(size_t)i * D + a];
-        double surface = 0, smallest_face = INFINITY, max_angle = 0;
-        double lo[V], hi[V], M[U][U], b[V][U];
+// The sweep's per-cell sums (finite_volume.hpp:486-515), in list order.
+template <int D, bool Moving, bool Grad, int Rule>
+struct Sums {
+    static constexpr int V = D + 2, U = D + 1;
+    double surface = 0, smallest_face = INFINITY, max_angle = 0, smallest = INFINITY;
+    double lo[V], hi[V], M[U][U], b[V][U];
+
+    __device__ __forceinline__ void init() {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             lo[v] = 1.7976931348623157e308;
@@ -133,66 +149,125 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(Geo g, Rows r, Gas gas, double 
         for (int p = 0; p < U; ++p)
 #pragma unroll
             for (int s = 0; s < U; ++s) M[p][s] = 0;
-        const int first = g.start[i], end = g.start[i + 1];
-        for (int k = first; k < end; ++k) {
-            const int e = g.list[k], f = e >= 0 ? e : ~e;
-            const int j = e >= 0 ? g.neighbour[f] : g.owner[f];
-            const double* Wrj = r.prims + (size_t)j * PW;
-            double Wj[V];
+    }
+    __device__ __forceinline__ void add(const Geo& g, const Near<D>& n, const double* xi, const double* centroid, const double* Wi,
+                                        double csi, double ri, double courant) {
+        double d[D];
+        separation<D>(g, xi, n.x, d);
+        const double magnitude = n.mag;
+        surface += magnitude;
+        smallest_face = fmin(smallest_face, magnitude);
+        const double dist = sqrt(dot<D>(d, d));
+        if constexpr (Moving) max_angle = fmax(max_angle, face_angle<D>(magnitude) / (0.5 * dist));
+        if constexpr (Grad) {
+            double X[U];
+            X[0] = 1;
 #pragma unroll
-            for (int v = 0; v < V; ++v) Wj[v] = Wrj[v];
-            double d[D];
-            separation<D>(g, i, j, d);
-            const double magnitude = g.magnitude[f];
-            surface += magnitude;
-            smallest_face = fmin(smallest_face, magnitude);
-            const double dist = sqrt(dot<D>(d, d));
-            if constexpr (Moving) max_angle = fmax(max_angle, face_angle<D>(magnitude) / (0.5 * dist));
-            if constexpr (Grad) {
-                double X[U];
-                X[0] = 1;
+            for (int p = 1; p < U; ++p) X[p] = d[p - 1] + n.c[p - 1] - centroid[p - 1];
+            const double w = magnitude;
 #pragma unroll
-                for (int p = 1; p < U; ++p) X[p] = d[p - 1] + g.centroid[(size_t)j * D + p - 1] - centroid[p - 1];
-                const double w = magnitude;
+            for (int p = 0; p < U; ++p)
 #pragma unroll
-                for (int p = 0; p < U; ++p)
+                for (int s = p; s < U; ++s) M[p][s] += w * X[p] * X[s];
 #pragma unroll
-                    for (int s = p; s < U; ++s) M[p][s] += w * X[p] * X[s];
+            for (int v = 0; v < V; ++v) {
+                const double dW = n.W[v] - Wi[v];
 #pragma unroll
-                for (int v = 0; v < V; ++v) {
-                    const double dW = Wj[v] - Wi[v];
-#pragma unroll
-                    for (int p = 0; p < U; ++p) b[v][p] += w * X[p] * dW;
-                    lo[v] = fmin(lo[v], Wj[v]);
-                    hi[v] = fmax(hi[v], Wj[v]);
-                }
-            }
-            if constexpr (Rule == 0) {  // FaceStep (:1079-1089) from this side
-                double dv[D];
-#pragma unroll
-                for (int a = 0; a < D; ++a) dv[a] = Wj[1 + a] - Wi[1 + a];
-                const double vsig = ci + Wrj[V] - dot<D>(dv, d) / dist;
-                if (vsig > 0) smallest = fmin(smallest, courant * ((dist + ri) / vsig));
+                for (int p = 0; p < U; ++p) b[v][p] += w * X[p] * dW;
+                lo[v] = fmin(lo[v], n.W[v]);
+                hi[v] = fmax(hi[v], n.W[v]);
             }
         }
-        const double tiny = 1e-6 * surface;
-        const bool any_tiny = smallest_face < tiny;
+        if constexpr (Rule == 0) {  // FaceStep (:1079-1089) from this side
+            double dv[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) dv[a] = n.W[1 + a] - Wi[1 + a];
+            const double vsig = csi + n.cs - dot<D>(dv, d) / dist;
+            if (vsig > 0) smallest = fmin(smallest, courant * ((dist + ri) / vsig));
+        }
+    }
+};
+
+// The sweep (finite_volume.hpp:463-569 Sweep::operator(), double policy) with the time step's terms (:1040-1104) fused:
+// each cell takes its own term and, for every face of its list, the face term with its own radius. The pair term vsig and
+// |d| have the same bits from either side, so the minimum is the reference's (the face kernel's terms are the same set).
+// The first block also resets the other dt slot and the step's fallback counters.
+template <int D, bool Moving, bool Grad, bool Limit, int Rule>
+__global__ __launch_bounds__(BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, Rows r, Gas gas, double courant, Counters c) {
+    constexpr int V = D + 2, U = D + 1, GR = grad_row<D>();
+    using C = CellAt<D>;
+    using FA = FaceAt<D>;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *c.dt_next = INF_BITS;
+        c.fallbacks[0] = 0;
+        c.fallbacks[1] = 0;
+    }
+    double smallest = INFINITY;
+    if (i < g.N) {
+        const double* mine = g.cell + (size_t)i * CELL_ROW;
+        double Wi[V], xi[D], centroid[D];
+#pragma unroll
+        for (int v = 0; v < V; ++v) Wi[v] = mine[C::W + v];
+        const double ci = mine[C::CS], ri = mine[C::RAD];
+        load_vec<D>(mine + C::X, xi);
+        load_vec<D>(mine + C::C, centroid);
+        Sums<D, Moving, Grad, Rule> s;
+        s.init();
+        const int count = g.count[i], base = first_slot(g, i);
+        // Groups of UN entries: their face magnitudes and rows in flight together, summed in list order; the next group's
+        // list entries are read one group ahead, so each group waits for one level of loads.
+        constexpr int UN = LUNA_SWEEP_UNROLL;
+        int ne[UN], nj[UN];
+#pragma unroll
+        for (int u = 0; u < UN; ++u)
+            if (u < count) {
+                ne[u] = g.list[base + u * SLICE];
+                nj[u] = g.other[base + u * SLICE];
+            }
+        int t = 0;
+        for (; t + UN - 1 < count; t += UN) {
+            int ce[UN], cj[UN];
+#pragma unroll
+            for (int u = 0; u < UN; ++u) {
+                ce[u] = ne[u];
+                cj[u] = nj[u];
+                if (t + UN + u < count) {
+                    ne[u] = g.list[base + (t + UN + u) * SLICE];
+                    nj[u] = g.other[base + (t + UN + u) * SLICE];
+                }
+            }
+            Near<D> n[UN];
+#pragma unroll
+            for (int u = 0; u < UN; ++u) load_near<D>(g, ce[u], cj[u], n[u]);
+#pragma unroll
+            for (int u = 0; u < UN; ++u) s.add(g, n[u], xi, centroid, Wi, ci, ri, courant);
+        }
+#pragma unroll
+        for (int u = 0; u < UN - 1; ++u)
+            if (t + u < count) {
+                Near<D> a;
+                load_near<D>(g, ne[u], nj[u], a);
+                s.add(g, a, xi, centroid, Wi, ci, ri, courant);
+            }
+        smallest = s.smallest;
+        const double tiny = 1e-6 * s.surface;
+        const bool any_tiny = s.smallest_face < tiny;
         if constexpr (Grad) {
             if (any_tiny) {  // neighbour_range (:572-588)
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
-                    lo[v] = 1.7976931348623157e308;
-                    hi[v] = -1.7976931348623157e308;
+                    s.lo[v] = 1.7976931348623157e308;
+                    s.hi[v] = -1.7976931348623157e308;
                 }
-                for (int k = first; k < end; ++k) {
-                    const int e = g.list[k], f = e >= 0 ? e : ~e;
-                    if (g.magnitude[f] < tiny) continue;
-                    const int j = e >= 0 ? g.neighbour[f] : g.owner[f];
-                    const double* Wrj = r.prims + (size_t)j * PW;
+                for (int k = 0; k < count; ++k) {
+                    const int e = g.list[base + k * SLICE], f = e >= 0 ? e : ~e;
+                    if (g.face[(size_t)f * FACE_ROW + FA::MAG] < tiny) continue;
+                    const double* row = g.cell + (size_t)g.other[base + k * SLICE] * CELL_ROW + C::W;
 #pragma unroll
                     for (int v = 0; v < V; ++v) {
-                        lo[v] = fmin(lo[v], Wrj[v]);
-                        hi[v] = fmax(hi[v], Wrj[v]);
+                        s.lo[v] = fmin(s.lo[v], row[v]);
+                        s.hi[v] = fmax(s.hi[v], row[v]);
                     }
                 }
             }
@@ -203,22 +278,22 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(Geo g, Rows r, Gas gas, double 
 #pragma unroll
             for (int p = 0; p < U; ++p)
 #pragma unroll
-                for (int s = 0; s < U; ++s) L[p][s] = 0;
+                for (int q = 0; q < U; ++q) L[p][q] = 0;
 #pragma unroll
             for (int p = 0; p < U; ++p)
 #pragma unroll
-                for (int s = 0; s <= p; ++s) {
-                    double sum = M[s][p];
+                for (int q = 0; q <= p; ++q) {
+                    double sum = s.M[q][p];
 #pragma unroll
-                    for (int m = 0; m < s; ++m) sum -= L[p][m] * L[s][m];
-                    L[p][s] = p == s ? sqrt(sum) : sum / L[s][s];
+                    for (int m = 0; m < q; ++m) sum -= L[p][m] * L[q][m];
+                    L[p][q] = p == q ? sqrt(sum) : sum / L[q][q];
                 }
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 double z[U], x[U];
 #pragma unroll
                 for (int p = 0; p < U; ++p) {
-                    double sum = b[v][p];
+                    double sum = s.b[v][p];
 #pragma unroll
                     for (int m = 0; m < p; ++m) sum -= L[p][m] * z[m];
                     z[p] = sum / L[p][p];
@@ -240,13 +315,13 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(Geo g, Rows r, Gas gas, double 
         if constexpr (Moving) {  // the regularised point velocity (:526-537; constants :74)
             const double shaping = 0.5, factor = 2.25;
             double fraction = 0;
-            if (max_angle > 0.75 * factor)
-                fraction = max_angle > factor ? shaping : shaping * (max_angle - 0.75 * factor) / (0.25 * factor);
+            if (s.max_angle > 0.75 * factor)
+                fraction = s.max_angle > factor ? shaping : shaping * (s.max_angle - 0.75 * factor) / (0.25 * factor);
             const double cd = sqrt(dot<D>(centroid, centroid));
             if (cd > 0 && fraction > 0) {
-                const double s = fraction * ci / cd;
+                const double sc = fraction * ci / cd;
 #pragma unroll
-                for (int a = 0; a < D; ++a) wi[a] = wi[a] + centroid[a] * s;
+                for (int a = 0; a < D; ++a) wi[a] = wi[a] + centroid[a] * sc;
             }
 #pragma unroll
             for (int a = 0; a < D; ++a) r.w[(size_t)i * D + a] = wi[a];
@@ -270,27 +345,52 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(Geo g, Rows r, Gas gas, double 
         }
         if constexpr (Grad) {
             if constexpr (Limit) {  // the face-centroid limiter, face by face in list order (:538-565)
-                for (int k = first; k < end; ++k) {
-                    const int e = g.list[k], f = e >= 0 ? e : ~e;
-                    if (any_tiny && g.magnitude[f] < tiny) continue;
-                    double offset[D];
-                    face_offset<D>(g, f, i, e >= 0, offset);
+                // Geometry::face_offset (:322-326) of UL entries loaded together, then applied in order.
+                constexpr int UL = LUNA_LIMIT_UNROLL;
+                for (int t2 = 0; t2 < count; t2 += UL) {
+                    double offset[UL][D];
+                    bool use[UL];
 #pragma unroll
-                    for (int v = 0; v < V; ++v) {
-                        const double dp = dot<D>(grad[v], offset);
-                        if (dp > 0 && Wi[v] + dp > hi[v]) {
-                            const double s = hi[v] > Wi[v] ? (hi[v] - Wi[v]) / dp : 0.0;
+                    for (int u = 0; u < UL; ++u) {
+                        use[u] = false;
+                        if (t2 + u < count) {
+                            const int slot = base + (t2 + u) * SLICE;
+                            const int e = g.list[slot], f = e >= 0 ? e : ~e;
+                            const double* face = g.face + (size_t)f * FACE_ROW;
+                            use[u] = !(any_tiny && face[FA::MAG] < tiny);
+                            double rel[D];
+                            load_vec<D>(face + FA::FC, rel);
+                            if (e < 0) {  // the face centroid is stored relative to the owner: move it to this generator
+                                double xo[D], sep[D];
+                                load_vec<D>(g.cell + (size_t)g.other[slot] * CELL_ROW + C::X, xo);
+                                separation<D>(g, xi, xo, sep);
 #pragma unroll
-                            for (int a = 0; a < D; ++a) grad[v][a] = grad[v][a] * s;
-                        } else if (dp < 0 && Wi[v] + dp < lo[v]) {
-                            const double s = lo[v] < Wi[v] ? (lo[v] - Wi[v]) / dp : 0.0;
+                                for (int a = 0; a < D; ++a) rel[a] = rel[a] + sep[a];
+                            }
 #pragma unroll
-                            for (int a = 0; a < D; ++a) grad[v][a] = grad[v][a] * s;
+                            for (int a = 0; a < D; ++a) offset[u][a] = rel[a] - centroid[a];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < UL; ++u) {
+                        if (!use[u]) continue;
+#pragma unroll
+                        for (int v = 0; v < V; ++v) {
+                            const double dp = dot<D>(grad[v], offset[u]);
+                            if (dp > 0 && Wi[v] + dp > s.hi[v]) {
+                                const double sc = s.hi[v] > Wi[v] ? (s.hi[v] - Wi[v]) / dp : 0.0;
+#pragma unroll
+                                for (int a = 0; a < D; ++a) grad[v][a] = grad[v][a] * sc;
+                            } else if (dp < 0 && Wi[v] + dp < s.lo[v]) {
+                                const double sc = s.lo[v] < Wi[v] ? (s.lo[v] - Wi[v]) / dp : 0.0;
+#pragma unroll
+                                for (int a = 0; a < D; ++a) grad[v][a] = grad[v][a] * sc;
+                            }
                         }
                     }
                 }
             }
-            double* out = r.grad + (size_t)i * V * D;
+            double* out = r.grad + (size_t)i * GR;
 #pragma unroll
             for (int v = 0; v < V; ++v)
 #pragma unroll
@@ -300,97 +400,105 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(Geo g, Rows r, Gas gas, double 
     block_min(smallest, c.dt);
 }
 
-// finite_volume.hpp:1306-1329 FaceFlux::side() at order 1: half step (over step_dt) + g.r, positivity fallback.
+// finite_volume.hpp:1306-1329 FaceFlux::side() at order 1: half step (over step_dt) + g.r, positivity fallback. prow,
+// grow: the cell's primitive half and gradient row, read here so that only one side's rows are live at a time.
 template <int D>
-__device__ __forceinline__ void face_side(const Rows& r, const Gas& gas, int i, const double* off, double step_dt, int order,
-                                          double* Wf, int& fallbacks) {
-    constexpr int V = D + 2, PW = V + 2;
-    const double* W = r.prims + (size_t)i * PW;
-    double Wc[V];
+__device__ __forceinline__ void face_side(const Gas& gas, const double* prow, const double* grow, const double* off, double step_dt,
+                                          int order, double* Wf, int& fallbacks) {
+    constexpr int V = D + 2;
+    double W[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) Wc[v] = W[v];
+    for (int v = 0; v < V; ++v) W[v] = prow[v];
     if (order == 0) {
 #pragma unroll
-        for (int v = 0; v < V; ++v) Wf[v] = Wc[v];
+        for (int v = 0; v < V; ++v) Wf[v] = W[v];
         return;
     }
     double gr[V * D];
 #pragma unroll
-    for (int k = 0; k < V * D; ++k) gr[k] = r.grad[(size_t)i * V * D + k];
-    half_step<D>(gas, Wc, gr, step_dt, Wf);
+    for (int k = 0; k < V * D; ++k) gr[k] = grow[k];
+    half_step<D>(gas, W, gr, step_dt, Wf);
 #pragma unroll
     for (int v = 0; v < V; ++v) Wf[v] += dot<D>(gr + v * D, off);
     if (Wf[0] > 0 && Wf[D + 1] > 0) return;
     ++fallbacks;
 #pragma unroll
-    for (int v = 0; v < V; ++v) Wf[v] = Wc[v];
+    for (int v = 0; v < V; ++v) Wf[v] = W[v];
 }
 
 // finite_volume.hpp:1451-1465 MovingFlux<false>::pass_side(): W + g.r on the mesh of t^n.
 template <int D>
-__device__ __forceinline__ void pass_side_first(const Rows& r, int i, const double* off, double* Wf, int& fallbacks) {
-    constexpr int V = D + 2, PW = V + 2;
-    const double* W = r.prims + (size_t)i * PW;
-    double Wc[V];
+__device__ __forceinline__ void pass_side_first(const double* prow, const double* grow, const double* off, double* Wf, int& fallbacks) {
+    constexpr int V = D + 2;
+    double W[V];
 #pragma unroll
-    for (int v = 0; v < V; ++v) Wc[v] = W[v];
+    for (int v = 0; v < V; ++v) W[v] = prow[v];
 #pragma unroll
     for (int v = 0; v < V; ++v) {
         double gv[D];
 #pragma unroll
-        for (int a = 0; a < D; ++a) gv[a] = r.grad[(size_t)i * V * D + v * D + a];
-        Wf[v] = Wc[v] + dot<D>(gv, off);
+        for (int a = 0; a < D; ++a) gv[a] = grow[v * D + a];
+        Wf[v] = W[v] + dot<D>(gv, off);
     }
     if (Wf[0] > 0 && Wf[D + 1] > 0) return;
     ++fallbacks;
 #pragma unroll
-    for (int v = 0; v < V; ++v) Wf[v] = Wc[v];
+    for (int v = 0; v < V; ++v) Wf[v] = W[v];
 }
 
-// finite_volume.hpp:1380-1397 FaceFlux::face_velocity()
+// The position and centroid of one side of a face, and where its primitive half and gradient row are.
 template <int D>
-__device__ __forceinline__ void face_velocity(const Geo& g, int f, int o, int j, const double* wL, const double* wR, double* wf) {
-    double d[D], c[D], dw[D];
-    separation<D>(g, o, j, d);
-    const double dist = sqrt(dot<D>(d, d));
-    const double inv_dist = 1.0 / dist;
-#pragma unroll
-    for (int a = 0; a < D; ++a) c[a] = g.fc[(size_t)f * D + a] - d[a] * 0.5;
-#pragma unroll
-    for (int a = 0; a < D; ++a) dw[a] = wL[a] - wR[a];
-    double facv = dot<D>(c, dw) * inv_dist;
-    const double limit = 0.9 * dist, c2 = dot<D>(c, c);
-    if (c2 > limit * limit) facv *= limit / sqrt(c2);
-#pragma unroll
-    for (int a = 0; a < D; ++a) wf[a] = (wL[a] + wR[a]) * 0.5 + d[a] * (facv * inv_dist);
+struct SideRows {
+    double x[D], c[D];
+    const double* p;
+    const double* g;
+};
+template <int D>
+__device__ __forceinline__ void load_side(const Geo& g, const Rows& r, int i, SideRows<D>& s) {
+    using C = CellAt<D>;
+    const double* row = g.cell + (size_t)i * CELL_ROW;
+    s.p = row + C::W;
+    s.g = r.grad + (size_t)i * grad_row<D>();
+    load_vec<D>(row + C::X, s.x);
+    load_vec<D>(row + C::C, s.c);
 }
 
 // One face's flux row: FaceFlux::operator() (:1415-1434) statically, MovingFlux::operator() (:1466-1490) on a moving step.
 template <int D, int Pass, int Riemann>
-__global__ __launch_bounds__(BLOCK) void k_flux(Geo g, Rows r, Gas gas, int order, double dt, double unit, unsigned long long* fallbacks) {
+__global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, Gas gas, int order, double dt, double unit, unsigned long long* fallbacks) {
     constexpr int V = D + 2;
-    const int f = blockIdx.x * BLOCK + threadIdx.x;
+    using FA = FaceAt<D>;
+    const int f = xcd_block() * BLOCK + threadIdx.x;
     int count = 0;
     if (f < g.F) {
-        const int o = g.owner[f], j = g.neighbour[f];
-        double a[D];
+        const double* face = g.face + (size_t)f * FACE_ROW;
+        double rec[FACE_ROW - 1];
 #pragma unroll
-        for (int k = 0; k < D; ++k) a[k] = g.area[(size_t)f * D + k];
-        const double magnitude = sqrt(dot<D>(a, a));
-        if constexpr (Pass == 1) g.magnitude_out[f] = magnitude;  // for the next sweep on this mesh
+        for (int k = 0; k < FACE_ROW - 1; ++k) rec[k] = face[k];
+        const int2 pair = *reinterpret_cast<const int2*>(face + FA::PAIR);
+        const int o = pair.x, j = pair.y;
+        const double magnitude = rec[FA::MAG];
         double F[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) F[v] = 0;
         if (magnitude != 0) {
-            double L[V], R[V], n[D], wf[D], ro[D], rj[D];
+            SideRows<D> so, sj;
+            load_side<D>(g, r, o, so);
+            load_side<D>(g, r, j, sj);
+            double L[V], R[V], n[D], wf[D], ro[D], rj[D], d[D];
             const double inv = 1.0 / magnitude;
 #pragma unroll
             for (int k = 0; k < D; ++k) {
-                n[k] = a[k] * inv;
+                n[k] = rec[FA::A + k] * inv;
                 wf[k] = 0;
             }
-            face_offset<D>(g, f, o, true, ro);
-            face_offset<D>(g, f, j, false, rj);
+            // face_offset (:322-326) of both sides
+            separation<D>(g, sj.x, so.x, d);  // the owner seen from the neighbour
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                ro[k] = rec[FA::FC + k] - so.c[k];
+                rj[k] = rec[FA::FC + k] + d[k] - sj.c[k];
+            }
             if constexpr (Pass < 0) {
                 const double half = 0.5 * dt;
 #pragma unroll
@@ -398,8 +506,8 @@ __global__ __launch_bounds__(BLOCK) void k_flux(Geo g, Rows r, Gas gas, int orde
                     ro[k] = ro[k] + wf[k] * half;
                     rj[k] = rj[k] + wf[k] * half;
                 }
-                face_side<D>(r, gas, o, ro, dt, order, L, count);
-                face_side<D>(r, gas, j, rj, dt, order, R, count);
+                face_side<D>(gas, so.p, so.g, ro, dt, order, L, count);
+                face_side<D>(gas, sj.p, sj.g, rj, dt, order, R, count);
             } else {
                 double wo[D], wj[D];
 #pragma unroll
@@ -407,10 +515,23 @@ __global__ __launch_bounds__(BLOCK) void k_flux(Geo g, Rows r, Gas gas, int orde
                     wo[k] = (double)r.steps[(size_t)o * D + k] * unit;
                     wj[k] = (double)r.steps[(size_t)j * D + k] * unit;
                 }
-                face_velocity<D>(g, f, o, j, wo, wj, wf);
+                // face_velocity (:1380-1397)
+                double sep[D], cf[D], dw[D];
+                separation<D>(g, so.x, sj.x, sep);
+                const double dist = sqrt(dot<D>(sep, sep));
+                const double inv_dist = 1.0 / dist;
+#pragma unroll
+                for (int k = 0; k < D; ++k) cf[k] = rec[FA::FC + k] - sep[k] * 0.5;
+#pragma unroll
+                for (int k = 0; k < D; ++k) dw[k] = wo[k] - wj[k];
+                double facv = dot<D>(cf, dw) * inv_dist;
+                const double limit = 0.9 * dist, c2 = dot<D>(cf, cf);
+                if (c2 > limit * limit) facv *= limit / sqrt(c2);
+#pragma unroll
+                for (int k = 0; k < D; ++k) wf[k] = (wo[k] + wj[k]) * 0.5 + sep[k] * (facv * inv_dist);
                 if constexpr (Pass == 0) {
-                    pass_side_first<D>(r, o, ro, L, count);
-                    pass_side_first<D>(r, j, rj, R, count);
+                    pass_side_first<D>(so.p, so.g, ro, L, count);
+                    pass_side_first<D>(sj.p, sj.g, rj, R, count);
                 } else {
                     // side(i, r + c^n+1 - c^n + w dt) with the FaceFlux dt = 2 dt (:1452-1454)
                     const double step_dt = 2 * dt;
@@ -418,15 +539,15 @@ __global__ __launch_bounds__(BLOCK) void k_flux(Geo g, Rows r, Gas gas, int orde
 #pragma unroll
                     for (int k = 0; k < D; ++k) {
                         const double moved = wo[k] * (0.5 * step_dt);
-                        r2[k] = ro[k] + g.centroid[(size_t)o * D + k] - r.old_centroid[(size_t)o * D + k] + moved;
+                        r2[k] = ro[k] + so.c[k] - r.old_centroid[(size_t)o * D + k] + moved;
                     }
-                    face_side<D>(r, gas, o, r2, step_dt, 1, L, count);
+                    face_side<D>(gas, so.p, so.g, r2, step_dt, 1, L, count);
 #pragma unroll
                     for (int k = 0; k < D; ++k) {
                         const double moved = wj[k] * (0.5 * step_dt);
-                        r2[k] = rj[k] + g.centroid[(size_t)j * D + k] - r.old_centroid[(size_t)j * D + k] + moved;
+                        r2[k] = rj[k] + sj.c[k] - r.old_centroid[(size_t)j * D + k] + moved;
                     }
-                    face_side<D>(r, gas, j, r2, step_dt, 1, R, count);
+                    face_side<D>(gas, sj.p, sj.g, r2, step_dt, 1, R, count);
                 }
             }
             if constexpr (Riemann == 1)
@@ -442,19 +563,41 @@ __global__ __launch_bounds__(BLOCK) void k_flux(Geo g, Rows r, Gas gas, int orde
     block_count(count, fallbacks);
 }
 
-// finite_volume.hpp:1587-1603 update_body: Q -= dt sum_k sign_k F in list order; NextPrims: the next primitive rows
-// (a moving step's second pass, and here every static step, which the next time_step() would compute the same way).
+// finite_volume.hpp:1587-1603 update_body: Q -= dt sum_k sign_k F in list order; NextPrims: the primitive half of the
+// cell row for the next step (a moving step's second pass, and every static step, which the next time_step() would
+// compute the same way).
 template <int D, bool NextPrims>
 __global__ __launch_bounds__(BLOCK) void k_update(Geo g, Rows r, Gas gas, double dt) {
-    constexpr int V = D + 2, PW = V + 2;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    constexpr int V = D + 2;
+    using C = CellAt<D>;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
     if (i >= g.N) return;
     double sum[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) sum[v] = 0;
-    const int first = g.start[i], end = g.start[i + 1];
-    for (int k = first; k < end; ++k) {
-        const int e = g.list[k], f = e >= 0 ? e : ~e;
+    const int count = g.count[i], base = first_slot(g, i);
+    constexpr int UN = LUNA_UPDATE_UNROLL;
+    int t = 0;
+    for (; t + UN - 1 < count; t += UN) {  // UN rows in flight, summed in list order
+        int e[UN];
+#pragma unroll
+        for (int u = 0; u < UN; ++u) e[u] = g.list[base + (t + u) * SLICE];
+        double F[UN][V];
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            const double* row = r.flux + (size_t)(e[u] >= 0 ? e[u] : ~e[u]) * V;
+#pragma unroll
+            for (int v = 0; v < V; ++v) F[u][v] = row[v];
+        }
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            const double sign = e[u] >= 0 ? 1.0 : -1.0;
+#pragma unroll
+            for (int v = 0; v < V; ++v) sum[v] += sign * F[u][v];
+        }
+    }
+    for (; t < count; ++t) {
+        const int e = g.list[base + t * SLICE], f = e >= 0 ? e : ~e;
         const double sign = e >= 0 ? 1.0 : -1.0;
 #pragma unroll
         for (int v = 0; v < V; ++v) sum[v] += sign * r.flux[(size_t)f * V + v];
@@ -463,16 +606,15 @@ __global__ __launch_bounds__(BLOCK) void k_update(Geo g, Rows r, Gas gas, double
 #pragma unroll
     for (int v = 0; v < V; ++v) r.q[(size_t)i * V + v] = next[v] = r.q[(size_t)i * V + v] - dt * sum[v];
     if constexpr (NextPrims) {
-        const double inv = 1.0 / g.volume[i];
+        double* row = g.cell + (size_t)i * CELL_ROW;
+        const double inv = 1.0 / row[C::VOL];
         double U[V], W[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) U[v] = next[v] * inv;
         primitive<D>(gas, U, W);
-        double* row = r.prims + (size_t)i * PW;
 #pragma unroll
-        for (int v = 0; v < V; ++v) row[v] = W[v];
-        row[V] = sound_speed<D>(gas, W);
-        row[V + 1] = g.radius[i];
+        for (int v = 0; v < V; ++v) row[C::W + v] = W[v];
+        row[C::CS] = sound_speed<D>(gas, W);
     }
 }
 
@@ -485,12 +627,12 @@ __device__ __forceinline__ long long snap_key(double x, const Grid& grid, int a)
 // finite_volume.hpp:1506-1514 keep_centroids: c^n kept; x to its grid point; the move to x + dt w in whole grid steps.
 template <int D>
 __global__ __launch_bounds__(BLOCK) void k_keep_centroids(Geo g, Rows r, Grid grid, double dt) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
     if (i >= g.N) return;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         const size_t at = (size_t)i * D + a;
-        r.old_centroid[at] = g.centroid[at];
+        r.old_centroid[at] = g.cell[(size_t)i * CELL_ROW + CellAt<D>::C + a];
         const double x = r.x[at];
         const long long k0 = snap_key(x, grid, a), k1 = snap_key(x + dt * r.w[at], grid, a);
         r.x[at] = (double)k0 * grid.step;
@@ -505,7 +647,7 @@ __global__ __launch_bounds__(BLOCK) void k_keep_centroids(Geo g, Rows r, Grid gr
 // finite_volume.hpp:1219-1231 drift on the grid: x to its grid point plus its whole steps, wrapped.
 template <int D>
 __global__ __launch_bounds__(BLOCK) void k_drift(int N, Rows r, Grid grid) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
     if (i >= N) return;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
@@ -524,13 +666,6 @@ template <int D>
 hipError_t launch_primitives(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas) {
     if (g.N == 0) return hipSuccess;
     hipLaunchKernelGGL(k_primitives<D>, blocks(g.N), dim3(BLOCK), 0, s, g, r, gas);
-    return hipGetLastError();
-}
-
-template <int D>
-hipError_t launch_magnitudes(hipStream_t s, const Geo& g, double* magnitude) {
-    if (g.F == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_magnitudes<D>, blocks(g.F), dim3(BLOCK), 0, s, g, magnitude);
     return hipGetLastError();
 }
 
@@ -602,7 +737,6 @@ hipError_t launch_drift(hipStream_t s, int N, const Rows& r, const Grid& grid) {
 
 #define LUNA_INSTANTIATE(D)                                                                                                          \
     template hipError_t launch_primitives<D>(hipStream_t, const Geo&, const Rows&, const Gas&);                                     \
-    template hipError_t launch_magnitudes<D>(hipStream_t, const Geo&, double*);                                                     \
     template hipError_t launch_sweep<D>(hipStream_t, const Geo&, const Rows&, const Gas&, double, bool, bool, bool, int,             \
                                         const Counters&);                                                                           \
     template hipError_t launch_flux<D>(hipStream_t, const Geo&, const Rows&, const Gas&, int, int, int, double, double,              \

@@ -206,7 +206,8 @@ __device__ __forceinline__ void riemann(int kind, const Gas& gas, const double* 
         hllc<D>(gas, L, R, n, w, F);
 }
 
-// The nearest periodic image of a separation (the role of tet_mesh::wrap in Geometry::separation, finite_volume.hpp:279-282).
-__device__ __forceinline__ double wrap(double d, double side) { return d - side * round(d / side); }
+// The nearest periodic image of a separation, given the box side and its inverse (the role of tet_mesh::wrap(d, side,
+// inverse) in Geometry::separation, finite_volume.hpp:279-282).
+__device__ __forceinline__ double wrap(double d, double side, double inverse) { return d - side * round(d * inverse); }
 
 }  // namespace luna

@@ -54,7 +54,8 @@ class FV1Options(ctypes.Structure):
 
 
 def library_path():
-    return os.path.join(HERE, "libluna_hydro.so")
+    """The built library; LUNA_HYDRO_LIBRARY names another build of the same sources (A/B timing of tuning knobs)."""
+    return os.environ.get("LUNA_HYDRO_LIBRARY") or os.path.join(HERE, "libluna_hydro.so")
 
 
 def lib():

@@ -191,8 +191,9 @@ static void riemann(int kind, int D, const orc_gas* gas, const double* L, const 
 
 /* ------------------------------------------------------------------ D = 2, 3 on a Voronoi mesh ----------------------------- */
 
-/* The nearest periodic image of a separation (the role of tet_mesh::wrap, finite_volume.hpp:279-282). */
-static double wrap(double d, double side) { return d - side * round(d / side); }
+/* The nearest periodic image of a separation, with the side's inverse as tet_mesh::wrap(d, side, inverse) takes it
+ * (finite_volume.hpp:279-282). */
+static double wrap(double d, double side) { return d - side * round(d * (1.0 / side)); }
 
 /* finite_volume.hpp:279-282 Geometry::separation(i, j): generator j seen from generator i. */
 static void separation(const orc_geometry* g, int i, int j, double* d) {
