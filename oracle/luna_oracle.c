@@ -836,3 +836,750 @@ double orc_line_step_fused(orc_line* l, double dt, double courant) {
     line_swap(l);
     return smallest;
 }
+
+/* ------------------------------------------------------------------ derivatives of a static step --------------------------- */
+/* Tangent: J dx; adjoint: J^T ybar (added into the bars). dt and the geometry are fixed. At every switch the forward takes
+ * (a floor, a wave case, a guard, a limiter clip, a positivity fallback) the derivative is that of the branch it took,
+ * as euler.hpp:115-117 states for the reference's own derivatives. Written from the forward arithmetic above by the
+ * chain rule. */
+
+/* euler.hpp:119-126: which floors primitive() applied (bit 0 density, bit 1 internal energy). */
+static int primitive_floors(int D, const orc_gas* gas, const double* U) {
+    const double rho = fmax(U[0], gas->smallr);
+    const double inv_rho = 1.0 / rho;
+    double v[MAXD];
+    for (int k = 0; k < D; ++k) v[k] = U[1 + k] * inv_rho;
+    const double smalle = gas->smallc * gas->smallc / gas->gamma / (gas->gamma - 1.0);
+    return (U[0] >= gas->smallr ? 0 : 1) | (U[D + 1] - 0.5 * rho * dot(D, v, v) >= smalle * rho ? 0 : 2);
+}
+
+/* euler.hpp:128-140 primitive_tangent(): a floored density or energy has derivative zero in what the floor replaced. */
+void orc_primitive_tangent(int D, const orc_gas* gas, const double* U, const double* dU, double* dW) {
+    const int floors = primitive_floors(D, gas, U);
+    const double rho = fmax(U[0], gas->smallr), inv_rho = 1.0 / rho;
+    const double drho = (floors & 1) ? 0.0 : dU[0];
+    double v[MAXD], dv[MAXD];
+    for (int k = 0; k < D; ++k) v[k] = U[1 + k] * inv_rho;
+    for (int k = 0; k < D; ++k) dv[k] = (dU[1 + k] - v[k] * drho) * inv_rho;
+    const double smalle = gas->smallc * gas->smallc / gas->gamma / (gas->gamma - 1.0);
+    const double dkinetic = 0.5 * drho * dot(D, v, v) + rho * dot(D, v, dv);
+    const double de = (floors & 2) ? smalle * drho : dU[D + 1] - dkinetic;
+    dW[0] = drho;
+    for (int k = 0; k < D; ++k) dW[1 + k] = dv[k];
+    dW[D + 1] = (gas->gamma - 1.0) * de;
+}
+
+/* euler.hpp:142-162 primitive_adjoint(): Ubar = J^T Wbar. */
+void orc_primitive_adjoint(int D, const orc_gas* gas, const double* U, const double* Wbar, double* Ubar) {
+    const int floors = primitive_floors(D, gas, U);
+    const double rho = fmax(U[0], gas->smallr), inv_rho = 1.0 / rho;
+    double v[MAXD], vbar[MAXD];
+    for (int k = 0; k < D; ++k) {
+        v[k] = U[1 + k] * inv_rho;
+        vbar[k] = Wbar[1 + k];
+    }
+    const double smalle = gas->smallc * gas->smallc / gas->gamma / (gas->gamma - 1.0);
+    const double ebar = (gas->gamma - 1.0) * Wbar[D + 1];
+    double rhobar = Wbar[0], energybar = 0.0;
+    if (floors & 2) {
+        rhobar += smalle * ebar; /* e = smalle rho */
+    } else {                     /* e = E - rho v.v / 2 */
+        energybar = ebar;
+        rhobar -= 0.5 * dot(D, v, v) * ebar;
+        for (int k = 0; k < D; ++k) vbar[k] = vbar[k] - v[k] * (rho * ebar);
+    }
+    rhobar -= dot(D, v, vbar) * inv_rho; /* v = m / rho */
+    Ubar[0] = (floors & 1) ? 0.0 : rhobar;
+    for (int k = 0; k < D; ++k) Ubar[1 + k] = vbar[k] * inv_rho;
+    Ubar[D + 1] = energybar;
+}
+
+/* euler.hpp:216-237 half_step_tangent() (dt fixed). g, dg: [V][D]. */
+void orc_half_step_tangent(int D, const orc_gas* gas, const double* W, const double* dW, const double* g, const double* dg, double dt,
+                           double* out) {
+    const int P = D + 1;
+    const double rho = W[0], inv_rho = 1.0 / W[0];
+    const double rho_c2 = gas->gamma * W[P];
+    const double drho = dW[0], dinv_rho = -inv_rho * inv_rho * dW[0], drho_c2 = gas->gamma * dW[P];
+    double dA[MAXV] = {0, 0, 0, 0, 0};
+    for (int d = 0; d < D; ++d) {
+        const double vn = W[1 + d], dvn = dW[1 + d];
+        dA[0] += dvn * g[0 * D + d] + vn * dg[0 * D + d] + drho * g[(1 + d) * D + d] + rho * dg[(1 + d) * D + d];
+        for (int k = 0; k < D; ++k) {
+            double row = dvn * g[(1 + k) * D + d] + vn * dg[(1 + k) * D + d];
+            if (k == d) row += dinv_rho * g[P * D + d] + inv_rho * dg[P * D + d];
+            dA[1 + k] += row;
+        }
+        dA[P] += drho_c2 * g[(1 + d) * D + d] + rho_c2 * dg[(1 + d) * D + d] + dvn * g[P * D + d] + vn * dg[P * D + d];
+    }
+    for (int v = 0; v < D + 2; ++v) out[v] = dW[v] - 0.5 * dt * dA[v];
+}
+
+/* euler.hpp:239-272 half_step_adjoint(): adds J^T hbar to Wbar and gbar. */
+void orc_half_step_adjoint(int D, const orc_gas* gas, const double* W, const double* g, double dt, const double* hbar, double* Wbar,
+                           double* gbar) {
+    const int P = D + 1;
+    const double rho = W[0], inv_rho = 1.0 / W[0], rho_c2 = gas->gamma * W[P];
+    double Abar[MAXV];
+    for (int v = 0; v < D + 2; ++v) {
+        Wbar[v] += hbar[v];
+        Abar[v] = -0.5 * dt * hbar[v];
+    }
+    double inv_rhobar = 0.0, rho_c2bar = 0.0;
+    for (int d = 0; d < D; ++d) {
+        const double vn = W[1 + d];
+        /* A[0] += vn g0_d + rho g(1+d)_d */
+        Wbar[1 + d] += Abar[0] * g[0 * D + d];
+        gbar[0 * D + d] += Abar[0] * vn;
+        Wbar[0] += Abar[0] * g[(1 + d) * D + d];
+        gbar[(1 + d) * D + d] += Abar[0] * rho;
+        /* A[1+k] += vn g(1+k)_d (+ inv_rho gP_d if k == d) */
+        for (int k = 0; k < D; ++k) {
+            Wbar[1 + d] += Abar[1 + k] * g[(1 + k) * D + d];
+            gbar[(1 + k) * D + d] += Abar[1 + k] * vn;
+            if (k == d) {
+                inv_rhobar += Abar[1 + k] * g[P * D + d];
+                gbar[P * D + d] += Abar[1 + k] * inv_rho;
+            }
+        }
+        /* A[P] += rho_c2 g(1+d)_d + vn gP_d */
+        rho_c2bar += Abar[P] * g[(1 + d) * D + d];
+        gbar[(1 + d) * D + d] += Abar[P] * rho_c2;
+        Wbar[1 + d] += Abar[P] * g[P * D + d];
+        gbar[P * D + d] += Abar[P] * vn;
+    }
+    Wbar[0] -= inv_rho * inv_rho * inv_rhobar;
+    Wbar[P] += gas->gamma * rho_c2bar;
+}
+
+/* euler.hpp:358-375: the tangent of side_with() with vn's tangent given (w fixed). */
+static void side_with_tangent(int D, const orc_gas* gas, const double* W, const double* dW, const double* n, const side_t* s,
+                              double dvn, side_t* d) {
+    const int P = D + 1;
+    for (int k = 0; k < D; ++k) d->v[k] = dW[1 + k];
+    d->vn = dvn;
+    d->c = 0;
+    d->E = dW[P] * gas->inverse_gm1 + dW[0] * (0.5 * dot(D, s->v, s->v)) + W[0] * dot(D, s->v, d->v);
+    const double m = W[0] * s->vn, dm = dW[0] * s->vn + W[0] * dvn;
+    d->F[0] = dm;
+    for (int k = 0; k < D; ++k) d->F[1 + k] = dm * s->v[k] + m * d->v[k] + dW[P] * n[k];
+    d->F[P] = (d->E + dW[P]) * s->vn + (s->E + W[P]) * dvn;
+    d->U[0] = dW[0];
+    for (int k = 0; k < D; ++k) d->U[1 + k] = dW[0] * s->v[k] + W[0] * d->v[k];
+    d->U[P] = d->E;
+}
+
+/* euler.hpp:387-411: the adjoint of side_with(): from the bars of F, U, E and v, adds J^T to Wbar; returns vn's bar. */
+static double side_with_adjoint(int D, const orc_gas* gas, const double* W, const double* n, const side_t* s, const double* Fbar,
+                                const double* Ubar, double Ebar, double* vbar, double* Wbar) {
+    const int P = D + 1;
+    const double m = W[0] * s->vn;
+    Ebar += Fbar[P] * s->vn + Ubar[P];
+    double Pbar = Fbar[P] * s->vn, vnbar = Fbar[P] * (s->E + W[P]), mbar = Fbar[0], rhobar = Ubar[0];
+    for (int k = 0; k < D; ++k) {
+        Pbar += Fbar[1 + k] * n[k];
+        mbar += Fbar[1 + k] * s->v[k];
+        vbar[k] += Fbar[1 + k] * m + Ubar[1 + k] * W[0];
+        rhobar += Ubar[1 + k] * s->v[k];
+    }
+    rhobar += mbar * s->vn;
+    vnbar += mbar * W[0];
+    Pbar += Ebar * gas->inverse_gm1;
+    rhobar += Ebar * (0.5 * dot(D, s->v, s->v));
+    for (int k = 0; k < D; ++k) vbar[k] = vbar[k] + s->v[k] * (Ebar * W[0]);
+    Wbar[0] += rhobar;
+    for (int k = 0; k < D; ++k) Wbar[1 + k] += vbar[k];
+    Wbar[P] += Pbar;
+    return vnbar;
+}
+
+/* euler.hpp:428-437: the adjoint of to_lab() (linear in F, w fixed). */
+static void to_lab_adjoint(int D, const double* Fbar, const double* w, double* G) {
+    const int P = D + 1;
+    for (int v = 0; v < D + 2; ++v) G[v] = Fbar[v];
+    G[0] += 0.5 * dot(D, w, w) * Fbar[P];
+    for (int k = 0; k < D; ++k) {
+        G[0] += w[k] * Fbar[1 + k];
+        G[1 + k] += w[k] * Fbar[P];
+    }
+}
+
+/* euler.hpp:649-695 HLLC::waves(): the wave speeds as flux() computes them and the branch pattern: bit 0 S_L from L, bit 1
+ * S_R from L, bit 2 left, bit 3 star, bit 4 ok, bits 5-6 c^2 > 0 on L, R, bits 7-9 the 1e-20 guard on S_M's, rho*'s and
+ * E*'s divisor. */
+typedef struct {
+    double vnL, vnR, inv_rhoL, inv_rhoR, c2L, c2R, cL, cR, SL, SR, denomL, denomR, den, SM;
+    int pattern;
+} hllc_waves;
+
+static void hllc_waves_of(int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, hllc_waves* a) {
+    const int P = D + 1;
+    const double eps = 1e-20;
+    double rl[MAXD], rr[MAXD];
+    for (int k = 0; k < D; ++k) {
+        rl[k] = L[1 + k] - w[k];
+        rr[k] = R[1 + k] - w[k];
+    }
+    a->vnL = dot(D, rl, n);
+    a->vnR = dot(D, rr, n);
+    a->inv_rhoL = 1.0 / L[0];
+    a->inv_rhoR = 1.0 / R[0];
+    a->c2L = gas->gamma * L[P] * a->inv_rhoL;
+    a->c2R = gas->gamma * R[P] * a->inv_rhoR;
+    const int ok = a->c2L >= 0 && a->c2R >= 0;
+    a->cL = a->c2L > 0 ? sqrt(a->c2L) : 0.0;
+    a->cR = a->c2R > 0 ? sqrt(a->c2R) : 0.0;
+    const double sl = a->vnL - a->cL, sr = a->vnR - a->cR, tl = a->vnL + a->cL, tr = a->vnR + a->cR;
+    a->SL = fmin(sl, sr);
+    a->SR = fmax(tl, tr);
+    a->denomL = L[0] * (a->SL - a->vnL);
+    a->denomR = R[0] * (a->SR - a->vnR);
+    a->den = a->denomL - a->denomR;
+    a->SM = (R[P] - L[P] + a->denomL * a->vnL - a->denomR * a->vnR) / nonzero(a->den);
+    const int left = ok && (a->SL >= 0 || a->SM >= 0);
+    const int star = ok && a->SL < 0 && (a->SM >= 0 || a->SR > 0);
+    const double S = left ? a->SL : a->SR, denom = left ? a->denomL : a->denomR;
+    a->pattern = (sr < sl ? 0 : 1) | (tr > tl ? 0 : 2) | left << 2 | star << 3 | ok << 4 | (a->c2L > 0) << 5 | (a->c2R > 0) << 6 |
+                 (fabs(a->den) < eps) << 7 | (star && fabs(S - a->SM) < eps) << 8 | (star && fabs(denom) < eps) << 9;
+}
+
+/* euler.hpp:697-745 HLLC::flux_tangent(): dF = J (dL, dR) at fixed n, w; a guarded divisor has derivative zero. F: the flux
+ * from the same intermediates (the forward's operations). */
+void orc_hllc_tangent(int D, const orc_gas* gas, const double* L, const double* R, const double* dL, const double* dR, const double* n,
+                      const double* w, double* F, double* dF) {
+    const int P = D + 1;
+    hllc_waves a;
+    hllc_waves_of(D, gas, L, R, n, w, &a);
+    const int left = (a.pattern >> 2) & 1, star = (a.pattern >> 3) & 1;
+    double dvl[MAXD], dvr[MAXD];
+    for (int k = 0; k < D; ++k) {
+        dvl[k] = dL[1 + k];
+        dvr[k] = dR[1 + k];
+    }
+    const double dvnL = dot(D, dvl, n), dvnR = dot(D, dvr, n);
+    const double dinvL = -a.inv_rhoL * a.inv_rhoL * dL[0], dinvR = -a.inv_rhoR * a.inv_rhoR * dR[0];
+    const double dcL = a.c2L > 0 ? gas->gamma * (dL[P] * a.inv_rhoL + L[P] * dinvL) / (2 * a.cL) : 0.0;
+    const double dcR = a.c2R > 0 ? gas->gamma * (dR[P] * a.inv_rhoR + R[P] * dinvR) / (2 * a.cR) : 0.0;
+    const double dSL = (a.pattern & 1) ? dvnL - dcL : dvnR - dcR, dSR = (a.pattern & 2) ? dvnL + dcL : dvnR + dcR;
+    const double ddenomL = dL[0] * (a.SL - a.vnL) + L[0] * (dSL - dvnL), ddenomR = dR[0] * (a.SR - a.vnR) + R[0] * (dSR - dvnR);
+    const double dnum = dR[P] - dL[P] + ddenomL * a.vnL + a.denomL * dvnL - ddenomR * a.vnR - a.denomR * dvnR;
+    const double dden = (a.pattern & 128) ? 0.0 : ddenomL - ddenomR, den = nonzero(a.den);
+    const double dSM = (dnum - a.SM * dden) / den;
+    double W[MAXV], dW[MAXV];
+    for (int v = 0; v < D + 2; ++v) {
+        W[v] = left ? L[v] : R[v];
+        dW[v] = left ? dL[v] : dR[v];
+    }
+    const double vn = left ? a.vnL : a.vnR, dvn = left ? dvnL : dvnR;
+    side_t s, ds;
+    side_with(D, gas, W, n, w, vn, &s);
+    side_with_tangent(D, gas, W, dW, n, &s, dvn, &ds);
+    if (!star) {
+        for (int v = 0; v < D + 2; ++v) {
+            F[v] = s.F[v];
+            dF[v] = ds.F[v];
+        }
+        to_lab(D, F, w);
+        to_lab(D, dF, w);
+        return;
+    }
+    const double S = left ? a.SL : a.SR, dS = left ? dSL : dSR;
+    const double denom = left ? a.denomL : a.denomR, ddenom = left ? ddenomL : ddenomR;
+    const double inv_side = left ? a.inv_rhoL : a.inv_rhoR, dinv_side = left ? dinvL : dinvR;
+    const double q = nonzero(S - a.SM), dq = (a.pattern & 256) ? 0.0 : dS - dSM;
+    const double rho_star = denom / q, drho_star = (ddenom - rho_star * dq) / q;
+    const double inv_denom = 1.0 / nonzero(denom), dinv_denom = (a.pattern & 512) ? 0.0 : -inv_denom * inv_denom * ddenom;
+    const double a2 = a.SM - s.vn, da2 = dSM - dvn;
+    const double a3 = a.SM + W[P] * inv_denom, da3 = dSM + dW[P] * inv_denom + W[P] * dinv_denom;
+    const double E_star = s.E * inv_side + a2 * a3;
+    const double dE_star = ds.E * inv_side + s.E * dinv_side + da2 * a3 + a2 * da3;
+    for (int v = 0; v < D + 2; ++v) {
+        double U_star, dU_star;
+        if (v == 0) {
+            U_star = rho_star;
+            dU_star = drho_star;
+        } else if (v == P) {
+            U_star = rho_star * E_star;
+            dU_star = drho_star * E_star + rho_star * dE_star;
+        } else {
+            const double vs = s.v[v - 1] + n[v - 1] * a2, dvs = ds.v[v - 1] + n[v - 1] * da2;
+            U_star = rho_star * vs;
+            dU_star = drho_star * vs + rho_star * dvs;
+        }
+        F[v] = s.F[v] + S * (U_star - s.U[v]);
+        dF[v] = ds.F[v] + dS * (U_star - s.U[v]) + S * (dU_star - ds.U[v]);
+    }
+    to_lab(D, F, w);
+    to_lab(D, dF, w);
+}
+
+/* euler.hpp:746-832 HLLC::flux_adjoint(): adds J^T Fbar to Lbar and Rbar; only the case the forward took is differentiated. */
+void orc_hllc_adjoint(int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, const double* Fbar_lab,
+                      double* Lbar, double* Rbar) {
+    const int P = D + 1;
+    hllc_waves a;
+    hllc_waves_of(D, gas, L, R, n, w, &a);
+    const int left = (a.pattern >> 2) & 1, star = (a.pattern >> 3) & 1;
+    double Fbar[MAXV];
+    to_lab_adjoint(D, Fbar_lab, w, Fbar);
+    double W[MAXV];
+    for (int v = 0; v < D + 2; ++v) W[v] = left ? L[v] : R[v];
+    double* Wbar = left ? Lbar : Rbar;
+    const double vn = left ? a.vnL : a.vnR;
+    side_t s;
+    side_with(D, gas, W, n, w, vn, &s);
+    double vnLbar = 0, vnRbar = 0;
+    if (!star) {
+        double Ubar0[MAXV] = {0, 0, 0, 0, 0}, vbar[MAXD] = {0, 0, 0};
+        const double vnbar = side_with_adjoint(D, gas, W, n, &s, Fbar, Ubar0, 0.0, vbar, Wbar);
+        if (left)
+            vnLbar += vnbar;
+        else
+            vnRbar += vnbar;
+    } else {
+        const double S = left ? a.SL : a.SR, denom = left ? a.denomL : a.denomR, inv_side = left ? a.inv_rhoL : a.inv_rhoR;
+        const double q = nonzero(S - a.SM), rho_star = denom / q;
+        const double inv_denom = 1.0 / nonzero(denom);
+        const double a2 = a.SM - s.vn, a3 = a.SM + W[P] * inv_denom;
+        const double E_star = s.E * inv_side + a2 * a3;
+        double U_star[MAXV], vs[MAXD];
+        U_star[0] = rho_star;
+        for (int k = 0; k < D; ++k) {
+            vs[k] = s.v[k] + n[k] * a2;
+            U_star[1 + k] = rho_star * vs[k];
+        }
+        U_star[P] = rho_star * E_star;
+        /* F = s.F + S (U* - s.U) */
+        double sFbar[MAXV], sUbar[MAXV], Ustarbar[MAXV], Sbar = 0;
+        for (int v = 0; v < D + 2; ++v) {
+            sFbar[v] = Fbar[v];
+            Sbar += Fbar[v] * (U_star[v] - s.U[v]);
+            Ustarbar[v] = S * Fbar[v];
+            sUbar[v] = -S * Fbar[v];
+        }
+        /* U* = (rho*, rho* v*, rho* E*) */
+        double rho_starbar = Ustarbar[0] + Ustarbar[P] * E_star, vsbar[MAXD];
+        for (int k = 0; k < D; ++k) {
+            rho_starbar += Ustarbar[1 + k] * vs[k];
+            vsbar[k] = Ustarbar[1 + k] * rho_star;
+        }
+        const double E_starbar = Ustarbar[P] * rho_star;
+        /* v* = s.v + n a2; E* = s.E inv_side + a2 a3 */
+        double svbar[MAXD], a2bar = E_starbar * a3;
+        for (int k = 0; k < D; ++k) {
+            svbar[k] = vsbar[k];
+            a2bar += vsbar[k] * n[k];
+        }
+        const double sEbar = E_starbar * inv_side, inv_sidebar = E_starbar * s.E, a3bar = E_starbar * a2;
+        /* a3 = SM + W_P inv_denom; a2 = SM - s.vn */
+        double SMbar = a3bar + a2bar;
+        Wbar[P] += a3bar * inv_denom;
+        const double inv_denombar = a3bar * W[P];
+        const double svnbar = -a2bar;
+        /* inv_denom = 1 / nonzero(denom); rho* = denom / q; q = nonzero(S - SM) */
+        double denombar = (a.pattern & 512) ? 0.0 : -inv_denom * inv_denom * inv_denombar;
+        denombar += rho_starbar / q;
+        const double qbar = -rho_starbar * rho_star / q;
+        if (!(a.pattern & 256)) {
+            Sbar += qbar;
+            SMbar -= qbar;
+        }
+        const double vnbar = side_with_adjoint(D, gas, W, n, &s, sFbar, sUbar, sEbar, svbar, Wbar) + svnbar;
+        if (left) {
+            vnLbar += vnbar;
+        } else {
+            vnRbar += vnbar;
+        }
+        /* the side's 1 / rho, wave speed and denominator */
+        double inv_rhoLbar = left ? inv_sidebar : 0.0, inv_rhoRbar = left ? 0.0 : inv_sidebar;
+        double SLbar = left ? Sbar : 0.0, SRbar = left ? 0.0 : Sbar;
+        double denomLbar = left ? denombar : 0.0, denomRbar = left ? 0.0 : denombar;
+        /* SM = num / nonzero(den), num = R_P - L_P + denomL vnL - denomR vnR, den = denomL - denomR */
+        const double den = nonzero(a.den);
+        const double numbar = SMbar / den;
+        const double denbar = (a.pattern & 128) ? 0.0 : -SMbar * a.SM / den;
+        Rbar[P] += numbar;
+        Lbar[P] -= numbar;
+        denomLbar += numbar * a.vnL + denbar;
+        vnLbar += numbar * a.denomL;
+        denomRbar += -numbar * a.vnR - denbar;
+        vnRbar -= numbar * a.denomR;
+        /* denomL = L0 (SL - vnL), denomR = R0 (SR - vnR) */
+        Lbar[0] += denomLbar * (a.SL - a.vnL);
+        SLbar += denomLbar * L[0];
+        vnLbar -= denomLbar * L[0];
+        Rbar[0] += denomRbar * (a.SR - a.vnR);
+        SRbar += denomRbar * R[0];
+        vnRbar -= denomRbar * R[0];
+        /* SL = min(vnL - cL, vnR - cR), SR = max(vnL + cL, vnR + cR): the side each took */
+        double cLbar = 0, cRbar = 0;
+        if (a.pattern & 1) {
+            vnLbar += SLbar;
+            cLbar -= SLbar;
+        } else {
+            vnRbar += SLbar;
+            cRbar -= SLbar;
+        }
+        if (a.pattern & 2) {
+            vnLbar += SRbar;
+            cLbar += SRbar;
+        } else {
+            vnRbar += SRbar;
+            cRbar += SRbar;
+        }
+        /* c = sqrt(gamma P inv_rho) where c^2 > 0 */
+        if (a.c2L > 0) {
+            const double c2bar = cLbar / (2 * a.cL);
+            Lbar[P] += c2bar * gas->gamma * a.inv_rhoL;
+            inv_rhoLbar += c2bar * gas->gamma * L[P];
+        }
+        if (a.c2R > 0) {
+            const double c2bar = cRbar / (2 * a.cR);
+            Rbar[P] += c2bar * gas->gamma * a.inv_rhoR;
+            inv_rhoRbar += c2bar * gas->gamma * R[P];
+        }
+        Lbar[0] -= inv_rhoLbar * a.inv_rhoL * a.inv_rhoL;
+        Rbar[0] -= inv_rhoRbar * a.inv_rhoR * a.inv_rhoR;
+    }
+    /* vn = (v - w) . n */
+    for (int k = 0; k < D; ++k) {
+        Lbar[1 + k] += vnLbar * n[k];
+        Rbar[1 + k] += vnRbar * n[k];
+    }
+}
+
+/* ---- D = 1 (finite_volume_1d.hpp:710-935) ---- */
+
+/* finite_volume_1d.hpp:234-253 clips(): for the face-centroid limiter, the last face that clipped variable v (-1: none),
+ * to which bound, and whether the clip zeroed the slope. */
+static void line_clips(const line_cell* cell, double gapL, double gapR, int v, int* last, int* upper, int* zero) {
+    const line_cell* me = cell;
+    const double dL = -gapL + cell[-1].c - me->c, dR = gapR + cell[1].c - me->c;
+    const double Wi = me->W[v], left = cell[-1].W[v], right = cell[1].W[v];
+    double g = ((right - Wi) - (left - Wi)) / (dR - dL);
+    const double r[2] = {-0.5 * gapL - me->c, 0.5 * gapR - me->c};
+    const double lo = fmin(left, right), hi = fmax(left, right);
+    *last = -1;
+    *upper = *zero = 0;
+    for (int f = 0; f < 2; ++f) {
+        const double dp = g * r[f];
+        if (dp > 0 && Wi + dp > hi) {
+            g = g * (hi > Wi ? (hi - Wi) / dp : 0.0);
+            *last = f, *upper = 1, *zero = !(hi > Wi);
+        } else if (dp < 0 && Wi + dp < lo) {
+            g = g * (lo < Wi ? (lo - Wi) / dp : 0.0);
+            *last = f, *upper = 0, *zero = !(lo < Wi);
+        }
+    }
+}
+
+/* finite_volume_1d.hpp:209-233 gradient_tangent(): d g of line_gradient() for the tangents of W[v] of the left neighbour, the
+ * cell and the right neighbour. A clipped slope is (bound - W_i) / r of its last clip; the bound is the neighbour fmax /
+ * fmin returned. */
+static double line_gradient_tangent(const orc_line* l, const line_cell* cell, double gapL, double gapR, int v, double dl, double di,
+                                    double dr) {
+    const line_cell* me = cell;
+    const double dL = -gapL + cell[-1].c - me->c, dR = gapR + cell[1].c - me->c;
+    const double Wi = me->W[v], left = cell[-1].W[v], right = cell[1].W[v];
+    if (l->limiter == 2 || l->limiter == 3) {
+        const double sL = (left - Wi) / dL, sR = (right - Wi) / dR, dsL = (dl - di) / dL, dsR = (dr - di) / dR;
+        if (!(sL * sR > 0)) return 0.0;
+        if (l->limiter == 3) {
+            const double sum = sL + sR, g = 2 * sL * sR / sum;
+            return (2 * (dsL * sR + sL * dsR) - g * (dsL + dsR)) / sum;
+        }
+        return fabs(sL) < fabs(sR) ? dsL : dsR;
+    }
+    const double dg = (dr - dl) / (dR - dL);
+    if (l->limiter == 1) return dg;
+    int last, upper, zero;
+    line_clips(cell, gapL, gapR, v, &last, &upper, &zero);
+    if (last < 0) return dg;
+    if (zero) return 0.0;
+    const double r[2] = {-0.5 * gapL - me->c, 0.5 * gapR - me->c};
+    const int from_left = upper ? !(right > left) : !(right < left);
+    return ((from_left ? dl : dr) - di) / r[last];
+}
+
+/* finite_volume_1d.hpp:255-290 gradient_adjoint(): adds J^T gbar to the bars of W[v] of the left neighbour, the cell and
+ * the right neighbour. */
+static void line_gradient_adjoint(const orc_line* l, const line_cell* cell, double gapL, double gapR, int v, double gbar, double* bl,
+                                  double* bi, double* br) {
+    const line_cell* me = cell;
+    const double dL = -gapL + cell[-1].c - me->c, dR = gapR + cell[1].c - me->c;
+    const double Wi = me->W[v], left = cell[-1].W[v], right = cell[1].W[v];
+    if (l->limiter == 2 || l->limiter == 3) {
+        const double sL = (left - Wi) / dL, sR = (right - Wi) / dR;
+        if (!(sL * sR > 0)) return;
+        double aL, aR; /* d g = aL dsL + aR dsR */
+        if (l->limiter == 3) {
+            const double sum = sL + sR, g = 2 * sL * sR / sum;
+            aL = (2 * sR - g) / sum, aR = (2 * sL - g) / sum;
+        } else {
+            const int pick_left = fabs(sL) < fabs(sR);
+            aL = pick_left ? 1.0 : 0.0, aR = pick_left ? 0.0 : 1.0;
+        }
+        *bl += gbar * aL / dL, *bi -= gbar * aL / dL;
+        *br += gbar * aR / dR, *bi -= gbar * aR / dR;
+        return;
+    }
+    int last = -1, upper = 0, zero = 0;
+    if (l->limiter != 1) line_clips(cell, gapL, gapR, v, &last, &upper, &zero);
+    if (last < 0) {
+        *br += gbar / (dR - dL), *bl -= gbar / (dR - dL);
+        return;
+    }
+    if (zero) return;
+    const double r[2] = {-0.5 * gapL - me->c, 0.5 * gapR - me->c};
+    const int from_left = upper ? !(right > left) : !(right < left);
+    if (from_left)
+        *bl += gbar / r[last];
+    else
+        *br += gbar / r[last];
+    *bi -= gbar / r[last];
+}
+
+/* finite_volume_1d.hpp:117-128 as a map of tangents (and its own adjoint): an outflow ghost copies the end cell, a
+ * reflective one mirrors its velocity. */
+static void line_ghost_tangent(const orc_line* l, const double* dW, int left, double* out) {
+    const int wall = left ? l->wall_left : l->wall_right;
+    for (int v = 0; v < 3; ++v) out[v] = dW[v];
+    if (wall == 0) out[1] = -out[1];
+}
+static void mirror3(const double* a, double* out) {
+    out[0] = a[0];
+    out[1] = -a[1];
+    out[2] = a[2];
+}
+
+static void line_slope(const orc_line* l, const line_cell cell[5], const double gap[4], int k, double* g) {
+    g[0] = g[1] = g[2] = 0;
+    if (l->order == 1 && cell[k].real)
+        for (int v = 0; v < 3; ++v) g[v] = line_gradient(l, cell + k, gap[k - 1], gap[k], v);
+}
+static void line_slope_tangent(const orc_line* l, const line_cell cell[5], const double gap[4], double dW[5][3], int k, double* dg) {
+    dg[0] = dg[1] = dg[2] = 0;
+    if (l->order == 1 && cell[k].real)
+        for (int v = 0; v < 3; ++v)
+            dg[v] = line_gradient_tangent(l, cell + k, gap[k - 1], gap[k], v, dW[k - 1][v], dW[k][v], dW[k + 1][v]);
+}
+
+/* finite_volume_1d.hpp:730-743 face_value(): line_face_state() with its fallback flag. */
+static void line_face_value(const orc_line* l, const double* W, const double* g, double r, double dt, double* out, int* fallback) {
+    *fallback = 0;
+    if (l->order == 0) {
+        for (int v = 0; v < 3; ++v) out[v] = W[v];
+        return;
+    }
+    orc_half_step(1, &l->gas, W, g, dt, out);
+    for (int v = 0; v < 3; ++v) out[v] += g[v] * r;
+    *fallback = !(out[0] > 0 && out[2] > 0);
+    if (*fallback)
+        for (int v = 0; v < 3; ++v) out[v] = W[v];
+}
+static void line_face_value_tangent(const orc_line* l, const double* W, const double* dW, const double* g, const double* dg, double r,
+                                    double dt, int fallback, double* out) {
+    if (l->order == 0 || fallback) {
+        for (int v = 0; v < 3; ++v) out[v] = dW[v];
+        return;
+    }
+    orc_half_step_tangent(1, &l->gas, W, dW, g, dg, dt, out);
+    for (int v = 0; v < 3; ++v) out[v] += dg[v] * r;
+}
+static void line_face_value_adjoint(const orc_line* l, const double* W, const double* g, double r, double dt, int fallback,
+                                    const double* Wfbar, double* Wbar, double* gbar) {
+    if (l->order == 0 || fallback) {
+        for (int v = 0; v < 3; ++v) Wbar[v] += Wfbar[v];
+        return;
+    }
+    double gb[3];
+    for (int v = 0; v < 3; ++v) gb[v] = Wfbar[v] * r;
+    orc_half_step_adjoint(1, &l->gas, W, g, dt, Wfbar, Wbar, gb);
+    for (int v = 0; v < 3; ++v) gbar[v] += gb[v];
+}
+
+/* finite_volume_1d.hpp:766-792 face_tangent(): the flux of the face between stencil positions a and a + 1, and its tangent. */
+static void line_face_tangent(const orc_line* l, const line_cell cell[5], const double gap[4], double dW[5][3], int a, double dt, double* F,
+                              double* dF) {
+    const int b = a + 1;
+    double ga[3], gb[3], dga[3], dgb[3];
+    line_slope(l, cell, gap, a, ga);
+    line_slope(l, cell, gap, b, gb);
+    line_slope_tangent(l, cell, gap, dW, a, dga);
+    line_slope_tangent(l, cell, gap, dW, b, dgb);
+    const double ra = 0.5 * gap[a] - cell[a].c, rb = -0.5 * gap[a] - cell[b].c;
+    double L[3] = {0, 0, 0}, R[3] = {0, 0, 0}, dLs[3] = {0, 0, 0}, dRs[3] = {0, 0, 0};
+    int fallback;
+    if (cell[a].real) {
+        line_face_value(l, cell[a].W, ga, ra, dt, L, &fallback);
+        line_face_value_tangent(l, cell[a].W, dW[a], ga, dga, ra, dt, fallback, dLs);
+    }
+    if (cell[b].real) {
+        line_face_value(l, cell[b].W, gb, rb, dt, R, &fallback);
+        line_face_value_tangent(l, cell[b].W, dW[b], gb, dgb, rb, dt, fallback, dRs);
+    }
+    if (!cell[a].real) {
+        line_ghost_face(l, R, cell[b].W, 1, L);
+        if (l->wall_left == 0)
+            mirror3(dRs, dLs);
+        else
+            line_ghost_tangent(l, dW[b], 1, dLs);
+    }
+    if (!cell[b].real) {
+        line_ghost_face(l, L, cell[a].W, 0, R);
+        if (l->wall_right == 0)
+            mirror3(dLs, dRs);
+        else
+            line_ghost_tangent(l, dW[a], 0, dRs);
+    }
+    const double one = 1.0, zero = 0.0;
+    orc_hllc_tangent(1, &l->gas, L, R, dLs, dRs, &one, &zero, F, dF);
+}
+
+/* finite_volume_1d.hpp:794-830 face_adjoint(): adds J^T Fbar of the face between positions a and a + 1 to the bars of the
+ * stencil's W (a ghost position's bar is mapped to its wall's end cell by the caller). */
+static void line_face_adjoint(const orc_line* l, const line_cell cell[5], const double gap[4], int a, double dt, const double* Fbar,
+                              double Wbar[5][3]) {
+    const int b = a + 1;
+    double ga[3], gb[3];
+    line_slope(l, cell, gap, a, ga);
+    line_slope(l, cell, gap, b, gb);
+    const double ra = 0.5 * gap[a] - cell[a].c, rb = -0.5 * gap[a] - cell[b].c;
+    double L[3] = {0, 0, 0}, R[3] = {0, 0, 0};
+    int fallback_a = 0, fallback_b = 0;
+    if (cell[a].real) line_face_value(l, cell[a].W, ga, ra, dt, L, &fallback_a);
+    if (cell[b].real) line_face_value(l, cell[b].W, gb, rb, dt, R, &fallback_b);
+    if (!cell[a].real) line_ghost_face(l, R, cell[b].W, 1, L);
+    if (!cell[b].real) line_ghost_face(l, L, cell[a].W, 0, R);
+    double Lbar[3] = {0, 0, 0}, Rbar[3] = {0, 0, 0};
+    const double one = 1.0, zero = 0.0;
+    orc_hllc_adjoint(1, &l->gas, L, R, &one, &zero, Fbar, Lbar, Rbar);
+    /* the ghost states in reverse order of the forward: R from L, then L from R */
+    if (!cell[b].real) {
+        double back[3];
+        if (l->wall_right == 0) {
+            mirror3(Rbar, back);
+            for (int v = 0; v < 3; ++v) Lbar[v] += back[v];
+        } else {
+            line_ghost_tangent(l, Rbar, 0, back);
+            for (int v = 0; v < 3; ++v) Wbar[a][v] += back[v];
+        }
+    }
+    if (!cell[a].real) {
+        double back[3];
+        if (l->wall_left == 0) {
+            mirror3(Lbar, back);
+            for (int v = 0; v < 3; ++v) Rbar[v] += back[v];
+        } else {
+            line_ghost_tangent(l, Lbar, 1, back);
+            for (int v = 0; v < 3; ++v) Wbar[b][v] += back[v];
+        }
+    }
+    double gabar[3] = {0, 0, 0}, gbbar[3] = {0, 0, 0};
+    if (cell[a].real) line_face_value_adjoint(l, cell[a].W, ga, ra, dt, fallback_a, Lbar, Wbar[a], gabar);
+    if (cell[b].real) line_face_value_adjoint(l, cell[b].W, gb, rb, dt, fallback_b, Rbar, Wbar[b], gbbar);
+    if (l->order == 1) {
+        if (cell[a].real)
+            for (int v = 0; v < 3; ++v)
+                line_gradient_adjoint(l, cell + a, gap[a - 1], gap[a], v, gabar[v], &Wbar[a - 1][v], &Wbar[a][v], &Wbar[a + 1][v]);
+        if (cell[b].real)
+            for (int v = 0; v < 3; ++v)
+                line_gradient_adjoint(l, cell + b, gap[b - 1], gap[b], v, gbbar[v], &Wbar[b - 1][v], &Wbar[b][v], &Wbar[b + 1][v]);
+    }
+}
+
+/* finite_volume_1d.hpp:831-848 load_tangent(): the W tangents of slot s's stencil from dq (ghosts through the ghost map). */
+static void line_load_tangent(const orc_line* l, const double* dq, int s, double dW[5][3]) {
+    int slot[5];
+    for (int k = 0; k < 5; ++k) {
+        slot[k] = line_slot(l, s, k - 2);
+        dW[k][0] = dW[k][1] = dW[k][2] = 0;
+        if (slot[k] < 0) continue;
+        const int i = slot[k];
+        const double inv = 1.0 / orc_line_volume(l, i);
+        const double U[3] = {l->q[i * 3] * inv, l->q[i * 3 + 1] * inv, l->q[i * 3 + 2] * inv};
+        const double dU[3] = {dq[i * 3] * inv, dq[i * 3 + 1] * inv, dq[i * 3 + 2] * inv};
+        orc_primitive_tangent(1, &l->gas, U, dU, dW[k]);
+    }
+    for (int k = 0; k < 5; ++k) {
+        if (slot[k] >= 0) continue;
+        const int end = k < 2 ? k + 1 : k - 1;
+        if (slot[end] >= 0) line_ghost_tangent(l, dW[end], k < 2, dW[k]);
+    }
+}
+
+/* finite_volume_1d.hpp:850-880 step_tangent(): the step (order 0 or 1, its walls) and dq -> J dq, dt fixed; q and dq are
+ * advanced in place (dq: a tangent of the extensive state). */
+void orc_line_step_tangent(orc_line* l, double dt, double* dq) {
+    double* dnext = (double*)malloc(sizeof(double) * 3 * (size_t)l->n);
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < l->n; ++s) {
+        line_cell cell[5];
+        double gap[4], dW[5][3], F0[3], F1[3], dF0[3], dF1[3];
+        line_load(l, s, cell, gap);
+        line_load_tangent(l, dq, s, dW);
+        line_face_tangent(l, cell, gap, dW, 1, dt, F0, dF0);
+        line_face_tangent(l, cell, gap, dW, 2, dt, F1, dF1);
+        for (int v = 0; v < 3; ++v) {
+            l->next[s * 3 + v] = l->q[s * 3 + v] - dt * (F1[v] - F0[v]);
+            dnext[s * 3 + v] = dq[s * 3 + v] - dt * (dF1[v] - dF0[v]);
+        }
+    }
+    memcpy(dq, dnext, sizeof(double) * 3 * (size_t)l->n);
+    free(dnext);
+    line_swap(l);
+}
+
+/* finite_volume_1d.hpp:882-933 step_adjoint(): qbar (after the step) -> J^T qbar (before it), in place; q is the state at the
+ * start of the step and stays. Per slot j: the adjoints of the faces whose fluxes read W_j (the left faces of slots
+ * j - 1 .. j + 2, and the right wall's face), then the primitive adjoint. */
+void orc_line_step_adjoint(orc_line* l, double dt, double* qbar) {
+    const int n = l->n;
+    double* out = (double*)malloc(sizeof(double) * 3 * (size_t)n);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < n; ++j) {
+        double Wj_bar[3] = {0, 0, 0};
+        for (int d = -1; d <= 3; ++d) {
+            int s, a;
+            if (d <= 2) {
+                s = line_slot(l, j, d);
+                a = 1;
+            } else {
+                if (l->periodic || j < n - 2) continue;
+                s = n - 1;
+                a = 2; /* the right wall's face */
+            }
+            if (s < 0) continue;
+            line_cell cell[5];
+            double gap[4];
+            line_load(l, s, cell, gap);
+            const int left = line_slot(l, s, a - 2), right = line_slot(l, s, a - 1);
+            double Fbar[3];
+            for (int v = 0; v < 3; ++v)
+                Fbar[v] = (right >= 0 ? dt * qbar[right * 3 + v] : 0.0) - (left >= 0 ? dt * qbar[left * 3 + v] : 0.0);
+            double Wbar[5][3];
+            memset(Wbar, 0, sizeof Wbar);
+            line_face_adjoint(l, cell, gap, a, dt, Fbar, Wbar);
+            for (int k = 0; k < 5; ++k) {
+                const int t = line_slot(l, s, k - 2);
+                if (t == j) {
+                    for (int v = 0; v < 3; ++v) Wj_bar[v] += Wbar[k][v];
+                } else if (t < 0) {
+                    const int end = k < 2 ? k + 1 : k - 1;
+                    if (line_slot(l, s, end - 2) == j) {
+                        double back[3];
+                        line_ghost_tangent(l, Wbar[k], k < 2, back);
+                        for (int v = 0; v < 3; ++v) Wj_bar[v] += back[v];
+                    }
+                }
+            }
+        }
+        const double inv = 1.0 / orc_line_volume(l, j);
+        const double U[3] = {l->q[j * 3] * inv, l->q[j * 3 + 1] * inv, l->q[j * 3 + 2] * inv};
+        double Ubar[3];
+        orc_primitive_adjoint(1, &l->gas, U, Wj_bar, Ubar);
+        for (int v = 0; v < 3; ++v) out[j * 3 + v] = qbar[j * 3 + v] + Ubar[v] * inv;
+    }
+    memcpy(qbar, out, sizeof(double) * 3 * (size_t)n);
+    free(out);
+}

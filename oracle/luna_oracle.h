@@ -94,6 +94,20 @@ double orc_line_time_step(orc_line* line, double courant, int luna_rule);
 void orc_line_step(orc_line* line, double dt);
 double orc_line_step_fused(orc_line* line, double dt, double courant);
 double orc_line_volume(const orc_line* line, int s);
+/* Derivatives of a static step, dt fixed (finite_volume_1d.hpp:710-935, euler.hpp:115-832): the tangent advances q and dq
+ * (J dq) in place; the adjoint maps qbar after the step to J^T qbar before it, with q the state before the step. */
+void orc_line_step_tangent(orc_line* line, double dt, double* dq);
+void orc_line_step_adjoint(orc_line* line, double dt, double* qbar);
+void orc_primitive_tangent(int D, const orc_gas* gas, const double* U, const double* dU, double* dW);
+void orc_primitive_adjoint(int D, const orc_gas* gas, const double* U, const double* Wbar, double* Ubar);
+void orc_half_step_tangent(int D, const orc_gas* gas, const double* W, const double* dW, const double* g, const double* dg, double dt,
+                           double* out);
+void orc_half_step_adjoint(int D, const orc_gas* gas, const double* W, const double* g, double dt, const double* hbar, double* Wbar,
+                           double* gbar);
+void orc_hllc_tangent(int D, const orc_gas* gas, const double* L, const double* R, const double* dL, const double* dR, const double* n,
+                      const double* w, double* F, double* dF);
+void orc_hllc_adjoint(int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, const double* Fbar,
+                      double* Lbar, double* Rbar);
 double orc_line_centroid(const orc_line* line, int s);
 
 #ifdef __cplusplus

@@ -66,9 +66,10 @@ def _ptr(a, kind=_dp):
     return a.ctypes.data_as(kind) if a is not None else None
 
 
-def gas(gamma):
+def gas(gamma, smallr=1e-10, smallc=1e-10):
     g = Gas()
     lib().orc_gas_init(ctypes.byref(g), ctypes.c_double(gamma))
+    g.smallr, g.smallc = smallr, smallc
     return g
 
 
@@ -99,6 +100,66 @@ def flux(L, R, n, w, gamma, riemann="hllc"):
     fn = lib().orc_hll if riemann == "hll" else lib().orc_hllc
     fn(len(L) - 2, ctypes.byref(g), _ptr(L), _ptr(R), _ptr(n), _ptr(w), _ptr(F))
     return F
+
+
+def primitive_tangent(U, dU, gamma, smallr=1e-10, smallc=1e-10):
+    U, dU = _vec(U), _vec(dU)
+    dW = np.empty_like(U)
+    g = gas(gamma, smallr, smallc)
+    lib().orc_primitive_tangent(len(U) - 2, ctypes.byref(g), _ptr(U), _ptr(dU), _ptr(dW))
+    return dW
+
+
+def primitive_adjoint(U, Wbar, gamma, smallr=1e-10, smallc=1e-10):
+    U, Wbar = _vec(U), _vec(Wbar)
+    Ubar = np.empty_like(U)
+    g = gas(gamma, smallr, smallc)
+    lib().orc_primitive_adjoint(len(U) - 2, ctypes.byref(g), _ptr(U), _ptr(Wbar), _ptr(Ubar))
+    return Ubar
+
+
+def half_step(W, grad, dt, gamma):
+    """W: [V]; grad: [V][D]."""
+    W, grad = _vec(W), _vec(grad)
+    out = np.empty_like(W)
+    g = gas(gamma)
+    lib().orc_half_step(len(W) - 2, ctypes.byref(g), _ptr(W), _ptr(grad), ctypes.c_double(dt), _ptr(out))
+    return out
+
+
+def half_step_tangent(W, dW, grad, dgrad, dt, gamma):
+    W, dW, grad, dgrad = _vec(W), _vec(dW), _vec(grad), _vec(dgrad)
+    out = np.empty_like(W)
+    g = gas(gamma)
+    lib().orc_half_step_tangent(len(W) - 2, ctypes.byref(g), _ptr(W), _ptr(dW), _ptr(grad), _ptr(dgrad), ctypes.c_double(dt), _ptr(out))
+    return out
+
+
+def half_step_adjoint(W, grad, dt, hbar, gamma):
+    """Returns (Wbar, gbar) = J^T hbar."""
+    W, grad, hbar = _vec(W), _vec(grad), _vec(hbar)
+    Wbar, gbar = np.zeros_like(W), np.zeros_like(grad)
+    g = gas(gamma)
+    lib().orc_half_step_adjoint(len(W) - 2, ctypes.byref(g), _ptr(W), _ptr(grad), ctypes.c_double(dt), _ptr(hbar), _ptr(Wbar), _ptr(gbar))
+    return Wbar, gbar
+
+
+def hllc_tangent(L, R, dL, dR, n, w, gamma):
+    """Returns (F, dF): the HLLC flux from the tangent's own intermediates and J (dL, dR)."""
+    L, R, dL, dR, n, w = _vec(L), _vec(R), _vec(dL), _vec(dR), _vec(n), _vec(w)
+    F, dF = np.empty_like(L), np.empty_like(L)
+    g = gas(gamma)
+    lib().orc_hllc_tangent(len(L) - 2, ctypes.byref(g), _ptr(L), _ptr(R), _ptr(dL), _ptr(dR), _ptr(n), _ptr(w), _ptr(F), _ptr(dF))
+    return F, dF
+
+
+def hllc_adjoint(L, R, n, w, Fbar, gamma):
+    """Returns (Lbar, Rbar) = J^T Fbar."""
+    L, R, n, w, Fbar = _vec(L), _vec(R), _vec(n), _vec(w), _vec(Fbar)
+    Lbar, Rbar = np.zeros_like(L), np.zeros_like(R)
+    g = gas(gamma)
+    lib().orc_hllc_adjoint(len(L) - 2, ctypes.byref(g), _ptr(L), _ptr(R), _ptr(n), _ptr(w), _ptr(Fbar), _ptr(Lbar), _ptr(Rbar))
+    return Lbar, Rbar
 
 
 def snap(x, step, period):
@@ -229,6 +290,20 @@ class Line:
 
     def step(self, dt):
         lib().orc_line_step(ctypes.byref(self.c), ctypes.c_double(dt))
+
+    def step_tangent(self, dt, dq):
+        """One step of q and of its tangent dq (extensive, N x 3): returns J dq; q advances."""
+        dq = np.ascontiguousarray(np.array(dq, dtype=np.float64))
+        self._sync()
+        lib().orc_line_step_tangent(ctypes.byref(self.c), ctypes.c_double(dt), _ptr(dq))
+        return dq
+
+    def step_adjoint(self, dt, qbar):
+        """J^T qbar of one step from the current q (which stays): qbar after the step -> before it."""
+        qbar = np.ascontiguousarray(np.array(qbar, dtype=np.float64))
+        self._sync()
+        lib().orc_line_step_adjoint(ctypes.byref(self.c), ctypes.c_double(dt), _ptr(qbar))
+        return qbar
 
     def step_and_time_step(self, dt, courant):
         return lib().orc_line_step_fused(ctypes.byref(self.c), ctypes.c_double(dt), ctypes.c_double(courant))
