@@ -1122,9 +1122,8 @@ void orc_hllc_adjoint(int D, const orc_gas* gas, const double* L, const double* 
     const int left = (a.pattern >> 2) & 1, star = (a.pattern >> 3) & 1;
     double Fbar[MAXV];
     to_lab_adjoint(D, Fbar_lab, w, Fbar);
-    double W[MAXV];
+    double W[MAXV], Wbar[MAXV] = {0, 0, 0, 0, 0}; /* the upwind side's bar, added to its side's at the end */
     for (int v = 0; v < D + 2; ++v) W[v] = left ? L[v] : R[v];
-    double* Wbar = left ? Lbar : Rbar;
     const double vn = left ? a.vnL : a.vnR;
     side_t s;
     side_with(D, gas, W, n, w, vn, &s);
@@ -1240,6 +1239,12 @@ void orc_hllc_adjoint(int D, const orc_gas* gas, const double* L, const double* 
         }
         Lbar[0] -= inv_rhoLbar * a.inv_rhoL * a.inv_rhoL;
         Rbar[0] -= inv_rhoRbar * a.inv_rhoR * a.inv_rhoR;
+    }
+    for (int v = 0; v < D + 2; ++v) {
+        if (left)
+            Lbar[v] += Wbar[v];
+        else
+            Rbar[v] += Wbar[v];
     }
     /* vn = (v - w) . n */
     for (int k = 0; k < D; ++k) {
@@ -1582,4 +1587,410 @@ void orc_line_step_adjoint(orc_line* l, double dt, double* qbar) {
     }
     memcpy(qbar, out, sizeof(double) * 3 * (size_t)n);
     free(out);
+}
+
+/* ---- D = 2, 3: the static order-1 step (finite_volume.hpp:1670-2231) ---- */
+
+/* Cholesky of M (upper triangle) and the solve with its factor, in Sweep::solve's operation order (:590-615). */
+static void cholesky(int D, double M[MAXD + 1][MAXD + 1], double L[MAXD + 1][MAXD + 1]) {
+    const int U = D + 1;
+    memset(L, 0, sizeof(double) * (MAXD + 1) * (MAXD + 1));
+    for (int p = 0; p < U; ++p)
+        for (int s = 0; s <= p; ++s) {
+            double sum = M[s][p];
+            for (int m = 0; m < s; ++m) sum -= L[p][m] * L[s][m];
+            L[p][s] = p == s ? sqrt(sum) : sum / L[s][s];
+        }
+}
+static void solve_factored(int D, double L[MAXD + 1][MAXD + 1], double b[MAXV][MAXD + 1], double x[MAXV][MAXD + 1]) {
+    const int U = D + 1, V = D + 2;
+    for (int v = 0; v < V; ++v) {
+        double z[MAXD + 1];
+        for (int p = 0; p < U; ++p) {
+            double sum = b[v][p];
+            for (int m = 0; m < p; ++m) sum -= L[p][m] * z[m];
+            z[p] = sum / L[p][p];
+        }
+        for (int p = U - 1; p >= 0; --p) {
+            double sum = z[p];
+            for (int m = p + 1; m < U; ++m) sum -= L[m][p] * x[v][m];
+            x[v][p] = sum / L[p][p];
+        }
+    }
+}
+
+/* finite_volume.hpp:709-799 limited_core(): what the limiter did per variable. last: the list position (from the cell's
+ * first entry) of the last clipping face (-1: none); extreme: the face whose neighbour holds the bound (the first entry in
+ * list order with the extreme value); offset: the last clipping face's offset. */
+typedef struct {
+    int last[MAXV], upper[MAXV], zero[MAXV], extreme[MAXV];
+    double offset[MAXV][MAXD], bound[MAXV], dbound[MAXV];
+} limit_pattern;
+
+/* The cell's fit and limiter as sweep_cell() computes them (the same bits for g), with what the derivatives need: the
+ * unlimited gradient g0 and, with dW (N x V tangents of the prims), its tangent dg0; M's Cholesky factor and first row M0
+ * (M0[p] = sum_k |A_k| X_k[p]); the limiter's pattern. */
+static void fit_cell(const orc_fv* fv, const orc_geometry* geo, int i, const double* dW, double* g, double* g0, double* dg0,
+                     double L[MAXD + 1][MAXD + 1], double* M0, limit_pattern* pat) {
+    const int D = fv->D, V = D + 2, U = D + 1, PW = V + 2;
+    const double* Wi = fv->prims + (size_t)i * PW;
+    double surface = 0, smallest = INFINITY;
+    double lo[MAXV], hi[MAXV], dlo[MAXV], dhi[MAXV];
+    int lo_f[MAXV], hi_f[MAXV];
+    for (int v = 0; v < V; ++v) {
+        lo[v] = DBL_MAX;
+        hi[v] = -DBL_MAX;
+        dlo[v] = dhi[v] = 0;
+        lo_f[v] = hi_f[v] = -1;
+    }
+    double M[MAXD + 1][MAXD + 1], b[MAXV][MAXD + 1], db[MAXV][MAXD + 1];
+    memset(M, 0, sizeof M);
+    memset(b, 0, sizeof b);
+    memset(db, 0, sizeof db);
+    const double* centroid = geo->centroid + (size_t)i * D;
+    const int first = geo->start[i], end = geo->start[i + 1];
+    for (int k = first; k < end; ++k) {
+        const int e = geo->list[k], f = list_face(e);
+        const int j = e >= 0 ? geo->neighbour[f] : geo->owner[f];
+        const double* Wj = fv->prims + (size_t)j * PW;
+        double d[MAXD], X[MAXD + 1];
+        separation(geo, i, j, d);
+        const double w = magnitude_of(geo, f);
+        surface += w;
+        smallest = fmin(smallest, w);
+        X[0] = 1;
+        for (int p = 1; p < U; ++p) X[p] = d[p - 1] + geo->centroid[(size_t)j * D + p - 1] - centroid[p - 1];
+        for (int p = 0; p < U; ++p)
+            for (int s = p; s < U; ++s) M[p][s] += w * X[p] * X[s];
+        for (int v = 0; v < V; ++v) {
+            const double dWv = Wj[v] - Wi[v];
+            for (int p = 0; p < U; ++p) b[v][p] += w * X[p] * dWv;
+            if (dW) {
+                const double ddW = w * (dW[(size_t)j * V + v] - dW[(size_t)i * V + v]);
+                for (int p = 0; p < U; ++p) db[v][p] += X[p] * ddW;
+            }
+            if (Wj[v] < lo[v]) lo[v] = Wj[v], lo_f[v] = k - first, dlo[v] = dW ? dW[(size_t)j * V + v] : 0.0;
+            if (Wj[v] > hi[v]) hi[v] = Wj[v], hi_f[v] = k - first, dhi[v] = dW ? dW[(size_t)j * V + v] : 0.0;
+        }
+    }
+    const double tiny = 1e-6 * surface;
+    const int any_tiny = smallest < tiny;
+    if (any_tiny) {
+        for (int v = 0; v < V; ++v) {
+            lo[v] = DBL_MAX;
+            hi[v] = -DBL_MAX;
+            lo_f[v] = hi_f[v] = -1;
+        }
+        for (int k = first; k < end; ++k) {
+            const int e = geo->list[k], f = list_face(e);
+            const int j = e >= 0 ? geo->neighbour[f] : geo->owner[f];
+            if (magnitude_of(geo, f) < tiny) continue;
+            const double* Wj = fv->prims + (size_t)j * PW;
+            for (int v = 0; v < V; ++v) {
+                if (Wj[v] < lo[v]) lo[v] = Wj[v], lo_f[v] = k - first, dlo[v] = dW ? dW[(size_t)j * V + v] : 0.0;
+                if (Wj[v] > hi[v]) hi[v] = Wj[v], hi_f[v] = k - first, dhi[v] = dW ? dW[(size_t)j * V + v] : 0.0;
+            }
+        }
+    }
+    cholesky(D, M, L);
+    for (int p = 0; p < U; ++p) M0[p] = M[0][p];
+    double xs[MAXV][MAXD + 1];
+    solve_factored(D, L, b, xs);
+    for (int v = 0; v < V; ++v)
+        for (int a = 0; a < D; ++a) g0[v * D + a] = g[v * D + a] = xs[v][1 + a];
+    if (dW) {
+        solve_factored(D, L, db, xs);
+        for (int v = 0; v < V; ++v)
+            for (int a = 0; a < D; ++a) dg0[v * D + a] = xs[v][1 + a];
+    }
+    for (int v = 0; v < V; ++v) {
+        pat->last[v] = pat->extreme[v] = -1;
+        pat->upper[v] = pat->zero[v] = 0;
+    }
+    if (fv->limiter == 0)
+        for (int k = first; k < end; ++k) {
+            const int f = list_face(geo->list[k]);
+            if (any_tiny && magnitude_of(geo, f) < tiny) continue;
+            double offset[MAXD];
+            face_offset(geo, f, i, offset);
+            for (int v = 0; v < V; ++v) {
+                double* gv = g + v * D;
+                const double dp = dot(D, gv, offset);
+                if (dp > 0 && Wi[v] + dp > hi[v]) {
+                    const double s = hi[v] > Wi[v] ? (hi[v] - Wi[v]) / dp : 0.0;
+                    for (int a = 0; a < D; ++a) gv[a] = gv[a] * s;
+                    pat->last[v] = k - first, pat->upper[v] = 1, pat->zero[v] = !(hi[v] > Wi[v]);
+                    memcpy(pat->offset[v], offset, sizeof(double) * D);
+                } else if (dp < 0 && Wi[v] + dp < lo[v]) {
+                    const double s = lo[v] < Wi[v] ? (lo[v] - Wi[v]) / dp : 0.0;
+                    for (int a = 0; a < D; ++a) gv[a] = gv[a] * s;
+                    pat->last[v] = k - first, pat->upper[v] = 0, pat->zero[v] = !(lo[v] < Wi[v]);
+                    memcpy(pat->offset[v], offset, sizeof(double) * D);
+                }
+            }
+        }
+    for (int v = 0; v < V; ++v) {
+        pat->extreme[v] = pat->last[v] < 0 ? -1 : pat->upper[v] ? hi_f[v] : lo_f[v];
+        pat->bound[v] = pat->upper[v] ? hi[v] : lo[v];
+        pat->dbound[v] = pat->upper[v] ? dhi[v] : dlo[v];
+    }
+}
+
+/* finite_volume.hpp:809-831 limited_gradient_tangent(): after the last clip g = g0 (bound - W_i) / (g0 . o), whatever
+ * clips came before (each only rescales g). */
+static void limited_tangent(int D, const double* Wi, const double* dWi, const double* g0, const double* dg0, const limit_pattern* pat,
+                            double* dg) {
+    const int V = D + 2;
+    for (int v = 0; v < V; ++v) {
+        const double* gv = g0 + v * D;
+        const double* dgv = dg0 + v * D;
+        if (pat->last[v] < 0) {
+            for (int a = 0; a < D; ++a) dg[v * D + a] = dgv[a];
+        } else if (pat->zero[v]) {
+            for (int a = 0; a < D; ++a) dg[v * D + a] = 0.0;
+        } else {
+            const double* o = pat->offset[v];
+            const double den = dot(D, gv, o);
+            const double s = (pat->bound[v] - Wi[v]) / den, ds = (pat->dbound[v] - dWi[v] - s * dot(D, dgv, o)) / den;
+            for (int a = 0; a < D; ++a) dg[v * D + a] = dgv[a] * s + gv[a] * ds;
+        }
+    }
+}
+
+/* One face's two sides as the static flux computes them (face_flux, pass -1), with their fallback flags and offsets. */
+static void face_sides(const orc_fv* fv, const orc_geometry* geo, int f, double dt, double* L, double* R, int* fallback, double* ro,
+                       double* rj, double* n, double* magnitude) {
+    const int D = fv->D, V = D + 2, PW = V + 2;
+    const int o = geo->owner[f], j = geo->neighbour[f];
+    const double* a = geo->area + (size_t)f * D;
+    *magnitude = sqrt(dot(D, a, a));
+    const double inv = 1.0 / *magnitude;
+    for (int k = 0; k < D; ++k) n[k] = a[k] * inv;
+    face_offset(geo, f, o, ro);
+    face_offset(geo, f, j, rj);
+    const double half = 0.5 * dt, wf = 0.0;
+    for (int k = 0; k < D; ++k) {
+        ro[k] = ro[k] + wf * half;
+        rj[k] = rj[k] + wf * half;
+    }
+    const int cells[2] = {o, j};
+    const double* rs[2] = {ro, rj};
+    double* out[2] = {L, R};
+    for (int s = 0; s < 2; ++s) {
+        const double* W = fv->prims + (size_t)cells[s] * PW;
+        const double* g = fv->grad + (size_t)cells[s] * V * D;
+        orc_half_step(D, &fv->gas, W, g, dt, out[s]);
+        for (int v = 0; v < V; ++v) out[s][v] += dot(D, g + v * D, rs[s]);
+        fallback[s] = !(out[s][0] > 0 && out[s][D + 1] > 0);
+        if (fallback[s])
+            for (int v = 0; v < V; ++v) out[s][v] = W[v];
+    }
+}
+
+static int static_derivatives_ok(const orc_fv* fv) { return !fv->moving && fv->order == 1 && fv->riemann == 0 && fv->D >= 2; }
+
+/* finite_volume.hpp:2039-2074 step_tangent(): the static step of q and dq -> J dq (dq: N x V, extensive), dt fixed.
+ * Returns 0, or -1 outside a static order-1 HLLC mesh step. */
+int orc_step_tangent(orc_fv* fv, const orc_geometry* geo, double dt, double* dq) {
+    if (!static_derivatives_ok(fv)) return -1;
+    const int D = fv->D, V = D + 2, N = fv->N, F = geo->F, PW = V + 2;
+    orc_primitives(fv, geo);
+    double* dW = (double*)malloc(sizeof(double) * (size_t)N * V);
+    double* dg = (double*)malloc(sizeof(double) * (size_t)N * V * D);
+    double* dflux = (double*)malloc(sizeof(double) * (size_t)F * V);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        const double inv = 1.0 / geo->volume[i];
+        double U[MAXV], dU[MAXV];
+        for (int v = 0; v < V; ++v) {
+            U[v] = fv->q[(size_t)i * V + v] * inv;
+            dU[v] = dq[(size_t)i * V + v] * inv;
+        }
+        orc_primitive_tangent(D, &fv->gas, U, dU, dW + (size_t)i * V);
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        double g0[MAXV * MAXD], dg0[MAXV * MAXD], L[MAXD + 1][MAXD + 1], M0[MAXD + 1];
+        limit_pattern pat;
+        fit_cell(fv, geo, i, dW, fv->grad + (size_t)i * V * D, g0, dg0, L, M0, &pat);
+        limited_tangent(D, fv->prims + (size_t)i * PW, dW + (size_t)i * V, g0, dg0, &pat, dg + (size_t)i * V * D);
+    }
+#pragma omp parallel for schedule(static)
+    for (int f = 0; f < F; ++f) {
+        double L[MAXV], R[MAXV], ro[MAXD], rj[MAXD], n[MAXD], magnitude, dLs[MAXV], dRs[MAXV];
+        int fallback[2];
+        double* Fv = fv->flux + (size_t)f * V;
+        double* dF = dflux + (size_t)f * V;
+        for (int v = 0; v < V; ++v) Fv[v] = dF[v] = 0;
+        const double* a = geo->area + (size_t)f * D;
+        if (sqrt(dot(D, a, a)) == 0) continue;
+        face_sides(fv, geo, f, dt, L, R, fallback, ro, rj, n, &magnitude);
+        const int cells[2] = {geo->owner[f], geo->neighbour[f]};
+        const double* rs[2] = {ro, rj};
+        double* outs[2] = {dLs, dRs};
+        for (int s = 0; s < 2; ++s) {
+            const int i = cells[s];
+            if (fallback[s]) {
+                for (int v = 0; v < V; ++v) outs[s][v] = dW[(size_t)i * V + v];
+                continue;
+            }
+            const double* gi = fv->grad + (size_t)i * V * D;
+            const double* dgi = dg + (size_t)i * V * D;
+            orc_half_step_tangent(D, &fv->gas, fv->prims + (size_t)i * PW, dW + (size_t)i * V, gi, dgi, dt, outs[s]);
+            for (int v = 0; v < V; ++v) outs[s][v] += dot(D, dgi + v * D, rs[s]);
+        }
+        const double w0[MAXD] = {0, 0, 0};
+        orc_hllc_tangent(D, &fv->gas, L, R, dLs, dRs, n, w0, Fv, dF);
+        for (int v = 0; v < V; ++v) {
+            Fv[v] *= magnitude;
+            dF[v] *= magnitude;
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        double sum[MAXV] = {0, 0, 0, 0, 0}, dsum[MAXV] = {0, 0, 0, 0, 0};
+        for (int k = geo->start[i]; k < geo->start[i + 1]; ++k) {
+            const int e = geo->list[k], f = list_face(e);
+            const double sign = e >= 0 ? 1.0 : -1.0;
+            for (int v = 0; v < V; ++v) {
+                sum[v] += sign * fv->flux[(size_t)f * V + v];
+                dsum[v] += sign * dflux[(size_t)f * V + v];
+            }
+        }
+        for (int v = 0; v < V; ++v) {
+            fv->q[(size_t)i * V + v] = fv->q[(size_t)i * V + v] - dt * sum[v];
+            dq[(size_t)i * V + v] = dq[(size_t)i * V + v] - dt * dsum[v];
+        }
+    }
+    free(dW);
+    free(dg);
+    free(dflux);
+    fv->prims_ready = 0;
+    return 0;
+}
+
+/* finite_volume.hpp:2201-2231 step_adjoint() (AdjointFaces :2078-2102, AdjointCell :2107-2169, AdjointGather :2171-2195):
+ * qbar after the static step -> J^T qbar before it, in place; q (the state before the step) stays. Per face: the
+ * Riemann adjoint of the flux's bar; per cell, in list order: its sides' adjoints (half step and g.r) into W bar and g bar,
+ * then the limited fit's adjoint (the bound's bar and b bar, which the neighbours collect); per cell, in list order: the
+ * neighbours' shares, then the primitive adjoint. Deterministic, no atomics. Returns 0, or -1 as orc_step_tangent. */
+int orc_step_adjoint(orc_fv* fv, const orc_geometry* geo, double dt, double* qbar) {
+    if (!static_derivatives_ok(fv)) return -1;
+    const int D = fv->D, V = D + 2, U = D + 1, N = fv->N, F = geo->F, PW = V + 2;
+    orc_primitives(fv, geo);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) sweep_cell(fv, geo, i);
+    double* sidebar = (double*)malloc(sizeof(double) * (size_t)F * 2 * V); /* L bar, R bar per face */
+    double* Wbar = (double*)malloc(sizeof(double) * (size_t)N * V);
+    double* bbar = (double*)malloc(sizeof(double) * (size_t)N * V * U);
+    double* boundbar = (double*)malloc(sizeof(double) * (size_t)N * V);
+    int* extreme = (int*)malloc(sizeof(int) * (size_t)N * V); /* face id of the bound's entry, -1 none */
+#pragma omp parallel for schedule(static)
+    for (int f = 0; f < F; ++f) {
+        double* Lbar = sidebar + (size_t)f * 2 * V;
+        double* Rbar = Lbar + V;
+        for (int v = 0; v < 2 * V; ++v) Lbar[v] = 0;
+        const double* a = geo->area + (size_t)f * D;
+        if (sqrt(dot(D, a, a)) == 0) continue;
+        double L[MAXV], R[MAXV], ro[MAXD], rj[MAXD], n[MAXD], magnitude, Fbar[MAXV];
+        int fallback[2];
+        face_sides(fv, geo, f, dt, L, R, fallback, ro, rj, n, &magnitude);
+        const int o = geo->owner[f], j = geo->neighbour[f];
+        for (int v = 0; v < V; ++v) Fbar[v] = (dt * qbar[(size_t)j * V + v] - dt * qbar[(size_t)o * V + v]) * magnitude;
+        const double w0[MAXD] = {0, 0, 0};
+        orc_hllc_adjoint(D, &fv->gas, L, R, n, w0, Fbar, Lbar, Rbar);
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        const double* Wi = fv->prims + (size_t)i * PW;
+        const double* gi = fv->grad + (size_t)i * V * D;
+        double Wb[MAXV] = {0, 0, 0, 0, 0}, gb[MAXV * MAXD];
+        memset(gb, 0, sizeof gb);
+        for (int k = geo->start[i]; k < geo->start[i + 1]; ++k) {
+            const int e = geo->list[k], f = list_face(e);
+            const double* a = geo->area + (size_t)f * D;
+            if (sqrt(dot(D, a, a)) == 0) continue;
+            double L[MAXV], R[MAXV], ro[MAXD], rj[MAXD], n[MAXD], magnitude;
+            int fallback[2];
+            face_sides(fv, geo, f, dt, L, R, fallback, ro, rj, n, &magnitude);
+            const int s = e >= 0 ? 0 : 1;
+            const double* side = sidebar + (size_t)f * 2 * V + s * V;
+            const double* r = s == 0 ? ro : rj;
+            if (fallback[s]) {
+                for (int v = 0; v < V; ++v) Wb[v] += side[v];
+                continue;
+            }
+            double hb[MAXV * MAXD];
+            for (int v = 0; v < V; ++v)
+                for (int c = 0; c < D; ++c) hb[v * D + c] = side[v] * r[c];
+            orc_half_step_adjoint(D, &fv->gas, Wi, gi, dt, side, Wb, hb);
+            for (int v = 0; v < V * D; ++v) gb[v] += hb[v];
+        }
+        /* the limited fit (finite_volume.hpp:832-864 limited_gradient_adjoint) */
+        double g[MAXV * MAXD], g0[MAXV * MAXD], Lc[MAXD + 1][MAXD + 1], M0[MAXD + 1];
+        limit_pattern pat;
+        fit_cell(fv, geo, i, NULL, g, g0, NULL, Lc, M0, &pat);
+        double xbar[MAXV][MAXD + 1], bb[MAXV][MAXD + 1];
+        for (int v = 0; v < V; ++v) {
+            double g0bar[MAXD];
+            for (int c = 0; c < D; ++c) g0bar[c] = gb[v * D + c];
+            boundbar[(size_t)i * V + v] = 0;
+            extreme[(size_t)i * V + v] = -1;
+            if (pat.last[v] >= 0 && pat.zero[v]) {
+                for (int c = 0; c < D; ++c) g0bar[c] = 0;
+            } else if (pat.last[v] >= 0) {
+                const double* o = pat.offset[v];
+                const double den = dot(D, g0 + v * D, o);
+                const double sc = (pat.bound[v] - Wi[v]) / den, t = dot(D, gb + v * D, g0 + v * D) / den;
+                for (int c = 0; c < D; ++c) g0bar[c] = gb[v * D + c] * sc - o[c] * (sc * t);
+                boundbar[(size_t)i * V + v] = t;
+                extreme[(size_t)i * V + v] = list_face(geo->list[geo->start[i] + pat.extreme[v]]);
+                Wb[v] -= t;
+            }
+            xbar[v][0] = 0;
+            for (int c = 0; c < D; ++c) xbar[v][1 + c] = g0bar[c];
+        }
+        solve_factored(D, Lc, xbar, bb); /* M is symmetric: b bar = M^-1 x bar */
+        for (int v = 0; v < V; ++v) {
+            for (int p = 0; p < U; ++p) {
+                bbar[((size_t)i * V + v) * U + p] = bb[v][p];
+                Wb[v] -= M0[p] * bb[v][p];
+            }
+            Wbar[(size_t)i * V + v] = Wb[v];
+        }
+    }
+    double* out = (double*)malloc(sizeof(double) * (size_t)N * V);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < N; ++j) {
+        double Wb[MAXV];
+        for (int v = 0; v < V; ++v) Wb[v] = Wbar[(size_t)j * V + v];
+        for (int k = geo->start[j]; k < geo->start[j + 1]; ++k) {
+            const int e = geo->list[k], f = list_face(e);
+            const int i = e >= 0 ? geo->neighbour[f] : geo->owner[f]; /* the neighbour whose fit read W_j */
+            double d[MAXD], X[MAXD + 1];
+            separation(geo, i, j, d);
+            const double w = magnitude_of(geo, f);
+            X[0] = 1;
+            for (int p = 1; p < U; ++p) X[p] = d[p - 1] + geo->centroid[(size_t)j * D + p - 1] - geo->centroid[(size_t)i * D + p - 1];
+            for (int v = 0; v < V; ++v) {
+                double share = 0;
+                for (int p = 0; p < U; ++p) share += w * X[p] * bbar[((size_t)i * V + v) * U + p];
+                if (extreme[(size_t)i * V + v] == f) share += boundbar[(size_t)i * V + v];
+                Wb[v] += share;
+            }
+        }
+        const double inv = 1.0 / geo->volume[j];
+        double Uj[MAXV], Ub[MAXV];
+        for (int v = 0; v < V; ++v) Uj[v] = fv->q[(size_t)j * V + v] * inv;
+        orc_primitive_adjoint(D, &fv->gas, Uj, Wb, Ub);
+        for (int v = 0; v < V; ++v) out[(size_t)j * V + v] = qbar[(size_t)j * V + v] + Ub[v] * inv;
+    }
+    memcpy(qbar, out, sizeof(double) * (size_t)N * V);
+    free(out);
+    free(sidebar);
+    free(Wbar);
+    free(bbar);
+    free(boundbar);
+    free(extreme);
+    return 0;
 }

@@ -75,6 +75,11 @@ double orc_time_step(orc_fv* fv, const orc_geometry* geo, double courant);
 void orc_step_static(orc_fv* fv, const orc_geometry* geo, double dt);
 void orc_moving_begin(orc_fv* fv, const orc_geometry* geo, double dt);
 void orc_moving_finish(orc_fv* fv, const orc_geometry* geo, double dt);
+/* Derivatives of the static order-1 HLLC step at fixed dt (finite_volume.hpp:1670-2231): the tangent advances q and dq (J dq)
+ * in place; the adjoint maps qbar after the step to J^T qbar before it (q, the state before the step, stays). Return -1
+ * for a moving mesh, order 0, HLL or D = 1 (see orc_line_*). */
+int orc_step_tangent(orc_fv* fv, const orc_geometry* geo, double dt, double* dq);
+int orc_step_adjoint(orc_fv* fv, const orc_geometry* geo, double dt, double* qbar);
 long long orc_snap(double x, double step, long long period);
 int orc_delta(long long k0, long long k1, long long period);
 

@@ -229,6 +229,20 @@ class FiniteVolume:
     def step(self, dt):
         lib().orc_step_static(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(dt))
 
+    def step_tangent(self, dt, dq):
+        """The static order-1 HLLC step of q and of its tangent dq (N x V, extensive): returns J dq; q advances."""
+        dq = np.ascontiguousarray(np.array(dq, dtype=np.float64))
+        if lib().orc_step_tangent(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(dt), _ptr(dq)) != 0:
+            raise ValueError("step_tangent: a static mesh, order 1, HLLC")
+        return dq
+
+    def step_adjoint(self, dt, qbar):
+        """J^T qbar of the static order-1 HLLC step from the current q (which stays)."""
+        qbar = np.ascontiguousarray(np.array(qbar, dtype=np.float64))
+        if lib().orc_step_adjoint(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(dt), _ptr(qbar)) != 0:
+            raise ValueError("step_adjoint: a static mesh, order 1, HLLC")
+        return qbar
+
     def moving_begin(self, dt):
         lib().orc_moving_begin(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(dt))
         return self.x.copy()

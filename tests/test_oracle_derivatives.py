@@ -144,3 +144,99 @@ def test_hllc_derivatives_fd_and_transpose(case):
     Fbar = rng_unit(rng, 5)
     Lbar, Rbar = oracle.hllc_adjoint(L, R, n, w, Fbar, gamma)
     assert abs(Fbar @ dF - (Lbar @ dL + Rbar @ dR)) < 1e-12 * (1 + abs(Fbar @ dF))
+
+
+@pytest.mark.parametrize("walls,periodic,limiter,order", [
+    (("reflective", "reflective"), False, "face", 1),
+    (("outflow", "reflective"), False, "vanleer", 1),
+    (("outflow", "outflow"), True, "minmod", 1),
+    (("reflective", "outflow"), False, "face", 0),
+    (("outflow", "outflow"), False, "none", 1),
+])
+def test_line_step_derivatives_dot_and_fd(walls, periodic, limiter, order):
+    """A rough jittered line with each wall kind: the step's adjoint is the transpose of its tangent (dot test), and the
+    tangent matches central differences of the step."""
+    rng = np.random.default_rng(5)
+    n = 200
+    keys, step, period = problems.line_keys(problems.line_points(n, 1.0, 0.3, 2), 1.0)
+    line = oracle.Line(keys, period, step, 1.4, periodic=periodic, walls=walls, limiter=limiter, order=order)
+    V = np.array([line.volume(s) for s in range(n)])
+    W = np.stack([1 + 0.3 * rng.uniform(size=n), 0.4 * rng.uniform(-1, 1, n), 1 + 0.3 * rng.uniform(size=n)], axis=1)
+    q = np.array([oracle.conserved(w, 1.4) for w in W]) * V[:, None]
+    dt, h = 2e-4, 1e-7
+    dq, qbar = rng.uniform(-1, 1, (n, 3)), rng.uniform(-1, 1, (n, 3))
+    line.set_state(q)
+    J = line.step_tangent(dt, dq)
+    line.set_state(q)
+    T = line.step_adjoint(dt, qbar)
+    lhs, rhs = np.sum(qbar * J), np.sum(T * dq)
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+    line.set_state(q + h * dq)
+    line.step(dt)
+    plus = line.state.copy()
+    line.set_state(q - h * dq)
+    line.step(dt)
+    fd = (plus - line.state) / (2 * h)
+    assert np.max(np.abs(fd - J)) / np.max(np.abs(J)) < 1e-6
+
+
+def mesh_state(dim, n, seed=1, amp=0.3):
+    import cases
+
+    m = cases.voronoi(n, dim, 0.25, 3)
+    rng = np.random.default_rng(seed)
+    N, V = m["cells"], dim + 2
+    W = np.zeros((N, V))
+    W[:, 0] = 1 + amp * rng.uniform(size=N)
+    W[:, 1:1 + dim] = amp * rng.uniform(-1, 1, (N, dim))
+    W[:, -1] = 1 + amp * rng.uniform(size=N)
+    q = np.array([oracle.conserved(w, 5.0 / 3.0) for w in W]) * m["volume"][:, None]
+    return m, q, rng
+
+
+def forward(fv, q, dt):
+    fv.set_state(q)
+    fv.time_step(0.3)  # the sweep the step reuses
+    fv.step(dt)
+    return fv.q.copy()
+
+
+@pytest.mark.parametrize("dim,n,limiter", [(3, 6, "face"), (3, 6, "none"), (2, 12, "face"), (2, 12, "none")])
+def test_mesh_step_derivatives_dot_fd_conservation(dim, n, limiter):
+    """The static order-1 step on a rough jittered Voronoi mesh: the tangent's own forward is the step (bits), the adjoint is
+    its transpose (dot test), the tangent matches central differences to 1e-6 relative, and J^T 1 = 1 (the step conserves
+    every sum of Q)."""
+    m, q, rng = mesh_state(dim, n)
+    N, V = q.shape
+    fv = oracle.FiniteVolume(m, 5.0 / 3.0, limiter=limiter)
+    fv.set_state(q)
+    dt = 0.3 * fv.time_step(0.3)
+    dq, qbar = rng.uniform(-1, 1, (N, V)), rng.uniform(-1, 1, (N, V))
+    fv.set_state(q)
+    J = fv.step_tangent(dt, dq)
+    assert np.array_equal(fv.q, forward(fv, q, dt))
+    fv.set_state(q)
+    T = fv.step_adjoint(dt, qbar)
+    lhs, rhs = np.sum(qbar * J), np.sum(T * dq)
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+    h = 1e-7
+    fd = (forward(fv, q + h * dq, dt) - forward(fv, q - h * dq, dt)) / (2 * h)
+    assert np.max(np.abs(fd - J)) / np.max(np.abs(J)) < 1e-6
+    fv.set_state(q)
+    assert np.array_equal(fv.step_adjoint(dt, np.ones((N, V))), np.ones((N, V)))
+
+
+def test_planted_adjoint_cell_swap_fails_dot_test():
+    """Teeth: the adjoint read back with two cells' rows swapped must fail the dot test."""
+    m, q, rng = mesh_state(3, 6)
+    N, V = q.shape
+    fv = oracle.FiniteVolume(m, 5.0 / 3.0)
+    fv.set_state(q)
+    dt = 0.3 * fv.time_step(0.3)
+    dq, qbar = rng.uniform(-1, 1, (N, V)), rng.uniform(-1, 1, (N, V))
+    fv.set_state(q)
+    J = fv.step_tangent(dt, dq)
+    fv.set_state(q)
+    T = fv.step_adjoint(dt, qbar)
+    T[[3, 17]] = T[[17, 3]]
+    assert abs(np.sum(qbar * J) - np.sum(T * dq)) > 1e-6
