@@ -143,6 +143,9 @@ struct luna_fv {
     Buf<double> cell, facerec;
     Buf<double> q, grad, w, x, old_centroid, flux;
     Buf<unsigned long long> counters;  // [0, 1] dt slots, [2, 3] fallbacks of the step's two flux passes
+    // the derivatives' rows (first use): luna::Derivs
+    Buf<double> d_dq, d_dW, d_dgrad, d_dflux, d_sidebar, d_bbar, d_boundbar, d_out;
+    Buf<int> d_extreme;
     unsigned long long* host = nullptr;  // pinned: read-backs
     double side[3] = {0, 0, 0};
     luna::Grid grid{};
@@ -382,6 +385,7 @@ int ensure_swept(luna_fv* fv) {
 }
 
 int flux_and_update(luna_fv* fv, int pass, double flux_dt, double apply_dt, double unit, bool next_prims) {
+    const luna::Grid* drift = pass == 0 ? &fv->grid : nullptr;
     const luna::Geo g = fv->geo();
     const luna::Rows r = fv->rows();
     const luna::Gas gas = fv->gas();
@@ -393,8 +397,8 @@ int flux_and_update(luna_fv* fv, int pass, double flux_dt, double apply_dt, doub
                           : luna::launch_flux<2>(s, g, r, gas, riemann, order, pass, flux_dt, unit, count);
     }));
     LUNA_HIP(timed(fv->prof, s, pass < 0 ? "update" : pass == 0 ? "update_pass1" : "update_pass2", [&] {
-        return fv->D == 3 ? luna::launch_update<3>(s, g, r, gas, apply_dt, next_prims)
-                          : luna::launch_update<2>(s, g, r, gas, apply_dt, next_prims);
+        return fv->D == 3 ? luna::launch_update<3>(s, g, r, gas, apply_dt, next_prims, drift)
+                          : luna::launch_update<2>(s, g, r, gas, apply_dt, next_prims, drift);
     }));
     return LUNA_OK;
 }
@@ -466,7 +470,8 @@ int luna_fv_destroy(luna_fv* fv) {
     if (fv->stream) (void)hipStreamSynchronize(fv->stream);
     fv->prof.destroy();
     for (auto* b : {&fv->cell, &fv->facerec, &fv->q, &fv->grad, &fv->w, &fv->x, &fv->old_centroid, &fv->flux}) b->release();
-    for (auto* b : {&fv->count, &fv->slice, &fv->list, &fv->other, &fv->steps}) b->release();
+    for (auto* b : {&fv->count, &fv->slice, &fv->list, &fv->other, &fv->steps, &fv->d_extreme}) b->release();
+    for (auto* b : {&fv->d_dq, &fv->d_dW, &fv->d_dgrad, &fv->d_dflux, &fv->d_sidebar, &fv->d_bbar, &fv->d_boundbar, &fv->d_out}) b->release();
     for (void* p : {static_cast<void*>(fv->stage_cell), static_cast<void*>(fv->stage_face), static_cast<void*>(fv->stage_list),
                     static_cast<void*>(fv->stage_other), static_cast<void*>(fv->stage_count)})
         if (p) (void)hipHostFree(p);
@@ -528,6 +533,76 @@ int luna_fv_step(luna_fv* fv, double dt) {
     return LUNA_OK;
 }
 
+namespace {
+
+int derivatives_ready(luna_fv* fv, const char* what) {
+    if (fv->opt.motion != LUNA_MOTION_STATIC || fv->opt.order != 1 || fv->opt.riemann != LUNA_RIEMANN_HLLC)
+        return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of a static order-1 HLLC step");
+    if (fv->begun) return fail(LUNA_ERR_LOGIC, std::string(what) + ": a moving step is between begin and finish");
+    const size_t N = fv->N, V = fv->V, F = fv->F;
+    LUNA_HIP(fv->d_dq.reserve(N * V));
+    LUNA_HIP(fv->d_dW.reserve(N * V));
+    LUNA_HIP(fv->d_dgrad.reserve(N * fv->grad_row()));
+    LUNA_HIP(fv->d_dflux.reserve(F * V > 0 ? F * V : 1));
+    LUNA_HIP(fv->d_sidebar.reserve(2 * F * V > 0 ? 2 * F * V : 1));
+    LUNA_HIP(fv->d_bbar.reserve(N * V * (fv->D + 1)));
+    LUNA_HIP(fv->d_boundbar.reserve(N * V));
+    LUNA_HIP(fv->d_extreme.reserve(N * V));
+    LUNA_HIP(fv->d_out.reserve(N * V));
+    return LUNA_OK;
+}
+
+luna::Derivs derivs_of(luna_fv* fv) {
+    return luna::Derivs{fv->d_dq.p, fv->d_dW.p, fv->d_dgrad.p, fv->d_dflux.p, fv->d_sidebar.p, fv->d_bbar.p, fv->d_boundbar.p,
+                        fv->d_extreme.p, fv->d_out.p};
+}
+
+}  // namespace
+
+int luna_fv_step_tangent(luna_fv* fv, double dt, double* dq, int device) {
+    if (!fv || !dq) return fail(LUNA_ERR_ARGUMENT, "luna_fv_step_tangent: null argument");
+    if (int rc = derivatives_ready(fv, "luna_fv_step_tangent")) return rc;
+    hipStream_t s = fv->stream;
+    const size_t bytes = sizeof(double) * fv->N * fv->V;
+    LUNA_HIP(hipMemcpyAsync(fv->d_dq.p, dq, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    const luna::Geo g = fv->geo();
+    const luna::Rows r = fv->rows();
+    const luna::Gas gas = fv->gas();
+    const luna::Derivs w = derivs_of(fv);
+    const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID;
+    LUNA_HIP(timed(fv->prof, s, "step_tangent", [&] {
+        return fv->D == 3 ? luna::launch_step_tangent<3>(s, g, r, gas, dt, limit, w) : luna::launch_step_tangent<2>(s, g, r, gas, dt, limit, w);
+    }));
+    LUNA_HIP(hipMemcpyAsync(dq, fv->d_dq.p, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipStreamSynchronize(s));
+    fv->prims_ready = false;
+    fv->swept = false;
+    return LUNA_OK;
+}
+
+int luna_fv_step_adjoint(luna_fv* fv, double dt, double* qbar, int device) {
+    if (!fv || !qbar) return fail(LUNA_ERR_ARGUMENT, "luna_fv_step_adjoint: null argument");
+    if (int rc = derivatives_ready(fv, "luna_fv_step_adjoint")) return rc;
+    hipStream_t s = fv->stream;
+    const size_t bytes = sizeof(double) * fv->N * fv->V;
+    LUNA_HIP(hipMemcpyAsync(fv->d_dq.p, qbar, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    if (int rc = primitives(fv)) return rc;  // the forward's rows at Q: primitives and the limited gradients
+    if (int rc = sweep(fv, 1.0)) return rc;
+    const luna::Geo g = fv->geo();
+    const luna::Rows r = fv->rows();
+    const luna::Gas gas = fv->gas();
+    const luna::Derivs w = derivs_of(fv);
+    const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID;
+    LUNA_HIP(timed(fv->prof, s, "step_adjoint", [&] {
+        return fv->D == 3 ? luna::launch_step_adjoint<3>(s, g, r, gas, dt, limit, w) : luna::launch_step_adjoint<2>(s, g, r, gas, dt, limit, w);
+    }));
+    LUNA_HIP(hipMemcpyAsync(qbar, fv->d_out.p, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipStreamSynchronize(s));
+    fv->prims_ready = false;
+    fv->swept = false;
+    return LUNA_OK;
+}
+
 int luna_fv_moving_begin(luna_fv* fv, double dt, double* positions) {
     if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_begin: null argument");
     if (fv->opt.motion != LUNA_MOTION_MOVING) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_begin: a static mesh steps with luna_fv_step");
@@ -543,10 +618,7 @@ int luna_fv_moving_begin(luna_fv* fv, double dt, double* positions) {
         return fv->D == 3 ? luna::launch_keep_centroids<3>(s, g, r, grid, dt) : luna::launch_keep_centroids<2>(s, g, r, grid, dt);
     }));
     const double unit = fv->grid.step / dt;
-    if (int rc = flux_and_update(fv, 0, 0.0, 0.5 * dt, unit, false)) return rc;
-    LUNA_HIP(timed(fv->prof, s, "drift", [&] {
-        return fv->D == 3 ? luna::launch_drift<3>(s, fv->N, r, grid) : luna::launch_drift<2>(s, fv->N, r, grid);
-    }));
+    if (int rc = flux_and_update(fv, 0, 0.0, 0.5 * dt, unit, false)) return rc;  // the update also drifts x
     if (positions) LUNA_HIP(hipMemcpyAsync(positions, fv->x.p, sizeof(double) * fv->N * fv->D, hipMemcpyDeviceToHost, s));
     LUNA_HIP(hipStreamSynchronize(s));
     fv->begun = true;
@@ -675,7 +747,7 @@ struct luna_fv1 {
     double step = 0;
     hipStream_t stream = nullptr;
     Buf<long long> keys;
-    Buf<double> q, next;
+    Buf<double> q, next, dq, dnext;  // dq, dnext: the derivatives' rows (first use)
     Buf<unsigned long long> counters;
     unsigned long long* host = nullptr;
     int parity = 0;
@@ -752,6 +824,8 @@ int luna_fv1_destroy(luna_fv1* fv) {
     fv->keys.release();
     fv->q.release();
     fv->next.release();
+    fv->dq.release();
+    fv->dnext.release();
     fv->counters.release();
     if (fv->host) (void)hipHostFree(fv->host);
     if (fv->stream) (void)hipStreamDestroy(fv->stream);
@@ -788,6 +862,33 @@ int luna_fv1_step(luna_fv1* fv, double dt) {
     if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_step: null argument");
     LUNA_HIP(luna::launch_line_step(fv->stream, fv->args(), dt, 0.0, false, fv->counters_now()));
     std::swap(fv->q, fv->next);
+    return LUNA_OK;
+}
+
+int luna_fv1_step_tangent(luna_fv1* fv, double dt, double* dq) {
+    if (!fv || !dq) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_step_tangent: null argument");
+    if (fv->opt.riemann != LUNA_RIEMANN_HLLC) return fail(LUNA_ERR_LOGIC, "luna_fv1_step_tangent: derivatives of the HLLC step");
+    const size_t bytes = sizeof(double) * 3 * fv->n;
+    LUNA_HIP(fv->dq.reserve(3 * static_cast<size_t>(fv->n)));
+    LUNA_HIP(fv->dnext.reserve(3 * static_cast<size_t>(fv->n)));
+    LUNA_HIP(hipMemcpyAsync(fv->dq.p, dq, bytes, hipMemcpyHostToDevice, fv->stream));
+    LUNA_HIP(luna::launch_line_tangent(fv->stream, fv->args(), dt, fv->dq.p, fv->dnext.p));
+    std::swap(fv->q, fv->next);
+    LUNA_HIP(hipMemcpyAsync(dq, fv->dnext.p, bytes, hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    return LUNA_OK;
+}
+
+int luna_fv1_step_adjoint(luna_fv1* fv, double dt, double* qbar) {
+    if (!fv || !qbar) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_step_adjoint: null argument");
+    if (fv->opt.riemann != LUNA_RIEMANN_HLLC) return fail(LUNA_ERR_LOGIC, "luna_fv1_step_adjoint: derivatives of the HLLC step");
+    const size_t bytes = sizeof(double) * 3 * fv->n;
+    LUNA_HIP(fv->dq.reserve(3 * static_cast<size_t>(fv->n)));
+    LUNA_HIP(fv->dnext.reserve(3 * static_cast<size_t>(fv->n)));
+    LUNA_HIP(hipMemcpyAsync(fv->dq.p, qbar, bytes, hipMemcpyHostToDevice, fv->stream));
+    LUNA_HIP(luna::launch_line_adjoint(fv->stream, fv->args(), dt, fv->dq.p, fv->dnext.p));
+    LUNA_HIP(hipMemcpyAsync(qbar, fv->dnext.p, bytes, hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
     return LUNA_OK;
 }
 

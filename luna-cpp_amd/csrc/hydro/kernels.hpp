@@ -52,7 +52,7 @@ struct Rows {
     double* grad;          // N x grad_row<D>()
     double* w;             // N x D point velocities (moving)
     double* x;             // N x D solver positions (moving)
-    double* old_centroid;  // N x D centroids of t^n (moving)
+    double* old_centroid;  // N x D centroids of t^n (moving; written by the sweep)
     int* steps;            // N x D whole grid steps over the step (moving)
     double* flux;          // F x V, face order
 };
@@ -78,12 +78,37 @@ hipError_t launch_sweep(hipStream_t s, const Geo& g, const Rows& r, const Gas& g
 template <int D>
 hipError_t launch_flux(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, int order, int pass, double dt, double unit,
                        unsigned long long* fallbacks);
+// drift: a moving step's first pass, which also drifts the positions on the grid (finite_volume.hpp:1219-1231).
 template <int D>
-hipError_t launch_update(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double dt, bool next_prims);
+hipError_t launch_update(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double dt, bool next_prims, const Grid* drift);
 template <int D>
 hipError_t launch_keep_centroids(hipStream_t s, const Geo& g, const Rows& r, const Grid& grid, double dt);
+
+// Derivatives of the static order-1 HLLC step at fixed dt (adjoint_kernels.hip). Device rows:
+//   dq       N x V   in: the tangent of Q (tangent) or Q bar after the step (adjoint); the tangent advances it to J dq
+//   dW       N x V   W tangents (tangent) or W bars (adjoint)
+//   dgrad    N x grad_row<D>()   gradient tangents
+//   dflux    F x V   flux-row tangents
+//   sidebar  F x 2V  the bars of each face's two sides
+//   bbar     N x V (D + 1), boundbar N x V, extreme N x V: each cell's fit adjoint for its neighbours
+//   out      N x V   J^T Q bar (adjoint)
+struct Derivs {
+    double* dq;
+    double* dW;
+    double* dgrad;
+    double* dflux;
+    double* sidebar;
+    double* bbar;
+    double* boundbar;
+    int* extreme;
+    double* out;
+};
+// The tangent writes Q' into r.q, its flux rows into r.flux and the limited gradients into r.grad; the adjoint needs the
+// primitive rows and the limited gradients of Q (the sweep) and leaves Q as it is.
 template <int D>
-hipError_t launch_drift(hipStream_t s, int N, const Rows& r, const Grid& grid);
+hipError_t launch_step_tangent(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double dt, bool limit, const Derivs& w);
+template <int D>
+hipError_t launch_step_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double dt, bool limit, const Derivs& w);
 
 // D = 1: the static sorted line.
 struct LineArgs {
@@ -97,5 +122,9 @@ struct LineArgs {
 };
 hipError_t launch_line_time_step(hipStream_t s, const LineArgs& a, double courant, int luna_rule, const Counters& c);
 hipError_t launch_line_step(hipStream_t s, const LineArgs& a, double dt, double courant, bool fused, const Counters& c);
+// Derivatives of the static step at fixed dt (HLLC): the tangent writes the step into a.next and J dq into dnext; the
+// adjoint writes J^T qbar (q: the state before the step) into out.
+hipError_t launch_line_tangent(hipStream_t s, const LineArgs& a, double dt, const double* dq, double* dnext);
+hipError_t launch_line_adjoint(hipStream_t s, const LineArgs& a, double dt, const double* qbar, double* out);
 
 }  // namespace luna

@@ -1,6 +1,7 @@
 // The static 1D step on a sorted line (finite_volume_1d.hpp), one work-item per slot: the five-cell stencil in
 // registers, three limited gradients, two face states, two Riemann fluxes and the new state in one kernel; fused, the
 // next time step by LUNA's rule from the new state as a block min-reduction. FP64, -ffp-contract=off.
+#include "derivatives.hpp"
 #include "kernels.hpp"
 
 namespace luna {
@@ -241,6 +242,360 @@ __global__ __launch_bounds__(BLOCK) void k_line_time_step(LineArgs l, double cou
     block_min(smallest, dt_out);
 }
 
+
+// ---- derivatives of the static step (finite_volume_1d.hpp:710-935), dt fixed: the face-by-face chain rule of k_line_step.
+
+// finite_volume_1d.hpp:234-253 clips(): the last face that clipped variable v under the face-centroid limiter (-1: none),
+// to which bound, and whether the clip zeroed the slope.
+__device__ __forceinline__ void clips(const Cell* cell, double gapL, double gapR, int v, int& last, bool& upper, bool& zero) {
+    const Cell& me = cell[0];
+    const double dL = -gapL + cell[-1].c - me.c, dR = gapR + cell[1].c - me.c;
+    const double Wi = me.W[v], left = cell[-1].W[v], right = cell[1].W[v];
+    double g = ((right - Wi) - (left - Wi)) / (dR - dL);
+    const double r[2] = {-0.5 * gapL - me.c, 0.5 * gapR - me.c};
+    const double lo = fmin(left, right), hi = fmax(left, right);
+    last = -1;
+    upper = zero = false;
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+        const double dp = g * r[f];
+        if (dp > 0 && Wi + dp > hi) {
+            g = g * (hi > Wi ? (hi - Wi) / dp : 0.0);
+            last = f, upper = true, zero = !(hi > Wi);
+        } else if (dp < 0 && Wi + dp < lo) {
+            g = g * (lo < Wi ? (lo - Wi) / dp : 0.0);
+            last = f, upper = false, zero = !(lo < Wi);
+        }
+    }
+}
+
+// finite_volume_1d.hpp:209-233: d g for the tangents of W[v] of the left neighbour, the cell and the right neighbour; a
+// clipped slope is (bound - W_i) / r of its last clip, the bound from the neighbour fmax / fmin returned.
+__device__ __forceinline__ double gradient_tangent(const LineArgs& l, const Cell* cell, double gapL, double gapR, int v, double dl, double di,
+                                                   double dr) {
+    const Cell& me = cell[0];
+    const double dL = -gapL + cell[-1].c - me.c, dR = gapR + cell[1].c - me.c;
+    const double Wi = me.W[v], left = cell[-1].W[v], right = cell[1].W[v];
+    if (l.limiter == 2 || l.limiter == 3) {
+        const double sL = (left - Wi) / dL, sR = (right - Wi) / dR, dsL = (dl - di) / dL, dsR = (dr - di) / dR;
+        if (!(sL * sR > 0)) return 0.0;
+        if (l.limiter == 3) {
+            const double sum = sL + sR, g = 2 * sL * sR / sum;
+            return (2 * (dsL * sR + sL * dsR) - g * (dsL + dsR)) / sum;
+        }
+        return fabs(sL) < fabs(sR) ? dsL : dsR;
+    }
+    const double dg = (dr - dl) / (dR - dL);
+    if (l.limiter == 1) return dg;
+    int last;
+    bool upper, zero;
+    clips(cell, gapL, gapR, v, last, upper, zero);
+    if (last < 0) return dg;
+    if (zero) return 0.0;
+    const double r = last == 0 ? -0.5 * gapL - me.c : 0.5 * gapR - me.c;
+    const bool from_left = upper ? !(right > left) : !(right < left);
+    return ((from_left ? dl : dr) - di) / r;
+}
+
+// finite_volume_1d.hpp:255-290: adds J^T gbar of gradient() to the bars of W[v] of the left neighbour, the cell and the
+// right neighbour.
+__device__ __forceinline__ void gradient_adjoint(const LineArgs& l, const Cell* cell, double gapL, double gapR, int v, double gbar, double& bl,
+                                                 double& bi, double& br) {
+    const Cell& me = cell[0];
+    const double dL = -gapL + cell[-1].c - me.c, dR = gapR + cell[1].c - me.c;
+    const double Wi = me.W[v], left = cell[-1].W[v], right = cell[1].W[v];
+    if (l.limiter == 2 || l.limiter == 3) {
+        const double sL = (left - Wi) / dL, sR = (right - Wi) / dR;
+        if (!(sL * sR > 0)) return;
+        double aL, aR;  // d g = aL dsL + aR dsR
+        if (l.limiter == 3) {
+            const double sum = sL + sR, g = 2 * sL * sR / sum;
+            aL = (2 * sR - g) / sum, aR = (2 * sL - g) / sum;
+        } else {
+            const bool pick_left = fabs(sL) < fabs(sR);
+            aL = pick_left ? 1.0 : 0.0, aR = pick_left ? 0.0 : 1.0;
+        }
+        bl += gbar * aL / dL, bi -= gbar * aL / dL;
+        br += gbar * aR / dR, bi -= gbar * aR / dR;
+        return;
+    }
+    int last = -1;
+    bool upper = false, zero = false;
+    if (l.limiter != 1) clips(cell, gapL, gapR, v, last, upper, zero);
+    if (last < 0) {
+        br += gbar / (dR - dL), bl -= gbar / (dR - dL);
+        return;
+    }
+    if (zero) return;
+    const double r = last == 0 ? -0.5 * gapL - me.c : 0.5 * gapR - me.c;
+    const bool from_left = upper ? !(right > left) : !(right < left);
+    if (from_left)
+        bl += gbar / r;
+    else
+        br += gbar / r;
+    bi -= gbar / r;
+}
+
+// ghost() as a map of tangents (its own adjoint): outflow copies the end cell, reflective mirrors its velocity.
+__device__ __forceinline__ void ghost_tangent(const LineArgs& l, const double* dW, bool left, double* out) {
+    const int wall = left ? l.wall_left : l.wall_right;
+    out[0] = dW[0];
+    out[1] = wall == 0 ? -dW[1] : dW[1];
+    out[2] = dW[2];
+}
+__device__ __forceinline__ void mirror(const double* a, double* out) {
+    out[0] = a[0];
+    out[1] = -a[1];
+    out[2] = a[2];
+}
+
+__device__ __forceinline__ void slope(const LineArgs& l, const Cell* cell, const double* gap, int k, double* g) {
+    g[0] = g[1] = g[2] = 0;
+    if (l.order == 1 && cell[k].real)
+#pragma unroll
+        for (int v = 0; v < 3; ++v) g[v] = gradient(l, cell + k, gap[k - 1], gap[k], v);
+}
+__device__ __forceinline__ void slope_tangent(const LineArgs& l, const Cell* cell, const double* gap, const double (*dW)[3], int k, double* dg) {
+    dg[0] = dg[1] = dg[2] = 0;
+    if (l.order == 1 && cell[k].real)
+#pragma unroll
+        for (int v = 0; v < 3; ++v) dg[v] = gradient_tangent(l, cell + k, gap[k - 1], gap[k], v, dW[k - 1][v], dW[k][v], dW[k + 1][v]);
+}
+
+// finite_volume_1d.hpp:730-743 face_value(): face_state() with its fallback flag.
+__device__ __forceinline__ void face_value(const LineArgs& l, const double* W, const double* g, double r, double dt, double* out, bool& fallback) {
+    fallback = false;
+    if (l.order == 0) {
+        out[0] = W[0];
+        out[1] = W[1];
+        out[2] = W[2];
+        return;
+    }
+    half_step<1>(l.gas, W, g, dt, out);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) out[v] += g[v] * r;
+    fallback = !(out[0] > 0 && out[2] > 0);
+    if (fallback) {
+        out[0] = W[0];
+        out[1] = W[1];
+        out[2] = W[2];
+    }
+}
+__device__ __forceinline__ void face_value_tangent(const LineArgs& l, const double* W, const double* dW, const double* g, const double* dg,
+                                                   double r, double dt, bool fallback, double* out) {
+    if (l.order == 0 || fallback) {
+        out[0] = dW[0];
+        out[1] = dW[1];
+        out[2] = dW[2];
+        return;
+    }
+    half_step_tangent<1>(l.gas, W, dW, g, dg, dt, out);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) out[v] += dg[v] * r;
+}
+__device__ __forceinline__ void face_value_adjoint(const LineArgs& l, const double* W, const double* g, double r, double dt, bool fallback,
+                                                   const double* Wfbar, double* Wbar, double* gbar) {
+    if (l.order == 0 || fallback) {
+#pragma unroll
+        for (int v = 0; v < 3; ++v) Wbar[v] += Wfbar[v];
+        return;
+    }
+    double gb[3];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) gb[v] = Wfbar[v] * r;
+    half_step_adjoint<1>(l.gas, W, g, dt, Wfbar, Wbar, gb);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) gbar[v] += gb[v];
+}
+
+// finite_volume_1d.hpp:766-792 face_tangent(): the flux of the face between stencil positions a and a + 1 and its tangent.
+__device__ __forceinline__ void face_tangent(const LineArgs& l, const Cell* cell, const double* gap, const double (*dW)[3], int a, double dt,
+                                             double* F, double* dF) {
+    const int b = a + 1;
+    double ga[3], gb[3], dga[3], dgb[3];
+    slope(l, cell, gap, a, ga);
+    slope(l, cell, gap, b, gb);
+    slope_tangent(l, cell, gap, dW, a, dga);
+    slope_tangent(l, cell, gap, dW, b, dgb);
+    const double ra = 0.5 * gap[a] - cell[a].c, rb = -0.5 * gap[a] - cell[b].c;
+    double L[3] = {0, 0, 0}, R[3] = {0, 0, 0}, dLs[3] = {0, 0, 0}, dRs[3] = {0, 0, 0};
+    bool fallback;
+    if (cell[a].real) {
+        face_value(l, cell[a].W, ga, ra, dt, L, fallback);
+        face_value_tangent(l, cell[a].W, dW[a], ga, dga, ra, dt, fallback, dLs);
+    }
+    if (cell[b].real) {
+        face_value(l, cell[b].W, gb, rb, dt, R, fallback);
+        face_value_tangent(l, cell[b].W, dW[b], gb, dgb, rb, dt, fallback, dRs);
+    }
+    if (!cell[a].real) {
+        ghost_face(l, R, cell[b].W, true, L);
+        if (l.wall_left == 0)
+            mirror(dRs, dLs);
+        else
+            ghost_tangent(l, dW[b], true, dLs);
+    }
+    if (!cell[b].real) {
+        ghost_face(l, L, cell[a].W, false, R);
+        if (l.wall_right == 0)
+            mirror(dLs, dRs);
+        else
+            ghost_tangent(l, dW[a], false, dRs);
+    }
+    const double one[1] = {1.0}, zero[1] = {0.0};
+    hllc_tangent<1>(l.gas, L, R, dLs, dRs, one, zero, F, dF);
+}
+
+// finite_volume_1d.hpp:794-830 face_adjoint(): adds J^T Fbar of the face between positions a and a + 1 to the bars of the
+// stencil's W (a ghost position's bar is mapped to its wall's end cell by the caller).
+__device__ __forceinline__ void face_adjoint(const LineArgs& l, const Cell* cell, const double* gap, int a, double dt, const double* Fbar,
+                                             double (*Wbar)[3]) {
+    const int b = a + 1;
+    double ga[3], gb[3];
+    slope(l, cell, gap, a, ga);
+    slope(l, cell, gap, b, gb);
+    const double ra = 0.5 * gap[a] - cell[a].c, rb = -0.5 * gap[a] - cell[b].c;
+    double L[3] = {0, 0, 0}, R[3] = {0, 0, 0};
+    bool fallback_a = false, fallback_b = false;
+    if (cell[a].real) face_value(l, cell[a].W, ga, ra, dt, L, fallback_a);
+    if (cell[b].real) face_value(l, cell[b].W, gb, rb, dt, R, fallback_b);
+    if (!cell[a].real) ghost_face(l, R, cell[b].W, true, L);
+    if (!cell[b].real) ghost_face(l, L, cell[a].W, false, R);
+    double Lbar[3] = {0, 0, 0}, Rbar[3] = {0, 0, 0};
+    const double one[1] = {1.0}, zero[1] = {0.0};
+    hllc_adjoint<1>(l.gas, L, R, one, zero, Fbar, Lbar, Rbar);
+    // the ghost states in reverse order of the forward: R from L, then L from R
+    if (!cell[b].real) {
+        double back[3];
+        if (l.wall_right == 0) {
+            mirror(Rbar, back);
+#pragma unroll
+            for (int v = 0; v < 3; ++v) Lbar[v] += back[v];
+        } else {
+            ghost_tangent(l, Rbar, false, back);
+#pragma unroll
+            for (int v = 0; v < 3; ++v) Wbar[a][v] += back[v];
+        }
+    }
+    if (!cell[a].real) {
+        double back[3];
+        if (l.wall_left == 0) {
+            mirror(Lbar, back);
+#pragma unroll
+            for (int v = 0; v < 3; ++v) Rbar[v] += back[v];
+        } else {
+            ghost_tangent(l, Lbar, true, back);
+#pragma unroll
+            for (int v = 0; v < 3; ++v) Wbar[b][v] += back[v];
+        }
+    }
+    double gabar[3] = {0, 0, 0}, gbbar[3] = {0, 0, 0};
+    if (cell[a].real) face_value_adjoint(l, cell[a].W, ga, ra, dt, fallback_a, Lbar, Wbar[a], gabar);
+    if (cell[b].real) face_value_adjoint(l, cell[b].W, gb, rb, dt, fallback_b, Rbar, Wbar[b], gbbar);
+    if (l.order == 1) {
+        if (cell[a].real)
+#pragma unroll
+            for (int v = 0; v < 3; ++v) gradient_adjoint(l, cell + a, gap[a - 1], gap[a], v, gabar[v], Wbar[a - 1][v], Wbar[a][v], Wbar[a + 1][v]);
+        if (cell[b].real)
+#pragma unroll
+            for (int v = 0; v < 3; ++v) gradient_adjoint(l, cell + b, gap[b - 1], gap[b], v, gbbar[v], Wbar[b - 1][v], Wbar[b][v], Wbar[b + 1][v]);
+    }
+}
+
+// finite_volume_1d.hpp:831-848 load_tangent(): the W tangents of slot s's stencil from dq (ghosts through the ghost map).
+__device__ __forceinline__ void load_tangent(const LineArgs& l, const double* dq, int s, double (*dW)[3]) {
+    int sl[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        sl[k] = slot(l, s, k - 2);
+        dW[k][0] = dW[k][1] = dW[k][2] = 0;
+        if (sl[k] < 0) continue;
+        const int i = sl[k];
+        const double inv = 1.0 / volume(l, i);
+        const double U[3] = {l.q[i * 3] * inv, l.q[i * 3 + 1] * inv, l.q[i * 3 + 2] * inv};
+        const double dU[3] = {dq[i * 3] * inv, dq[i * 3 + 1] * inv, dq[i * 3 + 2] * inv};
+        primitive_tangent<1>(l.gas, U, dU, dW[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        if (sl[k] >= 0) continue;
+        const int end = k < 2 ? k + 1 : k - 1;
+        if (sl[end] >= 0) ghost_tangent(l, dW[end], k < 2, dW[k]);
+    }
+}
+
+// finite_volume_1d.hpp:850-880 step_tangent(): next = the step of q; dnext = J dq (dq: a tangent of the extensive state).
+__global__ __launch_bounds__(BLOCK) void k_line_tangent(LineArgs l, double dt, const double* dq, double* dnext) {
+    const int s = blockIdx.x * BLOCK + threadIdx.x;
+    if (s >= l.n) return;
+    Cell cell[5];
+    double gap[4], dW[5][3], F0[3], F1[3], dF0[3], dF1[3];
+    load(l, s, cell, gap);
+    load_tangent(l, dq, s, dW);
+    face_tangent(l, cell, gap, dW, 1, dt, F0, dF0);
+    face_tangent(l, cell, gap, dW, 2, dt, F1, dF1);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+        l.next[s * 3 + v] = l.q[s * 3 + v] - dt * (F1[v] - F0[v]);
+        dnext[s * 3 + v] = dq[s * 3 + v] - dt * (dF1[v] - dF0[v]);
+    }
+}
+
+// finite_volume_1d.hpp:882-933 step_adjoint(): out = J^T qbar with q the state before the step. Per slot j, the adjoints of
+// the faces whose fluxes read W_j (the left faces of slots j - 1 .. j + 2, and the right wall's face), each from its own
+// stencil, then the primitive adjoint: no atomics.
+__global__ __launch_bounds__(BLOCK) void k_line_adjoint(LineArgs l, double dt, const double* qbar, double* out) {
+    const int j = blockIdx.x * BLOCK + threadIdx.x;
+    const int n = l.n;
+    if (j >= n) return;
+    double Wj_bar[3] = {0, 0, 0};
+    for (int d = -1; d <= 3; ++d) {
+        int s, a;
+        if (d <= 2) {
+            s = slot(l, j, d);
+            a = 1;
+        } else {
+            if (l.periodic || j < n - 2) continue;
+            s = n - 1;
+            a = 2;  // the right wall's face
+        }
+        if (s < 0) continue;
+        Cell cell[5];
+        double gap[4];
+        load(l, s, cell, gap);
+        const int left = slot(l, s, a - 2), right = slot(l, s, a - 1);
+        double Fbar[3];
+#pragma unroll
+        for (int v = 0; v < 3; ++v) Fbar[v] = (right >= 0 ? dt * qbar[right * 3 + v] : 0.0) - (left >= 0 ? dt * qbar[left * 3 + v] : 0.0);
+        double Wbar[5][3];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) Wbar[k][0] = Wbar[k][1] = Wbar[k][2] = 0;
+        face_adjoint(l, cell, gap, a, dt, Fbar, Wbar);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const int t = slot(l, s, k - 2);
+            if (t == j) {
+#pragma unroll
+                for (int v = 0; v < 3; ++v) Wj_bar[v] += Wbar[k][v];
+            } else if (t < 0) {
+                const int end = k < 2 ? k + 1 : k - 1;
+                if (slot(l, s, end - 2) == j) {
+                    double back[3];
+                    ghost_tangent(l, Wbar[k], k < 2, back);
+#pragma unroll
+                    for (int v = 0; v < 3; ++v) Wj_bar[v] += back[v];
+                }
+            }
+        }
+    }
+    const double inv = 1.0 / volume(l, j);
+    const double U[3] = {l.q[j * 3] * inv, l.q[j * 3 + 1] * inv, l.q[j * 3 + 2] * inv};
+    double Ubar[3];
+    primitive_adjoint<1>(l.gas, U, Wj_bar, Ubar);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) out[j * 3 + v] = qbar[j * 3 + v] + Ubar[v] * inv;
+}
+
 }  // namespace
 
 hipError_t launch_line_time_step(hipStream_t s, const LineArgs& a, double courant, int luna_rule, const Counters& c) {
@@ -250,6 +605,16 @@ hipError_t launch_line_time_step(hipStream_t s, const LineArgs& a, double couran
 
 hipError_t launch_line_step(hipStream_t s, const LineArgs& a, double dt, double courant, bool fused, const Counters& c) {
     hipLaunchKernelGGL(k_line_step, dim3((a.n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, a, dt, courant, fused ? 1 : 0, c.dt, c.dt_next);
+    return hipGetLastError();
+}
+
+hipError_t launch_line_tangent(hipStream_t s, const LineArgs& a, double dt, const double* dq, double* dnext) {
+    hipLaunchKernelGGL(k_line_tangent, dim3((a.n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, a, dt, dq, dnext);
+    return hipGetLastError();
+}
+
+hipError_t launch_line_adjoint(hipStream_t s, const LineArgs& a, double dt, const double* qbar, double* out) {
+    hipLaunchKernelGGL(k_line_adjoint, dim3((a.n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, a, dt, qbar, out);
     return hipGetLastError();
 }
 

@@ -30,7 +30,9 @@
 namespace luna {
 namespace {
 
-constexpr int BLOCK = 256;
+// Work-group sizes: the per-cell list walks (sweep, update) run one wavefront per work-group, so a wavefront that ends
+// early frees its slot without waiting for the others of its group; the rest run 256.
+constexpr int BLOCK = 256, CELL_BLOCK = 64;
 
 // The logical block of this work-group. Work-groups go to the 8 XCDs in turn (block b to XCD b mod 8), so XCD x gets the
 // contiguous logical run [x q + min(x, r), ...) of q + (x < r) blocks (q = n / 8, r = n mod 8): a bijection.
@@ -53,8 +55,9 @@ __device__ __forceinline__ int wave_sum(int v) {
 
 // The block's smallest value into *out (as the bits of a non-negative double: they order as unsigned integers; NaN
 // terms never win, as Kokkos::Min's `<` and fmin drop them).
+template <int B>
 __device__ __forceinline__ void block_min(double v, unsigned long long* out) {
-    __shared__ double part[BLOCK / 64];
+    __shared__ double part[B / 64];
     v = wave_min(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) part[wave] = v;
@@ -65,8 +68,9 @@ __device__ __forceinline__ void block_min(double v, unsigned long long* out) {
         if (m < INFINITY) atomicMin(out, (unsigned long long)__double_as_longlong(m));
     }
 }
+template <int B>
 __device__ __forceinline__ void block_count(int v, unsigned long long* out) {
-    __shared__ int part[BLOCK / 64];
+    __shared__ int part[B / 64];
     v = wave_sum(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) part[wave] = v;
@@ -193,11 +197,11 @@ struct Sums {
 // |d| have the same bits from either side, so the minimum is the reference's (the face kernel's terms are the same set).
 // The first block also resets the other dt slot and the step's fallback counters.
 template <int D, bool Moving, bool Grad, bool Limit, int Rule>
-__global__ __launch_bounds__(BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, Rows r, Gas gas, double courant, Counters c) {
+__global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, Rows r, Gas gas, double courant, Counters c) {
     constexpr int V = D + 2, U = D + 1, GR = grad_row<D>();
     using C = CellAt<D>;
     using FA = FaceAt<D>;
-    const int i = xcd_block() * BLOCK + threadIdx.x;
+    const int i = xcd_block() * CELL_BLOCK + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *c.dt_next = INF_BITS;
         c.fallbacks[0] = 0;
@@ -324,7 +328,10 @@ __global__ __launch_bounds__(BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, Rows r
                 for (int a = 0; a < D; ++a) wi[a] = wi[a] + centroid[a] * sc;
             }
 #pragma unroll
-            for (int a = 0; a < D; ++a) r.w[(size_t)i * D + a] = wi[a];
+            for (int a = 0; a < D; ++a) {
+                r.w[(size_t)i * D + a] = wi[a];
+                r.old_centroid[(size_t)i * D + a] = centroid[a];  // keep_centroids (:1506-1514): c^n for the second pass
+            }
         }
         if constexpr (Rule == 0) {  // CellStep (:1046-1051)
             double rel[D];
@@ -397,7 +404,7 @@ __global__ __launch_bounds__(BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, Rows r
                 for (int a = 0; a < D; ++a) out[v * D + a] = grad[v][a];
         }
     }
-    block_min(smallest, c.dt);
+    block_min<CELL_BLOCK>(smallest, c.dt);
 }
 
 // finite_volume.hpp:1306-1329 FaceFlux::side() at order 1: half step (over step_dt) + g.r, positivity fallback. prow,
@@ -560,17 +567,25 @@ __global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, 
 #pragma unroll
         for (int v = 0; v < V; ++v) r.flux[(size_t)f * V + v] = F[v];
     }
-    block_count(count, fallbacks);
+    block_count<BLOCK>(count, fallbacks);
+}
+
+// k mod p in [0, p): one conditional step for a key within a period of the box (every position and move here), the
+// 64-bit remainder (a long instruction sequence on the GPU) only beyond that.
+__device__ __forceinline__ long long wrap_key(long long k, long long p) {
+    long long m = k >= p ? k - p : k < 0 ? k + p : k;
+    if (m < 0 || m >= p) m = ((k % p) + p) % p;
+    return m;
 }
 
 // finite_volume.hpp:1587-1603 update_body: Q -= dt sum_k sign_k F in list order; NextPrims: the primitive half of the
 // cell row for the next step (a moving step's second pass, and every static step, which the next time_step() would
 // compute the same way).
-template <int D, bool NextPrims>
-__global__ __launch_bounds__(BLOCK) void k_update(Geo g, Rows r, Gas gas, double dt) {
+template <int D, bool NextPrims, bool Drift>
+__global__ __launch_bounds__(CELL_BLOCK) void k_update(Geo g, Rows r, Gas gas, double dt, Grid grid) {
     constexpr int V = D + 2;
     using C = CellAt<D>;
-    const int i = xcd_block() * BLOCK + threadIdx.x;
+    const int i = xcd_block() * CELL_BLOCK + threadIdx.x;
     if (i >= g.N) return;
     double sum[V];
 #pragma unroll
@@ -616,15 +631,22 @@ __global__ __launch_bounds__(BLOCK) void k_update(Geo g, Rows r, Gas gas, double
         for (int v = 0; v < V; ++v) row[C::W + v] = W[v];
         row[C::CS] = sound_speed<D>(gas, W);
     }
+    if constexpr (Drift) {  // finite_volume.hpp:1219-1231: x to its grid point plus its whole steps, wrapped
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const size_t at = (size_t)i * D + a;
+            const long long k = (long long)round(r.x[at] * grid.per_step) + r.steps[at];
+            r.x[at] = (double)wrap_key(k, grid.period[a]) * grid.step;
+        }
+    }
 }
 
 __device__ __forceinline__ long long snap_key(double x, const Grid& grid, int a) {
-    long long k = (long long)round(x * grid.per_step) % grid.period[a];
-    if (k < 0) k += grid.period[a];
-    return k;
+    return wrap_key((long long)round(x * grid.per_step), grid.period[a]);
 }
 
-// finite_volume.hpp:1506-1514 keep_centroids: c^n kept; x to its grid point; the move to x + dt w in whole grid steps.
+// finite_volume.hpp:1506-1514 keep_centroids: x to its grid point; the move to x + dt w in whole grid steps (c^n is kept
+// by the sweep, which reads it).
 template <int D>
 __global__ __launch_bounds__(BLOCK) void k_keep_centroids(Geo g, Rows r, Grid grid, double dt) {
     const int i = xcd_block() * BLOCK + threadIdx.x;
@@ -632,7 +654,6 @@ __global__ __launch_bounds__(BLOCK) void k_keep_centroids(Geo g, Rows r, Grid gr
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         const size_t at = (size_t)i * D + a;
-        r.old_centroid[at] = g.cell[(size_t)i * CELL_ROW + CellAt<D>::C + a];
         const double x = r.x[at];
         const long long k0 = snap_key(x, grid, a), k1 = snap_key(x + dt * r.w[at], grid, a);
         r.x[at] = (double)k0 * grid.step;
@@ -644,21 +665,7 @@ __global__ __launch_bounds__(BLOCK) void k_keep_centroids(Geo g, Rows r, Grid gr
     }
 }
 
-// finite_volume.hpp:1219-1231 drift on the grid: x to its grid point plus its whole steps, wrapped.
-template <int D>
-__global__ __launch_bounds__(BLOCK) void k_drift(int N, Rows r, Grid grid) {
-    const int i = xcd_block() * BLOCK + threadIdx.x;
-    if (i >= N) return;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-        const size_t at = (size_t)i * D + a;
-        const long long k = (long long)round(r.x[at] * grid.per_step) + r.steps[at];
-        const long long p = grid.period[a];
-        r.x[at] = (double)(((k % p) + p) % p) * grid.step;
-    }
-}
-
-inline dim3 blocks(long long n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
+inline dim3 blocks(long long n, int b = BLOCK) { return dim3((unsigned)((n + b - 1) / b)); }
 
 }  // namespace
 
@@ -671,11 +678,11 @@ hipError_t launch_primitives(hipStream_t s, const Geo& g, const Rows& r, const G
 
 template <int D, bool Moving, bool Grad, bool Limit>
 static hipError_t sweep_rule(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double courant, int rule, const Counters& c) {
-    const dim3 grid = blocks(g.N > 0 ? g.N : 1);
+    const dim3 grid = blocks(g.N > 0 ? g.N : 1, CELL_BLOCK);
     if (rule == 1)
-        hipLaunchKernelGGL((k_sweep<D, Moving, Grad, Limit, 1>), grid, dim3(BLOCK), 0, s, g, r, gas, courant, c);
+        hipLaunchKernelGGL((k_sweep<D, Moving, Grad, Limit, 1>), grid, dim3(CELL_BLOCK), 0, s, g, r, gas, courant, c);
     else
-        hipLaunchKernelGGL((k_sweep<D, Moving, Grad, Limit, 0>), grid, dim3(BLOCK), 0, s, g, r, gas, courant, c);
+        hipLaunchKernelGGL((k_sweep<D, Moving, Grad, Limit, 0>), grid, dim3(CELL_BLOCK), 0, s, g, r, gas, courant, c);
     return hipGetLastError();
 }
 
@@ -712,12 +719,15 @@ hipError_t launch_flux(hipStream_t s, const Geo& g, const Rows& r, const Gas& ga
 }
 
 template <int D>
-hipError_t launch_update(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double dt, bool next_prims) {
+hipError_t launch_update(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double dt, bool next_prims, const Grid* drift) {
     if (g.N == 0) return hipSuccess;
-    if (next_prims)
-        hipLaunchKernelGGL((k_update<D, true>), blocks(g.N), dim3(BLOCK), 0, s, g, r, gas, dt);
+    const Grid grid = drift ? *drift : Grid{};
+    if (drift)
+        hipLaunchKernelGGL((k_update<D, false, true>), blocks(g.N, CELL_BLOCK), dim3(CELL_BLOCK), 0, s, g, r, gas, dt, grid);
+    else if (next_prims)
+        hipLaunchKernelGGL((k_update<D, true, false>), blocks(g.N, CELL_BLOCK), dim3(CELL_BLOCK), 0, s, g, r, gas, dt, grid);
     else
-        hipLaunchKernelGGL((k_update<D, false>), blocks(g.N), dim3(BLOCK), 0, s, g, r, gas, dt);
+        hipLaunchKernelGGL((k_update<D, false, false>), blocks(g.N, CELL_BLOCK), dim3(CELL_BLOCK), 0, s, g, r, gas, dt, grid);
     return hipGetLastError();
 }
 
@@ -728,22 +738,14 @@ hipError_t launch_keep_centroids(hipStream_t s, const Geo& g, const Rows& r, con
     return hipGetLastError();
 }
 
-template <int D>
-hipError_t launch_drift(hipStream_t s, int N, const Rows& r, const Grid& grid) {
-    if (N == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_drift<D>, blocks(N), dim3(BLOCK), 0, s, N, r, grid);
-    return hipGetLastError();
-}
-
 #define LUNA_INSTANTIATE(D)                                                                                                          \
     template hipError_t launch_primitives<D>(hipStream_t, const Geo&, const Rows&, const Gas&);                                     \
     template hipError_t launch_sweep<D>(hipStream_t, const Geo&, const Rows&, const Gas&, double, bool, bool, bool, int,             \
                                         const Counters&);                                                                           \
     template hipError_t launch_flux<D>(hipStream_t, const Geo&, const Rows&, const Gas&, int, int, int, double, double,              \
                                        unsigned long long*);                                                                        \
-    template hipError_t launch_update<D>(hipStream_t, const Geo&, const Rows&, const Gas&, double, bool);                           \
-    template hipError_t launch_keep_centroids<D>(hipStream_t, const Geo&, const Rows&, const Grid&, double);                         \
-    template hipError_t launch_drift<D>(hipStream_t, int, const Rows&, const Grid&);
+    template hipError_t launch_update<D>(hipStream_t, const Geo&, const Rows&, const Gas&, double, bool, const Grid*);              \
+    template hipError_t launch_keep_centroids<D>(hipStream_t, const Geo&, const Rows&, const Grid&, double);
 
 LUNA_INSTANTIATE(2)
 LUNA_INSTANTIATE(3)

@@ -89,8 +89,9 @@ def exported_symbols():
              "luna_fv_time_step", "luna_fv_step", "luna_fv_moving_begin", "luna_fv_moving_finish", "luna_fv_fallbacks",
              "luna_fv_positions", "luna_fv_point_velocities", "luna_fv_gradients", "luna_fv_primitives", "luna_fv_faces", "luna_fv_profile",
              "luna_fv_profile_read", "luna_fv1_create", "luna_fv1_destroy", "luna_fv1_set_state", "luna_fv1_get_state",
-             "luna_fv1_time_step", "luna_fv1_step", "luna_fv1_step_and_time_step", "luna_last_error",
-             "luna_fv_set_geometry", "luna_synchronize"]
+             "luna_fv1_time_step", "luna_fv1_step", "luna_fv1_step_and_time_step", "luna_fv1_step_tangent", "luna_fv1_step_adjoint",
+             "luna_last_error",
+             "luna_fv_set_geometry", "luna_synchronize", "luna_fv_step_tangent", "luna_fv_step_adjoint"]
     return {n: hasattr(lib(), n) for n in names}
 
 
@@ -157,6 +158,27 @@ class FiniteVolume:
 
     def step(self, dt):
         _check(lib().luna_fv_step(self._h, ctypes.c_double(dt)), "step")
+
+    def step_tangent(self, dt, dq):
+        """FiniteVolume::step_tangent(): one static step of Q and of the tangent dq (N x V, extensive); returns J dq. dq: a
+        numpy array (host) or an object with a device data_ptr() (e.g. a torch tensor on the GPU, updated in place)."""
+        return self._derivative(lib().luna_fv_step_tangent, dt, dq, "step_tangent")
+
+    def step_adjoint(self, dt, qbar):
+        """FiniteVolume::step_adjoint(): J^T qbar of one static step from the current Q (which stays); qbar as step_tangent's dq."""
+        return self._derivative(lib().luna_fv_step_adjoint, dt, qbar, "step_adjoint")
+
+    def _derivative(self, fn, dt, rows, what):
+        if hasattr(rows, "data_ptr"):
+            if not rows.is_contiguous() or rows.numel() != self.N * self.V:
+                raise ValueError("%s: a contiguous N x V device array" % what)
+            _check(fn(self._h, ctypes.c_double(dt), ctypes.c_void_p(rows.data_ptr()), 1), what)
+            return rows
+        out = np.ascontiguousarray(np.array(rows, dtype=np.float64))
+        if out.size != self.N * self.V:
+            raise ValueError("%s: N x V rows" % what)
+        _check(fn(self._h, ctypes.c_double(dt), out.ctypes.data_as(_dp), 0), what)
+        return out
 
     def moving_begin(self, dt, positions=True):
         """The first half of a moving step; returns the new positions (None with positions=False)."""
@@ -261,6 +283,18 @@ class Line:
 
     def step(self, dt):
         _check(lib().luna_fv1_step(self._h, ctypes.c_double(dt)), "step")
+
+    def step_tangent(self, dt, dq):
+        """FiniteVolume<1>::step_tangent(): one step of Q and of the tangent dq (cells x 3, extensive); returns J dq."""
+        dq = np.ascontiguousarray(np.array(dq, dtype=np.float64))
+        _check(lib().luna_fv1_step_tangent(self._h, ctypes.c_double(dt), dq.ctypes.data_as(_dp)), "step_tangent")
+        return dq
+
+    def step_adjoint(self, dt, qbar):
+        """FiniteVolume<1>::step_adjoint(): J^T qbar of one step from the current Q (which stays)."""
+        qbar = np.ascontiguousarray(np.array(qbar, dtype=np.float64))
+        _check(lib().luna_fv1_step_adjoint(self._h, ctypes.c_double(dt), qbar.ctypes.data_as(_dp)), "step_adjoint")
+        return qbar
 
     def step_and_time_step(self, dt, courant):
         nxt = ctypes.c_double()
