@@ -2,6 +2,7 @@
 
   libluna_hydro.so  hand-written HIP for gfx950 (csrc/hydro/*.hip), hipcc, FP64 with -ffp-contract=off
   libluna_mesh.so   the periodic Voronoi geometry producer (csrc/mesh/voronoi.cpp), g++ with OpenMP
+  libluna_snapshot.so  LUNA-layout HDF5 snapshots (csrc/snapshot/snapshot.cpp), g++ with the image's HDF5
 
 Run `python luna-cpp_amd/luna_hydro/build.py`, or __graft_entry__.build().
 """
@@ -48,8 +49,20 @@ def build_mesh():
     return out
 
 
+HDF5 = os.environ.get("LUNA_HDF5", "/opt/conda")  # the image's HDF5 C library (include/, lib/)
+
+
+def build_snapshot():
+    out = os.path.join(HERE, "libluna_snapshot.so")
+    lib = os.path.join(HDF5, "lib")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-I" + INCLUDE, "-I" + os.path.join(HDF5, "include"),
+          "-o", out + ".tmp", os.path.join(CSRC, "snapshot", "snapshot.cpp"), "-L" + lib, "-lhdf5", "-Wl,-rpath," + lib])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
-    return [build_mesh(), build_hydro()]
+    return [build_mesh(), build_snapshot(), build_hydro()]
 
 
 if __name__ == "__main__":

@@ -1,0 +1,140 @@
+"""The 3D Sedov blast driver, as luna-cpp's hydro_sedov runs it (src/bench/hydro_sedov.cpp:300-406): a jittered periodic
+Voronoi mesh, the volume-weighted blast, time_step + step to t_end with snapshot times hit exactly, then its report lines
+(`run:`, `fallbacks:`, `conservation:`, `phases_ms_median:`, `checksum:` with FNV-1a over the bits of Q, positions and
+volume) so runs compare line by line.
+
+    python -m luna_hydro.sedov --n 64 [--moving] [--t-end 0.05] [--courant 0.3] [--snapshots 10] [--out DIR] [--max-steps K]
+"""
+import argparse
+import ctypes
+import math
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+from . import hydro, mesh, problems, snapshot
+
+GAMMA = 5.0 / 3.0
+FNV_START = 1469598103934665603  # the start value of the reference's checksum (hydro_sedov.cpp:162)
+
+
+def bits_hash(a, h=FNV_START):
+    """FNV-1a over the bits of every value of a (row-major), chained from h (hydro_sedov.cpp:159-174)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    snapshot.lib().luna_fnv1a.restype = ctypes.c_ulonglong
+    return snapshot.lib().luna_fnv1a(a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(a.nbytes), ctypes.c_ulonglong(h))
+
+
+def totals(q):
+    return {"mass": math.fsum(q[:, 0]), "px": math.fsum(q[:, 1]), "py": math.fsum(q[:, 2]), "pz": math.fsum(q[:, 3]),
+            "energy": math.fsum(q[:, 4])}
+
+
+def commit():
+    try:
+        here = os.path.dirname(os.path.abspath(__file__))
+        rev = subprocess.run(["git", "-C", here, "rev-parse", "--short", "HEAD"], capture_output=True, text=True, timeout=10)
+        dirty = subprocess.run(["git", "-C", here, "status", "--porcelain"], capture_output=True, text=True, timeout=10)
+        return rev.stdout.strip(), bool(dirty.stdout.strip())
+    except (OSError, subprocess.SubprocessError):
+        return "", False
+
+
+class _Engine:
+    """hydro.FiniteVolume (the GPU) or an object with its methods; the moving step needs the mesh producer between
+    moving_begin and moving_finish."""
+
+    def __init__(self, fv, moving):
+        self.fv, self.moving = fv, moving
+
+    def step(self, dt):
+        if not self.moving:
+            self.fv.step(dt)
+            return None
+        x = self.fv.moving_begin(dt)
+        m = mesh.build(x, 1.0)
+        self.fv.moving_finish(m, dt)
+        return m
+
+
+def run(n=64, moving=False, t_end=0.05, courant=0.3, snapshots=10, out="", max_steps=0, jitter=0.25, seed=1, log=print):
+    m = mesh.build(problems.jittered_lattice(n, 3, jitter, seed), 1.0)
+    q, deposited = problems.sedov_state(m)
+    N = m["cells"]
+    fv = hydro.FiniteVolume(m, GAMMA, motion="moving" if moving else "static")
+    fv.set_state(q)
+    engine = _Engine(fv, moving)
+    before = totals(q)
+    rev, dirty = commit()
+    current = m
+
+    def write(index, t):
+        if not out:
+            return
+        x = fv.positions() if moving else np.asarray(current["positions"]).reshape(-1, 3)
+        snapshot.write(out, index, t, x, current["volume"], fv.state(), GAMMA, commit=rev, dirty=dirty)
+
+    write(0, 0.0)
+    walls, steps, fallbacks, most, with_fallbacks, t, nxt = [], 0, 0, 0, 0, 0.0, 1
+    fv.profile(True)
+    start = time.perf_counter()
+    while t < t_end and (max_steps == 0 or steps < max_steps):
+        a = time.perf_counter()
+        dt = fv.time_step(courant)
+        t_next = t_end * nxt / snapshots
+        snap = t + dt >= t_next
+        if snap:
+            dt = t_next - t
+        new = engine.step(dt)
+        if new is not None:
+            current = new
+        step_fallbacks = fv.fallbacks()  # fences: the step is done
+        walls.append((time.perf_counter() - a) * 1e3)
+        fallbacks += step_fallbacks
+        most = max(most, step_fallbacks)
+        with_fallbacks += step_fallbacks > 0
+        t = t_next if snap else t + dt
+        steps += 1
+        if snap:
+            if out or nxt == snapshots:
+                write(nxt, t)
+            nxt += 1
+    wall_s = time.perf_counter() - start
+    qf = fv.state()
+    after = totals(qf)
+    median = float(np.median(walls)) if walls else float("nan")
+    kernels = fv.kernel_times()
+    log("run: precision=double order=1 update=face steps=%d limiter=face riemann=hllc mesh=%s blast=%d t=%.17g wall_s=%.6g "
+        "s_per_step=%.6g step_wall_ms_median=%.6g G_cell_updates_per_s=%.6g" % (steps, "moving" if moving else "static", deposited, t,
+                                                                                wall_s, wall_s / max(steps, 1), median,
+                                                                                N / (median * 1e-3) / 1e9))
+    log("fallbacks: total=%d steps_with=%d most_in_a_step=%d face_sides_per_step=%d" % (fallbacks, with_fallbacks, most, 2 * fv.faces()))
+    log("phases_ms_median: " + " ".join("%s=%.4g" % (k, v[1] / v[0]) for k, v in sorted(kernels.items())))
+    scale = before["mass"] * math.sqrt(2 * before["energy"] / before["mass"])
+    log("conservation: mass=%.3e energy=%.3e momentum=(%.3e,%.3e,%.3e) momentum_scale=%.6g" % (
+        (after["mass"] - before["mass"]) / before["mass"], (after["energy"] - before["energy"]) / before["energy"],
+        after["px"] - before["px"], after["py"] - before["py"], after["pz"] - before["pz"], scale))
+    x = fv.positions() if moving else np.asarray(current["positions"]).reshape(-1, 3)
+    log("checksum: steps=%d Q=%x x=%x volume=%x" % (steps, bits_hash(qf), bits_hash(x), bits_hash(current["volume"])))
+    fv.close()
+    return {"steps": steps, "t": t, "q": qf, "x": x, "volume": current["volume"]}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--moving", action="store_true")
+    ap.add_argument("--t-end", type=float, default=0.05)
+    ap.add_argument("--courant", type=float, default=0.3)
+    ap.add_argument("--snapshots", type=int, default=10)
+    ap.add_argument("--out", default="")
+    ap.add_argument("--max-steps", type=int, default=0)
+    a = ap.parse_args(argv)
+    run(a.n, a.moving, a.t_end, a.courant, a.snapshots, a.out, a.max_steps)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,53 @@
+"""The LUNA-layout HDF5 snapshot (snapshot.hpp:68-135): a written file read back gives the positions, volumes and U =
+Q / V bit for bit, the primitives as the oracle's primitive() computes them, internal energy P / (gamma - 1), and the
+attributes; the path follows snapdir_NNN/snapshot_NNN.0.hdf5."""
+import os
+
+import numpy as np
+
+import cases
+import oracle
+from luna_hydro import problems, snapshot
+
+
+def test_snapshot_round_trip(tmp_path):
+    m = cases.voronoi(6, 3, 0.25, 1)
+    q, _ = problems.sedov_state(m)
+    path = snapshot.write(str(tmp_path), 7, 0.0125, m["positions"], m["volume"], q, 5.0 / 3.0, commit="abc123", dirty=True)
+    assert path == os.path.join(str(tmp_path), "snapdir_007", "snapshot_007.0.hdf5")
+    s = snapshot.read(path)
+    x = np.asarray(m["positions"]).reshape(-1, 3)
+    assert np.array_equal(s["x"], x[:, 0]) and np.array_equal(s["y"], x[:, 1]) and np.array_equal(s["z"], x[:, 2])
+    assert np.array_equal(s["volume"], m["volume"])
+    U = q / m["volume"][:, None]
+    assert np.array_equal(s["U"], U.T)
+    W = np.array([oracle.primitive(u, 5.0 / 3.0) for u in U])
+    assert np.array_equal(s["density"], W[:, 0])
+    assert np.array_equal(s["velocity_y"], W[:, 2])
+    assert np.array_equal(s["pressure"], W[:, 4])
+    assert np.array_equal(s["internal_energy"], W[:, 4] / (5.0 / 3.0 - 1.0))
+    a = s["attributes"]
+    assert a["time"] == 0.0125 and a["step"] == 7 and a["spatial_dimension"] == 3 and a["N_cells"] == m["cells"]
+    assert a["mesh"] == "voronoi" and a["gamma"] == 5.0 / 3.0 and a["luna_cpp_dirty"] == 1
+
+
+def test_fnv1a_known_answers():
+    """FNV-1a 64 on the published test vectors (offset basis 0xcbf29ce484222325: "" and "a"), and the drivers' checksum:
+    the same loop from the reference's own start value 1469598103934665603 (hydro_sedov.cpp:162, which is not the
+    standard basis; kept so the checksum lines compare with the reference's), over each double's bytes in row order."""
+    import ctypes
+
+    from luna_hydro import sedov
+
+    lib = snapshot.lib()
+    lib.luna_fnv1a.restype = ctypes.c_ulonglong
+    basis = 0xCBF29CE484222325
+    assert lib.luna_fnv1a(b"", ctypes.c_size_t(0), ctypes.c_ulonglong(basis)) == basis
+    assert lib.luna_fnv1a(b"a", ctypes.c_size_t(1), ctypes.c_ulonglong(basis)) == 0xAF63DC4C8601EC8C
+    assert sedov.FNV_START == 1469598103934665603
+    assert sedov.bits_hash(np.zeros(0)) == sedov.FNV_START
+    a = np.array([[1.0, -2.5], [3.25, 1e-300]])
+    h = sedov.FNV_START
+    for b in a.tobytes():
+        h = ((h ^ b) * 1099511628211) % (1 << 64)
+    assert sedov.bits_hash(a) == h
