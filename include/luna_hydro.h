@@ -157,18 +157,18 @@ int luna_fv1_step(luna_fv1* fv, double dt);
 int luna_fv1_step_and_time_step(luna_fv1* fv, double dt, double courant, double* next);
 
 /* FiniteVolume<1>::step_tangent(dt, dq) (finite_volume_1d.hpp:850-880): one step of Q and of its tangent; dq (host, cells x 3,
- * a tangent of the extensive Q as state()) becomes J dq in place. Static line, HLLC, dt fixed. */
+ * a tangent of the extensive Q as state()) becomes J dq in place. Static line, dt fixed. */
 int luna_fv1_step_tangent(luna_fv1* fv, double dt, double* dq);
 
 /* FiniteVolume<1>::step_adjoint(dt, qbar) (finite_volume_1d.hpp:882-933): qbar (host, cells x 3) after the step becomes J^T qbar
- * before it, in place; Q is the state at the start of the step and stays. Static line, HLLC, dt fixed. */
+ * before it, in place; Q is the state at the start of the step and stays. Static line, dt fixed. */
 int luna_fv1_step_adjoint(luna_fv1* fv, double dt, double* qbar);
 
-/* Derivatives of one static step at fixed dt (finite_volume.hpp:1670-2231; static mesh, order 1, HLLC, either limiter), on
+/* Derivatives of one static step at fixed dt (finite_volume.hpp:1670-2231; static mesh, order 1, HLLC or HLL, either limiter), on
  * N x (dim + 2) rows of the extensive state; device = 1: dq / qbar are device pointers (as luna_fv_state_device), else host.
  * FiniteVolume::step_tangent(dt, dq) (:2039-2074): Q and dq advance by one step, dq becoming J dq.
  * FiniteVolume::step_adjoint(dt, qbar) (:2201-2231): qbar after the step becomes J^T qbar before it; Q (the state at the start
- * of the step) stays. Both block until done. LUNA_ERR_LOGIC on a moving mesh, order 0 or HLL. */
+ * of the step) stays. Both block until done. LUNA_ERR_LOGIC on a moving mesh or order 0. */
 int luna_fv_step_tangent(luna_fv* fv, double dt, double* dq, int device);
 int luna_fv_step_adjoint(luna_fv* fv, double dt, double* qbar, int device);
 

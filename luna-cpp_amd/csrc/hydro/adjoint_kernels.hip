@@ -1,4 +1,4 @@
-// Derivatives of the static order-1 HLLC step on a Voronoi mesh (D = 2, 3) at fixed dt, for gfx950: the tangent
+// Derivatives of the static order-1 step (HLLC or HLL) on a Voronoi mesh (D = 2, 3) at fixed dt, for gfx950: the tangent
 // (Q, dQ) -> (Q', J dQ) and the adjoint Qbar' -> J^T Qbar' (finite_volume.hpp:1670-2231). Every sum over a cell's faces
 // runs in list order, one work-item per cell, and every face's share is computed by one work-item per face, so the
 // results are deterministic (no atomics) and follow the CPU restatement's operation order. The forward pieces (fit,
@@ -344,7 +344,8 @@ __global__ __launch_bounds__(BLOCK) void k_fit_tangent(Geo g, Rows r, int limit,
 
 // One face's flux row and its tangent (x |A|).
 template <int D>
-__global__ __launch_bounds__(BLOCK) void k_flux_tangent(Geo g, Rows r, Gas gas, double dt, const double* dW, const double* dgrad, double* dflux) {
+__global__ __launch_bounds__(BLOCK) void k_flux_tangent(Geo g, Rows r, Gas gas, int riemann, double dt, const double* dW, const double* dgrad,
+                                                        double* dflux) {
     constexpr int V = D + 2, GR = grad_row<D>();
     using C = CellAt<D>;
     const int f = blockIdx.x * BLOCK + threadIdx.x;
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(BLOCK) void k_flux_tangent(Geo g, Rows r, Gas gas, 
     }
     const double w0[D] = {};
     double Fl[V], dFl[V];
-    hllc_tangent<D>(gas, Ls, Rs, dLs, dRs, n, w0, Fl, dFl);
+    riemann_tangent<D>(riemann, gas, Ls, Rs, dLs, dRs, n, w0, Fl, dFl);
 #pragma unroll
     for (int v = 0; v < V; ++v) {
         Fv[v] = Fl[v] * magnitude;
@@ -427,7 +428,7 @@ __global__ __launch_bounds__(BLOCK) void k_update_tangent(Geo g, Rows r, double 
 
 // Per face: the bars of its two sides, (L bar, R bar) = J^T Fbar of the Riemann solve, Fbar = (dt qbar_j - dt qbar_o) |A|.
 template <int D>
-__global__ __launch_bounds__(BLOCK) void k_face_adjoint(Geo g, Rows r, Gas gas, double dt, const double* qbar, double* sidebar) {
+__global__ __launch_bounds__(BLOCK) void k_face_adjoint(Geo g, Rows r, Gas gas, int riemann, double dt, const double* qbar, double* sidebar) {
     constexpr int V = D + 2;
     const int f = blockIdx.x * BLOCK + threadIdx.x;
     if (f >= g.F) return;
@@ -443,7 +444,7 @@ __global__ __launch_bounds__(BLOCK) void k_face_adjoint(Geo g, Rows r, Gas gas, 
 #pragma unroll
         for (int v = 0; v < V; ++v) Fbar[v] = (dt * qbar[(size_t)pair.y * V + v] - dt * qbar[(size_t)pair.x * V + v]) * magnitude;
         const double w0[D] = {};
-        hllc_adjoint<D>(gas, Ls, Rs, n, w0, Fbar, Lbar, Rbar);
+        riemann_adjoint<D>(riemann, gas, Ls, Rs, n, w0, Fbar, Lbar, Rbar);
     }
     double* out = sidebar + (size_t)f * 2 * V;
 #pragma unroll
@@ -606,28 +607,28 @@ __global__ __launch_bounds__(BLOCK) void k_gather_adjoint(Geo g, Rows r, Gas gas
 }  // namespace
 
 template <int D>
-hipError_t launch_step_tangent(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double dt, bool limit, const Derivs& w) {
+hipError_t launch_step_tangent(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, bool limit, const Derivs& w) {
     if (g.N == 0) return hipSuccess;
     hipLaunchKernelGGL(k_prim_tangent<D>, grid_of(g.N), dim3(BLOCK), 0, s, g, r, gas, w.dq, w.dW);
     hipLaunchKernelGGL(k_fit_tangent<D>, grid_of(g.N), dim3(BLOCK), 0, s, g, r, limit ? 1 : 0, w.dW, w.dgrad);
-    if (g.F > 0) hipLaunchKernelGGL(k_flux_tangent<D>, grid_of(g.F), dim3(BLOCK), 0, s, g, r, gas, dt, w.dW, w.dgrad, w.dflux);
+    if (g.F > 0) hipLaunchKernelGGL(k_flux_tangent<D>, grid_of(g.F), dim3(BLOCK), 0, s, g, r, gas, riemann, dt, w.dW, w.dgrad, w.dflux);
     hipLaunchKernelGGL(k_update_tangent<D>, grid_of(g.N), dim3(BLOCK), 0, s, g, r, dt, w.dflux, w.dq);
     return hipGetLastError();
 }
 
 template <int D>
-hipError_t launch_step_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double dt, bool limit, const Derivs& w) {
+hipError_t launch_step_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, bool limit, const Derivs& w) {
     if (g.N == 0) return hipSuccess;
-    if (g.F > 0) hipLaunchKernelGGL(k_face_adjoint<D>, grid_of(g.F), dim3(BLOCK), 0, s, g, r, gas, dt, w.dq, w.sidebar);
+    if (g.F > 0) hipLaunchKernelGGL(k_face_adjoint<D>, grid_of(g.F), dim3(BLOCK), 0, s, g, r, gas, riemann, dt, w.dq, w.sidebar);
     hipLaunchKernelGGL(k_cell_adjoint<D>, grid_of(g.N), dim3(BLOCK), 0, s, g, r, gas, dt, limit ? 1 : 0, w.sidebar, w.dW, w.bbar, w.boundbar,
                        w.extreme);
     hipLaunchKernelGGL(k_gather_adjoint<D>, grid_of(g.N), dim3(BLOCK), 0, s, g, r, gas, w.dq, w.dW, w.bbar, w.boundbar, w.extreme, w.out);
     return hipGetLastError();
 }
 
-template hipError_t launch_step_tangent<2>(hipStream_t, const Geo&, const Rows&, const Gas&, double, bool, const Derivs&);
-template hipError_t launch_step_tangent<3>(hipStream_t, const Geo&, const Rows&, const Gas&, double, bool, const Derivs&);
-template hipError_t launch_step_adjoint<2>(hipStream_t, const Geo&, const Rows&, const Gas&, double, bool, const Derivs&);
-template hipError_t launch_step_adjoint<3>(hipStream_t, const Geo&, const Rows&, const Gas&, double, bool, const Derivs&);
+template hipError_t launch_step_tangent<2>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&);
+template hipError_t launch_step_tangent<3>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&);
+template hipError_t launch_step_adjoint<2>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&);
+template hipError_t launch_step_adjoint<3>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&);
 
 }  // namespace luna

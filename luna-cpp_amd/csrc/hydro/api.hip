@@ -536,8 +536,8 @@ int luna_fv_step(luna_fv* fv, double dt) {
 namespace {
 
 int derivatives_ready(luna_fv* fv, const char* what) {
-    if (fv->opt.motion != LUNA_MOTION_STATIC || fv->opt.order != 1 || fv->opt.riemann != LUNA_RIEMANN_HLLC)
-        return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of a static order-1 HLLC step");
+    if (fv->opt.motion != LUNA_MOTION_STATIC || fv->opt.order != 1)
+        return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of a static order-1 step");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, std::string(what) + ": a moving step is between begin and finish");
     const size_t N = fv->N, V = fv->V, F = fv->F;
     LUNA_HIP(fv->d_dq.reserve(N * V));
@@ -571,7 +571,8 @@ int luna_fv_step_tangent(luna_fv* fv, double dt, double* dq, int device) {
     const luna::Derivs w = derivs_of(fv);
     const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID;
     LUNA_HIP(timed(fv->prof, s, "step_tangent", [&] {
-        return fv->D == 3 ? luna::launch_step_tangent<3>(s, g, r, gas, dt, limit, w) : luna::launch_step_tangent<2>(s, g, r, gas, dt, limit, w);
+        return fv->D == 3 ? luna::launch_step_tangent<3>(s, g, r, gas, fv->opt.riemann, dt, limit, w)
+                          : luna::launch_step_tangent<2>(s, g, r, gas, fv->opt.riemann, dt, limit, w);
     }));
     LUNA_HIP(hipMemcpyAsync(dq, fv->d_dq.p, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     LUNA_HIP(hipStreamSynchronize(s));
@@ -594,7 +595,8 @@ int luna_fv_step_adjoint(luna_fv* fv, double dt, double* qbar, int device) {
     const luna::Derivs w = derivs_of(fv);
     const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID;
     LUNA_HIP(timed(fv->prof, s, "step_adjoint", [&] {
-        return fv->D == 3 ? luna::launch_step_adjoint<3>(s, g, r, gas, dt, limit, w) : luna::launch_step_adjoint<2>(s, g, r, gas, dt, limit, w);
+        return fv->D == 3 ? luna::launch_step_adjoint<3>(s, g, r, gas, fv->opt.riemann, dt, limit, w)
+                          : luna::launch_step_adjoint<2>(s, g, r, gas, fv->opt.riemann, dt, limit, w);
     }));
     LUNA_HIP(hipMemcpyAsync(qbar, fv->d_out.p, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     LUNA_HIP(hipStreamSynchronize(s));
@@ -867,7 +869,6 @@ int luna_fv1_step(luna_fv1* fv, double dt) {
 
 int luna_fv1_step_tangent(luna_fv1* fv, double dt, double* dq) {
     if (!fv || !dq) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_step_tangent: null argument");
-    if (fv->opt.riemann != LUNA_RIEMANN_HLLC) return fail(LUNA_ERR_LOGIC, "luna_fv1_step_tangent: derivatives of the HLLC step");
     const size_t bytes = sizeof(double) * 3 * fv->n;
     LUNA_HIP(fv->dq.reserve(3 * static_cast<size_t>(fv->n)));
     LUNA_HIP(fv->dnext.reserve(3 * static_cast<size_t>(fv->n)));
@@ -881,7 +882,6 @@ int luna_fv1_step_tangent(luna_fv1* fv, double dt, double* dq) {
 
 int luna_fv1_step_adjoint(luna_fv1* fv, double dt, double* qbar) {
     if (!fv || !qbar) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_step_adjoint: null argument");
-    if (fv->opt.riemann != LUNA_RIEMANN_HLLC) return fail(LUNA_ERR_LOGIC, "luna_fv1_step_adjoint: derivatives of the HLLC step");
     const size_t bytes = sizeof(double) * 3 * fv->n;
     LUNA_HIP(fv->dq.reserve(3 * static_cast<size_t>(fv->n)));
     LUNA_HIP(fv->dnext.reserve(3 * static_cast<size_t>(fv->n)));

@@ -1,5 +1,5 @@
 // Derivatives of the pointwise physics for the device: tangents (J dx) and adjoints (J^T ybar, added into the bars) of
-// primitive(), half_step(), side_with(), to_lab() and the HLLC flux, at fixed dt, face normal and face velocity. At every
+// primitive(), half_step(), side_with() / side_full(), to_lab() and the HLLC and HLL fluxes, at fixed dt, face normal and face velocity. At every
 // switch of the forward (a floor, a wave case, a 1e-20 guard) the derivative is that of the branch the forward took, as
 // luna-cpp's own derivatives are defined (src/hydro/euler.hpp:115-117); written from the forward arithmetic of physics.hpp
 // by the chain rule. -ffp-contract=off like the rest.
@@ -458,6 +458,153 @@ __device__ __forceinline__ void hllc_adjoint(const Gas& gas, const double* L, co
         Lbar[1 + k] += vnLbar * n[k];
         Rbar[1 + k] += vnRbar * n[k];
     }
+}
+
+// ---- HLL (euler.hpp:444-540) ----
+
+// euler.hpp:377-385: the tangent of side_full(), the sound speed's included (zero where c^2 <= 0, as the forward).
+template <int D>
+__device__ __forceinline__ void side_full_tangent(const Gas& gas, const double* W, const double* dW, const double* n, const Side<D>& s, Side<D>& d) {
+    constexpr int P = D + 1;
+    double dv[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) dv[k] = dW[1 + k];
+    side_with_tangent<D>(gas, W, dW, n, s, dot<D>(dv, n), d);
+    const double c2 = gas.gamma * W[P] / W[0];
+    d.c = c2 > 0 ? (gas.gamma * dW[P] - c2 * dW[0]) / W[0] / (2 * s.c) : 0.0;
+}
+
+// euler.hpp:413-426: the adjoint of side_full() from the bars of F, U, vn and c; adds J^T to Wbar.
+template <int D>
+__device__ __forceinline__ void side_full_adjoint(const Gas& gas, const double* W, const double* n, const Side<D>& s, const double* Fbar,
+                                                  const double* Ubar, double vnbar, double cbar, double* Wbar) {
+    constexpr int P = D + 1;
+    double vbar[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) vbar[k] = 0;
+    vnbar += side_with_adjoint<D>(gas, W, n, s, Fbar, Ubar, 0.0, vbar, Wbar);
+#pragma unroll
+    for (int k = 0; k < D; ++k) Wbar[1 + k] += vnbar * n[k];
+    const double c2 = gas.gamma * W[P] / W[0];
+    if (c2 > 0) {
+        const double c2bar = cbar / (2 * s.c);
+        Wbar[P] += c2bar * gas.gamma / W[0];
+        Wbar[0] -= c2bar * c2 / W[0];
+    }
+}
+
+// HLL's wave speeds and case as hll() computes them: bit 0 S_L from L, bit 1 S_R from L; case 1 F_L, 2 F_R, 0 the HLL state.
+template <int D>
+__device__ __forceinline__ int hll_case(const Side<D>& l, const Side<D>& r, double& SL, double& SR, int& pattern) {
+    const double sl = l.vn - l.c, sr = r.vn - r.c, tl = l.vn + l.c, tr = r.vn + r.c;
+    SL = fmin(sl, sr);
+    SR = fmax(tl, tr);
+    pattern = (sr < sl ? 0 : 1) | (tr > tl ? 0 : 2);
+    return SL >= 0.0 ? 1 : SR <= 0.0 ? 2 : 0;
+}
+
+// euler.hpp:466-505: F from the same intermediates and dF = J (dL, dR), n and w fixed.
+template <int D>
+__device__ __forceinline__ void hll_tangent(const Gas& gas, const double* L, const double* R, const double* dL, const double* dR, const double* n,
+                                            const double* w, double* F, double* dF) {
+    Side<D> l, r, dl, dr;
+    side_full<D>(gas, L, n, w, l);
+    side_full<D>(gas, R, n, w, r);
+    side_full_tangent<D>(gas, L, dL, n, l, dl);
+    side_full_tangent<D>(gas, R, dR, n, r, dr);
+    double SL, SR;
+    int pattern;
+    const int which = hll_case<D>(l, r, SL, SR, pattern);
+    const double dSL = (pattern & 1) ? dl.vn - dl.c : dr.vn - dr.c, dSR = (pattern & 2) ? dl.vn + dl.c : dr.vn + dr.c;
+    if (which == 1) {
+#pragma unroll
+        for (int v = 0; v < D + 2; ++v) F[v] = l.F[v], dF[v] = dl.F[v];
+    } else if (which == 2) {
+#pragma unroll
+        for (int v = 0; v < D + 2; ++v) F[v] = r.F[v], dF[v] = dr.F[v];
+    } else {
+        const double inv_dS = 1.0 / (SR - SL), dinv = -inv_dS * inv_dS * (dSR - dSL);
+#pragma unroll
+        for (int v = 0; v < D + 2; ++v) {
+            const double N = SR * l.F[v] - SL * r.F[v] + SL * SR * (r.U[v] - l.U[v]);
+            const double dN = dSR * l.F[v] + SR * dl.F[v] - dSL * r.F[v] - SL * dr.F[v] + (dSL * SR + SL * dSR) * (r.U[v] - l.U[v]) +
+                              SL * SR * (dr.U[v] - dl.U[v]);
+            F[v] = N * inv_dS;
+            dF[v] = dN * inv_dS + N * dinv;
+        }
+    }
+    to_lab<D>(F, w);
+    to_lab<D>(dF, w);
+}
+
+// euler.hpp:507-540: adds J^T Fbar of hll() to Lbar and Rbar.
+template <int D>
+__device__ __forceinline__ void hll_adjoint(const Gas& gas, const double* L, const double* R, const double* n, const double* w, const double* Fbar_lab,
+                                            double* Lbar, double* Rbar) {
+    Side<D> l, r;
+    side_full<D>(gas, L, n, w, l);
+    side_full<D>(gas, R, n, w, r);
+    double SL, SR, Fbar[D + 2], zero[D + 2];
+    int pattern;
+    const int which = hll_case<D>(l, r, SL, SR, pattern);
+    to_lab_adjoint<D>(Fbar_lab, w, Fbar);
+#pragma unroll
+    for (int v = 0; v < D + 2; ++v) zero[v] = 0;
+    if (which == 1) {
+        side_full_adjoint<D>(gas, L, n, l, Fbar, zero, 0.0, 0.0, Lbar);
+        return;
+    }
+    if (which == 2) {
+        side_full_adjoint<D>(gas, R, n, r, Fbar, zero, 0.0, 0.0, Rbar);
+        return;
+    }
+    const double inv_dS = 1.0 / (SR - SL);
+    double lF[D + 2], rF[D + 2], lU[D + 2], rU[D + 2], SLbar = 0, SRbar = 0, inv_bar = 0;
+#pragma unroll
+    for (int v = 0; v < D + 2; ++v) {
+        const double dU = r.U[v] - l.U[v];
+        const double N = SR * l.F[v] - SL * r.F[v] + SL * SR * dU;
+        const double Nbar = Fbar[v] * inv_dS;
+        inv_bar += Fbar[v] * N;
+        SRbar += Nbar * (l.F[v] + SL * dU);
+        SLbar += Nbar * (SR * dU - r.F[v]);
+        lF[v] = Nbar * SR;
+        rF[v] = -Nbar * SL;
+        rU[v] = Nbar * (SL * SR);
+        lU[v] = -Nbar * (SL * SR);
+    }
+    const double gapbar = -inv_dS * inv_dS * inv_bar;  // inv_dS = 1 / (SR - SL)
+    SRbar += gapbar;
+    SLbar -= gapbar;
+    double lvn = 0, lc = 0, rvn = 0, rc = 0;
+    if (pattern & 1)
+        lvn += SLbar, lc -= SLbar;
+    else
+        rvn += SLbar, rc -= SLbar;
+    if (pattern & 2)
+        lvn += SRbar, lc += SRbar;
+    else
+        rvn += SRbar, rc += SRbar;
+    side_full_adjoint<D>(gas, L, n, l, lF, lU, lvn, lc, Lbar);
+    side_full_adjoint<D>(gas, R, n, r, rF, rU, rvn, rc, Rbar);
+}
+
+// The flux derivatives of the solver in use (0 HLLC, 1 HLL).
+template <int D>
+__device__ __forceinline__ void riemann_tangent(int kind, const Gas& gas, const double* L, const double* R, const double* dL, const double* dR,
+                                                const double* n, const double* w, double* F, double* dF) {
+    if (kind == 1)
+        hll_tangent<D>(gas, L, R, dL, dR, n, w, F, dF);
+    else
+        hllc_tangent<D>(gas, L, R, dL, dR, n, w, F, dF);
+}
+template <int D>
+__device__ __forceinline__ void riemann_adjoint(int kind, const Gas& gas, const double* L, const double* R, const double* n, const double* w,
+                                                const double* Fbar, double* Lbar, double* Rbar) {
+    if (kind == 1)
+        hll_adjoint<D>(gas, L, R, n, w, Fbar, Lbar, Rbar);
+    else
+        hllc_adjoint<D>(gas, L, R, n, w, Fbar, Lbar, Rbar);
 }
 
 }  // namespace luna

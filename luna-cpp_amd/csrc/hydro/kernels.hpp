@@ -84,7 +84,7 @@ hipError_t launch_update(hipStream_t s, const Geo& g, const Rows& r, const Gas& 
 template <int D>
 hipError_t launch_keep_centroids(hipStream_t s, const Geo& g, const Rows& r, const Grid& grid, double dt);
 
-// Derivatives of the static order-1 HLLC step at fixed dt (adjoint_kernels.hip). Device rows:
+// Derivatives of the static order-1 step (HLLC or HLL) at fixed dt (adjoint_kernels.hip). Device rows:
 //   dq       N x V   in: the tangent of Q (tangent) or Q bar after the step (adjoint); the tangent advances it to J dq
 //   dW       N x V   W tangents (tangent) or W bars (adjoint)
 //   dgrad    N x grad_row<D>()   gradient tangents
@@ -106,9 +106,9 @@ struct Derivs {
 // The tangent writes Q' into r.q, its flux rows into r.flux and the limited gradients into r.grad; the adjoint needs the
 // primitive rows and the limited gradients of Q (the sweep) and leaves Q as it is.
 template <int D>
-hipError_t launch_step_tangent(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double dt, bool limit, const Derivs& w);
+hipError_t launch_step_tangent(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, bool limit, const Derivs& w);
 template <int D>
-hipError_t launch_step_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double dt, bool limit, const Derivs& w);
+hipError_t launch_step_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, bool limit, const Derivs& w);
 
 // D = 1: the static sorted line.
 struct LineArgs {
@@ -122,7 +122,7 @@ struct LineArgs {
 };
 hipError_t launch_line_time_step(hipStream_t s, const LineArgs& a, double courant, int luna_rule, const Counters& c);
 hipError_t launch_line_step(hipStream_t s, const LineArgs& a, double dt, double courant, bool fused, const Counters& c);
-// Derivatives of the static step at fixed dt (HLLC): the tangent writes the step into a.next and J dq into dnext; the
+// Derivatives of the static step at fixed dt: the tangent writes the step into a.next and J dq into dnext; the
 // adjoint writes J^T qbar (q: the state before the step) into out.
 hipError_t launch_line_tangent(hipStream_t s, const LineArgs& a, double dt, const double* dq, double* dnext);
 hipError_t launch_line_adjoint(hipStream_t s, const LineArgs& a, double dt, const double* qbar, double* out);

@@ -443,7 +443,7 @@ __device__ __forceinline__ void face_tangent(const LineArgs& l, const Cell* cell
             ghost_tangent(l, dW[a], false, dRs);
     }
     const double one[1] = {1.0}, zero[1] = {0.0};
-    hllc_tangent<1>(l.gas, L, R, dLs, dRs, one, zero, F, dF);
+    riemann_tangent<1>(l.riemann, l.gas, L, R, dLs, dRs, one, zero, F, dF);
 }
 
 // finite_volume_1d.hpp:794-830 face_adjoint(): adds J^T Fbar of the face between positions a and a + 1 to the bars of the
@@ -463,7 +463,7 @@ __device__ __forceinline__ void face_adjoint(const LineArgs& l, const Cell* cell
     if (!cell[b].real) ghost_face(l, L, cell[a].W, false, R);
     double Lbar[3] = {0, 0, 0}, Rbar[3] = {0, 0, 0};
     const double one[1] = {1.0}, zero[1] = {0.0};
-    hllc_adjoint<1>(l.gas, L, R, one, zero, Fbar, Lbar, Rbar);
+    riemann_adjoint<1>(l.riemann, l.gas, L, R, one, zero, Fbar, Lbar, Rbar);
     // the ghost states in reverse order of the forward: R from L, then L from R
     if (!cell[b].real) {
         double back[3];

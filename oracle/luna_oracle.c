@@ -838,6 +838,10 @@ double orc_line_step_fused(orc_line* l, double dt, double courant) {
 }
 
 /* ------------------------------------------------------------------ derivatives of a static step --------------------------- */
+static void riemann_tangent(int kind, int D, const orc_gas* gas, const double* L, const double* R, const double* dL, const double* dR,
+                            const double* n, const double* w, double* F, double* dF);
+static void riemann_adjoint(int kind, int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w,
+                            const double* Fbar, double* Lbar, double* Rbar);
 /* Tangent: J dx; adjoint: J^T ybar (added into the bars). dt and the geometry are fixed. At every switch the forward takes
  * (a floor, a wave case, a guard, a limiter clip, a positivity fallback) the derivative is that of the branch it took,
  * as euler.hpp:115-117 states for the reference's own derivatives. Written from the forward arithmetic above by the
@@ -1438,7 +1442,7 @@ static void line_face_tangent(const orc_line* l, const line_cell cell[5], const 
             line_ghost_tangent(l, dW[a], 0, dRs);
     }
     const double one = 1.0, zero = 0.0;
-    orc_hllc_tangent(1, &l->gas, L, R, dLs, dRs, &one, &zero, F, dF);
+    riemann_tangent(l->riemann, 1, &l->gas, L, R, dLs, dRs, &one, &zero, F, dF);
 }
 
 /* finite_volume_1d.hpp:794-830 face_adjoint(): adds J^T Fbar of the face between positions a and a + 1 to the bars of the
@@ -1458,7 +1462,7 @@ static void line_face_adjoint(const orc_line* l, const line_cell cell[5], const 
     if (!cell[b].real) line_ghost_face(l, L, cell[a].W, 0, R);
     double Lbar[3] = {0, 0, 0}, Rbar[3] = {0, 0, 0};
     const double one = 1.0, zero = 0.0;
-    orc_hllc_adjoint(1, &l->gas, L, R, &one, &zero, Fbar, Lbar, Rbar);
+    riemann_adjoint(l->riemann, 1, &l->gas, L, R, &one, &zero, Fbar, Lbar, Rbar);
     /* the ghost states in reverse order of the forward: R from L, then L from R */
     if (!cell[b].real) {
         double back[3];
@@ -1787,7 +1791,7 @@ static void face_sides(const orc_fv* fv, const orc_geometry* geo, int f, double 
     }
 }
 
-static int static_derivatives_ok(const orc_fv* fv) { return !fv->moving && fv->order == 1 && fv->riemann == 0 && fv->D >= 2; }
+static int static_derivatives_ok(const orc_fv* fv) { return !fv->moving && fv->order == 1 && fv->D >= 2; }
 
 /* finite_volume.hpp:2039-2074 step_tangent(): the static step of q and dq -> J dq (dq: N x V, extensive), dt fixed.
  * Returns 0, or -1 outside a static order-1 HLLC mesh step. */
@@ -1840,7 +1844,7 @@ int orc_step_tangent(orc_fv* fv, const orc_geometry* geo, double dt, double* dq)
             for (int v = 0; v < V; ++v) outs[s][v] += dot(D, dgi + v * D, rs[s]);
         }
         const double w0[MAXD] = {0, 0, 0};
-        orc_hllc_tangent(D, &fv->gas, L, R, dLs, dRs, n, w0, Fv, dF);
+        riemann_tangent(fv->riemann, D, &fv->gas, L, R, dLs, dRs, n, w0, Fv, dF);
         for (int v = 0; v < V; ++v) {
             Fv[v] *= magnitude;
             dF[v] *= magnitude;
@@ -1898,7 +1902,7 @@ int orc_step_adjoint(orc_fv* fv, const orc_geometry* geo, double dt, double* qba
         const int o = geo->owner[f], j = geo->neighbour[f];
         for (int v = 0; v < V; ++v) Fbar[v] = (dt * qbar[(size_t)j * V + v] - dt * qbar[(size_t)o * V + v]) * magnitude;
         const double w0[MAXD] = {0, 0, 0};
-        orc_hllc_adjoint(D, &fv->gas, L, R, n, w0, Fbar, Lbar, Rbar);
+        riemann_adjoint(fv->riemann, D, &fv->gas, L, R, n, w0, Fbar, Lbar, Rbar);
     }
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < N; ++i) {
@@ -1993,4 +1997,131 @@ int orc_step_adjoint(orc_fv* fv, const orc_geometry* geo, double dt, double* qba
     free(boundbar);
     free(extreme);
     return 0;
+}
+
+/* ---- HLL (euler.hpp:444-540): tangent and adjoint, of the case the forward took ---- */
+
+/* euler.hpp:377-385: the tangent of side(), the sound speed's included (zero where c^2 <= 0, as the forward). */
+static void side_full_tangent(int D, const orc_gas* gas, const double* W, const double* dW, const double* n, const side_t* s, side_t* d) {
+    const int P = D + 1;
+    double dv[MAXD];
+    for (int k = 0; k < D; ++k) dv[k] = dW[1 + k];
+    side_with_tangent(D, gas, W, dW, n, s, dot(D, dv, n), d);
+    const double c2 = gas->gamma * W[P] / W[0];
+    d->c = c2 > 0 ? (gas->gamma * dW[P] - c2 * dW[0]) / W[0] / (2 * s->c) : 0.0;
+}
+
+/* euler.hpp:413-426: the adjoint of side() from the bars of F, U, vn and c; adds J^T to Wbar. */
+static void side_full_adjoint(int D, const orc_gas* gas, const double* W, const double* n, const side_t* s, const double* Fbar,
+                              const double* Ubar, double vnbar, double cbar, double* Wbar) {
+    const int P = D + 1;
+    double vbar[MAXD] = {0, 0, 0};
+    vnbar += side_with_adjoint(D, gas, W, n, s, Fbar, Ubar, 0.0, vbar, Wbar);
+    for (int k = 0; k < D; ++k) Wbar[1 + k] += vnbar * n[k];
+    const double c2 = gas->gamma * W[P] / W[0];
+    if (c2 > 0) {
+        const double c2bar = cbar / (2 * s->c);
+        Wbar[P] += c2bar * gas->gamma / W[0];
+        Wbar[0] -= c2bar * c2 / W[0];
+    }
+}
+
+/* HLL's wave speeds and case as orc_hll computes them: bit 0 S_L from L, bit 1 S_R from L; case 1 F_L, 2 F_R, 0 the HLL state. */
+static int hll_case(const side_t* l, const side_t* r, double* SL, double* SR, int* pattern) {
+    const double sl = l->vn - l->c, sr = r->vn - r->c, tl = l->vn + l->c, tr = r->vn + r->c;
+    *SL = fmin(sl, sr);
+    *SR = fmax(tl, tr);
+    *pattern = (sr < sl ? 0 : 1) | (tr > tl ? 0 : 2);
+    return *SL >= 0.0 ? 1 : *SR <= 0.0 ? 2 : 0;
+}
+
+/* euler.hpp:466-505 HLL::flux_tangent(): F from the same intermediates and dF = J (dL, dR), n and w fixed. */
+void orc_hll_tangent(int D, const orc_gas* gas, const double* L, const double* R, const double* dL, const double* dR, const double* n,
+                     const double* w, double* F, double* dF) {
+    side_t l, r, dl, dr;
+    side_full(D, gas, L, n, w, &l);
+    side_full(D, gas, R, n, w, &r);
+    side_full_tangent(D, gas, L, dL, n, &l, &dl);
+    side_full_tangent(D, gas, R, dR, n, &r, &dr);
+    double SL, SR;
+    int pattern;
+    const int which = hll_case(&l, &r, &SL, &SR, &pattern);
+    const double dSL = (pattern & 1) ? dl.vn - dl.c : dr.vn - dr.c, dSR = (pattern & 2) ? dl.vn + dl.c : dr.vn + dr.c;
+    if (which == 1) {
+        for (int v = 0; v < D + 2; ++v) F[v] = l.F[v], dF[v] = dl.F[v];
+    } else if (which == 2) {
+        for (int v = 0; v < D + 2; ++v) F[v] = r.F[v], dF[v] = dr.F[v];
+    } else {
+        const double inv_dS = 1.0 / (SR - SL), dinv = -inv_dS * inv_dS * (dSR - dSL);
+        for (int v = 0; v < D + 2; ++v) {
+            const double N = SR * l.F[v] - SL * r.F[v] + SL * SR * (r.U[v] - l.U[v]);
+            const double dN = dSR * l.F[v] + SR * dl.F[v] - dSL * r.F[v] - SL * dr.F[v] + (dSL * SR + SL * dSR) * (r.U[v] - l.U[v]) +
+                              SL * SR * (dr.U[v] - dl.U[v]);
+            F[v] = N * inv_dS;
+            dF[v] = dN * inv_dS + N * dinv;
+        }
+    }
+    to_lab(D, F, w);
+    to_lab(D, dF, w);
+}
+
+/* euler.hpp:507-540 HLL::flux_adjoint(): adds J^T Fbar to Lbar and Rbar. */
+void orc_hll_adjoint(int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, const double* Fbar_lab,
+                     double* Lbar, double* Rbar) {
+    side_t l, r;
+    side_full(D, gas, L, n, w, &l);
+    side_full(D, gas, R, n, w, &r);
+    double SL, SR, Fbar[MAXV];
+    int pattern;
+    const int which = hll_case(&l, &r, &SL, &SR, &pattern);
+    to_lab_adjoint(D, Fbar_lab, w, Fbar);
+    const double zero[MAXV] = {0, 0, 0, 0, 0};
+    if (which == 1) {
+        side_full_adjoint(D, gas, L, n, &l, Fbar, zero, 0.0, 0.0, Lbar);
+        return;
+    }
+    if (which == 2) {
+        side_full_adjoint(D, gas, R, n, &r, Fbar, zero, 0.0, 0.0, Rbar);
+        return;
+    }
+    const double inv_dS = 1.0 / (SR - SL);
+    double lF[MAXV], rF[MAXV], lU[MAXV], rU[MAXV], SLbar = 0, SRbar = 0, inv_bar = 0;
+    for (int v = 0; v < D + 2; ++v) {
+        const double dU = r.U[v] - l.U[v];
+        const double N = SR * l.F[v] - SL * r.F[v] + SL * SR * dU;
+        const double Nbar = Fbar[v] * inv_dS;
+        inv_bar += Fbar[v] * N;
+        SRbar += Nbar * (l.F[v] + SL * dU);
+        SLbar += Nbar * (SR * dU - r.F[v]);
+        lF[v] = Nbar * SR;
+        rF[v] = -Nbar * SL;
+        rU[v] = Nbar * (SL * SR);
+        lU[v] = -Nbar * (SL * SR);
+    }
+    const double gapbar = -inv_dS * inv_dS * inv_bar; /* inv_dS = 1 / (SR - SL) */
+    SRbar += gapbar;
+    SLbar -= gapbar;
+    double lvn = 0, lc = 0, rvn = 0, rc = 0;
+    if (pattern & 1) lvn += SLbar, lc -= SLbar;
+    else rvn += SLbar, rc -= SLbar;
+    if (pattern & 2) lvn += SRbar, lc += SRbar;
+    else rvn += SRbar, rc += SRbar;
+    side_full_adjoint(D, gas, L, n, &l, lF, lU, lvn, lc, Lbar);
+    side_full_adjoint(D, gas, R, n, &r, rF, rU, rvn, rc, Rbar);
+}
+
+/* The flux derivative of the solver in use (0 HLLC, 1 HLL). */
+static void riemann_tangent(int kind, int D, const orc_gas* gas, const double* L, const double* R, const double* dL, const double* dR,
+                            const double* n, const double* w, double* F, double* dF) {
+    if (kind == 1)
+        orc_hll_tangent(D, gas, L, R, dL, dR, n, w, F, dF);
+    else
+        orc_hllc_tangent(D, gas, L, R, dL, dR, n, w, F, dF);
+}
+static void riemann_adjoint(int kind, int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w,
+                            const double* Fbar, double* Lbar, double* Rbar) {
+    if (kind == 1)
+        orc_hll_adjoint(D, gas, L, R, n, w, Fbar, Lbar, Rbar);
+    else
+        orc_hllc_adjoint(D, gas, L, R, n, w, Fbar, Lbar, Rbar);
 }

@@ -75,9 +75,9 @@ double orc_time_step(orc_fv* fv, const orc_geometry* geo, double courant);
 void orc_step_static(orc_fv* fv, const orc_geometry* geo, double dt);
 void orc_moving_begin(orc_fv* fv, const orc_geometry* geo, double dt);
 void orc_moving_finish(orc_fv* fv, const orc_geometry* geo, double dt);
-/* Derivatives of the static order-1 HLLC step at fixed dt (finite_volume.hpp:1670-2231): the tangent advances q and dq (J dq)
+/* Derivatives of the static order-1 step (HLLC or HLL) at fixed dt (finite_volume.hpp:1670-2231): the tangent advances q and dq (J dq)
  * in place; the adjoint maps qbar after the step to J^T qbar before it (q, the state before the step, stays). Return -1
- * for a moving mesh, order 0, HLL or D = 1 (see orc_line_*). */
+ * for a moving mesh, order 0 or D = 1 (see orc_line_*). */
 int orc_step_tangent(orc_fv* fv, const orc_geometry* geo, double dt, double* dq);
 int orc_step_adjoint(orc_fv* fv, const orc_geometry* geo, double dt, double* qbar);
 long long orc_snap(double x, double step, long long period);
@@ -113,6 +113,10 @@ void orc_hllc_tangent(int D, const orc_gas* gas, const double* L, const double* 
                       const double* w, double* F, double* dF);
 void orc_hllc_adjoint(int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, const double* Fbar,
                       double* Lbar, double* Rbar);
+void orc_hll_tangent(int D, const orc_gas* gas, const double* L, const double* R, const double* dL, const double* dR, const double* n,
+                     const double* w, double* F, double* dF);
+void orc_hll_adjoint(int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, const double* Fbar,
+                     double* Lbar, double* Rbar);
 double orc_line_centroid(const orc_line* line, int s);
 
 #ifdef __cplusplus

@@ -144,21 +144,21 @@ def half_step_adjoint(W, grad, dt, hbar, gamma):
     return Wbar, gbar
 
 
-def hllc_tangent(L, R, dL, dR, n, w, gamma):
-    """Returns (F, dF): the HLLC flux from the tangent's own intermediates and J (dL, dR)."""
+def hllc_tangent(L, R, dL, dR, n, w, gamma, riemann="hllc"):
+    """Returns (F, dF): the flux (HLLC or HLL) from the tangent's own intermediates and J (dL, dR)."""
     L, R, dL, dR, n, w = _vec(L), _vec(R), _vec(dL), _vec(dR), _vec(n), _vec(w)
     F, dF = np.empty_like(L), np.empty_like(L)
     g = gas(gamma)
-    lib().orc_hllc_tangent(len(L) - 2, ctypes.byref(g), _ptr(L), _ptr(R), _ptr(dL), _ptr(dR), _ptr(n), _ptr(w), _ptr(F), _ptr(dF))
+    (lib().orc_hll_tangent if riemann == "hll" else lib().orc_hllc_tangent)(len(L) - 2, ctypes.byref(g), _ptr(L), _ptr(R), _ptr(dL), _ptr(dR), _ptr(n), _ptr(w), _ptr(F), _ptr(dF))
     return F, dF
 
 
-def hllc_adjoint(L, R, n, w, Fbar, gamma):
-    """Returns (Lbar, Rbar) = J^T Fbar."""
+def hllc_adjoint(L, R, n, w, Fbar, gamma, riemann="hllc"):
+    """Returns (Lbar, Rbar) = J^T Fbar of the flux (HLLC or HLL)."""
     L, R, n, w, Fbar = _vec(L), _vec(R), _vec(n), _vec(w), _vec(Fbar)
     Lbar, Rbar = np.zeros_like(L), np.zeros_like(R)
     g = gas(gamma)
-    lib().orc_hllc_adjoint(len(L) - 2, ctypes.byref(g), _ptr(L), _ptr(R), _ptr(n), _ptr(w), _ptr(Fbar), _ptr(Lbar), _ptr(Rbar))
+    (lib().orc_hll_adjoint if riemann == "hll" else lib().orc_hllc_adjoint)(len(L) - 2, ctypes.byref(g), _ptr(L), _ptr(R), _ptr(n), _ptr(w), _ptr(Fbar), _ptr(Lbar), _ptr(Rbar))
     return Lbar, Rbar
 
 
@@ -230,17 +230,17 @@ class FiniteVolume:
         lib().orc_step_static(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(dt))
 
     def step_tangent(self, dt, dq):
-        """The static order-1 HLLC step of q and of its tangent dq (N x V, extensive): returns J dq; q advances."""
+        """The static order-1 step of q and of its tangent dq (N x V, extensive): returns J dq; q advances."""
         dq = np.ascontiguousarray(np.array(dq, dtype=np.float64))
         if lib().orc_step_tangent(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(dt), _ptr(dq)) != 0:
-            raise ValueError("step_tangent: a static mesh, order 1, HLLC")
+            raise ValueError("step_tangent: a static mesh, order 1")
         return dq
 
     def step_adjoint(self, dt, qbar):
-        """J^T qbar of the static order-1 HLLC step from the current q (which stays)."""
+        """J^T qbar of the static order-1 step from the current q (which stays)."""
         qbar = np.ascontiguousarray(np.array(qbar, dtype=np.float64))
         if lib().orc_step_adjoint(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(dt), _ptr(qbar)) != 0:
-            raise ValueError("step_adjoint: a static mesh, order 1, HLLC")
+            raise ValueError("step_adjoint: a static mesh, order 1")
         return qbar
 
     def moving_begin(self, dt):

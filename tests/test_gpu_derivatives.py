@@ -73,20 +73,21 @@ def test_line_adjoint_matches_oracle_and_luna(case):
         assert close(bg * V[:, None], np.array(c["JTw"])) < 1e-8
 
 
-@pytest.mark.parametrize("walls,periodic,limiter,order", [
-    (("reflective", "reflective"), False, "face", 1),
-    (("outflow", "reflective"), False, "vanleer", 1),
-    (("outflow", "outflow"), True, "minmod", 1),
-    (("reflective", "outflow"), False, "face", 0),
+@pytest.mark.parametrize("walls,periodic,limiter,order,riemann", [
+    (("reflective", "reflective"), False, "face", 1, "hllc"),
+    (("outflow", "reflective"), False, "vanleer", 1, "hllc"),
+    (("outflow", "outflow"), True, "minmod", 1, "hllc"),
+    (("reflective", "outflow"), False, "face", 0, "hllc"),
+    (("reflective", "outflow"), False, "face", 1, "hll"),
 ])
-def test_line_derivatives_walls_dot_product(walls, periodic, limiter, order):
+def test_line_derivatives_walls_dot_product(walls, periodic, limiter, order, riemann):
     """Rough jittered lines with each wall: GPU tangent and adjoint against the oracle's, and the dot test
     <qbar, J dq> = <J^T qbar, dq> on the GPU."""
     rng = np.random.default_rng(5)
     n = 200
     x = problems.line_points(n, 1.0, 0.3, 2)
     keys, step, period = problems.line_keys(x, 1.0)
-    kw = dict(periodic=periodic, walls=walls, limiter=limiter, order=order)
+    kw = dict(periodic=periodic, walls=walls, limiter=limiter, order=order, riemann=riemann)
     gpu = hydro.Line(keys, period, step, 1.4, **kw)
     orc = oracle.Line(keys, period, step, 1.4, **kw)
     V = np.array([orc.volume(s) for s in range(n)])
@@ -124,14 +125,15 @@ def mesh_case(dim, n, seed=1, amp=0.3):
     return m, q, rng
 
 
-@pytest.mark.parametrize("dim,n,limiter", [(3, 10, "face"), (3, 8, "none"), (2, 24, "face"), (2, 24, "none")])
-def test_mesh_derivatives_match_oracle(dim, n, limiter):
+@pytest.mark.parametrize("dim,n,limiter,riemann", [(3, 10, "face", "hllc"), (3, 8, "none", "hllc"), (3, 8, "face", "hll"),
+                                                   (2, 24, "face", "hllc"), (2, 24, "none", "hll")])
+def test_mesh_derivatives_match_oracle(dim, n, limiter, riemann):
     """The static order-1 step's tangent and adjoint on a rough jittered Voronoi mesh: the GPU's against the oracle's
     (1e-12 relative), the GPU adjoint as the transpose of the GPU tangent (dot test), and J^T 1 = 1."""
     m, q, rng = mesh_case(dim, n)
     N, V = q.shape
-    gpu = hydro.FiniteVolume(m, 5.0 / 3.0, limiter=limiter)
-    orc = oracle.FiniteVolume(m, 5.0 / 3.0, limiter=limiter)
+    gpu = hydro.FiniteVolume(m, 5.0 / 3.0, limiter=limiter, riemann=riemann)
+    orc = oracle.FiniteVolume(m, 5.0 / 3.0, limiter=limiter, riemann=riemann)
     orc.set_state(q)
     dt = 0.3 * orc.time_step(0.3)
     dq, qbar = rng.uniform(-1, 1, (N, V)), rng.uniform(-1, 1, (N, V))
@@ -148,7 +150,7 @@ def test_mesh_derivatives_match_oracle(dim, n, limiter):
     lhs, rhs = np.sum(qbar * Jg), np.sum(Tg * dq)
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
     assert np.array_equal(gpu.step_adjoint(dt, np.ones((N, V))), np.ones((N, V)))
-    say("%dD n=%d %s: tangent bits %s, adjoint bits %s, dot %.2e" % (dim, n, limiter, np.array_equal(Jg, Jo), np.array_equal(Tg, To),
+    say("%dD n=%d %s %s: tangent bits %s, adjoint bits %s, dot %.2e" % (dim, n, limiter, riemann, np.array_equal(Jg, Jo), np.array_equal(Tg, To),
                                                                    abs(lhs - rhs) / abs(lhs)))
 
 

@@ -240,3 +240,64 @@ def test_planted_adjoint_cell_swap_fails_dot_test():
     T = fv.step_adjoint(dt, qbar)
     T[[3, 17]] = T[[17, 3]]
     assert abs(np.sum(qbar * J) - np.sum(T * dq)) > 1e-6
+
+
+@pytest.mark.parametrize("case", sorted(HLLC_CASES))
+def test_hll_derivatives_fd_and_transpose(case):
+    """HLL::flux_tangent and flux_adjoint (euler.hpp:466-540) in each case (F_L, the HLL state, F_R) on a moving face."""
+    rng = np.random.default_rng(9)
+    gamma = 5.0 / 3.0
+    L, R = (np.array(x) for x in HLLC_CASES[case])
+    n = np.array([0.6, 0.8, 0.0])
+    w = np.array([0.05, -0.02, 0.01])
+    dL, dR = rng_unit(rng, 5), rng_unit(rng, 5)
+    F, dF = oracle.hllc_tangent(L, R, dL, dR, n, w, gamma, riemann="hll")
+    assert np.array_equal(F, oracle.flux(L, R, n, w, gamma, riemann="hll"))
+    h = 1e-6
+    fd = (oracle.flux(L + h * dL, R + h * dR, n, w, gamma, "hll") - oracle.flux(L - h * dL, R - h * dR, n, w, gamma, "hll")) / (2 * h)
+    assert np.allclose(dF, fd, rtol=1e-7, atol=1e-8), (dF, fd)
+    Fbar = rng_unit(rng, 5)
+    Lbar, Rbar = oracle.hllc_adjoint(L, R, n, w, Fbar, gamma, riemann="hll")
+    assert abs(Fbar @ dF - (Lbar @ dL + Rbar @ dR)) < 1e-12 * (1 + abs(Fbar @ dF))
+
+
+@pytest.mark.parametrize("dim,n", [(3, 6), (2, 12)])
+def test_mesh_step_derivatives_hll(dim, n):
+    m, q, rng = mesh_state(dim, n)
+    N, V = q.shape
+    fv = oracle.FiniteVolume(m, 5.0 / 3.0, riemann="hll")
+    fv.set_state(q)
+    dt = 0.3 * fv.time_step(0.3)
+    dq, qbar = rng.uniform(-1, 1, (N, V)), rng.uniform(-1, 1, (N, V))
+    fv.set_state(q)
+    J = fv.step_tangent(dt, dq)
+    assert np.array_equal(fv.q, forward(fv, q, dt))
+    fv.set_state(q)
+    T = fv.step_adjoint(dt, qbar)
+    assert abs(np.sum(qbar * J) - np.sum(T * dq)) <= 1e-13 * abs(np.sum(qbar * J))
+    h = 1e-7
+    fd = (forward(fv, q + h * dq, dt) - forward(fv, q - h * dq, dt)) / (2 * h)
+    assert np.max(np.abs(fd - J)) / np.max(np.abs(J)) < 1e-6
+
+
+def test_line_step_derivatives_hll():
+    rng = np.random.default_rng(6)
+    n = 120
+    keys, step, period = problems.line_keys(problems.line_points(n, 1.0, 0.3, 2), 1.0)
+    line = oracle.Line(keys, period, step, 1.4, walls=("reflective", "outflow"), riemann="hll")
+    V = np.array([line.volume(s) for s in range(n)])
+    W = np.stack([1 + 0.3 * rng.uniform(size=n), 0.4 * rng.uniform(-1, 1, n), 1 + 0.3 * rng.uniform(size=n)], axis=1)
+    q = np.array([oracle.conserved(w, 1.4) for w in W]) * V[:, None]
+    dt, h = 2e-4, 1e-7
+    dq, qbar = rng.uniform(-1, 1, (n, 3)), rng.uniform(-1, 1, (n, 3))
+    line.set_state(q)
+    J = line.step_tangent(dt, dq)
+    line.set_state(q)
+    T = line.step_adjoint(dt, qbar)
+    assert abs(np.sum(qbar * J) - np.sum(T * dq)) <= 1e-13 * abs(np.sum(qbar * J))
+    line.set_state(q + h * dq)
+    line.step(dt)
+    plus = line.state.copy()
+    line.set_state(q - h * dq)
+    line.step(dt)
+    assert np.max(np.abs((plus - line.state) / (2 * h) - J)) / np.max(np.abs(J)) < 1e-6
