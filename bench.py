@@ -27,15 +27,44 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per cell of each kernel: every array a phase needs read once and each output written once
 # (unique bytes; 3D, F/N = 7.95 and E/N = 15.9 as luna-cpp measures them; FP64 values, int32 indices).
 #   sweep+dt: Q-derived rows W (40), cell geometry (V 8, c 24, x 24, CSR 4), list entries 63.6, face records 445.2,
-#             writes g 120 and W 40 -> 768.8 static; moving also writes w (24) -> 792.8
+#             writes g 120 and W 40 -> 768.8 static; moving also writes w (24) and keeps c^n (24) -> 816.8
+#   keep_centroids (moving): x read and snapped (48), w (24), grid steps written (12) -> 84
 #   flux:     faces 445.2, W + g + c + x of the cells 208, writes the flux rows 318 -> 971.2; moving pass 1 also reads
-#             the grid steps (w, 24) -> 995.2; pass 2 also the old centroids (24) -> 1019.2
+#             the grid steps (int32, 12) -> 983.2; pass 2 the steps and the old centroids (24) -> 1007.2
 #   update:   CSR 4, list 63.6, flux rows 318, Q read and written 80 -> 465.6; moving pass 1 with the drift (x r/w 48,
-#             w 24) -> 537.6
+#             grid steps 12) -> 525.6
 ALGO_BYTES = {
     "static": {"sweep": 768.8, "flux": 971.2, "update": 465.6},
-    "moving": {"sweep": 792.8, "flux_pass1": 995.2, "update_pass1": 537.6, "flux_pass2": 1019.2, "update_pass2": 465.6},
+    "moving": {"sweep": 816.8, "keep_centroids": 84.0, "flux_pass1": 983.2, "update_pass1": 525.6, "flux_pass2": 1007.2,
+               "update_pass2": 465.6},
 }
+
+# The kernel instantiation behind each phase (3D, HLLC, MUSCL-Hancock with the face-centroid limiter), as rocprofv3 names
+# it; k_update<3, true, false> is both the static update and the moving second pass.
+PHASE_KERNEL = {
+    "static": {"sweep": "k_sweep<3, false, true, true, 0>", "flux": "k_flux<3, -1, 0>", "update": "k_update<3, true, false>"},
+    "moving": {"sweep": "k_sweep<3, true, true, true, 0>", "keep_centroids": "k_keep_centroids<3>",
+               "flux_pass1": "k_flux<3, 0, 0>", "update_pass1": "k_update<3, false, true>", "flux_pass2": "k_flux<3, 1, 0>",
+               "update_pass2": "k_update<3, true, false>"},
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_sedov128.md")
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at n = 128 from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (its table
+    rows: | `kernel` | ... | bytes per launch |, the counters already scaled by the plain-copy calibration); None when
+    the summary is absent or the workload is another size."""
+    if n != 128 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cols) >= 2 and cols[0] == "`%s`" % kernel:
+                try:
+                    return float(cols[-1])
+                except ValueError:
+                    return None
+    return None
 
 
 def log(msg):
@@ -227,7 +256,7 @@ def main():
                                "two-pass moving step, CFL 0.3, FP64" % (n, N, F), "cells": N, "faces": F,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(PHASE_KERNEL["moving"][name], n),
                      "algorithmic_bytes_per_cell": per_cell, "avg_ms": avg_ms, "launches": launches},
         "cpu_baseline": cpu,
         "step_roofline": {"algorithmic_bytes_per_cell": step_bytes,
@@ -239,7 +268,7 @@ def main():
         "kernels_ms_avg": {k: v[1] / v[0] for k, v in m_times.items()},
         "static": {"value": static_rate, "unit": "cell updates/s", "ms_per_step": 1e3 * s_total / args.static_steps,
                    "step_median_ms": 1e3 * float(np.median(s_walls)), "kernel": sname, "achieved_GBs": s_ach,
-                   "frac": s_ach / HBM_PEAK_GBS, "step_frac": 2205.6 * static_rate / 1e9 / HBM_PEAK_GBS,
+                   "frac": s_ach / HBM_PEAK_GBS, "step_frac": sum(ALGO_BYTES["static"].values()) * static_rate / 1e9 / HBM_PEAK_GBS,
                    "kernels_ms_avg": {k: v[1] / v[0] for k, v in s_times.items()}},
     }
     print(json.dumps(out), flush=True)

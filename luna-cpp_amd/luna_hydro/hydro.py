@@ -64,6 +64,13 @@ def lib():
         path = library_path()
         if not os.path.exists(path):
             raise RuntimeError("libluna_hydro.so is not built (run __graft_entry__.build()); there is no CPU fallback")
+        # A process holds one libamdhip64.so.7, the first one loaded. torch ships its own and does not initialise on
+        # /opt/rocm's ("No HIP GPUs are available"), while this library runs on either: load torch's first when torch
+        # is installed, so that torch tensors and these entry points can share the process in any import order.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(path)
         L.luna_last_error.restype = ctypes.c_char_p
         _LIB = L
