@@ -39,27 +39,19 @@ ALGO_BYTES = {
                "update_pass2": 465.6},
 }
 
-# The kernel instantiation behind each phase (3D, HLLC, MUSCL-Hancock with the face-centroid limiter), as rocprofv3 names
-# it; k_update<3, true, false> is both the static update and the moving second pass.
-PHASE_KERNEL = {
-    "static": {"sweep": "k_sweep<3, false, true, true, 0>", "flux": "k_flux<3, -1, 0>", "update": "k_update<3, true, false>"},
-    "moving": {"sweep": "k_sweep<3, true, true, true, 0>", "keep_centroids": "k_keep_centroids<3>",
-               "flux_pass1": "k_flux<3, 0, 0>", "update_pass1": "k_update<3, false, true>", "flux_pass2": "k_flux<3, 1, 0>",
-               "update_pass2": "k_update<3, true, false>"},
-}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_sedov128.md")
 
 
-def measured_traffic(kernel, n):
-    """HBM bytes per launch of `kernel` at n = 128 from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (its table
-    rows: | `kernel` | ... | bytes per launch |, the counters already scaled by the plain-copy calibration); None when
-    the summary is absent or the workload is another size."""
+def measured_traffic(phase, n):
+    """HBM bytes per launch of a phase ("moving sweep", "static flux", ...) at n = 128 from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (its table rows: | phase | kernel | ... | bytes per launch |, the counters already
+    scaled by the plain-copy calibration); None when the summary is absent or the workload is another size."""
     if n != 128 or not os.path.exists(TRAFFIC_FILE):
         return None
     with open(TRAFFIC_FILE) as f:
         for line in f:
             cols = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cols) >= 2 and cols[0] == "`%s`" % kernel:
+            if len(cols) >= 2 and cols[0] == phase:
                 try:
                     return float(cols[-1])
                 except ValueError:
@@ -256,7 +248,7 @@ def main():
                                "two-pass moving step, CFL 0.3, FP64" % (n, N, F), "cells": N, "faces": F,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(PHASE_KERNEL["moving"][name], n),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("moving " + name, n),
                      "algorithmic_bytes_per_cell": per_cell, "avg_ms": avg_ms, "launches": launches},
         "cpu_baseline": cpu,
         "step_roofline": {"algorithmic_bytes_per_cell": step_bytes,
