@@ -26,8 +26,9 @@ struct Gas {
     double gamma = 5.0 / 3.0, smallr = 1e-10, smallc = 1e-10;
 };
 
+// learned: this build's learned limiter (no counterpart in the reference; 2D/3D, order 1; luna_hydro.h).
 enum class Limiter { face_centroid = LUNA_LIMITER_FACE_CENTROID, none = LUNA_LIMITER_NONE, minmod = LUNA_LIMITER_MINMOD,
-                     van_leer = LUNA_LIMITER_VAN_LEER };
+                     van_leer = LUNA_LIMITER_VAN_LEER, learned = LUNA_LIMITER_LEARNED };
 enum class Motion { static_mesh = LUNA_MOTION_STATIC, moving = LUNA_MOTION_MOVING };
 enum class Riemann { hllc = LUNA_RIEMANN_HLLC, hll = LUNA_RIEMANN_HLL };
 enum class StepRule { arepo = LUNA_STEP_AREPO, luna = LUNA_STEP_LUNA };
@@ -74,6 +75,11 @@ public:
         return q;
     }
     void set_state(const std::vector<double>& q) { check(luna_fv_set_state(fv_, q.data()), "set_state"); }
+    // The learned limiter's network (LUNA_LIMITER_PARAMS FP32 values; empty: none, the face-centroid limiter's gradients).
+    void set_limiter_network(const std::vector<float>& params) {
+        check(luna_fv_set_limiter_network(fv_, params.empty() ? nullptr : params.data(), static_cast<int>(params.size())),
+              "set_limiter_network");
+    }
     double* state_device() {
         double* q = nullptr;
         check(luna_fv_state_device(fv_, &q), "state_device");

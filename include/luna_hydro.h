@@ -22,8 +22,8 @@ enum {
     LUNA_ERR_MEMORY = 4    /* device allocation failed */
 };
 
-/* finite_volume.hpp:25-34 */
-enum { LUNA_LIMITER_FACE_CENTROID = 0, LUNA_LIMITER_NONE = 1, LUNA_LIMITER_MINMOD = 2, LUNA_LIMITER_VAN_LEER = 3 };
+/* finite_volume.hpp:25-34; LUNA_LIMITER_LEARNED (2D/3D, order 1) has no counterpart in the reference: see below. */
+enum { LUNA_LIMITER_FACE_CENTROID = 0, LUNA_LIMITER_NONE = 1, LUNA_LIMITER_MINMOD = 2, LUNA_LIMITER_VAN_LEER = 3, LUNA_LIMITER_LEARNED = 4 };
 enum { LUNA_RIEMANN_HLLC = 0, LUNA_RIEMANN_HLL = 1 };
 enum { LUNA_MOTION_STATIC = 0, LUNA_MOTION_MOVING = 1 };
 enum { LUNA_STEP_AREPO = 0, LUNA_STEP_LUNA = 1 };
@@ -34,7 +34,7 @@ enum { LUNA_WALL_REFLECTIVE = 0, LUNA_WALL_OUTFLOW = 1 };
 typedef struct {
     int dim;       /* 2 or 3 */
     int order;     /* 0 or 1 */
-    int limiter;   /* LUNA_LIMITER_FACE_CENTROID or LUNA_LIMITER_NONE */
+    int limiter;   /* LUNA_LIMITER_FACE_CENTROID, LUNA_LIMITER_NONE or LUNA_LIMITER_LEARNED */
     int riemann;   /* LUNA_RIEMANN_HLLC or LUNA_RIEMANN_HLL */
     int motion;    /* LUNA_MOTION_STATIC or LUNA_MOTION_MOVING (order 1) */
     int step_rule; /* LUNA_STEP_AREPO, or LUNA_STEP_LUNA in 2D (set_step_rule, finite_volume.hpp:181) */
@@ -171,6 +171,50 @@ int luna_fv1_step_adjoint(luna_fv1* fv, double dt, double* qbar);
  * of the step) stays. Both block until done. LUNA_ERR_LOGIC on a moving mesh or order 0. */
 int luna_fv_step_tangent(luna_fv* fv, double dt, double* dq, int device);
 int luna_fv_step_adjoint(luna_fv* fv, double dt, double* qbar, int device);
+
+/* ------------------------------------------------------------ the learned slope limiter (no counterpart in luna-cpp)
+ *
+ * With LUNA_LIMITER_LEARNED each sweep runs the face-centroid limiter (finite_volume.hpp:538-565) and records, per cell and
+ * variable, 16 features (BF16) and a troubled-cell flag; with a network set, an MLP 16 -> 64 -> 64 -> 1 (ReLU; BF16
+ * operands, FP32 accumulation on the matrix cores) gives each (cell, variable) a fraction sigmoid(z) in [0, 1] that
+ * scales its limited gradient, except in troubled cells (a compressive pressure jump, or a neighbour at under a tenth of
+ * the cell's density or pressure), which keep the face-centroid gradient. The scaled gradient is never steeper than the
+ * face-centroid one, so every face-centroid value stays within the neighbours' range. Without a network the learned mode
+ * gives the face-centroid limiter's results bit for bit. The parameters are FP32 in the order W1 [64][16], b1 [64],
+ * W2 [64][64], b2 [64], w3 [64], b3: h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2), z = w3 . h2 + b3. */
+#define LUNA_LIMITER_FEATURES 16
+#define LUNA_LIMITER_PARAMS 5313
+
+/* Sets (params host, count LUNA_LIMITER_PARAMS) or, with params NULL, removes the network of a learned-limiter solver. */
+int luna_fv_set_limiter_network(luna_fv* fv, const float* params, int count);
+/* Host copies of the last sweep's features (N x (dim + 2) rows of LUNA_LIMITER_FEATURES BF16 bit patterns, row i (dim + 2) + v)
+ * and troubled flags (N bytes). Feature k of variable v: 0 the face-centroid factor phi_v (limited = phi_v x unlimited),
+ * 1 (max - W)/S, 2 (W - min)/S, 3 the largest unlimited face-centroid increment / S, 4 minus the smallest / S,
+ * 5 the least-squares intercept / S, 6 clipped faces / list length, 7 min over variables of phi, 8 density range /
+ * density, 9 pressure range / pressure, 10 div(v) r / c, 11 |curl v| r / c, 12 |v| / c, 13 [v = density],
+ * 14 [v = pressure], 15 S / scale; min, max over the neighbours (tiny faces excluded as the limiter does),
+ * S = max - min + 1e-12 scale + 1e-300, scale = c for a velocity and |W| otherwise, each feature clamped to [-8, 8]. */
+int luna_fv_limiter_features(luna_fv* fv, unsigned short* features, unsigned char* troubled);
+
+/* The network's trainer, on the GPU: FP32 parameters with Adam (beta 0.9, 0.999, eps 1e-8) and the BF16 copy the kernels
+ * read. A batch is rows of features x (rows x 16 BF16), targets t and weights w (rows FP32), all device pointers that are
+ * ready when the call is made; the loss is sum_r w_r (sigmoid(z_r) - t_r)^2 / weight_sum. Sums run in a fixed order, so a
+ * batch gives the same bits every time. */
+typedef struct luna_mlp luna_mlp;
+int luna_mlp_create(luna_mlp** out);
+int luna_mlp_destroy(luna_mlp* mlp);
+/* Sets the parameters (host, LUNA_LIMITER_PARAMS) and resets Adam. */
+int luna_mlp_set_params(luna_mlp* mlp, const float* params, int count);
+int luna_mlp_get_params(luna_mlp* mlp, float* params, int count);
+/* fraction (device, rows FP32) = sigmoid(z) of each row. Blocks. */
+int luna_mlp_forward(luna_mlp* mlp, const unsigned short* x, int rows, float* fraction);
+/* The loss and its gradient in the parameters (host outputs). Blocks. */
+int luna_mlp_gradient(luna_mlp* mlp, const unsigned short* x, const float* target, const float* weight, int rows, double weight_sum,
+                      float* grad, float* loss);
+/* One Adam step at learning rate lr; loss (host, the loss before the step) if not NULL, which blocks; else asynchronous. */
+int luna_mlp_train_step(luna_mlp* mlp, const unsigned short* x, const float* target, const float* weight, int rows, double weight_sum,
+                        float lr, float* loss);
+int luna_mlp_synchronize(luna_mlp* mlp);
 
 /* The last error message of this thread. */
 const char* luna_last_error(void);

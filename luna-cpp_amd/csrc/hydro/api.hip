@@ -140,12 +140,16 @@ struct luna_fv {
     int D = 3, V = 5, N = 0, F = 0, E = 0, slots = 0;
     hipStream_t stream = nullptr;
     Buf<int> count, slice, list, other, steps;
-    Buf<double> cell, facerec;
+    Buf<double> cell, facerec, slotgeo;
     Buf<double> q, grad, w, x, old_centroid, flux;
     Buf<unsigned long long> counters;  // [0, 1] dt slots, [2, 3] fallbacks of the step's two flux passes
     // the derivatives' rows (first use): luna::Derivs
     Buf<double> d_dq, d_dW, d_dgrad, d_dflux, d_sidebar, d_bbar, d_boundbar, d_out;
     Buf<int> d_extreme;
+    // the learned limiter: its features and troubled flags (recorded by each sweep) and the network copy
+    Buf<unsigned short> features;
+    Buf<unsigned char> troubled, net;
+    bool has_net = false, features_ready = false;
     unsigned long long* host = nullptr;  // pinned: read-backs
     double side[3] = {0, 0, 0};
     luna::Grid grid{};
@@ -158,7 +162,8 @@ struct luna_fv {
     int* stage_list = nullptr;
     int* stage_other = nullptr;
     int* stage_count = nullptr;
-    size_t stage_cell_cap = 0, stage_face_cap = 0, stage_list_cap = 0, stage_other_cap = 0, stage_count_cap = 0;
+    double* stage_slot = nullptr;
+    size_t stage_cell_cap = 0, stage_face_cap = 0, stage_list_cap = 0, stage_other_cap = 0, stage_count_cap = 0, stage_slot_cap = 0;
 
     luna::Geo geo() const {
         luna::Geo g{};
@@ -170,6 +175,8 @@ struct luna_fv {
         g.other = other.p;
         g.cell = cell.p;
         g.face = facerec.p;
+        g.slot = slotgeo.p;
+        g.slots = slots;
         for (int a = 0; a < 3; ++a) {
             g.side[a] = side[a];
             g.inverse[a] = side[a] > 0 ? 1.0 / side[a] : 0.0;
@@ -178,7 +185,7 @@ struct luna_fv {
     }
     int grad_row() const { return D == 3 ? luna::grad_row<3>() : luna::grad_row<2>(); }
     luna::Rows rows() const {
-        return luna::Rows{q.p, grad.p, w.p, x.p, old_centroid.p, steps.p, flux.p};
+        return luna::Rows{q.p, grad.p, w.p, x.p, old_centroid.p, steps.p, flux.p, features.p, troubled.p};
     }
     luna::Gas gas() const { return luna::Gas{opt.gamma, opt.smallr, opt.smallc, 1.0 / (opt.gamma - 1.0)}; }
     luna::Counters counters_now() const {
@@ -269,6 +276,7 @@ int upload(luna_fv* fv, const luna_geometry* m) {
     LUNA_HIP(fv->slice.reserve(S));
     LUNA_HIP(fv->list.reserve(slots > 0 ? slots : 1));
     LUNA_HIP(fv->other.reserve(slots > 0 ? slots : 1));
+    LUNA_HIP(fv->slotgeo.reserve(static_cast<size_t>(D + 1) * (slots > 0 ? slots : 1)));
     LUNA_HIP(fv->facerec.reserve(static_cast<size_t>(F) * luna::FACE_ROW));
     LUNA_HIP(fv->flux.reserve(static_cast<size_t>(F) * fv->V));
     LUNA_HIP(grow_pinned(fv->stage_cell, fv->stage_cell_cap, static_cast<size_t>(N) * luna::GEO_ROW));
@@ -276,12 +284,38 @@ int upload(luna_fv* fv, const luna_geometry* m) {
     LUNA_HIP(grow_pinned(fv->stage_list, fv->stage_list_cap, static_cast<size_t>(slots)));
     LUNA_HIP(grow_pinned(fv->stage_other, fv->stage_other_cap, static_cast<size_t>(slots)));
     LUNA_HIP(grow_pinned(fv->stage_count, fv->stage_count_cap, static_cast<size_t>(N)));
+    LUNA_HIP(grow_pinned(fv->stage_slot, fv->stage_slot_cap, static_cast<size_t>(D + 1) * slots));
     hipStream_t s = fv->stream;
     double* cell = fv->stage_cell;
     double* face = fv->stage_face;
     int* list = fv->stage_list;
     int* other = fv->stage_other;
     int* count = fv->stage_count;
+    double* slotgeo = fv->stage_slot;
+    double side[3], inverse[3];
+    for (int a = 0; a < 3; ++a) {
+        side[a] = a < D ? m->side[a] : 0.0;
+        inverse[a] = side[a] > 0 ? 1.0 / side[a] : 0.0;
+    }
+    // The sweep's per-slot geometry: |A| as the face record holds it, and the face centroid relative to this cell's
+    // centroid, Geometry::face_offset (finite_volume.hpp:322-326): the stored centroid (relative to the owner's
+    // generator), moved to this cell's generator by the nearest-image separation when the cell does not own the face.
+    auto slot_geometry = [&](long long slot, long long i, int e) {
+        const int f = e >= 0 ? e : ~e;
+        const double* a = m->area + static_cast<size_t>(f) * D;
+        slotgeo[slot] = std::sqrt(D == 3 ? a[0] * a[0] + a[1] * a[1] + a[2] * a[2] : a[0] * a[0] + a[1] * a[1]);
+        for (int k = 0; k < D; ++k) {
+            double rel = m->face_centroid[static_cast<size_t>(f) * D + k];
+            if (e < 0) {
+                const double d = m->positions[static_cast<size_t>(m->owner[f]) * D + k] - m->positions[i * D + k];
+                rel = rel + (d - side[k] * std::round(d * inverse[k]));
+            }
+            slotgeo[(k + 1) * slots + slot] = rel - m->centroid[i * D + k];
+        }
+    };
+    auto slot_padding = [&](long long slot) {
+        for (int k = 0; k <= D; ++k) slotgeo[k * slots + slot] = 0.0;
+    };
     parallel_for(N, [&](long long lo, long long hi) {
         for (long long i = lo; i < hi; ++i) {
             double* row = cell + i * luna::GEO_ROW;
@@ -302,9 +336,11 @@ int upload(luna_fv* fv, const luna_geometry* m) {
                     const int e = m->list[first + t], f = e >= 0 ? e : ~e;
                     list[slot] = e;
                     other[slot] = e >= 0 ? m->neighbour[f] : m->owner[f];
+                    slot_geometry(slot, i, e);
                 } else {  // padding: a valid index that no kernel reads
                     list[slot] = 0;
                     other[slot] = 0;
+                    slot_padding(slot);
                 }
             }
         }
@@ -315,6 +351,7 @@ int upload(luna_fv* fv, const luna_geometry* m) {
         for (long long slot = slice[g] + lane; slot < slots; slot += luna::SLICE) {
             list[slot] = 0;
             other[slot] = 0;
+            slot_padding(slot);
         }
     }
     parallel_for(F, [&](long long lo, long long hi) {
@@ -335,6 +372,7 @@ int upload(luna_fv* fv, const luna_geometry* m) {
     LUNA_HIP(hipMemcpyAsync(fv->slice.p, slice.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
     LUNA_HIP(hipMemcpyAsync(fv->list.p, list, sizeof(int) * slots, hipMemcpyHostToDevice, s));
     LUNA_HIP(hipMemcpyAsync(fv->other.p, other, sizeof(int) * slots, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->slotgeo.p, slotgeo, sizeof(double) * (D + 1) * slots, hipMemcpyHostToDevice, s));
     LUNA_HIP(hipMemcpy2DAsync(fv->cell.p, sizeof(double) * luna::CELL_ROW, cell, sizeof(double) * luna::GEO_ROW, sizeof(double) * luna::GEO_ROW,
                               N, hipMemcpyHostToDevice, s));
     LUNA_HIP(hipMemcpyAsync(fv->facerec.p, face, sizeof(double) * static_cast<size_t>(F) * luna::FACE_ROW, hipMemcpyHostToDevice, s));
@@ -360,19 +398,58 @@ int primitives(luna_fv* fv) {
     return LUNA_OK;
 }
 
+// The network copy the kernels read, carved from one device buffer: W1, W2 (BF16), then b1, b2, w3, b3 (FP32).
+constexpr size_t NET_W2 = sizeof(unsigned short) * luna::HIDDEN * luna::FEATURES;
+constexpr size_t NET_REST = NET_W2 + sizeof(unsigned short) * luna::HIDDEN * luna::HIDDEN;
+constexpr size_t NET_BYTES = NET_REST + sizeof(float) * (3 * luna::HIDDEN + 1);
+
+luna::Network network_at(unsigned char* p) {
+    const float* rest = reinterpret_cast<const float*>(p + NET_REST);
+    return luna::Network{reinterpret_cast<const unsigned short*>(p), reinterpret_cast<const unsigned short*>(p + NET_W2), rest,
+                         rest + luna::HIDDEN, rest + 2 * luna::HIDDEN, rest + 3 * luna::HIDDEN};
+}
+
+// The network copy of FP32 parameters (luna::P_* order), on the host.
+std::vector<unsigned char> network_bytes(const float* params) {
+    std::vector<unsigned char> out(NET_BYTES);
+    unsigned short* w1 = reinterpret_cast<unsigned short*>(out.data());
+    unsigned short* w2 = reinterpret_cast<unsigned short*>(out.data() + NET_W2);
+    float* rest = reinterpret_cast<float*>(out.data() + NET_REST);
+    for (int k = 0; k < luna::HIDDEN * luna::FEATURES; ++k) w1[k] = luna::bf16_bits(params[luna::P_W1 + k]);
+    for (int k = 0; k < luna::HIDDEN * luna::HIDDEN; ++k) w2[k] = luna::bf16_bits(params[luna::P_W2 + k]);
+    for (int k = 0; k < luna::HIDDEN; ++k) {
+        rest[k] = params[luna::P_B1 + k];
+        rest[luna::HIDDEN + k] = params[luna::P_B2 + k];
+        rest[2 * luna::HIDDEN + k] = params[luna::P_W3 + k];
+    }
+    rest[3 * luna::HIDDEN] = params[luna::P_B3];
+    return out;
+}
+
 int sweep(luna_fv* fv, double courant) {
     const luna::Geo g = fv->geo();
     const luna::Rows r = fv->rows();
     const luna::Gas gas = fv->gas();
     const luna::Counters c = fv->counters_now();
     const bool moving = fv->opt.motion == LUNA_MOTION_MOVING, grad = fv->opt.order == 1;
-    const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID;
+    const bool learned = fv->opt.limiter == LUNA_LIMITER_LEARNED;
+    const int limit = learned ? 2 : fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID ? 1 : 0;
     hipStream_t s = fv->stream;
     LUNA_HIP(timed(fv->prof, s, "sweep", [&] {
         return fv->D == 3 ? luna::launch_sweep<3>(s, g, r, gas, courant, moving, grad, limit, fv->opt.step_rule, c)
                           : luna::launch_sweep<2>(s, g, r, gas, courant, moving, grad, limit, fv->opt.step_rule, c);
     }));
     fv->parity ^= 1;
+    if (learned) {
+        fv->features_ready = true;
+        if (fv->has_net) {
+            const luna::Network net = network_at(fv->net.p);
+            LUNA_HIP(timed(fv->prof, s, "limiter", [&] {
+                return luna::launch_limiter_apply(s, net, fv->features.p, fv->troubled.p, fv->N * fv->V, fv->V, fv->D, fv->grad_row(),
+                                                  fv->grad.p, nullptr);
+            }));
+        }
+    }
     return LUNA_OK;
 }
 
@@ -414,8 +491,10 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
     *out = nullptr;
     if (opt->dim != 2 && opt->dim != 3) return fail(LUNA_ERR_ARGUMENT, "hydro::FiniteVolume: D = 2 or 3 (1D: luna_fv1_create)");
     if (opt->order != 0 && opt->order != 1) return fail(LUNA_ERR_ARGUMENT, "hydro::FiniteVolume: order 0 or 1");
-    if (opt->limiter != LUNA_LIMITER_FACE_CENTROID && opt->limiter != LUNA_LIMITER_NONE)
-        return fail(LUNA_ERR_ARGUMENT, "hydro::FiniteVolume: limiter face_centroid or none in 2D/3D");
+    if (opt->limiter != LUNA_LIMITER_FACE_CENTROID && opt->limiter != LUNA_LIMITER_NONE && opt->limiter != LUNA_LIMITER_LEARNED)
+        return fail(LUNA_ERR_ARGUMENT, "hydro::FiniteVolume: limiter face_centroid, none or learned in 2D/3D");
+    if (opt->limiter == LUNA_LIMITER_LEARNED && opt->order != 1)
+        return fail(LUNA_ERR_ARGUMENT, "luna: the learned limiter needs order 1");
     if (opt->riemann != LUNA_RIEMANN_HLLC && opt->riemann != LUNA_RIEMANN_HLL) return fail(LUNA_ERR_ARGUMENT, "luna: unknown Riemann solver");
     if (opt->motion != LUNA_MOTION_STATIC && opt->motion != LUNA_MOTION_MOVING) return fail(LUNA_ERR_ARGUMENT, "luna: unknown motion");
     if (opt->motion == LUNA_MOTION_MOVING && opt->order != 1)
@@ -440,6 +519,11 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
         (e = fv->cell.reserve(N * luna::CELL_ROW)) != hipSuccess ||
         (e = fv->counters.reserve(4)) != hipSuccess)
         return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
+    if (opt->limiter == LUNA_LIMITER_LEARNED) {
+        if ((e = fv->features.reserve(N * V * luna::FEATURES)) != hipSuccess || (e = fv->troubled.reserve(N)) != hipSuccess ||
+            (e = fv->net.reserve(NET_BYTES)) != hipSuccess)
+            return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
+    }
     if (opt->motion == LUNA_MOTION_MOVING) {
         if ((e = fv->x.reserve(N * D)) != hipSuccess || (e = fv->old_centroid.reserve(N * D)) != hipSuccess ||
             (e = fv->steps.reserve(N * D)) != hipSuccess)
@@ -469,13 +553,16 @@ int luna_fv_destroy(luna_fv* fv) {
     if (!fv) return LUNA_OK;
     if (fv->stream) (void)hipStreamSynchronize(fv->stream);
     fv->prof.destroy();
-    for (auto* b : {&fv->cell, &fv->facerec, &fv->q, &fv->grad, &fv->w, &fv->x, &fv->old_centroid, &fv->flux}) b->release();
+    for (auto* b : {&fv->cell, &fv->facerec, &fv->slotgeo, &fv->q, &fv->grad, &fv->w, &fv->x, &fv->old_centroid, &fv->flux}) b->release();
     for (auto* b : {&fv->count, &fv->slice, &fv->list, &fv->other, &fv->steps, &fv->d_extreme}) b->release();
     for (auto* b : {&fv->d_dq, &fv->d_dW, &fv->d_dgrad, &fv->d_dflux, &fv->d_sidebar, &fv->d_bbar, &fv->d_boundbar, &fv->d_out}) b->release();
     for (void* p : {static_cast<void*>(fv->stage_cell), static_cast<void*>(fv->stage_face), static_cast<void*>(fv->stage_list),
-                    static_cast<void*>(fv->stage_other), static_cast<void*>(fv->stage_count)})
+                    static_cast<void*>(fv->stage_other), static_cast<void*>(fv->stage_count), static_cast<void*>(fv->stage_slot)})
         if (p) (void)hipHostFree(p);
     fv->counters.release();
+    fv->features.release();
+    fv->troubled.release();
+    fv->net.release();
     if (fv->host) (void)hipHostFree(fv->host);
     if (fv->stream) (void)hipStreamDestroy(fv->stream);
     delete fv;
@@ -538,6 +625,8 @@ namespace {
 int derivatives_ready(luna_fv* fv, const char* what) {
     if (fv->opt.motion != LUNA_MOTION_STATIC || fv->opt.order != 1)
         return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of a static order-1 step");
+    if (fv->opt.limiter == LUNA_LIMITER_LEARNED)
+        return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of the face-centroid limiter or none (not the learned one)");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, std::string(what) + ": a moving step is between begin and finish");
     const size_t N = fv->N, V = fv->V, F = fv->F;
     LUNA_HIP(fv->d_dq.reserve(N * V));
@@ -901,6 +990,171 @@ int luna_fv1_step_and_time_step(luna_fv1* fv, double dt, double courant, double*
     LUNA_HIP(hipMemcpyAsync(fv->host, fv->counters.p + slot, sizeof(unsigned long long), hipMemcpyDeviceToHost, fv->stream));
     LUNA_HIP(hipStreamSynchronize(fv->stream));
     *next = bits_to_double(fv->host[0]);
+    return LUNA_OK;
+}
+
+int luna_fv_set_limiter_network(luna_fv* fv, const float* params, int count) {
+    if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv_set_limiter_network: null handle");
+    if (fv->opt.limiter != LUNA_LIMITER_LEARNED) return fail(LUNA_ERR_LOGIC, "luna_fv_set_limiter_network: the solver's limiter is not the learned one");
+    if (!params) {
+        fv->has_net = false;
+        return LUNA_OK;
+    }
+    if (count != luna::PARAMS) return fail(LUNA_ERR_ARGUMENT, "luna_fv_set_limiter_network: count must be LUNA_LIMITER_PARAMS");
+    for (int k = 0; k < count; ++k)
+        if (!std::isfinite(params[k])) return fail(LUNA_ERR_ARGUMENT, "luna_fv_set_limiter_network: parameters must be finite");
+    const std::vector<unsigned char> bytes = network_bytes(params);
+    LUNA_HIP(hipMemcpyAsync(fv->net.p, bytes.data(), bytes.size(), hipMemcpyHostToDevice, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    fv->has_net = true;
+    fv->swept = false;  // the next step sweeps (and limits) again
+    return LUNA_OK;
+}
+
+int luna_fv_limiter_features(luna_fv* fv, unsigned short* features, unsigned char* troubled) {
+    if (!fv || !features || !troubled) return fail(LUNA_ERR_ARGUMENT, "luna_fv_limiter_features: null argument");
+    if (fv->opt.limiter != LUNA_LIMITER_LEARNED) return fail(LUNA_ERR_LOGIC, "luna_fv_limiter_features: the solver's limiter is not the learned one");
+    if (!fv->features_ready) return fail(LUNA_ERR_LOGIC, "luna_fv_limiter_features: no sweep yet (call luna_fv_time_step)");
+    const size_t rows = static_cast<size_t>(fv->N) * fv->V;
+    LUNA_HIP(hipMemcpyAsync(features, fv->features.p, sizeof(unsigned short) * rows * luna::FEATURES, hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipMemcpyAsync(troubled, fv->troubled.p, fv->N, hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    return LUNA_OK;
+}
+
+}  // extern "C"
+
+// The network's trainer: FP32 parameters with Adam's moments, the BF16 network copy the kernels read, the per-wavefront
+// partial sums.
+struct luna_mlp {
+    hipStream_t stream = nullptr;
+    Buf<float> master, m, v, partials, out;
+    Buf<unsigned char> net;
+    int t = 0;
+};
+
+namespace {
+
+int mlp_batch(luna_mlp* mlp, const unsigned short* x, const float* target, const float* weight, int rows, double weight_sum,
+              luna::Batch& b, int& waves, const char* what) {
+    if (!mlp || !x || !target || !weight) return fail(LUNA_ERR_ARGUMENT, std::string(what) + ": null argument");
+    if (rows <= 0) return fail(LUNA_ERR_ARGUMENT, std::string(what) + ": rows must be positive");
+    if (!(weight_sum > 0) || !std::isfinite(weight_sum)) return fail(LUNA_ERR_ARGUMENT, std::string(what) + ": weight_sum must be positive");
+    b = luna::Batch{x, target, weight, rows, static_cast<float>(1.0 / weight_sum)};
+    waves = luna::mlp_waves(rows);
+    LUNA_HIP(mlp->partials.reserve(static_cast<size_t>(waves) * luna::PARTIAL_ROW));
+    return LUNA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int luna_mlp_create(luna_mlp** out) {
+    if (!out) return fail(LUNA_ERR_ARGUMENT, "luna_mlp_create: null argument");
+    *out = nullptr;
+    luna_mlp* mlp = new luna_mlp;
+    hipError_t e = hipStreamCreate(&mlp->stream);
+    if (e == hipSuccess) e = mlp->master.reserve(luna::PARAMS);
+    if (e == hipSuccess) e = mlp->m.reserve(luna::PARAMS);
+    if (e == hipSuccess) e = mlp->v.reserve(luna::PARAMS);
+    if (e == hipSuccess) e = mlp->out.reserve(luna::PARAMS + 1);
+    if (e == hipSuccess) e = mlp->net.reserve(NET_BYTES);
+    if (e != hipSuccess) {
+        luna_mlp_destroy(mlp);
+        return fail(LUNA_ERR_MEMORY, std::string("luna_mlp_create: ") + hipGetErrorString(e));
+    }
+    *out = mlp;
+    return LUNA_OK;
+}
+
+int luna_mlp_destroy(luna_mlp* mlp) {
+    if (!mlp) return LUNA_OK;
+    if (mlp->stream) (void)hipStreamSynchronize(mlp->stream);
+    for (auto* b : {&mlp->master, &mlp->m, &mlp->v, &mlp->partials, &mlp->out}) b->release();
+    mlp->net.release();
+    if (mlp->stream) (void)hipStreamDestroy(mlp->stream);
+    delete mlp;
+    return LUNA_OK;
+}
+
+int luna_mlp_set_params(luna_mlp* mlp, const float* params, int count) {
+    if (!mlp || !params) return fail(LUNA_ERR_ARGUMENT, "luna_mlp_set_params: null argument");
+    if (count != luna::PARAMS) return fail(LUNA_ERR_ARGUMENT, "luna_mlp_set_params: count must be LUNA_LIMITER_PARAMS");
+    const std::vector<unsigned char> bytes = network_bytes(params);
+    const std::vector<float> zero(luna::PARAMS, 0.0f);
+    LUNA_HIP(hipMemcpyAsync(mlp->master.p, params, sizeof(float) * luna::PARAMS, hipMemcpyHostToDevice, mlp->stream));
+    LUNA_HIP(hipMemcpyAsync(mlp->m.p, zero.data(), sizeof(float) * luna::PARAMS, hipMemcpyHostToDevice, mlp->stream));
+    LUNA_HIP(hipMemcpyAsync(mlp->v.p, zero.data(), sizeof(float) * luna::PARAMS, hipMemcpyHostToDevice, mlp->stream));
+    LUNA_HIP(hipMemcpyAsync(mlp->net.p, bytes.data(), bytes.size(), hipMemcpyHostToDevice, mlp->stream));
+    LUNA_HIP(hipStreamSynchronize(mlp->stream));
+    mlp->t = 0;
+    return LUNA_OK;
+}
+
+int luna_mlp_get_params(luna_mlp* mlp, float* params, int count) {
+    if (!mlp || !params) return fail(LUNA_ERR_ARGUMENT, "luna_mlp_get_params: null argument");
+    if (count != luna::PARAMS) return fail(LUNA_ERR_ARGUMENT, "luna_mlp_get_params: count must be LUNA_LIMITER_PARAMS");
+    LUNA_HIP(hipMemcpyAsync(params, mlp->master.p, sizeof(float) * luna::PARAMS, hipMemcpyDeviceToHost, mlp->stream));
+    LUNA_HIP(hipStreamSynchronize(mlp->stream));
+    return LUNA_OK;
+}
+
+int luna_mlp_forward(luna_mlp* mlp, const unsigned short* x, int rows, float* fraction) {
+    if (!mlp || !x || !fraction) return fail(LUNA_ERR_ARGUMENT, "luna_mlp_forward: null argument");
+    if (rows < 0) return fail(LUNA_ERR_ARGUMENT, "luna_mlp_forward: rows must not be negative");
+    LUNA_HIP(luna::launch_limiter_apply(mlp->stream, network_at(mlp->net.p), x, nullptr, rows, 1, 1, 0, nullptr, fraction));
+    LUNA_HIP(hipStreamSynchronize(mlp->stream));
+    return LUNA_OK;
+}
+
+int luna_mlp_gradient(luna_mlp* mlp, const unsigned short* x, const float* target, const float* weight, int rows, double weight_sum,
+                      float* grad, float* loss) {
+    if (!grad || !loss) return fail(LUNA_ERR_ARGUMENT, "luna_mlp_gradient: null argument");
+    luna::Batch b{};
+    int waves = 0;
+    if (int rc = mlp_batch(mlp, x, target, weight, rows, weight_sum, b, waves, "luna_mlp_gradient")) return rc;
+    LUNA_HIP(luna::launch_mlp_partials(mlp->stream, network_at(mlp->net.p), b, mlp->partials.p, waves));
+    LUNA_HIP(luna::launch_mlp_reduce(mlp->stream, mlp->partials.p, waves, mlp->out.p, nullptr));
+    LUNA_HIP(hipMemcpyAsync(grad, mlp->out.p, sizeof(float) * luna::PARAMS, hipMemcpyDeviceToHost, mlp->stream));
+    LUNA_HIP(hipMemcpyAsync(loss, mlp->out.p + luna::PARAMS, sizeof(float), hipMemcpyDeviceToHost, mlp->stream));
+    LUNA_HIP(hipStreamSynchronize(mlp->stream));
+    return LUNA_OK;
+}
+
+int luna_mlp_train_step(luna_mlp* mlp, const unsigned short* x, const float* target, const float* weight, int rows, double weight_sum,
+                        float lr, float* loss) {
+    luna::Batch b{};
+    int waves = 0;
+    if (int rc = mlp_batch(mlp, x, target, weight, rows, weight_sum, b, waves, "luna_mlp_train_step")) return rc;
+    if (!(lr > 0) || !std::isfinite(lr)) return fail(LUNA_ERR_ARGUMENT, "luna_mlp_train_step: lr must be positive");
+    mlp->t += 1;
+    const double beta1 = 0.9, beta2 = 0.999;
+    luna::Adam a{};
+    a.master = mlp->master.p;
+    a.m = mlp->m.p;
+    a.v = mlp->v.p;
+    a.lr = lr;
+    a.beta1 = static_cast<float>(beta1);
+    a.beta2 = static_cast<float>(beta2);
+    a.eps = 1e-8f;
+    a.c1 = static_cast<float>(1.0 / (1.0 - std::pow(beta1, mlp->t)));
+    a.c2 = static_cast<float>(1.0 / (1.0 - std::pow(beta2, mlp->t)));
+    a.w1 = reinterpret_cast<unsigned short*>(mlp->net.p);
+    a.w2 = reinterpret_cast<unsigned short*>(mlp->net.p + NET_W2);
+    a.rest = reinterpret_cast<float*>(mlp->net.p + NET_REST);
+    LUNA_HIP(luna::launch_mlp_partials(mlp->stream, network_at(mlp->net.p), b, mlp->partials.p, waves));
+    LUNA_HIP(luna::launch_mlp_reduce(mlp->stream, mlp->partials.p, waves, mlp->out.p, &a));
+    if (loss) {
+        LUNA_HIP(hipMemcpyAsync(loss, mlp->out.p, sizeof(float), hipMemcpyDeviceToHost, mlp->stream));
+        LUNA_HIP(hipStreamSynchronize(mlp->stream));
+    }
+    return LUNA_OK;
+}
+
+int luna_mlp_synchronize(luna_mlp* mlp) {
+    if (!mlp) return fail(LUNA_ERR_ARGUMENT, "luna_mlp_synchronize: null handle");
+    LUNA_HIP(hipStreamSynchronize(mlp->stream));
     return LUNA_OK;
 }
 

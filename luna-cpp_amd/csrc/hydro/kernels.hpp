@@ -2,6 +2,7 @@
 // the row layouts in HBM. Launched by api.hip on the handle's stream. FP64 throughout, -ffp-contract=off (physics.hpp).
 #pragma once
 
+#include "learned.hpp"
 #include "physics.hpp"
 
 namespace luna {
@@ -43,6 +44,10 @@ struct Geo {
     const int* other;   // slots: the cell on the other side of the entry's face
     double* cell;       // N x CELL_ROW (the kernels write only the primitive half)
     const double* face; // F x FACE_ROW
+    // per slot, in the slices' layout, D + 1 planes of `slots` doubles: the face's |A|, then the face centroid relative
+    // to this cell's centroid (Geometry::face_offset, finite_volume.hpp:322-326), one plane per axis
+    const double* slot;
+    long long slots;
     double side[3], inverse[3];  // the periodic box and 1 / side
 };
 
@@ -55,6 +60,8 @@ struct Rows {
     double* old_centroid;  // N x D centroids of t^n (moving; written by the sweep)
     int* steps;            // N x D whole grid steps over the step (moving)
     double* flux;          // F x V, face order
+    unsigned short* features;  // N V x 16 BF16 (the learned limiter's, recorded by the sweep with limit 2)
+    unsigned char* troubled;   // N (the same)
 };
 
 struct Grid {
@@ -71,8 +78,9 @@ struct Counters {
 // Kernel launchers (api.hip calls these; each returns the HIP launch status).
 template <int D>
 hipError_t launch_primitives(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas);
+// limit: 0 none, 1 the face-centroid limiter, 2 the same recording the learned limiter's features (r.features, r.troubled).
 template <int D>
-hipError_t launch_sweep(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double courant, bool moving, bool grad, bool limit,
+hipError_t launch_sweep(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double courant, bool moving, bool grad, int limit,
                         int rule, const Counters& c);
 // pass: -1 static, 0 a moving step's first pass (mesh of t^n), 1 its second (mesh of t^n+1).
 template <int D>

@@ -1,0 +1,329 @@
+// The learned limiter's network on CDNA4 matrix cores: inference fused with the gradient scaling, and the training
+// kernels (forward, backward, deterministic reduction, Adam), BF16 operands with FP32 accumulation
+// (v_mfma_f32_16x16x16_bf16).
+//
+// Every product is computed transposed, rows in the MFMA's N position: a tile is 16 feature rows, and lane l (group
+// g = l / 16, row n = l % 16) holds row n's hidden units 16 b + 4 g + j (j < 4) of each 16-unit block b. That is the
+// MFMA's output layout and also its B-operand layout for the next layer's 16-wide slice of the contraction, so the
+// activations go from one layer to the next in registers. The weight gradients contract over rows, so their operands
+// go through LDS once per tile.
+#include <algorithm>
+
+#include "kernels.hpp"
+
+namespace luna {
+namespace {
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// D = A B + C for 16 x 16 tiles: lane l gives A[l % 16][4 (l / 16) + j] and B[4 (l / 16) + j][l % 16] and gets
+// D[4 (l / 16) + j][l % 16].
+__device__ __forceinline__ float4_t mfma16(short4_t a, short4_t b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ short bf16s(float f) { return (short)bf16_bits(f); }
+
+__device__ __forceinline__ short4_t load4(const unsigned short* p) {  // 4 BF16, 8-byte aligned
+    const uint2 u = *reinterpret_cast<const uint2*>(p);
+    short4_t r;
+    r[0] = (short)(u.x & 0xffffu);
+    r[1] = (short)(u.x >> 16);
+    r[2] = (short)(u.y & 0xffffu);
+    r[3] = (short)(u.y >> 16);
+    return r;
+}
+__device__ __forceinline__ float4_t loadf4(const float* p) {  // 16-byte aligned
+    const float4 u = *reinterpret_cast<const float4*>(p);
+    float4_t r;
+    r[0] = u.x;
+    r[1] = u.y;
+    r[2] = u.z;
+    r[3] = u.w;
+    return r;
+}
+
+// This lane's operands of the forward pass.
+struct Weights {
+    short4_t w1[4];     // A of layer 1, block b: W1[16 b + n][4 g + j]
+    short4_t w2[4][4];  // A of layer 2, [out block o][in block c]: W2[16 o + n][16 c + 4 g + j]
+    float4_t b1[4], b2[4], w3[4];  // b1[16 b + 4 g + j], ...
+    float b3;
+};
+__device__ __forceinline__ void load_weights(const Network& net, int g, int n, Weights& W) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        W.w1[b] = load4(net.w1 + (16 * b + n) * FEATURES + 4 * g);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) W.w2[b][c] = load4(net.w2 + (16 * b + n) * HIDDEN + 16 * c + 4 * g);
+        W.b1[b] = loadf4(net.b1 + 16 * b + 4 * g);
+        W.b2[b] = loadf4(net.b2 + 16 * b + 4 * g);
+        W.w3[b] = loadf4(net.w3 + 16 * b + 4 * g);
+    }
+    W.b3 = net.b3[0];
+}
+
+// One tile's forward pass. x: this lane's 4 features 4 g .. 4 g + 3 of row n.
+struct Tile {
+    float4_t pre1[4];  // layer-1 pre-activations (hidden 16 b + 4 g + j of row n)
+    short4_t h1[4];    // relu, BF16
+    float4_t pre2[4];
+    float z;           // the output of row n (all four groups hold it)
+};
+__device__ __forceinline__ void forward(const Weights& W, short4_t x, Tile& t) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        t.pre1[b] = mfma16(W.w1[b], x, W.b1[b]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t.h1[b][j] = bf16s(fmaxf(t.pre1[b][j], 0.0f));
+    }
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+        float4_t acc = W.b2[o];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc = mfma16(W.w2[o][c], t.h1[c], acc);
+        t.pre2[o] = acc;
+    }
+    float part = 0.0f;
+#pragma unroll
+    for (int o = 0; o < 4; ++o)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) part += W.w3[o][j] * fmaxf(t.pre2[o][j], 0.0f);
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    t.z = part + W.b3;
+}
+
+__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
+
+constexpr int APPLY_BLOCK = 256, APPLY_TILES = 4;  // tiles per wavefront
+
+__global__ __launch_bounds__(APPLY_BLOCK) void k_limiter_apply(Network net, const unsigned short* x, const unsigned char* troubled, int rows,
+                                                               int V, int D, int grad_row, double* grad, float* fraction) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
+    const int wave = blockIdx.x * (APPLY_BLOCK / 64) + (threadIdx.x >> 6);
+    const int tiles = (rows + 15) >> 4;
+    const int first = wave * APPLY_TILES, last = min(tiles, first + APPLY_TILES);
+    if (first >= tiles) return;  // the whole wavefront
+    Weights W;
+    load_weights(net, g, n, W);
+    for (int t = first; t < last; ++t) {  // uniform over the wavefront: the MFMAs run with every lane
+        const int row = t * 16 + n;
+        short4_t xv = {0, 0, 0, 0};
+        if (row < rows) xv = load4(x + (size_t)row * FEATURES + 4 * g);
+        Tile tl;
+        forward(W, xv, tl);
+        if (g == 0 && row < rows) {
+            const float frac = sigmoid(tl.z);
+            if (fraction) fraction[row] = frac;
+            if (grad) {
+                const int i = row / V, v = row - i * V;
+                if (!troubled[i]) {
+                    double* p = grad + (size_t)i * grad_row + v * D;
+                    for (int a = 0; a < D; ++a) p[a] = p[a] * (double)frac;
+                }
+            }
+        }
+    }
+}
+
+// Per wavefront: the gradient of sum_r w_r (sigmoid(z_r) - t_r)^2 inv_weight over tiles wave, wave + waves, ..., and
+// the loss, into its row of partials.
+__global__ __launch_bounds__(64) void k_mlp_partials(Network net, Batch b, float* partials) {
+    __shared__ short h1s[16][HIDDEN], dh2s[16][HIDDEN], dh1s[16][HIDDEN], xs[16][FEATURES];
+    const int lane = threadIdx.x, g = lane >> 4, n = lane & 15;
+    const int wave = blockIdx.x, waves = gridDim.x;
+    const int tiles = (b.rows + 15) >> 4;
+    Weights W;
+    load_weights(net, g, n, W);
+    short4_t w2t[4][4];  // A of the layer-2 transpose, [hidden-1 block k][hidden-2 block m]: W2[16 m + 4 g + j][16 k + n]
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) w2t[k][m][j] = (short)net.w2[(16 * m + 4 * g + j) * HIDDEN + 16 * k + n];
+    const float4_t zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    float4_t gW1[4], gW2[4][4], gb1[4], gb2[4], gw3[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        gW1[k] = zero;
+        gb1[k] = zero;
+        gb2[k] = zero;
+        gw3[k] = zero;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) gW2[k][m] = zero;
+    }
+    float gb3 = 0.0f, loss = 0.0f;
+    for (int t = wave; t < tiles; t += waves) {
+        const int row = t * 16 + n;
+        const bool valid = row < b.rows;
+        short4_t xv = {0, 0, 0, 0};
+        if (valid) xv = load4(b.x + (size_t)row * FEATURES + 4 * g);
+        Tile tl;
+        forward(W, xv, tl);
+        const float sig = sigmoid(tl.z);
+        const float target = valid ? b.target[row] : 0.0f, weight = valid ? b.weight[row] : 0.0f;
+        const float diff = sig - target;
+        const float dz = 2.0f * weight * diff * sig * (1.0f - sig) * b.inv_weight;
+        if (g == 0) {
+            loss += weight * diff * diff * b.inv_weight;
+            gb3 += dz;
+        }
+        short4_t dh2b[4];
+#pragma unroll
+        for (int o = 0; o < 4; ++o)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float pre = tl.pre2[o][j];
+                const float d = pre > 0.0f ? dz * W.w3[o][j] : 0.0f;
+                gw3[o][j] += dz * fmaxf(pre, 0.0f);
+                gb2[o][j] += d;
+                dh2b[o][j] = bf16s(d);
+            }
+        short4_t dh1b[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float4_t acc = zero;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) acc = mfma16(w2t[k][m], dh2b[m], acc);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float d = tl.pre1[k][j] > 0.0f ? acc[j] : 0.0f;
+                gb1[k][j] += d;
+                dh1b[k][j] = bf16s(d);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                h1s[n][16 * k + 4 * g + j] = tl.h1[k][j];
+                dh2s[n][16 * k + 4 * g + j] = dh2b[k][j];
+                dh1s[n][16 * k + 4 * g + j] = dh1b[k][j];
+            }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xs[n][4 * g + j] = xv[j];
+        __syncthreads();
+        // dW2[16 m + 4 g + j][16 k + n] += sum over the tile's rows r of dh2[r][16 m + .] h1[r][16 k + .]
+        short4_t xr;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xr[j] = xs[4 * g + j][n];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            short4_t a, a1;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                a[j] = dh2s[4 * g + j][16 * m + n];
+                a1[j] = dh1s[4 * g + j][16 * m + n];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                short4_t bb;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) bb[j] = h1s[4 * g + j][16 * k + n];
+                gW2[m][k] = mfma16(a, bb, gW2[m][k]);
+            }
+            gW1[m] = mfma16(a1, xr, gW1[m]);  // dW1[16 m + 4 g + j][n]
+        }
+        __syncthreads();
+    }
+    float* out = partials + (size_t)wave * PARTIAL_ROW;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int h = 16 * m + 4 * g + j;
+            out[P_W1 + h * FEATURES + n] = gW1[m][j];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) out[P_W2 + h * HIDDEN + 16 * k + n] = gW2[m][k][j];
+        }
+    // The bias and w3 sums over the 16 rows of each group, the scalars over the wavefront.
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float s1 = gb1[k][j], s2 = gb2[k][j], s3 = gw3[k][j];
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                s1 += __shfl_xor(s1, o);
+                s2 += __shfl_xor(s2, o);
+                s3 += __shfl_xor(s3, o);
+            }
+            if (n == 0) {
+                out[P_B1 + 16 * k + 4 * g + j] = s1;
+                out[P_B2 + 16 * k + 4 * g + j] = s2;
+                out[P_W3 + 16 * k + 4 * g + j] = s3;
+            }
+        }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        gb3 += __shfl_xor(gb3, o);
+        loss += __shfl_xor(loss, o);
+    }
+    if (lane == 0) {
+        out[P_B3] = gb3;
+        out[PARAMS] = loss;
+    }
+}
+
+// Sums the partial rows in wave order (so a batch always gives the same bits), then either writes the sums (gradient
+// and loss) or takes one Adam step of the FP32 parameters and refreshes the network copy (loss into out[0] if out).
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* partials, int waves, float* out, Adam a, int adam) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p > PARAMS) return;
+    float s = 0.0f;
+    for (int w = 0; w < waves; ++w) s += partials[(size_t)w * PARTIAL_ROW + p];
+    if (!adam) {
+        out[p] = s;
+        return;
+    }
+    if (p == PARAMS) {
+        if (out) out[0] = s;
+        return;
+    }
+    const float m = a.beta1 * a.m[p] + (1.0f - a.beta1) * s;
+    const float v = a.beta2 * a.v[p] + (1.0f - a.beta2) * s * s;
+    a.m[p] = m;
+    a.v[p] = v;
+    const float value = a.master[p] - a.lr * (m * a.c1) / (sqrtf(v * a.c2) + a.eps);
+    a.master[p] = value;
+    if (p < P_B1)
+        a.w1[p - P_W1] = bf16_bits(value);
+    else if (p < P_W2)
+        a.rest[p - P_B1] = value;
+    else if (p < P_B2)
+        a.w2[p - P_W2] = bf16_bits(value);
+    else
+        a.rest[HIDDEN + (p - P_B2)] = value;
+}
+
+}  // namespace
+
+hipError_t launch_limiter_apply(hipStream_t s, const Network& net, const unsigned short* features, const unsigned char* troubled, int rows,
+                                int V, int D, int grad_row, double* grad, float* fraction) {
+    if (rows <= 0) return hipSuccess;
+    const long long tiles = (rows + 15) / 16, waves = (tiles + APPLY_TILES - 1) / APPLY_TILES;
+    const unsigned blocks = (unsigned)((waves + APPLY_BLOCK / 64 - 1) / (APPLY_BLOCK / 64));
+    hipLaunchKernelGGL(k_limiter_apply, dim3(blocks), dim3(APPLY_BLOCK), 0, s, net, features, troubled, rows, V, D, grad_row, grad, fraction);
+    return hipGetLastError();
+}
+
+int mlp_waves(int rows) {
+    const int tiles = (rows + 15) / 16;
+    return std::max(1, std::min(1024, (tiles + 3) / 4));
+}
+
+hipError_t launch_mlp_partials(hipStream_t s, const Network& net, const Batch& b, float* partials, int waves) {
+    hipLaunchKernelGGL(k_mlp_partials, dim3(waves), dim3(64), 0, s, net, b, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_reduce(hipStream_t s, const float* partials, int waves, float* out, const Adam* adam) {
+    const Adam a = adam ? *adam : Adam{};
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((PARAMS + 1 + 255) / 256), dim3(256), 0, s, partials, waves, out, a, adam ? 1 : 0);
+    return hipGetLastError();
+}
+
+}  // namespace luna

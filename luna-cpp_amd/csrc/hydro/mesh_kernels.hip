@@ -116,16 +116,15 @@ __global__ __launch_bounds__(BLOCK) void k_primitives(Geo g, Rows r, Gas gas) {
     row[C::CS] = sound_speed<D>(gas, W);
 }
 
-// One list entry as the sweep's first walk reads it: the face's |A| and the other cell's row.
+// One list entry as the sweep's first walk reads it: the face's |A| (from the entry's slot) and the other cell's row.
 template <int D>
 struct Near {
     double mag, x[D], c[D], W[D + 2], cs;
 };
 template <int D>
-__device__ __forceinline__ void load_near(const Geo& g, int e, int j, Near<D>& n) {
+__device__ __forceinline__ void load_near(const Geo& g, int slot, int j, Near<D>& n) {
     using C = CellAt<D>;
-    const int f = e >= 0 ? e : ~e;
-    n.mag = g.face[(size_t)f * FACE_ROW + FaceAt<D>::MAG];
+    n.mag = g.slot[slot];
     const double* row = g.cell + (size_t)j * CELL_ROW;
     load_vec<D>(row + C::X, n.x);
     load_vec<D>(row + C::C, n.c);
@@ -195,12 +194,12 @@ struct Sums {
 // The sweep (finite_volume.hpp:463-569 Sweep::operator(), double policy) with the time step's terms (:1040-1104) fused:
 // each cell takes its own term and, for every face of its list, the face term with its own radius. The pair term vsig and
 // |d| have the same bits from either side, so the minimum is the reference's (the face kernel's terms are the same set).
-// The first block also resets the other dt slot and the step's fallback counters.
-template <int D, bool Moving, bool Grad, bool Limit, int Rule>
+// The first block also resets the other dt slot and the step's fallback counters. Limit: 0 none, 1 the face-centroid
+// limiter, 2 the same limiter recording the learned limiter's features and troubled-cell flag (the gradients are those of 1).
+template <int D, bool Moving, bool Grad, int Limit, int Rule>
 __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, Rows r, Gas gas, double courant, Counters c) {
     constexpr int V = D + 2, U = D + 1, GR = grad_row<D>();
     using C = CellAt<D>;
-    using FA = FaceAt<D>;
     const int i = xcd_block() * CELL_BLOCK + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *c.dt_next = INF_BITS;
@@ -220,30 +219,23 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
         s.init();
         const int count = g.count[i], base = first_slot(g, i);
         // Groups of UN entries: their face magnitudes and rows in flight together, summed in list order; the next group's
-        // list entries are read one group ahead, so each group waits for one level of loads.
+        // other-cell indices are read one group ahead, so each group waits for one level of loads.
         constexpr int UN = LUNA_SWEEP_UNROLL;
-        int ne[UN], nj[UN];
+        int nj[UN];
 #pragma unroll
         for (int u = 0; u < UN; ++u)
-            if (u < count) {
-                ne[u] = g.list[base + u * SLICE];
-                nj[u] = g.other[base + u * SLICE];
-            }
+            if (u < count) nj[u] = g.other[base + u * SLICE];
         int t = 0;
         for (; t + UN - 1 < count; t += UN) {
-            int ce[UN], cj[UN];
+            int cj[UN];
 #pragma unroll
             for (int u = 0; u < UN; ++u) {
-                ce[u] = ne[u];
                 cj[u] = nj[u];
-                if (t + UN + u < count) {
-                    ne[u] = g.list[base + (t + UN + u) * SLICE];
-                    nj[u] = g.other[base + (t + UN + u) * SLICE];
-                }
+                if (t + UN + u < count) nj[u] = g.other[base + (t + UN + u) * SLICE];
             }
             Near<D> n[UN];
 #pragma unroll
-            for (int u = 0; u < UN; ++u) load_near<D>(g, ce[u], cj[u], n[u]);
+            for (int u = 0; u < UN; ++u) load_near<D>(g, base + (t + u) * SLICE, cj[u], n[u]);
 #pragma unroll
             for (int u = 0; u < UN; ++u) s.add(g, n[u], xi, centroid, Wi, ci, ri, courant);
         }
@@ -251,7 +243,7 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
         for (int u = 0; u < UN - 1; ++u)
             if (t + u < count) {
                 Near<D> a;
-                load_near<D>(g, ne[u], nj[u], a);
+                load_near<D>(g, base + (t + u) * SLICE, nj[u], a);
                 s.add(g, a, xi, centroid, Wi, ci, ri, courant);
             }
         smallest = s.smallest;
@@ -265,8 +257,7 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
                     s.hi[v] = -1.7976931348623157e308;
                 }
                 for (int k = 0; k < count; ++k) {
-                    const int e = g.list[base + k * SLICE], f = e >= 0 ? e : ~e;
-                    if (g.face[(size_t)f * FACE_ROW + FA::MAG] < tiny) continue;
+                    if (g.slot[base + k * SLICE] < tiny) continue;
                     const double* row = g.cell + (size_t)g.other[base + k * SLICE] * CELL_ROW + C::W;
 #pragma unroll
                     for (int v = 0; v < V; ++v) {
@@ -276,7 +267,7 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
                 }
             }
         }
-        double grad[V][D];
+        double grad[V][D], intercept[V];
         if constexpr (Grad) {  // Sweep::solve (:590-615): Cholesky of M, then M x = b per variable
             double L[U][U];
 #pragma unroll
@@ -311,6 +302,7 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
                 }
 #pragma unroll
                 for (int a = 0; a < D; ++a) grad[v][a] = x[1 + a];
+                intercept[v] = x[0];
             }
         }
         double wi[D];
@@ -351,9 +343,11 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
             smallest = fmin(smallest, courant / rate);
         }
         if constexpr (Grad) {
-            if constexpr (Limit) {  // the face-centroid limiter, face by face in list order (:538-565)
-                // Geometry::face_offset (:322-326) of UL entries loaded together, then applied in order.
+            if constexpr (Limit > 0) {  // the face-centroid limiter, face by face in list order (:538-565)
+                // Geometry::face_offset (:322-326), kept per slot, of UL entries loaded together, then applied in order.
                 constexpr int UL = LUNA_LIMIT_UNROLL;
+                Record<D, Limit == 2> rec;
+                rec.init(grad);
                 for (int t2 = 0; t2 < count; t2 += UL) {
                     double offset[UL][D];
                     bool use[UL];
@@ -361,26 +355,16 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
                     for (int u = 0; u < UL; ++u) {
                         use[u] = false;
                         if (t2 + u < count) {
-                            const int slot = base + (t2 + u) * SLICE;
-                            const int e = g.list[slot], f = e >= 0 ? e : ~e;
-                            const double* face = g.face + (size_t)f * FACE_ROW;
-                            use[u] = !(any_tiny && face[FA::MAG] < tiny);
-                            double rel[D];
-                            load_vec<D>(face + FA::FC, rel);
-                            if (e < 0) {  // the face centroid is stored relative to the owner: move it to this generator
-                                double xo[D], sep[D];
-                                load_vec<D>(g.cell + (size_t)g.other[slot] * CELL_ROW + C::X, xo);
-                                separation<D>(g, xi, xo, sep);
+                            const long long slot = base + (t2 + u) * SLICE;
+                            use[u] = !(any_tiny && g.slot[slot] < tiny);
 #pragma unroll
-                                for (int a = 0; a < D; ++a) rel[a] = rel[a] + sep[a];
-                            }
-#pragma unroll
-                            for (int a = 0; a < D; ++a) offset[u][a] = rel[a] - centroid[a];
+                            for (int a = 0; a < D; ++a) offset[u][a] = g.slot[(a + 1) * g.slots + slot];
                         }
                     }
 #pragma unroll
                     for (int u = 0; u < UL; ++u) {
                         if (!use[u]) continue;
+                        rec.face(offset[u]);
 #pragma unroll
                         for (int v = 0; v < V; ++v) {
                             const double dp = dot<D>(grad[v], offset[u]);
@@ -388,13 +372,29 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
                                 const double sc = s.hi[v] > Wi[v] ? (s.hi[v] - Wi[v]) / dp : 0.0;
 #pragma unroll
                                 for (int a = 0; a < D; ++a) grad[v][a] = grad[v][a] * sc;
+                                rec.clip(v, sc);
                             } else if (dp < 0 && Wi[v] + dp < s.lo[v]) {
                                 const double sc = s.lo[v] < Wi[v] ? (s.lo[v] - Wi[v]) / dp : 0.0;
 #pragma unroll
                                 for (int a = 0; a < D; ++a) grad[v][a] = grad[v][a] * sc;
+                                rec.clip(v, sc);
                             }
                         }
                     }
+                }
+                if constexpr (Limit == 2) {
+                    Cell<D> cell;
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        cell.W[v] = Wi[v];
+                        cell.lo[v] = s.lo[v];
+                        cell.hi[v] = s.hi[v];
+                        cell.intercept[v] = intercept[v];
+                    }
+                    cell.cs = ci;
+                    cell.r = ri;
+                    cell.count = count;
+                    r.troubled[i] = limiter_features<D>(cell, rec, r.features + (size_t)i * V * FEATURES);
                 }
             }
             double* out = r.grad + (size_t)i * GR;
@@ -676,7 +676,7 @@ hipError_t launch_primitives(hipStream_t s, const Geo& g, const Rows& r, const G
     return hipGetLastError();
 }
 
-template <int D, bool Moving, bool Grad, bool Limit>
+template <int D, bool Moving, bool Grad, int Limit>
 static hipError_t sweep_rule(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double courant, int rule, const Counters& c) {
     const dim3 grid = blocks(g.N > 0 ? g.N : 1, CELL_BLOCK);
     if (rule == 1)
@@ -687,16 +687,16 @@ static hipError_t sweep_rule(hipStream_t s, const Geo& g, const Rows& r, const G
 }
 
 template <int D>
-hipError_t launch_sweep(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double courant, bool moving, bool grad, bool limit,
+hipError_t launch_sweep(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double courant, bool moving, bool grad, int limit,
                         int rule, const Counters& c) {
     if (moving) {
-        if (grad) return limit ? sweep_rule<D, true, true, true>(s, g, r, gas, courant, rule, c)
-                               : sweep_rule<D, true, true, false>(s, g, r, gas, courant, rule, c);
-        return sweep_rule<D, true, false, false>(s, g, r, gas, courant, rule, c);
+        if (!grad) return sweep_rule<D, true, false, 0>(s, g, r, gas, courant, rule, c);
+        if (limit == 2) return sweep_rule<D, true, true, 2>(s, g, r, gas, courant, rule, c);
+        return limit ? sweep_rule<D, true, true, 1>(s, g, r, gas, courant, rule, c) : sweep_rule<D, true, true, 0>(s, g, r, gas, courant, rule, c);
     }
-    if (grad) return limit ? sweep_rule<D, false, true, true>(s, g, r, gas, courant, rule, c)
-                           : sweep_rule<D, false, true, false>(s, g, r, gas, courant, rule, c);
-    return sweep_rule<D, false, false, false>(s, g, r, gas, courant, rule, c);
+    if (!grad) return sweep_rule<D, false, false, 0>(s, g, r, gas, courant, rule, c);
+    if (limit == 2) return sweep_rule<D, false, true, 2>(s, g, r, gas, courant, rule, c);
+    return limit ? sweep_rule<D, false, true, 1>(s, g, r, gas, courant, rule, c) : sweep_rule<D, false, true, 0>(s, g, r, gas, courant, rule, c);
 }
 
 template <int D, int Pass>
@@ -740,7 +740,7 @@ hipError_t launch_keep_centroids(hipStream_t s, const Geo& g, const Rows& r, con
 
 #define LUNA_INSTANTIATE(D)                                                                                                          \
     template hipError_t launch_primitives<D>(hipStream_t, const Geo&, const Rows&, const Gas&);                                     \
-    template hipError_t launch_sweep<D>(hipStream_t, const Geo&, const Rows&, const Gas&, double, bool, bool, bool, int,             \
+    template hipError_t launch_sweep<D>(hipStream_t, const Geo&, const Rows&, const Gas&, double, bool, bool, int, int,              \
                                         const Counters&);                                                                           \
     template hipError_t launch_flux<D>(hipStream_t, const Geo&, const Rows&, const Gas&, int, int, int, double, double,              \
                                        unsigned long long*);                                                                        \

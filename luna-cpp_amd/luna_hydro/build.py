@@ -16,7 +16,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["hydro/mesh_kernels.hip", "hydro/line_kernels.hip", "hydro/adjoint_kernels.hip", "hydro/api.hip"]
+HIP_SOURCES = ["hydro/mesh_kernels.hip", "hydro/line_kernels.hip", "hydro/adjoint_kernels.hip", "hydro/limiter_kernels.hip",
+               "hydro/api.hip"]
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
 

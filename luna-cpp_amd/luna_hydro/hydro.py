@@ -19,7 +19,8 @@ _ip = ctypes.POINTER(ctypes.c_int)
 _lp = ctypes.POINTER(ctypes.c_longlong)
 
 OK, ERR_ARGUMENT, ERR_LOGIC, ERR_DEVICE, ERR_MEMORY = range(5)
-LIMITERS = {"face": 0, "none": 1, "minmod": 2, "vanleer": 3}
+LIMITERS = {"face": 0, "none": 1, "minmod": 2, "vanleer": 3, "learned": 4}
+LIMITER_FEATURES, LIMITER_PARAMS = 16, 5313
 RIEMANN = {"hllc": 0, "hll": 1}
 MOTION = {"static": 0, "moving": 1}
 RULES = {"arepo": 0, "luna": 1}
@@ -98,7 +99,9 @@ def exported_symbols():
              "luna_fv_profile_read", "luna_fv1_create", "luna_fv1_destroy", "luna_fv1_set_state", "luna_fv1_get_state",
              "luna_fv1_time_step", "luna_fv1_step", "luna_fv1_step_and_time_step", "luna_fv1_step_tangent", "luna_fv1_step_adjoint",
              "luna_last_error",
-             "luna_fv_set_geometry", "luna_synchronize", "luna_fv_step_tangent", "luna_fv_step_adjoint"]
+             "luna_fv_set_geometry", "luna_synchronize", "luna_fv_step_tangent", "luna_fv_step_adjoint",
+             "luna_fv_set_limiter_network", "luna_fv_limiter_features", "luna_mlp_create", "luna_mlp_destroy", "luna_mlp_set_params",
+             "luna_mlp_get_params", "luna_mlp_forward", "luna_mlp_gradient", "luna_mlp_train_step", "luna_mlp_synchronize"]
     return {n: hasattr(lib(), n) for n in names}
 
 
@@ -186,6 +189,24 @@ class FiniteVolume:
             raise ValueError("%s: N x V rows" % what)
         _check(fn(self._h, ctypes.c_double(dt), out.ctypes.data_as(_dp), 0), what)
         return out
+
+    def set_limiter_network(self, params):
+        """The learned limiter's network (LIMITER_PARAMS FP32 values), or None to remove it (then the learned mode limits
+        as the face-centroid limiter and still records features)."""
+        if params is None:
+            _check(lib().luna_fv_set_limiter_network(self._h, None, 0), "set_limiter_network")
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        _check(lib().luna_fv_set_limiter_network(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size),
+               "set_limiter_network")
+
+    def limiter_features(self):
+        """The last sweep's learned-limiter features, (N V) x 16 BF16 bit patterns (uint16), and troubled flags (N, uint8)."""
+        x = np.empty((self.N * self.V, LIMITER_FEATURES), dtype=np.uint16)
+        t = np.empty(self.N, dtype=np.uint8)
+        _check(lib().luna_fv_limiter_features(self._h, x.ctypes.data_as(ctypes.POINTER(ctypes.c_ushort)),
+                                              t.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))), "limiter_features")
+        return x, t
 
     def moving_begin(self, dt, positions=True):
         """The first half of a moving step; returns the new positions (None with positions=False)."""

@@ -1,0 +1,400 @@
+"""The learned slope limiter (no counterpart in luna-cpp): its trainer on the GPU, the coarse/fine training pairs that the
+plain solver generates, and its evaluation against the plain method.
+
+    from luna_hydro import learned, hydro
+    x, t, w = learned.kh_pairs(n=64, seeds=[1, 2], marks=[0.5, 1.0])   # features, target fractions, weights
+    params, history = learned.train(x, t, w, steps=3000)
+    fv = hydro.FiniteVolume(mesh, 5 / 3, limiter="learned"); fv.set_limiter_network(params)
+
+The limiter (include/luna_hydro.h, LUNA_LIMITER_LEARNED) scales each cell's face-centroid-limited gradient of each
+variable by a fraction f = sigmoid(MLP(features)) in [0, 1], except in troubled cells. A training pair is a coarse
+mesh with the state restricted from a plain run at `factor` x the resolution: the features are those the coarse sweep
+records on that state, and the target is the fraction that best matches the fine data inside each coarse cell, the
+projection of the fine sub-cell slope g_fine on the face-centroid gradient g: t = clip(g_fine . g / |g|^2, 0, 1)
+(the limiter cannot steepen g). Rows are weighted by phi max(up, down) / S (features 0, 3, 4: the size of the limited
+face increments against the neighbours' range), and rows of troubled cells get weight 0.
+
+The network's kernels (BF16 operands, FP32 accumulation on the matrix cores, csrc/hydro/limiter_kernels.hip) run the
+training: forward, backward, a fixed-order reduction of the per-wavefront sums and Adam; torch only holds the device
+arrays. Every routine here fails without the HIP library (there is no CPU path).
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import hydro, mesh as meshlib, problems
+
+FEATURES, HIDDEN = 16, 64
+P_W1 = 0
+P_B1 = P_W1 + HIDDEN * FEATURES
+P_W2 = P_B1 + HIDDEN
+P_B2 = P_W2 + HIDDEN * HIDDEN
+P_W3 = P_B2 + HIDDEN
+P_B3 = P_W3 + HIDDEN
+PARAMS = P_B3 + 1
+assert PARAMS == hydro.LIMITER_PARAMS
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def init_params(seed=0):
+    """He-normal weights, zero biases, and b3 = 3 (every fraction ~0.95 at the start: near the face-centroid limiter)."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(PARAMS, dtype=np.float32)
+    p[P_W1:P_B1] = rng.normal(0, np.sqrt(2.0 / FEATURES), HIDDEN * FEATURES)
+    p[P_W2:P_B2] = rng.normal(0, np.sqrt(2.0 / HIDDEN), HIDDEN * HIDDEN)
+    p[P_W3:P_B3] = rng.normal(0, np.sqrt(1.0 / HIDDEN), HIDDEN)
+    p[P_B3] = 3.0
+    return p
+
+
+def bf16_value(bits):
+    """The float32 values of BF16 bit patterns."""
+    return (np.ascontiguousarray(bits, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("the learned limiter's trainer needs a GPU (there is no CPU path)")
+    return torch
+
+
+def device_rows(x_bits, target=None, weight=None):
+    """Feature rows (uint16 BF16 bits, rows x 16) and optional targets / weights as contiguous GPU tensors."""
+    torch = _torch()
+    x = torch.from_numpy(np.ascontiguousarray(x_bits, dtype=np.uint16).view(np.int16)).cuda()
+    out = [x]
+    for a in (target, weight):
+        if a is not None:
+            out.append(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda())
+    torch.cuda.synchronize()
+    return out if len(out) > 1 else out[0]
+
+
+class Trainer:
+    """luna_mlp: the network's FP32 parameters with Adam, and its BF16 copy, on the GPU."""
+
+    def __init__(self, params=None, seed=0):
+        self._torch = _torch()
+        h = ctypes.c_void_p()
+        hydro._check(hydro.lib().luna_mlp_create(ctypes.byref(h)), "luna_mlp_create")
+        self._h = h
+        self.set_params(init_params(seed) if params is None else params)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            hydro.lib().luna_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != PARAMS:
+            raise ValueError("expected %d parameters" % PARAMS)
+        hydro._check(hydro.lib().luna_mlp_set_params(self._h, p.ctypes.data_as(_fp), PARAMS), "luna_mlp_set_params")
+
+    def params(self):
+        p = np.empty(PARAMS, dtype=np.float32)
+        hydro._check(hydro.lib().luna_mlp_get_params(self._h, p.ctypes.data_as(_fp), PARAMS), "luna_mlp_get_params")
+        return p
+
+    def _ready(self, *tensors):
+        for t in tensors:
+            if not (t.is_cuda and t.is_contiguous()):
+                raise ValueError("contiguous GPU tensors expected")
+        self._torch.cuda.synchronize()  # the kernels run on the trainer's own stream
+
+    def forward(self, x):
+        """sigmoid(z) of each feature row (x: GPU int16 tensor, rows x 16 BF16 bits) as a GPU float32 tensor."""
+        self._ready(x)
+        rows = x.shape[0]
+        out = self._torch.empty(rows, dtype=self._torch.float32, device=x.device)
+        hydro._check(hydro.lib().luna_mlp_forward(self._h, ctypes.c_void_p(x.data_ptr()), rows, ctypes.c_void_p(out.data_ptr())),
+                     "luna_mlp_forward")
+        return out
+
+    def gradient(self, x, target, weight, weight_sum=None):
+        """(loss, gradient in the parameter order) of a batch."""
+        self._ready(x, target, weight)
+        ws = float(weight.sum().item()) if weight_sum is None else float(weight_sum)
+        g = np.empty(PARAMS, dtype=np.float32)
+        loss = ctypes.c_float()
+        hydro._check(hydro.lib().luna_mlp_gradient(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(target.data_ptr()),
+                                                   ctypes.c_void_p(weight.data_ptr()), x.shape[0], ctypes.c_double(ws),
+                                                   g.ctypes.data_as(_fp), ctypes.byref(loss)), "luna_mlp_gradient")
+        return float(loss.value), g
+
+    def step(self, x, target, weight, weight_sum, lr, want_loss=False):
+        """One Adam step on a batch (inputs ready on the GPU); returns the loss before it if want_loss (blocks)."""
+        loss = ctypes.c_float()
+        hydro._check(hydro.lib().luna_mlp_train_step(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(target.data_ptr()),
+                                                     ctypes.c_void_p(weight.data_ptr()), x.shape[0], ctypes.c_double(weight_sum),
+                                                     ctypes.c_float(lr), ctypes.byref(loss) if want_loss else None),
+                     "luna_mlp_train_step")
+        return float(loss.value) if want_loss else None
+
+    def synchronize(self):
+        hydro._check(hydro.lib().luna_mlp_synchronize(self._h), "luna_mlp_synchronize")
+
+
+def train(x_bits, target, weight, steps=3000, batch=65536, lr=1e-3, seed=0, params=None, log=None, every=500):
+    """Adam on the weighted squared error of sigmoid(MLP(x)) against the targets, mini-batches drawn without
+    replacement (seeded); returns (parameters, [(step, loss of the whole set)])."""
+    torch = _torch()
+    x, t, w = device_rows(x_bits, target, weight)
+    rows = x.shape[0]
+    batch = min(batch, rows)
+    trainer = Trainer(params=params, seed=seed)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    total = float(w.sum().item())
+    history = []
+
+    def whole():
+        return trainer.gradient(x, t, w, total)[0]
+
+    history.append((0, whole()))
+    perm, at = torch.randperm(rows, device="cuda", generator=gen), 0
+    for k in range(1, steps + 1):
+        if at + batch > rows:
+            perm, at = torch.randperm(rows, device="cuda", generator=gen), 0
+        idx = perm[at:at + batch]
+        at += batch
+        xb, tb, wb = x[idx].contiguous(), t[idx].contiguous(), w[idx].contiguous()
+        wsum = float(wb.sum().item())  # also waits for torch's gathers
+        if wsum > 0:
+            trainer.step(xb, tb, wb, wsum, lr)
+            trainer.synchronize()  # before torch may reuse the batch's memory
+        if k % every == 0 or k == steps:
+            trainer.synchronize()
+            history.append((k, whole()))
+            if log:
+                log("  train step %d: loss %.6g" % history[-1])
+    p = trainer.params()
+    trainer.close()
+    return p, history
+
+
+# ------------------------------------------------------------------------------------------------ training pairs
+
+
+def primitives(q, volume, gamma, D):
+    """W (N x V) of the extensive Q (host, numpy)."""
+    U = np.asarray(q) / np.asarray(volume)[:, None]
+    rho = U[:, 0]
+    v = U[:, 1:D + 1] / rho[:, None]
+    P = (gamma - 1.0) * (U[:, D + 1] - 0.5 * rho * (v * v).sum(axis=1))
+    return np.column_stack([rho, v, P])
+
+
+def run_to(fv, marks, courant, log=None):
+    """Steps a static solver to each time in marks (the last step of each stretch shortened to land on it); returns the
+    states at the marks and the number of steps."""
+    t, out, steps = 0.0, [], 0
+    for mark in marks:
+        while t < mark * (1 - 1e-12):
+            dt = fv.time_step(courant)
+            if not (dt > 0):
+                raise RuntimeError("dt <= 0")
+            dt = min(dt, mark - t)
+            fv.step(dt)
+            t += dt
+            steps += 1
+        out.append(fv.state())
+        if log:
+            log("  t = %.3f after %d steps" % (t, steps))
+    return out, steps
+
+
+def owners(coarse, fine):
+    """The coarse cell holding each fine generator: the nearest coarse generator (periodic box)."""
+    from scipy.spatial import cKDTree
+
+    D = coarse["dim"]
+    side = np.asarray(coarse["side"][:D], dtype=np.float64)
+    xc = np.asarray(coarse["positions"]).reshape(-1, D) % side
+    xf = np.asarray(fine["positions"]).reshape(-1, D) % side
+    return cKDTree(xc, boxsize=side).query(xf)[1]
+
+
+def restrict(q_fine, fine, coarse, owner):
+    """The fine state on the coarse cells: conserved densities averaged over the fine cells each coarse cell holds,
+    times the coarse volume (so the coarse densities are the fine averages)."""
+    Nc, V = coarse["cells"], q_fine.shape[1]
+    vol_f = np.asarray(fine["volume"])
+    mass = np.zeros((Nc, V))
+    for v in range(V):
+        mass[:, v] = np.bincount(owner, weights=q_fine[:, v], minlength=Nc)
+    held = np.bincount(owner, weights=vol_f, minlength=Nc)
+    if np.any(held <= 0):
+        raise ValueError("a coarse cell holds no fine generator")
+    return mass / held[:, None] * np.asarray(coarse["volume"])[:, None]
+
+
+def sub_cell_slopes(W_fine, fine, coarse, owner):
+    """Per coarse cell and variable, the volume-weighted least-squares slope (with intercept) of the fine primitives
+    against the fine generators' positions (periodic separations from the coarse generator): N x V x D."""
+    D, Nc = coarse["dim"], coarse["cells"]
+    side = np.asarray(coarse["side"][:D], dtype=np.float64)
+    xc = np.asarray(coarse["positions"]).reshape(-1, D)
+    xf = np.asarray(fine["positions"]).reshape(-1, D)
+    d = xf - xc[owner]
+    d -= side * np.round(d / side)
+    w = np.asarray(fine["volume"])
+    V = W_fine.shape[1]
+
+    def s(vals):
+        return np.bincount(owner, weights=vals, minlength=Nc)
+
+    S0 = s(w)
+    S1 = np.stack([s(w * d[:, a]) for a in range(D)], axis=1)
+    S2 = np.stack([np.stack([s(w * d[:, a] * d[:, b]) for b in range(D)], axis=1) for a in range(D)], axis=1)
+    T0 = np.stack([s(w * W_fine[:, v]) for v in range(V)], axis=1)
+    T1 = np.stack([np.stack([s(w * d[:, a] * W_fine[:, v]) for v in range(V)], axis=1) for a in range(D)], axis=1)  # Nc x D x V
+    C = S2 - S1[:, :, None] * S1[:, None, :] / S0[:, None, None]
+    B = T1 - S1[:, :, None] * T0[:, None, :] / S0[:, None, None]
+    C += np.eye(D)[None] * 1e-30
+    g = np.linalg.solve(C, B)  # Nc x D x V
+    return np.transpose(g, (0, 2, 1))
+
+
+def pair_rows(coarse, q_coarse, g_fine, gamma, motion="static"):
+    """Feature rows, targets and weights of one coarse state whose ideal slopes are g_fine (N x V x D)."""
+    D, N = coarse["dim"], coarse["cells"]
+    V = D + 2
+    fv = hydro.FiniteVolume(coarse, gamma, limiter="learned", motion=motion, step_rule="luna" if D == 2 else "arepo")
+    try:
+        fv.set_state(q_coarse)
+        fv.time_step(1.0)
+        x, troubled = fv.limiter_features()
+        g = np.empty((N, V * D))
+        hydro._check(hydro.lib().luna_fv_gradients(fv._h, g.ctypes.data_as(hydro._dp)), "gradients")
+    finally:
+        fv.close()
+    g = g.reshape(N * V, D)
+    gf = g_fine.reshape(N * V, D)
+    gg = (g * g).sum(axis=1)
+    t = np.where(gg > 0, (gf * g).sum(axis=1) / np.where(gg > 0, gg, 1.0), 1.0)
+    t = np.clip(t, 0.0, 1.0)
+    f = bf16_value(x)
+    w = np.clip(f[:, 0] * np.maximum(f[:, 3], f[:, 4]), 0.0, 1.0).astype(np.float64)
+    w[np.repeat(troubled.astype(bool), V)] = 0.0
+    w[gg == 0] = 0.0
+    return x, t.astype(np.float32), w.astype(np.float32)
+
+
+def kh_mesh(n, seed, jitter=0.25):
+    return meshlib.build(problems.jittered_lattice(n, 2, jitter, seed), 1.0)
+
+
+def kh_pairs(n=64, seeds=(1,), marks=(0.5, 1.0), factor=4, courant=0.8, gamma=5.0 / 3.0, log=None):
+    """Training rows of the Kelvin-Helmholtz shear layer: for each seed, a plain static run at factor x n per axis
+    (LUNA rule, CFL courant) to each mark, restricted to the n^2 mesh of the same seed."""
+    xs, ts, ws = [], [], []
+    for seed in seeds:
+        coarse = kh_mesh(n, seed)
+        fine = kh_mesh(factor * n, seed + 1000)
+        owner = owners(coarse, fine)
+        fv = hydro.FiniteVolume(fine, gamma, step_rule="luna")
+        try:
+            fv.set_state(problems.kelvin_helmholtz_state(fine, gamma))
+            states, _ = run_to(fv, list(marks), courant, log)
+        finally:
+            fv.close()
+        for q in [problems.kelvin_helmholtz_state(fine, gamma)] + states:
+            W_f = primitives(q, fine["volume"], gamma, 2)
+            qc = restrict(q, fine, coarse, owner)
+            x, t, w = pair_rows(coarse, qc, sub_cell_slopes(W_f, fine, coarse, owner), gamma)
+            xs.append(x)
+            ts.append(t)
+            ws.append(w)
+        if log:
+            log("  seed %d: %d rows so far" % (seed, sum(len(a) for a in ts)))
+    return np.concatenate(xs), np.concatenate(ts), np.concatenate(ws)
+
+
+# ------------------------------------------------------------------------------------------------ evaluation
+
+
+def run_timed(m, q0, t_end, gamma, courant, limiter="face", params=None, motion="static", log=None):
+    """A run to t_end; returns (final state, final mesh, GPU-side seconds of the hydro calls (the mesh producer
+    excluded), steps). The timed calls are fenced, so the seconds are the GPU work and its launches."""
+    D = m["dim"]
+    fv = hydro.FiniteVolume(m, gamma, limiter=limiter, motion=motion, step_rule="luna" if D == 2 else "arepo")
+    try:
+        if params is not None:
+            fv.set_limiter_network(params)
+        fv.set_state(q0)
+        t, steps, busy, current = 0.0, 0, 0.0, m
+        while t < t_end * (1 - 1e-12):
+            a = time.perf_counter()
+            dt = min(fv.time_step(courant), t_end - t)
+            if motion == "static":
+                fv.step(dt)
+                fv.fallbacks()
+                busy += time.perf_counter() - a
+            else:
+                x = fv.moving_begin(dt)
+                busy += time.perf_counter() - a
+                current = meshlib.build(x, np.asarray(m["side"][:D]))
+                a = time.perf_counter()
+                fv.moving_finish(current, dt)
+                fv.fallbacks()
+                busy += time.perf_counter() - a
+            t += dt
+            steps += 1
+        return fv.state(), current, busy, steps
+    finally:
+        fv.close()
+
+
+def density_l1(q, m, q_ref):
+    """sum_i V_i |rho_i - rho_ref_i| / sum_i V_i on the mesh m (q, q_ref extensive on m)."""
+    vol = np.asarray(m["volume"])
+    return float(np.sum(np.abs(q[:, 0] - q_ref[:, 0])) / np.sum(vol))
+
+
+def kh_curve(ns, seed, t_end, params, ref_n, courant=0.8, gamma=5.0 / 3.0, motion="static", log=None):
+    """Error against cost: plain and learned runs of the KH shear layer at each n in ns (meshes of `seed`) against one
+    plain reference run at ref_n (static), restricted to each run's final mesh. Returns rows
+    {"n", "limiter", "l1", "seconds", "steps"} and the reference's cost."""
+    fine = kh_mesh(ref_n, seed + 1000)
+    fq, fm, fine_s, fine_steps = run_timed(fine, problems.kelvin_helmholtz_state(fine, gamma), t_end, gamma, courant)
+    if log:
+        log("  reference n=%d: %.2f GPU-s, %d steps" % (ref_n, fine_s, fine_steps))
+    rows = []
+    for n in ns:
+        coarse = kh_mesh(n, seed)
+        q0 = problems.kelvin_helmholtz_state(coarse, gamma)
+        for name, limiter, p in (("plain", "face", None), ("learned", "learned", params)):
+            q, cm, secs, steps = run_timed(coarse, q0, t_end, gamma, courant, limiter, p, motion)
+            l1 = density_l1(q, cm, restrict(fq, fm, cm, owners(cm, fm)))
+            rows.append({"n": n, "limiter": name, "l1": l1, "seconds": secs, "steps": steps})
+            if log:
+                log("  %s n=%d: L1 %.5g in %.3f GPU-s (%d steps)" % (name, n, l1, secs, steps))
+    return rows, {"n": ref_n, "seconds": fine_s, "steps": fine_steps}
+
+
+def kh_compare(n, seed, t_end, params, factor=4, courant=0.8, gamma=5.0 / 3.0, motion="static", log=None):
+    """Plain and learned runs of the KH shear layer on the n^2 mesh of `seed` against the plain run at factor x n
+    restricted to the final coarse mesh: density L1 errors and GPU-seconds."""
+    coarse = kh_mesh(n, seed)
+    fine = kh_mesh(factor * n, seed + 1000)
+    fq, fm, fine_s, fine_steps = run_timed(fine, problems.kelvin_helmholtz_state(fine, gamma), t_end, gamma, courant)
+    out = {"n": n, "seed": seed, "t_end": t_end, "motion": motion, "reference": {"n": factor * n, "seconds": fine_s, "steps": fine_steps}}
+    q0 = problems.kelvin_helmholtz_state(coarse, gamma)
+    for name, limiter, p in (("plain", "face", None), ("learned", "learned", params)):
+        q, cm, secs, steps = run_timed(coarse, q0, t_end, gamma, courant, limiter, p, motion, log)
+        ref = restrict(fq, fm, cm, owners(cm, fm))
+        out[name] = {"l1": density_l1(q, cm, ref), "seconds": secs, "steps": steps}
+        if log:
+            log("  %s n=%d: L1 %.4g in %.2f GPU-s (%d steps)" % (name, n, out[name]["l1"], secs, steps))
+    return out

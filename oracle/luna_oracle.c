@@ -246,7 +246,7 @@ void orc_primitives(orc_fv* fv, const orc_geometry* geo) {
 }
 
 /* finite_volume.hpp:590-615 Sweep::solve(): Cholesky of M (upper triangle), then M x_v = b_v. */
-static void solve_fit(int D, double M[MAXD + 1][MAXD + 1], double b[MAXV][MAXD + 1], double* grad) {
+static void solve_fit(int D, double M[MAXD + 1][MAXD + 1], double b[MAXV][MAXD + 1], double* grad, double* intercept) {
     const int U = D + 1, V = D + 2;
     double L[MAXD + 1][MAXD + 1];
     memset(L, 0, sizeof L);
@@ -269,7 +269,100 @@ static void solve_fit(int D, double M[MAXD + 1][MAXD + 1], double b[MAXV][MAXD +
             x[p] = sum / L[p][p];
         }
         for (int a = 0; a < D; ++a) grad[v * D + a] = x[1 + a];
+        if (intercept) intercept[v] = x[0];
     }
+}
+
+/* The learned limiter's record of the limiter walk (no counterpart in luna-cpp): the unlimited gradient, the product of
+ * the clip factors, the extreme unlimited face-centroid increments (0 included) and the clip count of each variable. */
+typedef struct {
+    double g0[MAXV * MAXD], phi[MAXV], up[MAXV], down[MAXV];
+    int clips[MAXV];
+} limiter_record;
+
+static void record_init(limiter_record* r, int D, const double* grad) {
+    for (int v = 0; v < D + 2; ++v) {
+        for (int a = 0; a < D; ++a) r->g0[v * D + a] = grad[v * D + a];
+        r->phi[v] = 1.0;
+        r->up[v] = 0.0;
+        r->down[v] = 0.0;
+        r->clips[v] = 0;
+    }
+}
+static void record_face(limiter_record* r, int D, const double* offset) {
+    for (int v = 0; v < D + 2; ++v) {
+        const double dp = dot(D, r->g0 + v * D, offset);
+        r->up[v] = fmax(r->up[v], dp);
+        r->down[v] = fmin(r->down[v], dp);
+    }
+}
+static void record_clip(limiter_record* r, int v, double s) {
+    r->phi[v] = r->phi[v] * s;
+    r->clips[v] += 1;
+}
+
+/* Round to BF16 bits: double to float (nearest even), then float to BF16 (nearest even); NaN stays a quiet NaN. */
+unsigned short orc_bf16_bits(double x) {
+    const float f = (float)x;
+    unsigned u;
+    memcpy(&u, &f, sizeof u);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+static double clamp8(double x) { return fmin(fmax(x, -8.0), 8.0); }
+
+/* The 16 features of each variable and the troubled flag (the definitions are those of include/luna_hydro.h,
+ * luna_fv_limiter_features). */
+static unsigned char limiter_features(int D, const double* W, const double* lo, const double* hi, const double* intercept, double cs,
+                                      double r, int count, const limiter_record* rec, unsigned short* out) {
+    const int V = D + 2;
+    const double inv_c = 1.0 / cs;
+    double speed2 = 0, div = 0;
+    for (int a = 0; a < D; ++a) {
+        speed2 += W[1 + a] * W[1 + a];
+        div += rec->g0[(1 + a) * D + a];
+    }
+    double curl;
+    if (D == 2) {
+        curl = fabs(rec->g0[2 * D + 0] - rec->g0[1 * D + 1]);
+    } else {
+        const double cx = rec->g0[3 * D + 1] - rec->g0[2 * D + 2], cy = rec->g0[1 * D + 2] - rec->g0[3 * D + 0],
+                     cz = rec->g0[2 * D + 0] - rec->g0[1 * D + 1];
+        curl = sqrt(cx * cx + cy * cy + cz * cz);
+    }
+    double phimin = rec->phi[0];
+    for (int v = 1; v < V; ++v) phimin = fmin(phimin, rec->phi[v]);
+    const double rho_jump = (hi[0] - lo[0]) / W[0], p_jump = (hi[V - 1] - lo[V - 1]) / W[V - 1];
+    const double compression = div * r * inv_c;
+    for (int v = 0; v < V; ++v) {
+        const double scale = (v > 0 && v < V - 1) ? cs : fabs(W[v]);
+        const double S = (hi[v] - lo[v]) + 1e-12 * scale + 1e-300;
+        const double inv = 1.0 / S;
+        double f[ORC_FEATURES];
+        f[0] = rec->phi[v];
+        f[1] = clamp8((hi[v] - W[v]) * inv);
+        f[2] = clamp8((W[v] - lo[v]) * inv);
+        f[3] = clamp8(rec->up[v] * inv);
+        f[4] = clamp8(-rec->down[v] * inv);
+        f[5] = clamp8(intercept[v] * inv);
+        f[6] = (double)rec->clips[v] / (double)count;
+        f[7] = phimin;
+        f[8] = clamp8(rho_jump);
+        f[9] = clamp8(p_jump);
+        f[10] = clamp8(compression);
+        f[11] = clamp8(curl * r * inv_c);
+        f[12] = clamp8(sqrt(speed2) * inv_c);
+        f[13] = v == 0 ? 1.0 : 0.0;
+        f[14] = v == V - 1 ? 1.0 : 0.0;
+        f[15] = clamp8(S / (scale + 1e-300));
+        /* + 0.0: a zero is always +0 (fmin / fmax may return either zero of a +0 / -0 tie) */
+        for (int k = 0; k < ORC_FEATURES; ++k) out[v * ORC_FEATURES + k] = orc_bf16_bits(f[k] + 0.0);
+    }
+    const int shock = compression < -0.05 && p_jump > 0.5;
+    const int thin = lo[0] < 0.1 * W[0] || lo[V - 1] < 0.1 * W[V - 1];
+    return (unsigned char)((shock || thin) ? 1 : 0);
 }
 
 /* finite_volume.hpp:463-569 Sweep::operator() (double policy, the face-centroid limiter or none), with the moving mesh's
@@ -332,8 +425,8 @@ static void sweep_cell(orc_fv* fv, const orc_geometry* geo, int i) {
             }
         }
     }
-    double grad[MAXV * MAXD];
-    if (grad_on) solve_fit(D, M, b, grad);
+    double grad[MAXV * MAXD], intercept[MAXV];
+    if (grad_on) solve_fit(D, M, b, grad, intercept);
     if (fv->moving) {
         const double shaping = 0.5, factor = 2.25;
         double fraction = 0;
@@ -349,24 +442,32 @@ static void sweep_cell(orc_fv* fv, const orc_geometry* geo, int i) {
         for (int a = 0; a < D; ++a) fv->w[(size_t)i * D + a] = wi[a];
     }
     if (!grad_on) return;
-    if (fv->limiter == 0) {
+    if (fv->limiter == 0 || fv->limiter == 2) {
+        limiter_record rec;
+        record_init(&rec, D, grad);
         for (int k = first; k < end; ++k) {
             const int f = list_face(geo->list[k]);
             if (any_tiny && magnitude_of(geo, f) < tiny) continue;
             double offset[MAXD];
             face_offset(geo, f, i, offset);
+            record_face(&rec, D, offset);
             for (int v = 0; v < V; ++v) {
                 double* g = grad + v * D;
                 const double dp = dot(D, g, offset);
                 if (dp > 0 && Wi[v] + dp > hi[v]) {
                     const double s = hi[v] > Wi[v] ? (hi[v] - Wi[v]) / dp : 0.0;
                     for (int a = 0; a < D; ++a) g[a] = g[a] * s;
+                    record_clip(&rec, v, s);
                 } else if (dp < 0 && Wi[v] + dp < lo[v]) {
                     const double s = lo[v] < Wi[v] ? (lo[v] - Wi[v]) / dp : 0.0;
                     for (int a = 0; a < D; ++a) g[a] = g[a] * s;
+                    record_clip(&rec, v, s);
                 }
             }
         }
+        if (fv->limiter == 2)
+            fv->troubled[i] = limiter_features(D, Wi, lo, hi, intercept, Wi[V], Wi[V + 1], end - first, &rec,
+                                               fv->features + (size_t)i * V * ORC_FEATURES);
     }
     memcpy(fv->grad + (size_t)i * V * D, grad, sizeof(double) * V * D);
 }

@@ -53,7 +53,11 @@ typedef struct {
     long long period[3];         /* grid steps per side */
 } orc_geometry;
 
-/* Solver state (arrays owned by the caller). limiter: 0 face centroid, 1 none. riemann: 0 HLLC, 1 HLL.
+#define ORC_FEATURES 16 /* features per cell and variable of the learned limiter */
+unsigned short orc_bf16_bits(double x);
+
+/* Solver state (arrays owned by the caller). limiter: 0 face centroid, 1 none, 2 face centroid recording the learned
+ * limiter's features and troubled flags (the gradients are those of 0). riemann: 0 HLLC, 1 HLL.
  * step_rule: 0 AREPO, 1 LUNA (2D only). */
 typedef struct {
     int D, N, order, limiter, riemann, moving, step_rule;
@@ -68,6 +72,8 @@ typedef struct {
     double* flux;         /* F x V scratch (at least the largest mesh's F) */
     long long fallbacks[2];
     int prims_ready;
+    unsigned short* features; /* limiter 2 (learned): N V x 16 BF16 bit patterns, written by each sweep */
+    unsigned char* troubled;  /* limiter 2: N */
 } orc_fv;
 
 void orc_primitives(orc_fv* fv, const orc_geometry* geo);

@@ -32,7 +32,8 @@ class _FV(ctypes.Structure):
     _fields_ = [("D", ctypes.c_int), ("N", ctypes.c_int), ("order", ctypes.c_int), ("limiter", ctypes.c_int),
                 ("riemann", ctypes.c_int), ("moving", ctypes.c_int), ("step_rule", ctypes.c_int), ("gas", Gas),
                 ("q", _dp), ("prims", _dp), ("grad", _dp), ("w", _dp), ("x", _dp), ("old_centroid", _dp),
-                ("steps", _ip), ("flux", _dp), ("fallbacks", ctypes.c_longlong * 2), ("prims_ready", ctypes.c_int)]
+                ("steps", _ip), ("flux", _dp), ("fallbacks", ctypes.c_longlong * 2), ("prims_ready", ctypes.c_int),
+                ("features", ctypes.POINTER(ctypes.c_ushort)), ("troubled", ctypes.POINTER(ctypes.c_ubyte))]
 
 
 class _Line(ctypes.Structure):
@@ -202,7 +203,7 @@ class FiniteVolume:
         self.flux = np.zeros((max(1, mesh["faces"]), V))
         c = _FV()
         c.D, c.N, c.order = D, N, order
-        c.limiter = {"face": 0, "none": 1}[limiter]
+        c.limiter = {"face": 0, "none": 1, "learned": 2}[limiter]
         c.riemann = {"hllc": 0, "hll": 1}[riemann]
         c.moving = int(moving)
         c.step_rule = {"arepo": 0, "luna": 1}[step_rule]
@@ -210,6 +211,12 @@ class FiniteVolume:
         c.q, c.prims, c.grad, c.w = _ptr(self.q), _ptr(self.prims), _ptr(self.grad), _ptr(self.w)
         c.x, c.old_centroid, c.steps, c.flux = _ptr(self.x), _ptr(self.old_centroid), _ptr(self.steps, _ip), _ptr(self.flux)
         c.prims_ready = 0
+        # the learned limiter's record of each sweep (limiter "learned"; its gradients are the face limiter's until
+        # apply_fractions scales them)
+        self.features = np.zeros((N * V, 16), dtype=np.uint16)
+        self.troubled = np.zeros(N, dtype=np.uint8)
+        c.features = self.features.ctypes.data_as(ctypes.POINTER(ctypes.c_ushort))
+        c.troubled = self.troubled.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
         self.c = c
         self.set_mesh(mesh)
 
@@ -228,6 +235,15 @@ class FiniteVolume:
 
     def step(self, dt):
         lib().orc_step_static(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(dt))
+
+    def apply_fractions(self, fraction):
+        """The learned limiter's scaling of the last sweep's gradients: row i V + v of fraction (float32, N V) scales
+        variable v of cell i, in double, unless the cell is troubled (the device's k_limiter_apply)."""
+        frac = np.asarray(fraction, dtype=np.float32).reshape(self.N, self.V).astype(np.float64)
+        keep = self.troubled.astype(bool)
+        frac[keep] = 1.0
+        g = self.grad.reshape(self.N, self.V, self.D)
+        g *= frac[:, :, None]
 
     def step_tangent(self, dt, dq):
         """The static order-1 step of q and of its tangent dq (N x V, extensive): returns J dq; q advances."""
