@@ -2,7 +2,7 @@
 plain solver generates, and its evaluation against the plain method.
 
     from luna_hydro import learned, hydro
-    x, t, w = learned.kh_pairs(n=64, seeds=[1, 2], marks=[0.5, 1.0])   # features, target fractions, weights
+    x, t, w = learned.pairs(learned.KH, n=64, seeds=[1, 2], marks=[0.5, 1.0])   # features, target fractions, weights
     params, history = learned.train(x, t, w, steps=3000)
     fv = hydro.FiniteVolume(mesh, 5 / 3, limiter="learned"); fv.set_limiter_network(params)
 
@@ -291,33 +291,51 @@ def pair_rows(coarse, q_coarse, g_fine, gamma, motion="static"):
     return x, t.astype(np.float32), w.astype(np.float32)
 
 
-def kh_mesh(n, seed, jitter=0.25):
-    return meshlib.build(problems.jittered_lattice(n, 2, jitter, seed), 1.0)
+class Problem:
+    """A benchmark problem as the training and the evaluation use it: its dimension, CFL number (LUNA rule in 2D,
+    AREPO in 3D), the jittered mesh of n generators per axis and seed, and the initial state on a mesh of resolution n."""
+
+    def __init__(self, name, dim, courant, state):
+        self.name, self.dim, self.courant, self._state = name, dim, courant, state
+
+    def mesh(self, n, seed, jitter=0.25):
+        return meshlib.build(problems.jittered_lattice(n, self.dim, jitter, seed), 1.0)
+
+    def state(self, m, n, gamma=5.0 / 3.0):
+        return self._state(m, n, gamma)
 
 
-def kh_pairs(n=64, seeds=(1,), marks=(0.5, 1.0), factor=4, courant=0.8, gamma=5.0 / 3.0, log=None):
-    """Training rows of the Kelvin-Helmholtz shear layer: for each seed, a plain static run at factor x n per axis
-    (LUNA rule, CFL courant) to each mark, restricted to the n^2 mesh of the same seed."""
+# The Kelvin-Helmholtz shear layer (2D; sharp, and luna-cpp's smooth one) and the Sedov blast (3D) with its energy in a
+# fixed physical radius of 1/16 (two cells at 32^3), so that every resolution starts from the same flow.
+KH = Problem("kelvin-helmholtz", 2, 0.8, lambda m, n, gamma: problems.kelvin_helmholtz_state(m, gamma))
+KH_SMOOTH = Problem("kelvin-helmholtz-smooth", 2, 0.8, lambda m, n, gamma: problems.kelvin_helmholtz_smooth_state(m, gamma))
+SEDOV = Problem("sedov", 3, 0.3, lambda m, n, gamma: problems.sedov_state(m, gamma, radius_cells=n / 16.0, n=n)[0])
+
+
+def pairs(problem, n=64, seeds=(1,), marks=(0.5, 1.0), factor=4, gamma=5.0 / 3.0, log=None):
+    """Training rows of a problem: for each seed, a plain static run at factor x n per axis to each mark (and its
+    initial state), restricted to the n-mesh of the same seed."""
     xs, ts, ws = [], [], []
     for seed in seeds:
-        coarse = kh_mesh(n, seed)
-        fine = kh_mesh(factor * n, seed + 1000)
+        coarse = problem.mesh(n, seed)
+        fine = problem.mesh(factor * n, seed + 1000)
         owner = owners(coarse, fine)
-        fv = hydro.FiniteVolume(fine, gamma, step_rule="luna")
+        q0 = problem.state(fine, factor * n, gamma)
+        fv = hydro.FiniteVolume(fine, gamma, step_rule="luna" if problem.dim == 2 else "arepo")
         try:
-            fv.set_state(problems.kelvin_helmholtz_state(fine, gamma))
-            states, _ = run_to(fv, list(marks), courant, log)
+            fv.set_state(q0)
+            states, _ = run_to(fv, list(marks), problem.courant, log)
         finally:
             fv.close()
-        for q in [problems.kelvin_helmholtz_state(fine, gamma)] + states:
-            W_f = primitives(q, fine["volume"], gamma, 2)
+        for q in [q0] + states:
+            W_f = primitives(q, fine["volume"], gamma, problem.dim)
             qc = restrict(q, fine, coarse, owner)
             x, t, w = pair_rows(coarse, qc, sub_cell_slopes(W_f, fine, coarse, owner), gamma)
             xs.append(x)
             ts.append(t)
             ws.append(w)
         if log:
-            log("  seed %d: %d rows so far" % (seed, sum(len(a) for a in ts)))
+            log("  %s seed %d: %d rows so far" % (problem.name, seed, sum(len(a) for a in ts)))
     return np.concatenate(xs), np.concatenate(ts), np.concatenate(ws)
 
 
@@ -362,39 +380,41 @@ def density_l1(q, m, q_ref):
     return float(np.sum(np.abs(q[:, 0] - q_ref[:, 0])) / np.sum(vol))
 
 
-def kh_curve(ns, seed, t_end, params, ref_n, courant=0.8, gamma=5.0 / 3.0, motion="static", log=None):
-    """Error against cost: plain and learned runs of the KH shear layer at each n in ns (meshes of `seed`) against one
-    plain reference run at ref_n (static), restricted to each run's final mesh. Returns rows
-    {"n", "limiter", "l1", "seconds", "steps"} and the reference's cost."""
-    fine = kh_mesh(ref_n, seed + 1000)
-    fq, fm, fine_s, fine_steps = run_timed(fine, problems.kelvin_helmholtz_state(fine, gamma), t_end, gamma, courant)
+def curve(problem, ns, seed, t_end, params, ref_n, gamma=5.0 / 3.0, motion="static", log=None):
+    """Error against cost: plain and learned runs at each n in ns (meshes of `seed`) against one plain static reference
+    run at ref_n (another seed's mesh), restricted to each run's final mesh. Returns rows {"n", "limiter", "l1",
+    "seconds", "steps"} and the reference's cost."""
+    fine = problem.mesh(ref_n, seed + 1000)
+    fq, fm, fine_s, fine_steps = run_timed(fine, problem.state(fine, ref_n, gamma), t_end, gamma, problem.courant)
     if log:
-        log("  reference n=%d: %.2f GPU-s, %d steps" % (ref_n, fine_s, fine_steps))
+        log("  %s reference n=%d t=%g: %.2f GPU-s, %d steps" % (problem.name, ref_n, t_end, fine_s, fine_steps))
     rows = []
     for n in ns:
-        coarse = kh_mesh(n, seed)
-        q0 = problems.kelvin_helmholtz_state(coarse, gamma)
+        coarse = problem.mesh(n, seed)
+        q0 = problem.state(coarse, n, gamma)
         for name, limiter, p in (("plain", "face", None), ("learned", "learned", params)):
-            q, cm, secs, steps = run_timed(coarse, q0, t_end, gamma, courant, limiter, p, motion)
+            q, cm, secs, steps = run_timed(coarse, q0, t_end, gamma, problem.courant, limiter, p, motion)
             l1 = density_l1(q, cm, restrict(fq, fm, cm, owners(cm, fm)))
             rows.append({"n": n, "limiter": name, "l1": l1, "seconds": secs, "steps": steps})
             if log:
-                log("  %s n=%d: L1 %.5g in %.3f GPU-s (%d steps)" % (name, n, l1, secs, steps))
+                log("  %s %s n=%d: L1 %.5g in %.3f GPU-s (%d steps)" % (problem.name, name, n, l1, secs, steps))
     return rows, {"n": ref_n, "seconds": fine_s, "steps": fine_steps}
 
 
-def kh_compare(n, seed, t_end, params, factor=4, courant=0.8, gamma=5.0 / 3.0, motion="static", log=None):
-    """Plain and learned runs of the KH shear layer on the n^2 mesh of `seed` against the plain run at factor x n
-    restricted to the final coarse mesh: density L1 errors and GPU-seconds."""
-    coarse = kh_mesh(n, seed)
-    fine = kh_mesh(factor * n, seed + 1000)
-    fq, fm, fine_s, fine_steps = run_timed(fine, problems.kelvin_helmholtz_state(fine, gamma), t_end, gamma, courant)
-    out = {"n": n, "seed": seed, "t_end": t_end, "motion": motion, "reference": {"n": factor * n, "seconds": fine_s, "steps": fine_steps}}
-    q0 = problems.kelvin_helmholtz_state(coarse, gamma)
-    for name, limiter, p in (("plain", "face", None), ("learned", "learned", params)):
-        q, cm, secs, steps = run_timed(coarse, q0, t_end, gamma, courant, limiter, p, motion, log)
-        ref = restrict(fq, fm, cm, owners(cm, fm))
-        out[name] = {"l1": density_l1(q, cm, ref), "seconds": secs, "steps": steps}
-        if log:
-            log("  %s n=%d: L1 %.4g in %.2f GPU-s (%d steps)" % (name, n, out[name]["l1"], secs, steps))
+def speedup_at_equal_error(rows):
+    """For each learned run, the plain method's GPU-seconds to reach the same L1 (log-log interpolation along the plain
+    runs, which must bracket it) divided by the learned run's: > 1 means the learned limiter gets there sooner. None
+    where the plain runs do not bracket the learned error."""
+    plain = sorted((r["seconds"], r["l1"]) for r in rows if r["limiter"] == "plain")
+    out = {}
+    for r in rows:
+        if r["limiter"] != "learned":
+            continue
+        val = None
+        for (s0, e0), (s1, e1) in zip(plain, plain[1:]):
+            if min(e0, e1) <= r["l1"] <= max(e0, e1) and e0 != e1:
+                a = (np.log(r["l1"]) - np.log(e0)) / (np.log(e1) - np.log(e0))
+                val = float(np.exp(np.log(s0) + a * (np.log(s1) - np.log(s0))) / r["seconds"])
+                break
+        out[r["n"]] = val
     return out

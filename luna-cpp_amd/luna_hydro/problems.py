@@ -4,7 +4,7 @@
 shock_into_wave :46-51), Woodward-Colella (:242, :55-59), the linear sound wave (:271, eigenmode :69-74), as exact
 cell averages of the conserved state per unit length. 2D/3D setups are the benchmark configurations of this project:
 jittered lattices (the convention of src/hydro/problems_3d.hpp:36-54, uniform jitter in [-j, j] spacings per axis),
-the Sedov blast, the Kelvin-Helmholtz shear layer and the smooth acoustic wave.
+the Sedov blast, the Kelvin-Helmholtz shear layer (sharp, and luna-cpp's smooth one) and the smooth acoustic wave.
 """
 import math
 from dataclasses import dataclass, field
@@ -229,6 +229,29 @@ def kelvin_helmholtz_state(mesh, gamma=5.0 / 3.0):
     W[:, 1] = np.where(inner, 0.5, -0.5)
     s = 0.05
     W[:, 2] = 0.01 * np.sin(4 * np.pi * X) * (np.exp(-(Y - 0.25) ** 2 / (2 * s * s)) + np.exp(-(Y - 0.75) ** 2 / (2 * s * s)))
+    W[:, 3] = 2.5
+    return conserved_rows(W, gamma) * np.asarray(mesh["volume"])[:, None]
+
+
+def _mcnally(y, outer, inner, layer=0.025):
+    """McNally et al. (2012) eq. 1 for y in [0, 1), mirrored about 1/2: `outer` for |y - 1/2| > 1/4, `inner` inside,
+    joined by exponential ramps of width `layer` (hydro_kh2d.cpp:57-60)."""
+    h = np.where(y < 0.5, y, 1.0 - y)
+    half = 0.5 * (outer - inner)
+    return np.where(h < 0.25, outer - half * np.exp((h - 0.25) / layer), inner + half * np.exp((0.25 - h) / layer))
+
+
+def kelvin_helmholtz_smooth_state(mesh, gamma=5.0 / 3.0):
+    """The smooth shear layer of luna-cpp's KH benchmark (hydro_kh2d.cpp:27, 170-184): rho from 1 (outside) to 2 (inside)
+    and vx from 0.5 to -0.5 along McNally's ramps, vy = 0.01 sin(4 pi x), p = 2.5, at the cell centroids. Unlike the
+    sharp layer it converges under refinement, so errors against a finer run measure the method."""
+    N = mesh["cells"]
+    x = np.asarray(mesh["positions"]).reshape(N, 2) + np.asarray(mesh["centroid"]).reshape(N, 2)
+    X, Y = x[:, 0], x[:, 1] - np.floor(x[:, 1])
+    W = np.zeros((N, 4))
+    W[:, 0] = _mcnally(Y, 1.0, 2.0)
+    W[:, 1] = _mcnally(Y, 0.5, -0.5)
+    W[:, 2] = 0.01 * np.sin(4 * np.pi * X)
     W[:, 3] = 2.5
     return conserved_rows(W, gamma) * np.asarray(mesh["volume"])[:, None]
 
