@@ -2,7 +2,7 @@
 moving (the headline: the hydro part of the two-pass moving step, the mesh producer excluded and reported apart) and
 static, FP64, with the dominant kernel's HBM roofline fraction from HIP events and the CPU restatement timed beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--static-steps S] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--static-steps S] [--no-cpu] [--no-learned]
 
 One JSON line on stdout (the last line); progress on stderr.
 """
@@ -157,6 +157,22 @@ def cpu_baseline(m, q, steps=2):
                       "restatement of luna-cpp's algorithm, OMP threads = cores" % steps}
 
 
+def learned_vs_plain(log):
+    """The learned limiter (the shipped network) against the plain face-centroid limiter: density L1 against a plain
+    run at 4x the resolution, and GPU-seconds, on the smooth KH layer at 64^2 to t = 1 (static mesh, LUNA rule CFL 0.8)."""
+    from luna_hydro import learned
+
+    rows, ref = learned.curve(learned.KH_SMOOTH, [64], 99, 1.0, learned.default_params(), 256, log=log)
+    out = {"workload": "smooth Kelvin-Helmholtz layer, 64^2 jittered static mesh, t = 1, reference: plain at 256^2",
+           "reference_gpu_s": ref["seconds"]}
+    for r in rows:
+        out[r["limiter"]] = {"l1": r["l1"], "gpu_s": r["seconds"], "steps": r["steps"],
+                             "l1_per_gpu_s": r["l1"] / r["seconds"]}
+    out["l1_ratio"] = out["learned"]["l1"] / out["plain"]["l1"]
+    out["gpu_s_ratio"] = out["learned"]["gpu_s"] / out["plain"]["gpu_s"]
+    return out
+
+
 def calibrate_copy(mib):
     """A plain device-to-device copy of `mib` MiB (the runtime's blit kernel), three times: under rocprofv3 --pmc its
     FETCH_SIZE / WRITE_SIZE per dispatch against the known bytes give the counters' scale."""
@@ -186,6 +202,7 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--static-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-limiter comparison")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (scale of the rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -230,6 +247,11 @@ def main():
         cpu = cpu_baseline(m, q)
         log("cpu: %.3g cell updates/s" % cpu["value"])
 
+    learned_cmp = None
+    if not args.no_learned:
+        log("learned limiter vs plain (smooth KH 64^2, reference 256^2)")
+        learned_cmp = learned_vs_plain(log)
+
     step_bytes = sum(ALGO_BYTES["moving"].values())
     out = {
         "metric": "cell updates/s at 128^3 moving mesh (hydro part, mesh producer excluded)",
@@ -251,6 +273,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("moving " + name, n),
                      "algorithmic_bytes_per_cell": per_cell, "avg_ms": avg_ms, "launches": launches},
         "cpu_baseline": cpu,
+        "learned_vs_plain": learned_cmp,
         "step_roofline": {"algorithmic_bytes_per_cell": step_bytes,
                           "achieved_GBs": step_bytes * N * args.steps / hydro_total / 1e9,
                           "frac": step_bytes * N * args.steps / hydro_total / 1e9 / HBM_PEAK_GBS},

@@ -15,7 +15,7 @@
 #define LUNA_SWEEP_UNROLL 2
 #endif
 #ifndef LUNA_LIMIT_UNROLL
-#define LUNA_LIMIT_UNROLL 2
+#define LUNA_LIMIT_UNROLL 4
 #endif
 #ifndef LUNA_UPDATE_UNROLL
 #define LUNA_UPDATE_UNROLL 4

@@ -181,3 +181,13 @@ def test_forward_fraction_in_unit_interval():
     x = mlp.bf16_bits(rng.normal(0, 3, (500, 16)))
     f = mlp.forward(p, x)["fraction"]
     assert np.all((f > 0) & (f < 1))
+
+
+def test_shipped_network_file():
+    """The shipped parameters load without executing anything from the file, in the documented layout."""
+    from luna_hydro import learned
+
+    p = learned.default_params()
+    assert p.dtype == np.float32 and p.shape == (mlp.PARAMS,) and np.all(np.isfinite(p))
+    f = mlp.forward(p, mlp.bf16_bits(np.zeros((4, 16))))["fraction"]
+    assert np.all((f > 0) & (f < 1))
