@@ -31,3 +31,57 @@ def test_sedov_driver_checksum_matches_oracle(tmp_path):
     assert lines[-1].startswith(want), (lines[-1], want)
     snap = snapshot.read(str(tmp_path / "snapdir_000" / "snapshot_000.0.hdf5"))
     assert np.array_equal(snap["U"], (q / m["volume"][:, None]).T)
+
+
+@pytest.mark.parametrize("ic", ["smooth", "sharp"])
+def test_kh_driver_hash_and_modes_match_oracle(ic):
+    """The KH driver (hydro_kh2d's loop: outputs every 0.02, dt cut to land on them) on 24^2 to t = 0.06: its run line's
+    hash is the FNV-1a of the oracle's final state after the same steps, and its mode series is the oracle's."""
+    from luna_hydro import kh2d
+
+    lines = []
+    res = kh2d.run(n=24, ic=ic, t_end=0.06, log=lines.append)
+    print("\n".join(lines), flush=True)
+    assert [l.split(":")[0] for l in lines] == ["totals", "run", "ours"]
+    m = mesh.build(problems.jittered_lattice(24, 2, 0.25, 1), 1.0)
+    state = problems.kelvin_helmholtz_smooth_state if ic == "smooth" else problems.kelvin_helmholtz_state
+    orc = oracle.FiniteVolume(m, 5.0 / 3.0, step_rule="luna")
+    orc.set_state(state(m, 5.0 / 3.0))
+    t, series = 0.0, []
+    for k in range(1, 4):
+        target = k * kh2d.OUTPUT_EVERY
+        while t < target * (1 - 1e-13):
+            dt = min(orc.time_step(0.8), target - t)
+            orc.step(dt)
+            t = target if dt == target - t else t + dt
+        series.append(kh2d.mode_amplitude(orc.q, m["positions"], m["centroid"], m["volume"]))
+    assert "hash=%016x" % sedov.bits_hash(orc.q, kh2d.FNV_BASIS) in lines[1]
+    assert [s[1:] for s in res["series"][1:]] == series
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_1d_driver_hash_matches_oracle(fused):
+    """The 1D driver (hydro_1d's loop, fused LUNA-rule steps or time_step + step) on Sod with 256 cells: its hash is the
+    FNV-1a of the oracle's final state after the same steps, and repeats give the same bits."""
+    from luna_hydro import hydro1d
+
+    lines = []
+    res = hydro1d.run("sod", 256, fused=fused, repeats=2, log=lines.append)
+    print("\n".join(lines), flush=True)
+    assert [l.split(":")[0] for l in lines] == ["run", "run", "ours"] and res["same_bits"]
+    pb = problems.problem_1d("sod")
+    keys, step, period = problems.line_keys(problems.line_points(256, pb.L), pb.L)
+    orc = oracle.Line(keys, period, step, pb.gamma, periodic=pb.periodic, walls=pb.walls)
+    orc.set_state(problems.line_initial_state(pb, keys, step, period))
+    t, steps = 0.0, 0
+    nxt = orc.time_step(0.8) if fused else 0.0
+    while t < pb.t_end * (1 - 1e-12):
+        dt = min(nxt if fused else orc.time_step(0.8), pb.t_end - t)
+        if fused:
+            nxt = orc.step_and_time_step(dt, 0.8)
+        else:
+            orc.step(dt)
+        t += dt
+        steps += 1
+    assert steps == res["steps"]
+    assert res["hash"] == sedov.bits_hash(orc.state, hydro1d.FNV_BASIS)
