@@ -1,6 +1,6 @@
 // The learned limiter's network on CDNA4 matrix cores: inference fused with the gradient scaling, and the training
 // kernels (forward, backward, deterministic reduction, Adam), BF16 operands with FP32 accumulation
-// (v_mfma_f32_16x16x16_bf16).
+// (v_mfma_f32_16x16x16_bf16, and gfx950's double-rate v_mfma_f32_16x16x32_bf16 for the 64 x 64 layer).
 //
 // Every product is computed transposed, rows in the MFMA's N position: a tile is 16 feature rows, and lane l (group
 // g = l / 16, row n = l % 16) holds row n's hidden units 16 b + 4 g + j (j < 4) of each 16-unit block b. That is the
@@ -15,6 +15,8 @@ namespace luna {
 namespace {
 
 typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef short short8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 // D = A B + C for 16 x 16 tiles: lane l gives A[l % 16][4 (l / 16) + j] and B[4 (l / 16) + j][l % 16] and gets
@@ -23,7 +25,20 @@ __device__ __forceinline__ float4_t mfma16(short4_t a, short4_t b, float4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ short bf16s(float f) { return (short)bf16_bits(f); }
+// D = A B + C over a contraction of 32: lane l gives 8 values of A's row l % 16 and 8 of B's column l % 16, at the same 8
+// contraction indices (for A and B alike) chosen by l / 16 and the element. A sum over all of them does not depend on
+// which indices those are, so the contraction slots can carry any fixed assignment of hidden units (see Weights::w2).
+__device__ __forceinline__ float4_t mfma32(short8_t a, short8_t b, float4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+}
+
+// FP32 to BF16 bits, nearest even (one v_cvt_pk_bf16_f32 on gfx950; the values here are finite).
+__device__ __forceinline__ short bf16s(float f) { return __builtin_bit_cast(short, (__bf16)f); }
+
+__device__ __forceinline__ short8_t join(short4_t a, short4_t b) {  // as 32-bit words: no byte shuffles
+    const uint2 x = __builtin_bit_cast(uint2, a), y = __builtin_bit_cast(uint2, b);
+    return __builtin_bit_cast(short8_t, make_uint4(x.x, x.y, y.x, y.y));
+}
 
 __device__ __forceinline__ short4_t load4(const unsigned short* p) {  // 4 BF16, 8-byte aligned
     const uint2 u = *reinterpret_cast<const uint2*>(p);
@@ -44,10 +59,12 @@ __device__ __forceinline__ float4_t loadf4(const float* p) {  // 16-byte aligned
     return r;
 }
 
-// This lane's operands of the forward pass.
+// This lane's operands of the forward pass. Layer 2 contracts over hidden-1 in two chunks of 32: the lane's 8 slots of
+// chunk c carry units 32 c + 4 g + j (its layer-1 block 2 c) and 32 c + 16 + 4 g + j (block 2 c + 1), j < 4, which is
+// what the lane holds after layer 1, in A and B alike.
 struct Weights {
     short4_t w1[4];     // A of layer 1, block b: W1[16 b + n][4 g + j]
-    short4_t w2[4][4];  // A of layer 2, [out block o][in block c]: W2[16 o + n][16 c + 4 g + j]
+    short8_t w2[4][2];  // A of layer 2, [out block o][chunk c]: W2[16 o + n][32 c + 4 g + j], W2[16 o + n][32 c + 16 + 4 g + j]
     float4_t b1[4], b2[4], w3[4];  // b1[16 b + 4 g + j], ...
     float b3;
 };
@@ -56,7 +73,8 @@ __device__ __forceinline__ void load_weights(const Network& net, int g, int n, W
     for (int b = 0; b < 4; ++b) {
         W.w1[b] = load4(net.w1 + (16 * b + n) * FEATURES + 4 * g);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) W.w2[b][c] = load4(net.w2 + (16 * b + n) * HIDDEN + 16 * c + 4 * g);
+        for (int c = 0; c < 2; ++c)
+            W.w2[b][c] = join(load4(net.w2 + (16 * b + n) * HIDDEN + 32 * c + 4 * g), load4(net.w2 + (16 * b + n) * HIDDEN + 32 * c + 16 + 4 * g));
         W.b1[b] = loadf4(net.b1 + 16 * b + 4 * g);
         W.b2[b] = loadf4(net.b2 + 16 * b + 4 * g);
         W.w3[b] = loadf4(net.w3 + 16 * b + 4 * g);
@@ -78,11 +96,12 @@ __device__ __forceinline__ void forward(const Weights& W, short4_t x, Tile& t) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) t.h1[b][j] = bf16s(fmaxf(t.pre1[b][j], 0.0f));
     }
+    const short8_t h1c[2] = {join(t.h1[0], t.h1[1]), join(t.h1[2], t.h1[3])};
 #pragma unroll
     for (int o = 0; o < 4; ++o) {
         float4_t acc = W.b2[o];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc = mfma16(W.w2[o][c], t.h1[c], acc);
+        for (int c = 0; c < 2; ++c) acc = mfma32(W.w2[o][c], h1c[c], acc);
         t.pre2[o] = acc;
     }
     float part = 0.0f;
@@ -97,31 +116,31 @@ __device__ __forceinline__ void forward(const Weights& W, short4_t x, Tile& t) {
 
 __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + expf(-z)); }
 
-constexpr int APPLY_BLOCK = 256, APPLY_TILES = 4;  // tiles per wavefront
+// A persistent grid: each wavefront loads the weights once and takes tiles wave, wave + waves, ...
+constexpr int APPLY_BLOCK = 256, APPLY_WAVES = 256 * 16;  // 256 CUs, 4 wavefronts per SIMD (126 registers)
 
 __global__ __launch_bounds__(APPLY_BLOCK) void k_limiter_apply(Network net, const unsigned short* x, const unsigned char* troubled, int rows,
                                                                int V, int D, int grad_row, double* grad, float* fraction) {
     const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
-    const int wave = blockIdx.x * (APPLY_BLOCK / 64) + (threadIdx.x >> 6);
+    const int wave = blockIdx.x * (APPLY_BLOCK / 64) + (threadIdx.x >> 6), waves = gridDim.x * (APPLY_BLOCK / 64);
     const int tiles = (rows + 15) >> 4;
-    const int first = wave * APPLY_TILES, last = min(tiles, first + APPLY_TILES);
-    if (first >= tiles) return;  // the whole wavefront
+    if (wave >= tiles) return;  // the whole wavefront
     Weights W;
     load_weights(net, g, n, W);
-    for (int t = first; t < last; ++t) {  // uniform over the wavefront: the MFMAs run with every lane
+    for (int t = wave; t < tiles; t += waves) {  // uniform over the wavefront: the MFMAs run with every lane
         const int row = t * 16 + n;
         short4_t xv = {0, 0, 0, 0};
         if (row < rows) xv = load4(x + (size_t)row * FEATURES + 4 * g);
         Tile tl;
         forward(W, xv, tl);
-        if (g == 0 && row < rows) {
-            const float frac = sigmoid(tl.z);
-            if (fraction) fraction[row] = frac;
-            if (grad) {
+        if (row < rows) {
+            const float frac = sigmoid(tl.z);  // every group holds z: group g scales component g of the row's gradient
+            if (fraction && g == 0) fraction[row] = frac;
+            if (grad && g < D) {
                 const int i = row / V, v = row - i * V;
                 if (!troubled[i]) {
-                    double* p = grad + (size_t)i * grad_row + v * D;
-                    for (int a = 0; a < D; ++a) p[a] = p[a] * (double)frac;
+                    double* p = grad + (size_t)i * grad_row + v * D + g;
+                    *p = *p * (double)frac;
                 }
             }
         }
@@ -132,18 +151,14 @@ __global__ __launch_bounds__(APPLY_BLOCK) void k_limiter_apply(Network net, cons
 // the loss, into its row of partials.
 __global__ __launch_bounds__(64) void k_mlp_partials(Network net, Batch b, float* partials) {
     __shared__ short h1s[16][HIDDEN], dh2s[16][HIDDEN], dh1s[16][HIDDEN], xs[16][FEATURES];
+    __shared__ short w2s[HIDDEN][HIDDEN];  // W2, read transposed as the A operand of dh1
     const int lane = threadIdx.x, g = lane >> 4, n = lane & 15;
     const int wave = blockIdx.x, waves = gridDim.x;
     const int tiles = (b.rows + 15) >> 4;
     Weights W;
     load_weights(net, g, n, W);
-    short4_t w2t[4][4];  // A of the layer-2 transpose, [hidden-1 block k][hidden-2 block m]: W2[16 m + 4 g + j][16 k + n]
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) w2t[k][m][j] = (short)net.w2[(16 * m + 4 * g + j) * HIDDEN + 16 * k + n];
+    for (int k = lane; k < HIDDEN * HIDDEN; k += 64) w2s[k / HIDDEN][k % HIDDEN] = (short)net.w2[k];
+    __syncthreads();
     const float4_t zero = {0.0f, 0.0f, 0.0f, 0.0f};
     float4_t gW1[4], gW2[4][4], gb1[4], gb2[4], gw3[4];
 #pragma unroll
@@ -187,7 +202,12 @@ __global__ __launch_bounds__(64) void k_mlp_partials(Network net, Batch b, float
         for (int k = 0; k < 4; ++k) {
             float4_t acc = zero;
 #pragma unroll
-            for (int m = 0; m < 4; ++m) acc = mfma16(w2t[k][m], dh2b[m], acc);
+            for (int m = 0; m < 4; ++m) {
+                short4_t a;  // W2[16 m + 4 g + j][16 k + n]
+#pragma unroll
+                for (int j = 0; j < 4; ++j) a[j] = w2s[16 * m + 4 * g + j][16 * k + n];
+                acc = mfma16(a, dh2b[m], acc);
+            }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const float d = tl.pre1[k][j] > 0.0f ? acc[j] : 0.0f;
@@ -304,7 +324,8 @@ __global__ __launch_bounds__(256) void k_mlp_reduce(const float* partials, int w
 hipError_t launch_limiter_apply(hipStream_t s, const Network& net, const unsigned short* features, const unsigned char* troubled, int rows,
                                 int V, int D, int grad_row, double* grad, float* fraction) {
     if (rows <= 0) return hipSuccess;
-    const long long tiles = (rows + 15) / 16, waves = (tiles + APPLY_TILES - 1) / APPLY_TILES;
+    if (D > 4) return hipErrorInvalidValue;  // one gradient component per lane group
+    const long long tiles = (rows + 15) / 16, waves = std::min<long long>(tiles, APPLY_WAVES);
     const unsigned blocks = (unsigned)((waves + APPLY_BLOCK / 64 - 1) / (APPLY_BLOCK / 64));
     hipLaunchKernelGGL(k_limiter_apply, dim3(blocks), dim3(APPLY_BLOCK), 0, s, net, features, troubled, rows, V, D, grad_row, grad, fraction);
     return hipGetLastError();
