@@ -26,6 +26,11 @@
 #ifndef LUNA_FLUX_WAVES
 #define LUNA_FLUX_WAVES 1
 #endif
+// 1: the sweep takes the neighbours' range (min, max of W) in a loop of its own after the fit, re-reading the
+// neighbours' W (cache-resident), instead of carrying it through the first walk (20 registers fewer there).
+#ifndef LUNA_RANGE_PASS
+#define LUNA_RANGE_PASS 0
+#endif
 
 namespace luna {
 namespace {
@@ -177,8 +182,10 @@ struct Sums {
                 const double dW = n.W[v] - Wi[v];
 #pragma unroll
                 for (int p = 0; p < U; ++p) b[v][p] += w * X[p] * dW;
+#if !LUNA_RANGE_PASS
                 lo[v] = fmin(lo[v], n.W[v]);
                 hi[v] = fmax(hi[v], n.W[v]);
+#endif
             }
         }
         if constexpr (Rule == 0) {  // FaceStep (:1079-1089) from this side
@@ -249,23 +256,25 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
         smallest = s.smallest;
         const double tiny = 1e-6 * s.surface;
         const bool any_tiny = s.smallest_face < tiny;
-        if constexpr (Grad) {
-            if (any_tiny) {  // neighbour_range (:572-588)
+        // neighbour_range (:572-588): the range of the neighbours' W, faces under 1e-6 of the surface left out if any
+        auto neighbour_range = [&]() {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                s.lo[v] = 1.7976931348623157e308;
+                s.hi[v] = -1.7976931348623157e308;
+            }
+            for (int k = 0; k < count; ++k) {
+                if (any_tiny && g.slot[base + k * SLICE] < tiny) continue;
+                const double* row = g.cell + (size_t)g.other[base + k * SLICE] * CELL_ROW + C::W;
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
-                    s.lo[v] = 1.7976931348623157e308;
-                    s.hi[v] = -1.7976931348623157e308;
-                }
-                for (int k = 0; k < count; ++k) {
-                    if (g.slot[base + k * SLICE] < tiny) continue;
-                    const double* row = g.cell + (size_t)g.other[base + k * SLICE] * CELL_ROW + C::W;
-#pragma unroll
-                    for (int v = 0; v < V; ++v) {
-                        s.lo[v] = fmin(s.lo[v], row[v]);
-                        s.hi[v] = fmax(s.hi[v], row[v]);
-                    }
+                    s.lo[v] = fmin(s.lo[v], row[v]);
+                    s.hi[v] = fmax(s.hi[v], row[v]);
                 }
             }
+        };
+        if constexpr (Grad && !LUNA_RANGE_PASS) {
+            if (any_tiny) neighbour_range();
         }
         double grad[V][D], intercept[V];
         if constexpr (Grad) {  // Sweep::solve (:590-615): Cholesky of M, then M x = b per variable
@@ -343,6 +352,7 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
             smallest = fmin(smallest, courant / rate);
         }
         if constexpr (Grad) {
+            if constexpr (Limit > 0 && LUNA_RANGE_PASS) neighbour_range();
             if constexpr (Limit > 0) {  // the face-centroid limiter, face by face in list order (:538-565)
                 // Geometry::face_offset (:322-326), kept per slot, of UL entries loaded together, then applied in order.
                 constexpr int UL = LUNA_LIMIT_UNROLL;

@@ -1,8 +1,8 @@
 """Initial conditions and point sets of the benchmark problems (host side, numpy).
 
-1D problems restate luna-cpp's src/hydro/problems_1d.hpp (LUNA's setups): Sod (Toro 1, :212), Shu-Osher (:234-237,
-shock_into_wave :46-51), Woodward-Colella (:242, :55-59), the linear sound wave (:271, eigenmode :69-74), as exact
-cell averages of the conserved state per unit length. 2D/3D setups are the benchmark configurations of this project:
+1D problems restate luna-cpp's src/hydro/problems_1d.hpp table (:209-284; LUNA's setups and the ones it lacks): the
+Toro, Einfeldt, LA-14379, LeBlanc, pure-shock tubes, Noh, Shu-Osher, Titarev-Toro, Woodward-Colella, the linear waves,
+the advected sound wave and the simple wave, as exact cell averages of the conserved state per unit length. 2D/3D setups are the benchmark configurations of this project:
 jittered lattices (the convention of src/hydro/problems_3d.hpp:36-54, uniform jitter in [-j, j] spacings per axis),
 the Sedov blast, the Kelvin-Helmholtz shear layer (sharp, and luna-cpp's smooth one) and the smooth acoustic wave.
 """
@@ -24,6 +24,9 @@ class Problem1D:
     walls: Tuple[str, str]
     average: Callable  # (a, b) -> conserved state per unit length, averaged over [a, b]
     periodic: bool = False
+    left: Tuple[float, float, float] = None   # a Riemann problem's primitives either side of its jump at x0
+    right: Tuple[float, float, float] = None
+    x0: float = 0.0
 
 
 def _conserved1(W, gamma):
@@ -42,7 +45,7 @@ def _tube(L, origin, t_end, gamma, left, right, x0, walls):
         f = _fraction_below(x0, a, b)
         return tuple(f * l[k] + (1 - f) * r[k] for k in range(3))
 
-    return Problem1D(L, origin, t_end, gamma, walls, average)
+    return Problem1D(L, origin, t_end, gamma, walls, average, False, tuple(left), tuple(right), x0)
 
 
 def _shock_into_wave(a, b, left, x0, amp, k, gamma):
@@ -72,19 +75,125 @@ def _eigenmode(a, b, eps, v0, drho, dv, dp):
             P0 / (G - 1) + 0.5 * v0 * v0 + eps * (dp / (G - 1) + 0.5 * v0 * v0 * drho + v0 * dv) * s)
 
 
-def problem_1d(name):
+PI = 3.141592653589793
+LINEAR_AMPLITUDE = 1e-6  # LUNA's EPS of its linear-wave tests
+
+
+def _noh(a, b):
+    # problems_1d.hpp:63-67: cold inflow onto the wall at 0, brought to rest by a ramp towards the far wall, at the centre
+    P0, U0, L, A = 1e-6, -1.0, 2.0, 1.0
+    x = 0.5 * (a + b)
+    u = U0 * min(max((L - x) / (L - A), 0.0), 1.0)
+    return (1.0, u, P0 / (5.0 / 3.0 - 1) + 0.5 * u * u)
+
+
+def _advected(a, b, eps):
+    # problems_1d.hpp:79-83: the advected sound wave at the cell's centre
+    s = eps * math.sin(2 * PI * 0.5 * (a + b))
+    return _conserved1((1 + s, 1 + s, 1 + s), 5.0 / 3.0)
+
+
+GAUSS4 = ((-0.8611363115940526, 0.3478548451374538), (-0.3399810435848563, 0.6521451548625461),
+          (0.3399810435848563, 0.6521451548625461), (0.8611363115940526, 0.3478548451374538))
+
+
+def gauss_average(a, b, f):
+    """4-point Gauss-Legendre average over [a, b] of a pointwise conserved state f(x) (problems_1d.hpp:86-97)."""
+    u = [0.0, 0.0, 0.0]
+    for x, w in GAUSS4:
+        at = f(0.5 * (a + b) + 0.5 * (b - a) * x)
+        for v in range(3):
+            u[v] += 0.5 * w * at[v]
+    return tuple(u)
+
+
+def _simple_wave(a, b, amp):
+    # problems_1d.hpp:100-106: the isentropic simple wave, gamma 1.4: c = 1 + amp sin(2 pi x), u = 5 (c - 1), rho = c^5
+    def at(x):
+        g = 1.4
+        c = 1 + amp * math.sin(2 * PI * x)
+        rho = c ** (2 / (g - 1))
+        return _conserved1((rho, 2 / (g - 1) * (c - 1), rho ** g / g), g)
+
+    return gauss_average(a, b, at)
+
+
+def _pure_shock(mach, walls):
+    # problems_1d.hpp:109-115: a shock of Mach `mach` at x = 0.1 running right into (1, 0, 1)
+    g = 1.4
+    m2, s = mach * mach, mach * math.sqrt(g)
+    rho2 = (g + 1) * m2 / ((g - 1) * m2 + 2)
+    p2 = (2 * g * m2 - (g - 1)) / (g + 1)
+    v2 = s * (1 - 1 / rho2)
+    return _tube(1.0, 0.0, 0.5 / s, g, (rho2, v2, p2), (1, 0, 1), 0.1, walls)
+
+
+def problem_1d(name, amplitude=0.0):
+    """luna-cpp's 1D problem table (problems_1d.hpp:209-284): LUNA's setups and the ones it lacks; amplitude: of the
+    linear waves (signed) and of the simple wave (0: their defaults, 1e-6 and 0.1)."""
     outflow, closed = ("outflow", "outflow"), ("reflective", "reflective")
+
+    def toro(left, right, t):
+        return _tube(1.0, 0.0, t, 1.4, left, right, 0.5, outflow)
+
     if name in ("sod", "toro-1"):
-        return _tube(1.0, 0.0, 0.25, 1.4, (1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 0.5, outflow)
+        return toro((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 0.25)
+    if name == "toro-2":
+        return toro((1.0, -2.0, 0.4), (1.0, 2.0, 0.4), 0.15)
+    if name == "toro-3":
+        return toro((1.0, 0.0, 1000.0), (1.0, 0.0, 0.01), 0.012)
+    if name == "toro-4":
+        return toro((1.0, 0.0, 0.01), (1.0, 0.0, 100.0), 0.035)
+    if name == "toro-5":
+        return toro((5.99924, 19.5975, 460.894), (5.99242, -6.19633, 46.0950), 0.035)
+    if name == "einfeldt-strong":
+        return toro((1.0, -2.0, 0.4 * (3.0 - 0.5 * 4.0)), (1.0, 2.0, 0.4 * (3.0 - 0.5 * 4.0)), 0.15)
+    if name == "einfeldt-shear":
+        return toro((1.0, -1.0, 0.4 * (5.0 - 0.5 * 1.0)), (1.0, 1.0, 0.4 * (5.0 - 0.5 * 1.0)), 0.15)
+    if name == "la-peak":
+        return _tube(0.5, 0.1, 3.9e-3, 1.4, (0.1261192, 8.9047029, 782.92899), (6.591493, 2.2654207, 3.1544874), 0.5, outflow)
+    if name == "la-rcvcr":
+        return toro((1.0, -4.0, 0.4), (1.0, 4.0, 0.4), 0.15)
+    if name == "la-stationary":
+        return _tube(1.0, 0.0, 0.012, 1.4, (1.0, -19.59745, 1e3), (1.0, -19.59745, 1e-2), 0.8, outflow)
+    if name == "la-slow":
+        return toro((3.857143, -0.810631, 10.33333), (1.0, -3.44, 1.0), 1.0)
+    if name == "la-streams":
+        return toro((1.0, 2.0, 0.1), (1.0, -2.0, 0.1), 0.8)
+    if name == "la-vacuum":
+        return _tube(6.0, -5.0, 0.75, 1.4, (0.0, 0.0, 0.0), (1.0, 0.0, 1.0), 0.0, outflow)
+    if name == "la-near-vacuum":
+        return _tube(6.0, -5.0, 0.75, 1.4, (1e-6, 0.0, 1e-6), (1.0, 0.0, 1.0), 0.0, outflow)
+    if name == "leblanc":
+        return _tube(1.0, 0.0, 0.5, 5.0 / 3.0, (1.0, 0.0, (2.0 / 3.0) * 1e-1), (1e-3, 0.0, (2.0 / 3.0) * 1e-10), 0.3, outflow)
+    if name == "noh":
+        return Problem1D(2.0, 0.0, 0.6, 5.0 / 3.0, closed, _noh)
     if name == "shu-osher":
         left = (3.857143, 2.629369, 10.33333)
-        return Problem1D(10.0, -5.0, 1.8, 1.4, outflow,
-                         lambda a, b: _shock_into_wave(a, b, left, -4, 0.2, 5, 1.4))
+        return Problem1D(10.0, -5.0, 1.8, 1.4, outflow, lambda a, b: _shock_into_wave(a, b, left, -4, 0.2, 5, 1.4), False, left)
+    if name == "titarev-toro":
+        left = (1.515695, 0.523346, 1.805)
+        return Problem1D(10.0, -5.0, 5.0, 1.4, outflow, lambda a, b: _shock_into_wave(a, b, left, -4.5, 0.1, 20 * PI, 1.4))
     if name == "woodward-colella":
         return Problem1D(1.0, 0.0, 0.038, 1.4, closed, _woodward_colella)
+    eps = amplitude if amplitude != 0 else LINEAR_AMPLITUDE
+
+    def wave(t, v0, drho, dv, dp):
+        return Problem1D(1.0, 0.0, t, 5.0 / 3.0, outflow, lambda a, b: _eigenmode(a, b, eps, v0, drho, dv, dp), periodic=True)
+
     if name == "sound-right":
-        return Problem1D(1.0, 0.0, 1.0, 5.0 / 3.0, outflow,
-                         lambda a, b: _eigenmode(a, b, 1e-6, 0.0, 1.0, 1.0, 1.0), periodic=True)
+        return wave(1.0, 0.0, 1.0, 1.0, 1.0)
+    if name == "sound-left":
+        return wave(1.0, 0.0, 1.0, -1.0, 1.0)
+    if name == "entropy":
+        return wave(2.0, 0.5, 1.0, 0.0, 0.0)
+    if name == "sound-advected":
+        return Problem1D(1.0, 0.0, 1.0, 5.0 / 3.0, outflow, lambda a, b: _advected(a, b, eps), periodic=True)
+    if name == "simple-wave":
+        amp = amplitude if amplitude != 0 else 0.1
+        return Problem1D(1.0, 0.0, 0.2, 1.4, outflow, lambda a, b: _simple_wave(a, b, amp), periodic=True)
+    if name.startswith("shock-"):
+        return _pure_shock(float(name[6:]), outflow)
     raise ValueError("unknown 1D problem " + name)
 
 
@@ -110,25 +219,37 @@ def line_keys(x, L):
     return np.sort(keys), step, period
 
 
-def line_initial_state(problem, keys, step, period):
-    """Q (extensive, n x 3) from the problem's exact cell averages over each cell [c - V/2, c + V/2]."""
+def line_cells(keys, step, period, periodic):
+    """Each slot's cell [a, b] and volume V on the line (without the problem's origin), from the gaps of the sorted keys
+    as the line computes V and the centroid (a wall's gap twice the distance to it): lists a, b, V."""
     n = len(keys)
     gaps = np.empty(n + 1, dtype=np.int64)  # gaps[s] = gap(s - 1)
     inner = np.diff(keys)
-    if problem.periodic:
+    if periodic:
         wrap = keys[0] + period - keys[-1]
         gaps[0], gaps[n] = wrap, wrap
     else:
         gaps[0], gaps[n] = 2 * keys[0], 2 * (period - keys[-1])
     gaps[1:n] = inner
-    q = np.empty((n, 3))
+    a, b, vol = [], [], []
     for s in range(n):
         V = (float(gaps[s]) + float(gaps[s + 1])) * (0.5 * step)
         c = (float(gaps[s + 1]) - float(gaps[s])) * (0.25 * step)
         x = float(keys[s]) * step
-        a = x + c - 0.5 * V
-        b = a + V
-        u = problem.average(a + problem.origin, b + problem.origin)
+        lo = x + c - 0.5 * V
+        a.append(lo)
+        b.append(lo + V)
+        vol.append(V)
+    return a, b, vol
+
+
+def line_initial_state(problem, keys, step, period):
+    """Q (extensive, n x 3) from the problem's exact cell averages over each cell [c - V/2, c + V/2]."""
+    a, b, vol = line_cells(keys, step, period, problem.periodic)
+    q = np.empty((len(keys), 3))
+    for s in range(len(keys)):
+        V = vol[s]
+        u = problem.average(a[s] + problem.origin, b[s] + problem.origin)
         q[s] = [u[k] * V for k in range(3)]
     return q
 

@@ -2,8 +2,9 @@
 
 Written from Toro, "Riemann Solvers and Numerical Methods for Fluid Dynamics" (3rd ed., 2009), chapter 4: the pressure
 functions f_K(p) (eqs. 4.6, 4.7) and their derivatives (4.37), Newton iteration for the star pressure (sec. 4.3.2), the
-star velocity (4.9), and the sampling of the wave pattern (sec. 4.5, figs. 4.14-4.15), with the conservative cell average
-over midpoint sub-samples that luna-cpp's tests use (exact_riemann.hpp:87-101). No vacuum generation (not needed here).
+star velocity (4.9), the sampling of the wave pattern (sec. 4.5, figs. 4.14-4.15) and, for data that generate vacuum
+or hold it on one side, the two rarefaction fans around empty space (sec. 4.6), with the conservative cell average over
+midpoint sub-samples that luna-cpp's tests use (exact_riemann.hpp:87-101).
 """
 import math
 
@@ -14,9 +15,12 @@ class Solver:
         g = gamma
         rl, ul, pl = self.L
         rr, ur, pr = self.R
-        self.cl, self.cr = math.sqrt(g * pl / rl), math.sqrt(g * pr / rr)
-        if 2 * (self.cl + self.cr) / (g - 1) <= ur - ul:
-            raise ValueError("the data generate vacuum")
+        self.cl = math.sqrt(g * pl / rl) if rl > 0 else 0.0
+        self.cr = math.sqrt(g * pr / rr) if rr > 0 else 0.0
+        self.vacuum = 2 * (self.cl + self.cr) / (g - 1) <= ur - ul or rl == 0 or rr == 0
+        if self.vacuum:
+            self.p, self.u = 0.0, 0.0
+            return
         p = max(1e-12, 0.5 * (pl + pr))
         for _ in range(100):
             fl, dl = self._f(p, self.L, self.cl)
@@ -43,7 +47,28 @@ class Solver:
         return (2 * c / (g - 1) * ((p / pk) ** ((g - 1) / (2 * g)) - 1),
                 1 / (rk * c) * (p / pk) ** (-(g + 1) / (2 * g)))
 
+    def _vacuum_sample(self, s):
+        """Toro sec. 4.6: the left state, its fan to vacuum (head u_L - c_L, tail u_L + 2 c_L / (g - 1)), empty space,
+        the right fan (tail u_R - 2 c_R / (g - 1), head u_R + c_R), the right state; a side with no gas has no fan."""
+        g = self.g
+        rl, ul, pl = self.L
+        rr, ur, pr = self.R
+        cl, cr = self.cl, self.cr
+        if rl > 0 and s < ul - cl:
+            return self.L
+        if rr > 0 and s >= ur + cr:
+            return self.R
+        if rl > 0 and s < ul + 2 * cl / (g - 1):
+            c = 2 / (g + 1) * (cl + (g - 1) / 2 * (ul - s))
+            return (rl * (c / cl) ** (2 / (g - 1)), 2 / (g + 1) * (cl + (g - 1) / 2 * ul + s), pl * (c / cl) ** (2 * g / (g - 1)))
+        if rr > 0 and s > ur - 2 * cr / (g - 1):
+            c = 2 / (g + 1) * (cr - (g - 1) / 2 * (ur - s))
+            return (rr * (c / cr) ** (2 / (g - 1)), 2 / (g + 1) * (-cr + (g - 1) / 2 * ur + s), pr * (c / cr) ** (2 * g / (g - 1)))
+        return (0.0, 0.0, 0.0)
+
     def sample(self, s):
+        if self.vacuum:
+            return self._vacuum_sample(s)
         g, p, u = self.g, self.p, self.u
         if s <= u:
             rk, uk, pk = self.L
