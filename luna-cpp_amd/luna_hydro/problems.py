@@ -197,6 +197,22 @@ def problem_1d(name, amplitude=0.0):
     raise ValueError("unknown 1D problem " + name)
 
 
+def moving_walls(pb):
+    """The walls of a moving run (problems_1d.hpp:285-300): the static walls, except that an outflow end where the
+    problem's state enters supersonically becomes an inflow wall holding that state. Returns (walls, (left, right)
+    inflow primitives)."""
+    walls, inflow = list(pb.walls), [(0.0, 0.0, 0.0), (0.0, 0.0, 0.0)]
+
+    def entering(W, side):
+        return W is not None and W[0] > 0 and side * W[1] > math.sqrt(pb.gamma * W[2] / W[0])
+
+    if pb.walls[0] == "outflow" and entering(pb.left, 1):
+        walls[0], inflow[0] = "inflow", tuple(pb.left)
+    if pb.walls[1] == "outflow" and entering(pb.right, -1):
+        walls[1], inflow[1] = "inflow", tuple(pb.right)
+    return tuple(walls), tuple(inflow)
+
+
 def line_points(n, L, jitter=0.0, seed=1):
     """n generators on [0, L]: L (i + 1/2 + jitter u_i) / n, u_i uniform in [-1/2, 1/2) from mt19937_64(seed)
     (hydro_1d.cpp:136-139); the jitter draws use numpy's MT19937 and are not the reference's bits."""

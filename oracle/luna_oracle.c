@@ -744,6 +744,8 @@ static double line_length(const orc_line* l, int s) { return (double)line_gap(l,
 /* finite_volume_1d.hpp:173, 176 (load_inner): V and the centroid relative to the generator from the two gaps. */
 double orc_line_volume(const orc_line* l, int s) { return ((double)line_gap(l, s - 1) + (double)line_gap(l, s)) * (0.5 * l->step); }
 double orc_line_centroid(const orc_line* l, int s) { return ((double)line_gap(l, s) - (double)line_gap(l, s - 1)) * (0.25 * l->step); }
+/* line.order(s): the cell in slot s. */
+static int line_id(const orc_line* l, int s) { return l->id ? l->id[s] : s; }
 static int line_slot(const orc_line* l, int s, int k) {
     const int t = s + k;
     if (l->periodic) return ((t % l->n) + l->n) % l->n;
@@ -753,14 +755,16 @@ static int line_slot(const orc_line* l, int s, int k) {
 /* finite_volume_1d.hpp:107-111 primitive_of() */
 static void line_primitive(const orc_line* l, int s, double* W) {
     const double inv = 1.0 / orc_line_volume(l, s);
-    const double U[3] = {l->q[s * 3] * inv, l->q[s * 3 + 1] * inv, l->q[s * 3 + 2] * inv};
+    const int i = line_id(l, s);
+    const double U[3] = {l->q[i * 3] * inv, l->q[i * 3 + 1] * inv, l->q[i * 3 + 2] * inv};
     orc_primitive(1, &l->gas, U, W);
 }
 
-/* finite_volume_1d.hpp:117-128 ghost(), ghost_face() */
+/* finite_volume_1d.hpp:117-128 ghost(), ghost_face(): reflective mirrors the velocity, inflow is the wall's state,
+ * outflow copies. */
 static void line_ghost(const orc_line* l, const double* W, int left, double* out) {
     const int wall = left ? l->wall_left : l->wall_right;
-    for (int v = 0; v < 3; ++v) out[v] = W[v];
+    for (int v = 0; v < 3; ++v) out[v] = wall == 2 ? l->inflow[left ? 0 : 1][v] : W[v];
     if (wall == 0) out[1] = -out[1];
 }
 static void line_ghost_face(const orc_line* l, const double* Wf, const double* W, int left, double* out) {
@@ -870,9 +874,10 @@ static double line_body(const orc_line* l, int s, double dt, double courant, int
         riemann(l->riemann, 1, &l->gas, L, R, &one, &zero, F[f]);
     }
     double Unew[3];
+    const int i = line_id(l, s);
     for (int v = 0; v < 3; ++v) {
-        Unew[v] = l->q[s * 3 + v] - dt * (F[1][v] - F[0][v]);
-        l->next[s * 3 + v] = Unew[v];
+        Unew[v] = l->q[i * 3 + v] - dt * (F[1][v] - F[0][v]);
+        l->next[i * 3 + v] = Unew[v];
     }
     if (!fused) return INFINITY;
     const double volume = cell[2].V;
@@ -884,7 +889,157 @@ static double line_body(const orc_line* l, int s, double dt, double courant, int
     return courant * volume / (relative + orc_sound_speed(1, &l->gas, Wnew));
 }
 
-/* finite_volume_1d.hpp:332-386 time_step() on a static line: LUNA's rule or AREPO's (with the wall ghosts' pair terms). */
+/* finite_volume_1d.hpp:298-314 drift(): the point's velocity relative to its cell's from AREPO-ng's regularisation, for
+ * primitives W, density gradient g, centroid c (relative to the generator) and volume. */
+#define LINE_SHAPING_SPEED 0.5 /* cell_shaping_speed, finite_volume_1d.hpp:26 */
+static double line_drift(const orc_line* l, const double* W, double g, double c, double volume) {
+    const double r = 0.5 * volume;
+    if (!l->regularise || fabs(c) <= 0.5 * r) return 0.0;
+    double offset = c;
+    const double dgrad = fabs(g);
+    if (dgrad > 0) {
+        const double tmp = 3 * r + W[0] / dgrad;
+        const double x = (tmp - sqrt(tmp * tmp - 8 * r * r)) / 4;
+        if (x < 0.25 * r) offset += x / dgrad * g;
+    }
+    const double d = fabs(offset);
+    if (d <= 0.75 * r) return 0.0;
+    const double fraction = d > r ? LINE_SHAPING_SPEED : LINE_SHAPING_SPEED * (d - 0.75 * r) / (0.25 * r);
+    return offset * (fraction * orc_sound_speed(1, &l->gas, W) / d);
+}
+
+/* finite_volume_1d.hpp:316-328 point_velocity(): drift(), plus, for a generator beyond 0.75 r of its centroid, up to the
+ * fluid's velocity at the generator, linear between this cell's centroid and that of the neighbour on that side. */
+static double line_point_velocity(const orc_line* l, const line_cell* cell, double gapL, double gapR, double g_rho, double volume) {
+    const line_cell* me = cell;
+    const double r = 0.5 * volume, d = fabs(me->c), weight = fmin(1.0, (d - 0.75 * r) / (0.25 * r));
+    const double drifting = line_drift(l, me->W, g_rho, me->c, volume);
+    if (!(weight > 0)) return drifting;
+    const int side = me->c < 0 ? 1 : -1;
+    const double between = side > 0 ? gapR + cell[1].c - me->c : gapL + me->c - cell[-1].c;
+    return drifting + weight * (cell[side].W[1] - me->W[1]) * (d / between);
+}
+
+/* finite_volume_1d.hpp:667-671 moved(): where a point at x0 is after dt (wrapped on a periodic line, held between the
+ * walls otherwise). */
+static double line_moved(const orc_line* l, double x0, double velocity, double dt) {
+    const double side = l->side;
+    const double y = fma(dt, velocity, x0);
+    return l->periodic ? y - side * floor(y / side) : fmin(fmax(y, 0.0), side);
+}
+
+/* sorted_mesh::snap (voronoi-cpp, absent): the nearest grid key of x, wrapped into [0, period) on a periodic line. */
+static long long line_snap(const orc_line* l, double x) {
+    long long k = (long long)round(x * (1.0 / l->step));
+    if (l->periodic) {
+        k %= l->period;
+        if (k < 0) k += l->period;
+    }
+    return k;
+}
+
+/* sorted_mesh::before (voronoi-cpp, absent): the line's order, by key, then by cell. */
+static int line_before(long long ka, int a, long long kb, int b) { return ka < kb || (ka == kb && a < b); }
+
+/* finite_volume_1d.hpp:508-524 extrapolated(): W + g r, a whole step on (MUSCL-Hancock) when `second`; W itself if rho or
+ * P would not stay positive. */
+static void line_extrapolated(const orc_line* l, const double* W, const double* g, double r, int second, double dt, double* out) {
+    if (l->order == 0) {
+        for (int v = 0; v < 3; ++v) out[v] = W[v];
+        return;
+    }
+    if (second)
+        orc_half_step(1, &l->gas, W, g, 2 * dt, out);
+    else
+        for (int v = 0; v < 3; ++v) out[v] = W[v];
+    for (int v = 0; v < 3; ++v) out[v] += g[v] * r;
+    if (out[0] > 0 && out[2] > 0) return;
+    for (int v = 0; v < 3; ++v) out[v] = W[v];
+}
+
+/* finite_volume_1d.hpp:555-661 Kernel::body() (moving): the gradients and point velocities of slots s - 1, s, s + 1, their
+ * new keys and moves in grid steps; each face's flux the mean of those on the meshes of t^n and t^n+1, the face moving by
+ * the mean of its generators' moves; the new state and position of the slot's cell, its new key into key_next. Returns the
+ * next dt by LUNA's rule (fused) and whether slot s and the next slot are out of order after the move (*descent). */
+static double line_body_moving(const orc_line* l, int s, double dt, double courant, int fused, const double* x, double* x_next,
+                               long long* key_next, int* descent) {
+    line_cell cell[5];
+    double gap[4];
+    line_load(l, s, cell, gap);
+    double grad[5][3], point[5] = {0, 0, 0, 0, 0};
+    memset(grad, 0, sizeof grad);
+    for (int k = 1; k <= 3; ++k) {
+        if (!cell[k].real) continue;
+        if (l->order == 1)
+            for (int v = 0; v < 3; ++v) grad[k][v] = line_gradient(l, cell + k, gap[k - 1], gap[k], v);
+        point[k] = cell[k].W[1] + line_point_velocity(l, cell + k, gap[k - 1], gap[k], grad[k][0], 0.5 * (gap[k - 1] + gap[k]));
+    }
+    long long key[5] = {0, 0, 0, 0, 0};
+    double moves[5] = {0, 0, 0, 0, 0};
+    for (int k = 1; k <= 3; ++k) {
+        if (!cell[k].real) continue;
+        const int t = line_slot(l, s, k - 2);
+        key[k] = line_snap(l, line_moved(l, x[line_id(l, t)], point[k], dt));
+        long long d = key[k] - l->key[t];
+        if (l->periodic) d = d > l->period / 2 ? d - l->period : d < -l->period / 2 ? d + l->period : d;
+        moves[k] = (double)d;
+    }
+    double F[2][3];
+    for (int f = 0; f < 2; ++f) {
+        const int a = 1 + f, b = 2 + f;
+        const int inside = cell[a].real && cell[b].real;
+        const double travel = inside ? 0.5 * (moves[a] + moves[b]) * l->step : 0.0;
+        const double w = travel / dt;
+        const double ra = 0.5 * gap[a] - cell[a].c, rb = -0.5 * gap[a] - cell[b].c;
+        const double one = 1.0;
+        double on[2][3];
+        const int meshes = l->order == 0 ? 1 : 2;
+        for (int second = 0; second < meshes; ++second) {
+            double L[3] = {0, 0, 0}, R[3] = {0, 0, 0};
+            if (cell[a].real) line_extrapolated(l, cell[a].W, grad[a], ra + second * travel, second, dt, L);
+            if (cell[b].real) line_extrapolated(l, cell[b].W, grad[b], rb + second * travel, second, dt, R);
+            if (!cell[a].real) line_ghost_face(l, R, cell[b].W, 1, L);
+            if (!cell[b].real) line_ghost_face(l, L, cell[a].W, 0, R);
+            riemann(l->riemann, 1, &l->gas, L, R, &one, &w, on[second]);
+        }
+        for (int v = 0; v < 3; ++v) F[f][v] = meshes == 1 ? on[0][v] : 0.5 * (on[0][v] + on[1][v]);
+    }
+    const int i = line_id(l, s);
+    double Unew[3];
+    for (int v = 0; v < 3; ++v) {
+        Unew[v] = l->q[i * 3 + v] - dt * (F[1][v] - F[0][v]);
+        l->next[i * 3 + v] = Unew[v];
+    }
+    x_next[i] = line_moved(l, x[i], point[2], dt);
+    key_next[s] = key[2];
+    *descent = 0;
+    if (cell[3].real && s < l->n - 1) *descent = !line_before(key[2], i, key[3], line_id(l, s + 1));
+    if (!fused) return INFINITY;
+    /* :636-659: V from the new keys (a wall's gap twice the distance to it), |w - v| of t^n. */
+    double left, right;
+    if (cell[1].real) {
+        const long long d = key[2] - key[1];
+        left = (double)(d < 0 && l->periodic ? d + l->period : d);
+    } else {
+        left = (double)(2 * key[2]);
+    }
+    if (cell[3].real) {
+        const long long d = key[3] - key[2];
+        right = (double)(d < 0 && l->periodic ? d + l->period : d);
+    } else {
+        right = (double)(2 * (l->period - key[2]));
+    }
+    const double volume = (left + right) * (0.5 * l->step);
+    const double relative = fabs(point[2] - cell[2].W[1]);
+    const double inv = 1.0 / volume;
+    const double U[3] = {Unew[0] * inv, Unew[1] * inv, Unew[2] * inv};
+    double Wnew[3];
+    orc_primitive(1, &l->gas, U, Wnew);
+    return courant * volume / (relative + orc_sound_speed(1, &l->gas, Wnew));
+}
+
+/* finite_volume_1d.hpp:332-386 time_step(): LUNA's rule or AREPO's (with the wall ghosts' pair terms); on a moving line |v|
+ * is |w - v| of the point velocity, from the cell's density gradient. */
 double orc_line_time_step(orc_line* l, double courant, int luna_rule) {
     double smallest = INFINITY;
 #pragma omp parallel for reduction(min : smallest)
@@ -893,7 +1048,14 @@ double orc_line_time_step(orc_line* l, double courant, int luna_rule) {
         line_primitive(l, s, Wi);
         const double ci = orc_sound_speed(1, &l->gas, Wi);
         const double volume = orc_line_volume(l, s);
-        const double vi = fabs(Wi[1]);
+        double vi = fabs(Wi[1]);
+        if (l->moving) {
+            line_cell cell[5];
+            double gap[4];
+            line_load(l, s, cell, gap);
+            const double g = l->order == 1 ? line_gradient(l, cell + 2, gap[1], gap[2], 0) : 0.0;
+            vi = fabs(line_point_velocity(l, cell + 2, gap[1], gap[2], g, volume));
+        }
         if (luna_rule) {
             smallest = fmin(smallest, courant * volume / (vi + ci));
             continue;
@@ -935,6 +1097,293 @@ double orc_line_step_fused(orc_line* l, double dt, double courant) {
 #pragma omp parallel for reduction(min : smallest) schedule(static)
     for (int s = 0; s < l->n; ++s) smallest = fmin(smallest, line_body(l, s, dt, courant, 1));
     line_swap(l);
+    return smallest;
+}
+
+/* ------------------------------------------------------------------ D = 1 on a moving line ----------------------------------- */
+
+struct orc_mline {
+    orc_line l;               /* l.key, l.id, l.q, l.next point into the arrays below */
+    long long *keys, *keys_next;
+    int* ids;
+    double *q, *next, *x, *x_next;
+    int capacity;
+    int supply_left, supply_right, watching, looked, wants, until_look;
+    double spacing;
+    long long added, updates, sorts;
+};
+
+static void mline_bind(orc_mline* m) {
+    m->l.key = m->keys;
+    m->l.id = m->ids;
+    m->l.q = m->q;
+    m->l.next = m->next;
+}
+
+/* finite_volume_1d.hpp:83-86 room_at(): an end cell of this volume, its generator d grid steps from the wall. */
+static int mline_room_at(const orc_mline* m, double volume, long long d) {
+    return volume > 2 * m->spacing && (double)d * m->l.step >= 1.5 * m->spacing;
+}
+/* finite_volume_1d.hpp:88-93 wants_cell(): 0..2 cells end slot s asks its wall for. */
+static int mline_wants_cell(const orc_mline* m, int s) {
+    const orc_line* l = &m->l;
+    int wants = 0;
+    if (s == 0 && m->supply_left) wants += mline_room_at(m, orc_line_volume(l, 0), l->key[0]);
+    if (s == l->n - 1 && m->supply_right) wants += mline_room_at(m, orc_line_volume(l, s), l->period - l->key[s]);
+    return wants;
+}
+
+/* finite_volume_1d.hpp:63-68 set_inflow_spacing(): an end supplies cells on a moving line with walls, unless reflective. */
+static void mline_set_spacing(orc_mline* m, double spacing) {
+    m->spacing = spacing;
+    m->supply_left = !m->l.periodic && m->l.wall_left != 0 && spacing > 0;
+    m->supply_right = !m->l.periodic && m->l.wall_right != 0 && spacing > 0;
+}
+
+orc_mline* orc_mline_create(int n, const double* x, double side, long long period, int periodic, int wall_left, int wall_right,
+                            const double* inflow_left, const double* inflow_right, int order, int limiter, int riemann, double gamma,
+                            double spacing, int regularise) {
+    orc_mline* m = (orc_mline*)calloc(1, sizeof(orc_mline));
+    m->capacity = n;
+    m->keys = (long long*)malloc(sizeof(long long) * n);
+    m->keys_next = (long long*)malloc(sizeof(long long) * n);
+    m->ids = (int*)malloc(sizeof(int) * n);
+    m->q = (double*)calloc(3 * (size_t)n, sizeof(double));
+    m->next = (double*)calloc(3 * (size_t)n, sizeof(double));
+    m->x = (double*)malloc(sizeof(double) * n);
+    m->x_next = (double*)malloc(sizeof(double) * n);
+    orc_line* l = &m->l;
+    l->n = n;
+    l->periodic = periodic;
+    l->wall_left = wall_left;
+    l->wall_right = wall_right;
+    l->order = order;
+    l->limiter = limiter;
+    l->riemann = riemann;
+    orc_gas_init(&l->gas, gamma);
+    l->period = period;
+    l->step = side / (double)period;
+    l->side = side;
+    l->moving = 1;
+    l->regularise = regularise;
+    for (int v = 0; v < 3; ++v) {
+        l->inflow[0][v] = inflow_left ? inflow_left[v] : 0.0;
+        l->inflow[1][v] = inflow_right ? inflow_right[v] : 0.0;
+    }
+    for (int i = 0; i < n; ++i) {
+        m->x[i] = x[i];
+        m->ids[i] = i;
+    }
+    mline_bind(m);
+    for (int i = 0; i < n; ++i) m->keys[i] = line_snap(l, x[i]);
+    mline_set_spacing(m, spacing);
+    return m;
+}
+
+void orc_mline_destroy(orc_mline* m) {
+    if (!m) return;
+    free(m->keys);
+    free(m->keys_next);
+    free(m->ids);
+    free(m->q);
+    free(m->next);
+    free(m->x);
+    free(m->x_next);
+    free(m);
+}
+
+int orc_mline_cells(const orc_mline* m) { return m->l.n; }
+void orc_mline_set_state(orc_mline* m, const double* q) { memcpy(m->q, q, sizeof(double) * 3 * m->l.n); }
+void orc_mline_get_state(const orc_mline* m, double* q) { memcpy(q, m->q, sizeof(double) * 3 * m->l.n); }
+void orc_mline_get_positions(const orc_mline* m, double* x) { memcpy(x, m->x, sizeof(double) * m->l.n); }
+void orc_mline_get_line(const orc_mline* m, long long* keys, int* ids) {
+    memcpy(keys, m->keys, sizeof(long long) * m->l.n);
+    memcpy(ids, m->ids, sizeof(int) * m->l.n);
+}
+long long orc_mline_stat(const orc_mline* m, int what) { return what == 0 ? m->added : what == 1 ? m->updates : m->sorts; }
+
+/* finite_volume_1d.hpp:332-386, 372-381: time_step(); while gas enters at an end, also what the end slots ask their walls for. */
+double orc_mline_time_step(orc_mline* m, double courant, int luna_rule) {
+    const double dt = orc_line_time_step(&m->l, courant, luna_rule);
+    if (m->watching) {
+        m->wants = mline_wants_cell(m, 0) + mline_wants_cell(m, m->l.n - 1);
+        m->looked = 1;
+    }
+    return dt;
+}
+
+typedef struct {
+    long long key;
+    int id;
+} mline_entry;
+static int mline_compare(const void* a, const void* b) {
+    const mline_entry* x = (const mline_entry*)a;
+    const mline_entry* y = (const mline_entry*)b;
+    if (line_before(x->key, x->id, y->key, y->id)) return -1;
+    return line_before(y->key, y->id, x->key, x->id) ? 1 : 0;
+}
+
+/* Mesh<1>::order(points) (voronoi-cpp, absent): the slots in key order again after the points moved. The keys are those
+ * the step computed for the new positions; a sort runs only when two slots are out of order. A periodic line is kept
+ * starting at its smallest key, so a point crossing the seam also re-sorts (each slot's neighbours are the same either way). */
+static void mline_order(orc_mline* m, int descents) {
+    m->updates += 1;
+    if (descents == 0) return;
+    const int n = m->l.n;
+    mline_entry* e = (mline_entry*)malloc(sizeof(mline_entry) * n);
+    for (int s = 0; s < n; ++s) {
+        e[s].key = m->keys[s];
+        e[s].id = m->ids[s];
+    }
+    qsort(e, n, sizeof(mline_entry), mline_compare);
+    for (int s = 0; s < n; ++s) {
+        m->keys[s] = e[s].key;
+        m->ids[s] = e[s].id;
+    }
+    free(e);
+    m->sorts += 1;
+}
+
+/* finite_volume_1d.hpp:965-981 reserve_cells(): room for `count` cells, a quarter more when full. */
+static void mline_reserve(orc_mline* m, int count) {
+    if (count <= m->capacity) return;
+    int cap = m->capacity + m->capacity / 4;
+    if (cap < count) cap = count;
+    m->keys = (long long*)realloc(m->keys, sizeof(long long) * cap);
+    m->keys_next = (long long*)realloc(m->keys_next, sizeof(long long) * cap);
+    m->ids = (int*)realloc(m->ids, sizeof(int) * cap);
+    m->q = (double*)realloc(m->q, sizeof(double) * 3 * cap);
+    m->next = (double*)realloc(m->next, sizeof(double) * 3 * cap);
+    m->x = (double*)realloc(m->x, sizeof(double) * cap);
+    m->x_next = (double*)realloc(m->x_next, sizeof(double) * cap);
+    m->capacity = cap;
+    mline_bind(m);
+}
+
+/* finite_volume_1d.hpp:419-458 supply_cells(): the ends' distances from their walls and wishes; while gas enters at an open
+ * end the line is watched every step. A wanted cell's generator goes `spacing` inside the end generator's distance from the
+ * wall; Mesh<1>::insert (voronoi-cpp, absent) puts it in a new end slot, and the new cell takes from the old end cell the
+ * fraction of it that it overlaps (its whole length, from the wall to halfway to the old end generator) of mass, momentum
+ * and energy. New cells are numbered after the last, the left wall's first. */
+static int mline_supply(orc_mline* m) {
+    orc_line* l = &m->l;
+    const int last = l->n - 1;
+    double ends[5];
+    ends[0] = (double)l->key[0] * l->step;
+    ends[1] = mline_wants_cell(m, 0);
+    ends[2] = (double)(l->period - l->key[last]) * l->step;
+    ends[3] = mline_wants_cell(m, last);
+    double W0[3], Wl[3];
+    line_primitive(l, 0, W0);
+    line_primitive(l, last, Wl);
+    ends[4] = (m->supply_left && W0[1] > 0) || (m->supply_right && Wl[1] < 0);
+    m->watching = m->watching || ends[4] > 0;
+    double wanted[2];
+    int left[2], requests = 0;
+    if (ends[1] > 0) {
+        left[requests] = 1;
+        wanted[requests++] = ends[0] - m->spacing;
+    }
+    if (ends[3] > 0) {
+        left[requests] = 0;
+        wanted[requests++] = l->side - (ends[2] - m->spacing);
+    }
+    if (requests == 0) return 0;
+    mline_reserve(m, l->n + requests);
+    l = &m->l;
+    for (int r = 0; r < requests; ++r) {
+        const int n = l->n, i = n;
+        const long long k = line_snap(l, wanted[r]);
+        double overlap, volume;
+        int old;
+        if (left[r]) {
+            old = m->ids[0];
+            overlap = ((double)(2 * k) + (double)(m->keys[0] - k)) * (0.5 * l->step);
+            volume = orc_line_volume(l, 0);
+            memmove(m->keys + 1, m->keys, sizeof(long long) * n);
+            memmove(m->ids + 1, m->ids, sizeof(int) * n);
+            m->keys[0] = k;
+            m->ids[0] = i;
+        } else {
+            old = m->ids[n - 1];
+            overlap = ((double)(k - m->keys[n - 1]) + (double)(2 * (l->period - k))) * (0.5 * l->step);
+            volume = orc_line_volume(l, n - 1);
+            m->keys[n] = k;
+            m->ids[n] = i;
+        }
+        for (int v = 0; v < 3; ++v) m->q[i * 3 + v] = 0;
+        const double fraction = overlap / volume;
+        for (int v = 0; v < 3; ++v) {
+            const double part = fraction * m->q[old * 3 + v];
+            m->q[old * 3 + v] -= part;
+            m->q[i * 3 + v] += part;
+        }
+        m->x[i] = wanted[r];
+        l->n = n + 1;
+        m->added += 1;
+    }
+    return 1;
+}
+
+/* finite_volume_1d.hpp:950-961 due(): look at the ends after this step: as the kernels asked while gas enters, else every
+ * 64 steps. */
+static int mline_due(orc_mline* m) {
+    if (!m->supply_left && !m->supply_right) return 0;
+    if (m->watching) return !m->looked || m->wants > 0;
+    if (--m->until_look > 0) return 0;
+    m->until_look = 64;
+    return 1;
+}
+
+static int mline_kernel(orc_mline* m, double dt, double courant, int fused, double* smallest) {
+    orc_line* l = &m->l;
+    int descents = 0;
+    double least = INFINITY;
+#pragma omp parallel for reduction(min : least) reduction(+ : descents) schedule(static)
+    for (int s = 0; s < l->n; ++s) {
+        int d = 0;
+        least = fmin(least, line_body_moving(l, s, dt, courant, fused, m->x, m->x_next, m->keys_next, &d));
+        descents += d;
+    }
+    *smallest = least;
+    double* t = m->q;
+    m->q = m->next;
+    m->next = t;
+    t = m->x;
+    m->x = m->x_next;
+    m->x_next = t;
+    long long* k = m->keys;
+    m->keys = m->keys_next;
+    m->keys_next = k;
+    mline_bind(m);
+    return descents;
+}
+
+/* finite_volume_1d.hpp:389-407 step() (moving): the kernel, Q and the positions swap, the order is updated, and the ends
+ * supply cells when due. */
+void orc_mline_step(orc_mline* m, double dt) {
+    double unused;
+    const int descents = mline_kernel(m, dt, 0.0, 0, &unused);
+    mline_order(m, descents);
+    if (mline_due(m)) mline_supply(m);
+    m->looked = 0;
+}
+
+/* finite_volume_1d.hpp:412-417, 679-708 step_and_time_step() (moving): the step and the next dt by LUNA's rule from the new
+ * keys; after a sort or new cells the next dt is time_step(courant, luna) of the new line. */
+double orc_mline_step_fused(orc_mline* m, double dt, double courant) {
+    int wants = 0;
+    if (m->watching) wants = mline_wants_cell(m, 0) + mline_wants_cell(m, m->l.n - 1);
+    double smallest;
+    const int descents = mline_kernel(m, dt, courant, 1, &smallest);
+    int again = 0;
+    mline_order(m, descents);
+    if (descents != 0) again = 1;
+    m->wants = wants;
+    m->looked = m->watching;
+    if (mline_due(m)) again = mline_supply(m) || again;
+    m->looked = 0;
+    if (again) return orc_mline_time_step(m, courant, 1);
     return smallest;
 }
 

@@ -89,16 +89,22 @@ int orc_step_adjoint(orc_fv* fv, const orc_geometry* geo, double dt, double* qba
 long long orc_snap(double x, double step, long long period);
 int orc_delta(long long k0, long long k1, long long period);
 
-/* 1D sorted line (finite_volume_1d.hpp), static; slots are cells (order = identity).
- * wall kinds: 0 reflective, 1 outflow. limiter: 0 face centroid (MC on a uniform line), 1 none, 2 minmod, 3 van Leer. */
+/* 1D sorted line (finite_volume_1d.hpp). Static: slots are cells (id NULL). Moving: see orc_mline_*, which owns one.
+ * wall kinds: 0 reflective, 1 outflow, 2 inflow (moving only). limiter: 0 face centroid (MC on a uniform line), 1 none,
+ * 2 minmod, 3 van Leer. */
 typedef struct {
     int n, periodic, wall_left, wall_right, order, limiter, riemann;
     orc_gas gas;
-    const long long* key; /* ascending grid keys */
+    const long long* key; /* ascending grid keys, by slot */
     long long period;
     double step;
-    double* q;    /* n x 3 */
-    double* next; /* n x 3 */
+    double* q;    /* n x 3, by cell */
+    double* next; /* n x 3, by cell */
+    /* moving line (zero / NULL on a static one) */
+    int moving, regularise;
+    const int* id;       /* slot -> cell; NULL: slot s holds cell s */
+    double side;         /* period * step */
+    double inflow[2][3]; /* primitives of an inflow wall, left and right */
 } orc_line;
 
 double orc_line_time_step(orc_line* line, double courant, int luna_rule);
@@ -124,6 +130,26 @@ void orc_hll_tangent(int D, const orc_gas* gas, const double* L, const double* R
 void orc_hll_adjoint(int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, const double* Fbar,
                      double* Lbar, double* Rbar);
 double orc_line_centroid(const orc_line* line, int s);
+
+/* 1D moving line (finite_volume_1d.hpp with Motion::moving): the generators drift with the gas plus the regularisation,
+ * the line keeps its slots in key order, and an open wall (outflow or inflow) supplies cells split from the end cell.
+ * Positions x in [0, side] (walls) or [0, side) (periodic); the grid has `period` keys per side. Cells are numbered in
+ * the order they were made; q and x are by cell, the line by slot. */
+typedef struct orc_mline orc_mline;
+orc_mline* orc_mline_create(int n, const double* x, double side, long long period, int periodic, int wall_left, int wall_right,
+                            const double* inflow_left, const double* inflow_right, int order, int limiter, int riemann, double gamma,
+                            double spacing, int regularise);
+void orc_mline_destroy(orc_mline* m);
+int orc_mline_cells(const orc_mline* m);
+void orc_mline_set_state(orc_mline* m, const double* q);
+void orc_mline_get_state(const orc_mline* m, double* q);
+void orc_mline_get_positions(const orc_mline* m, double* x);
+void orc_mline_get_line(const orc_mline* m, long long* keys, int* ids);
+/* what: 0 cells added, 1 order updates, 2 sorts */
+long long orc_mline_stat(const orc_mline* m, int what);
+double orc_mline_time_step(orc_mline* m, double courant, int luna_rule);
+void orc_mline_step(orc_mline* m, double dt);
+double orc_mline_step_fused(orc_mline* m, double dt, double courant);
 
 #ifdef __cplusplus
 }

@@ -40,7 +40,8 @@ class _Line(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("periodic", ctypes.c_int), ("wall_left", ctypes.c_int),
                 ("wall_right", ctypes.c_int), ("order", ctypes.c_int), ("limiter", ctypes.c_int),
                 ("riemann", ctypes.c_int), ("gas", Gas), ("key", _lp), ("period", ctypes.c_longlong),
-                ("step", ctypes.c_double), ("q", _dp), ("next", _dp)]
+                ("step", ctypes.c_double), ("q", _dp), ("next", _dp), ("moving", ctypes.c_int), ("regularise", ctypes.c_int),
+                ("id", _ip), ("side", ctypes.c_double), ("inflow", (ctypes.c_double * 3) * 2)]
 
 
 def lib():
@@ -59,6 +60,22 @@ def lib():
         L.orc_snap.restype = ctypes.c_longlong
         L.orc_snap.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_longlong]
         L.orc_delta.argtypes = [ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong]
+        L.orc_mline_create.restype = ctypes.c_void_p
+        L.orc_mline_create.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.orc_mline_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_mline_cells.argtypes = [ctypes.c_void_p]
+        L.orc_mline_set_state.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_mline_get_state.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_mline_get_positions.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_mline_get_line.argtypes = [ctypes.c_void_p, _lp, _ip]
+        L.orc_mline_stat.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_mline_stat.restype = ctypes.c_longlong
+        L.orc_mline_time_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+        L.orc_mline_time_step.restype = ctypes.c_double
+        L.orc_mline_step.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_mline_step_fused.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.orc_mline_step_fused.restype = ctypes.c_double
         _LIB = L
     return _LIB
 
@@ -271,7 +288,7 @@ class FiniteVolume:
         return int(self.c.fallbacks[0] + self.c.fallbacks[1])
 
 
-WALLS = {"reflective": 0, "outflow": 1}
+WALLS = {"reflective": 0, "outflow": 1, "inflow": 2}
 LIMITERS = {"face": 0, "none": 1, "minmod": 2, "vanleer": 3}
 
 
@@ -337,3 +354,68 @@ class Line:
 
     def step_and_time_step(self, dt, courant):
         return lib().orc_line_step_fused(ctypes.byref(self.c), ctypes.c_double(dt), ctypes.c_double(courant))
+
+
+class MovingLine:
+    """The oracle's FiniteVolume<1> on a moving line: n generators at positions x on [0, side] (walls) or a periodic
+    line; walls as ("outflow", "inflow", ...) with the inflow primitives in `inflow` (left, right); cells come from open
+    walls every `spacing`. state() and positions() are by cell (in the order cells were made), line() is (keys, ids) by
+    slot."""
+
+    def __init__(self, x, side, period, gamma, periodic=False, walls=("outflow", "outflow"), inflow=((0, 0, 0), (0, 0, 0)),
+                 order=1, limiter="face", riemann="hllc", spacing=0.0, regularise=True):
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+        il = np.ascontiguousarray(np.asarray(inflow[0], dtype=np.float64))
+        ir = np.ascontiguousarray(np.asarray(inflow[1], dtype=np.float64))
+        self._h = lib().orc_mline_create(len(x), _ptr(x), float(side), int(period), int(periodic), WALLS[walls[0]], WALLS[walls[1]],
+                                         _ptr(il), _ptr(ir), order, LIMITERS[limiter], {"hllc": 0, "hll": 1}[riemann], float(gamma),
+                                         float(spacing), int(regularise))
+        self.period, self.side, self.grid_step, self.periodic = int(period), float(side), float(side) / int(period), bool(periodic)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().orc_mline_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def n(self):
+        return lib().orc_mline_cells(self._h)
+
+    def set_state(self, q):
+        q = np.ascontiguousarray(np.asarray(q, dtype=np.float64))
+        assert q.shape == (self.n, 3)
+        lib().orc_mline_set_state(self._h, _ptr(q))
+
+    def state(self):
+        q = np.empty((self.n, 3))
+        lib().orc_mline_get_state(self._h, _ptr(q))
+        return q
+
+    def positions(self):
+        x = np.empty(self.n)
+        lib().orc_mline_get_positions(self._h, _ptr(x))
+        return x
+
+    def line(self):
+        keys = np.empty(self.n, dtype=np.int64)
+        ids = np.empty(self.n, dtype=np.int32)
+        lib().orc_mline_get_line(self._h, _ptr(keys, _lp), _ptr(ids, _ip))
+        return keys, ids
+
+    def cells_added(self):
+        return lib().orc_mline_stat(self._h, 0)
+
+    def sorts(self):
+        return lib().orc_mline_stat(self._h, 2)
+
+    def time_step(self, courant, rule="arepo"):
+        return lib().orc_mline_time_step(self._h, courant, int(rule == "luna"))
+
+    def step(self, dt):
+        lib().orc_mline_step(self._h, dt)
+
+    def step_and_time_step(self, dt, courant):
+        return lib().orc_mline_step_fused(self._h, dt, courant)

@@ -3,9 +3,12 @@ this build's static line: each problem on N uniform cells to its t_end under LUN
 hydro1d_cases_test.cpp:46-110 simulate()), then the L1 errors of rho, v, P with the rules of LUNA's tests
 (:164-221: exact cell averages of the Riemann solution, Noh's exact profile away from the wall, the linear waves from
 their +eps and -eps runs, the simple wave from its characteristics, three problems against stored fine-run densities).
-A row passes when each L1 equals LUNA's to the row's tolerance (or LUNA's is below the row's floor).
+A row passes when each L1 equals LUNA's to the row's tolerance (or LUNA's is below the row's floor). The moving gates
+(:314-327) compare with the reference's own moving runs (AREPO's rule at 0.4, cells supplied at open walls): the run
+reaches t_end and each L1 is at most 1.3 times theirs.
 
-The runs go through the oracle (CPU) or hydro.Line (the GPU): `engine` is a constructor with the oracle.Line signature.
+The runs go through the oracle (CPU) or the GPU: `engine` is a constructor with the oracle.Line signature (static) or
+the oracle.MovingLine signature (moving).
 """
 import math
 
@@ -42,6 +45,41 @@ def simulate(engine, name, n, limiter="face", amplitude=0.0):
     for s in range(n):
         W = oracle.primitive(q[s] / vol[s], pb.gamma)
         cells.append((a[s] + pb.origin, b[s] + pb.origin, W))
+    cells.sort(key=lambda c: c[0])
+    if hasattr(line, "close"):
+        line.close()
+    return cells, t, steps
+
+
+def simulate_moving(engine, name, n, limiter="face", amplitude=0.0):
+    """A moving run as the reference's simulate() with Motion::moving (hydro1d_cases_test.cpp:48-110): generators at
+    L (i + 1/2) / n, the problem's moving walls (an inflow wall where gas enters supersonically), cells supplied every
+    L / n, AREPO's rule at 0.4. `engine` has the oracle.MovingLine signature. cells [(a, b, W)] sorted by a, t, steps."""
+    pb = problems.problem_1d(name, amplitude)
+    step, period = problems.line_grid(pb.L)
+    x = pb.L * (np.arange(n) + 0.5) / n
+    walls, inflow = problems.moving_walls(pb)
+    line = engine(x, pb.L, period, pb.gamma, periodic=pb.periodic, walls=walls, inflow=inflow, order=1, limiter=limiter,
+                  spacing=pb.L / n)
+    keys, ids = line.line()
+    a, b, vol = problems.line_cells(keys, step, period, pb.periodic)
+    q = np.empty((n, 3))
+    for s in range(n):
+        u = pb.average(a[s] + pb.origin, b[s] + pb.origin)
+        q[ids[s]] = [u[k] * vol[s] for k in range(3)]
+    line.set_state(q)
+    t, steps = 0.0, 0
+    while t < pb.t_end * (1 - 1e-12) and steps < STEPS_PER_CELL * n:
+        dt = min(line.time_step(0.4, "arepo"), pb.t_end - t)
+        if not (dt > 1e-15 * pb.t_end):
+            break
+        line.step(dt)
+        t += dt
+        steps += 1
+    q = line.state()
+    keys, ids = line.line()
+    a, b, vol = problems.line_cells(keys, step, period, pb.periodic)
+    cells = [(a[s] + pb.origin, b[s] + pb.origin, oracle.primitive(q[ids[s]] / vol[s], pb.gamma)) for s in range(len(keys))]
     cells.sort(key=lambda c: c[0])
     if hasattr(line, "close"):
         line.close()
@@ -138,13 +176,14 @@ def reference_l1(cells, ref):
     return [sum(h * abs(avg[k] - rho[k]) for k in range(bins)), float("nan"), float("nan")]
 
 
-def measure(engine, name, n, limiter="face"):
-    """(L1 per variable, t reached) of a problem at N (hydro1d_cases_test.cpp:210-228)."""
+def measure(engine, name, n, limiter="face", moving=False):
+    """(L1 per variable, t reached) of a problem at N (hydro1d_cases_test.cpp:210-228), static or moving."""
+    run = simulate_moving if moving else simulate
     if is_linear(name):
-        plus, t, _ = simulate(engine, name, n, limiter, problems.LINEAR_AMPLITUDE)
-        minus, _, _ = simulate(engine, name, n, limiter, -problems.LINEAR_AMPLITUDE)
+        plus, t, _ = run(engine, name, n, limiter, problems.LINEAR_AMPLITUDE)
+        minus, _, _ = run(engine, name, n, limiter, -problems.LINEAR_AMPLITUDE)
         return l1_of(name, plus, minus), t
-    cells, t, _ = simulate(engine, name, n, limiter)
+    cells, t, _ = run(engine, name, n, limiter)
     if name in REFERENCES:
         return reference_l1(cells, REFERENCES[name]), t
     return l1_of(name, cells), t
@@ -173,3 +212,23 @@ def static_cases(max_n=None):
             if row[series][0] is not None:
                 cases.append((row, series))
     return cases
+
+
+def check_moving_row(engine, row):
+    """[(variable, L1, the reference's moving L1, ok)] of one moving gate (hydro1d_cases_test.cpp:314-327): the run reaches
+    t_end and each L1 is at most 1.3 times the reference's own moving run's; plus whether t_end was reached."""
+    pb = problems.problem_1d(row["problem"])
+    got, t = measure(engine, row["problem"], row["n"], "face", moving=True)
+    reached = t >= pb.t_end * (1 - 1e-12)
+    out = []
+    for v in range(3):
+        ref = row["moving"][v]
+        if ref is None:
+            continue
+        out.append((v, got[v], ref, got[v] <= 1.3 * ref))
+    return out, reached
+
+
+def moving_cases(max_n=None):
+    """The moving gates (rows with a measured moving run and no open issue), optionally up to N = max_n."""
+    return [row for row in GATES["rows"] if row["issue"] == 0 and row["moving"][0] is not None and not (max_n and row["n"] > max_n)]
