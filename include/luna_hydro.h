@@ -27,7 +27,7 @@ enum { LUNA_LIMITER_FACE_CENTROID = 0, LUNA_LIMITER_NONE = 1, LUNA_LIMITER_MINMO
 enum { LUNA_RIEMANN_HLLC = 0, LUNA_RIEMANN_HLL = 1 };
 enum { LUNA_MOTION_STATIC = 0, LUNA_MOTION_MOVING = 1 };
 enum { LUNA_STEP_AREPO = 0, LUNA_STEP_LUNA = 1 };
-enum { LUNA_WALL_REFLECTIVE = 0, LUNA_WALL_OUTFLOW = 1 };
+enum { LUNA_WALL_REFLECTIVE = 0, LUNA_WALL_OUTFLOW = 1, LUNA_WALL_INFLOW = 2 }; /* hydro::Wall, finite_volume_1d.hpp:9 */
 
 /* hydro::Gas (euler.hpp:82-101) and the constructor's options (finite_volume.hpp:79-80). The update loop is the
  * reference's per-face one (Update::per_face): flux rows, then a deterministic per-cell gather in list order. */
@@ -122,7 +122,7 @@ typedef struct {
 int luna_fv_profile(luna_fv* fv, int on);
 int luna_fv_profile_read(luna_fv* fv, luna_kernel_time* out, int max, int* count);
 
-/* ---------------------------------------------------------------- D = 1: FiniteVolume<1> on a static sorted line */
+/* ---------------------------------------------------------------- D = 1: FiniteVolume<1> on a sorted line */
 
 /* The line as finite_volume_1d.hpp reads voronoi-cpp's sorted_mesh::Line: cells in ascending order of their integer grid
  * keys, grid step `step`, `period` keys per side; walls at 0 and period * step, or periodic. */
@@ -155,6 +155,37 @@ int luna_fv1_time_step(luna_fv1* fv, double courant, int rule, double* dt);
 int luna_fv1_step(luna_fv1* fv, double dt);
 /* step_and_time_step(dt, courant) (finite_volume_1d.hpp:412-417): the step and the next dt by LUNA's rule, one launch. */
 int luna_fv1_step_and_time_step(luna_fv1* fv, double dt, double courant, double* next);
+
+/* The moving line: the generators as voronoi-cpp's Mesh<1> holds them, with the walls and the cell supply of
+ * FiniteVolume<1>(exec, mesh, gas, order, limiter, Motion::moving, walls, regularise) (finite_volume_1d.hpp:41-53) and
+ * set_inflow_spacing() (:62-67). The line's keys are the positions snapped to the grid (step = side / period). */
+typedef struct {
+    int cells;
+    const double* positions; /* cells generators in [0, side] (walls) or [0, side) (periodic) */
+    double side;
+    long long period; /* grid keys per side */
+    int periodic;
+    int wall_left, wall_right;              /* LUNA_WALL_*; an inflow wall holds inflow_left / inflow_right (Walls, :11-14) */
+    double inflow_left[3], inflow_right[3]; /* primitives rho, v, P */
+    double spacing;                         /* inflow_spacing(): cells from open walls this far apart; 0: none */
+    int regularise;                         /* the point velocity's regularisation (drift(), :301-314) */
+} luna_moving_line;
+
+/* FiniteVolume<1> with Motion::moving. Then time_step (AREPO's or LUNA's rule on |w - v|), step (the points drift, the
+ * order is updated, open walls supply cells when due, :389-407, :419-465) and step_and_time_step (:679-708) as on a static
+ * line; set_state / get_state take cells x 3 rows by cell, for the current number of cells (luna_fv1_cells), which grows
+ * as walls supply cells (new cells numbered after the last). */
+int luna_fv1_create_moving(const luna_fv1_options* options, const luna_moving_line* line, luna_fv1** out);
+/* cells() (:58) and cells_added() (:59). */
+int luna_fv1_cells(luna_fv1* fv, int* cells, long long* added);
+/* positions() (:57): cells positions, by cell. */
+int luna_fv1_positions(luna_fv1* fv, double* x);
+/* The line (Mesh<1>::line()): per slot in key order, its key and its cell; cells entries each. */
+int luna_fv1_line(luna_fv1* fv, long long* keys, int* cells);
+/* set_inflow_spacing(spacing) (:62-67). */
+int luna_fv1_set_inflow_spacing(luna_fv1* fv, double spacing);
+/* mesh_stats() (:71): order updates and the sorts among them. */
+int luna_fv1_mesh_stats(luna_fv1* fv, long long* updates, long long* sorts);
 
 /* FiniteVolume<1>::step_tangent(dt, dq) (finite_volume_1d.hpp:850-880): one step of Q and of its tangent; dq (host, cells x 3,
  * a tangent of the extensive Q as state()) becomes J dq in place. Static line, dt fixed. */

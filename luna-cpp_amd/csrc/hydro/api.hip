@@ -843,6 +843,28 @@ struct luna_fv1 {
     unsigned long long* host = nullptr;
     int parity = 0;
 
+    // The moving line. Q, next, x, x_next by cell, `capacity` cells; the line's slots at [lo, lo + n) of line_key[cur]
+    // (its keys), line_key[cur ^ 1] (the keys after a step) and line_id, `slots` long, so that a cell from the left wall
+    // takes slot lo - 1 without moving the others.
+    bool moving = false;
+    int regularise = 1;
+    double side = 0, inflow[2][3] = {};
+    int capacity = 0, slots = 0, lo = 0, cur = 0, dparity = 0;
+    Buf<long long> line_key[2];
+    Buf<int> line_id;
+    Buf<double> x, x_next, ends, inflow_dev;
+    Buf<unsigned long long> descents, packed;
+    Buf<unsigned char> sort_temp;
+    size_t sort_bytes = 0;
+    double* host_ends = nullptr;
+    unsigned long long* host_descents = nullptr;
+    // The cell supply (finite_volume_1d.hpp:991-996): which ends supply, whether gas enters (watching), whether this
+    // step's time_step() looked at the ends, what they asked for, steps to the next look; counters.
+    bool supply_left = false, supply_right = false, watching = false, looked = false;
+    double spacing = 0;
+    int wants = 0, until_look = 0;
+    long long added = 0, updates = 0, sorts = 0;
+
     luna::LineArgs args() const {
         luna::LineArgs a{};
         a.n = n;
@@ -858,10 +880,175 @@ struct luna_fv1 {
         a.step = step;
         a.q = q.p;
         a.next = next.p;
+        if (moving) {
+            a.moving = 1;
+            a.regularise = regularise;
+            a.key = line_key[cur].p + lo;
+            a.key_next = line_key[cur ^ 1].p + lo;
+            a.id = line_id.p + lo;
+            a.side = side;
+            a.inflow = inflow_dev.p;
+            a.x = x.p;
+            a.x_next = x_next.p;
+        }
         return a;
     }
     luna::Counters counters_now() const { return luna::Counters{counters.p + parity, counters.p + (parity ^ 1), counters.p + 2}; }
 };
+
+namespace {
+
+// A device array grown to `count` elements keeping `keep` of them (moved from offset `from` to offset `to`).
+template <class T>
+hipError_t grow(Buf<T>& b, size_t count, size_t keep, size_t from, size_t to, hipStream_t s) {
+    Buf<T> nb;
+    hipError_t e = nb.reserve(count);
+    if (e != hipSuccess) return e;
+    if (keep) e = hipMemcpyAsync(nb.p + to, b.p + from, sizeof(T) * keep, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        nb.release();
+        return e;
+    }
+    b.release();
+    b = nb;
+    return hipSuccess;
+}
+
+// finite_volume_1d.hpp:965-981 reserve_cells(): room for `count` cells, a quarter more when full.
+hipError_t reserve_cells(luna_fv1* fv, int count) {
+    if (count <= fv->capacity) return hipSuccess;
+    const int cap = std::max(count, fv->capacity + fv->capacity / 4);
+    hipError_t e;
+    if ((e = grow(fv->q, 3 * static_cast<size_t>(cap), 3 * static_cast<size_t>(fv->n), 0, 0, fv->stream)) != hipSuccess) return e;
+    if ((e = grow(fv->next, 3 * static_cast<size_t>(cap), 0, 0, 0, fv->stream)) != hipSuccess) return e;
+    if ((e = grow(fv->x, cap, fv->n, 0, 0, fv->stream)) != hipSuccess) return e;
+    if ((e = grow(fv->x_next, cap, 0, 0, 0, fv->stream)) != hipSuccess) return e;
+    fv->capacity = cap;
+    return hipSuccess;
+}
+
+// Room for a slot before the line (left) and after it (right): the slot arrays re-made with a margin of a quarter of
+// the line at each end when one is short.
+hipError_t reserve_slots(luna_fv1* fv, bool left, bool right) {
+    if ((!left || fv->lo > 0) && (!right || fv->lo + fv->n < fv->slots)) return hipSuccess;
+    const int margin = std::max(64, fv->n / 4);
+    const int slots = fv->n + 2 * margin + 2, lo = margin + 1;
+    hipError_t e;
+    if ((e = grow(fv->line_key[fv->cur], slots, fv->n, fv->lo, lo, fv->stream)) != hipSuccess) return e;
+    if ((e = grow(fv->line_key[fv->cur ^ 1], slots, 0, 0, 0, fv->stream)) != hipSuccess) return e;
+    if ((e = grow(fv->line_id, slots, fv->n, fv->lo, lo, fv->stream)) != hipSuccess) return e;
+    fv->slots = slots;
+    fv->lo = lo;
+    return hipSuccess;
+}
+
+double host_snap(const luna_fv1* fv, double x) {
+    long long k = static_cast<long long>(std::round(x * (1.0 / fv->step)));
+    if (fv->periodic) {
+        k %= fv->period;
+        if (k < 0) k += fv->period;
+    }
+    return static_cast<double>(k);
+}
+
+// The ends of the line (luna::LINE_ENDS values) into fv->host_ends; blocks.
+int read_ends(luna_fv1* fv) {
+    LUNA_HIP(luna::launch_line_ends(fv->stream, fv->args(), fv->spacing, fv->supply_left, fv->supply_right, fv->ends.p));
+    LUNA_HIP(hipMemcpyAsync(fv->host_ends, fv->ends.p, sizeof(double) * luna::LINE_ENDS, hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    return LUNA_OK;
+}
+
+// finite_volume_1d.hpp:419-465 supply_cells(): a wanted cell's generator goes `spacing` inside the end generator's
+// distance from its wall, in a new end slot; it takes from the old end cell the fraction it overlaps (its whole length,
+// from the wall to halfway to the old end generator) of mass, momentum and energy. New cells are numbered after the
+// last, the left wall's first. Returns whether cells were added into *added_any.
+int supply_cells(luna_fv1* fv, bool* added_any) {
+    *added_any = false;
+    const int rc = read_ends(fv);
+    if (rc != LUNA_OK) return rc;
+    const double* e = fv->host_ends;
+    fv->watching = fv->watching || e[4] > 0;
+    const bool left = e[1] > 0, right = e[3] > 0;
+    if (!left && !right) return LUNA_OK;
+    const int requests = int(left) + int(right);
+    LUNA_HIP(reserve_cells(fv, fv->n + requests));
+    LUNA_HIP(reserve_slots(fv, left, right));
+    luna::LineSplit split[2];
+    int r = 0;
+    const long long key0 = static_cast<long long>(e[7]), keyl = static_cast<long long>(e[8]);
+    int n = fv->n;
+    if (left) {
+        const double wanted = e[0] - fv->spacing;
+        const long long k = static_cast<long long>(host_snap(fv, wanted));
+        const double overlap = (static_cast<double>(2 * k) + static_cast<double>(key0 - k)) * (0.5 * fv->step);
+        split[r] = luna::LineSplit{n, static_cast<int>(e[9]), overlap / e[5], wanted, k, fv->line_key[fv->cur].p + fv->lo - 1,
+                                   fv->line_id.p + fv->lo - 1};
+        fv->lo -= 1;
+        ++n;
+        ++r;
+    }
+    if (right) {
+        const double wanted = fv->side - (e[2] - fv->spacing);
+        const long long k = static_cast<long long>(host_snap(fv, wanted));
+        const double overlap = (static_cast<double>(k - keyl) + static_cast<double>(2 * (fv->period - k))) * (0.5 * fv->step);
+        split[r] = luna::LineSplit{n, static_cast<int>(e[10]), overlap / e[6], wanted, k, fv->line_key[fv->cur].p + fv->lo + n,
+                                   fv->line_id.p + fv->lo + n};
+        ++n;
+        ++r;
+    }
+    LUNA_HIP(luna::launch_line_split(fv->stream, fv->q.p, fv->x.p, split, r));
+    fv->n = n;
+    fv->added += r;
+    *added_any = true;
+    return LUNA_OK;
+}
+
+// finite_volume_1d.hpp:954-961 due(): look at the ends after this step: as the kernels asked while gas enters, else
+// every 64 steps.
+bool due(luna_fv1* fv) {
+    if (!fv->supply_left && !fv->supply_right) return false;
+    if (fv->watching) return !fv->looked || fv->wants > 0;
+    if (--fv->until_look > 0) return false;
+    fv->until_look = 64;
+    return true;
+}
+
+// Mesh<1>::order(): the slots in key order again when the step's new keys left two adjacent slots out of order (the
+// points crossed, or one crossed a periodic line's seam: the line is kept starting at its smallest key).
+int order_line(luna_fv1* fv, unsigned long long descents, bool* sorted) {
+    fv->updates += 1;
+    *sorted = descents != 0;
+    if (!descents) return LUNA_OK;
+    const int n = fv->n;
+    LUNA_HIP(fv->packed.reserve(2 * static_cast<size_t>(n)));
+    size_t bytes = 0;
+    LUNA_HIP(luna::launch_line_sort(fv->stream, nullptr, nullptr, n, fv->packed.p, nullptr, &bytes));
+    LUNA_HIP(fv->sort_temp.reserve(bytes > 0 ? bytes : 1));
+    bytes = fv->sort_temp.cap;
+    LUNA_HIP(luna::launch_line_sort(fv->stream, fv->line_key[fv->cur].p + fv->lo, fv->line_id.p + fv->lo, n, fv->packed.p, fv->sort_temp.p,
+                                    &bytes));
+    fv->sorts += 1;
+    return LUNA_OK;
+}
+
+// The moving kernel, then Q, the positions and the keys swap; returns the descents it counted (blocks).
+int moving_kernel(luna_fv1* fv, double dt, double courant, bool fused, unsigned long long* descents) {
+    const int d = fv->dparity;
+    LUNA_HIP(luna::launch_line_step_moving(fv->stream, fv->args(), dt, courant, fused, fv->counters_now(), fv->descents.p + d,
+                                           fv->descents.p + (d ^ 1)));
+    fv->dparity ^= 1;
+    std::swap(fv->q, fv->next);
+    std::swap(fv->x, fv->x_next);
+    fv->cur ^= 1;
+    LUNA_HIP(hipMemcpyAsync(fv->host_descents, fv->descents.p + d, sizeof(unsigned long long), hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    *descents = fv->host_descents[0];
+    return LUNA_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -909,9 +1096,143 @@ int luna_fv1_create(const luna_fv1_options* opt, const luna_line* line, luna_fv1
     return LUNA_OK;
 }
 
+int luna_fv1_create_moving(const luna_fv1_options* opt, const luna_moving_line* line, luna_fv1** out) {
+    if (!opt || !line || !out || !line->positions) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_create_moving: null argument");
+    *out = nullptr;
+    if (opt->order != 0 && opt->order != 1) return fail(LUNA_ERR_ARGUMENT, "hydro::FiniteVolume<1>: order 0 or 1");
+    if (opt->limiter < 0 || opt->limiter > 3) return fail(LUNA_ERR_ARGUMENT, "luna: unknown limiter");
+    if (opt->riemann != LUNA_RIEMANN_HLLC && opt->riemann != LUNA_RIEMANN_HLL) return fail(LUNA_ERR_ARGUMENT, "luna: unknown Riemann solver");
+    if (line->cells < 3) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_create_moving: at least 3 cells");
+    if (line->period <= 0 || line->period > (1ll << 30) || !(line->side > 0) || !(line->spacing >= 0))
+        return fail(LUNA_ERR_ARGUMENT, "luna_fv1_create_moving: bad grid (period in 1 .. 2^30, side > 0, spacing >= 0)");
+    for (int wall : {line->wall_left, line->wall_right})
+        if (!line->periodic && wall != LUNA_WALL_REFLECTIVE && wall != LUNA_WALL_OUTFLOW && wall != LUNA_WALL_INFLOW)
+            return fail(LUNA_ERR_ARGUMENT, "luna_fv1_create_moving: unknown wall");
+    for (int s = 0; s < line->cells; ++s) {
+        const double p = line->positions[s];
+        if (!(p >= 0) || !(line->periodic ? p < line->side : p <= line->side))
+            return fail(LUNA_ERR_ARGUMENT, "luna_fv1_create_moving: a position outside the line");
+    }
+    luna_fv1* fv = new luna_fv1;
+    fv->opt = *opt;
+    fv->moving = true;
+    fv->n = line->cells;
+    fv->periodic = line->periodic;
+    fv->wall_left = line->periodic ? LUNA_WALL_OUTFLOW : line->wall_left;
+    fv->wall_right = line->periodic ? LUNA_WALL_OUTFLOW : line->wall_right;
+    fv->period = line->period;
+    fv->side = line->side;
+    fv->step = line->side / static_cast<double>(line->period);
+    fv->regularise = line->regularise != 0;
+    for (int v = 0; v < 3; ++v) {
+        fv->inflow[0][v] = line->inflow_left[v];
+        fv->inflow[1][v] = line->inflow_right[v];
+    }
+    auto cleanup = [&](int rc) {
+        luna_fv1_destroy(fv);
+        return rc;
+    };
+    const int n = fv->n;
+    // The slots in (key, cell) order of the snapped positions (Mesh<1>::update).
+    std::vector<std::pair<long long, int>> order(n);
+    for (int i = 0; i < n; ++i) order[i] = {static_cast<long long>(host_snap(fv, line->positions[i])), i};
+    std::sort(order.begin(), order.end());
+    const int margin = std::max(64, n / 4);
+    fv->slots = n + 2 * margin + 2;
+    fv->lo = margin + 1;
+    fv->capacity = n;
+    std::vector<long long> key(fv->slots, 0);
+    std::vector<int> id(fv->slots, 0);
+    for (int s = 0; s < n; ++s) {
+        key[fv->lo + s] = order[s].first;
+        id[fv->lo + s] = order[s].second;
+    }
+    if (hipStreamCreate(&fv->stream) != hipSuccess) return cleanup(fail(LUNA_ERR_DEVICE, "luna: hipStreamCreate failed"));
+    if (fv->q.reserve(3 * static_cast<size_t>(n)) != hipSuccess || fv->next.reserve(3 * static_cast<size_t>(n)) != hipSuccess ||
+        fv->x.reserve(n) != hipSuccess || fv->x_next.reserve(n) != hipSuccess || fv->line_key[0].reserve(fv->slots) != hipSuccess ||
+        fv->line_key[1].reserve(fv->slots) != hipSuccess || fv->line_id.reserve(fv->slots) != hipSuccess ||
+        fv->counters.reserve(4) != hipSuccess || fv->descents.reserve(2) != hipSuccess || fv->ends.reserve(luna::LINE_ENDS) != hipSuccess ||
+        fv->inflow_dev.reserve(6) != hipSuccess)
+        return cleanup(fail(LUNA_ERR_MEMORY, "luna: device allocation failed"));
+    if (hipHostMalloc(reinterpret_cast<void**>(&fv->host), 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&fv->host_ends), luna::LINE_ENDS * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&fv->host_descents), sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(LUNA_ERR_MEMORY, "luna: pinned host allocation failed"));
+    const unsigned long long init[4] = {luna::INF_BITS, luna::INF_BITS, 0, 0}, zeros[2] = {0, 0};
+    std::vector<double> zero(3 * static_cast<size_t>(n), 0.0);
+    if (hipMemcpy(fv->counters.p, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(fv->descents.p, zeros, sizeof zeros, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(fv->line_key[0].p, key.data(), sizeof(long long) * fv->slots, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(fv->line_id.p, id.data(), sizeof(int) * fv->slots, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(fv->x.p, line->positions, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(fv->inflow_dev.p, fv->inflow, sizeof fv->inflow, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(fv->q.p, zero.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(LUNA_ERR_DEVICE, "luna: initial upload failed"));
+    fv->spacing = line->spacing;
+    (void)luna_fv1_set_inflow_spacing(fv, line->spacing);
+    *out = fv;
+    return LUNA_OK;
+}
+
+int luna_fv1_set_inflow_spacing(luna_fv1* fv, double spacing) {
+    if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_set_inflow_spacing: null handle");
+    if (!(spacing >= 0)) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_set_inflow_spacing: spacing >= 0");
+    fv->spacing = spacing;
+    fv->supply_left = fv->moving && !fv->periodic && fv->wall_left != LUNA_WALL_REFLECTIVE && spacing > 0;
+    fv->supply_right = fv->moving && !fv->periodic && fv->wall_right != LUNA_WALL_REFLECTIVE && spacing > 0;
+    return LUNA_OK;
+}
+
+int luna_fv1_cells(luna_fv1* fv, int* cells, long long* added) {
+    if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_cells: null handle");
+    if (cells) *cells = fv->n;
+    if (added) *added = fv->added;
+    return LUNA_OK;
+}
+
+int luna_fv1_positions(luna_fv1* fv, double* x) {
+    if (!fv || !x) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_positions: null argument");
+    if (!fv->moving) return fail(LUNA_ERR_LOGIC, "luna_fv1_positions: a static line holds no positions");
+    LUNA_HIP(hipMemcpyAsync(x, fv->x.p, sizeof(double) * fv->n, hipMemcpyDeviceToHost, fv->stream));
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    return LUNA_OK;
+}
+
+int luna_fv1_line(luna_fv1* fv, long long* keys, int* cells) {
+    if (!fv || !keys || !cells) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_line: null argument");
+    if (fv->moving) {
+        LUNA_HIP(hipMemcpyAsync(keys, fv->line_key[fv->cur].p + fv->lo, sizeof(long long) * fv->n, hipMemcpyDeviceToHost, fv->stream));
+        LUNA_HIP(hipMemcpyAsync(cells, fv->line_id.p + fv->lo, sizeof(int) * fv->n, hipMemcpyDeviceToHost, fv->stream));
+    } else {
+        LUNA_HIP(hipMemcpyAsync(keys, fv->keys.p, sizeof(long long) * fv->n, hipMemcpyDeviceToHost, fv->stream));
+        for (int s = 0; s < fv->n; ++s) cells[s] = s;
+    }
+    LUNA_HIP(hipStreamSynchronize(fv->stream));
+    return LUNA_OK;
+}
+
+int luna_fv1_mesh_stats(luna_fv1* fv, long long* updates, long long* sorts) {
+    if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_mesh_stats: null handle");
+    if (updates) *updates = fv->updates;
+    if (sorts) *sorts = fv->sorts;
+    return LUNA_OK;
+}
+
 int luna_fv1_destroy(luna_fv1* fv) {
     if (!fv) return LUNA_OK;
     if (fv->stream) (void)hipStreamSynchronize(fv->stream);
+    fv->line_key[0].release();
+    fv->line_key[1].release();
+    fv->line_id.release();
+    fv->x.release();
+    fv->x_next.release();
+    fv->ends.release();
+    fv->inflow_dev.release();
+    fv->descents.release();
+    fv->packed.release();
+    fv->sort_temp.release();
+    if (fv->host_ends) (void)hipHostFree(fv->host_ends);
+    if (fv->host_descents) (void)hipHostFree(fv->host_descents);
     fv->keys.release();
     fv->q.release();
     fv->next.release();
@@ -944,6 +1265,13 @@ int luna_fv1_time_step(luna_fv1* fv, double courant, int rule, double* dt) {
     LUNA_HIP(luna::launch_line_time_step(fv->stream, fv->args(), courant, rule == LUNA_STEP_LUNA, fv->counters_now()));
     fv->parity ^= 1;
     LUNA_HIP(hipMemcpyAsync(fv->host, fv->counters.p + slot, sizeof(unsigned long long), hipMemcpyDeviceToHost, fv->stream));
+    if (fv->moving && fv->watching) {
+        // finite_volume_1d.hpp:372-381: while gas enters, what the end slots ask their walls for
+        const int rc = read_ends(fv);
+        if (rc != LUNA_OK) return rc;
+        fv->wants = static_cast<int>(fv->host_ends[1] + fv->host_ends[3]);
+        fv->looked = true;
+    }
     LUNA_HIP(hipStreamSynchronize(fv->stream));
     *dt = bits_to_double(fv->host[0]);
     return LUNA_OK;
@@ -951,13 +1279,25 @@ int luna_fv1_time_step(luna_fv1* fv, double courant, int rule, double* dt) {
 
 int luna_fv1_step(luna_fv1* fv, double dt) {
     if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_step: null argument");
-    LUNA_HIP(luna::launch_line_step(fv->stream, fv->args(), dt, 0.0, false, fv->counters_now()));
-    std::swap(fv->q, fv->next);
+    if (!fv->moving) {
+        LUNA_HIP(luna::launch_line_step(fv->stream, fv->args(), dt, 0.0, false, fv->counters_now()));
+        std::swap(fv->q, fv->next);
+        return LUNA_OK;
+    }
+    // finite_volume_1d.hpp:389-407 (moving): the kernel, the order, and the walls' cells when due.
+    unsigned long long descents = 0;
+    int rc = moving_kernel(fv, dt, 0.0, false, &descents);
+    if (rc != LUNA_OK) return rc;
+    bool sorted = false, added = false;
+    if ((rc = order_line(fv, descents, &sorted)) != LUNA_OK) return rc;
+    if (due(fv) && (rc = supply_cells(fv, &added)) != LUNA_OK) return rc;
+    fv->looked = false;
     return LUNA_OK;
 }
 
 int luna_fv1_step_tangent(luna_fv1* fv, double dt, double* dq) {
     if (!fv || !dq) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_step_tangent: null argument");
+    if (fv->moving) return fail(LUNA_ERR_LOGIC, "hydro::FiniteVolume<1>::step_tangent: a static line");
     const size_t bytes = sizeof(double) * 3 * fv->n;
     LUNA_HIP(fv->dq.reserve(3 * static_cast<size_t>(fv->n)));
     LUNA_HIP(fv->dnext.reserve(3 * static_cast<size_t>(fv->n)));
@@ -971,6 +1311,7 @@ int luna_fv1_step_tangent(luna_fv1* fv, double dt, double* dq) {
 
 int luna_fv1_step_adjoint(luna_fv1* fv, double dt, double* qbar) {
     if (!fv || !qbar) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_step_adjoint: null argument");
+    if (fv->moving) return fail(LUNA_ERR_LOGIC, "hydro::FiniteVolume<1>::step_adjoint: a static line");
     const size_t bytes = sizeof(double) * 3 * fv->n;
     LUNA_HIP(fv->dq.reserve(3 * static_cast<size_t>(fv->n)));
     LUNA_HIP(fv->dnext.reserve(3 * static_cast<size_t>(fv->n)));
@@ -984,6 +1325,32 @@ int luna_fv1_step_adjoint(luna_fv1* fv, double dt, double* qbar) {
 int luna_fv1_step_and_time_step(luna_fv1* fv, double dt, double courant, double* next) {
     if (!fv || !next) return fail(LUNA_ERR_ARGUMENT, "luna_fv1_step_and_time_step: null argument");
     const int slot = fv->parity;
+    if (fv->moving) {
+        // finite_volume_1d.hpp:679-708 (moving): the ends' wishes on the line before the step, the fused kernel, the
+        // order; after a sort or new cells the next dt is time_step(courant, luna) of the new line.
+        int wants = 0, rc;
+        if (fv->watching) {
+            if ((rc = read_ends(fv)) != LUNA_OK) return rc;
+            wants = static_cast<int>(fv->host_ends[1] + fv->host_ends[3]);
+        }
+        unsigned long long descents = 0;
+        if ((rc = moving_kernel(fv, dt, courant, true, &descents)) != LUNA_OK) return rc;
+        fv->parity ^= 1;
+        LUNA_HIP(hipMemcpyAsync(fv->host, fv->counters.p + slot, sizeof(unsigned long long), hipMemcpyDeviceToHost, fv->stream));
+        LUNA_HIP(hipStreamSynchronize(fv->stream));
+        *next = bits_to_double(fv->host[0]);
+        bool again = false, added = false;
+        if ((rc = order_line(fv, descents, &again)) != LUNA_OK) return rc;
+        fv->wants = wants;
+        fv->looked = fv->watching;
+        if (due(fv)) {
+            if ((rc = supply_cells(fv, &added)) != LUNA_OK) return rc;
+            again = again || added;
+        }
+        fv->looked = false;
+        if (again) return luna_fv1_time_step(fv, courant, LUNA_STEP_LUNA, next);
+        return LUNA_OK;
+    }
     LUNA_HIP(luna::launch_line_step(fv->stream, fv->args(), dt, courant, true, fv->counters_now()));
     fv->parity ^= 1;
     std::swap(fv->q, fv->next);

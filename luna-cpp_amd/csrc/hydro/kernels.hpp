@@ -118,18 +118,49 @@ hipError_t launch_step_tangent(hipStream_t s, const Geo& g, const Rows& r, const
 template <int D>
 hipError_t launch_step_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, bool limit, const Derivs& w);
 
-// D = 1: the static sorted line.
+// D = 1: the sorted line. Slots in key order; Q (and on a moving line the positions) by cell. Walls: 0 reflective,
+// 1 outflow, 2 inflow (a moving line's open wall holding the state inflow[side]).
 struct LineArgs {
     int n, periodic, wall_left, wall_right, order, limiter, riemann;
     Gas gas;
-    const long long* key;
+    const long long* key;  // by slot
     long long period;
     double step;
-    const double* q;
+    const double* q;  // by cell
     double* next;
+    // moving line
+    int moving, regularise;
+    const int* id;  // slot -> cell; nullptr: slot s holds cell s
+    double side;    // period * step
+    const double* inflow;  // device: the left, then the right inflow wall's primitives (6)
+    const double* x;  // positions by cell
+    double* x_next;
+    long long* key_next;  // by slot: the key of the slot's point after the step
 };
 hipError_t launch_line_time_step(hipStream_t s, const LineArgs& a, double courant, int luna_rule, const Counters& c);
 hipError_t launch_line_step(hipStream_t s, const LineArgs& a, double dt, double courant, bool fused, const Counters& c);
+// The moving step: Q, positions and keys after the step into a.next, a.x_next, a.key_next; the count of adjacent slots
+// out of order after the move into *descents (*descents_next reset for the next step); fused, the next dt by LUNA's rule.
+hipError_t launch_line_step_moving(hipStream_t s, const LineArgs& a, double dt, double courant, bool fused, const Counters& c,
+                                   unsigned long long* descents, unsigned long long* descents_next);
+// The ends of a moving line with walls, as the cell supply reads them (11 doubles): the left end generator's distance
+// from its wall, the cells that end asks for, the same at the right, whether gas enters at an open end, the end cells'
+// volumes, the end keys and the end cells.
+constexpr int LINE_ENDS = 11;
+hipError_t launch_line_ends(hipStream_t s, const LineArgs& a, double spacing, int supply_left, int supply_right, double* out);
+// A cell supplied by a wall: cell `cell` takes `fraction` of Q of cell `donor` and sits at `x`; the line gets the slot
+// (*key_at, *id_at) = (key, cell).
+struct LineSplit {
+    int cell, donor;
+    double fraction, x;
+    long long key;
+    long long* key_at;
+    int* id_at;
+};
+hipError_t launch_line_split(hipStream_t s, double* q, double* x, const LineSplit* splits, int count);
+// The slots in (key, cell) order again: device radix sort of key * 2^32 + cell. temp: scratch of *temp_bytes (a null
+// temp only sets *temp_bytes); packed: 2 n 64-bit words.
+hipError_t launch_line_sort(hipStream_t s, long long* key, int* id, int n, unsigned long long* packed, void* temp, size_t* temp_bytes);
 // Derivatives of the static step at fixed dt: the tangent writes the step into a.next and J dq into dnext; the
 // adjoint writes J^T qbar (q: the state before the step) into out.
 hipError_t launch_line_tangent(hipStream_t s, const LineArgs& a, double dt, const double* dq, double* dnext);

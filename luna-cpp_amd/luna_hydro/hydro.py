@@ -24,7 +24,7 @@ LIMITER_FEATURES, LIMITER_PARAMS = 16, 5313
 RIEMANN = {"hllc": 0, "hll": 1}
 MOTION = {"static": 0, "moving": 1}
 RULES = {"arepo": 0, "luna": 1}
-WALLS = {"reflective": 0, "outflow": 1}
+WALLS = {"reflective": 0, "outflow": 1, "inflow": 2}
 
 
 class FVOptions(ctypes.Structure):
@@ -47,6 +47,13 @@ class KernelTime(ctypes.Structure):
 class LineRecord(ctypes.Structure):
     _fields_ = [("cells", ctypes.c_int), ("keys", _lp), ("period", ctypes.c_longlong), ("step", ctypes.c_double),
                 ("periodic", ctypes.c_int), ("wall_left", ctypes.c_int), ("wall_right", ctypes.c_int)]
+
+
+class MovingLineRecord(ctypes.Structure):
+    _fields_ = [("cells", ctypes.c_int), ("positions", _dp), ("side", ctypes.c_double), ("period", ctypes.c_longlong),
+                ("periodic", ctypes.c_int), ("wall_left", ctypes.c_int), ("wall_right", ctypes.c_int),
+                ("inflow_left", ctypes.c_double * 3), ("inflow_right", ctypes.c_double * 3), ("spacing", ctypes.c_double),
+                ("regularise", ctypes.c_int)]
 
 
 class FV1Options(ctypes.Structure):
@@ -98,6 +105,8 @@ def exported_symbols():
              "luna_fv_positions", "luna_fv_point_velocities", "luna_fv_gradients", "luna_fv_primitives", "luna_fv_faces", "luna_fv_profile",
              "luna_fv_profile_read", "luna_fv1_create", "luna_fv1_destroy", "luna_fv1_set_state", "luna_fv1_get_state",
              "luna_fv1_time_step", "luna_fv1_step", "luna_fv1_step_and_time_step", "luna_fv1_step_tangent", "luna_fv1_step_adjoint",
+             "luna_fv1_create_moving", "luna_fv1_cells", "luna_fv1_positions", "luna_fv1_line", "luna_fv1_set_inflow_spacing",
+             "luna_fv1_mesh_stats",
              "luna_last_error",
              "luna_fv_set_geometry", "luna_synchronize", "luna_fv_step_tangent", "luna_fv_step_adjoint",
              "luna_fv_set_limiter_network", "luna_fv_limiter_features", "luna_mlp_create", "luna_mlp_destroy", "luna_mlp_set_params",
@@ -329,3 +338,58 @@ class Line:
         _check(lib().luna_fv1_step_and_time_step(self._h, ctypes.c_double(dt), ctypes.c_double(courant),
                                                   ctypes.byref(nxt)), "step_and_time_step")
         return nxt.value
+
+
+class MovingLine(Line):
+    """hydro::FiniteVolume<1> on a moving line (Motion::moving): n generators at positions x on [0, side] (walls) or a
+    periodic line with `period` grid keys; walls as ("outflow", "inflow", ...) with the inflow primitives in `inflow`
+    (left, right); open walls supply cells `spacing` apart. state() and positions() are by cell (new cells after the
+    last), line() is (keys, cells) by slot."""
+
+    def __init__(self, x, side, period, gamma, periodic=False, walls=("outflow", "outflow"), inflow=((0, 0, 0), (0, 0, 0)),
+                 order=1, limiter="face", riemann="hllc", spacing=0.0, regularise=True, smallr=1e-10, smallc=1e-10):
+        x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+        rec = MovingLineRecord(len(x), x.ctypes.data_as(_dp), float(side), int(period), int(periodic), WALLS[walls[0]],
+                               WALLS[walls[1]], (ctypes.c_double * 3)(*inflow[0]), (ctypes.c_double * 3)(*inflow[1]),
+                               float(spacing), int(regularise))
+        opt = FV1Options(order, LIMITERS[limiter], RIEMANN[riemann], gamma, smallr, smallc)
+        h = ctypes.c_void_p()
+        _check(lib().luna_fv1_create_moving(ctypes.byref(opt), ctypes.byref(rec), ctypes.byref(h)), "FiniteVolume<1> (moving)")
+        self._h = h
+        self.period, self.side, self.periodic = int(period), float(side), bool(periodic)
+
+    @property
+    def n(self):
+        cells = ctypes.c_int()
+        _check(lib().luna_fv1_cells(self._h, ctypes.byref(cells), None), "cells")
+        return cells.value
+
+    def cells_added(self):
+        added = ctypes.c_longlong()
+        _check(lib().luna_fv1_cells(self._h, None, ctypes.byref(added)), "cells_added")
+        return added.value
+
+    def positions(self):
+        x = np.empty(self.n)
+        _check(lib().luna_fv1_positions(self._h, x.ctypes.data_as(_dp)), "positions")
+        return x
+
+    def line(self):
+        n = self.n
+        keys, cells = np.empty(n, dtype=np.int64), np.empty(n, dtype=np.int32)
+        _check(lib().luna_fv1_line(self._h, keys.ctypes.data_as(_lp), cells.ctypes.data_as(_ip)), "line")
+        return keys, cells
+
+    def set_inflow_spacing(self, spacing):
+        _check(lib().luna_fv1_set_inflow_spacing(self._h, ctypes.c_double(spacing)), "set_inflow_spacing")
+
+    def mesh_stats(self):
+        updates, sorts = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().luna_fv1_mesh_stats(self._h, ctypes.byref(updates), ctypes.byref(sorts)), "mesh_stats")
+        return updates.value, sorts.value
+
+    def sorts(self):
+        return self.mesh_stats()[1]
+
+    def time_step(self, courant, rule="arepo"):
+        return super().time_step(courant, rule)

@@ -889,7 +889,7 @@ static double line_body(const orc_line* l, int s, double dt, double courant, int
     return courant * volume / (relative + orc_sound_speed(1, &l->gas, Wnew));
 }
 
-/* finite_volume_1d.hpp:298-314 drift(): the point's velocity relative to its cell's from AREPO-ng's regularisation, for
+/* finite_volume_1d.hpp:301-314 drift(): the point's velocity relative to its cell's from AREPO-ng's regularisation, for
  * primitives W, density gradient g, centroid c (relative to the generator) and volume. */
 #define LINE_SHAPING_SPEED 0.5 /* cell_shaping_speed, finite_volume_1d.hpp:26 */
 static double line_drift(const orc_line* l, const double* W, double g, double c, double volume) {
@@ -908,7 +908,7 @@ static double line_drift(const orc_line* l, const double* W, double g, double c,
     return offset * (fraction * orc_sound_speed(1, &l->gas, W) / d);
 }
 
-/* finite_volume_1d.hpp:316-328 point_velocity(): drift(), plus, for a generator beyond 0.75 r of its centroid, up to the
+/* finite_volume_1d.hpp:319-328 point_velocity(): drift(), plus, for a generator beyond 0.75 r of its centroid, up to the
  * fluid's velocity at the generator, linear between this cell's centroid and that of the neighbour on that side. */
 static double line_point_velocity(const orc_line* l, const line_cell* cell, double gapL, double gapR, double g_rho, double volume) {
     const line_cell* me = cell;
@@ -920,7 +920,7 @@ static double line_point_velocity(const orc_line* l, const line_cell* cell, doub
     return drifting + weight * (cell[side].W[1] - me->W[1]) * (d / between);
 }
 
-/* finite_volume_1d.hpp:667-671 moved(): where a point at x0 is after dt (wrapped on a periodic line, held between the
+/* finite_volume_1d.hpp:546-550 moved(): where a point at x0 is after dt (wrapped on a periodic line, held between the
  * walls otherwise). */
 static double line_moved(const orc_line* l, double x0, double velocity, double dt) {
     const double side = l->side;
@@ -941,7 +941,7 @@ static long long line_snap(const orc_line* l, double x) {
 /* sorted_mesh::before (voronoi-cpp, absent): the line's order, by key, then by cell. */
 static int line_before(long long ka, int a, long long kb, int b) { return ka < kb || (ka == kb && a < b); }
 
-/* finite_volume_1d.hpp:508-524 extrapolated(): W + g r, a whole step on (MUSCL-Hancock) when `second`; W itself if rho or
+/* finite_volume_1d.hpp:509-524 extrapolated(): W + g r, a whole step on (MUSCL-Hancock) when `second`; W itself if rho or
  * P would not stay positive. */
 static void line_extrapolated(const orc_line* l, const double* W, const double* g, double r, int second, double dt, double* out) {
     if (l->order == 0) {
@@ -1120,11 +1120,11 @@ static void mline_bind(orc_mline* m) {
     m->l.next = m->next;
 }
 
-/* finite_volume_1d.hpp:83-86 room_at(): an end cell of this volume, its generator d grid steps from the wall. */
+/* finite_volume_1d.hpp:86-88 room_at(): an end cell of this volume, its generator d grid steps from the wall. */
 static int mline_room_at(const orc_mline* m, double volume, long long d) {
     return volume > 2 * m->spacing && (double)d * m->l.step >= 1.5 * m->spacing;
 }
-/* finite_volume_1d.hpp:88-93 wants_cell(): 0..2 cells end slot s asks its wall for. */
+/* finite_volume_1d.hpp:89-94 wants_cell(): 0..2 cells end slot s asks its wall for. */
 static int mline_wants_cell(const orc_mline* m, int s) {
     const orc_line* l = &m->l;
     int wants = 0;
@@ -1133,7 +1133,7 @@ static int mline_wants_cell(const orc_mline* m, int s) {
     return wants;
 }
 
-/* finite_volume_1d.hpp:63-68 set_inflow_spacing(): an end supplies cells on a moving line with walls, unless reflective. */
+/* finite_volume_1d.hpp:62-67 set_inflow_spacing(): an end supplies cells on a moving line with walls, unless reflective. */
 static void mline_set_spacing(orc_mline* m, double spacing) {
     m->spacing = spacing;
     m->supply_left = !m->l.periodic && m->l.wall_left != 0 && spacing > 0;
@@ -1260,7 +1260,7 @@ static void mline_reserve(orc_mline* m, int count) {
     mline_bind(m);
 }
 
-/* finite_volume_1d.hpp:419-458 supply_cells(): the ends' distances from their walls and wishes; while gas enters at an open
+/* finite_volume_1d.hpp:419-465 supply_cells(): the ends' distances from their walls and wishes; while gas enters at an open
  * end the line is watched every step. A wanted cell's generator goes `spacing` inside the end generator's distance from the
  * wall; Mesh<1>::insert (voronoi-cpp, absent) puts it in a new end slot, and the new cell takes from the old end cell the
  * fraction of it that it overlaps (its whole length, from the wall to halfway to the old end generator) of mass, momentum
@@ -1325,7 +1325,7 @@ static int mline_supply(orc_mline* m) {
     return 1;
 }
 
-/* finite_volume_1d.hpp:950-961 due(): look at the ends after this step: as the kernels asked while gas enters, else every
+/* finite_volume_1d.hpp:954-961 due(): look at the ends after this step: as the kernels asked while gas enters, else every
  * 64 steps. */
 static int mline_due(orc_mline* m) {
     if (!m->supply_left && !m->supply_right) return 0;

@@ -85,3 +85,24 @@ def test_1d_driver_hash_matches_oracle(fused):
         steps += 1
     assert steps == res["steps"]
     assert res["hash"] == sedov.bits_hash(orc.state, hydro1d.FNV_BASIS)
+
+
+def test_1d_driver_moving_hash_matches_oracle():
+    """The 1D driver's moving mode (hydro_1d --moving: AREPO's rule at 0.4, cells from the inflow wall) on Shu-Osher with
+    256 cells: its hash (Q, then the positions) is the oracle's after the same steps, with the same final cells."""
+    from luna_hydro import hydro1d
+
+    lines = []
+    res = hydro1d.run("shu-osher", 256, moving=True, repeats=2, log=lines.append)
+    print("\n".join(lines), flush=True)
+    assert res["same_bits"] and "mesh=moving" in lines[-1]
+    pb, orc = hydro1d.make_line("shu-osher", 256, moving=True, engine=oracle.MovingLine)
+    t, steps = 0.0, 0
+    while t < pb.t_end * (1 - 1e-12):
+        dt = min(orc.time_step(0.4, "arepo"), pb.t_end - t)
+        orc.step(dt)
+        t += dt
+        steps += 1
+    assert steps == res["steps"] and orc.n == res["cells"] > 256
+    h = sedov.bits_hash(orc.positions(), sedov.bits_hash(orc.state(), hydro1d.FNV_BASIS))
+    assert res["hash"] == h
