@@ -134,30 +134,49 @@ private:
     int n_ = 0;
 };
 
-// FiniteVolume<1> (finite_volume_1d.hpp:41-53) on a static sorted line.
+// FiniteVolume<1> (finite_volume_1d.hpp:41-53) on a sorted line: static (the line's keys) or moving (the generators'
+// positions; the walls supply cells, so cells() grows; the accessors of :57-71).
 class Line {
 public:
-    Line(const luna_line& line, const Gas& gas, int order, Limiter limiter = Limiter::face_centroid, Riemann riemann = Riemann::hllc)
-        : n_(line.cells) {
-        luna_fv1_options o{};
-        o.order = order;
-        o.limiter = static_cast<int>(limiter);
-        o.riemann = static_cast<int>(riemann);
-        o.gamma = gas.gamma;
-        o.smallr = gas.smallr;
-        o.smallc = gas.smallc;
+    Line(const luna_line& line, const Gas& gas, int order, Limiter limiter = Limiter::face_centroid, Riemann riemann = Riemann::hllc) {
+        const luna_fv1_options o = options(gas, order, limiter, riemann);
         check(luna_fv1_create(&o, &line, &fv_), "hydro::FiniteVolume<1>");
+    }
+    Line(const luna_moving_line& line, const Gas& gas, int order, Limiter limiter = Limiter::face_centroid, Riemann riemann = Riemann::hllc) {
+        const luna_fv1_options o = options(gas, order, limiter, riemann);
+        check(luna_fv1_create_moving(&o, &line, &fv_), "hydro::FiniteVolume<1> (moving)");
     }
     Line(const Line&) = delete;
     Line& operator=(const Line&) = delete;
     ~Line() { luna_fv1_destroy(fv_); }
-    int cells() const { return n_; }
+    int cells() const {
+        int n = 0;
+        check(luna_fv1_cells(fv_, &n, nullptr), "cells");
+        return n;
+    }
+    long long cells_added() const {
+        long long added = 0;
+        check(luna_fv1_cells(fv_, nullptr, &added), "cells_added");
+        return added;
+    }
     std::vector<double> state() const {
-        std::vector<double> q(static_cast<size_t>(n_) * 3);
+        std::vector<double> q(static_cast<size_t>(cells()) * 3);
         check(luna_fv1_get_state(fv_, q.data()), "state");
         return q;
     }
     void set_state(const std::vector<double>& q) { check(luna_fv1_set_state(fv_, q.data()), "set_state"); }
+    std::vector<double> positions() const {
+        std::vector<double> x(static_cast<size_t>(cells()));
+        check(luna_fv1_positions(fv_, x.data()), "positions");
+        return x;
+    }
+    // The line's slots in key order: their keys and cells (Mesh<1>::line()).
+    void line(std::vector<long long>& keys, std::vector<int>& cells_of_slots) const {
+        keys.resize(static_cast<size_t>(cells()));
+        cells_of_slots.resize(keys.size());
+        check(luna_fv1_line(fv_, keys.data(), cells_of_slots.data()), "line");
+    }
+    void set_inflow_spacing(double spacing) { check(luna_fv1_set_inflow_spacing(fv_, spacing), "set_inflow_spacing"); }
     double time_step(double courant, StepRule rule = StepRule::arepo) {
         double dt = 0;
         check(luna_fv1_time_step(fv_, courant, static_cast<int>(rule), &dt), "time_step");
@@ -171,8 +190,17 @@ public:
     }
 
 private:
+    static luna_fv1_options options(const Gas& gas, int order, Limiter limiter, Riemann riemann) {
+        luna_fv1_options o{};
+        o.order = order;
+        o.limiter = static_cast<int>(limiter);
+        o.riemann = static_cast<int>(riemann);
+        o.gamma = gas.gamma;
+        o.smallr = gas.smallr;
+        o.smallc = gas.smallc;
+        return o;
+    }
     luna_fv1* fv_ = nullptr;
-    int n_ = 0;
 };
 
 }  // namespace luna
