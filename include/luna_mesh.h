@@ -38,6 +38,22 @@ int luna_mesh_build(int dim, int cells, const double* positions, const double* s
 void luna_mesh_release(luna_mesh_record* record);
 const char* luna_mesh_last_error(void);
 
+/* The derivative of a mesh's geometry in its generator positions at fixed topology (the same faces and lists), as the
+ * moving step's derivatives read it: the role of voronoi-cpp's GeometryDerivative behind the reference's
+ * hydro::GeometryDerivative (mesh_derivative.hpp:21-46). Each vertex is the meeting point of its cell's bisectors; the
+ * volumes, centroids, area vectors and face centroids are differentiated through the producer's own formulas.
+ * create: from a built record (its positions, lists and faces); the topology is that mesh's. */
+typedef struct luna_mesh_derivative luna_mesh_derivative;
+int luna_mesh_derivative_create(const luna_mesh_record* mesh, luna_mesh_derivative** out);
+void luna_mesh_derivative_destroy(luna_mesh_derivative* derivative);
+/* GeometryDerivative::tangent (mesh_derivative.hpp:26-27): dx (cells x dim) -> dV (cells), dc (cells x dim), dA, dxf
+ * (faces x dim), written. */
+int luna_mesh_derivative_tangent(const luna_mesh_derivative* derivative, const double* dx, double* dV, double* dc, double* dA,
+                                 double* dxf);
+/* GeometryDerivative::adjoint (:28-29): from the bars of V, c (cells) and A, x_f (faces), adds to xbar (cells x dim). */
+int luna_mesh_derivative_adjoint(const luna_mesh_derivative* derivative, const double* Vbar, const double* cbar, const double* Abar,
+                                 const double* xfbar, double* xbar);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -40,9 +41,19 @@ struct FaceOut {
     double fc[3];     // face centroid relative to this cell's generator
 };
 
+// The topology of one face of a cell's polyhedron (polygon) for the geometry's derivative: the neighbour behind it, the
+// sign that makes its area positive, and its vertices in cycle order, each as the other generators whose bisectors meet
+// there (3D: va and vb, two; 2D: va, one per end, the neighbour of the adjacent edge).
+struct FaceTopo {
+    int j, sigma;
+    std::vector<int> va, vb;
+};
+
 struct CellOut {
     double volume = 0, centroid[3] = {0, 0, 0};
     std::vector<FaceOut> faces;
+    bool want_topo = false;
+    std::vector<FaceTopo> topo;
 };
 
 double wrap(double d, double side) { return d - side * std::round(d / side); }
@@ -301,8 +312,10 @@ struct Poly2 {
 bool finish3(Poly3& P, const std::vector<Candidate>& cands, CellOut& cell, std::string& why) {
     (void)cands;
     cell.faces.clear();
+    cell.topo.clear();
     double volume = 0, moment[3] = {0, 0, 0};
     static thread_local std::vector<int> seen;
+    static thread_local std::vector<int> other_a, other_b;
     seen.assign(P.np, 0);
     for (int s = 0; s < P.nt; ++s)
         for (int e = 0; e < 3; ++e) {
@@ -316,10 +329,16 @@ bool finish3(Poly3& P, const std::vector<Candidate>& cands, CellOut& cell, std::
             // Walk the face of plane k: from triangle (k, x, y) on to the triangle holding the edge k -> y.
             double poly[MAXT][3];
             int count = 0, cur = s;
+            other_a.clear();
+            other_b.clear();
             do {
                 int r = 0;
                 while (P.t[cur][r] != k) ++r;
                 const int y = P.t[cur][(r + 2) % 3];
+                if (cell.want_topo) {
+                    other_a.push_back(P.pj[P.t[cur][(r + 1) % 3]]);
+                    other_b.push_back(P.pj[y]);
+                }
                 std::memcpy(poly[count++], P.v[cur], sizeof poly[0]);
                 cur = P.edge[k][y];
                 if (count >= MAXT) {
@@ -342,11 +361,13 @@ bool finish3(Poly3& P, const std::vector<Candidate>& cands, CellOut& cell, std::
                 area += tri;
                 for (int x = 0; x < 3; ++x) c[x] += tri * (poly[0][x] + poly[q][x] + poly[q + 1][x]) * (1.0 / 3.0);
             }
+            const int sigma = area < 0 ? -1 : 1;
             if (area < 0) {
                 area = -area;
                 for (double& x : c) x = -x;
             }
             if (!(area > 0)) continue;  // a face of no extent: no face
+            if (cell.want_topo) cell.topo.push_back(FaceTopo{P.pj[k], sigma, other_a, other_b});
             FaceOut f;
             f.j = P.pj[k];
             f.area = area;
@@ -363,6 +384,7 @@ bool finish3(Poly3& P, const std::vector<Candidate>& cands, CellOut& cell, std::
 
 bool finish2(Poly2& P, CellOut& cell, std::string& why) {
     cell.faces.clear();
+    cell.topo.clear();
     double volume = 0, moment[2] = {0, 0};
     for (int i = 0; i < P.m; ++i) {
         const int k = P.label[i];
@@ -378,6 +400,10 @@ bool finish2(Poly2& P, CellOut& cell, std::string& why) {
         // distance of the edge's line from the generator: the bisector at |d| / 2
         const double h = std::fabs(P.v[i][0] * ey - P.v[i][1] * ex) / length;
         const double tri = 0.5 * length * h;
+        if (cell.want_topo) {
+            const double cross = P.v[i][0] * P.v[u][1] - P.v[i][1] * P.v[u][0];
+            cell.topo.push_back(FaceTopo{P.pj[k], cross < 0 ? -1 : 1, {P.pj[P.label[(i + P.m - 1) % P.m]], P.pj[P.label[u]]}, {}});
+        }
         FaceOut f;
         f.j = P.pj[k];
         f.area = length;
@@ -393,6 +419,76 @@ bool finish2(Poly2& P, CellOut& cell, std::string& why) {
     cell.centroid[0] = moment[0] / volume;
     cell.centroid[1] = moment[1] / volume;
     cell.centroid[2] = 0;
+    return true;
+}
+
+// Every cell's polyhedron (polygon) from the snapped positions x; with `topo`, also each face's vertex topology. Sets
+// g_error and returns false when a cell cannot be built.
+bool build_cells(int D, int N, const double* xp, const double* side, bool topo, std::vector<CellOut>& cells) {
+    Bins bins;
+    bins.build(D, N, xp, side);
+    for (CellOut& c : cells) c.want_topo = topo;
+    std::atomic<int> failed{-1};
+    std::string why;
+    const double quarter = 0.25 * *std::min_element(side, side + D);
+#pragma omp parallel
+    {
+        std::vector<Candidate> cands;
+        Poly3* p3 = D == 3 ? new Poly3 : nullptr;
+        Poly2* p2 = D == 2 ? new Poly2 : nullptr;
+        std::string local;
+#pragma omp for schedule(dynamic, 256)
+        for (int i = 0; i < N; ++i) {
+            if (failed.load() >= 0) continue;
+            bool ok = false;
+            for (int r = 2; r <= 64 && !ok; r *= 2) {
+                const double complete = gather(bins, D, xp, side, i, r, cands);
+                const double H = 0.5 * *std::min_element(side, side + D);
+                if (D == 3)
+                    p3->box(H);
+                else
+                    p2->box(H);
+                double r2 = D == 3 ? p3->rmax2() : p2->rmax2();
+                bool overflow = false;
+                size_t c = 0;
+                for (; c < cands.size(); ++c) {
+                    if (cands[c].d2 > 4 * r2) break;
+                    const int res = D == 3 ? p3->clip(cands[c].d, cands[c].d2, cands[c].j) : p2->clip(cands[c].d, cands[c].d2, cands[c].j);
+                    if (res < 0) {
+                        overflow = true;
+                        break;
+                    }
+                    if (res > 0) r2 = D == 3 ? p3->rmax2() : p2->rmax2();
+                }
+                if (overflow) {
+                    local = "cell " + std::to_string(i) + ": clipping overflow (degenerate or far too many neighbours)";
+                    break;
+                }
+                if (4 * r2 > complete * complete) continue;  // a generator beyond the searched bins could still cut
+                if (std::sqrt(r2) > quarter) {
+                    local = "cell " + std::to_string(i) + ": larger than a quarter of the box (nearest images only)";
+                    break;
+                }
+                ok = D == 3 ? finish3(*p3, cands, cells[i], local) : finish2(*p2, cells[i], local);
+                if (!ok) local = "cell " + std::to_string(i) + ": " + local;
+                break;
+            }
+            if (!ok) {
+                if (local.empty()) local = "cell " + std::to_string(i) + ": neighbour search did not converge";
+                int expected = -1;
+                if (failed.compare_exchange_strong(expected, i)) {
+#pragma omp critical
+                    why = local;
+                }
+            }
+        }
+        delete p3;
+        delete p2;
+    }
+    if (failed.load() >= 0) {
+        g_error = "luna_mesh_build: " + why;
+        return false;
+    }
     return true;
 }
 
@@ -456,70 +552,8 @@ extern "C" int luna_mesh_build(int D, int N, const double* positions, const doub
             if (k < 0) k += period[a];
             x[static_cast<size_t>(i) * D + a] = static_cast<double>(k) * step;
         }
-    Bins bins;
-    bins.build(D, N, x.data(), side);
     std::vector<CellOut> cells(N);
-    std::atomic<int> failed{-1};
-    std::string why;
-    const double quarter = 0.25 * *std::min_element(side, side + D);
-#pragma omp parallel
-    {
-        std::vector<Candidate> cands;
-        Poly3* p3 = D == 3 ? new Poly3 : nullptr;
-        Poly2* p2 = D == 2 ? new Poly2 : nullptr;
-        std::string local;
-#pragma omp for schedule(dynamic, 256)
-        for (int i = 0; i < N; ++i) {
-            if (failed.load() >= 0) continue;
-            bool ok = false;
-            for (int r = 2; r <= 64 && !ok; r *= 2) {
-                const double complete = gather(bins, D, x.data(), side, i, r, cands);
-                const double H = 0.5 * *std::min_element(side, side + D);
-                if (D == 3)
-                    p3->box(H);
-                else
-                    p2->box(H);
-                double r2 = D == 3 ? p3->rmax2() : p2->rmax2();
-                bool overflow = false;
-                size_t c = 0;
-                for (; c < cands.size(); ++c) {
-                    if (cands[c].d2 > 4 * r2) break;
-                    const int res = D == 3 ? p3->clip(cands[c].d, cands[c].d2, cands[c].j) : p2->clip(cands[c].d, cands[c].d2, cands[c].j);
-                    if (res < 0) {
-                        overflow = true;
-                        break;
-                    }
-                    if (res > 0) r2 = D == 3 ? p3->rmax2() : p2->rmax2();
-                }
-                if (overflow) {
-                    local = "cell " + std::to_string(i) + ": clipping overflow (degenerate or far too many neighbours)";
-                    break;
-                }
-                if (4 * r2 > complete * complete) continue;  // a generator beyond the searched bins could still cut
-                if (std::sqrt(r2) > quarter) {
-                    local = "cell " + std::to_string(i) + ": larger than a quarter of the box (nearest images only)";
-                    break;
-                }
-                ok = D == 3 ? finish3(*p3, cands, cells[i], local) : finish2(*p2, cells[i], local);
-                if (!ok) local = "cell " + std::to_string(i) + ": " + local;
-                break;
-            }
-            if (!ok) {
-                if (local.empty()) local = "cell " + std::to_string(i) + ": neighbour search did not converge";
-                int expected = -1;
-                if (failed.compare_exchange_strong(expected, i)) {
-#pragma omp critical
-                    why = local;
-                }
-            }
-        }
-        delete p3;
-        delete p2;
-    }
-    if (failed.load() >= 0) {
-        g_error = "luna_mesh_build: " + why;
-        return 2;
-    }
+    if (!build_cells(D, N, x.data(), side, false, cells)) return 2;
     // Faces: a pair is kept when both cells see each other; numbered by owner (lower index), then neighbour.
     std::vector<int> owned(N + 1, 0);
     auto sees = [&](int j, int i) {
@@ -599,5 +633,344 @@ extern "C" int luna_mesh_build(int D, int N, const double* positions, const doub
     }
     out->step = step;
     out->build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// The geometry's derivative in the generator positions at fixed topology (the role of voronoi-cpp's GeometryDerivative,
+// mesh_derivative.hpp:21-46 of the reference): the derivative of the same formulas as above with every vertex the meeting
+// point of its bisectors. A 3D vertex v of cell i (relative to x_i) on the planes of neighbours a, b, c solves
+// d_a . v = |d_a|^2 / 2 (d_a = x_a - x_i, nearest image) for all three, so d_a . dv = dd_a . (d_a - v); a 2D vertex the
+// same with two. Faces: tri_q = (w_q - w_0) x (w_q+1 - w_0) . n / 2 (n = d / |d|), |A| = sigma sum tri_q,
+// x_f = sigma sum tri_q (w_0 + w_q + w_q+1) / 3 / |A|, A = |A| n; 2D: |A| = |w_1 - w_0|, x_f the midpoint. Cells:
+// V = sum |A| |d| / 6 (2D: sum |w_0 x w_1| / 2), c = sum 3/4 (2D: 2/3) pyramid x_f / V. The face rows are the owner's.
+
+namespace {
+
+using V3 = std::array<double, 3>;
+inline V3 add(V3 a, V3 b) { return {a[0] + b[0], a[1] + b[1], a[2] + b[2]}; }
+inline V3 sub(V3 a, V3 b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+inline V3 mul(V3 a, double s) { return {a[0] * s, a[1] * s, a[2] * s}; }
+inline double dot3(V3 a, V3 b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline V3 cross(V3 a, V3 b) { return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]}; }
+inline double cross2(V3 a, V3 b) { return a[0] * b[1] - a[1] * b[0]; }
+
+// Solves M y = r for the D x D matrix of rows m[0..D-1] (Gaussian elimination with partial pivoting); transpose: M^T y = r.
+void solve(int D, const V3* m, const double* r, double* y, bool transpose) {
+    double a[3][4];
+    for (int p = 0; p < D; ++p) {
+        for (int q = 0; q < D; ++q) a[p][q] = transpose ? m[q][p] : m[p][q];
+        a[p][3] = r[p];
+    }
+    for (int c = 0; c < D; ++c) {
+        int piv = c;
+        for (int p = c + 1; p < D; ++p)
+            if (std::fabs(a[p][c]) > std::fabs(a[piv][c])) piv = p;
+        if (piv != c)
+            for (int q = 0; q < 4; ++q) std::swap(a[c][q], a[piv][q]);
+        for (int p = c + 1; p < D; ++p) {
+            const double f = a[p][c] / a[c][c];
+            for (int q = c; q < 4; ++q) a[p][q] -= f * a[c][q];
+        }
+    }
+    for (int c = D - 1; c >= 0; --c) {
+        double s = a[c][3];
+        for (int q = c + 1; q < D; ++q) s -= a[c][q] * y[q];
+        y[c] = s / a[c][c];
+    }
+}
+
+}  // namespace
+
+struct luna_mesh_derivative {
+    int D = 3, N = 0;
+    double side[3] = {0, 0, 0};
+    std::vector<double> x;                 // snapped positions
+    std::vector<std::vector<FaceTopo>> topo;
+    std::vector<std::vector<int>> face_id;  // per cell and topo face: the mesh's face row it gives (owner's kept faces), or -1
+
+    V3 sep(int i, int j) const {
+        V3 d{0, 0, 0};
+        for (int a = 0; a < D; ++a) d[a] = wrap(x[static_cast<size_t>(j) * D + a] - x[static_cast<size_t>(i) * D + a], side[a]);
+        return d;
+    }
+    V3 row(const double* p, int i) const {
+        V3 r{0, 0, 0};
+        for (int a = 0; a < D; ++a) r[a] = p[static_cast<size_t>(i) * D + a];
+        return r;
+    }
+    // The vertex of cell i on the bisectors of js[0..D-1] (relative to x_i), with the rows it solved.
+    V3 vertex(int i, const int* js, V3* m) const {
+        double r[3];
+        for (int p = 0; p < D; ++p) {
+            m[p] = sep(i, js[p]);
+            r[p] = 0.5 * dot3(m[p], m[p]);
+        }
+        double y[3] = {0, 0, 0};
+        solve(D, m, r, y, false);
+        return {y[0], y[1], D == 3 ? y[2] : 0.0};
+    }
+    // dv from dx (tangent).
+    V3 vertex_tangent(int i, const int* js, const V3* m, V3 v, const double* dx) const {
+        double s[3];
+        for (int p = 0; p < D; ++p) s[p] = dot3(sub(row(dx, js[p]), row(dx, i)), sub(m[p], v));
+        double y[3] = {0, 0, 0};
+        solve(D, m, s, y, false);
+        return {y[0], y[1], D == 3 ? y[2] : 0.0};
+    }
+    // vbar -> the bars of x_i and the js (adjoint), into acc.
+    void vertex_adjoint(int i, const int* js, const V3* m, V3 v, V3 vbar, std::vector<std::pair<int, V3>>& acc) const {
+        const double r[3] = {vbar[0], vbar[1], vbar[2]};
+        double sbar[3] = {0, 0, 0};
+        solve(D, m, r, sbar, true);
+        for (int p = 0; p < D; ++p) {
+            const V3 g = mul(sub(m[p], v), sbar[p]);
+            acc.push_back({js[p], g});
+            acc.push_back({i, mul(g, -1.0)});
+        }
+    }
+};
+
+extern "C" int luna_mesh_derivative_create(const luna_mesh_record* mesh, luna_mesh_derivative** out) {
+    if (!mesh || !out) {
+        g_error = "luna_mesh_derivative_create: null argument";
+        return 1;
+    }
+    *out = nullptr;
+    auto* g = new luna_mesh_derivative;
+    g->D = mesh->dim;
+    g->N = mesh->cells;
+    for (int a = 0; a < 3; ++a) g->side[a] = mesh->side[a];
+    g->x.assign(mesh->positions, mesh->positions + static_cast<size_t>(g->N) * g->D);
+    std::vector<CellOut> cells(g->N);
+    if (!build_cells(g->D, g->N, g->x.data(), g->side, true, cells)) {
+        delete g;
+        return 2;
+    }
+    g->topo.resize(g->N);
+    g->face_id.resize(g->N);
+    for (int i = 0; i < g->N; ++i) {
+        g->topo[i] = std::move(cells[i].topo);
+        g->face_id[i].assign(g->topo[i].size(), -1);
+        for (size_t t = 0; t < g->topo[i].size(); ++t) {
+            const int j = g->topo[i][t].j;
+            if (j <= i) continue;
+            for (int k = mesh->start[i]; k < mesh->start[i + 1]; ++k) {
+                const int e = mesh->list[k];
+                if (e >= 0 && mesh->neighbour[e] == j) g->face_id[i][t] = e;
+            }
+        }
+    }
+    *out = g;
+    return 0;
+}
+
+extern "C" void luna_mesh_derivative_destroy(luna_mesh_derivative* g) { delete g; }
+
+extern "C" int luna_mesh_derivative_tangent(const luna_mesh_derivative* g, const double* dx, double* dV, double* dc, double* dA, double* dxf) {
+    if (!g || !dx || !dV || !dc || !dA || !dxf) {
+        g_error = "luna_mesh_derivative_tangent: null argument";
+        return 1;
+    }
+    const int D = g->D;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < g->N; ++i) {
+        double V = 0, dVi = 0;
+        V3 moment{0, 0, 0}, dmoment{0, 0, 0};
+        const auto& faces = g->topo[i];
+        for (size_t t = 0; t < faces.size(); ++t) {
+            const FaceTopo& F = faces[t];
+            const V3 d = g->sep(i, F.j), dd = sub(g->row(dx, F.j), g->row(dx, i));
+            const double dist = std::sqrt(dot3(d, d)), ddist = dot3(d, dd) / dist;
+            const V3 n = mul(d, 1.0 / dist), dn = mul(sub(dd, mul(n, dot3(n, dd))), 1.0 / dist);
+            const int m = static_cast<int>(F.va.size());
+            std::vector<V3> w(m), dw(m);
+            for (int q = 0; q < m; ++q) {
+                const int js[3] = {F.j, F.va[q], D == 3 ? F.vb[q] : 0};
+                V3 rows[3];
+                w[q] = g->vertex(i, js, rows);
+                dw[q] = g->vertex_tangent(i, js, rows, w[q], dx);
+            }
+            double area = 0, darea = 0, pyr, dpyr;
+            V3 fc, dfc;
+            if (D == 3) {
+                V3 cnum{0, 0, 0}, dcnum{0, 0, 0};
+                for (int q = 1; q + 1 < m; ++q) {
+                    const V3 a = sub(w[q], w[0]), b = sub(w[q + 1], w[0]), da = sub(dw[q], dw[0]), db = sub(dw[q + 1], dw[0]);
+                    const V3 cr = cross(a, b), dcr = add(cross(da, b), cross(a, db));
+                    const double tri = 0.5 * dot3(cr, n), dtri = 0.5 * (dot3(dcr, n) + dot3(cr, dn));
+                    const V3 gq = mul(add(add(w[0], w[q]), w[q + 1]), 1.0 / 3.0), dgq = mul(add(add(dw[0], dw[q]), dw[q + 1]), 1.0 / 3.0);
+                    area += tri;
+                    darea += dtri;
+                    cnum = add(cnum, mul(gq, tri));
+                    dcnum = add(dcnum, add(mul(gq, dtri), mul(dgq, tri)));
+                }
+                area *= F.sigma;
+                darea *= F.sigma;
+                cnum = mul(cnum, F.sigma);
+                dcnum = mul(dcnum, F.sigma);
+                fc = mul(cnum, 1.0 / area);
+                dfc = mul(sub(dcnum, mul(fc, darea)), 1.0 / area);
+                pyr = area * dist / 6.0;
+                dpyr = (darea * dist + area * ddist) / 6.0;
+                moment = add(moment, mul(fc, 0.75 * pyr));
+                dmoment = add(dmoment, mul(add(mul(dfc, pyr), mul(fc, dpyr)), 0.75));
+            } else {
+                const V3 e = sub(w[1], w[0]), de = sub(dw[1], dw[0]);
+                area = std::sqrt(dot3(e, e));
+                darea = dot3(e, de) / area;
+                fc = mul(add(w[0], w[1]), 0.5);
+                dfc = mul(add(dw[0], dw[1]), 0.5);
+                pyr = 0.5 * F.sigma * cross2(w[0], w[1]);
+                dpyr = 0.5 * F.sigma * (cross2(dw[0], w[1]) + cross2(w[0], dw[1]));
+                moment = add(moment, mul(fc, (2.0 / 3.0) * pyr));
+                dmoment = add(dmoment, mul(add(mul(dfc, pyr), mul(fc, dpyr)), 2.0 / 3.0));
+            }
+            V += pyr;
+            dVi += dpyr;
+            const int f = g->face_id[i][t];
+            if (f >= 0) {
+                const V3 dAv = add(mul(n, darea), mul(dn, area));
+                for (int a = 0; a < D; ++a) {
+                    dA[static_cast<size_t>(f) * D + a] = dAv[a];
+                    dxf[static_cast<size_t>(f) * D + a] = dfc[a];
+                }
+            }
+        }
+        dV[i] = dVi;
+        const V3 c = mul(moment, 1.0 / V);
+        const V3 dci = mul(sub(dmoment, mul(c, dVi)), 1.0 / V);
+        for (int a = 0; a < D; ++a) dc[static_cast<size_t>(i) * D + a] = dci[a];
+    }
+    return 0;
+}
+
+extern "C" int luna_mesh_derivative_adjoint(const luna_mesh_derivative* g, const double* Vbar, const double* cbar, const double* Abar,
+                                            const double* xfbar, double* xbar) {
+    if (!g || !Vbar || !cbar || !Abar || !xfbar || !xbar) {
+        g_error = "luna_mesh_derivative_adjoint: null argument";
+        return 1;
+    }
+    const int D = g->D, N = g->N;
+    std::vector<std::vector<std::pair<int, V3>>> acc(N);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < N; ++i) {
+        auto& out = acc[i];
+        const auto& faces = g->topo[i];
+        // Forward: V, moment (the cell's own faces), then c = moment / V.
+        double V = 0;
+        V3 moment{0, 0, 0};
+        struct Face {
+            V3 d, n, fc;
+            double dist, area, pyr;
+            std::vector<V3> w, rows;
+        };
+        std::vector<Face> fs(faces.size());
+        for (size_t t = 0; t < faces.size(); ++t) {
+            const FaceTopo& F = faces[t];
+            Face& s = fs[t];
+            s.d = g->sep(i, F.j);
+            s.dist = std::sqrt(dot3(s.d, s.d));
+            s.n = mul(s.d, 1.0 / s.dist);
+            const int m = static_cast<int>(F.va.size());
+            s.w.resize(m);
+            s.rows.resize(static_cast<size_t>(m) * 3);
+            for (int q = 0; q < m; ++q) {
+                const int js[3] = {F.j, F.va[q], D == 3 ? F.vb[q] : 0};
+                s.w[q] = g->vertex(i, js, &s.rows[static_cast<size_t>(q) * 3]);
+            }
+            if (D == 3) {
+                double area = 0;
+                V3 cnum{0, 0, 0};
+                for (int q = 1; q + 1 < m; ++q) {
+                    const double tri = 0.5 * dot3(cross(sub(s.w[q], s.w[0]), sub(s.w[q + 1], s.w[0])), s.n);
+                    area += tri;
+                    cnum = add(cnum, mul(add(add(s.w[0], s.w[q]), s.w[q + 1]), tri / 3.0));
+                }
+                s.area = area * F.sigma;
+                s.fc = mul(cnum, F.sigma / s.area);
+                s.pyr = s.area * s.dist / 6.0;
+                moment = add(moment, mul(s.fc, 0.75 * s.pyr));
+            } else {
+                const V3 e = sub(s.w[1], s.w[0]);
+                s.area = std::sqrt(dot3(e, e));
+                s.fc = mul(add(s.w[0], s.w[1]), 0.5);
+                s.pyr = 0.5 * F.sigma * cross2(s.w[0], s.w[1]);
+                moment = add(moment, mul(s.fc, (2.0 / 3.0) * s.pyr));
+            }
+            V += s.pyr;
+        }
+        const V3 c = mul(moment, 1.0 / V);
+        // Reverse: c = moment / V.
+        const V3 cb = g->row(cbar, i);
+        const V3 momentbar = mul(cb, 1.0 / V);
+        const double Vb = Vbar[i] - dot3(cb, c) / V;
+        const double share = D == 3 ? 0.75 : 2.0 / 3.0;
+        for (size_t t = 0; t < faces.size(); ++t) {
+            const FaceTopo& F = faces[t];
+            const Face& s = fs[t];
+            const int m = static_cast<int>(s.w.size());
+            double pyrbar = Vb + share * dot3(momentbar, s.fc);
+            V3 fcbar = mul(momentbar, share * s.pyr);
+            double areabar = 0, distbar = 0;
+            V3 nbar{0, 0, 0};
+            const int f = g->face_id[i][t];
+            if (f >= 0) {
+                const V3 Ab = g->row(Abar, f);
+                areabar += dot3(Ab, s.n);
+                nbar = add(nbar, mul(Ab, s.area));
+                fcbar = add(fcbar, g->row(xfbar, f));
+            }
+            std::vector<V3> wbar(m, V3{0, 0, 0});
+            if (D == 3) {
+                areabar += pyrbar * s.dist / 6.0;
+                distbar += pyrbar * s.area / 6.0;
+                // fc = cnum / area
+                const V3 cnumbar = mul(fcbar, 1.0 / s.area);
+                areabar -= dot3(fcbar, s.fc) / s.area;
+                for (int q = 1; q + 1 < m; ++q) {
+                    const V3 a = sub(s.w[q], s.w[0]), b = sub(s.w[q + 1], s.w[0]);
+                    const V3 cr = cross(a, b);
+                    const double tri = 0.5 * dot3(cr, s.n);
+                    const V3 gq = mul(add(add(s.w[0], s.w[q]), s.w[q + 1]), 1.0 / 3.0);
+                    // area = sigma sum tri, cnum = sigma sum tri gq
+                    const double tribar = F.sigma * (areabar + dot3(cnumbar, gq));
+                    const V3 gbar = mul(cnumbar, F.sigma * tri / 3.0);
+                    wbar[0] = add(wbar[0], gbar);
+                    wbar[q] = add(wbar[q], gbar);
+                    wbar[q + 1] = add(wbar[q + 1], gbar);
+                    // tri = cr . n / 2
+                    const V3 crbar = mul(s.n, 0.5 * tribar);
+                    nbar = add(nbar, mul(cr, 0.5 * tribar));
+                    const V3 abar = cross(b, crbar), bbar = cross(crbar, a);
+                    wbar[q] = add(wbar[q], abar);
+                    wbar[q + 1] = add(wbar[q + 1], bbar);
+                    wbar[0] = sub(wbar[0], add(abar, bbar));
+                }
+            } else {
+                // pyr = sigma w0 x w1 / 2, fc = (w0 + w1) / 2, area = |w1 - w0|
+                const double pb = 0.5 * F.sigma * pyrbar;
+                wbar[0] = add(wbar[0], V3{pb * s.w[1][1], -pb * s.w[1][0], 0});
+                wbar[1] = add(wbar[1], V3{-pb * s.w[0][1], pb * s.w[0][0], 0});
+                wbar[0] = add(wbar[0], mul(fcbar, 0.5));
+                wbar[1] = add(wbar[1], mul(fcbar, 0.5));
+                const V3 e = sub(s.w[1], s.w[0]);
+                const V3 eb = mul(e, areabar / s.area);
+                wbar[1] = add(wbar[1], eb);
+                wbar[0] = sub(wbar[0], eb);
+            }
+            // n = d / |d|, |d|
+            V3 dbar = add(mul(sub(nbar, mul(s.n, dot3(s.n, nbar))), 1.0 / s.dist), mul(s.n, distbar));
+            out.push_back({F.j, dbar});
+            out.push_back({i, mul(dbar, -1.0)});
+            for (int q = 0; q < m; ++q) {
+                const int js[3] = {F.j, F.va[q], D == 3 ? F.vb[q] : 0};
+                g->vertex_adjoint(i, js, &s.rows[static_cast<size_t>(q) * 3], s.w[q], wbar[q], out);
+            }
+        }
+    }
+    for (int i = 0; i < N; ++i)  // in cell order: the same sums every run
+        for (const auto& p : acc[i])
+            for (int a = 0; a < D; ++a) xbar[static_cast<size_t>(p.first) * D + a] += p.second[a];
     return 0;
 }

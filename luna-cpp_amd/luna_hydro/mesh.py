@@ -32,6 +32,10 @@ def lib():
         L.luna_mesh_build.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.POINTER(Record)]
         L.luna_mesh_release.argtypes = [ctypes.POINTER(Record)]
         L.luna_mesh_last_error.restype = ctypes.c_char_p
+        L.luna_mesh_derivative_create.argtypes = [ctypes.POINTER(Record), ctypes.POINTER(ctypes.c_void_p)]
+        L.luna_mesh_derivative_destroy.argtypes = [ctypes.c_void_p]
+        L.luna_mesh_derivative_tangent.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.luna_mesh_derivative_adjoint.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -72,3 +76,59 @@ def build(points, side=1.0):
     finally:
         lib().luna_mesh_release(ctypes.byref(rec))
     return mesh
+
+
+def record_of(mesh):
+    """A luna_mesh_record over the arrays of a mesh dict (kept alive by the returned holder)."""
+    m = dict(mesh)
+    for k in ("owner", "neighbour", "start", "list"):
+        m[k] = np.ascontiguousarray(m[k], dtype=np.int32)
+    for k in ("area", "face_centroid", "centroid", "volume", "positions"):
+        m[k] = np.ascontiguousarray(m[k], dtype=np.float64)
+    rec = Record()
+    rec.dim, rec.cells, rec.faces = m["dim"], m["cells"], m["faces"]
+    for k in ("owner", "neighbour", "start", "list"):
+        setattr(rec, k, m[k].ctypes.data_as(_ip))
+    for k in ("area", "face_centroid", "centroid", "volume", "positions"):
+        setattr(rec, k, m[k].ctypes.data_as(_dp))
+    for a in range(3):
+        rec.side[a] = m["side"][a] if a < len(m["side"]) else 0.0
+        rec.period[a] = int(m["period"][a]) if a < len(m["period"]) else 0
+    rec.step = m["step"]
+    return rec, m
+
+
+class GeometryDerivative:
+    """The derivative of a mesh's geometry (V, c per cell; A, x_f per face) in the generator positions at fixed topology
+    (luna_mesh_derivative_*; the role of the reference's GeometryDerivative, mesh_derivative.hpp:21-46)."""
+
+    def __init__(self, mesh):
+        self.D, self.N, self.F = mesh["dim"], mesh["cells"], mesh["faces"]
+        rec, self._keep = record_of(mesh)
+        h = ctypes.c_void_p()
+        if lib().luna_mesh_derivative_create(ctypes.byref(rec), ctypes.byref(h)) != 0:
+            raise RuntimeError(lib().luna_mesh_last_error().decode())
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().luna_mesh_derivative_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def tangent(self, dx):
+        """dx (N x D) -> dV (N), dc (N x D), dA (F x D), dxf (F x D)."""
+        dx = np.ascontiguousarray(dx, dtype=np.float64).reshape(self.N, self.D)
+        dV, dc = np.zeros(self.N), np.zeros((self.N, self.D))
+        dA, dxf = np.zeros((self.F, self.D)), np.zeros((self.F, self.D))
+        lib().luna_mesh_derivative_tangent(self._h, *(a.ctypes.data_as(_dp) for a in (dx, dV, dc, dA, dxf)))
+        return dV, dc, dA, dxf
+
+    def adjoint(self, Vbar, cbar, Abar, xfbar, xbar=None):
+        """The transpose: adds to xbar (N x D, zeros if None) and returns it."""
+        args = [np.ascontiguousarray(a, dtype=np.float64) for a in (Vbar, cbar, Abar, xfbar)]
+        xbar = np.zeros((self.N, self.D)) if xbar is None else np.ascontiguousarray(xbar, dtype=np.float64)
+        lib().luna_mesh_derivative_adjoint(self._h, *(a.ctypes.data_as(_dp) for a in args), xbar.ctypes.data_as(_dp))
+        return xbar

@@ -115,3 +115,28 @@ def test_pyramid_formulas(D, n):
     # face centroids lie on the bisector plane: (fc - d/2) . d = 0 (relative to the owner's generator)
     off = np.einsum("ij,ij->i", m["face_centroid"] - 0.5 * d, d) / dist
     assert np.abs(off).max() < 1e-13
+
+
+@pytest.mark.parametrize("D,n", [(2, 12), (3, 6)])
+def test_geometry_derivative_against_the_producer(D, n):
+    """The geometry's derivative at fixed topology (luna_mesh_derivative_*): its tangent equals central differences of the
+    producer itself (positions moved by whole grid steps, so the snapping adds nothing; the topology unchanged), and its
+    adjoint is the tangent's transpose (dot test)."""
+    x0 = problems.jittered_lattice(n, D, 0.25, 3)
+    m0 = mesh.build(x0, 1.0)
+    x, step = m0["positions"], m0["step"]
+    g = mesh.GeometryDerivative(m0)
+    rng = np.random.default_rng(5)
+    dx = rng.integers(-300, 301, size=x.shape).astype(np.float64) * step
+    mp, mm = mesh.build(x + dx, 1.0), mesh.build(x - dx, 1.0)
+    for key in ("owner", "neighbour", "start", "list"):
+        assert np.array_equal(mp[key], m0[key]) and np.array_equal(mm[key], m0[key])
+    dV, dc, dA, dxf = g.tangent(dx)
+    for got, key in ((dV, "volume"), (dc, "centroid"), (dA, "area"), (dxf, "face_centroid")):
+        fd = (mp[key] - mm[key]) / 2
+        assert np.max(np.abs(got - fd)) < 1e-6 * np.max(np.abs(fd)), key
+    bars = [rng.normal(size=a.shape) for a in (dV, dc, dA, dxf)]
+    xbar = g.adjoint(*bars)
+    lhs = sum(np.sum(b * t) for b, t in zip(bars, (dV, dc, dA, dxf)))
+    assert abs(lhs - np.sum(xbar * dx)) < 1e-12 * abs(lhs)
+    g.close()
