@@ -709,7 +709,7 @@ void orc_moving_begin(orc_fv* fv, const orc_geometry* geo, double dt) {
             const long long k0 = orc_snap(fv->x[at], geo->step, geo->period[a]);
             const long long k1 = orc_snap(fv->x[at] + dt * fv->w[at], geo->step, geo->period[a]);
             fv->x[at] = (double)k0 * geo->step;
-            fv->steps[at] = orc_delta(k0, k1, geo->period[a]);
+            if (!fv->hold_steps) fv->steps[at] = orc_delta(k0, k1, geo->period[a]);
         }
     fv->fallbacks[1] = 0;
     apply(fv, geo, 0.0, 0.5 * dt, 0, geo->step / dt);
@@ -2184,8 +2184,15 @@ typedef struct {
 /* The cell's fit and limiter as sweep_cell() computes them (the same bits for g), with what the derivatives need: the
  * unlimited gradient g0 and, with dW (N x V tangents of the prims), its tangent dg0; M's Cholesky factor and first row M0
  * (M0[p] = sum_k |A_k| X_k[p]); the limiter's pattern. */
+static void fit_cell_x(const orc_fv* fv, const orc_geometry* geo, int i, const double* dW, double* g, double* g0, double* dg0,
+                       double L[MAXD + 1][MAXD + 1], double* M0, limit_pattern* pat, double* intercept);
 static void fit_cell(const orc_fv* fv, const orc_geometry* geo, int i, const double* dW, double* g, double* g0, double* dg0,
                      double L[MAXD + 1][MAXD + 1], double* M0, limit_pattern* pat) {
+    fit_cell_x(fv, geo, i, dW, g, g0, dg0, L, M0, pat, NULL);
+}
+/* fit_cell() with the fit's intercepts (intercept: V, or NULL). */
+static void fit_cell_x(const orc_fv* fv, const orc_geometry* geo, int i, const double* dW, double* g, double* g0, double* dg0,
+                       double L[MAXD + 1][MAXD + 1], double* M0, limit_pattern* pat, double* intercept) {
     const int D = fv->D, V = D + 2, U = D + 1, PW = V + 2;
     const double* Wi = fv->prims + (size_t)i * PW;
     double surface = 0, smallest = INFINITY;
@@ -2252,6 +2259,8 @@ static void fit_cell(const orc_fv* fv, const orc_geometry* geo, int i, const dou
     solve_factored(D, L, b, xs);
     for (int v = 0; v < V; ++v)
         for (int a = 0; a < D; ++a) g0[v * D + a] = g[v * D + a] = xs[v][1 + a];
+    if (intercept)
+        for (int v = 0; v < V; ++v) intercept[v] = xs[v][0];
     if (dW) {
         solve_factored(D, L, db, xs);
         for (int v = 0; v < V; ++v)
@@ -2546,6 +2555,437 @@ int orc_step_adjoint(orc_fv* fv, const orc_geometry* geo, double dt, double* qba
     free(bbar);
     free(boundbar);
     free(extreme);
+    return 0;
+}
+
+/* ---- The moving step's reverse, order 1, fixed topology (finite_volume.hpp:2233-3155) ---- */
+
+/* euler.hpp:848-851 turn_bar(): the n bar of <a, Omega b>, Omega = dn n^T - n dn^T: out += sign (a (n.b) - b (n.a)). */
+static void turn_bar_add(int D, const double* n, const double* a, const double* b, double sign, double* out) {
+    const double nb = dot(D, n, b), na = dot(D, n, a);
+    for (int k = 0; k < D; ++k) out[k] += sign * (a[k] * nb - b[k] * na);
+}
+
+/* euler.hpp:893-916 MovingFace::adjoint(): G = |A| F(L, R, A / |A|, w); adds J^T Gbar to Lbar, Rbar, Abar, wbar. The normal's
+ * and the face velocity's parts come from the flux's rotation and Galilean invariance (euler.hpp:835-841): only the
+ * Riemann solver's (L, R) adjoint is needed. */
+static void moving_face_adjoint(int kind, int D, const orc_gas* gas, const double* L, const double* R, const double* A, const double* w,
+                                const double* Gbar, double* Lbar, double* Rbar, double* Abar, double* wbar) {
+    const int V = D + 2;
+    const double s = sqrt(dot(D, A, A));
+    double n[MAXD], F[MAXV], Fr[MAXV], X[MAXV], mw[MAXD];
+    for (int k = 0; k < D; ++k) {
+        n[k] = A[k] * (1.0 / s);
+        mw[k] = -w[k];
+    }
+    riemann(kind, D, gas, L, R, n, w, F);
+    memcpy(Fr, F, sizeof(double) * V);
+    to_lab(D, Fr, mw); /* the flux in the face's frame */
+    double sbar = 0;
+    for (int v = 0; v < V; ++v) {
+        sbar += Gbar[v] * F[v];
+        X[v] = s * Gbar[v];
+    }
+    double nbar[MAXD] = {0, 0, 0}, shiftbar[MAXD];
+    turn_bar_add(D, n, X + 1, F + 1, 1.0, nbar);
+    for (int k = 0; k < D; ++k) shiftbar[k] = X[1 + k] * Fr[0] + X[D + 1] * (Fr[1 + k] + w[k] * Fr[0]);
+    double Lr[MAXV] = {0, 0, 0, 0, 0}, Rr[MAXV] = {0, 0, 0, 0, 0};
+    riemann_adjoint(kind, D, gas, L, R, n, w, X, Lr, Rr);
+    for (int v = 0; v < V; ++v) {
+        Lbar[v] += Lr[v];
+        Rbar[v] += Rr[v];
+    }
+    for (int k = 0; k < D; ++k) shiftbar[k] -= Lr[1 + k] + Rr[1 + k];
+    turn_bar_add(D, n, Lr + 1, L + 1, -1.0, nbar);
+    turn_bar_add(D, n, Rr + 1, R + 1, -1.0, nbar);
+    for (int k = 0; k < D; ++k) wbar[k] += shiftbar[k];
+    turn_bar_add(D, n, shiftbar, w, -1.0, nbar);
+    const double nn = dot(D, n, nbar);
+    for (int k = 0; k < D; ++k) Abar[k] += (nbar[k] - n[k] * nn) * (1.0 / s) + n[k] * sbar;
+}
+
+/* finite_volume.hpp:2331-2369 FaceVelocity::adjoint(): of face_velocity(d, x_f, w_L, w_R) (d owner to neighbour, x_f relative to
+ * the owner), at the forward's branch. */
+static void face_velocity_adjoint(int D, const double* d, const double* xf, const double* wL, const double* wR, const double* wfbar,
+                                  double* dbar, double* xfbar, double* wLbar, double* wRbar) {
+    double c[MAXD], u[MAXD], cbar[MAXD] = {0, 0, 0}, ubar[MAXD];
+    const double dist = sqrt(dot(D, d, d));
+    for (int a = 0; a < D; ++a) {
+        c[a] = xf[a] - d[a] * 0.5;
+        u[a] = wL[a] - wR[a];
+    }
+    const double f0 = dot(D, c, u) / dist, limit = 0.9 * dist, c2 = dot(D, c, c), cc = sqrt(c2);
+    const int clipped = c2 > limit * limit;
+    const double sc = clipped ? limit / cc : 1.0, facv = f0 * sc;
+    for (int a = 0; a < D; ++a) {
+        wLbar[a] += wfbar[a] * 0.5;
+        wRbar[a] += wfbar[a] * 0.5;
+        dbar[a] += wfbar[a] * (facv / dist);
+    }
+    const double qb = dot(D, wfbar, d), facvbar = qb / dist;
+    double distbar = -qb * facv / (dist * dist);
+    const double f0bar = facvbar * sc;
+    if (clipped) {
+        const double scbar = facvbar * f0;
+        distbar += scbar * 0.9 / cc;
+        for (int a = 0; a < D; ++a) cbar[a] += c[a] * (-scbar * sc / (cc * cc));
+    }
+    for (int a = 0; a < D; ++a) {
+        cbar[a] += u[a] * (f0bar / dist);
+        ubar[a] = c[a] * (f0bar / dist);
+    }
+    distbar -= f0bar * f0 / dist;
+    for (int a = 0; a < D; ++a) {
+        wLbar[a] += ubar[a];
+        wRbar[a] -= ubar[a];
+        xfbar[a] += cbar[a];
+        dbar[a] += -cbar[a] * 0.5 + d[a] * (distbar / dist);
+    }
+}
+
+/* finite_volume.hpp:2237-2243 face_angle_slope(): d face_angle / d |A|. */
+static double face_angle_slope(int D, double area) {
+    if (D == 3) return area > 0 ? 0.5 / (M_PI * sqrt(area / M_PI)) : 0.0;
+    return 0.5;
+}
+
+/* Bars of the geometry a moving step reads: per cell c (centroid), V, x (through the separations it reads); per face A,
+ * x_f. */
+typedef struct {
+    double *V, *c, *x, *A, *xf;
+} geo_bars;
+
+static void add_sep_bar(int D, geo_bars* b, int from, int to, const double* bar) { /* bar of separation(from, to) = x_to - x_from */
+    for (int a = 0; a < D; ++a) {
+        b->x[(size_t)to * D + a] += bar[a];
+        b->x[(size_t)from * D + a] -= bar[a];
+    }
+}
+
+/* The bar of face_offset(f, i) (x_f relative to c_i; through the owner's separation when i does not own f). */
+static void face_offset_bar(const orc_geometry* g, geo_bars* b, int f, int i, const double* rbar) {
+    const int D = g->D, o = g->owner[f];
+    for (int a = 0; a < D; ++a) {
+        b->xf[(size_t)f * D + a] += rbar[a];
+        b->c[(size_t)i * D + a] -= rbar[a];
+    }
+    if (o != i) add_sep_bar(D, b, i, o, rbar);
+}
+
+/* One moving pass's adjoint (finite_volume.hpp:2876-2983 moving_pass_adjoint / moving_pass_gather): Q -= dt/2 sum_f +-G_f
+ * with G = |A| F(side_o, side_j, A/|A|, face_velocity(d, x_f, w-hat_o, w-hat_j)). First pass (mesh t^n): side W + g.r;
+ * second (mesh t^n+1): half_step(W, g, 2 dt) + g.(r + c^n+1 - c^n + dt w-hat). Adds to Wbar, gbar, whbar, c0bar (per cell) and
+ * to the mesh's bars. */
+static void moving_pass_adjoint(const orc_fv* fv, const orc_geometry* g, int second, double dt, const double* wh, const double* qbar,
+                                double* Wbar, double* gbar, double* whbar, double* c0bar, geo_bars* b) {
+    const int D = fv->D, V = D + 2, PW = V + 2;
+    for (int f = 0; f < g->F; ++f) {
+        const int o = g->owner[f], j = g->neighbour[f];
+        const double* A = g->area + (size_t)f * D;
+        if (sqrt(dot(D, A, A)) == 0) continue;
+        double d[MAXD], wf[MAXD], sides[2][MAXV], r[2][MAXD];
+        int fallback[2];
+        separation(g, o, j, d);
+        face_velocity(g, f, o, j, wh + (size_t)o * D, wh + (size_t)j * D, wf);
+        const int cells[2] = {o, j};
+        for (int s2 = 0; s2 < 2; ++s2) {
+            const int i = cells[s2];
+            const double* W = fv->prims + (size_t)i * PW;
+            const double* gi = fv->grad + (size_t)i * V * D;
+            face_offset(g, f, i, r[s2]);
+            if (second) {
+                for (int a = 0; a < D; ++a)
+                    r[s2][a] = r[s2][a] + g->centroid[(size_t)i * D + a] - fv->old_centroid[(size_t)i * D + a] + wh[(size_t)i * D + a] * (0.5 * (2 * dt));
+                orc_half_step(D, &fv->gas, W, gi, 2 * dt, sides[s2]);
+            } else {
+                for (int v = 0; v < V; ++v) sides[s2][v] = W[v];
+            }
+            for (int v = 0; v < V; ++v) sides[s2][v] += dot(D, gi + v * D, r[s2]);
+            fallback[s2] = !(sides[s2][0] > 0 && sides[s2][D + 1] > 0);
+            if (fallback[s2])
+                for (int v = 0; v < V; ++v) sides[s2][v] = W[v];
+        }
+        double Gbar[MAXV], Lbar[MAXV] = {0, 0, 0, 0, 0}, Rbar[MAXV] = {0, 0, 0, 0, 0}, wfbar[MAXD] = {0, 0, 0};
+        for (int v = 0; v < V; ++v) Gbar[v] = -0.5 * dt * (qbar[(size_t)o * V + v] - qbar[(size_t)j * V + v]);
+        moving_face_adjoint(fv->riemann, D, &fv->gas, sides[0], sides[1], A, wf, Gbar, Lbar, Rbar, b->A + (size_t)f * D, wfbar);
+        double dbar[MAXD] = {0, 0, 0};
+        face_velocity_adjoint(D, d, g->face_centroid + (size_t)f * D, wh + (size_t)o * D, wh + (size_t)j * D, wfbar, dbar,
+                              b->xf + (size_t)f * D, whbar + (size_t)o * D, whbar + (size_t)j * D);
+        add_sep_bar(D, b, o, j, dbar);
+        const double* sidebar[2] = {Lbar, Rbar};
+        for (int s2 = 0; s2 < 2; ++s2) {
+            const int i = cells[s2];
+            const double* W = fv->prims + (size_t)i * PW;
+            const double* gi = fv->grad + (size_t)i * V * D;
+            const double* sb = sidebar[s2];
+            double* Wb = Wbar + (size_t)i * V;
+            double* gb = gbar + (size_t)i * V * D;
+            if (fallback[s2]) {
+                for (int v = 0; v < V; ++v) Wb[v] += sb[v];
+                continue;
+            }
+            if (second) {
+                orc_half_step_adjoint(D, &fv->gas, W, gi, 2 * dt, sb, Wb, gb);
+            } else {
+                for (int v = 0; v < V; ++v) Wb[v] += sb[v];
+            }
+            double rbar[MAXD] = {0, 0, 0};
+            for (int v = 0; v < V; ++v)
+                for (int a = 0; a < D; ++a) {
+                    gb[v * D + a] += sb[v] * r[s2][a];
+                    rbar[a] += sb[v] * gi[v * D + a];
+                }
+            face_offset_bar(g, b, f, i, rbar);
+            if (second)
+                for (int a = 0; a < D; ++a) {
+                    b->c[(size_t)i * D + a] += rbar[a];
+                    c0bar[(size_t)i * D + a] -= rbar[a];
+                    whbar[(size_t)i * D + a] += dt * rbar[a];
+                }
+        }
+    }
+}
+
+/* finite_volume.hpp:2245-2306 Regularised::adjoint() for sweep_cell()'s point velocity w = v + c (fraction c_s / |c|),
+ * fraction from the largest face angle: adds to Wbar (velocity), *csbar, and the bars of c_i, of the angle face's A and of its
+ * separation. */
+static void regularised_adjoint(const orc_fv* fv, const orc_geometry* g, int i, const double* wbar, double* Wbar, double* csbar, geo_bars* b) {
+    const int D = fv->D, V = D + 2, PW = V + 2;
+    const double* Wi = fv->prims + (size_t)i * PW;
+    double amax = 0;
+    int fmax = -1, other = -1;
+    for (int k = g->start[i]; k < g->start[i + 1]; ++k) {
+        const int e = g->list[k], f = list_face(e), j = e >= 0 ? g->neighbour[f] : g->owner[f];
+        double d[MAXD];
+        separation(g, i, j, d);
+        const double angle = face_angle(D, magnitude_of(g, f)) / (0.5 * sqrt(dot(D, d, d)));
+        if (angle > amax) {
+            amax = angle;
+            fmax = f;
+            other = j;
+        }
+    }
+    const double F = 2.25, shaping = 0.5;
+    double fraction = 0, slope = 0;
+    if (amax > 0.75 * F) {
+        fraction = amax > F ? shaping : shaping * (amax - 0.75 * F) / (0.25 * F);
+        slope = amax > F ? 0.0 : shaping / (0.25 * F);
+    }
+    for (int a = 0; a < D; ++a) Wbar[1 + a] += wbar[a];
+    const double* c = g->centroid + (size_t)i * D;
+    const double cd = sqrt(dot(D, c, c)), cs = Wi[V];
+    if (!(cd > 0 && fraction > 0)) return;
+    const double k = fraction * cs / cd, kbar = dot(D, wbar, c);
+    const double fractionbar = kbar * cs / cd, cdbar = -kbar * k / cd;
+    *csbar += kbar * fraction / cd;
+    for (int a = 0; a < D; ++a) b->c[(size_t)i * D + a] += wbar[a] * k + c[a] * (cdbar / cd);
+    if (slope == 0) return;
+    const double abar = slope * fractionbar;
+    const double* A = g->area + (size_t)fmax * D;
+    double d[MAXD], Ab[MAXD], db[MAXD];
+    separation(g, i, other, d);
+    const double mag = sqrt(dot(D, A, A)), dist = sqrt(dot(D, d, d));
+    for (int a = 0; a < D; ++a) {
+        Ab[a] = A[a] * (abar * face_angle_slope(D, mag) / mag / (0.5 * dist));
+        db[a] = -d[a] * (abar * amax / (dist * dist));
+        b->A[(size_t)fmax * D + a] += Ab[a];
+    }
+    add_sep_bar(D, b, i, other, db);
+}
+
+/* finite_volume.hpp:2629-2760 moving_gradient_adjoint(): the limited fit's adjoint in W and in the geometry it reads. The fit
+ * x_v = M^-1 b_v (X_k = [1, d_k + c_j - c_i], weights |A_k|) moves by M^-1 (db - dM x); the last clip's scale
+ * s = (bound - W_i) / (g0 . o) also by o's bar. Writes beta (V x (D + 1): M^-1 x_v bar), boundbar, the extreme face and
+ * adds W_i's own bar; the neighbours' W bars are gathered from beta by each neighbour. */
+static void moving_fit_adjoint(const orc_fv* fv, const orc_geometry* g, int i, const double* gb, double* Wb_i, double* beta, double* boundbar,
+                               int* extreme, geo_bars* b) {
+    const int D = fv->D, V = D + 2, U = D + 1, PW = V + 2;
+    const double* Wi = fv->prims + (size_t)i * PW;
+    double gl[MAXV * MAXD], g0[MAXV * MAXD], Lc[MAXD + 1][MAXD + 1], M0[MAXD + 1], icpt[MAXV];
+    limit_pattern pat;
+    fit_cell_x(fv, g, i, NULL, gl, g0, NULL, Lc, M0, &pat, icpt);
+    const int first = g->start[i];
+    double xbar[MAXV][MAXD + 1], bb[MAXV][MAXD + 1];
+    for (int v = 0; v < V; ++v) {
+        double g0bar[MAXD];
+        for (int a = 0; a < D; ++a) g0bar[a] = gb[v * D + a];
+        boundbar[v] = 0;
+        extreme[v] = -1;
+        if (pat.last[v] >= 0 && pat.zero[v]) {
+            for (int a = 0; a < D; ++a) g0bar[a] = 0;
+        } else if (pat.last[v] >= 0) {
+            const double* o = pat.offset[v];
+            const double den = dot(D, g0 + v * D, o);
+            const double sc = (pat.bound[v] - Wi[v]) / den, t = dot(D, gb + v * D, g0 + v * D) / den;
+            double obar[MAXD];
+            for (int a = 0; a < D; ++a) {
+                g0bar[a] = gb[v * D + a] * sc - o[a] * (sc * t);
+                obar[a] = -sc * t * g0[v * D + a];
+            }
+            boundbar[v] = t;
+            extreme[v] = list_face(g->list[first + pat.extreme[v]]);
+            Wb_i[v] -= t;
+            face_offset_bar(g, b, list_face(g->list[first + pat.last[v]]), i, obar);
+        }
+        xbar[v][0] = 0;
+        for (int a = 0; a < D; ++a) xbar[v][1 + a] = g0bar[a];
+    }
+    solve_factored(D, Lc, xbar, bb); /* beta_v = M^-1 x_v bar (M symmetric) */
+    double x[MAXV][MAXD + 1], Mbar[MAXD + 1][MAXD + 1];
+    for (int v = 0; v < V; ++v) {
+        x[v][0] = icpt[v];
+        for (int a = 0; a < D; ++a) x[v][1 + a] = g0[v * D + a];
+    }
+    for (int p = 0; p < U; ++p)
+        for (int q = 0; q < U; ++q) {
+            Mbar[p][q] = 0;
+            for (int v = 0; v < V; ++v) Mbar[p][q] -= bb[v][p] * x[v][q];
+        }
+    for (int v = 0; v < V; ++v)
+        for (int p = 0; p < U; ++p) {
+            beta[v * U + p] = bb[v][p];
+            Wb_i[v] -= M0[p] * bb[v][p];
+        }
+    const double* ci = g->centroid + (size_t)i * D;
+    for (int k = first; k < g->start[i + 1]; ++k) {
+        const int e = g->list[k], f = list_face(e), j = e >= 0 ? g->neighbour[f] : g->owner[f];
+        const double* Wj = fv->prims + (size_t)j * PW;
+        double d[MAXD], X[MAXD + 1], Xbar[MAXD + 1] = {0, 0, 0, 0};
+        separation(g, i, j, d);
+        const double w = magnitude_of(g, f);
+        X[0] = 1;
+        for (int p = 1; p < U; ++p) X[p] = d[p - 1] + g->centroid[(size_t)j * D + p - 1] - ci[p - 1];
+        double wbar = 0;
+        for (int v = 0; v < V; ++v) {
+            const double dW = Wj[v] - Wi[v];
+            double xb = 0;
+            for (int p = 0; p < U; ++p) xb += X[p] * bb[v][p];
+            wbar += xb * dW;
+            for (int p = 0; p < U; ++p) Xbar[p] += w * dW * bb[v][p];
+        }
+        for (int p = 0; p < U; ++p)
+            for (int q = 0; q < U; ++q) {
+                wbar += X[p] * Mbar[p][q] * X[q];
+                Xbar[p] += w * (Mbar[p][q] + Mbar[q][p]) * X[q];
+            }
+        if (w > 0) {
+            const double* A = g->area + (size_t)f * D;
+            for (int a = 0; a < D; ++a) b->A[(size_t)f * D + a] += wbar * A[a] / w;
+        }
+        /* between = d + c_j - c_i */
+        add_sep_bar(D, b, i, j, Xbar + 1);
+        for (int a = 0; a < D; ++a) {
+            b->c[(size_t)j * D + a] += Xbar[1 + a];
+            b->c[(size_t)i * D + a] -= Xbar[1 + a];
+        }
+    }
+}
+
+int orc_moving_adjoint(orc_fv* fv, const orc_geometry* g0, const orc_geometry* g1, double dt, int steps_fixed, double* qbar, double* xbar,
+                       orc_geometry_adjoint adjoint0, void* ctx0, orc_geometry_adjoint adjoint1, void* ctx1) {
+    if (!fv->moving || fv->order != 1 || fv->D < 2) return -1;
+    const int D = fv->D, V = D + 2, U = D + 1, N = fv->N, PW = V + 2;
+    /* t^n: W, the limited gradients and the point velocities (the sweep), c^n, the grid steps and w-hat. */
+    orc_primitives(fv, g0);
+    for (int i = 0; i < N; ++i) sweep_cell(fv, g0, i);
+    const double unit = g0->step / dt;
+    double* wh = (double*)calloc((size_t)N * D, sizeof(double));
+    for (int i = 0; i < N; ++i)
+        for (int a = 0; a < D; ++a) {
+            const size_t at = (size_t)i * D + a;
+            fv->old_centroid[at] = g0->centroid[at];
+            if (!fv->hold_steps) {
+                const long long k0 = orc_snap(fv->x[at], g0->step, g0->period[a]);
+                const long long k1 = orc_snap(fv->x[at] + dt * fv->w[at], g0->step, g0->period[a]);
+                fv->steps[at] = orc_delta(k0, k1, g0->period[a]);
+            }
+            wh[at] = (double)fv->steps[at] * unit;
+        }
+    double* Wbar = (double*)calloc((size_t)N * V, sizeof(double));
+    double* gbar = (double*)calloc((size_t)N * V * D, sizeof(double));
+    double* whbar = (double*)calloc((size_t)N * D, sizeof(double));
+    double* c0bar = (double*)calloc((size_t)N * D, sizeof(double));
+    const int F1 = g1->F, F0 = g0->F;
+    geo_bars b1 = {(double*)calloc(N, sizeof(double)), (double*)calloc((size_t)N * D, sizeof(double)), (double*)calloc((size_t)N * D, sizeof(double)),
+                   (double*)calloc((size_t)(F1 > 0 ? F1 : 1) * D, sizeof(double)), (double*)calloc((size_t)(F1 > 0 ? F1 : 1) * D, sizeof(double))};
+    geo_bars b0 = {(double*)calloc(N, sizeof(double)), (double*)calloc((size_t)N * D, sizeof(double)), (double*)calloc((size_t)N * D, sizeof(double)),
+                   (double*)calloc((size_t)(F0 > 0 ? F0 : 1) * D, sizeof(double)), (double*)calloc((size_t)(F0 > 0 ? F0 : 1) * D, sizeof(double))};
+    /* Pass 2 on t^n+1, then x^n+1's bar: the caller's, the separations', the geometry's. */
+    moving_pass_adjoint(fv, g1, 1, dt, wh, qbar, Wbar, gbar, whbar, c0bar, &b1);
+    for (size_t k = 0; k < (size_t)N * D; ++k) b1.x[k] += xbar[k];
+    adjoint1(ctx1, b1.V, b1.c, b1.A, b1.xf, b1.x);
+    /* x^n+1 = x^n + dt w-hat (the quantisation the identity; held steps: w-hat fixed). */
+    for (size_t k = 0; k < (size_t)N * D; ++k) {
+        b0.x[k] = b1.x[k];
+        if (!steps_fixed) whbar[k] += dt * b1.x[k];
+    }
+    /* Pass 1 on t^n. */
+    moving_pass_adjoint(fv, g0, 0, dt, wh, qbar, Wbar, gbar, whbar, c0bar, &b0);
+    for (size_t k = 0; k < (size_t)N * D; ++k) b0.c[k] += c0bar[k];
+    /* Per cell: the point velocity (w-hat = w), the sound speed, the limited fit. */
+    double* beta = (double*)calloc((size_t)N * V * U, sizeof(double));
+    double* boundbar = (double*)calloc((size_t)N * V, sizeof(double));
+    int* extreme = (int*)malloc(sizeof(int) * (size_t)N * V);
+    for (int i = 0; i < N; ++i) {
+        const double* Wi = fv->prims + (size_t)i * PW;
+        double* Wb = Wbar + (size_t)i * V;
+        double csbar = 0;
+        if (!steps_fixed) regularised_adjoint(fv, g0, i, whbar + (size_t)i * D, Wb, &csbar, &b0);
+        if (Wi[V] > 0) { /* c_s = sqrt(gamma P / rho) */
+            const double c2bar = csbar / (2 * Wi[V]);
+            Wb[D + 1] += c2bar * fv->gas.gamma / Wi[0];
+            Wb[0] -= c2bar * fv->gas.gamma * Wi[D + 1] / (Wi[0] * Wi[0]);
+        }
+        moving_fit_adjoint(fv, g0, i, gbar + (size_t)i * V * D, Wb, beta + (size_t)i * V * U, boundbar + (size_t)i * V, extreme + (size_t)i * V, &b0);
+    }
+    /* Per cell: the neighbours' fits' bars of its W, then W = primitive(Q / V): Q's and V's bars. */
+    for (int j = 0; j < N; ++j) {
+        double Wb[MAXV];
+        for (int v = 0; v < V; ++v) Wb[v] = Wbar[(size_t)j * V + v];
+        for (int k = g0->start[j]; k < g0->start[j + 1]; ++k) {
+            const int e = g0->list[k], f = list_face(e);
+            const int i = e >= 0 ? g0->neighbour[f] : g0->owner[f];
+            double d[MAXD], X[MAXD + 1];
+            separation(g0, i, j, d);
+            const double w = magnitude_of(g0, f);
+            X[0] = 1;
+            for (int p = 1; p < U; ++p) X[p] = d[p - 1] + g0->centroid[(size_t)j * D + p - 1] - g0->centroid[(size_t)i * D + p - 1];
+            for (int v = 0; v < V; ++v) {
+                double share = 0;
+                for (int p = 0; p < U; ++p) share += w * X[p] * beta[((size_t)i * V + v) * U + p];
+                if (extreme[(size_t)i * V + v] == f) share += boundbar[(size_t)i * V + v];
+                Wb[v] += share;
+            }
+        }
+        const double inv = 1.0 / g0->volume[j];
+        double Uj[MAXV], Ub[MAXV];
+        for (int v = 0; v < V; ++v) Uj[v] = fv->q[(size_t)j * V + v] * inv;
+        orc_primitive_adjoint(D, &fv->gas, Uj, Wb, Ub);
+        double vb = 0;
+        for (int v = 0; v < V; ++v) {
+            qbar[(size_t)j * V + v] += Ub[v] * inv;
+            vb -= Ub[v] * Uj[v] * inv;
+        }
+        b0.V[j] += vb;
+    }
+    adjoint0(ctx0, b0.V, b0.c, b0.A, b0.xf, b0.x);
+    memcpy(xbar, b0.x, sizeof(double) * (size_t)N * D);
+    free(wh);
+    free(Wbar);
+    free(gbar);
+    free(whbar);
+    free(c0bar);
+    free(beta);
+    free(boundbar);
+    free(extreme);
+    geo_bars* all[2] = {&b0, &b1};
+    for (int k = 0; k < 2; ++k) {
+        free(all[k]->V);
+        free(all[k]->c);
+        free(all[k]->x);
+        free(all[k]->A);
+        free(all[k]->xf);
+    }
     return 0;
 }
 

@@ -74,6 +74,7 @@ typedef struct {
     int prims_ready;
     unsigned short* features; /* limiter 2 (learned): N V x 16 BF16 bit patterns, written by each sweep */
     unsigned char* troubled;  /* limiter 2: N */
+    int hold_steps;           /* moving: orc_moving_begin keeps the given steps (the adjoint's central differences) */
 } orc_fv;
 
 void orc_primitives(orc_fv* fv, const orc_geometry* geo);
@@ -86,6 +87,16 @@ void orc_moving_finish(orc_fv* fv, const orc_geometry* geo, double dt);
  * for a moving mesh, order 0 or D = 1 (see orc_line_*). */
 int orc_step_tangent(orc_fv* fv, const orc_geometry* geo, double dt, double* dq);
 int orc_step_adjoint(orc_fv* fv, const orc_geometry* geo, double dt, double* qbar);
+/* The moving step's reverse at order 1 and fixed topology (finite_volume.hpp:2984-3155 step_moving_adjoint_1): fv holds the
+ * state at t^n (q, x on the grid; its prims and sweep are recomputed on g0), g1 is the mesh of the positions the forward
+ * step reached. qbar, xbar (N x V, N x D: the bars of Q, x after the step) become those before it. The geometries'
+ * adjoints are the caller's (the mesh library's derivative): geometry_adjoint(ctx, Vbar, cbar, Abar, xfbar, xbar) adds to
+ * xbar, called for g1 (ctx1), then g0 (ctx0). steps_fixed: the whole grid steps the forward took are held (derivative 0);
+ * otherwise the quantisation is the identity (x^n+1 = x^n + dt w). The forward with held steps: orc_moving_begin with
+ * fv->hold_steps set and fv->steps given. */
+typedef void (*orc_geometry_adjoint)(void* ctx, const double* Vbar, const double* cbar, const double* Abar, const double* xfbar, double* xbar);
+int orc_moving_adjoint(orc_fv* fv, const orc_geometry* g0, const orc_geometry* g1, double dt, int steps_fixed, double* qbar, double* xbar,
+                       orc_geometry_adjoint adjoint0, void* ctx0, orc_geometry_adjoint adjoint1, void* ctx1);
 long long orc_snap(double x, double step, long long period);
 int orc_delta(long long k0, long long k1, long long period);
 

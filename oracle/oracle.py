@@ -33,7 +33,11 @@ class _FV(ctypes.Structure):
                 ("riemann", ctypes.c_int), ("moving", ctypes.c_int), ("step_rule", ctypes.c_int), ("gas", Gas),
                 ("q", _dp), ("prims", _dp), ("grad", _dp), ("w", _dp), ("x", _dp), ("old_centroid", _dp),
                 ("steps", _ip), ("flux", _dp), ("fallbacks", ctypes.c_longlong * 2), ("prims_ready", ctypes.c_int),
-                ("features", ctypes.POINTER(ctypes.c_ushort)), ("troubled", ctypes.POINTER(ctypes.c_ubyte))]
+                ("features", ctypes.POINTER(ctypes.c_ushort)), ("troubled", ctypes.POINTER(ctypes.c_ubyte)),
+                ("hold_steps", ctypes.c_int)]
+
+
+_GEO_ADJOINT = ctypes.CFUNCTYPE(None, ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp)
 
 
 class _Line(ctypes.Structure):
@@ -60,6 +64,8 @@ def lib():
         L.orc_snap.restype = ctypes.c_longlong
         L.orc_snap.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_longlong]
         L.orc_delta.argtypes = [ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong]
+        L.orc_moving_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                         _GEO_ADJOINT, ctypes.c_void_p, _GEO_ADJOINT, ctypes.c_void_p]
         L.orc_mline_create.restype = ctypes.c_void_p
         L.orc_mline_create.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]
@@ -286,6 +292,29 @@ class FiniteVolume:
 
     def fallbacks(self):
         return int(self.c.fallbacks[0] + self.c.fallbacks[1])
+
+    def moving_adjoint(self, dt, qbar, xbar, mesh1, adjoint0, adjoint1, steps_fixed=True):
+        """The moving step's reverse (order 1, fixed topology) from the state held (q, x at t^n on the current mesh):
+        qbar, xbar (bars of Q, x after the step) -> before it. mesh1: the mesh the forward step reached. adjoint0/1:
+        the geometry adjoints of the two meshes, f(Vbar, cbar, Abar, xfbar, xbar) adding into xbar (numpy views)."""
+        D, N, V = self.D, self.N, self.V
+        geo1 = Geometry(mesh1)
+        F0, F1 = self.geo.c.F, geo1.c.F
+
+        def wrap(fn, F):
+            def cb(_ctx, Vb, cb_, Ab, xfb, xb):
+                as_ = np.ctypeslib.as_array
+                fn(as_(Vb, (N,)), as_(cb_, (N, D)), as_(Ab, (max(F, 1), D))[:F], as_(xfb, (max(F, 1), D))[:F], as_(xb, (N, D)))
+            return _GEO_ADJOINT(cb)
+
+        c0, c1 = wrap(adjoint0, F0), wrap(adjoint1, F1)
+        qb = np.ascontiguousarray(np.array(qbar, dtype=np.float64).reshape(N, V))
+        xb = np.ascontiguousarray(np.array(xbar, dtype=np.float64).reshape(N, D))
+        rc = lib().orc_moving_adjoint(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.byref(geo1.c), ctypes.c_double(dt),
+                                      int(steps_fixed), _ptr(qb), _ptr(xb), c0, None, c1, None)
+        if rc != 0:
+            raise ValueError("moving_adjoint: a moving mesh, order 1")
+        return qb, xb
 
 
 WALLS = {"reflective": 0, "outflow": 1, "inflow": 2}
