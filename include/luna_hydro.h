@@ -203,6 +203,22 @@ int luna_fv1_step_adjoint(luna_fv1* fv, double dt, double* qbar);
 int luna_fv_step_tangent(luna_fv* fv, double dt, double* dq, int device);
 int luna_fv_step_adjoint(luna_fv* fv, double dt, double* qbar, int device);
 
+/* The geometry's derivative the moving step's reverse reads (hydro::GeometryDerivative::adjoint, mesh_derivative.hpp:28-29): from
+ * the bars of V, c (cells) and A, x_f (faces), add to xbar (cells x dim); host arrays; 0 on success. The mesh library's
+ * luna_mesh_derivative_adjoint (include/luna_mesh.h) has this shape, with its handle as ctx. */
+typedef int (*luna_geometry_adjoint_fn)(const void* ctx, const double* Vbar, const double* cbar, const double* Abar, const double* xfbar,
+                                         double* xbar);
+
+/* FiniteVolume::step_moving_adjoint(dt, qbar, xbar, geometry, steps_fixed) at order 1 (finite_volume.hpp:2488-2495, 2984-3155), at
+ * fixed topology: the handle holds Q at t^n (as set); mesh0 is the mesh of t^n (its positions are x^n), mesh1 the mesh the forward
+ * step reached from it. qbar (cells x (dim + 2)) and xbar (cells x dim), host, the bars of Q and x after the step, become those
+ * before it. adjoint0 / adjoint1: the two meshes' geometry adjoints (called on the host, mesh1's first). steps_fixed: the whole
+ * grid steps the forward took are held (derivative 0, as central differences small enough to change no step see it);
+ * otherwise the quantisation is the identity (x^n+1 = x^n + dt w). Leaves the handle on mesh0 with Q unchanged; blocks.
+ * LUNA_ERR_LOGIC on a static mesh, order 0 or the learned limiter. */
+int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, const luna_geometry* mesh1, luna_geometry_adjoint_fn adjoint0,
+                           const void* ctx0, luna_geometry_adjoint_fn adjoint1, const void* ctx1, int steps_fixed, double* qbar, double* xbar);
+
 /* ------------------------------------------------------------ the learned slope limiter (no counterpart in luna-cpp)
  *
  * With LUNA_LIMITER_LEARNED each sweep runs the face-centroid limiter (finite_volume.hpp:538-565) and records, per cell and

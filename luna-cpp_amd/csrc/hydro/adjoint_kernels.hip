@@ -58,7 +58,7 @@ struct Pattern {
 // dg0 (Tangent), M's Cholesky factor L and first row M0, and the limiter's pattern. dW: N x V tangents of W.
 template <int D, bool Tangent>
 __device__ void fit_cell(const Geo& g, int i, const double* dW, bool limit, double* gl, double* g0, double* dg0, double (*L)[D + 1],
-                         double* M0, Pattern<D>& pat) {
+                         double* M0, Pattern<D>& pat, double* intercept = nullptr) {
     constexpr int V = D + 2, U = D + 1;
     using C = CellAt<D>;
     using FA = FaceAt<D>;
@@ -187,6 +187,7 @@ __device__ void fit_cell(const Geo& g, int i, const double* dW, bool limit, doub
                 else
                     dg0[v * D + a] = x[1 + a];
             }
+            if (pass == 0 && intercept) intercept[v] = x[0];
         }
     }
 #pragma unroll
@@ -604,6 +605,666 @@ __global__ __launch_bounds__(BLOCK) void k_gather_adjoint(Geo g, Rows r, Gas gas
     for (int v = 0; v < V; ++v) out[(size_t)j * V + v] = qbar[(size_t)j * V + v] + Ub[v] * inv;
 }
 
+// ---- the moving step's reverse, order 1, fixed topology (finite_volume.hpp:2233-3155) ----
+// Two passes (mesh t^n+1, then t^n), each one work-item per face writing its sides' bars into a face row and its own
+// face's bars, then one per cell gathering its entries in list order; then per cell the point velocity's and the fit's
+// adjoints (entry rows for the neighbours), a gather per cell and one per face. No atomics.
+
+// Face row of a pass: [W bar, g bar, r bar] of the owner's side, the same of the neighbour's, w-hat bars of both cells,
+// the bar of the separation owner -> neighbour.
+template <int D>
+constexpr int pass_row() {
+    return 2 * ((D + 2) + (D + 2) * D + D) + 3 * D;
+}
+// Entry row of the fit's adjoint (per list slot): the bar of the entry's between vector, of its weight |A|, of its face
+// offset (the limiter's last clip).
+template <int D>
+constexpr int entry_row() {
+    return 2 * D + 1;
+}
+
+// euler.hpp:848-851 turn_bar(): the n bar of <a, Omega b>: out += sign (a (n.b) - b (n.a)).
+template <int D>
+__device__ __forceinline__ void turn_bar_add(const double* n, const double* a, const double* b, double sign, double* out) {
+    const double nb = dot<D>(n, b), na = dot<D>(n, a);
+#pragma unroll
+    for (int k = 0; k < D; ++k) out[k] += sign * (a[k] * nb - b[k] * na);
+}
+
+// euler.hpp:893-916 MovingFace::adjoint(): G = |A| F(L, R, A / |A|, w); adds J^T Gbar to Lbar, Rbar, Abar, wbar (the normal's
+// and the face velocity's parts from the flux's rotation and Galilean invariance, euler.hpp:835-841).
+template <int D>
+__device__ void moving_face_adjoint(int riemann_kind, const Gas& gas, const double* L, const double* R, const double* A, const double* w,
+                                    const double* Gbar, double* Lbar, double* Rbar, double* Abar, double* wbar) {
+    constexpr int V = D + 2;
+    const double s = sqrt(dot<D>(A, A));
+    double n[D], F[V], Fr[V], X[V], mw[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        n[k] = A[k] * (1.0 / s);
+        mw[k] = -w[k];
+    }
+    riemann<D>(riemann_kind, gas, L, R, n, w, F);
+#pragma unroll
+    for (int v = 0; v < V; ++v) Fr[v] = F[v];
+    to_lab<D>(Fr, mw);  // the flux in the face's frame
+    double sbar = 0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        sbar += Gbar[v] * F[v];
+        X[v] = s * Gbar[v];
+    }
+    double nbar[D], shiftbar[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) nbar[k] = 0;
+    turn_bar_add<D>(n, X + 1, F + 1, 1.0, nbar);
+#pragma unroll
+    for (int k = 0; k < D; ++k) shiftbar[k] = X[1 + k] * Fr[0] + X[D + 1] * (Fr[1 + k] + w[k] * Fr[0]);
+    double Lr[V], Rr[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) Lr[v] = Rr[v] = 0;
+    riemann_adjoint<D>(riemann_kind, gas, L, R, n, w, X, Lr, Rr);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        Lbar[v] += Lr[v];
+        Rbar[v] += Rr[v];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) shiftbar[k] -= Lr[1 + k] + Rr[1 + k];
+    turn_bar_add<D>(n, Lr + 1, L + 1, -1.0, nbar);
+    turn_bar_add<D>(n, Rr + 1, R + 1, -1.0, nbar);
+#pragma unroll
+    for (int k = 0; k < D; ++k) wbar[k] += shiftbar[k];
+    turn_bar_add<D>(n, shiftbar, w, -1.0, nbar);
+    const double nn = dot<D>(n, nbar);
+#pragma unroll
+    for (int k = 0; k < D; ++k) Abar[k] += (nbar[k] - n[k] * nn) * (1.0 / s) + n[k] * sbar;
+}
+
+// FaceFlux::face_velocity() (finite_volume.hpp:1380-1397) and its adjoint (:2331-2369), at the forward's branch.
+template <int D>
+__device__ __forceinline__ void face_velocity(const double* d, const double* xf, const double* wL, const double* wR, double* wf) {
+    double c[D], dw[D];
+    const double dist = sqrt(dot<D>(d, d)), inv_dist = 1.0 / dist;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        c[a] = xf[a] - d[a] * 0.5;
+        dw[a] = wL[a] - wR[a];
+    }
+    double facv = dot<D>(c, dw) * inv_dist;
+    const double limit = 0.9 * dist, c2 = dot<D>(c, c);
+    if (c2 > limit * limit) facv *= limit / sqrt(c2);
+#pragma unroll
+    for (int a = 0; a < D; ++a) wf[a] = (wL[a] + wR[a]) * 0.5 + d[a] * (facv * inv_dist);
+}
+template <int D>
+__device__ void face_velocity_adjoint(const double* d, const double* xf, const double* wL, const double* wR, const double* wfbar, double* dbar,
+                                      double* xfbar, double* wLbar, double* wRbar) {
+    double c[D], u[D], cbar[D];
+    const double dist = sqrt(dot<D>(d, d));
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        c[a] = xf[a] - d[a] * 0.5;
+        u[a] = wL[a] - wR[a];
+        cbar[a] = 0;
+    }
+    const double f0 = dot<D>(c, u) / dist, limit = 0.9 * dist, c2 = dot<D>(c, c), cc = sqrt(c2);
+    const bool clipped = c2 > limit * limit;
+    const double sc = clipped ? limit / cc : 1.0, facv = f0 * sc;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        wLbar[a] += wfbar[a] * 0.5;
+        wRbar[a] += wfbar[a] * 0.5;
+        dbar[a] += wfbar[a] * (facv / dist);
+    }
+    const double qb = dot<D>(wfbar, d), facvbar = qb / dist;
+    double distbar = -qb * facv / (dist * dist);
+    const double f0bar = facvbar * sc;
+    if (clipped) {
+        const double scbar = facvbar * f0;
+        distbar += scbar * 0.9 / cc;
+#pragma unroll
+        for (int a = 0; a < D; ++a) cbar[a] += c[a] * (-scbar * sc / (cc * cc));
+    }
+    distbar -= f0bar * f0 / dist;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        cbar[a] += u[a] * (f0bar / dist);
+        const double ub = c[a] * (f0bar / dist);
+        wLbar[a] += ub;
+        wRbar[a] -= ub;
+        xfbar[a] += cbar[a];
+        dbar[a] += -cbar[a] * 0.5 + d[a] * (distbar / dist);
+    }
+}
+
+// One moving pass's face: its two sides (first pass W + g.r; second half_step(W, g, 2 dt) + g.(r + c - c^n + dt w-hat)),
+// G = |A| F at the face velocity, Gbar = -dt/2 (qbar_o - qbar_j); the sides' bars (W, g, offset), the w-hat bars, the
+// separation's bar into the row; A's and x_f's bars (x_f also from both offsets) into this face's rows of Ab, Xfb.
+template <int D, bool Second>
+__global__ __launch_bounds__(BLOCK) void k_mpass_face(Geo g, Rows r, Gas gas, int riemann_kind, double dt, double unit, const double* qbar,
+                                                      double* row, double* Ab, double* Xfb) {
+    constexpr int V = D + 2, RW = pass_row<D>(), GR = grad_row<D>(), SIDE = V + V * D + D;
+    using C = CellAt<D>;
+    using FA = FaceAt<D>;
+    const int f = blockIdx.x * BLOCK + threadIdx.x;
+    if (f >= g.F) return;
+    double* out = row + (size_t)f * RW;
+#pragma unroll
+    for (int k = 0; k < RW; ++k) out[k] = 0;
+    double Abar[D], Xfbar[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) Abar[a] = Xfbar[a] = 0;
+    const double* face = g.face + (size_t)f * FACE_ROW;
+    if (face[FA::MAG] != 0) {
+        const int2 pair = *reinterpret_cast<const int2*>(face + FA::PAIR);
+        const int cells[2] = {pair.x, pair.y};
+        double A[D], fc[D], xs[2][D], d[D], back[D], wh[2][D], wf[D];
+        load_vec<D>(face + FA::A, A);
+        load_vec<D>(face + FA::FC, fc);
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+            load_vec<D>(g.cell + (size_t)cells[s2] * CELL_ROW + C::X, xs[s2]);
+#pragma unroll
+            for (int a = 0; a < D; ++a) wh[s2][a] = (double)r.steps[(size_t)cells[s2] * D + a] * unit;
+        }
+        separation<D>(g, xs[0], xs[1], d);     // owner -> neighbour
+        separation<D>(g, xs[1], xs[0], back);  // neighbour -> owner
+        face_velocity<D>(d, fc, wh[0], wh[1], wf);
+        double W[2][V], gr[2][V * D], rr[2][D], sides[2][V];
+        bool fallback[2];
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+            const int i = cells[s2];
+            const double* cr = g.cell + (size_t)i * CELL_ROW;
+#pragma unroll
+            for (int v = 0; v < V; ++v) W[s2][v] = cr[C::W + v];
+#pragma unroll
+            for (int k = 0; k < V * D; ++k) gr[s2][k] = r.grad[(size_t)i * GR + k];
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                rr[s2][a] = (s2 == 0 ? fc[a] : fc[a] + back[a]) - cr[C::C + a];
+                if constexpr (Second) rr[s2][a] = rr[s2][a] + cr[C::C + a] - r.old_centroid[(size_t)i * D + a] + wh[s2][a] * (0.5 * (2 * dt));
+            }
+            if constexpr (Second) {
+                half_step<D>(gas, W[s2], gr[s2], 2 * dt, sides[s2]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v) sides[s2][v] = W[s2][v];
+            }
+#pragma unroll
+            for (int v = 0; v < V; ++v) sides[s2][v] += dot<D>(gr[s2] + v * D, rr[s2]);
+            fallback[s2] = !(sides[s2][0] > 0 && sides[s2][D + 1] > 0);
+            if (fallback[s2])
+#pragma unroll
+                for (int v = 0; v < V; ++v) sides[s2][v] = W[s2][v];
+        }
+        double Gbar[V], Lbar[V], Rbar[V], wfbar[D], dbar[D];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            Gbar[v] = -0.5 * dt * (qbar[(size_t)pair.x * V + v] - qbar[(size_t)pair.y * V + v]);
+            Lbar[v] = Rbar[v] = 0;
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) wfbar[a] = dbar[a] = 0;
+        moving_face_adjoint<D>(riemann_kind, gas, sides[0], sides[1], A, wf, Gbar, Lbar, Rbar, Abar, wfbar);
+        double* whb_o = out + 2 * SIDE;
+        double* whb_j = whb_o + D;
+        face_velocity_adjoint<D>(d, fc, wh[0], wh[1], wfbar, dbar, Xfbar, whb_o, whb_j);
+#pragma unroll
+        for (int a = 0; a < D; ++a) out[2 * SIDE + 2 * D + a] = dbar[a];
+#pragma unroll
+        for (int s2 = 0; s2 < 2; ++s2) {
+            const double* sb = s2 == 0 ? Lbar : Rbar;
+            double* o = out + s2 * SIDE;  // [W bar V, g bar V D, r bar D]
+            if (fallback[s2]) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) o[v] = sb[v];
+                continue;
+            }
+            double Wb[V], gb[V * D];
+#pragma unroll
+            for (int v = 0; v < V; ++v) Wb[v] = 0;
+#pragma unroll
+            for (int k = 0; k < V * D; ++k) gb[k] = 0;
+            if constexpr (Second) {
+                half_step_adjoint<D>(gas, W[s2], gr[s2], 2 * dt, sb, Wb, gb);
+            } else {
+#pragma unroll
+                for (int v = 0; v < V; ++v) Wb[v] += sb[v];
+            }
+            double rb[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) rb[a] = 0;
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    gb[v * D + a] += sb[v] * rr[s2][a];
+                    rb[a] += sb[v] * gr[s2][v * D + a];
+                }
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = Wb[v];
+#pragma unroll
+            for (int k = 0; k < V * D; ++k) o[V + k] = gb[k];
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                o[V + V * D + a] = rb[a];
+                Xfbar[a] += rb[a];
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        Ab[(size_t)f * D + a] = Abar[a];
+        Xfb[(size_t)f * D + a] = Xfbar[a];
+    }
+}
+
+// Per cell, in list order: its sides' rows into W bar, g bar, w-hat bar; the offsets' bars into c bar (and on the second
+// pass into c^n's bar and w-hat's); the separations' bars into x bar. Adds to every array.
+template <int D, bool Second>
+__global__ __launch_bounds__(BLOCK) void k_mpass_gather(Geo g, double dt, const double* row, double* Wbar, double* gbar, double* whbar,
+                                                        double* c0bar, double* cbar, double* xbar) {
+    constexpr int V = D + 2, RW = pass_row<D>(), SIDE = V + V * D + D;
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= g.N) return;
+    double Wb[V], gb[V * D], whb[D], c0b[D], cb[D], xb[D];
+#pragma unroll
+    for (int v = 0; v < V; ++v) Wb[v] = Wbar[(size_t)i * V + v];
+#pragma unroll
+    for (int k = 0; k < V * D; ++k) gb[k] = gbar[(size_t)i * V * D + k];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        whb[a] = whbar[(size_t)i * D + a];
+        c0b[a] = c0bar[(size_t)i * D + a];
+        cb[a] = cbar[(size_t)i * D + a];
+        xb[a] = xbar[(size_t)i * D + a];
+    }
+    const int count = g.count[i], base = first_slot(g, i);
+    for (int k = 0; k < count; ++k) {
+        const int e = g.list[base + k * SLICE], f = e >= 0 ? e : ~e;
+        const double* rw = row + (size_t)f * RW;
+        const int s2 = e >= 0 ? 0 : 1;
+        const double* o = rw + s2 * SIDE;
+#pragma unroll
+        for (int v = 0; v < V; ++v) Wb[v] += o[v];
+#pragma unroll
+        for (int q = 0; q < V * D; ++q) gb[q] += o[V + q];
+        const double* rb = o + V + V * D;
+        const double* wb = rw + 2 * SIDE + s2 * D;
+        const double* db = rw + 2 * SIDE + 2 * D;
+        const double* rb_j = rw + SIDE + V + V * D;  // the neighbour side's offset bar: its separation neighbour -> owner
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            whb[a] += wb[a];
+            cb[a] -= rb[a];
+            if constexpr (Second) {
+                cb[a] += rb[a];
+                c0b[a] -= rb[a];
+                whb[a] += dt * rb[a];
+            }
+            if (s2 == 0)
+                xb[a] += rb_j[a] - db[a];
+            else
+                xb[a] += db[a] - rb_j[a];
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) Wbar[(size_t)i * V + v] = Wb[v];
+#pragma unroll
+    for (int k = 0; k < V * D; ++k) gbar[(size_t)i * V * D + k] = gb[k];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        whbar[(size_t)i * D + a] = whb[a];
+        c0bar[(size_t)i * D + a] = c0b[a];
+        cbar[(size_t)i * D + a] = cb[a];
+        xbar[(size_t)i * D + a] = xb[a];
+    }
+}
+
+// x^n+1 = x^n + dt w-hat: w-hat's bar from x^n+1's (the quantisation the identity).
+__global__ __launch_bounds__(BLOCK) void k_drift_adjoint(long long n, double dt, const double* xbar, double* whbar) {
+    const long long k = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (k < n) whbar[k] += dt * xbar[k];
+}
+
+// finite_volume.hpp:2237-2243 face_angle_slope()
+template <int D>
+__device__ __forceinline__ double face_angle_slope(double area) {
+    if constexpr (D == 3) return area > 0 ? 0.5 / (M_PI * sqrt(area / M_PI)) : 0.0;
+    return 0.5;
+}
+
+// Per cell on t^n: the point velocity's adjoint (finite_volume.hpp:2245-2306 Regularised, unless the steps are held), the
+// sound speed's, and the limited fit's with its geometry (:2629-2760): W_i's own bar, beta = M^-1 xbar, the bound's bar;
+// per list slot the bars of its between vector, weight and face offset; the angle face and the bars of its A and
+// separation; the cell's own c and x bars.
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_mcell_adjoint(Geo g, Rows r, Gas gas, int limit, int steps_fixed, double* Wbar, const double* gbar,
+                                                         const double* whbar, double* cbar, double* xbar, double* beta, double* boundbar,
+                                                         int* extreme, double* entry, int* fmax_of, double* angle) {
+    constexpr int V = D + 2, U = D + 1, ER = entry_row<D>();
+    using C = CellAt<D>;
+    using FA = FaceAt<D>;
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= g.N) return;
+    const double* mine = g.cell + (size_t)i * CELL_ROW;
+    double Wi[V], xi[D], ci[D], Wb[V], cb[D], xb[D];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        Wi[v] = mine[C::W + v];
+        Wb[v] = Wbar[(size_t)i * V + v];
+    }
+    load_vec<D>(mine + C::X, xi);
+    load_vec<D>(mine + C::C, ci);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        cb[a] = cbar[(size_t)i * D + a];
+        xb[a] = xbar[(size_t)i * D + a];
+    }
+    const int count = g.count[i], base = first_slot(g, i);
+    // the point velocity w = v + c (fraction c_s / |c|), fraction from the largest face angle
+    double csbar = 0, angle_A[D], angle_d[D];
+    int fmax = -1;
+#pragma unroll
+    for (int a = 0; a < D; ++a) angle_A[a] = angle_d[a] = 0;
+    if (!steps_fixed) {
+        double amax = 0, dmax[D];
+        int other = -1;
+#pragma unroll
+        for (int a = 0; a < D; ++a) dmax[a] = 0;
+        for (int k = 0; k < count; ++k) {
+            const int slot = base + k * SLICE;
+            const int e = g.list[slot], f = e >= 0 ? e : ~e, j = g.other[slot];
+            double xj[D], d[D];
+            load_vec<D>(g.cell + (size_t)j * CELL_ROW + C::X, xj);
+            separation<D>(g, xi, xj, d);
+            const double ang = face_angle<D>(g.face[(size_t)f * FACE_ROW + FA::MAG]) / (0.5 * sqrt(dot<D>(d, d)));
+            if (ang > amax) {
+                amax = ang;
+                fmax = f;
+                other = j;
+#pragma unroll
+                for (int a = 0; a < D; ++a) dmax[a] = d[a];
+            }
+        }
+        const double Fc = 2.25, shaping = 0.5;
+        double fraction = 0, slope = 0;
+        if (amax > 0.75 * Fc) {
+            fraction = amax > Fc ? shaping : shaping * (amax - 0.75 * Fc) / (0.25 * Fc);
+            slope = amax > Fc ? 0.0 : shaping / (0.25 * Fc);
+        }
+        double wbar[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            wbar[a] = whbar[(size_t)i * D + a];
+            Wb[1 + a] += wbar[a];
+        }
+        const double cd = sqrt(dot<D>(ci, ci)), cs = mine[C::CS];
+        if (cd > 0 && fraction > 0) {
+            const double k = fraction * cs / cd, kbar = dot<D>(wbar, ci);
+            const double fractionbar = kbar * cs / cd, cdbar = -kbar * k / cd;
+            csbar += kbar * fraction / cd;
+#pragma unroll
+            for (int a = 0; a < D; ++a) cb[a] += wbar[a] * k + ci[a] * (cdbar / cd);
+            if (slope != 0) {
+                const double abar = slope * fractionbar;
+                const double* A = g.face + (size_t)fmax * FACE_ROW + FA::A;
+                const double mag = sqrt(dot<D>(A, A)), dist = sqrt(dot<D>(dmax, dmax));
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    angle_A[a] = A[a] * (abar * face_angle_slope<D>(mag) / mag / (0.5 * dist));
+                    angle_d[a] = -dmax[a] * (abar * amax / (dist * dist));
+                    xb[a] -= angle_d[a];  // d = x_other - x_i
+                }
+            }
+        }
+        (void)other;
+    }
+    fmax_of[i] = fmax;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        angle[(size_t)i * 2 * D + a] = angle_A[a];
+        angle[(size_t)i * 2 * D + D + a] = angle_d[a];
+    }
+    if (mine[C::CS] > 0) {  // c_s = sqrt(gamma P / rho)
+        const double c2bar = csbar / (2 * mine[C::CS]);
+        Wb[D + 1] += c2bar * gas.gamma / Wi[0];
+        Wb[0] -= c2bar * gas.gamma * Wi[D + 1] / (Wi[0] * Wi[0]);
+    }
+    // the limited fit
+    double gl[V * D], g0[V * D], dg0[V * D], Lc[U][U], M0[U], icpt[V], gb[V * D];
+    Pattern<D> pat;
+    fit_cell<D, false>(g, i, nullptr, limit != 0, gl, g0, dg0, Lc, M0, pat, icpt);
+#pragma unroll
+    for (int k = 0; k < V * D; ++k) gb[k] = gbar[(size_t)i * V * D + k];
+    for (int k = 0; k < count; ++k) {
+        double* er = entry + (size_t)(base + k * SLICE) * ER;
+#pragma unroll
+        for (int q = 0; q < ER; ++q) er[q] = 0;
+    }
+    double xbv[V][U];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        double g0bar[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) g0bar[a] = gb[v * D + a];
+        double t = 0;
+        int ext = -1;
+        if (pat.last[v] >= 0 && pat.zero[v]) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) g0bar[a] = 0;
+        } else if (pat.last[v] >= 0) {
+            const double* o = pat.offset[v];
+            const double den = dot<D>(g0 + v * D, o);
+            const double sc = (pat.bound[v] - Wi[v]) / den;
+            t = dot<D>(gb + v * D, g0 + v * D) / den;
+            double* ob = entry + (size_t)(base + pat.last[v] * SLICE) * ER + D + 1;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                g0bar[a] = gb[v * D + a] * sc - o[a] * (sc * t);
+                ob[a] += -sc * t * g0[v * D + a];
+            }
+            const int ee = g.list[base + pat.extreme[v] * SLICE];
+            ext = ee >= 0 ? ee : ~ee;
+            Wb[v] -= t;
+        }
+        boundbar[(size_t)i * V + v] = t;
+        extreme[(size_t)i * V + v] = ext;
+        xbv[v][0] = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) xbv[v][1 + a] = g0bar[a];
+    }
+    double bb[V][U], x[V][U];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        double z[U];
+#pragma unroll
+        for (int p = 0; p < U; ++p) {
+            double sum = xbv[v][p];
+#pragma unroll
+            for (int m = 0; m < p; ++m) sum -= Lc[p][m] * z[m];
+            z[p] = sum / Lc[p][p];
+        }
+#pragma unroll
+        for (int p = U - 1; p >= 0; --p) {
+            double sum = z[p];
+#pragma unroll
+            for (int m = p + 1; m < U; ++m) sum -= Lc[m][p] * bb[v][m];
+            bb[v][p] = sum / Lc[p][p];
+        }
+#pragma unroll
+        for (int p = 0; p < U; ++p) {
+            beta[((size_t)i * V + v) * U + p] = bb[v][p];
+            Wb[v] -= M0[p] * bb[v][p];
+        }
+        x[v][0] = icpt[v];
+#pragma unroll
+        for (int a = 0; a < D; ++a) x[v][1 + a] = g0[v * D + a];
+    }
+    double Mbar[U][U];
+#pragma unroll
+    for (int p = 0; p < U; ++p)
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            double m = 0;
+#pragma unroll
+            for (int v = 0; v < V; ++v) m -= bb[v][p] * x[v][q];
+            Mbar[p][q] = m;
+        }
+    for (int k = 0; k < count; ++k) {
+        const int slot = base + k * SLICE;
+        const int e = g.list[slot], f = e >= 0 ? e : ~e, j = g.other[slot];
+        const double* row = g.cell + (size_t)j * CELL_ROW;
+        double xj[D], d[D], X[U], Xbar[U];
+        load_vec<D>(row + C::X, xj);
+        separation<D>(g, xi, xj, d);
+        const double w = g.face[(size_t)f * FACE_ROW + FA::MAG];
+        X[0] = 1;
+#pragma unroll
+        for (int p = 1; p < U; ++p) X[p] = d[p - 1] + row[C::C + p - 1] - ci[p - 1];
+#pragma unroll
+        for (int p = 0; p < U; ++p) Xbar[p] = 0;
+        double wbar = 0;
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const double dW = row[C::W + v] - Wi[v];
+            double xb2 = 0;
+#pragma unroll
+            for (int p = 0; p < U; ++p) xb2 += X[p] * bb[v][p];
+            wbar += xb2 * dW;
+#pragma unroll
+            for (int p = 0; p < U; ++p) Xbar[p] += w * dW * bb[v][p];
+        }
+#pragma unroll
+        for (int p = 0; p < U; ++p)
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                wbar += X[p] * Mbar[p][q] * X[q];
+                Xbar[p] += w * (Mbar[p][q] + Mbar[q][p]) * X[q];
+            }
+        double* er = entry + (size_t)slot * ER;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            er[a] = Xbar[1 + a];
+            // between = d + c_j - c_i (own parts: - c_i, - x_i); offset = x_f (+ x_o - x_i if not owned) - c_i
+            cb[a] -= Xbar[1 + a] + er[D + 1 + a];
+            xb[a] -= Xbar[1 + a];
+            if (e < 0) xb[a] -= er[D + 1 + a];
+        }
+        er[D] = wbar;
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) Wbar[(size_t)i * V + v] = Wb[v];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        cbar[(size_t)i * D + a] = cb[a];
+        xbar[(size_t)i * D + a] = xb[a];
+    }
+}
+
+// The slot of face f in cell i's list.
+__device__ __forceinline__ int slot_of(const Geo& g, int i, int f) {
+    const int count = g.count[i], base = first_slot(g, i);
+    for (int k = 0; k < count; ++k) {
+        const int e = g.list[base + k * SLICE];
+        if ((e >= 0 ? e : ~e) == f) return base + k * SLICE;
+    }
+    return base;
+}
+
+// Per cell j on t^n, in list order: W_j's shares in its neighbours' fits and bounds; the neighbours' entry rows' bars of
+// c_j and x_j (their between vectors, their offsets through x_owner, their angle face's separation); then the primitive
+// adjoint: out = qbar + Ubar / V, and V's bar.
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_mgather_adjoint(Geo g, Rows r, Gas gas, const double* qbar, const double* Wbar, const double* beta,
+                                                           const double* boundbar, const int* extreme, const double* entry, const int* fmax_of,
+                                                           const double* angle, double* cbar, double* xbar, double* Vbar, double* out) {
+    constexpr int V = D + 2, U = D + 1, ER = entry_row<D>();
+    using C = CellAt<D>;
+    const int j = blockIdx.x * BLOCK + threadIdx.x;
+    if (j >= g.N) return;
+    const double* mine = g.cell + (size_t)j * CELL_ROW;
+    double xj[D], cj[D], Wb[V], cb[D], xb[D];
+    load_vec<D>(mine + C::X, xj);
+    load_vec<D>(mine + C::C, cj);
+#pragma unroll
+    for (int v = 0; v < V; ++v) Wb[v] = Wbar[(size_t)j * V + v];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        cb[a] = cbar[(size_t)j * D + a];
+        xb[a] = xbar[(size_t)j * D + a];
+    }
+    const int count = g.count[j], base = first_slot(g, j);
+    for (int k = 0; k < count; ++k) {
+        const int slot = base + k * SLICE;
+        const int e = g.list[slot], f = e >= 0 ? e : ~e, i = g.other[slot];
+        const double* row = g.cell + (size_t)i * CELL_ROW;
+        double xi[D], d[D], X[U];
+        load_vec<D>(row + C::X, xi);
+        separation<D>(g, xi, xj, d);
+        const double w = g.face[(size_t)f * FACE_ROW + FaceAt<D>::MAG];
+        X[0] = 1;
+#pragma unroll
+        for (int p = 1; p < U; ++p) X[p] = d[p - 1] + cj[p - 1] - row[C::C + p - 1];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            double share = 0;
+#pragma unroll
+            for (int p = 0; p < U; ++p) share += w * X[p] * beta[((size_t)i * V + v) * U + p];
+            if (extreme[(size_t)i * V + v] == f) share += boundbar[(size_t)i * V + v];
+            Wb[v] += share;
+        }
+        const double* er = entry + (size_t)slot_of(g, i, f) * ER;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            cb[a] += er[a];  // i's between = x_j - x_i + c_j - c_i
+            xb[a] += er[a];
+            if (e >= 0) xb[a] += er[D + 1 + a];  // j owns f: i's offset carries x_j - x_i
+            if (fmax_of[i] == f) xb[a] += angle[(size_t)i * 2 * D + D + a];
+        }
+    }
+    const double inv = 1.0 / mine[C::VOL];
+    double Uj[V], Ub[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) Uj[v] = r.q[(size_t)j * V + v] * inv;
+    primitive_adjoint<D>(gas, Uj, Wb, Ub);
+    double vb = 0;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        out[(size_t)j * V + v] = qbar[(size_t)j * V + v] + Ub[v] * inv;
+        vb -= Ub[v] * Uj[v] * inv;
+    }
+    Vbar[j] = vb;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        cbar[(size_t)j * D + a] = cb[a];
+        xbar[(size_t)j * D + a] = xb[a];
+    }
+}
+
+// Per face on t^n: A's bar from the two entries' weight bars and the angle faces; x_f's from the two entries' offsets.
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_mfaces_bar(Geo g, const double* entry, const int* fmax_of, const double* angle, double* Ab, double* Xfb) {
+    constexpr int ER = entry_row<D>();
+    using FA = FaceAt<D>;
+    const int f = blockIdx.x * BLOCK + threadIdx.x;
+    if (f >= g.F) return;
+    const double* face = g.face + (size_t)f * FACE_ROW;
+    const int2 pair = *reinterpret_cast<const int2*>(face + FA::PAIR);
+    const double* eo = entry + (size_t)slot_of(g, pair.x, f) * ER;
+    const double* ej = entry + (size_t)slot_of(g, pair.y, f) * ER;
+    const double mag = face[FA::MAG];
+    const double wb = mag > 0 ? (eo[D] + ej[D]) / mag : 0.0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        Ab[(size_t)f * D + a] += face[FA::A + a] * wb + (fmax_of[pair.x] == f ? angle[(size_t)pair.x * 2 * D + a] : 0.0) +
+                                 (fmax_of[pair.y] == f ? angle[(size_t)pair.y * 2 * D + a] : 0.0);
+        Xfb[(size_t)f * D + a] += eo[D + 1 + a] + ej[D + 1 + a];
+    }
+}
+
 }  // namespace
 
 template <int D>
@@ -626,6 +1287,48 @@ hipError_t launch_step_adjoint(hipStream_t s, const Geo& g, const Rows& r, const
     return hipGetLastError();
 }
 
+template <int D>
+hipError_t launch_moving_pass_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, double unit, bool second,
+                                      const double* qbar, const MovingBars& b) {
+    if (g.N == 0) return hipSuccess;
+    if (g.F > 0) {
+        if (second)
+            hipLaunchKernelGGL((k_mpass_face<D, true>), grid_of(g.F), dim3(BLOCK), 0, s, g, r, gas, riemann, dt, unit, qbar, b.row, b.Ab, b.Xfb);
+        else
+            hipLaunchKernelGGL((k_mpass_face<D, false>), grid_of(g.F), dim3(BLOCK), 0, s, g, r, gas, riemann, dt, unit, qbar, b.row, b.Ab, b.Xfb);
+    }
+    if (second)
+        hipLaunchKernelGGL((k_mpass_gather<D, true>), grid_of(g.N), dim3(BLOCK), 0, s, g, dt, b.row, b.Wbar, b.gbar, b.whbar, b.c0bar, b.cbar, b.xbar);
+    else
+        hipLaunchKernelGGL((k_mpass_gather<D, false>), grid_of(g.N), dim3(BLOCK), 0, s, g, dt, b.row, b.Wbar, b.gbar, b.whbar, b.c0bar, b.cbar,
+                           b.xbar);
+    return hipGetLastError();
+}
+
+hipError_t launch_drift_adjoint(hipStream_t s, long long n, double dt, const double* xbar, double* whbar) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_drift_adjoint, grid_of(n), dim3(BLOCK), 0, s, n, dt, xbar, whbar);
+    return hipGetLastError();
+}
+
+template <int D>
+hipError_t launch_moving_cell_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, bool limit, bool steps_fixed, const double* qbar,
+                                      const MovingBars& b) {
+    if (g.N == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mcell_adjoint<D>, grid_of(g.N), dim3(BLOCK), 0, s, g, r, gas, limit ? 1 : 0, steps_fixed ? 1 : 0, b.Wbar, b.gbar, b.whbar,
+                       b.cbar, b.xbar, b.beta, b.boundbar, b.extreme, b.entry, b.fmax_of, b.angle);
+    hipLaunchKernelGGL(k_mgather_adjoint<D>, grid_of(g.N), dim3(BLOCK), 0, s, g, r, gas, qbar, b.Wbar, b.beta, b.boundbar, b.extreme, b.entry,
+                       b.fmax_of, b.angle, b.cbar, b.xbar, b.Vbar, b.out);
+    if (g.F > 0) hipLaunchKernelGGL(k_mfaces_bar<D>, grid_of(g.F), dim3(BLOCK), 0, s, g, b.entry, b.fmax_of, b.angle, b.Ab, b.Xfb);
+    return hipGetLastError();
+}
+
+template hipError_t launch_moving_pass_adjoint<2>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, double, bool, const double*,
+                                                  const MovingBars&);
+template hipError_t launch_moving_pass_adjoint<3>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, double, bool, const double*,
+                                                  const MovingBars&);
+template hipError_t launch_moving_cell_adjoint<2>(hipStream_t, const Geo&, const Rows&, const Gas&, bool, bool, const double*, const MovingBars&);
+template hipError_t launch_moving_cell_adjoint<3>(hipStream_t, const Geo&, const Rows&, const Gas&, bool, bool, const double*, const MovingBars&);
 template hipError_t launch_step_tangent<2>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&);
 template hipError_t launch_step_tangent<3>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&);
 template hipError_t launch_step_adjoint<2>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&);

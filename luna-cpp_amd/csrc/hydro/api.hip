@@ -146,6 +146,9 @@ struct luna_fv {
     // the derivatives' rows (first use): luna::Derivs
     Buf<double> d_dq, d_dW, d_dgrad, d_dflux, d_sidebar, d_bbar, d_boundbar, d_out;
     Buf<int> d_extreme;
+    // the moving step's reverse (first use)
+    Buf<double> m_row, m_Ab, m_Xfb, m_Wbar, m_gbar, m_whbar, m_c0bar, m_cbar, m_xbar, m_beta, m_boundbar, m_entry, m_angle, m_Vbar, m_out, m_qbar;
+    Buf<int> m_extreme, m_fmax;
     // the learned limiter: its features and troubled flags (recorded by each sweep) and the network copy
     Buf<unsigned short> features;
     Buf<unsigned char> troubled, net;
@@ -689,6 +692,115 @@ int luna_fv_step_adjoint(luna_fv* fv, double dt, double* qbar, int device) {
     }));
     LUNA_HIP(hipMemcpyAsync(qbar, fv->d_out.p, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     LUNA_HIP(hipStreamSynchronize(s));
+    fv->prims_ready = false;
+    fv->swept = false;
+    return LUNA_OK;
+}
+
+int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, const luna_geometry* mesh1, luna_geometry_adjoint_fn adjoint0,
+                           const void* ctx0, luna_geometry_adjoint_fn adjoint1, const void* ctx1, int steps_fixed, double* qbar, double* xbar) {
+    if (!fv || !mesh0 || !mesh1 || !adjoint0 || !adjoint1 || !qbar || !xbar) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_adjoint: null argument");
+    if (fv->opt.motion != LUNA_MOTION_MOVING || fv->opt.order != 1)
+        return fail(LUNA_ERR_LOGIC, "hydro::FiniteVolume::step_moving_adjoint: a moving mesh, order 1");
+    if (fv->opt.limiter == LUNA_LIMITER_LEARNED)
+        return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: derivatives of the face-centroid limiter or none (not the learned one)");
+    if (fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: a moving step is between begin and finish");
+    if (!(dt > 0)) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_adjoint: dt must be positive");
+    if (mesh0->cells != fv->N || mesh1->cells != fv->N) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_adjoint: both meshes must have the handle's cells");
+    const int D = fv->D, V = fv->V, N = fv->N, U = D + 1;
+    const size_t ND = static_cast<size_t>(N) * D, NV = static_cast<size_t>(N) * V;
+    hipStream_t s = fv->stream;
+    const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID;
+    const int row_width = D == 3 ? 2 * (5 + 15 + 3) + 9 : 2 * (4 + 8 + 2) + 6;
+    // t^n: its mesh and positions, W, the limited gradients and point velocities (the sweep keeps c^n), the grid steps.
+    if (int rc = luna_fv_set_geometry(fv, mesh0)) return rc;
+    LUNA_HIP(hipMemcpyAsync(fv->x.p, mesh0->positions, sizeof(double) * ND, hipMemcpyHostToDevice, s));
+    if (int rc = primitives(fv)) return rc;
+    if (int rc = sweep(fv, 1.0)) return rc;
+    {
+        const luna::Geo g = fv->geo();
+        const luna::Rows r = fv->rows();
+        LUNA_HIP(D == 3 ? luna::launch_keep_centroids<3>(s, g, r, fv->grid, dt) : luna::launch_keep_centroids<2>(s, g, r, fv->grid, dt));
+    }
+    const double unit = fv->grid.step / dt;
+    // work arrays (zeroed where summed into)
+    const size_t Fmax = static_cast<size_t>(std::max(std::max(mesh0->faces, mesh1->faces), 1));
+    LUNA_HIP(fv->m_row.reserve(Fmax * row_width));
+    LUNA_HIP(fv->m_Ab.reserve(Fmax * D));
+    LUNA_HIP(fv->m_Xfb.reserve(Fmax * D));
+    for (Buf<double>* b : {&fv->m_whbar, &fv->m_c0bar, &fv->m_cbar, &fv->m_xbar, &fv->m_angle}) LUNA_HIP(b->reserve(2 * ND));
+    LUNA_HIP(fv->m_Wbar.reserve(NV));
+    LUNA_HIP(fv->m_gbar.reserve(NV * D));
+    LUNA_HIP(fv->m_beta.reserve(NV * U));
+    LUNA_HIP(fv->m_boundbar.reserve(NV));
+    LUNA_HIP(fv->m_Vbar.reserve(N));
+    LUNA_HIP(fv->m_out.reserve(NV));
+    LUNA_HIP(fv->m_qbar.reserve(NV));
+    LUNA_HIP(fv->m_extreme.reserve(NV));
+    LUNA_HIP(fv->m_fmax.reserve(N));
+    LUNA_HIP(hipMemsetAsync(fv->m_Wbar.p, 0, sizeof(double) * NV, s));
+    LUNA_HIP(hipMemsetAsync(fv->m_gbar.p, 0, sizeof(double) * NV * D, s));
+    LUNA_HIP(hipMemsetAsync(fv->m_whbar.p, 0, sizeof(double) * ND, s));
+    LUNA_HIP(hipMemsetAsync(fv->m_c0bar.p, 0, sizeof(double) * ND, s));
+    LUNA_HIP(hipMemsetAsync(fv->m_cbar.p, 0, sizeof(double) * ND, s));
+    LUNA_HIP(hipMemcpyAsync(fv->m_qbar.p, qbar, sizeof(double) * NV, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->m_xbar.p, xbar, sizeof(double) * ND, hipMemcpyHostToDevice, s));
+    auto bars = [&]() {
+        return luna::MovingBars{fv->m_row.p, fv->m_Ab.p, fv->m_Xfb.p, fv->m_Wbar.p, fv->m_gbar.p, fv->m_whbar.p, fv->m_c0bar.p, fv->m_cbar.p,
+                                fv->m_xbar.p, fv->m_beta.p, fv->m_boundbar.p, nullptr, fv->m_angle.p, fv->m_Vbar.p, fv->m_out.p,
+                                fv->m_extreme.p, fv->m_fmax.p};
+    };
+    const luna::Gas gas = fv->gas();
+    // Pass 2 on t^n+1 (the rows W^n, g^n, c^n and the steps stay), then x^n+1's bar with its geometry's adjoint.
+    fv->begun = true;  // keeps W, g across the upload
+    int rc = upload(fv, mesh1);
+    fv->begun = false;
+    if (rc != LUNA_OK) return rc;
+    {
+        const luna::Geo g = fv->geo();
+        const luna::Rows r = fv->rows();
+        const luna::MovingBars b = bars();
+        LUNA_HIP(D == 3 ? luna::launch_moving_pass_adjoint<3>(s, g, r, gas, fv->opt.riemann, dt, unit, true, fv->m_qbar.p, b)
+                        : luna::launch_moving_pass_adjoint<2>(s, g, r, gas, fv->opt.riemann, dt, unit, true, fv->m_qbar.p, b));
+    }
+    const size_t F1 = static_cast<size_t>(mesh1->faces), F0 = static_cast<size_t>(mesh0->faces);
+    std::vector<double> hV(N, 0.0), hc(ND), hA(Fmax * D), hxf(Fmax * D), hx(ND);
+    LUNA_HIP(hipMemcpyAsync(hc.data(), fv->m_cbar.p, sizeof(double) * ND, hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipMemcpyAsync(hA.data(), fv->m_Ab.p, sizeof(double) * F1 * D, hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipMemcpyAsync(hxf.data(), fv->m_Xfb.p, sizeof(double) * F1 * D, hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipMemcpyAsync(hx.data(), fv->m_xbar.p, sizeof(double) * ND, hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipStreamSynchronize(s));
+    if (adjoint1(ctx1, hV.data(), hc.data(), hA.data(), hxf.data(), hx.data()) != 0)
+        return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_adjoint: the geometry adjoint of the mesh of t^n+1 failed");
+    // Back to t^n: x^n's bar starts as x^n+1's (x^n+1 = x^n + dt w-hat), w-hat's gets dt of it unless the steps are held.
+    fv->begun = true;
+    rc = upload(fv, mesh0);
+    fv->begun = false;
+    if (rc != LUNA_OK) return rc;
+    LUNA_HIP(hipMemcpyAsync(fv->m_xbar.p, hx.data(), sizeof(double) * ND, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->m_cbar.p, fv->m_c0bar.p, sizeof(double) * ND, hipMemcpyDeviceToDevice, s));
+    if (!steps_fixed) LUNA_HIP(luna::launch_drift_adjoint(s, static_cast<long long>(ND), dt, fv->m_xbar.p, fv->m_whbar.p));
+    LUNA_HIP(fv->m_entry.reserve(static_cast<size_t>(fv->slots > 0 ? fv->slots : 1) * (2 * D + 1)));
+    {
+        const luna::Geo g = fv->geo();
+        const luna::Rows r = fv->rows();
+        luna::MovingBars b = bars();
+        b.entry = fv->m_entry.p;
+        LUNA_HIP(D == 3 ? luna::launch_moving_pass_adjoint<3>(s, g, r, gas, fv->opt.riemann, dt, unit, false, fv->m_qbar.p, b)
+                        : luna::launch_moving_pass_adjoint<2>(s, g, r, gas, fv->opt.riemann, dt, unit, false, fv->m_qbar.p, b));
+        LUNA_HIP(D == 3 ? luna::launch_moving_cell_adjoint<3>(s, g, r, gas, limit, steps_fixed != 0, fv->m_qbar.p, b)
+                        : luna::launch_moving_cell_adjoint<2>(s, g, r, gas, limit, steps_fixed != 0, fv->m_qbar.p, b));
+    }
+    LUNA_HIP(hipMemcpyAsync(hV.data(), fv->m_Vbar.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipMemcpyAsync(hc.data(), fv->m_cbar.p, sizeof(double) * ND, hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipMemcpyAsync(hA.data(), fv->m_Ab.p, sizeof(double) * F0 * D, hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipMemcpyAsync(hxf.data(), fv->m_Xfb.p, sizeof(double) * F0 * D, hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipMemcpyAsync(hx.data(), fv->m_xbar.p, sizeof(double) * ND, hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipMemcpyAsync(qbar, fv->m_out.p, sizeof(double) * NV, hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipStreamSynchronize(s));
+    if (adjoint0(ctx0, hV.data(), hc.data(), hA.data(), hxf.data(), hx.data()) != 0)
+        return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_adjoint: the geometry adjoint of the mesh of t^n failed");
+    std::memcpy(xbar, hx.data(), sizeof(double) * ND);
     fv->prims_ready = false;
     fv->swept = false;
     return LUNA_OK;

@@ -118,6 +118,27 @@ hipError_t launch_step_tangent(hipStream_t s, const Geo& g, const Rows& r, const
 template <int D>
 hipError_t launch_step_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, bool limit, const Derivs& w);
 
+// The moving step's reverse (order 1, fixed topology): device work arrays. Per face of the pass's mesh: row
+// (pass_row), Ab, Xfb (A's and x_f's bars); per cell: W, g, w-hat, c^n bars (summed over both passes), the mesh's c and x
+// bars, the fit's beta, bound bar and extreme face, V's bar and the Q bar out; per slot the entry rows; per cell the
+// angle face and its bars.
+struct MovingBars {
+    double *row, *Ab, *Xfb, *Wbar, *gbar, *whbar, *c0bar, *cbar, *xbar, *beta, *boundbar, *entry, *angle, *Vbar, *out;
+    int *extreme, *fmax_of;
+};
+// One pass on the mesh the Geo holds (second: t^n+1): the faces' rows and A, x_f bars (written), then the cells' gathers
+// (added to W, g, w-hat, c^n, c and x bars).
+template <int D>
+hipError_t launch_moving_pass_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, double unit, bool second,
+                                      const double* qbar, const MovingBars& b);
+// w-hat bar += dt x bar (n entries).
+hipError_t launch_drift_adjoint(hipStream_t s, long long n, double dt, const double* xbar, double* whbar);
+// On t^n: the point velocity's and the fit's adjoints per cell, the neighbours' gather with the primitive adjoint (out, V
+// bar), the faces' A and x_f bars (added).
+template <int D>
+hipError_t launch_moving_cell_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, bool limit, bool steps_fixed, const double* qbar,
+                                      const MovingBars& b);
+
 // D = 1: the sorted line. Slots in key order; Q (and on a moving line the positions) by cell. Walls: 0 reflective,
 // 1 outflow, 2 inflow (a moving line's open wall holding the state inflow[side]).
 struct LineArgs {

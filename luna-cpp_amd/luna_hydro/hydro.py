@@ -108,7 +108,7 @@ def exported_symbols():
              "luna_fv1_create_moving", "luna_fv1_cells", "luna_fv1_positions", "luna_fv1_line", "luna_fv1_set_inflow_spacing",
              "luna_fv1_mesh_stats",
              "luna_last_error",
-             "luna_fv_set_geometry", "luna_synchronize", "luna_fv_step_tangent", "luna_fv_step_adjoint",
+             "luna_fv_set_geometry", "luna_synchronize", "luna_fv_step_tangent", "luna_fv_step_adjoint", "luna_fv_moving_adjoint",
              "luna_fv_set_limiter_network", "luna_fv_limiter_features", "luna_mlp_create", "luna_mlp_destroy", "luna_mlp_set_params",
              "luna_mlp_get_params", "luna_mlp_forward", "luna_mlp_gradient", "luna_mlp_train_step", "luna_mlp_synchronize"]
     return {n: hasattr(lib(), n) for n in names}
@@ -186,6 +186,28 @@ class FiniteVolume:
     def step_adjoint(self, dt, qbar):
         """FiniteVolume::step_adjoint(): J^T qbar of one static step from the current Q (which stays); qbar as step_tangent's dq."""
         return self._derivative(lib().luna_fv_step_adjoint, dt, qbar, "step_adjoint")
+
+    def moving_adjoint(self, dt, mesh0, mesh1, qbar, xbar, steps_fixed=False):
+        """FiniteVolume::step_moving_adjoint() at order 1 (luna_fv_moving_adjoint): the handle's Q is the state at t^n, mesh0
+        the mesh of t^n, mesh1 the mesh the forward step reached; qbar (N x V) and xbar (N x D), the bars after the step,
+        -> those before it, with this build's mesh library's geometry derivatives of both meshes. Leaves the handle on
+        mesh0."""
+        from . import mesh as mesh_lib
+
+        g0, a0 = geometry_record(mesh0)
+        g1, a1 = geometry_record(mesh1)
+        d0, d1 = mesh_lib.GeometryDerivative(mesh0), mesh_lib.GeometryDerivative(mesh1)
+        fn = ctypes.cast(mesh_lib.lib().luna_mesh_derivative_adjoint, ctypes.c_void_p)
+        qb = np.ascontiguousarray(np.array(qbar, dtype=np.float64).reshape(self.N, self.V))
+        xb = np.ascontiguousarray(np.array(xbar, dtype=np.float64).reshape(self.N, self.D))
+        try:
+            _check(lib().luna_fv_moving_adjoint(self._h, ctypes.c_double(dt), ctypes.byref(g0), ctypes.byref(g1), fn, d0._h, fn, d1._h,
+                                                int(steps_fixed), qb.ctypes.data_as(_dp), xb.ctypes.data_as(_dp)), "moving_adjoint")
+        finally:
+            d0.close()
+            d1.close()
+        self._mesh_arrays = a0
+        return qb, xb
 
     def _derivative(self, fn, dt, rows, what):
         if hasattr(rows, "data_ptr"):
