@@ -91,7 +91,8 @@ int luna_fv_moving_begin(luna_fv* fv, double dt, double* positions);
 int luna_fv_moving_finish(luna_fv* fv, const luna_geometry* mesh, double dt);
 
 /* Uploads a mesh. Between moving_begin and moving_finish: the mesh of the new positions. Otherwise: a caller moved the
- * mesh itself (mesh_moved(), finite_volume.hpp:152-156), and the cached rows are dropped. */
+ * mesh itself (mesh_moved(), finite_volume.hpp:152-156), and the cached rows are dropped; a moving solver's positions
+ * become the mesh's (a restart from a checkpoint of Q and x). */
 int luna_fv_set_geometry(luna_fv* fv, const luna_geometry* mesh);
 
 /* fallbacks() (finite_volume.hpp:230-238): face sides of the last step that took the cell's state. Fences. */

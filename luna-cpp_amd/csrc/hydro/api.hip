@@ -713,8 +713,7 @@ int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, c
     const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID;
     const int row_width = D == 3 ? 2 * (5 + 15 + 3) + 9 : 2 * (4 + 8 + 2) + 6;
     // t^n: its mesh and positions, W, the limited gradients and point velocities (the sweep keeps c^n), the grid steps.
-    if (int rc = luna_fv_set_geometry(fv, mesh0)) return rc;
-    LUNA_HIP(hipMemcpyAsync(fv->x.p, mesh0->positions, sizeof(double) * ND, hipMemcpyHostToDevice, s));
+    if (int rc = luna_fv_set_geometry(fv, mesh0)) return rc;  // and x^n
     if (int rc = primitives(fv)) return rc;
     if (int rc = sweep(fv, 1.0)) return rc;
     {
@@ -834,9 +833,13 @@ int luna_fv_set_geometry(luna_fv* fv, const luna_geometry* mesh) {
     if (int rc = upload(fv, mesh)) return rc;
     if (fv->begun) {
         fv->new_mesh = true;
-    } else {  // mesh_moved() (finite_volume.hpp:152-156): the cached rows go
+    } else {  // mesh_moved() (finite_volume.hpp:152-156): the cached rows go; a moving solver takes the mesh's positions
         fv->prims_ready = false;
         fv->swept = false;
+        if (fv->opt.motion == LUNA_MOTION_MOVING) {
+            LUNA_HIP(hipMemcpyAsync(fv->x.p, mesh->positions, sizeof(double) * fv->N * fv->D, hipMemcpyHostToDevice, fv->stream));
+            LUNA_HIP(hipStreamSynchronize(fv->stream));
+        }
     }
     return LUNA_OK;
 }

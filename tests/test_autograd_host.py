@@ -71,3 +71,56 @@ def test_checkpoint_schedule():
     assert autograd.checkpoints(5, 20) == [0]
     with pytest.raises(ValueError):
         autograd.checkpoints(5, 0)
+
+
+class PairSolver:
+    """A stand-in moving solver: the state is (q, x); step: q <- A q + B x, x <- x + C q (matrices by dt); adjoint: the
+    transpose."""
+
+    def __init__(self, n, seed=3):
+        self.g = torch.Generator().manual_seed(seed)
+        self.n, self.mats, self.state_ = n, {}, None
+
+    def M(self, dt):
+        if dt not in self.mats:
+            n = self.n
+            r = lambda: 0.1 * torch.randn(n, n, generator=self.g, dtype=torch.float64)  # noqa: E731
+            self.mats[dt] = (torch.eye(n, dtype=torch.float64) + r(), r(), r())
+        return self.mats[dt]
+
+    def set_state(self, state):
+        self.state_ = (state[0].detach().clone(), state[1].detach().clone())
+
+    def state(self):
+        return (self.state_[0].clone(), self.state_[1].clone())
+
+    def step(self, dt):
+        A, B, C = self.M(dt)
+        q, x = self.state_
+        self.state_ = (A @ q + B @ x, x + C @ q)
+
+    def adjoint(self, dt, bars):
+        A, B, C = self.M(dt)
+        qb, xb = bars
+        return (A.T @ qb + C.T @ xb, B.T @ qb + xb)
+
+
+@pytest.mark.parametrize("every", [1, 2, 5, 9])
+def test_moving_rollout_gradient_matches_autograd(every):
+    """moving_rollout's checkpointed backward over a pair state equals plain autograd through the same maps."""
+    n, steps = 5, 9
+    solver = PairSolver(n)
+    dts = [0.1 * (k % 3 + 1) for k in range(steps)]
+    gen = torch.Generator().manual_seed(4)
+    q0 = torch.randn(n, 2, dtype=torch.float64, generator=gen, requires_grad=True)
+    x0 = torch.randn(n, 2, dtype=torch.float64, generator=gen, requires_grad=True)
+    wq, wx = torch.randn(n, 2, dtype=torch.float64, generator=gen), torch.randn(n, 2, dtype=torch.float64, generator=gen)
+    qT, xT = autograd.moving_rollout(q0, x0, solver, dts, every)
+    ((wq * qT).sum() + (wx * xT).sum()).backward()
+    rq, rx = q0.detach().clone().requires_grad_(True), x0.detach().clone().requires_grad_(True)
+    q, x = rq, rx
+    for dt in dts:
+        A, B, C = solver.M(dt)
+        q, x = A @ q + B @ x, x + C @ q
+    ((wq * q).sum() + (wx * x).sum()).backward()
+    assert torch.allclose(q0.grad, rq.grad, rtol=1e-13, atol=1e-13) and torch.allclose(x0.grad, rx.grad, rtol=1e-13, atol=1e-13)

@@ -68,3 +68,118 @@ def test_moving_adjoint_uniform_cotangent():
         qb, xb = fv.moving_adjoint(dt, m0, m1, qbar, np.zeros((N, 3)))
         assert np.array_equal(qb, qbar) and not np.any(xb)
     fv.close()
+
+
+def _oracle_run(q0, x0, dts, held=None, riemann="hllc"):
+    """The oracle's moving steps from (Q, x): per step its input Q and mesh, the steps taken; the final (Q, x, mesh)."""
+    m = mesh.build(x0, 1.0)
+    fvo = oracle.FiniteVolume(m, GAMMA, moving=True, riemann=riemann)
+    fvo.set_state(q0)
+    qs, meshes, steps = [], [m], []
+    for k, dt in enumerate(dts):
+        qs.append(fvo.q.copy())
+        fvo.time_step(0.3)
+        if held is not None:
+            fvo.c.hold_steps = 1
+            fvo.steps[...] = held[k]
+        x1 = fvo.moving_begin(dt)
+        steps.append(fvo.steps.copy())
+        m1 = mesh.build(x1, 1.0)
+        fvo.moving_finish(m1, dt)
+        meshes.append(m1)
+    return fvo.q.copy(), meshes, qs, steps
+
+
+def _misfit(q, m, target):
+    rho = q[:, 0] / m["volume"]
+    return 0.5 * np.sum(m["volume"] * (rho - target) ** 2)
+
+
+def _rollout_gradient(fv, q0, x0, dts, target, fixed, every):
+    torch = pytest.importorskip("torch")
+    from luna_hydro import autograd
+
+    solver = autograd.MovingSolver(fv, steps_fixed=fixed)
+    qt = torch.tensor(q0, requires_grad=True)
+    xt = torch.tensor(x0, requires_grad=True)
+    qTt, xTt = autograd.moving_rollout(qt, xt, solver, dts, every)
+    autograd.density_misfit(qTt, xTt, torch.tensor(target)).backward()
+    return qTt.detach().numpy(), qt.grad.numpy(), xt.grad.numpy()
+
+
+def _fd(q0, x0, dts, steps, target, riemann, gq, gx, rng, step, trials=3):
+    worst = 0.0
+    for s in range(trials):
+        dq = rng.normal(size=q0.shape) * np.abs(q0) * 1e-7
+        dx = rng.integers(-100, 101, size=x0.shape).astype(float) * step * (s > 0)
+        L = []
+        for sign in (1, -1):
+            q, ms, _, _ = _oracle_run(q0 + sign * dq, x0 + sign * dx, dts, held=steps, riemann=riemann)
+            L.append(_misfit(q, ms[-1], target))
+        fd = (L[0] - L[1]) / 2
+        worst = max(worst, abs(fd - (np.sum(gq * dq) + np.sum(gx * dx))) / abs(fd))
+    return worst
+
+
+def test_moving_rollout_acoustic_wave():
+    """The moving step's rollout through torch.autograd on a small case of the smooth acoustic wave (rho = 1 + 1e-3
+    sin(2 pi (x + y)), p = 3/5, gamma 5/3, HLLC; jittered 6^3 moving mesh, 9 steps, checkpoints every 4), loss the density
+    misfit against a seeded perturbed target: the gradient in (Q0, x0) equals the oracle's chain of step adjoints (steps
+    free; the rollout's own steps equal the oracle's bit for bit), and checkpointing changes nothing. Central differences
+    are reported: on this smooth state a perturbation crosses the limiter's ties (p uniform, v = 0: every clip at its kink)
+    and, once the faces move with the flow, HLLC's contact switch at S* = w, as the reference's own multi-step HLLC check
+    reports (hydro_derivatives_test.cpp:1641-1643); the gated check is the next test's."""
+    rng = np.random.default_rng(9)
+    m0 = mesh.build(problems.jittered_lattice(6, 3, 0.25, 9), 1.0)
+    q0, x0 = problems.acoustic_state(m0), m0["positions"].copy()
+    fv = hydro.FiniteVolume(m0, GAMMA, motion="moving")
+    fv.set_state(q0)
+    dts = [0.5 * fv.time_step(0.3)] * 9
+    qT, meshes, qs, steps = _oracle_run(q0, x0, dts)
+    target = qT[:, 0] / meshes[-1]["volume"] * (1 + 1e-3 * rng.uniform(-1, 1, len(qT)))
+    qT4, gq, gx = _rollout_gradient(fv, q0, x0, dts, target, False, 4)
+    qT1, gq1, gx1 = _rollout_gradient(fv, q0, x0, dts, target, False, 1)
+    assert np.array_equal(qT4, qT) and np.array_equal(qT1, qT)
+    assert np.array_equal(gq, gq1) and np.array_equal(gx, gx1)
+    mT = meshes[-1]
+    rho = qT[:, 0] / mT["volume"]
+    qbar = np.zeros_like(qT)
+    qbar[:, 0] = rho - target
+    gT = mesh.GeometryDerivative(mT)
+    xbar = gT.adjoint(0.5 * (target ** 2 - rho ** 2), np.zeros((len(qT), 3)), np.zeros((mT["faces"], 3)), np.zeros((mT["faces"], 3)))
+    for k in reversed(range(len(dts))):
+        qbar, xbar = _oracle_adjoint(meshes[k], meshes[k + 1], qs[k], dts[k], qbar, xbar, False, "hllc")
+    err_q = np.max(np.abs(gq - qbar)) / np.max(np.abs(qbar))
+    err_x = np.max(np.abs(gx - xbar)) / np.max(np.abs(xbar))
+    print("moving rollout (acoustic wave, 9 steps): GPU vs the oracle's chain: Q %.2e, x %.2e" % (err_q, err_x), flush=True)
+    assert err_q < 1e-10 and err_x < 1e-10
+    _, hq, hx = _rollout_gradient(fv, q0, x0, dts, target, True, 4)
+    worst = _fd(q0, x0, dts, steps, target, "hllc", hq, hx, rng, m0["step"], trials=2)
+    print("moving rollout (acoustic wave), steps held: |FD - adjoint| / |FD| %.2e (reported)" % worst, flush=True)
+    assert worst < 0.1
+    fv.close()
+
+
+@pytest.mark.parametrize("steps_n", [1, 2])
+def test_moving_rollout_central_differences(steps_n):
+    """The rollout's gradient (HLL, the grid steps held, every step a checkpoint) against central differences of the
+    oracle's forward with the same steps held, on a rough random state (no tie within reach of the perturbation), along Q
+    directions and whole-grid-step position directions: 1e-6."""
+    rng = np.random.default_rng(21)
+    m0 = mesh.build(problems.jittered_lattice(6, 3, 0.25, 21), 1.0)
+    N = m0["cells"]
+    W = np.zeros((N, 5))
+    W[:, 0] = 1 + 0.2 * rng.uniform(-1, 1, N)
+    W[:, 1:4] = 0.2 * rng.uniform(-1, 1, (N, 3))
+    W[:, 4] = 0.6 * (1 + 0.2 * rng.uniform(-1, 1, N))
+    q0, x0 = problems.conserved_rows(W, GAMMA) * m0["volume"][:, None], m0["positions"].copy()
+    fv = hydro.FiniteVolume(m0, GAMMA, riemann="hll", motion="moving")
+    fv.set_state(q0)
+    dts = [0.5 * fv.time_step(0.3)] * steps_n
+    qT, meshes, qs, steps = _oracle_run(q0, x0, dts, riemann="hll")
+    target = qT[:, 0] / meshes[-1]["volume"] * (1 + 1e-3 * rng.uniform(-1, 1, N))
+    _, gq, gx = _rollout_gradient(fv, q0, x0, dts, target, True, 1)
+    worst = _fd(q0, x0, dts, steps, target, "hll", gq, gx, rng, m0["step"])
+    print("moving rollout (HLL, %d steps, steps held): |FD - adjoint| / |FD| %.2e" % (steps_n, worst), flush=True)
+    assert worst < 1e-6
+    fv.close()
