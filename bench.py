@@ -69,8 +69,8 @@ def dominant(times, table):
     return best, launches, total / launches
 
 
-def run_static(m, q, steps, warmup):
-    fv = hydro.FiniteVolume(m, GAMMA)
+def run_static(m, q, steps, warmup, precision="double"):
+    fv = hydro.FiniteVolume(m, GAMMA, precision=precision)
     fv.set_state(q)
     for _ in range(warmup):
         fv.step(fv.time_step(COURANT))
@@ -92,8 +92,8 @@ def run_static(m, q, steps, warmup):
     return total, walls, times
 
 
-def run_moving(m, q, steps, warmup):
-    fv = hydro.FiniteVolume(m, GAMMA, motion="moving")
+def run_moving(m, q, steps, warmup, precision="double"):
+    fv = hydro.FiniteVolume(m, GAMMA, motion="moving", precision=precision)
     fv.set_state(q)
     mesh_s, hydro_s, walls = [], [], []
     current = m
@@ -203,6 +203,7 @@ def main():
     ap.add_argument("--static-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-limiter comparison")
+    ap.add_argument("--no-precision", action="store_true", help="skip the grad32/grad16 policy runs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (scale of the rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -247,6 +248,21 @@ def main():
         cpu = cpu_baseline(m, q)
         log("cpu: %.3g cell updates/s" % cpu["value"])
 
+    policies = None
+    if not args.no_precision:
+        # The reference's reduced-precision policies (its published A100 numbers are grad16, its 3D default): the same
+        # mesh and state, static and moving, reported beside the FP64 headline.
+        policies = {}
+        for p in ("grad32", "grad16"):
+            t_s, w_s, k_s = run_static(m, q, args.static_steps, args.warmup, p)
+            t_m, h_m, _, k_m = run_moving(m, q, max(2, args.steps // 2), args.warmup, p)
+            h_total = float(np.sum(h_m))
+            policies[p] = {"static": N * args.static_steps / t_s, "static_ms_per_step": 1e3 * t_s / args.static_steps,
+                           "moving": N * len(h_m) / h_total, "moving_hydro_ms_per_step": 1e3 * h_total / len(h_m),
+                           "static_kernels_ms_avg": {k: v[1] / v[0] for k, v in k_s.items()},
+                           "moving_kernels_ms_avg": {k: v[1] / v[0] for k, v in k_m.items()}}
+            log("%s: static %.3g, moving %.3g cell updates/s" % (p, policies[p]["static"], policies[p]["moving"]))
+
     learned_cmp = None
     if not args.no_learned:
         log("learned limiter vs plain (smooth KH 64^2, reference 256^2)")
@@ -274,6 +290,7 @@ def main():
                      "algorithmic_bytes_per_cell": per_cell, "avg_ms": avg_ms, "launches": launches},
         "cpu_baseline": cpu,
         "learned_vs_plain": learned_cmp,
+        "precision_policies": policies,
         "step_roofline": {"algorithmic_bytes_per_cell": step_bytes,
                           "achieved_GBs": step_bytes * N * args.steps / hydro_total / 1e9,
                           "frac": step_bytes * N * args.steps / hydro_total / 1e9 / HBM_PEAK_GBS},

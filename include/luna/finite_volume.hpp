@@ -32,8 +32,10 @@ enum class Limiter { face_centroid = LUNA_LIMITER_FACE_CENTROID, none = LUNA_LIM
 enum class Motion { static_mesh = LUNA_MOTION_STATIC, moving = LUNA_MOTION_MOVING };
 enum class Riemann { hllc = LUNA_RIEMANN_HLLC, hll = LUNA_RIEMANN_HLL };
 enum class StepRule { arepo = LUNA_STEP_AREPO, luna = LUNA_STEP_LUNA };
+// The gradient slot of hydro::Precision (precision.hpp:41-54): Precision<>, Grad32, Grad16. A run-time choice here.
+enum class Precision { double_ = LUNA_PRECISION_DOUBLE, grad32 = LUNA_PRECISION_GRAD32, grad16 = LUNA_PRECISION_GRAD16 };
 
-// hydro::FiniteVolume<D> for D = 2, 3 (Update::per_face, Precision<> = FP64 throughout).
+// hydro::FiniteVolume<D> for D = 2, 3 (Update::per_face; the precision policy a constructor argument, double by default).
 template <int D>
 class FiniteVolume {
 public:
@@ -43,7 +45,7 @@ public:
     // FiniteVolume(exec, mesh, gas, order, update, limiter, motion) (finite_volume.hpp:79-107); mesh: the voronoi-cpp
     // output contract as luna_geometry.
     FiniteVolume(const luna_geometry& mesh, const Gas& gas, int order, Limiter limiter = Limiter::face_centroid,
-                 Motion motion = Motion::static_mesh, Riemann riemann = Riemann::hllc)
+                 Motion motion = Motion::static_mesh, Riemann riemann = Riemann::hllc, Precision precision = Precision::double_)
         : n_(mesh.cells) {
         luna_fv_options o{};
         o.dim = D;
@@ -55,6 +57,7 @@ public:
         o.gamma = gas.gamma;
         o.smallr = gas.smallr;
         o.smallc = gas.smallc;
+        o.precision = static_cast<int>(precision);
         options_ = o;
         check(luna_fv_create(&o, &mesh, &fv_), "hydro::FiniteVolume");
     }

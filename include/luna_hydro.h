@@ -28,6 +28,11 @@ enum { LUNA_RIEMANN_HLLC = 0, LUNA_RIEMANN_HLL = 1 };
 enum { LUNA_MOTION_STATIC = 0, LUNA_MOTION_MOVING = 1 };
 enum { LUNA_STEP_AREPO = 0, LUNA_STEP_LUNA = 1 };
 enum { LUNA_WALL_REFLECTIVE = 0, LUNA_WALL_OUTFLOW = 1, LUNA_WALL_INFLOW = 2 }; /* hydro::Wall, finite_volume_1d.hpp:9 */
+/* The gradient slot of hydro::Precision (precision.hpp:26-54): double (Precision<>), grad32 (Grad32: the fit, the limiter
+ * and the face-state change in float, float rows), grad16 (Grad16: the same arithmetic, rows of FP16 with one
+ * power-of-two exponent per group rho, v, P). Q, W, volumes, centroids, the Riemann solve and the flux sums stay double
+ * in every policy. */
+enum { LUNA_PRECISION_DOUBLE = 0, LUNA_PRECISION_GRAD32 = 1, LUNA_PRECISION_GRAD16 = 2 };
 
 /* hydro::Gas (euler.hpp:82-101) and the constructor's options (finite_volume.hpp:79-80). The update loop is the
  * reference's per-face one (Update::per_face): flux rows, then a deterministic per-cell gather in list order. */
@@ -39,6 +44,10 @@ typedef struct {
     int motion;    /* LUNA_MOTION_STATIC or LUNA_MOTION_MOVING (order 1) */
     int step_rule; /* LUNA_STEP_AREPO, or LUNA_STEP_LUNA in 2D (set_step_rule, finite_volume.hpp:181) */
     double gamma, smallr, smallc;
+    int precision; /* LUNA_PRECISION_*; a zero-initialised struct asks for double. The reduced policies step forward
+                      (static and moving, order 1) with the face-centroid limiter or none; the learned limiter and the
+                      derivatives need double. The reference's default (DefaultPrecision<D>, precision.hpp:49-54) is
+                      grad16 in 3D and grad32 in 2D; here the default is double. */
 } luna_fv_options;
 
 /* The mesh arrays the step reads, host pointers: voronoi-cpp's output contract as finite_volume.hpp:257-327 and :353-364
@@ -103,7 +112,8 @@ int luna_fv_positions(luna_fv* fv, double* x);
 int luna_fv_point_velocities(luna_fv* fv, double* w);
 
 /* Host copy of the limited gradients of the last time_step(), N x (dim + 2) x dim (density_gradients(),
- * finite_volume.hpp:165-180, gives the first dim of each row). */
+ * finite_volume.hpp:165-180, gives the first dim of each row); under grad32/grad16 the stored rows, decoded (FP16 times
+ * 2^-k per group, FaceFlux::gradient(), finite_volume.hpp:1288-1300) and widened to double. */
 int luna_fv_gradients(luna_fv* fv, double* g);
 
 /* Host copy of the primitive rows the last sweep read (or the next one will), N x (dim + 4): W, sound speed, radius

@@ -127,6 +127,14 @@ hipError_t timed(Profiler& prof, hipStream_t s, const char* name, F&& launch) {
     return hipSuccess;
 }
 
+// IEEE binary16 bits -> float (exact): subnormals m 2^-24, normals (1024 + m) 2^(e - 25).
+float half_value(unsigned short h) {
+    const int e = (h >> 10) & 0x1f, m = h & 0x3ff;
+    const float sign = (h & 0x8000) ? -1.0f : 1.0f;
+    if (e == 0x1f) return m ? NAN : sign * INFINITY;
+    return sign * (e == 0 ? std::ldexp(static_cast<float>(m), -24) : std::ldexp(static_cast<float>(1024 + m), e - 25));
+}
+
 double bits_to_double(unsigned long long b) {
     double d;
     std::memcpy(&d, &b, sizeof d);
@@ -142,6 +150,7 @@ struct luna_fv {
     Buf<int> count, slice, list, other, steps;
     Buf<double> cell, facerec, slotgeo;
     Buf<double> q, grad, w, x, old_centroid, flux;
+    Buf<unsigned int> exponents;  // grad16: the per-cell group exponents (int8 x 3)
     Buf<unsigned long long> counters;  // [0, 1] dt slots, [2, 3] fallbacks of the step's two flux passes
     // the derivatives' rows (first use): luna::Derivs
     Buf<double> d_dq, d_dW, d_dgrad, d_dflux, d_sidebar, d_bbar, d_boundbar, d_out;
@@ -188,7 +197,7 @@ struct luna_fv {
     }
     int grad_row() const { return D == 3 ? luna::grad_row<3>() : luna::grad_row<2>(); }
     luna::Rows rows() const {
-        return luna::Rows{q.p, grad.p, w.p, x.p, old_centroid.p, steps.p, flux.p, features.p, troubled.p};
+        return luna::Rows{q.p, grad.p, w.p, x.p, old_centroid.p, steps.p, flux.p, features.p, troubled.p, opt.precision, exponents.p};
     }
     luna::Gas gas() const { return luna::Gas{opt.gamma, opt.smallr, opt.smallc, 1.0 / (opt.gamma - 1.0)}; }
     luna::Counters counters_now() const {
@@ -503,6 +512,10 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
     if (opt->motion == LUNA_MOTION_MOVING && opt->order != 1)
         return fail(LUNA_ERR_ARGUMENT, "luna: a moving mesh needs order 1 (the two-pass MUSCL-Hancock step)");
     if (opt->step_rule == LUNA_STEP_LUNA && opt->dim != 2) return fail(LUNA_ERR_ARGUMENT, "StepRule::luna: 2D");
+    if (opt->precision < LUNA_PRECISION_DOUBLE || opt->precision > LUNA_PRECISION_GRAD16)
+        return fail(LUNA_ERR_ARGUMENT, "hydro::Precision: double, grad32 or grad16");
+    if (opt->precision != LUNA_PRECISION_DOUBLE && opt->limiter == LUNA_LIMITER_LEARNED)
+        return fail(LUNA_ERR_ARGUMENT, "luna: the learned limiter's features need the double policy");
     if (!(opt->gamma > 1)) return fail(LUNA_ERR_ARGUMENT, "luna: gamma must exceed 1");
     if (mesh->cells <= 0) return fail(LUNA_ERR_ARGUMENT, "luna: no cells");
     luna_fv* fv = new luna_fv;
@@ -527,6 +540,8 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
             (e = fv->net.reserve(NET_BYTES)) != hipSuccess)
             return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
     }
+    if (opt->precision == LUNA_PRECISION_GRAD16 && (e = fv->exponents.reserve(N)) != hipSuccess)
+        return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
     if (opt->motion == LUNA_MOTION_MOVING) {
         if ((e = fv->x.reserve(N * D)) != hipSuccess || (e = fv->old_centroid.reserve(N * D)) != hipSuccess ||
             (e = fv->steps.reserve(N * D)) != hipSuccess)
@@ -630,6 +645,7 @@ int derivatives_ready(luna_fv* fv, const char* what) {
         return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of a static order-1 step");
     if (fv->opt.limiter == LUNA_LIMITER_LEARNED)
         return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of the face-centroid limiter or none (not the learned one)");
+    if (fv->opt.precision != LUNA_PRECISION_DOUBLE) return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of the double policy");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, std::string(what) + ": a moving step is between begin and finish");
     const size_t N = fv->N, V = fv->V, F = fv->F;
     LUNA_HIP(fv->d_dq.reserve(N * V));
@@ -704,6 +720,7 @@ int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, c
         return fail(LUNA_ERR_LOGIC, "hydro::FiniteVolume::step_moving_adjoint: a moving mesh, order 1");
     if (fv->opt.limiter == LUNA_LIMITER_LEARNED)
         return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: derivatives of the face-centroid limiter or none (not the learned one)");
+    if (fv->opt.precision != LUNA_PRECISION_DOUBLE) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: derivatives of the double policy");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: a moving step is between begin and finish");
     if (!(dt > 0)) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_adjoint: dt must be positive");
     if (mesh0->cells != fv->N || mesh1->cells != fv->N) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_adjoint: both meshes must have the handle's cells");
@@ -896,9 +913,35 @@ int luna_fv_point_velocities(luna_fv* fv, double* w) {
 int luna_fv_gradients(luna_fv* fv, double* g) {
     if (!fv || !g) return fail(LUNA_ERR_ARGUMENT, "luna_fv_gradients: null argument");
     if (fv->opt.order != 1) return fail(LUNA_ERR_LOGIC, "luna_fv_gradients: order 0 keeps no gradients");
-    const size_t width = sizeof(double) * fv->V * fv->D;
-    LUNA_HIP(hipMemcpy2DAsync(g, width, fv->grad.p, sizeof(double) * fv->grad_row(), width, fv->N, hipMemcpyDeviceToHost, fv->stream));
+    const size_t VD = static_cast<size_t>(fv->V) * fv->D, N = fv->N;
+    if (fv->opt.precision == LUNA_PRECISION_DOUBLE) {
+        const size_t width = sizeof(double) * VD;
+        LUNA_HIP(hipMemcpy2DAsync(g, width, fv->grad.p, sizeof(double) * fv->grad_row(), width, fv->N, hipMemcpyDeviceToHost, fv->stream));
+        LUNA_HIP(hipStreamSynchronize(fv->stream));
+        return LUNA_OK;
+    }
+    // reduced precision: the stored rows, decoded on the host
+    const size_t row = fv->opt.precision == LUNA_PRECISION_GRAD32 ? sizeof(float) * (fv->D == 3 ? luna::lp_row<3>() : luna::lp_row<2>())
+                                                                  : sizeof(unsigned short) * luna::HALF_ROW;
+    std::vector<unsigned char> raw(N * row);
+    std::vector<unsigned int> k(fv->opt.precision == LUNA_PRECISION_GRAD16 ? N : 0);
+    LUNA_HIP(hipMemcpyAsync(raw.data(), fv->grad.p, raw.size(), hipMemcpyDeviceToHost, fv->stream));
+    if (!k.empty()) LUNA_HIP(hipMemcpyAsync(k.data(), fv->exponents.p, sizeof(unsigned int) * N, hipMemcpyDeviceToHost, fv->stream));
     LUNA_HIP(hipStreamSynchronize(fv->stream));
+    for (size_t i = 0; i < N; ++i)
+        for (size_t c = 0; c < VD; ++c) {
+            if (k.empty()) {
+                float f;
+                std::memcpy(&f, raw.data() + i * row + c * sizeof(float), sizeof f);
+                g[i * VD + c] = f;
+            } else {
+                unsigned short h;
+                std::memcpy(&h, raw.data() + i * row + c * sizeof h, sizeof h);
+                const int group = c < static_cast<size_t>(fv->D) ? 0 : c < static_cast<size_t>(fv->D + fv->D * fv->D) ? 1 : 2;
+                const int e = static_cast<signed char>(static_cast<unsigned char>(k[i] >> (8 * group)));
+                g[i * VD + c] = static_cast<double>(half_value(h) * std::ldexp(1.0f, -e));
+            }
+        }
     return LUNA_OK;
 }
 

@@ -62,7 +62,18 @@ struct Rows {
     double* flux;          // F x V, face order
     unsigned short* features;  // N V x 16 BF16 (the learned limiter's, recorded by the sweep with limit 2)
     unsigned char* troubled;   // N (the same)
+    // The precision policy of the gradient rows (precision.hpp:26-54): 0 double (rows of grad_row<D>() doubles); 1 grad32
+    // (fit, limiter and face-state change in float, rows of lp_row<D>() floats); 2 grad16 (the same arithmetic, rows of
+    // 16 FP16 values with one power-of-two exponent per group rho, v, P in `exponents`, int8 each). The rows share the
+    // double rows' allocation.
+    int precision;
+    unsigned int* exponents;  // N (grad16)
 };
+template <int D>
+constexpr int lp_row() {  // row_width<float>(V D): whole 32-byte sectors
+    return D == 3 ? 16 : 8;
+}
+constexpr int HALF_ROW = 16;  // row_width<uint16_t>(V D) in 2D and 3D: 32 bytes
 
 struct Grid {
     double step, per_step;  // grid step and its inverse

@@ -6,6 +6,10 @@
 // lists are stored in 64-cell slices, so a wavefront reads the t-th entries of its cells as one contiguous run and the
 // entries of the next group are in flight while the current group is summed; work-groups are mapped so that each XCD
 // takes a contiguous run of cells (neighbours in Morton order share its L2).
+#include <hip/hip_fp16.h>
+
+#include <type_traits>
+
 #include "kernels.hpp"
 
 // Tuning knobs (compile-time; the defaults are the measured best): entries of the sweep's first walk and of its limiter
@@ -138,18 +142,100 @@ __device__ __forceinline__ void load_near(const Geo& g, int slot, int j, Near<D>
     n.cs = row[C::CS];
 }
 
-// The sweep's per-cell sums (finite_volume.hpp:486-515), in list order.
-template <int D, bool Moving, bool Grad, int Rule>
+// The gradient rows of the precision policies (precision.hpp:26-54, 55-130). FP16 (grad16): each group (rho, v, P) of the
+// cell's gradient is scaled by 2^k so that its largest component is in [2^14, 2^15) (k = 0 for an all-zero group, within
+// [-114, 126]); the three k are kept as int8 in one 32-bit word per cell. The conversions are the hardware's
+// round-to-nearest-even ones.
+template <int D>
+__device__ __forceinline__ int grad_group(int c) {  // half::group(): 0 rho, 1 velocity, 2 pressure
+    return c < D ? 0 : c < D + D * D ? 1 : 2;
+}
+__device__ __forceinline__ float power_of_two(int k) { return __uint_as_float((unsigned)(k + 127) << 23); }
+__device__ __forceinline__ int group_exponent(float m) {
+    if (!(m > 0)) return 0;
+    const int e = (int)((__float_as_uint(m) >> 23) & 0xff) - 127;
+    return min(max(14 - e, -114), 126);
+}
+template <int D, int Prec, class G>
+__device__ __forceinline__ void store_gradient(const Rows& r, int i, const G (&grad)[D + 2][D]) {
+    constexpr int V = D + 2;
+    if constexpr (Prec == 0) {
+        double* out = r.grad + (size_t)i * grad_row<D>();
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int a = 0; a < D; ++a) out[v * D + a] = grad[v][a];
+    } else if constexpr (Prec == 1) {
+        float* out = reinterpret_cast<float*>(r.grad) + (size_t)i * lp_row<D>();
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int a = 0; a < D; ++a) out[v * D + a] = grad[v][a];
+    } else {
+        float largest[3] = {0, 0, 0};
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                const int q = grad_group<D>(D * v + a);
+                largest[q] = fmaxf(largest[q], fabsf(grad[v][a]));
+            }
+        int k[3];
+        float scale[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            k[q] = group_exponent(largest[q]);
+            scale[q] = power_of_two(k[q]);
+        }
+        unsigned short* out = reinterpret_cast<unsigned short*>(r.grad) + (size_t)i * HALF_ROW;
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int a = 0; a < D; ++a) out[v * D + a] = __half_as_ushort(__float2half_rn(grad[v][a] * scale[grad_group<D>(D * v + a)]));
+        r.exponents[i] = (unsigned)(unsigned char)(signed char)k[0] | (unsigned)(unsigned char)(signed char)k[1] << 8 |
+                         (unsigned)(unsigned char)(signed char)k[2] << 16;
+    }
+}
+// The gradient row of cell i in float (reduced-precision policies), decoded from FP16 if need be (FaceFlux::gradient,
+// :1288-1300).
+template <int D, int Prec>
+__device__ __forceinline__ void load_gradient_lp(const Rows& r, int i, float* gr) {
+    constexpr int VD = (D + 2) * D;
+    if constexpr (Prec == 1) {
+        const float* p = reinterpret_cast<const float*>(r.grad) + (size_t)i * lp_row<D>();
+#pragma unroll
+        for (int c = 0; c < VD; ++c) gr[c] = p[c];
+    } else {
+        const unsigned short* p = reinterpret_cast<const unsigned short*>(r.grad) + (size_t)i * HALF_ROW;
+        const unsigned k = r.exponents[i];
+        float scale[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) scale[q] = power_of_two(-(int)(signed char)(unsigned char)(k >> (8 * q)));
+#pragma unroll
+        for (int c = 0; c < VD; ++c) gr[c] = __half2float(__ushort_as_half(p[c])) * scale[grad_group<D>(c)];
+    }
+}
+
+// The largest finite value of the fit's type (finite_max_v).
+template <class G>
+__device__ __forceinline__ constexpr G finite_max() {
+    return sizeof(G) == 8 ? (G)1.7976931348623157e308 : (G)3.4028234663852886e38f;
+}
+
+// The sweep's per-cell sums (finite_volume.hpp:486-515), in list order. G: the fit's type (double, or float for the
+// reduced-precision policies, whose range is of W_j - W_i, rounded once, :466-469, 510-512).
+template <int D, bool Moving, bool Grad, int Rule, class G = double>
 struct Sums {
     static constexpr int V = D + 2, U = D + 1;
+    static constexpr bool exact = sizeof(G) == 8;
     double surface = 0, smallest_face = INFINITY, max_angle = 0, smallest = INFINITY;
-    double lo[V], hi[V], M[U][U], b[V][U];
+    G lo[V], hi[V], M[U][U], b[V][U];
 
     __device__ __forceinline__ void init() {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            lo[v] = 1.7976931348623157e308;
-            hi[v] = -1.7976931348623157e308;
+            lo[v] = finite_max<G>();
+            hi[v] = -finite_max<G>();
 #pragma unroll
             for (int p = 0; p < U; ++p) b[v][p] = 0;
         }
@@ -168,11 +254,11 @@ struct Sums {
         const double dist = sqrt(dot<D>(d, d));
         if constexpr (Moving) max_angle = fmax(max_angle, face_angle<D>(magnitude) / (0.5 * dist));
         if constexpr (Grad) {
-            double X[U];
+            G X[U];
             X[0] = 1;
 #pragma unroll
-            for (int p = 1; p < U; ++p) X[p] = d[p - 1] + n.c[p - 1] - centroid[p - 1];
-            const double w = magnitude;
+            for (int p = 1; p < U; ++p) X[p] = (G)(d[p - 1] + n.c[p - 1] - centroid[p - 1]);
+            const G w = (G)magnitude;
 #pragma unroll
             for (int p = 0; p < U; ++p)
 #pragma unroll
@@ -181,10 +267,11 @@ struct Sums {
             for (int v = 0; v < V; ++v) {
                 const double dW = n.W[v] - Wi[v];
 #pragma unroll
-                for (int p = 0; p < U; ++p) b[v][p] += w * X[p] * dW;
+                for (int p = 0; p < U; ++p) b[v][p] += w * X[p] * (G)dW;
 #if !LUNA_RANGE_PASS
-                lo[v] = fmin(lo[v], n.W[v]);
-                hi[v] = fmax(hi[v], n.W[v]);
+                const G value = exact ? (G)n.W[v] : (G)dW;
+                lo[v] = fmin(lo[v], value);
+                hi[v] = fmax(hi[v], value);
 #endif
             }
         }
@@ -203,9 +290,12 @@ struct Sums {
 // |d| have the same bits from either side, so the minimum is the reference's (the face kernel's terms are the same set).
 // The first block also resets the other dt slot and the step's fallback counters. Limit: 0 none, 1 the face-centroid
 // limiter, 2 the same limiter recording the learned limiter's features and troubled-cell flag (the gradients are those of 1).
-template <int D, bool Moving, bool Grad, int Limit, int Rule>
+template <int D, bool Moving, bool Grad, int Limit, int Rule, int Prec = 0>
 __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, Rows r, Gas gas, double courant, Counters c) {
-    constexpr int V = D + 2, U = D + 1, GR = grad_row<D>();
+    constexpr int V = D + 2, U = D + 1;
+    using G = typename std::conditional<Prec == 0, double, float>::type;
+    constexpr bool exact = Prec == 0;
+    static_assert(exact || Limit < 2, "the learned limiter's features need the double policy");
     using C = CellAt<D>;
     const int i = xcd_block() * CELL_BLOCK + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -222,7 +312,7 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
         const double ci = mine[C::CS], ri = mine[C::RAD];
         load_vec<D>(mine + C::X, xi);
         load_vec<D>(mine + C::C, centroid);
-        Sums<D, Moving, Grad, Rule> s;
+        Sums<D, Moving, Grad, Rule, G> s;
         s.init();
         const int count = g.count[i], base = first_slot(g, i);
         // Groups of UN entries: their face magnitudes and rows in flight together, summed in list order; the next group's
@@ -260,25 +350,26 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
         auto neighbour_range = [&]() {
 #pragma unroll
             for (int v = 0; v < V; ++v) {
-                s.lo[v] = 1.7976931348623157e308;
-                s.hi[v] = -1.7976931348623157e308;
+                s.lo[v] = finite_max<G>();
+                s.hi[v] = -finite_max<G>();
             }
             for (int k = 0; k < count; ++k) {
                 if (any_tiny && g.slot[base + k * SLICE] < tiny) continue;
                 const double* row = g.cell + (size_t)g.other[base + k * SLICE] * CELL_ROW + C::W;
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
-                    s.lo[v] = fmin(s.lo[v], row[v]);
-                    s.hi[v] = fmax(s.hi[v], row[v]);
+                    const G value = exact ? (G)row[v] : (G)(row[v] - Wi[v]);
+                    s.lo[v] = fmin(s.lo[v], value);
+                    s.hi[v] = fmax(s.hi[v], value);
                 }
             }
         };
         if constexpr (Grad && !LUNA_RANGE_PASS) {
             if (any_tiny) neighbour_range();
         }
-        double grad[V][D], intercept[V];
+        G grad[V][D], intercept[V];
         if constexpr (Grad) {  // Sweep::solve (:590-615): Cholesky of M, then M x = b per variable
-            double L[U][U];
+            G L[U][U];
 #pragma unroll
             for (int p = 0; p < U; ++p)
 #pragma unroll
@@ -287,24 +378,24 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
             for (int p = 0; p < U; ++p)
 #pragma unroll
                 for (int q = 0; q <= p; ++q) {
-                    double sum = s.M[q][p];
+                    G sum = s.M[q][p];
 #pragma unroll
                     for (int m = 0; m < q; ++m) sum -= L[p][m] * L[q][m];
                     L[p][q] = p == q ? sqrt(sum) : sum / L[q][q];
                 }
 #pragma unroll
             for (int v = 0; v < V; ++v) {
-                double z[U], x[U];
+                G z[U], x[U];
 #pragma unroll
                 for (int p = 0; p < U; ++p) {
-                    double sum = s.b[v][p];
+                    G sum = s.b[v][p];
 #pragma unroll
                     for (int m = 0; m < p; ++m) sum -= L[p][m] * z[m];
                     z[p] = sum / L[p][p];
                 }
 #pragma unroll
                 for (int p = U - 1; p >= 0; --p) {
-                    double sum = z[p];
+                    G sum = z[p];
 #pragma unroll
                     for (int m = p + 1; m < U; ++m) sum -= L[m][p] * x[m];
                     x[p] = sum / L[p][p];
@@ -357,7 +448,7 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
                 // Geometry::face_offset (:322-326), kept per slot, of UL entries loaded together, then applied in order.
                 constexpr int UL = LUNA_LIMIT_UNROLL;
                 Record<D, Limit == 2> rec;
-                rec.init(grad);
+                if constexpr (exact) rec.init(grad);
                 for (int t2 = 0; t2 < count; t2 += UL) {
                     double offset[UL][D];
                     bool use[UL];
@@ -374,25 +465,44 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
 #pragma unroll
                     for (int u = 0; u < UL; ++u) {
                         if (!use[u]) continue;
-                        rec.face(offset[u]);
+                        if constexpr (exact) {
+                            rec.face(offset[u]);
 #pragma unroll
-                        for (int v = 0; v < V; ++v) {
-                            const double dp = dot<D>(grad[v], offset[u]);
-                            if (dp > 0 && Wi[v] + dp > s.hi[v]) {
-                                const double sc = s.hi[v] > Wi[v] ? (s.hi[v] - Wi[v]) / dp : 0.0;
+                            for (int v = 0; v < V; ++v) {
+                                const double dp = dot<D>(grad[v], offset[u]);
+                                if (dp > 0 && Wi[v] + dp > s.hi[v]) {
+                                    const double sc = s.hi[v] > Wi[v] ? (s.hi[v] - Wi[v]) / dp : 0.0;
 #pragma unroll
-                                for (int a = 0; a < D; ++a) grad[v][a] = grad[v][a] * sc;
-                                rec.clip(v, sc);
-                            } else if (dp < 0 && Wi[v] + dp < s.lo[v]) {
-                                const double sc = s.lo[v] < Wi[v] ? (s.lo[v] - Wi[v]) / dp : 0.0;
+                                    for (int a = 0; a < D; ++a) grad[v][a] = grad[v][a] * sc;
+                                    rec.clip(v, sc);
+                                } else if (dp < 0 && Wi[v] + dp < s.lo[v]) {
+                                    const double sc = s.lo[v] < Wi[v] ? (s.lo[v] - Wi[v]) / dp : 0.0;
 #pragma unroll
-                                for (int a = 0; a < D; ++a) grad[v][a] = grad[v][a] * sc;
-                                rec.clip(v, sc);
+                                    for (int a = 0; a < D; ++a) grad[v][a] = grad[v][a] * sc;
+                                    rec.clip(v, sc);
+                                }
+                            }
+                        } else {  // reduced precision (:558-562): the offset, the face's change and the range relative to W_i in G
+                            G rg[D];
+#pragma unroll
+                            for (int a = 0; a < D; ++a) rg[a] = (G)offset[u][a];
+#pragma unroll
+                            for (int v = 0; v < V; ++v) {
+                                const G dp = dot<D>(grad[v], rg);
+                                if (dp > 0 && dp > s.hi[v]) {
+                                    const G sc = s.hi[v] > 0 ? s.hi[v] / dp : G(0);
+#pragma unroll
+                                    for (int a = 0; a < D; ++a) grad[v][a] = grad[v][a] * sc;
+                                } else if (dp < 0 && dp < s.lo[v]) {
+                                    const G sc = s.lo[v] < 0 ? s.lo[v] / dp : G(0);
+#pragma unroll
+                                    for (int a = 0; a < D; ++a) grad[v][a] = grad[v][a] * sc;
+                                }
                             }
                         }
                     }
                 }
-                if constexpr (Limit == 2) {
+                if constexpr (Limit == 2 && exact) {
                     Cell<D> cell;
 #pragma unroll
                     for (int v = 0; v < V; ++v) {
@@ -407,11 +517,7 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
                     r.troubled[i] = limiter_features<D>(cell, rec, r.features + (size_t)i * V * FEATURES);
                 }
             }
-            double* out = r.grad + (size_t)i * GR;
-#pragma unroll
-            for (int v = 0; v < V; ++v)
-#pragma unroll
-                for (int a = 0; a < D; ++a) out[v * D + a] = grad[v][a];
+            store_gradient<D, Prec>(r, i, grad);
         }
     }
     block_min<CELL_BLOCK>(smallest, c.dt);
@@ -463,6 +569,63 @@ __device__ __forceinline__ void pass_side_first(const double* prow, const double
     for (int v = 0; v < V; ++v) Wf[v] = W[v];
 }
 
+// The sides of the reduced-precision policies (FaceFlux::side(), :1306-1329, float branch; MovingFlux<false>::pass_side(),
+// :1451-1465): W in double; the gradient, the offset, the half step's rate and dt / 2 in float; W + (double)(g.r - dt/2
+// rate) or W + (double)(g.r).
+template <int D, int Prec>
+__device__ __forceinline__ void face_side_p(const Gas& gas, const Rows& r, int i, const double* prow, const double* grow, const double* off,
+                                            double step_dt, int order, double* Wf, int& fallbacks) {
+    constexpr int V = D + 2;
+    if constexpr (Prec == 0) {
+        face_side<D>(gas, prow, grow, off, step_dt, order, Wf, fallbacks);
+    } else {
+        double W[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) W[v] = prow[v];
+        if (order == 0) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) Wf[v] = W[v];
+            return;
+        }
+        float gr[V * D], Wg[V], A[V], rg[D];
+        load_gradient_lp<D, Prec>(r, i, gr);
+#pragma unroll
+        for (int v = 0; v < V; ++v) Wg[v] = (float)W[v];
+        half_step_rate<D, float>(gas, Wg, gr, A);
+#pragma unroll
+        for (int a = 0; a < D; ++a) rg[a] = (float)off[a];
+        const float h = 0.5f * (float)step_dt;
+#pragma unroll
+        for (int v = 0; v < V; ++v) Wf[v] = W[v] + (double)(dot<D>(gr + v * D, rg) - h * A[v]);
+        if (Wf[0] > 0 && Wf[D + 1] > 0) return;
+        ++fallbacks;
+#pragma unroll
+        for (int v = 0; v < V; ++v) Wf[v] = W[v];
+    }
+}
+template <int D, int Prec>
+__device__ __forceinline__ void pass_side_first_p(const Rows& r, int i, const double* prow, const double* grow, const double* off, double* Wf,
+                                                  int& fallbacks) {
+    constexpr int V = D + 2;
+    if constexpr (Prec == 0) {
+        pass_side_first<D>(prow, grow, off, Wf, fallbacks);
+    } else {
+        double W[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) W[v] = prow[v];
+        float gr[V * D], rg[D];
+        load_gradient_lp<D, Prec>(r, i, gr);
+#pragma unroll
+        for (int a = 0; a < D; ++a) rg[a] = (float)off[a];
+#pragma unroll
+        for (int v = 0; v < V; ++v) Wf[v] = W[v] + (double)dot<D>(gr + v * D, rg);
+        if (Wf[0] > 0 && Wf[D + 1] > 0) return;
+        ++fallbacks;
+#pragma unroll
+        for (int v = 0; v < V; ++v) Wf[v] = W[v];
+    }
+}
+
 // The position and centroid of one side of a face, and where its primitive half and gradient row are.
 template <int D>
 struct SideRows {
@@ -481,7 +644,7 @@ __device__ __forceinline__ void load_side(const Geo& g, const Rows& r, int i, Si
 }
 
 // One face's flux row: FaceFlux::operator() (:1415-1434) statically, MovingFlux::operator() (:1466-1490) on a moving step.
-template <int D, int Pass, int Riemann>
+template <int D, int Pass, int Riemann, int Prec = 0>
 __global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, Gas gas, int order, double dt, double unit, unsigned long long* fallbacks) {
     constexpr int V = D + 2;
     using FA = FaceAt<D>;
@@ -523,8 +686,8 @@ __global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, 
                     ro[k] = ro[k] + wf[k] * half;
                     rj[k] = rj[k] + wf[k] * half;
                 }
-                face_side<D>(gas, so.p, so.g, ro, dt, order, L, count);
-                face_side<D>(gas, sj.p, sj.g, rj, dt, order, R, count);
+                face_side_p<D, Prec>(gas, r, o, so.p, so.g, ro, dt, order, L, count);
+                face_side_p<D, Prec>(gas, r, j, sj.p, sj.g, rj, dt, order, R, count);
             } else {
                 double wo[D], wj[D];
 #pragma unroll
@@ -547,8 +710,8 @@ __global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, 
 #pragma unroll
                 for (int k = 0; k < D; ++k) wf[k] = (wo[k] + wj[k]) * 0.5 + sep[k] * (facv * inv_dist);
                 if constexpr (Pass == 0) {
-                    pass_side_first<D>(so.p, so.g, ro, L, count);
-                    pass_side_first<D>(sj.p, sj.g, rj, R, count);
+                    pass_side_first_p<D, Prec>(r, o, so.p, so.g, ro, L, count);
+                    pass_side_first_p<D, Prec>(r, j, sj.p, sj.g, rj, R, count);
                 } else {
                     // side(i, r + c^n+1 - c^n + w dt) with the FaceFlux dt = 2 dt (:1452-1454)
                     const double step_dt = 2 * dt;
@@ -558,13 +721,13 @@ __global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, 
                         const double moved = wo[k] * (0.5 * step_dt);
                         r2[k] = ro[k] + so.c[k] - r.old_centroid[(size_t)o * D + k] + moved;
                     }
-                    face_side<D>(gas, so.p, so.g, r2, step_dt, 1, L, count);
+                    face_side_p<D, Prec>(gas, r, o, so.p, so.g, r2, step_dt, 1, L, count);
 #pragma unroll
                     for (int k = 0; k < D; ++k) {
                         const double moved = wj[k] * (0.5 * step_dt);
                         r2[k] = rj[k] + sj.c[k] - r.old_centroid[(size_t)j * D + k] + moved;
                     }
-                    face_side<D>(gas, sj.p, sj.g, r2, step_dt, 1, R, count);
+                    face_side_p<D, Prec>(gas, r, j, sj.p, sj.g, r2, step_dt, 1, R, count);
                 }
             }
             if constexpr (Riemann == 1)
@@ -686,14 +849,22 @@ hipError_t launch_primitives(hipStream_t s, const Geo& g, const Rows& r, const G
     return hipGetLastError();
 }
 
-template <int D, bool Moving, bool Grad, int Limit>
-static hipError_t sweep_rule(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double courant, int rule, const Counters& c) {
+template <int D, bool Moving, bool Grad, int Limit, int Prec>
+static hipError_t sweep_prec(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double courant, int rule, const Counters& c) {
     const dim3 grid = blocks(g.N > 0 ? g.N : 1, CELL_BLOCK);
     if (rule == 1)
-        hipLaunchKernelGGL((k_sweep<D, Moving, Grad, Limit, 1>), grid, dim3(CELL_BLOCK), 0, s, g, r, gas, courant, c);
+        hipLaunchKernelGGL((k_sweep<D, Moving, Grad, Limit, 1, Prec>), grid, dim3(CELL_BLOCK), 0, s, g, r, gas, courant, c);
     else
-        hipLaunchKernelGGL((k_sweep<D, Moving, Grad, Limit, 0>), grid, dim3(CELL_BLOCK), 0, s, g, r, gas, courant, c);
+        hipLaunchKernelGGL((k_sweep<D, Moving, Grad, Limit, 0, Prec>), grid, dim3(CELL_BLOCK), 0, s, g, r, gas, courant, c);
     return hipGetLastError();
+}
+template <int D, bool Moving, bool Grad, int Limit>
+static hipError_t sweep_rule(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double courant, int rule, const Counters& c) {
+    if constexpr (Grad && Limit < 2) {
+        if (r.precision == 1) return sweep_prec<D, Moving, Grad, Limit, 1>(s, g, r, gas, courant, rule, c);
+        if (r.precision == 2) return sweep_prec<D, Moving, Grad, Limit, 2>(s, g, r, gas, courant, rule, c);
+    }
+    return sweep_prec<D, Moving, Grad, Limit, 0>(s, g, r, gas, courant, rule, c);
 }
 
 template <int D>
@@ -709,15 +880,22 @@ hipError_t launch_sweep(hipStream_t s, const Geo& g, const Rows& r, const Gas& g
     return limit ? sweep_rule<D, false, true, 1>(s, g, r, gas, courant, rule, c) : sweep_rule<D, false, true, 0>(s, g, r, gas, courant, rule, c);
 }
 
-template <int D, int Pass>
-static hipError_t flux_pass(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, int order, double dt, double unit,
+template <int D, int Pass, int Prec>
+static hipError_t flux_prec(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, int order, double dt, double unit,
                             unsigned long long* fallbacks) {
     const dim3 grid = blocks(g.F > 0 ? g.F : 1);
     if (riemann == 1)
-        hipLaunchKernelGGL((k_flux<D, Pass, 1>), grid, dim3(BLOCK), 0, s, g, r, gas, order, dt, unit, fallbacks);
+        hipLaunchKernelGGL((k_flux<D, Pass, 1, Prec>), grid, dim3(BLOCK), 0, s, g, r, gas, order, dt, unit, fallbacks);
     else
-        hipLaunchKernelGGL((k_flux<D, Pass, 0>), grid, dim3(BLOCK), 0, s, g, r, gas, order, dt, unit, fallbacks);
+        hipLaunchKernelGGL((k_flux<D, Pass, 0, Prec>), grid, dim3(BLOCK), 0, s, g, r, gas, order, dt, unit, fallbacks);
     return hipGetLastError();
+}
+template <int D, int Pass>
+static hipError_t flux_pass(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, int order, double dt, double unit,
+                            unsigned long long* fallbacks) {
+    if (order == 1 && r.precision == 1) return flux_prec<D, Pass, 1>(s, g, r, gas, riemann, order, dt, unit, fallbacks);
+    if (order == 1 && r.precision == 2) return flux_prec<D, Pass, 2>(s, g, r, gas, riemann, order, dt, unit, fallbacks);
+    return flux_prec<D, Pass, 0>(s, g, r, gas, riemann, order, dt, unit, fallbacks);
 }
 
 template <int D>

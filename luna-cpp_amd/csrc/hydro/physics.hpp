@@ -20,6 +20,13 @@ __device__ __forceinline__ double dot(const double* a, const double* b) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
+template <int D>
+__device__ __forceinline__ float dot(const float* a, const float* b) {
+    if constexpr (D == 1) return a[0] * b[0];
+    if constexpr (D == 2) return a[0] * b[0] + a[1] * b[1];
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
 // euler.hpp:104-113 primitive(): U -> W with the RAMSES floors.
 template <int D>
 __device__ __forceinline__ void primitive(const Gas& gas, const double* U, double* W) {
@@ -74,6 +81,28 @@ __device__ __forceinline__ void half_step(const Gas& gas, const double* W, const
     }
 #pragma unroll
     for (int v = 0; v < D + 2; ++v) out[v] = W[v] - 0.5 * dt * A[v];
+}
+
+// euler.hpp:186-204 half_step_rate() in the type T of a reduced-precision policy (float): sum_d A_d(W) dW/dx_d, with W
+// and g already in T. Used by the reduced-precision face sides.
+template <int D, class T>
+__device__ __forceinline__ void half_step_rate(const Gas& gas, const T* W, const T* g, T* A) {
+    constexpr int P = D + 1;
+    const T rho = W[0], inv_rho = T(1) / W[0], rho_c2 = T(gas.gamma) * W[P];
+#pragma unroll
+    for (int v = 0; v < D + 2; ++v) A[v] = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const T vn = W[1 + d];
+        A[0] += vn * g[0 * D + d] + rho * g[(1 + d) * D + d];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            T row = vn * g[(1 + k) * D + d];
+            if (k == d) row = row + inv_rho * g[P * D + d];
+            A[1 + k] += row;
+        }
+        A[P] += rho_c2 * g[(1 + d) * D + d] + vn * g[P * D + d];
+    }
 }
 
 // euler.hpp:316-336 Side and side_with(): one side in the face frame.

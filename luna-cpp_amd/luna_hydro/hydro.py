@@ -25,12 +25,13 @@ RIEMANN = {"hllc": 0, "hll": 1}
 MOTION = {"static": 0, "moving": 1}
 RULES = {"arepo": 0, "luna": 1}
 WALLS = {"reflective": 0, "outflow": 1, "inflow": 2}
+PRECISIONS = {"double": 0, "grad32": 1, "grad16": 2}  # hydro::Precision's gradient slot (precision.hpp:41-54)
 
 
 class FVOptions(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("limiter", ctypes.c_int), ("riemann", ctypes.c_int),
                 ("motion", ctypes.c_int), ("step_rule", ctypes.c_int), ("gamma", ctypes.c_double),
-                ("smallr", ctypes.c_double), ("smallc", ctypes.c_double)]
+                ("smallr", ctypes.c_double), ("smallc", ctypes.c_double), ("precision", ctypes.c_int)]
 
 
 class Geometry(ctypes.Structure):
@@ -138,11 +139,11 @@ class FiniteVolume:
     """hydro::FiniteVolume<D> (D = 2, 3) on the GPU: FiniteVolume(exec, mesh, gas, order, per_face, limiter, motion)."""
 
     def __init__(self, mesh, gamma, order=1, limiter="face", riemann="hllc", motion="static", step_rule="arepo",
-                 smallr=1e-10, smallc=1e-10):
+                 smallr=1e-10, smallc=1e-10, precision="double"):
         self.D, self.N = mesh["dim"], mesh["cells"]
         self.V = self.D + 2
         opt = FVOptions(self.D, order, LIMITERS[limiter], RIEMANN[riemann], MOTION[motion], RULES[step_rule],
-                        gamma, smallr, smallc)
+                        gamma, smallr, smallc, PRECISIONS[precision])
         g, self._mesh_arrays = geometry_record(mesh)
         h = ctypes.c_void_p()
         _check(lib().luna_fv_create(ctypes.byref(opt), ctypes.byref(g), ctypes.byref(h)), "FiniteVolume")

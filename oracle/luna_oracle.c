@@ -7,6 +7,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -64,6 +65,23 @@ static void half_step_rate(int D, const orc_gas* gas, const double* W, const dou
         A[0] += vn * g[0 * D + d] + rho * g[(1 + d) * D + d];
         for (int k = 0; k < D; ++k) {
             double row = vn * g[(1 + k) * D + d];
+            if (k == d) row = row + inv_rho * g[P * D + d];
+            A[1 + k] += row;
+        }
+        A[P] += rho_c2 * g[(1 + d) * D + d] + vn * g[P * D + d];
+    }
+}
+
+/* euler.hpp:186-204 half_step_rate() in float (the reduced-precision face sides): W and g rounded to float first. */
+static void half_step_rate_f(int D, const orc_gas* gas, const float* W, const float* g, float* A) {
+    const int P = D + 1;
+    const float rho = W[0], inv_rho = 1.0f / W[0], rho_c2 = (float)gas->gamma * W[P];
+    for (int v = 0; v < D + 2; ++v) A[v] = 0;
+    for (int d = 0; d < D; ++d) {
+        const float vn = W[1 + d];
+        A[0] += vn * g[0 * D + d] + rho * g[(1 + d) * D + d];
+        for (int k = 0; k < D; ++k) {
+            float row = vn * g[(1 + k) * D + d];
             if (k == d) row = row + inv_rho * g[P * D + d];
             A[1 + k] += row;
         }
@@ -365,9 +383,190 @@ static unsigned char limiter_features(int D, const double* W, const double* lo, 
     return (unsigned char)((shock || thin) ? 1 : 0);
 }
 
+/* precision.hpp:55-130. binary16: sign, 5 exponent bits (bias 15), 10 fraction bits; below 2^-14 the values are
+ * multiples of 2^-24. The encoder rounds to nearest, ties to even; 65520 and above become infinity. */
+unsigned short orc_half_from_float(float f) {
+    uint32_t x;
+    memcpy(&x, &f, sizeof x);
+    const unsigned short sign = (unsigned short)((x >> 16) & 0x8000u);
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return (unsigned short)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u));
+    if (ax >= 0x477ff000u) return (unsigned short)(sign | 0x7c00u); /* |f| >= 65520 */
+    if (ax < 0x38800000u) {                                          /* |f| < 2^-14: a multiple of 2^-24 */
+        float a;
+        memcpy(&a, &ax, sizeof a);
+        return (unsigned short)(sign | (unsigned short)rintf(a * 16777216.0f)); /* the product is exact; rintf: ties even */
+    }
+    const uint32_t r = ax + 0xfffu + ((ax >> 13) & 1u); /* 23 -> 10 fraction bits, ties to even; a carry moves the exponent */
+    return (unsigned short)(sign | ((r - (112u << 23)) >> 13));
+}
+float orc_half_to_float(unsigned short h) {
+    const int e = (h >> 10) & 0x1f, m = h & 0x3ff;
+    const float sign = (h & 0x8000) ? -1.0f : 1.0f;
+    if (e == 0x1f) return m ? NAN : sign * INFINITY;
+    return sign * (e == 0 ? ldexpf((float)m, -24) : ldexpf((float)(1024 + m), e - 25));
+}
+float orc_power_of_two(int k) { return ldexpf(1.0f, k); }
+/* The exponent k that puts a group's largest magnitude m at m 2^k in [2^14, 2^15), from m's biased exponent bits
+ * (a subnormal m counts as 2^-127), 0 for m = 0, within [-114, 126]. */
+int orc_group_exponent(float m) {
+    if (!(m > 0)) return 0;
+    uint32_t bits;
+    memcpy(&bits, &m, sizeof bits);
+    const int e = (int)((bits >> 23) & 0xffu) - 127, k = 14 - e;
+    return k < -114 ? -114 : k > 126 ? 126 : k;
+}
+static int half_group(int D, int c) { return c < D ? 0 : c < D + D * D ? 1 : 2; }
+
+static float dotf(int D, const float* a, const float* b) {
+    if (D == 1) return a[0] * b[0];
+    if (D == 2) return a[0] * b[0] + a[1] * b[1];
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* finite_volume.hpp:526-537 (constants :74): the regularised point velocity of cell i. */
+static void regularised_velocity(orc_fv* fv, int i, const double* Wi, const double* centroid, double max_angle) {
+    const int D = fv->D, V = D + 2;
+    const double shaping = 0.5, factor = 2.25;
+    double fraction = 0;
+    if (max_angle > 0.75 * factor) fraction = max_angle > factor ? shaping : shaping * (max_angle - 0.75 * factor) / (0.25 * factor);
+    double wi[MAXD];
+    for (int a = 0; a < D; ++a) wi[a] = Wi[1 + a];
+    const double cd = sqrt(dot(D, centroid, centroid));
+    if (cd > 0 && fraction > 0) {
+        const double s = fraction * Wi[V] / cd;
+        for (int a = 0; a < D; ++a) wi[a] = wi[a] + centroid[a] * s;
+    }
+    for (int a = 0; a < D; ++a) fv->w[(size_t)i * D + a] = wi[a];
+}
+
+/* finite_volume.hpp:463-569 Sweep::operator() under Grad32 / Grad16 (G = float): the fit's sums in float from the double
+ * separations, the neighbours' range of W_j - W_i rounded once (:469, 510-512), the float Cholesky solve (:590-615), the
+ * limiter on float offsets against that range (:558-562); the row as stored (:618-641): float, or FP16 of g 2^k per group
+ * with k from the group's largest |g|, kept here as FaceFlux::gradient() (:1288-1300) decodes it. */
+static void sweep_cell_lp(orc_fv* fv, const orc_geometry* geo, int i) {
+    const int D = fv->D, V = D + 2, U = D + 1, PW = V + 2;
+    const double* Wi = fv->prims + (size_t)i * PW;
+    double surface = 0, smallest = INFINITY, max_angle = 0;
+    float lo[MAXV], hi[MAXV], M[MAXD + 1][MAXD + 1], b[MAXV][MAXD + 1];
+    for (int v = 0; v < V; ++v) {
+        lo[v] = FLT_MAX;
+        hi[v] = -FLT_MAX;
+        for (int p = 0; p < U; ++p) b[v][p] = 0;
+    }
+    for (int p = 0; p < U; ++p)
+        for (int s = 0; s < U; ++s) M[p][s] = 0;
+    const double* centroid = geo->centroid + (size_t)i * D;
+    const int first = geo->start[i], end = geo->start[i + 1];
+    for (int k = first; k < end; ++k) {
+        const int e = geo->list[k], f = list_face(e);
+        const int j = e >= 0 ? geo->neighbour[f] : geo->owner[f];
+        const double* Wj = fv->prims + (size_t)j * PW;
+        double d[MAXD];
+        separation(geo, i, j, d);
+        const double magnitude = magnitude_of(geo, f);
+        surface += magnitude;
+        smallest = fmin(smallest, magnitude);
+        if (fv->moving) max_angle = fmax(max_angle, face_angle(D, magnitude) / (0.5 * sqrt(dot(D, d, d))));
+        float X[MAXD + 1];
+        X[0] = 1;
+        for (int p = 1; p < U; ++p) X[p] = (float)(d[p - 1] + geo->centroid[(size_t)j * D + p - 1] - centroid[p - 1]);
+        const float w = (float)magnitude;
+        for (int p = 0; p < U; ++p)
+            for (int s = p; s < U; ++s) M[p][s] += w * X[p] * X[s];
+        for (int v = 0; v < V; ++v) {
+            const double dW = Wj[v] - Wi[v];
+            for (int p = 0; p < U; ++p) b[v][p] += w * X[p] * (float)dW;
+            lo[v] = fminf(lo[v], (float)dW);
+            hi[v] = fmaxf(hi[v], (float)dW);
+        }
+    }
+    const double tiny = 1e-6 * surface;
+    const int any_tiny = smallest < tiny;
+    if (any_tiny) {
+        for (int v = 0; v < V; ++v) {
+            lo[v] = FLT_MAX;
+            hi[v] = -FLT_MAX;
+        }
+        for (int k = first; k < end; ++k) {
+            const int e = geo->list[k], f = list_face(e);
+            const int j = e >= 0 ? geo->neighbour[f] : geo->owner[f];
+            if (magnitude_of(geo, f) < tiny) continue;
+            const double* Wj = fv->prims + (size_t)j * PW;
+            for (int v = 0; v < V; ++v) {
+                lo[v] = fminf(lo[v], (float)(Wj[v] - Wi[v]));
+                hi[v] = fmaxf(hi[v], (float)(Wj[v] - Wi[v]));
+            }
+        }
+    }
+    float L[MAXD + 1][MAXD + 1], grad[MAXV * MAXD];
+    for (int p = 0; p < U; ++p)
+        for (int s = 0; s < U; ++s) L[p][s] = 0;
+    for (int p = 0; p < U; ++p)
+        for (int s = 0; s <= p; ++s) {
+            float sum = M[s][p];
+            for (int m = 0; m < s; ++m) sum -= L[p][m] * L[s][m];
+            L[p][s] = p == s ? sqrtf(sum) : sum / L[s][s];
+        }
+    for (int v = 0; v < V; ++v) {
+        float z[MAXD + 1], x[MAXD + 1];
+        for (int p = 0; p < U; ++p) {
+            float sum = b[v][p];
+            for (int m = 0; m < p; ++m) sum -= L[p][m] * z[m];
+            z[p] = sum / L[p][p];
+        }
+        for (int p = U - 1; p >= 0; --p) {
+            float sum = z[p];
+            for (int m = p + 1; m < U; ++m) sum -= L[m][p] * x[m];
+            x[p] = sum / L[p][p];
+        }
+        for (int a = 0; a < D; ++a) grad[v * D + a] = x[1 + a];
+    }
+    if (fv->moving) regularised_velocity(fv, i, Wi, centroid, max_angle);
+    if (fv->limiter == 0) {
+        for (int k = first; k < end; ++k) {
+            const int f = list_face(geo->list[k]);
+            if (any_tiny && magnitude_of(geo, f) < tiny) continue;
+            double offset[MAXD];
+            face_offset(geo, f, i, offset);
+            float rg[MAXD];
+            for (int a = 0; a < D; ++a) rg[a] = (float)offset[a];
+            for (int v = 0; v < V; ++v) {
+                float* g = grad + v * D;
+                const float dp = dotf(D, g, rg);
+                if (dp > 0 && dp > hi[v]) {
+                    const float s = hi[v] > 0 ? hi[v] / dp : 0.0f;
+                    for (int a = 0; a < D; ++a) g[a] = g[a] * s;
+                } else if (dp < 0 && dp < lo[v]) {
+                    const float s = lo[v] < 0 ? lo[v] / dp : 0.0f;
+                    for (int a = 0; a < D; ++a) g[a] = g[a] * s;
+                }
+            }
+        }
+    }
+    double* out = fv->grad + (size_t)i * V * D;
+    if (fv->precision == 1) {
+        for (int c = 0; c < V * D; ++c) out[c] = grad[c];
+        return;
+    }
+    float largest[3] = {0, 0, 0};
+    for (int c = 0; c < V * D; ++c) largest[half_group(D, c)] = fmaxf(largest[half_group(D, c)], fabsf(grad[c]));
+    int kq[3];
+    for (int q = 0; q < 3; ++q) kq[q] = orc_group_exponent(largest[q]);
+    for (int c = 0; c < V * D; ++c) {
+        const int q = half_group(D, c);
+        const unsigned short h = orc_half_from_float(grad[c] * orc_power_of_two(kq[q]));
+        out[c] = orc_half_to_float(h) * orc_power_of_two(-kq[q]);
+    }
+}
+
 /* finite_volume.hpp:463-569 Sweep::operator() (double policy, the face-centroid limiter or none), with the moving mesh's
  * point velocity (:526-537, constants :74) and neighbour_range (:572-588). */
 static void sweep_cell(orc_fv* fv, const orc_geometry* geo, int i) {
+    if (fv->precision != 0 && fv->order == 1) {
+        sweep_cell_lp(fv, geo, i);
+        return;
+    }
     const int D = fv->D, V = D + 2, U = D + 1, PW = V + 2;
     const double* Wi = fv->prims + (size_t)i * PW;
     double surface = 0, smallest = INFINITY, max_angle = 0;
@@ -427,20 +626,7 @@ static void sweep_cell(orc_fv* fv, const orc_geometry* geo, int i) {
     }
     double grad[MAXV * MAXD], intercept[MAXV];
     if (grad_on) solve_fit(D, M, b, grad, intercept);
-    if (fv->moving) {
-        const double shaping = 0.5, factor = 2.25;
-        double fraction = 0;
-        if (max_angle > 0.75 * factor)
-            fraction = max_angle > factor ? shaping : shaping * (max_angle - 0.75 * factor) / (0.25 * factor);
-        double wi[MAXD];
-        for (int a = 0; a < D; ++a) wi[a] = Wi[1 + a];
-        const double cd = sqrt(dot(D, centroid, centroid));
-        if (cd > 0 && fraction > 0) {
-            const double s = fraction * Wi[V] / cd;
-            for (int a = 0; a < D; ++a) wi[a] = wi[a] + centroid[a] * s;
-        }
-        for (int a = 0; a < D; ++a) fv->w[(size_t)i * D + a] = wi[a];
-    }
+    if (fv->moving) regularised_velocity(fv, i, Wi, centroid, max_angle);
     if (!grad_on) return;
     if (fv->limiter == 0 || fv->limiter == 2) {
         limiter_record rec;
@@ -546,8 +732,18 @@ static void face_side(const orc_fv* fv, int i, const double* r, double step_dt, 
         for (int v = 0; v < V; ++v) Wf[v] = W[v];
         return;
     }
-    orc_half_step(D, &fv->gas, W, g, step_dt, Wf);
-    for (int v = 0; v < V; ++v) Wf[v] += dot(D, g + v * D, r);
+    if (fv->precision != 0) { /* :1319-1324: W in double, the rest in float */
+        float gf[MAXV * MAXD], Wg[MAXV], A[MAXV], rg[MAXD];
+        for (int c = 0; c < V * D; ++c) gf[c] = (float)g[c];
+        for (int v = 0; v < V; ++v) Wg[v] = (float)W[v];
+        half_step_rate_f(D, &fv->gas, Wg, gf, A);
+        for (int a = 0; a < D; ++a) rg[a] = (float)r[a];
+        const float h = 0.5f * (float)step_dt;
+        for (int v = 0; v < V; ++v) Wf[v] = W[v] + (double)(dotf(D, gf + v * D, rg) - h * A[v]);
+    } else {
+        orc_half_step(D, &fv->gas, W, g, step_dt, Wf);
+        for (int v = 0; v < V; ++v) Wf[v] += dot(D, g + v * D, r);
+    }
     if (Wf[0] > 0 && Wf[D + 1] > 0) return;
     ++*fallbacks;
     for (int v = 0; v < V; ++v) Wf[v] = W[v];
@@ -558,7 +754,14 @@ static void pass_side_first(const orc_fv* fv, int i, const double* r, double* Wf
     const int D = fv->D, V = D + 2, PW = V + 2;
     const double* W = fv->prims + (size_t)i * PW;
     const double* g = fv->grad + (size_t)i * V * D;
-    for (int v = 0; v < V; ++v) Wf[v] = W[v] + dot(D, g + v * D, r);
+    if (fv->precision != 0) {
+        float gf[MAXV * MAXD], rg[MAXD];
+        for (int c = 0; c < V * D; ++c) gf[c] = (float)g[c];
+        for (int a = 0; a < D; ++a) rg[a] = (float)r[a];
+        for (int v = 0; v < V; ++v) Wf[v] = W[v] + (double)dotf(D, gf + v * D, rg);
+    } else {
+        for (int v = 0; v < V; ++v) Wf[v] = W[v] + dot(D, g + v * D, r);
+    }
     if (Wf[0] > 0 && Wf[D + 1] > 0) return;
     ++*fallbacks;
     for (int v = 0; v < V; ++v) Wf[v] = W[v];
@@ -2350,7 +2553,7 @@ static void face_sides(const orc_fv* fv, const orc_geometry* geo, int f, double 
     }
 }
 
-static int static_derivatives_ok(const orc_fv* fv) { return !fv->moving && fv->order == 1 && fv->D >= 2; }
+static int static_derivatives_ok(const orc_fv* fv) { return !fv->moving && fv->order == 1 && fv->D >= 2 && fv->precision == 0; }
 
 /* finite_volume.hpp:2039-2074 step_tangent(): the static step of q and dq -> J dq (dq: N x V, extensive), dt fixed.
  * Returns 0, or -1 outside a static order-1 HLLC mesh step. */
@@ -2883,7 +3086,7 @@ static void moving_fit_adjoint(const orc_fv* fv, const orc_geometry* g, int i, c
 
 int orc_moving_adjoint(orc_fv* fv, const orc_geometry* g0, const orc_geometry* g1, double dt, int steps_fixed, double* qbar, double* xbar,
                        orc_geometry_adjoint adjoint0, void* ctx0, orc_geometry_adjoint adjoint1, void* ctx1) {
-    if (!fv->moving || fv->order != 1 || fv->D < 2) return -1;
+    if (!fv->moving || fv->order != 1 || fv->D < 2 || fv->precision != 0) return -1;
     const int D = fv->D, V = D + 2, U = D + 1, N = fv->N, PW = V + 2;
     /* t^n: W, the limited gradients and the point velocities (the sweep), c^n, the grid steps and w-hat. */
     orc_primitives(fv, g0);

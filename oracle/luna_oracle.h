@@ -75,6 +75,8 @@ typedef struct {
     unsigned short* features; /* limiter 2 (learned): N V x 16 BF16 bit patterns, written by each sweep */
     unsigned char* troubled;  /* limiter 2: N */
     int hold_steps;           /* moving: orc_moving_begin keeps the given steps (the adjoint's central differences) */
+    int precision;            /* the gradient policy (precision.hpp:26-54): 0 double, 1 grad32, 2 grad16; grad keeps the
+                                 stored rows decoded to double (floats, exactly) */
 } orc_fv;
 
 void orc_primitives(orc_fv* fv, const orc_geometry* geo);
@@ -98,6 +100,12 @@ typedef void (*orc_geometry_adjoint)(void* ctx, const double* Vbar, const double
 int orc_moving_adjoint(orc_fv* fv, const orc_geometry* g0, const orc_geometry* g1, double dt, int steps_fixed, double* qbar, double* xbar,
                        orc_geometry_adjoint adjoint0, void* ctx0, orc_geometry_adjoint adjoint1, void* ctx1);
 long long orc_snap(double x, double step, long long period);
+/* precision.hpp:55-130, half::from_float / to_float (IEEE binary16, round to nearest even), group_exponent,
+ * power_of_two. */
+unsigned short orc_half_from_float(float f);
+float orc_half_to_float(unsigned short h);
+int orc_group_exponent(float m);
+float orc_power_of_two(int k);
 int orc_delta(long long k0, long long k1, long long period);
 
 /* 1D sorted line (finite_volume_1d.hpp). Static: slots are cells (id NULL). Moving: see orc_mline_*, which owns one.

@@ -34,7 +34,7 @@ class _FV(ctypes.Structure):
                 ("q", _dp), ("prims", _dp), ("grad", _dp), ("w", _dp), ("x", _dp), ("old_centroid", _dp),
                 ("steps", _ip), ("flux", _dp), ("fallbacks", ctypes.c_longlong * 2), ("prims_ready", ctypes.c_int),
                 ("features", ctypes.POINTER(ctypes.c_ushort)), ("troubled", ctypes.POINTER(ctypes.c_ubyte)),
-                ("hold_steps", ctypes.c_int)]
+                ("hold_steps", ctypes.c_int), ("precision", ctypes.c_int)]
 
 
 _GEO_ADJOINT = ctypes.CFUNCTYPE(None, ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp)
@@ -62,6 +62,9 @@ def lib():
         L.orc_line_centroid.restype = ctypes.c_double
         L.orc_sound_speed.restype = ctypes.c_double
         L.orc_snap.restype = ctypes.c_longlong
+        L.orc_half_from_float.restype = ctypes.c_ushort
+        L.orc_half_to_float.restype = ctypes.c_float
+        L.orc_power_of_two.restype = ctypes.c_float
         L.orc_snap.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_longlong]
         L.orc_delta.argtypes = [ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong]
         L.orc_moving_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp, _dp,
@@ -142,6 +145,19 @@ def primitive_adjoint(U, Wbar, gamma, smallr=1e-10, smallc=1e-10):
     return Ubar
 
 
+def half_from_float(f):
+    """precision.hpp half::from_float: the binary16 bits of a float (round to nearest even)."""
+    return lib().orc_half_from_float(ctypes.c_float(f))
+
+
+def half_to_float(h):
+    return lib().orc_half_to_float(ctypes.c_ushort(h))
+
+
+def group_exponent(m):
+    return lib().orc_group_exponent(ctypes.c_float(m))
+
+
 def half_step(W, grad, dt, gamma):
     """W: [V]; grad: [V][D]."""
     W, grad = _vec(W), _vec(grad)
@@ -212,8 +228,10 @@ class Geometry:
 class FiniteVolume:
     """The oracle's FiniteVolume<D> (D = 2, 3): state, time_step, step, and the moving step split at the mesh update."""
 
-    def __init__(self, mesh, gamma, order=1, limiter="face", riemann="hllc", moving=False, step_rule="arepo"):
+    def __init__(self, mesh, gamma, order=1, limiter="face", riemann="hllc", moving=False, step_rule="arepo", precision="double"):
         D, N = mesh["dim"], mesh["cells"]
+        if precision != "double" and limiter == "learned":
+            raise ValueError("the learned limiter's features need the double policy")
         V = D + 2
         self.D, self.N, self.V = D, N, V
         self.q = np.zeros((N, V))
@@ -230,6 +248,7 @@ class FiniteVolume:
         c.riemann = {"hllc": 0, "hll": 1}[riemann]
         c.moving = int(moving)
         c.step_rule = {"arepo": 0, "luna": 1}[step_rule]
+        c.precision = {"double": 0, "grad32": 1, "grad16": 2}[precision]
         c.gas = gas(gamma)
         c.q, c.prims, c.grad, c.w = _ptr(self.q), _ptr(self.prims), _ptr(self.grad), _ptr(self.w)
         c.x, c.old_centroid, c.steps, c.flux = _ptr(self.x), _ptr(self.old_centroid), _ptr(self.steps, _ip), _ptr(self.flux)
