@@ -28,6 +28,16 @@ WALLS = {"reflective": 0, "outflow": 1, "inflow": 2}
 PRECISIONS = {"double": 0, "grad32": 1, "grad16": 2}  # hydro::Precision's gradient slot (precision.hpp:41-54)
 
 
+def policy(name, dim):
+    """The drivers' --precision: default (DefaultPrecision<D>, precision.hpp:49-54: grad16 in 3D, grad32 in 2D), double,
+    grad32 or grad16."""
+    if name == "default":
+        return "grad16" if dim == 3 else "grad32"
+    if name not in PRECISIONS:
+        raise ValueError("unknown --precision %s (default, double, grad32 or grad16)" % name)
+    return name
+
+
 class FVOptions(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("limiter", ctypes.c_int), ("riemann", ctypes.c_int),
                 ("motion", ctypes.c_int), ("step_rule", ctypes.c_int), ("gamma", ctypes.c_double),

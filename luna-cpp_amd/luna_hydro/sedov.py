@@ -4,6 +4,9 @@ Voronoi mesh, the volume-weighted blast, time_step + step to t_end with snapshot
 volume) so runs compare line by line.
 
     python -m luna_hydro.sedov --n 64 [--moving] [--t-end 0.05] [--courant 0.3] [--snapshots 10] [--out DIR] [--max-steps K]
+                               [--precision default|double|grad32|grad16]
+
+--precision as the reference's (hydro_sedov.cpp:409-428): default is its DefaultPrecision<3>, grad16.
 """
 import argparse
 import ctypes
@@ -60,11 +63,12 @@ class _Engine:
         return m
 
 
-def run(n=64, moving=False, t_end=0.05, courant=0.3, snapshots=10, out="", max_steps=0, jitter=0.25, seed=1, log=print):
+def run(n=64, moving=False, t_end=0.05, courant=0.3, snapshots=10, out="", max_steps=0, jitter=0.25, seed=1, log=print,
+        precision="double"):
     m = mesh.build(problems.jittered_lattice(n, 3, jitter, seed), 1.0)
     q, deposited = problems.sedov_state(m)
     N = m["cells"]
-    fv = hydro.FiniteVolume(m, GAMMA, motion="moving" if moving else "static")
+    fv = hydro.FiniteVolume(m, GAMMA, motion="moving" if moving else "static", precision=hydro.policy(precision, 3))
     fv.set_state(q)
     engine = _Engine(fv, moving)
     before = totals(q)
@@ -107,8 +111,8 @@ def run(n=64, moving=False, t_end=0.05, courant=0.3, snapshots=10, out="", max_s
     after = totals(qf)
     median = float(np.median(walls)) if walls else float("nan")
     kernels = fv.kernel_times()
-    log("run: precision=double order=1 update=face steps=%d limiter=face riemann=hllc mesh=%s blast=%d t=%.17g wall_s=%.6g "
-        "s_per_step=%.6g step_wall_ms_median=%.6g G_cell_updates_per_s=%.6g" % (steps, "moving" if moving else "static", deposited, t,
+    log("run: precision=%s order=1 update=face steps=%d limiter=face riemann=hllc mesh=%s blast=%d t=%.17g wall_s=%.6g "
+        "s_per_step=%.6g step_wall_ms_median=%.6g G_cell_updates_per_s=%.6g" % (precision, steps, "moving" if moving else "static", deposited, t,
                                                                                 wall_s, wall_s / max(steps, 1), median,
                                                                                 N / (median * 1e-3) / 1e9))
     log("fallbacks: total=%d steps_with=%d most_in_a_step=%d face_sides_per_step=%d" % (fallbacks, with_fallbacks, most, 2 * fv.faces()))
@@ -132,8 +136,10 @@ def main(argv=None):
     ap.add_argument("--snapshots", type=int, default=10)
     ap.add_argument("--out", default="")
     ap.add_argument("--max-steps", type=int, default=0)
+    ap.add_argument("--precision", choices=["default", "double", "grad32", "grad16"], default="default",
+                    help="default: the reference's DefaultPrecision<3> (grad16)")
     a = ap.parse_args(argv)
-    run(a.n, a.moving, a.t_end, a.courant, a.snapshots, a.out, a.max_steps)
+    run(a.n, a.moving, a.t_end, a.courant, a.snapshots, a.out, a.max_steps, precision=a.precision)
 
 
 if __name__ == "__main__":

@@ -9,14 +9,19 @@ from luna_hydro import mesh, problems, sedov, snapshot
 pytestmark = pytest.mark.gpu
 
 
-def test_sedov_driver_checksum_matches_oracle(tmp_path):
+@pytest.mark.parametrize("precision", ["double", "default"])
+def test_sedov_driver_checksum_matches_oracle(tmp_path, precision):
+    """--precision default is the reference's 3D default, grad16."""
+    from luna_hydro import hydro
+
     lines = []
-    res = sedov.run(n=10, moving=False, t_end=0.05, courant=0.3, snapshots=10, out=str(tmp_path), max_steps=4, log=lines.append)
+    res = sedov.run(n=10, moving=False, t_end=0.05, courant=0.3, snapshots=10, out=str(tmp_path), max_steps=4, log=lines.append,
+                    precision=precision)
     print("\n".join(lines), flush=True)
     assert [l.split(":")[0] for l in lines] == ["run", "fallbacks", "phases_ms_median", "conservation", "checksum"]
     m = mesh.build(problems.jittered_lattice(10, 3, 0.25, 1), 1.0)
     q, _ = problems.sedov_state(m)
-    orc = oracle.FiniteVolume(m, 5.0 / 3.0)
+    orc = oracle.FiniteVolume(m, 5.0 / 3.0, precision=hydro.policy(precision, 3))
     orc.set_state(q)
     t, nxt = 0.0, 1
     for _ in range(res["steps"]):
@@ -33,19 +38,20 @@ def test_sedov_driver_checksum_matches_oracle(tmp_path):
     assert np.array_equal(snap["U"], (q / m["volume"][:, None]).T)
 
 
-@pytest.mark.parametrize("ic", ["smooth", "sharp"])
-def test_kh_driver_hash_and_modes_match_oracle(ic):
+@pytest.mark.parametrize("ic,precision", [("smooth", "double"), ("sharp", "double"), ("smooth", "default")])
+def test_kh_driver_hash_and_modes_match_oracle(ic, precision):
     """The KH driver (hydro_kh2d's loop: outputs every 0.02, dt cut to land on them) on 24^2 to t = 0.06: its run line's
-    hash is the FNV-1a of the oracle's final state after the same steps, and its mode series is the oracle's."""
-    from luna_hydro import kh2d
+    hash is the FNV-1a of the oracle's final state after the same steps, and its mode series is the oracle's. --precision
+    default is the reference's 2D default, grad32."""
+    from luna_hydro import hydro, kh2d
 
     lines = []
-    res = kh2d.run(n=24, ic=ic, t_end=0.06, log=lines.append)
+    res = kh2d.run(n=24, ic=ic, t_end=0.06, log=lines.append, precision=precision)
     print("\n".join(lines), flush=True)
     assert [l.split(":")[0] for l in lines] == ["totals", "run", "ours"]
     m = mesh.build(problems.jittered_lattice(24, 2, 0.25, 1), 1.0)
     state = problems.kelvin_helmholtz_smooth_state if ic == "smooth" else problems.kelvin_helmholtz_state
-    orc = oracle.FiniteVolume(m, 5.0 / 3.0, step_rule="luna")
+    orc = oracle.FiniteVolume(m, 5.0 / 3.0, step_rule="luna", precision=hydro.policy(precision, 2))
     orc.set_state(state(m, 5.0 / 3.0))
     t, series = 0.0, []
     for k in range(1, 4):
