@@ -573,6 +573,7 @@ int luna_fv_destroy(luna_fv* fv) {
     fv->prof.destroy();
     for (auto* b : {&fv->cell, &fv->facerec, &fv->slotgeo, &fv->q, &fv->grad, &fv->w, &fv->x, &fv->old_centroid, &fv->flux}) b->release();
     for (auto* b : {&fv->count, &fv->slice, &fv->list, &fv->other, &fv->steps, &fv->d_extreme}) b->release();
+    fv->exponents.release();
     for (auto* b : {&fv->d_dq, &fv->d_dW, &fv->d_dgrad, &fv->d_dflux, &fv->d_sidebar, &fv->d_bbar, &fv->d_boundbar, &fv->d_out}) b->release();
     for (void* p : {static_cast<void*>(fv->stage_cell), static_cast<void*>(fv->stage_face), static_cast<void*>(fv->stage_list),
                     static_cast<void*>(fv->stage_other), static_cast<void*>(fv->stage_count), static_cast<void*>(fv->stage_slot)})

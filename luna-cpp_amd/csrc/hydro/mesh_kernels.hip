@@ -16,7 +16,7 @@
 // walk loaded together, flux rows of the update loaded together, and the occupancy floor (waves per SIMD) of the sweep
 // and flux kernels.
 #ifndef LUNA_SWEEP_UNROLL
-#define LUNA_SWEEP_UNROLL 2
+#define LUNA_SWEEP_UNROLL 1
 #endif
 #ifndef LUNA_LIMIT_UNROLL
 #define LUNA_LIMIT_UNROLL 4
@@ -25,13 +25,15 @@
 #define LUNA_UPDATE_UNROLL 4
 #endif
 #ifndef LUNA_SWEEP_WAVES
-#define LUNA_SWEEP_WAVES 1
+#define LUNA_SWEEP_WAVES 3
 #endif
 #ifndef LUNA_FLUX_WAVES
 #define LUNA_FLUX_WAVES 1
 #endif
-// 1: the sweep takes the neighbours' range (min, max of W) in a loop of its own after the fit, re-reading the
-// neighbours' W (cache-resident), instead of carrying it through the first walk (20 registers fewer there).
+// 1: every sweep takes the neighbours' range (min, max of W) in a loop of its own after the fit, re-reading the
+// neighbours' W (cache-resident), instead of carrying it through the first walk (20 registers fewer there); 0: only the
+// moving FP64 sweep does (the static one fits 3 waves per SIMD without it; the moving one spills with it carried). The
+// reduced-precision sweeps load two entries together and carry the range (their float sums are half the registers).
 #ifndef LUNA_RANGE_PASS
 #define LUNA_RANGE_PASS 0
 #endif
@@ -224,7 +226,7 @@ __device__ __forceinline__ constexpr G finite_max() {
 
 // The sweep's per-cell sums (finite_volume.hpp:486-515), in list order. G: the fit's type (double, or float for the
 // reduced-precision policies, whose range is of W_j - W_i, rounded once, :466-469, 510-512).
-template <int D, bool Moving, bool Grad, int Rule, class G = double>
+template <int D, bool Moving, bool Grad, int Rule, class G = double, bool RangePass = false>
 struct Sums {
     static constexpr int V = D + 2, U = D + 1;
     static constexpr bool exact = sizeof(G) == 8;
@@ -268,11 +270,11 @@ struct Sums {
                 const double dW = n.W[v] - Wi[v];
 #pragma unroll
                 for (int p = 0; p < U; ++p) b[v][p] += w * X[p] * (G)dW;
-#if !LUNA_RANGE_PASS
-                const G value = exact ? (G)n.W[v] : (G)dW;
-                lo[v] = fmin(lo[v], value);
-                hi[v] = fmax(hi[v], value);
-#endif
+                if constexpr (!RangePass) {
+                    const G value = exact ? (G)n.W[v] : (G)dW;
+                    lo[v] = fmin(lo[v], value);
+                    hi[v] = fmax(hi[v], value);
+                }
             }
         }
         if constexpr (Rule == 0) {  // FaceStep (:1079-1089) from this side
@@ -291,7 +293,7 @@ struct Sums {
 // The first block also resets the other dt slot and the step's fallback counters. Limit: 0 none, 1 the face-centroid
 // limiter, 2 the same limiter recording the learned limiter's features and troubled-cell flag (the gradients are those of 1).
 template <int D, bool Moving, bool Grad, int Limit, int Rule, int Prec = 0>
-__global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, Rows r, Gas gas, double courant, Counters c) {
+__global__ __launch_bounds__(CELL_BLOCK, Limit == 2 ? 1 : LUNA_SWEEP_WAVES) void k_sweep(Geo g, Rows r, Gas gas, double courant, Counters c) {
     constexpr int V = D + 2, U = D + 1;
     using G = typename std::conditional<Prec == 0, double, float>::type;
     constexpr bool exact = Prec == 0;
@@ -312,12 +314,13 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
         const double ci = mine[C::CS], ri = mine[C::RAD];
         load_vec<D>(mine + C::X, xi);
         load_vec<D>(mine + C::C, centroid);
-        Sums<D, Moving, Grad, Rule, G> s;
+        constexpr bool range_pass = LUNA_RANGE_PASS || (Moving && exact);  // the float sums fit 3 waves without it
+        Sums<D, Moving, Grad, Rule, G, range_pass> s;
         s.init();
         const int count = g.count[i], base = first_slot(g, i);
         // Groups of UN entries: their face magnitudes and rows in flight together, summed in list order; the next group's
         // other-cell indices are read one group ahead, so each group waits for one level of loads.
-        constexpr int UN = LUNA_SWEEP_UNROLL;
+        constexpr int UN = exact ? LUNA_SWEEP_UNROLL : 2;
         int nj[UN];
 #pragma unroll
         for (int u = 0; u < UN; ++u)
@@ -364,7 +367,7 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
                 }
             }
         };
-        if constexpr (Grad && !LUNA_RANGE_PASS) {
+        if constexpr (Grad && !range_pass) {
             if (any_tiny) neighbour_range();
         }
         G grad[V][D], intercept[V];
@@ -443,7 +446,7 @@ __global__ __launch_bounds__(CELL_BLOCK, LUNA_SWEEP_WAVES) void k_sweep(Geo g, R
             smallest = fmin(smallest, courant / rate);
         }
         if constexpr (Grad) {
-            if constexpr (Limit > 0 && LUNA_RANGE_PASS) neighbour_range();
+            if constexpr (Limit > 0 && range_pass) neighbour_range();
             if constexpr (Limit > 0) {  // the face-centroid limiter, face by face in list order (:538-565)
                 // Geometry::face_offset (:322-326), kept per slot, of UL entries loaded together, then applied in order.
                 constexpr int UL = LUNA_LIMIT_UNROLL;
