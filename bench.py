@@ -69,8 +69,8 @@ def dominant(times, table):
     return best, launches, total / launches
 
 
-def run_static(m, q, steps, warmup, precision="double"):
-    fv = hydro.FiniteVolume(m, GAMMA, precision=precision)
+def run_static(m, q, steps, warmup, precision="double", hll=0.0):
+    fv = hydro.FiniteVolume(m, GAMMA, precision=precision, hll_compression=hll)
     fv.set_state(q)
     for _ in range(warmup):
         fv.step(fv.time_step(COURANT))
@@ -92,8 +92,8 @@ def run_static(m, q, steps, warmup, precision="double"):
     return total, walls, times
 
 
-def run_moving(m, q, steps, warmup, precision="double"):
-    fv = hydro.FiniteVolume(m, GAMMA, motion="moving", precision=precision)
+def run_moving(m, q, steps, warmup, precision="double", hll=0.0):
+    fv = hydro.FiniteVolume(m, GAMMA, motion="moving", precision=precision, hll_compression=hll)
     fv.set_state(q)
     mesh_s, hydro_s, walls = [], [], []
     current = m
@@ -204,6 +204,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-limiter comparison")
     ap.add_argument("--no-precision", action="store_true", help="skip the grad32/grad16 policy runs")
+    ap.add_argument("--hll-compression", type=float, default=0.0,
+                    help="the HLL fallback at extreme compression in the FP64 runs (threshold; 0: off, the reference's scheme)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (scale of the rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -224,13 +226,13 @@ def main():
     log("mesh: %d cells, %d faces (F/N %.3f), built in %.1f s; blast in %d cells" % (N, F, F / N, build_s, deposited))
 
     log("static: %d warm-up + %d timed steps" % (args.warmup, args.static_steps))
-    s_total, s_walls, s_times = run_static(m, q, args.static_steps, args.warmup)
+    s_total, s_walls, s_times = run_static(m, q, args.static_steps, args.warmup, hll=args.hll_compression)
     static_rate = N * args.static_steps / s_total
     log("static: %.3f ms/step, %.3g cell updates/s; kernels %s" % (1e3 * s_total / args.static_steps, static_rate,
                                                                     {k: round(v[1] / v[0], 4) for k, v in s_times.items()}))
 
     log("moving: %d warm-up + %d timed steps" % (args.warmup, args.steps))
-    m_total, hydro_s, mesh_s, m_times = run_moving(m, q, args.steps, args.warmup)
+    m_total, hydro_s, mesh_s, m_times = run_moving(m, q, args.steps, args.warmup, hll=args.hll_compression)
     hydro_total = float(np.sum(hydro_s))
     value = N * args.steps / hydro_total
     log("moving: hydro %.3f ms/step, %.3g cell updates/s; kernels %s" % (1e3 * hydro_total / args.steps, value,
@@ -284,7 +286,7 @@ def main():
         "data": "synthetic: seeded 3D Sedov blast on a jittered lattice (jitter 0.25, seed 1), E = 1 in r < 2h",
         "config": {"workload": "3D Sedov blast, moving Voronoi mesh, %d^3 = %d cells, %d faces, HLLC, MUSCL-Hancock "
                                "two-pass moving step, CFL 0.3, FP64" % (n, N, F), "cells": N, "faces": F,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "hll_compression": args.hll_compression},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("moving " + name, n),
                      "algorithmic_bytes_per_cell": per_cell, "avg_ms": avg_ms, "launches": launches},

@@ -45,7 +45,8 @@ public:
     // FiniteVolume(exec, mesh, gas, order, update, limiter, motion) (finite_volume.hpp:79-107); mesh: the voronoi-cpp
     // output contract as luna_geometry.
     FiniteVolume(const luna_geometry& mesh, const Gas& gas, int order, Limiter limiter = Limiter::face_centroid,
-                 Motion motion = Motion::static_mesh, Riemann riemann = Riemann::hllc, Precision precision = Precision::double_)
+                 Motion motion = Motion::static_mesh, Riemann riemann = Riemann::hllc, Precision precision = Precision::double_,
+                 double hll_compression = 0)  // the HLL fallback at extreme compression (luna_hydro.h; none in the reference)
         : n_(mesh.cells) {
         luna_fv_options o{};
         o.dim = D;
@@ -58,6 +59,7 @@ public:
         o.smallr = gas.smallr;
         o.smallc = gas.smallc;
         o.precision = static_cast<int>(precision);
+        o.hll_compression = hll_compression;
         options_ = o;
         check(luna_fv_create(&o, &mesh, &fv_), "hydro::FiniteVolume");
     }

@@ -48,6 +48,11 @@ typedef struct {
                       (static and moving, order 1) with the face-centroid limiter or none; the learned limiter and the
                       derivatives need double. The reference's default (DefaultPrecision<D>, precision.hpp:49-54) is
                       grad16 in 3D and grad32 in 2D; here the default is double. */
+    double hll_compression; /* > 0: the HLL fallback at extreme compression, which has no counterpart in the reference.
+                               Each time_step's sweep flags cell i when r_i (div v)_i / c_s,i < -hll_compression,
+                               div v = sum_a dv_a/dx_a of the limited gradient and r the cell radius. The HLLC solver then
+                               takes HLL at every face with a flagged side. 0 (a zero-initialised struct): off. Needs
+                               order 1, the double policy and the face-centroid limiter or none. The derivatives refuse it. */
 } luna_fv_options;
 
 /* The mesh arrays the step reads, host pointers: voronoi-cpp's output contract as finite_volume.hpp:257-327 and :353-364

@@ -151,6 +151,7 @@ struct luna_fv {
     Buf<double> cell, facerec, slotgeo;
     Buf<double> q, grad, w, x, old_centroid, flux;
     Buf<unsigned int> exponents;  // grad16: the per-cell group exponents (int8 x 3)
+    Buf<unsigned char> hll;       // the HLL fallback's per-cell compression flags
     Buf<unsigned long long> counters;  // [0, 1] dt slots, [2, 3] fallbacks of the step's two flux passes
     // the derivatives' rows (first use): luna::Derivs
     Buf<double> d_dq, d_dW, d_dgrad, d_dflux, d_sidebar, d_bbar, d_boundbar, d_out;
@@ -197,7 +198,12 @@ struct luna_fv {
     }
     int grad_row() const { return D == 3 ? luna::grad_row<3>() : luna::grad_row<2>(); }
     luna::Rows rows() const {
-        return luna::Rows{q.p, grad.p, w.p, x.p, old_centroid.p, steps.p, flux.p, features.p, troubled.p, opt.precision, exponents.p};
+        return luna::Rows{q.p, grad.p, w.p, x.p, old_centroid.p, steps.p, flux.p, features.p, troubled.p, opt.precision, exponents.p,
+                          opt.hll_compression > 0 ? hll.p : nullptr, opt.hll_compression};
+    }
+    // the kernels' Riemann selector: 2 = HLLC with the HLL fallback at flagged cells
+    int riemann() const {
+        return opt.hll_compression > 0 && opt.riemann == LUNA_RIEMANN_HLLC ? 2 : opt.riemann;
     }
     luna::Gas gas() const { return luna::Gas{opt.gamma, opt.smallr, opt.smallc, 1.0 / (opt.gamma - 1.0)}; }
     luna::Counters counters_now() const {
@@ -480,7 +486,7 @@ int flux_and_update(luna_fv* fv, int pass, double flux_dt, double apply_dt, doub
     const luna::Gas gas = fv->gas();
     hipStream_t s = fv->stream;
     unsigned long long* count = fv->counters.p + (pass == 1 ? 3 : 2);
-    const int riemann = fv->opt.riemann, order = fv->opt.order;
+    const int riemann = fv->riemann(), order = fv->opt.order;
     LUNA_HIP(timed(fv->prof, s, pass < 0 ? "flux" : pass == 0 ? "flux_pass1" : "flux_pass2", [&] {
         return fv->D == 3 ? luna::launch_flux<3>(s, g, r, gas, riemann, order, pass, flux_dt, unit, count)
                           : luna::launch_flux<2>(s, g, r, gas, riemann, order, pass, flux_dt, unit, count);
@@ -516,6 +522,11 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
         return fail(LUNA_ERR_ARGUMENT, "hydro::Precision: double, grad32 or grad16");
     if (opt->precision != LUNA_PRECISION_DOUBLE && opt->limiter == LUNA_LIMITER_LEARNED)
         return fail(LUNA_ERR_ARGUMENT, "luna: the learned limiter's features need the double policy");
+    if (!(opt->hll_compression >= 0) || std::isinf(opt->hll_compression))
+        return fail(LUNA_ERR_ARGUMENT, "luna: hll_compression must be finite and not negative (0: off)");
+    if (opt->hll_compression > 0 &&
+        (opt->order != 1 || opt->precision != LUNA_PRECISION_DOUBLE || opt->limiter == LUNA_LIMITER_LEARNED))
+        return fail(LUNA_ERR_ARGUMENT, "luna: the HLL fallback needs order 1, the double policy and the face-centroid limiter or none");
     if (!(opt->gamma > 1)) return fail(LUNA_ERR_ARGUMENT, "luna: gamma must exceed 1");
     if (mesh->cells <= 0) return fail(LUNA_ERR_ARGUMENT, "luna: no cells");
     luna_fv* fv = new luna_fv;
@@ -541,6 +552,8 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
             return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
     }
     if (opt->precision == LUNA_PRECISION_GRAD16 && (e = fv->exponents.reserve(N)) != hipSuccess)
+        return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
+    if (opt->hll_compression > 0 && (e = fv->hll.reserve(N)) != hipSuccess)
         return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
     if (opt->motion == LUNA_MOTION_MOVING) {
         if ((e = fv->x.reserve(N * D)) != hipSuccess || (e = fv->old_centroid.reserve(N * D)) != hipSuccess ||
@@ -574,6 +587,7 @@ int luna_fv_destroy(luna_fv* fv) {
     for (auto* b : {&fv->cell, &fv->facerec, &fv->slotgeo, &fv->q, &fv->grad, &fv->w, &fv->x, &fv->old_centroid, &fv->flux}) b->release();
     for (auto* b : {&fv->count, &fv->slice, &fv->list, &fv->other, &fv->steps, &fv->d_extreme}) b->release();
     fv->exponents.release();
+    fv->hll.release();
     for (auto* b : {&fv->d_dq, &fv->d_dW, &fv->d_dgrad, &fv->d_dflux, &fv->d_sidebar, &fv->d_bbar, &fv->d_boundbar, &fv->d_out}) b->release();
     for (void* p : {static_cast<void*>(fv->stage_cell), static_cast<void*>(fv->stage_face), static_cast<void*>(fv->stage_list),
                     static_cast<void*>(fv->stage_other), static_cast<void*>(fv->stage_count), static_cast<void*>(fv->stage_slot)})
@@ -647,6 +661,7 @@ int derivatives_ready(luna_fv* fv, const char* what) {
     if (fv->opt.limiter == LUNA_LIMITER_LEARNED)
         return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of the face-centroid limiter or none (not the learned one)");
     if (fv->opt.precision != LUNA_PRECISION_DOUBLE) return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of the double policy");
+    if (fv->opt.hll_compression > 0) return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives without the HLL fallback");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, std::string(what) + ": a moving step is between begin and finish");
     const size_t N = fv->N, V = fv->V, F = fv->F;
     LUNA_HIP(fv->d_dq.reserve(N * V));
@@ -722,6 +737,7 @@ int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, c
     if (fv->opt.limiter == LUNA_LIMITER_LEARNED)
         return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: derivatives of the face-centroid limiter or none (not the learned one)");
     if (fv->opt.precision != LUNA_PRECISION_DOUBLE) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: derivatives of the double policy");
+    if (fv->opt.hll_compression > 0) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: derivatives without the HLL fallback");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: a moving step is between begin and finish");
     if (!(dt > 0)) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_adjoint: dt must be positive");
     if (mesh0->cells != fv->N || mesh1->cells != fv->N) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_adjoint: both meshes must have the handle's cells");

@@ -68,6 +68,11 @@ struct Rows {
     // double rows' allocation.
     int precision;
     unsigned int* exponents;  // N (grad16)
+    // The HLL fallback at extreme compression (no counterpart in the reference): the sweep flags cell i when
+    // r_i (div v)_i / c_s,i < -hll_threshold, div v from the limited gradient; faces with a flagged side take HLL
+    // instead of HLLC. Null when off.
+    unsigned char* hll;
+    double hll_threshold;
 };
 template <int D>
 constexpr int lp_row() {  // row_width<float>(V D): whole 32-byte sectors

@@ -521,6 +521,14 @@ __global__ __launch_bounds__(CELL_BLOCK, Limit == 2 ? 1 : LUNA_SWEEP_WAVES) void
                 }
             }
             store_gradient<D, Prec>(r, i, grad);
+            if constexpr (exact && Limit < 2) {
+                if (r.hll) {  // the compression indicator of the HLL fallback
+                    double div = 0;
+#pragma unroll
+                    for (int a = 0; a < D; ++a) div += grad[1 + a][a];
+                    r.hll[i] = div * ri / ci < -r.hll_threshold ? 1 : 0;
+                }
+            }
         }
     }
     block_min<CELL_BLOCK>(smallest, c.dt);
@@ -735,7 +743,12 @@ __global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, 
             }
             if constexpr (Riemann == 1)
                 hll<D>(gas, L, R, n, wf, F);
-            else
+            else if constexpr (Riemann == 2) {  // HLLC, HLL where a side is flagged as extremely compressed
+                if (r.hll[o] | r.hll[j])
+                    hll<D>(gas, L, R, n, wf, F);
+                else
+                    hllc<D>(gas, L, R, n, wf, F);
+            } else
                 hllc<D>(gas, L, R, n, wf, F);
 #pragma unroll
             for (int v = 0; v < V; ++v) F[v] *= magnitude;
@@ -889,6 +902,8 @@ static hipError_t flux_prec(hipStream_t s, const Geo& g, const Rows& r, const Ga
     const dim3 grid = blocks(g.F > 0 ? g.F : 1);
     if (riemann == 1)
         hipLaunchKernelGGL((k_flux<D, Pass, 1, Prec>), grid, dim3(BLOCK), 0, s, g, r, gas, order, dt, unit, fallbacks);
+    else if (Prec == 0 && riemann == 2)
+        hipLaunchKernelGGL((k_flux<D, Pass, 2, 0>), grid, dim3(BLOCK), 0, s, g, r, gas, order, dt, unit, fallbacks);
     else
         hipLaunchKernelGGL((k_flux<D, Pass, 0, Prec>), grid, dim3(BLOCK), 0, s, g, r, gas, order, dt, unit, fallbacks);
     return hipGetLastError();

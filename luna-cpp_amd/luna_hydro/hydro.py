@@ -41,7 +41,8 @@ def policy(name, dim):
 class FVOptions(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("limiter", ctypes.c_int), ("riemann", ctypes.c_int),
                 ("motion", ctypes.c_int), ("step_rule", ctypes.c_int), ("gamma", ctypes.c_double),
-                ("smallr", ctypes.c_double), ("smallc", ctypes.c_double), ("precision", ctypes.c_int)]
+                ("smallr", ctypes.c_double), ("smallc", ctypes.c_double), ("precision", ctypes.c_int),
+                ("hll_compression", ctypes.c_double)]
 
 
 class Geometry(ctypes.Structure):
@@ -149,11 +150,11 @@ class FiniteVolume:
     """hydro::FiniteVolume<D> (D = 2, 3) on the GPU: FiniteVolume(exec, mesh, gas, order, per_face, limiter, motion)."""
 
     def __init__(self, mesh, gamma, order=1, limiter="face", riemann="hllc", motion="static", step_rule="arepo",
-                 smallr=1e-10, smallc=1e-10, precision="double"):
+                 smallr=1e-10, smallc=1e-10, precision="double", hll_compression=0.0):
         self.D, self.N = mesh["dim"], mesh["cells"]
         self.V = self.D + 2
         opt = FVOptions(self.D, order, LIMITERS[limiter], RIEMANN[riemann], MOTION[motion], RULES[step_rule],
-                        gamma, smallr, smallc, PRECISIONS[precision])
+                        gamma, smallr, smallc, PRECISIONS[precision], hll_compression)
         g, self._mesh_arrays = geometry_record(mesh)
         h = ctypes.c_void_p()
         _check(lib().luna_fv_create(ctypes.byref(opt), ctypes.byref(g), ctypes.byref(h)), "FiniteVolume")

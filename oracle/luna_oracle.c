@@ -656,6 +656,11 @@ static void sweep_cell(orc_fv* fv, const orc_geometry* geo, int i) {
                                                fv->features + (size_t)i * V * ORC_FEATURES);
     }
     memcpy(fv->grad + (size_t)i * V * D, grad, sizeof(double) * V * D);
+    if (fv->hll_compression > 0 && fv->limiter != 2) { /* the HLL fallback's compression indicator */
+        double div = 0;
+        for (int a = 0; a < D; ++a) div += grad[(1 + a) * D + a];
+        fv->hll[i] = div * Wi[V + 1] / Wi[V] < -fv->hll_compression ? 1 : 0;
+    }
 }
 
 /* finite_volume.hpp:1040-1052 CellStep, :1054-1071 LunaStep, :1074-1090 FaceStep, :1107-1149 sweep(). */
@@ -840,7 +845,9 @@ static void face_flux(const orc_fv* fv, const orc_geometry* geo, int f, double d
             }
         }
     }
-    riemann(fv->riemann, D, &fv->gas, L, R, n, wf, F);
+    /* the HLL fallback: HLLC takes HLL where a side is flagged as extremely compressed */
+    const int solver = fv->riemann == 0 && fv->hll_compression > 0 && (fv->hll[o] || fv->hll[j]) ? 1 : fv->riemann;
+    riemann(solver, D, &fv->gas, L, R, n, wf, F);
     for (int v = 0; v < V; ++v) F[v] *= magnitude;
 }
 
@@ -2553,7 +2560,9 @@ static void face_sides(const orc_fv* fv, const orc_geometry* geo, int f, double 
     }
 }
 
-static int static_derivatives_ok(const orc_fv* fv) { return !fv->moving && fv->order == 1 && fv->D >= 2 && fv->precision == 0; }
+static int static_derivatives_ok(const orc_fv* fv) {
+    return !fv->moving && fv->order == 1 && fv->D >= 2 && fv->precision == 0 && !(fv->hll_compression > 0);
+}
 
 /* finite_volume.hpp:2039-2074 step_tangent(): the static step of q and dq -> J dq (dq: N x V, extensive), dt fixed.
  * Returns 0, or -1 outside a static order-1 HLLC mesh step. */
@@ -3086,7 +3095,7 @@ static void moving_fit_adjoint(const orc_fv* fv, const orc_geometry* g, int i, c
 
 int orc_moving_adjoint(orc_fv* fv, const orc_geometry* g0, const orc_geometry* g1, double dt, int steps_fixed, double* qbar, double* xbar,
                        orc_geometry_adjoint adjoint0, void* ctx0, orc_geometry_adjoint adjoint1, void* ctx1) {
-    if (!fv->moving || fv->order != 1 || fv->D < 2 || fv->precision != 0) return -1;
+    if (!fv->moving || fv->order != 1 || fv->D < 2 || fv->precision != 0 || fv->hll_compression > 0) return -1;
     const int D = fv->D, V = D + 2, U = D + 1, N = fv->N, PW = V + 2;
     /* t^n: W, the limited gradients and the point velocities (the sweep), c^n, the grid steps and w-hat. */
     orc_primitives(fv, g0);

@@ -77,6 +77,10 @@ typedef struct {
     int hold_steps;           /* moving: orc_moving_begin keeps the given steps (the adjoint's central differences) */
     int precision;            /* the gradient policy (precision.hpp:26-54): 0 double, 1 grad32, 2 grad16; grad keeps the
                                  stored rows decoded to double (floats, exactly) */
+    double hll_compression;   /* > 0: the HLL fallback at extreme compression (no counterpart in the reference): the sweep
+                                 sets hll[i] when r_i (div v)_i / c_s,i < -hll_compression (div v of the limited gradient);
+                                 HLLC takes HLL at faces with a flagged side */
+    unsigned char* hll;       /* N, written by the sweep when hll_compression > 0 */
 } orc_fv;
 
 void orc_primitives(orc_fv* fv, const orc_geometry* geo);

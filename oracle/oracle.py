@@ -34,7 +34,8 @@ class _FV(ctypes.Structure):
                 ("q", _dp), ("prims", _dp), ("grad", _dp), ("w", _dp), ("x", _dp), ("old_centroid", _dp),
                 ("steps", _ip), ("flux", _dp), ("fallbacks", ctypes.c_longlong * 2), ("prims_ready", ctypes.c_int),
                 ("features", ctypes.POINTER(ctypes.c_ushort)), ("troubled", ctypes.POINTER(ctypes.c_ubyte)),
-                ("hold_steps", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("hold_steps", ctypes.c_int), ("precision", ctypes.c_int), ("hll_compression", ctypes.c_double),
+                ("hll", ctypes.POINTER(ctypes.c_ubyte))]
 
 
 _GEO_ADJOINT = ctypes.CFUNCTYPE(None, ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp)
@@ -228,7 +229,8 @@ class Geometry:
 class FiniteVolume:
     """The oracle's FiniteVolume<D> (D = 2, 3): state, time_step, step, and the moving step split at the mesh update."""
 
-    def __init__(self, mesh, gamma, order=1, limiter="face", riemann="hllc", moving=False, step_rule="arepo", precision="double"):
+    def __init__(self, mesh, gamma, order=1, limiter="face", riemann="hllc", moving=False, step_rule="arepo", precision="double",
+                 hll_compression=0.0):
         D, N = mesh["dim"], mesh["cells"]
         if precision != "double" and limiter == "learned":
             raise ValueError("the learned limiter's features need the double policy")
@@ -249,6 +251,10 @@ class FiniteVolume:
         c.moving = int(moving)
         c.step_rule = {"arepo": 0, "luna": 1}[step_rule]
         c.precision = {"double": 0, "grad32": 1, "grad16": 2}[precision]
+        # the HLL fallback's flags (written by each sweep when hll_compression > 0)
+        self.hll = np.zeros(N, dtype=np.uint8)
+        c.hll_compression = hll_compression
+        c.hll = self.hll.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
         c.gas = gas(gamma)
         c.q, c.prims, c.grad, c.w = _ptr(self.q), _ptr(self.prims), _ptr(self.grad), _ptr(self.w)
         c.x, c.old_centroid, c.steps, c.flux = _ptr(self.x), _ptr(self.old_centroid), _ptr(self.steps, _ip), _ptr(self.flux)
