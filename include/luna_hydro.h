@@ -41,7 +41,7 @@ typedef struct {
     int order;     /* 0 or 1 */
     int limiter;   /* LUNA_LIMITER_FACE_CENTROID, LUNA_LIMITER_NONE or LUNA_LIMITER_LEARNED */
     int riemann;   /* LUNA_RIEMANN_HLLC or LUNA_RIEMANN_HLL */
-    int motion;    /* LUNA_MOTION_STATIC or LUNA_MOTION_MOVING (order 1) */
+    int motion;    /* LUNA_MOTION_STATIC or LUNA_MOTION_MOVING */
     int step_rule; /* LUNA_STEP_AREPO, or LUNA_STEP_LUNA in 2D (set_step_rule, finite_volume.hpp:181) */
     double gamma, smallr, smallc;
     int precision; /* LUNA_PRECISION_*; a zero-initialised struct asks for double. The reduced policies step forward
@@ -103,6 +103,9 @@ int luna_fv_step(luna_fv* fv, double dt);
  * writes the next primitives. */
 int luna_fv_moving_begin(luna_fv* fv, double dt, double* positions);
 int luna_fv_moving_finish(luna_fv* fv, const luna_geometry* mesh, double dt);
+/* At order 0 (advance<0, true>() + move(), finite_volume.hpp:216-225, 1211-1250): begin applies one flux pass over dt on
+ * the mesh of t^n, with face velocities from the sweep's point velocities, and moves the points off the grid
+ * (x + dt w wrapped into the box); finish takes the new mesh and writes its primitives (Q is unchanged). */
 
 /* Uploads a mesh. Between moving_begin and moving_finish: the mesh of the new positions. Otherwise: a caller moved the
  * mesh itself (mesh_moved(), finite_volume.hpp:152-156), and the cached rows are dropped; a moving solver's positions

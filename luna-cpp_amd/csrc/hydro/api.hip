@@ -487,11 +487,11 @@ int flux_and_update(luna_fv* fv, int pass, double flux_dt, double apply_dt, doub
     hipStream_t s = fv->stream;
     unsigned long long* count = fv->counters.p + (pass == 1 ? 3 : 2);
     const int riemann = fv->riemann(), order = fv->opt.order;
-    LUNA_HIP(timed(fv->prof, s, pass < 0 ? "flux" : pass == 0 ? "flux_pass1" : "flux_pass2", [&] {
+    LUNA_HIP(timed(fv->prof, s, pass < 0 || pass == 2 ? "flux" : pass == 0 ? "flux_pass1" : "flux_pass2", [&] {
         return fv->D == 3 ? luna::launch_flux<3>(s, g, r, gas, riemann, order, pass, flux_dt, unit, count)
                           : luna::launch_flux<2>(s, g, r, gas, riemann, order, pass, flux_dt, unit, count);
     }));
-    LUNA_HIP(timed(fv->prof, s, pass < 0 ? "update" : pass == 0 ? "update_pass1" : "update_pass2", [&] {
+    LUNA_HIP(timed(fv->prof, s, pass < 0 || pass == 2 ? "update" : pass == 0 ? "update_pass1" : "update_pass2", [&] {
         return fv->D == 3 ? luna::launch_update<3>(s, g, r, gas, apply_dt, next_prims, drift)
                           : luna::launch_update<2>(s, g, r, gas, apply_dt, next_prims, drift);
     }));
@@ -515,8 +515,6 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
         return fail(LUNA_ERR_ARGUMENT, "luna: the learned limiter needs order 1");
     if (opt->riemann != LUNA_RIEMANN_HLLC && opt->riemann != LUNA_RIEMANN_HLL) return fail(LUNA_ERR_ARGUMENT, "luna: unknown Riemann solver");
     if (opt->motion != LUNA_MOTION_STATIC && opt->motion != LUNA_MOTION_MOVING) return fail(LUNA_ERR_ARGUMENT, "luna: unknown motion");
-    if (opt->motion == LUNA_MOTION_MOVING && opt->order != 1)
-        return fail(LUNA_ERR_ARGUMENT, "luna: a moving mesh needs order 1 (the two-pass MUSCL-Hancock step)");
     if (opt->step_rule == LUNA_STEP_LUNA && opt->dim != 2) return fail(LUNA_ERR_ARGUMENT, "StepRule::luna: 2D");
     if (opt->precision < LUNA_PRECISION_DOUBLE || opt->precision > LUNA_PRECISION_GRAD16)
         return fail(LUNA_ERR_ARGUMENT, "hydro::Precision: double, grad32 or grad16");
@@ -850,6 +848,17 @@ int luna_fv_moving_begin(luna_fv* fv, double dt, double* positions) {
     const luna::Rows r = fv->rows();
     hipStream_t s = fv->stream;
     const luna::Grid grid = fv->grid;
+    if (fv->opt.order == 0) {  // finite_volume.hpp:216-225: advance<0, true>() on the mesh of t^n, then move()
+        if (int rc = flux_and_update(fv, 2, dt, dt, 0.0, false)) return rc;
+        LUNA_HIP(timed(fv->prof, s, "drift", [&] {
+            return fv->D == 3 ? luna::launch_drift<3>(s, g, r, dt) : luna::launch_drift<2>(s, g, r, dt);
+        }));
+        if (positions) LUNA_HIP(hipMemcpyAsync(positions, fv->x.p, sizeof(double) * fv->N * fv->D, hipMemcpyDeviceToHost, s));
+        LUNA_HIP(hipStreamSynchronize(s));
+        fv->begun = true;
+        fv->new_mesh = false;
+        return LUNA_OK;
+    }
     LUNA_HIP(timed(fv->prof, s, "keep_centroids", [&] {
         return fv->D == 3 ? luna::launch_keep_centroids<3>(s, g, r, grid, dt) : luna::launch_keep_centroids<2>(s, g, r, grid, dt);
     }));
@@ -885,6 +894,19 @@ int luna_fv_moving_finish(luna_fv* fv, const luna_geometry* mesh, double dt) {
         if (int rc = upload(fv, mesh)) return rc;
     } else if (!fv->new_mesh) {
         return fail(LUNA_ERR_LOGIC, "luna_fv_moving_finish: no mesh of the new positions (pass one or call luna_fv_set_geometry)");
+    }
+    if (fv->opt.order == 0) {  // Q stays; the new mesh's primitives
+        const luna::Geo g = fv->geo();
+        const luna::Rows r = fv->rows();
+        const luna::Gas gas = fv->gas();
+        hipStream_t s = fv->stream;
+        LUNA_HIP(timed(fv->prof, s, "primitives", [&] {
+            return fv->D == 3 ? luna::launch_primitives<3>(s, g, r, gas) : luna::launch_primitives<2>(s, g, r, gas);
+        }));
+        fv->prims_ready = true;
+        fv->begun = false;
+        fv->new_mesh = false;
+        return LUNA_OK;
     }
     const double unit = fv->grid.step / dt;
     if (int rc = flux_and_update(fv, 1, dt, 0.5 * dt, unit, true)) return rc;

@@ -107,6 +107,9 @@ template <int D>
 hipError_t launch_update(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double dt, bool next_prims, const Grid* drift);
 template <int D>
 hipError_t launch_keep_centroids(hipStream_t s, const Geo& g, const Rows& r, const Grid& grid, double dt);
+// move() off the grid (order 0 on a moving mesh): x += dt w, wrapped into the box.
+template <int D>
+hipError_t launch_drift(hipStream_t s, const Geo& g, const Rows& r, double dt);
 
 // Derivatives of the static order-1 step (HLLC or HLL) at fixed dt (adjoint_kernels.hip). Device rows:
 //   dq       N x V   in: the tangent of Q (tangent) or Q bar after the step (adjoint); the tangent advances it to J dq

@@ -703,8 +703,13 @@ __global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, 
                 double wo[D], wj[D];
 #pragma unroll
                 for (int k = 0; k < D; ++k) {
-                    wo[k] = (double)r.steps[(size_t)o * D + k] * unit;
-                    wj[k] = (double)r.steps[(size_t)j * D + k] * unit;
+                    if constexpr (Pass == 2) {  // order 0 on a moving mesh: the sweep's point velocities (FaceFlux<0, true>)
+                        wo[k] = r.w[(size_t)o * D + k];
+                        wj[k] = r.w[(size_t)j * D + k];
+                    } else {
+                        wo[k] = (double)r.steps[(size_t)o * D + k] * unit;
+                        wj[k] = (double)r.steps[(size_t)j * D + k] * unit;
+                    }
                 }
                 // face_velocity (:1380-1397)
                 double sep[D], cf[D], dw[D];
@@ -720,7 +725,10 @@ __global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, 
                 if (c2 > limit * limit) facv *= limit / sqrt(c2);
 #pragma unroll
                 for (int k = 0; k < D; ++k) wf[k] = (wo[k] + wj[k]) * 0.5 + sep[k] * (facv * inv_dist);
-                if constexpr (Pass == 0) {
+                if constexpr (Pass == 2) {
+                    face_side_p<D, Prec>(gas, r, o, so.p, so.g, ro, dt, 0, L, count);
+                    face_side_p<D, Prec>(gas, r, j, sj.p, sj.g, rj, dt, 0, R, count);
+                } else if constexpr (Pass == 0) {
                     pass_side_first_p<D, Prec>(r, o, so.p, so.g, ro, L, count);
                     pass_side_first_p<D, Prec>(r, j, sj.p, sj.g, rj, R, count);
                 } else {
@@ -854,9 +862,29 @@ __global__ __launch_bounds__(BLOCK) void k_keep_centroids(Geo g, Rows r, Grid gr
     }
 }
 
+// finite_volume.hpp:1227-1228 move() off the grid (order 0 on a moving mesh): x + dt w, wrapped into [0, side).
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_drift(Geo g, Rows r, double dt) {
+    const int i = xcd_block() * BLOCK + threadIdx.x;
+    if (i >= g.N) return;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const size_t at = (size_t)i * D + a;
+        const double moved = r.x[at] + dt * r.w[at];
+        r.x[at] = moved - g.side[a] * floor(moved / g.side[a]);
+    }
+}
+
 inline dim3 blocks(long long n, int b = BLOCK) { return dim3((unsigned)((n + b - 1) / b)); }
 
 }  // namespace
+
+template <int D>
+hipError_t launch_drift(hipStream_t s, const Geo& g, const Rows& r, double dt) {
+    if (g.N == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_drift<D>, blocks(g.N), dim3(BLOCK), 0, s, g, r, dt);
+    return hipGetLastError();
+}
 
 template <int D>
 hipError_t launch_primitives(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas) {
@@ -921,6 +949,7 @@ hipError_t launch_flux(hipStream_t s, const Geo& g, const Rows& r, const Gas& ga
                        unsigned long long* fallbacks) {
     if (pass < 0) return flux_pass<D, -1>(s, g, r, gas, riemann, order, dt, unit, fallbacks);
     if (pass == 0) return flux_pass<D, 0>(s, g, r, gas, riemann, order, dt, unit, fallbacks);
+    if (pass == 2) return flux_pass<D, 2>(s, g, r, gas, riemann, 0, dt, unit, fallbacks);
     return flux_pass<D, 1>(s, g, r, gas, riemann, order, dt, unit, fallbacks);
 }
 
@@ -951,7 +980,8 @@ hipError_t launch_keep_centroids(hipStream_t s, const Geo& g, const Rows& r, con
     template hipError_t launch_flux<D>(hipStream_t, const Geo&, const Rows&, const Gas&, int, int, int, double, double,              \
                                        unsigned long long*);                                                                        \
     template hipError_t launch_update<D>(hipStream_t, const Geo&, const Rows&, const Gas&, double, bool, const Grid*);              \
-    template hipError_t launch_keep_centroids<D>(hipStream_t, const Geo&, const Rows&, const Grid&, double);
+    template hipError_t launch_keep_centroids<D>(hipStream_t, const Geo&, const Rows&, const Grid&, double);                      \
+    template hipError_t launch_drift<D>(hipStream_t, const Geo&, const Rows&, double);
 
 LUNA_INSTANTIATE(2)
 LUNA_INSTANTIATE(3)

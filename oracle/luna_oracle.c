@@ -818,12 +818,22 @@ static void face_flux(const orc_fv* fv, const orc_geometry* geo, int f, double d
         const double inv = 1.0 / magnitude;
         for (int k = 0; k < D; ++k) n[k] = a[k] * inv;
         double wo[MAXD], wj[MAXD];
-        grid_velocity(fv, o, unit, wo);
-        grid_velocity(fv, j, unit, wj);
+        if (pass == 2) { /* order 0 on a moving mesh: the point velocities of the sweep (FaceFlux<0, true>) */
+            for (int k = 0; k < D; ++k) {
+                wo[k] = fv->w[(size_t)o * D + k];
+                wj[k] = fv->w[(size_t)j * D + k];
+            }
+        } else {
+            grid_velocity(fv, o, unit, wo);
+            grid_velocity(fv, j, unit, wj);
+        }
         face_velocity(geo, f, o, j, wo, wj, wf);
         face_offset(geo, f, o, ro);
         face_offset(geo, f, j, rj);
-        if (pass == 0) {
+        if (pass == 2) {
+            face_side(fv, o, ro, dt, L, fallbacks);
+            face_side(fv, j, rj, dt, R, fallbacks);
+        } else if (pass == 0) {
             pass_side_first(fv, o, ro, L, fallbacks);
             pass_side_first(fv, j, rj, R, fallbacks);
         } else {
@@ -911,6 +921,18 @@ int orc_delta(long long k0, long long k1, long long period) {
  * drift (:1219-1231) onto the grid point the steps reach. The caller then builds the mesh of the new positions. */
 void orc_moving_begin(orc_fv* fv, const orc_geometry* geo, double dt) {
     const int D = fv->D;
+    if (fv->order == 0) { /* finite_volume.hpp:216-225: advance<0, true>() on the mesh of t^n, then move(): x += dt w */
+        fv->fallbacks[1] = 0;
+        apply(fv, geo, dt, dt, 2, 0.0);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < fv->N; ++i)
+            for (int a = 0; a < D; ++a) { /* :1227-1228, off the grid */
+                const size_t at = (size_t)i * D + a;
+                const double moved = fv->x[at] + dt * fv->w[at];
+                fv->x[at] = moved - geo->side[a] * floor(moved / geo->side[a]);
+            }
+        return;
+    }
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < fv->N; ++i)
         for (int a = 0; a < D; ++a) {
@@ -935,6 +957,11 @@ void orc_moving_begin(orc_fv* fv, const orc_geometry* geo, double dt) {
 
 /* finite_volume.hpp:1517-1518: the second pass on the mesh of t^n+1 (half dt); its update writes the next primitives. */
 void orc_moving_finish(orc_fv* fv, const orc_geometry* geo, double dt) {
+    if (fv->order == 0) { /* the new mesh's primitives */
+        orc_primitives(fv, geo);
+        fv->prims_ready = 1;
+        return;
+    }
     apply(fv, geo, dt, 0.5 * dt, 1, geo->step / dt);
     fv->prims_ready = 1;
 }

@@ -100,6 +100,29 @@ def test_moving_steps_match_oracle(dim, n):
     say("  bits equal: %s" % np.array_equal(gpu.state(), orc.q))
 
 
+@pytest.mark.parametrize("dim,n,kw", [(3, 10, {}), (2, 24, {}), (3, 10, {"riemann": "hll"})])
+def test_moving_order0_steps_match_oracle(dim, n, kw):
+    """Order 0 on a moving mesh (advance<0, true>() and move() off the grid, finite_volume.hpp:216-225)."""
+    say("moving order 0 parity dim=%d n=%d %s" % (dim, n, kw))
+    m = cases.voronoi(n, dim)
+    q = cases.rough_state(m)
+    gpu, orc = moving_pair(m, q, order=0, **kw)
+    try:
+        for s in range(3):
+            dt = orc.time_step(0.3)
+            assert gpu.time_step(0.3) == dt
+            assert np.array_equal(gpu.point_velocities(), orc.w)
+            xg = gpu.moving_begin(dt)
+            assert np.array_equal(xg, orc.moving_begin(dt)), s
+            m = mesh.build(xg, 1.0)
+            gpu.moving_finish(m, dt)
+            orc.moving_finish(m, dt)
+            assert np.array_equal(gpu.state(), orc.q), s
+            assert gpu.fallbacks() == orc.fallbacks()
+    finally:
+        gpu.close()
+
+
 def line_pair(name, n, limiter):
     pb = problems.problem_1d(name)
     keys, step, period = problems.line_keys(problems.line_points(n, pb.L), pb.L)
