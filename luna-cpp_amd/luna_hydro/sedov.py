@@ -4,7 +4,8 @@ Voronoi mesh, the volume-weighted blast, time_step + step to t_end with snapshot
 volume) so runs compare line by line.
 
     python -m luna_hydro.sedov --n 64 [--moving] [--t-end 0.05] [--courant 0.3] [--snapshots 10] [--out DIR] [--max-steps K]
-                               [--precision default|double|grad32|grad16]
+                               [--precision default|double|grad32|grad16] [--order 0|1] [--limiter face|none]
+                               [--riemann hllc|hll] [--hll-compression K]
 
 --precision as the reference's (hydro_sedov.cpp:409-428): default is its DefaultPrecision<3>, grad16.
 """
@@ -64,11 +65,12 @@ class _Engine:
 
 
 def run(n=64, moving=False, t_end=0.05, courant=0.3, snapshots=10, out="", max_steps=0, jitter=0.25, seed=1, log=print,
-        precision="double"):
+        precision="double", order=1, limiter="face", riemann="hllc", hll_compression=0.0):
     m = mesh.build(problems.jittered_lattice(n, 3, jitter, seed), 1.0)
     q, deposited = problems.sedov_state(m)
     N = m["cells"]
-    fv = hydro.FiniteVolume(m, GAMMA, motion="moving" if moving else "static", precision=hydro.policy(precision, 3))
+    fv = hydro.FiniteVolume(m, GAMMA, order=order, limiter=limiter, riemann=riemann, motion="moving" if moving else "static",
+                            precision=hydro.policy(precision, 3) if order == 1 else "double", hll_compression=hll_compression)
     fv.set_state(q)
     engine = _Engine(fv, moving)
     before = totals(q)
@@ -111,8 +113,9 @@ def run(n=64, moving=False, t_end=0.05, courant=0.3, snapshots=10, out="", max_s
     after = totals(qf)
     median = float(np.median(walls)) if walls else float("nan")
     kernels = fv.kernel_times()
-    log("run: precision=%s order=1 update=face steps=%d limiter=face riemann=hllc mesh=%s blast=%d t=%.17g wall_s=%.6g "
-        "s_per_step=%.6g step_wall_ms_median=%.6g G_cell_updates_per_s=%.6g" % (precision, steps, "moving" if moving else "static", deposited, t,
+    log("run: precision=%s order=%d update=face steps=%d limiter=%s riemann=%s mesh=%s blast=%d t=%.17g wall_s=%.6g "
+        "s_per_step=%.6g step_wall_ms_median=%.6g G_cell_updates_per_s=%.6g" % (precision, order, steps, limiter, riemann,
+                                                                                "moving" if moving else "static", deposited, t,
                                                                                 wall_s, wall_s / max(steps, 1), median,
                                                                                 N / (median * 1e-3) / 1e9))
     log("fallbacks: total=%d steps_with=%d most_in_a_step=%d face_sides_per_step=%d" % (fallbacks, with_fallbacks, most, 2 * fv.faces()))
@@ -138,8 +141,13 @@ def main(argv=None):
     ap.add_argument("--max-steps", type=int, default=0)
     ap.add_argument("--precision", choices=["default", "double", "grad32", "grad16"], default="default",
                     help="default: the reference's DefaultPrecision<3> (grad16)")
+    ap.add_argument("--order", type=int, choices=[0, 1], default=1)
+    ap.add_argument("--limiter", choices=["face", "none"], default="face")
+    ap.add_argument("--riemann", choices=["hllc", "hll"], default="hllc")
+    ap.add_argument("--hll-compression", type=float, default=0.0, help="the HLL fallback at extreme compression (0: off)")
     a = ap.parse_args(argv)
-    run(a.n, a.moving, a.t_end, a.courant, a.snapshots, a.out, a.max_steps, precision=a.precision)
+    run(a.n, a.moving, a.t_end, a.courant, a.snapshots, a.out, a.max_steps, precision=a.precision, order=a.order, limiter=a.limiter,
+        riemann=a.riemann, hll_compression=a.hll_compression)
 
 
 if __name__ == "__main__":
