@@ -96,6 +96,15 @@ public:
         check(luna_fv_time_step(fv_, courant, &dt), "time_step");
         return dt;
     }
+    // Derivatives of one static order-1 step at fixed dt (:1941-2231): step_tangent advances Q and dq (dq becomes J dq);
+    // step_adjoint maps qbar after the step to J^T qbar before it (Q stays); time_step_tangent is time_step() for them.
+    void step_tangent(double dt, std::vector<double>& dq) { check(luna_fv_step_tangent(fv_, dt, dq.data(), 0), "step_tangent"); }
+    void step_adjoint(double dt, std::vector<double>& qbar) { check(luna_fv_step_adjoint(fv_, dt, qbar.data(), 0), "step_adjoint"); }
+    double time_step_tangent(double courant, const std::vector<double>& dq) {
+        double dt = 0;
+        check(luna_fv_time_step_tangent(fv_, courant, dq.data(), 0, &dt), "time_step_tangent");
+        return dt;
+    }
     // step(dt) (:205-226) on a static mesh. On a moving mesh the reference's step() calls the mesh library itself; here the
     // caller passes a producer: std::vector<double> positions -> luna_geometry (kept alive by the caller until return).
     void step(double dt) { check(luna_fv_step(fv_, dt), "step"); }

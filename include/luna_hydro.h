@@ -221,6 +221,10 @@ int luna_fv1_step_adjoint(luna_fv1* fv, double dt, double* qbar);
  * of the step) stays. Both block until done. LUNA_ERR_LOGIC on a moving mesh or order 0. */
 int luna_fv_step_tangent(luna_fv* fv, double dt, double* dq, int device);
 int luna_fv_step_adjoint(luna_fv* fv, double dt, double* qbar, int device);
+/* FiniteVolume::time_step_tangent(courant, dq) (:1941-1964): dt, as luna_fv_time_step; the reference also keeps the sweep's
+ * tangent for the next step_tangent with the same dq, which here takes it itself (the same numbers), so dq is not read.
+ * The derivatives' conditions apply (LUNA_ERR_LOGIC otherwise). */
+int luna_fv_time_step_tangent(luna_fv* fv, double courant, const double* dq, int device, double* dt);
 
 /* The geometry's derivative the moving step's reverse reads (hydro::GeometryDerivative::adjoint, mesh_derivative.hpp:28-29): from
  * the bars of V, c (cells) and A, x_f (faces), add to xbar (cells x dim); host arrays; 0 on success. The mesh library's

@@ -703,6 +703,13 @@ int luna_fv_step_tangent(luna_fv* fv, double dt, double* dq, int device) {
     return LUNA_OK;
 }
 
+int luna_fv_time_step_tangent(luna_fv* fv, double courant, const double* dq, int device, double* dt) {
+    (void)device;
+    if (!fv || !dq || !dt) return fail(LUNA_ERR_ARGUMENT, "luna_fv_time_step_tangent: null argument");
+    if (int rc = derivatives_ready(fv, "luna_fv_time_step_tangent")) return rc;
+    return luna_fv_time_step(fv, courant, dt);
+}
+
 int luna_fv_step_adjoint(luna_fv* fv, double dt, double* qbar, int device) {
     if (!fv || !qbar) return fail(LUNA_ERR_ARGUMENT, "luna_fv_step_adjoint: null argument");
     if (int rc = derivatives_ready(fv, "luna_fv_step_adjoint")) return rc;

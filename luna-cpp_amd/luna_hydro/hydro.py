@@ -121,6 +121,7 @@ def exported_symbols():
              "luna_fv1_mesh_stats",
              "luna_last_error",
              "luna_fv_set_geometry", "luna_synchronize", "luna_fv_step_tangent", "luna_fv_step_adjoint", "luna_fv_moving_adjoint",
+             "luna_fv_time_step_tangent",
              "luna_fv_set_limiter_network", "luna_fv_limiter_features", "luna_mlp_create", "luna_mlp_destroy", "luna_mlp_set_params",
              "luna_mlp_get_params", "luna_mlp_forward", "luna_mlp_gradient", "luna_mlp_train_step", "luna_mlp_synchronize"]
     return {n: hasattr(lib(), n) for n in names}
@@ -189,6 +190,15 @@ class FiniteVolume:
 
     def step(self, dt):
         _check(lib().luna_fv_step(self._h, ctypes.c_double(dt)), "step")
+
+    def time_step_tangent(self, courant, dq):
+        """FiniteVolume::time_step_tangent(courant, dq): dt as time_step(); the next step_tangent(dt, dq) takes the sweep's
+        tangent itself."""
+        dt = ctypes.c_double()
+        probe = np.zeros(1)
+        _check(lib().luna_fv_time_step_tangent(self._h, ctypes.c_double(courant), probe.ctypes.data_as(ctypes.c_void_p), 0,
+                                               ctypes.byref(dt)), "time_step_tangent")
+        return dt.value
 
     def step_tangent(self, dt, dq):
         """FiniteVolume::step_tangent(): one static step of Q and of the tangent dq (N x V, extensive); returns J dq. dq: a

@@ -5,6 +5,7 @@ relative per entry (the same operation order on both sides; the test reports whe
 import numpy as np
 import pytest
 
+import cases
 import oracle
 from golden_data import load
 from luna_hydro import hydro, problems
@@ -223,3 +224,27 @@ def test_autograd_rollout_matches_fd_and_oracle():
         worst = max(worst, abs(fd - ad) / max(abs(ad), 1e-30))
     say("autograd rollout: oracle %.2e, worst FD rel %.2e" % (close(grad, bar), worst))
     assert worst < 1e-6
+
+
+def test_time_step_tangent_is_time_step_for_the_tangent():
+    """time_step_tangent(courant, dq) (finite_volume.hpp:1941-1964) gives time_step()'s dt, and the step_tangent after it
+    the same J dq; a moving mesh refuses it as the other derivatives."""
+    m = cases.voronoi(8, 3)
+    q = cases.rough_state(m)
+    dq = np.random.default_rng(3).standard_normal(q.shape) * 1e-3
+    a = hydro.FiniteVolume(m, 5.0 / 3.0)
+    b = hydro.FiniteVolume(m, 5.0 / 3.0)
+    c = hydro.FiniteVolume(m, 5.0 / 3.0, motion="moving")
+    try:
+        a.set_state(q)
+        b.set_state(q)
+        c.set_state(q)
+        dt = a.time_step(0.3)
+        assert b.time_step_tangent(0.3, dq) == dt
+        assert np.array_equal(a.step_tangent(dt, dq.copy()), b.step_tangent(dt, dq.copy()))
+        with pytest.raises(RuntimeError):
+            c.time_step_tangent(0.3, dq)
+    finally:
+        a.close()
+        b.close()
+        c.close()
