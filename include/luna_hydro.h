@@ -238,7 +238,9 @@ typedef int (*luna_geometry_adjoint_fn)(const void* ctx, const double* Vbar, con
  * before it. adjoint0 / adjoint1: the two meshes' geometry adjoints (called on the host, mesh1's first). steps_fixed: the whole
  * grid steps the forward took are held (derivative 0, as central differences small enough to change no step see it);
  * otherwise the quantisation is the identity (x^n+1 = x^n + dt w). Leaves the handle on mesh0 with Q unchanged; blocks.
- * LUNA_ERR_LOGIC on a static mesh, order 0 or the learned limiter. */
+ * At order 0 (step_moving_adjoint, :2490-2598: one pass on mesh0 with the point velocities, x^n+1 = x^n + dt w) mesh1,
+ * adjoint1 and steps_fixed are not used (pass mesh0 and adjoint0 for them). LUNA_ERR_LOGIC on a static mesh, the learned
+ * limiter, a reduced precision policy or the HLL fallback. */
 int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, const luna_geometry* mesh1, luna_geometry_adjoint_fn adjoint0,
                            const void* ctx0, luna_geometry_adjoint_fn adjoint1, const void* ctx1, int steps_fixed, double* qbar, double* xbar);
 

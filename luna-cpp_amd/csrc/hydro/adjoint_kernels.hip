@@ -738,12 +738,13 @@ __device__ void face_velocity_adjoint(const double* d, const double* xf, const d
     }
 }
 
-// One moving pass's face: its two sides (first pass W + g.r; second half_step(W, g, 2 dt) + g.(r + c - c^n + dt w-hat)),
+// One moving pass's face (order0: the order-0 step's pass, sides W, w-hat = w, Gbar = -dt (qbar_o - qbar_j), no gradient
+// bars): its two sides (first pass W + g.r; second half_step(W, g, 2 dt) + g.(r + c - c^n + dt w-hat)),
 // G = |A| F at the face velocity, Gbar = -dt/2 (qbar_o - qbar_j); the sides' bars (W, g, offset), the w-hat bars, the
 // separation's bar into the row; A's and x_f's bars (x_f also from both offsets) into this face's rows of Ab, Xfb.
 template <int D, bool Second>
 __global__ __launch_bounds__(BLOCK) void k_mpass_face(Geo g, Rows r, Gas gas, int riemann_kind, double dt, double unit, const double* qbar,
-                                                      double* row, double* Ab, double* Xfb) {
+                                                      double* row, double* Ab, double* Xfb, int order0) {
     constexpr int V = D + 2, RW = pass_row<D>(), GR = grad_row<D>(), SIDE = V + V * D + D;
     using C = CellAt<D>;
     using FA = FaceAt<D>;
@@ -766,7 +767,8 @@ __global__ __launch_bounds__(BLOCK) void k_mpass_face(Geo g, Rows r, Gas gas, in
         for (int s2 = 0; s2 < 2; ++s2) {
             load_vec<D>(g.cell + (size_t)cells[s2] * CELL_ROW + C::X, xs[s2]);
 #pragma unroll
-            for (int a = 0; a < D; ++a) wh[s2][a] = (double)r.steps[(size_t)cells[s2] * D + a] * unit;
+            for (int a = 0; a < D; ++a)  // order 0: the sweep's point velocity itself
+                wh[s2][a] = order0 ? r.w[(size_t)cells[s2] * D + a] : (double)r.steps[(size_t)cells[s2] * D + a] * unit;
         }
         separation<D>(g, xs[0], xs[1], d);     // owner -> neighbour
         separation<D>(g, xs[1], xs[0], back);  // neighbour -> owner
@@ -780,7 +782,7 @@ __global__ __launch_bounds__(BLOCK) void k_mpass_face(Geo g, Rows r, Gas gas, in
 #pragma unroll
             for (int v = 0; v < V; ++v) W[s2][v] = cr[C::W + v];
 #pragma unroll
-            for (int k = 0; k < V * D; ++k) gr[s2][k] = r.grad[(size_t)i * GR + k];
+            for (int k = 0; k < V * D; ++k) gr[s2][k] = order0 ? 0.0 : r.grad[(size_t)i * GR + k];  // order 0: no gradient
 #pragma unroll
             for (int a = 0; a < D; ++a) {
                 rr[s2][a] = (s2 == 0 ? fc[a] : fc[a] + back[a]) - cr[C::C + a];
@@ -802,7 +804,7 @@ __global__ __launch_bounds__(BLOCK) void k_mpass_face(Geo g, Rows r, Gas gas, in
         double Gbar[V], Lbar[V], Rbar[V], wfbar[D], dbar[D];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            Gbar[v] = -0.5 * dt * (qbar[(size_t)pair.x * V + v] - qbar[(size_t)pair.y * V + v]);
+            Gbar[v] = (order0 ? -dt : -0.5 * dt) * (qbar[(size_t)pair.x * V + v] - qbar[(size_t)pair.y * V + v]);
             Lbar[v] = Rbar[v] = 0;
         }
 #pragma unroll
@@ -840,7 +842,7 @@ __global__ __launch_bounds__(BLOCK) void k_mpass_face(Geo g, Rows r, Gas gas, in
             for (int v = 0; v < V; ++v)
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
-                    gb[v * D + a] += sb[v] * rr[s2][a];
+                    if (!order0) gb[v * D + a] += sb[v] * rr[s2][a];
                     rb[a] += sb[v] * gr[s2][v * D + a];
                 }
 #pragma unroll
@@ -1289,13 +1291,14 @@ hipError_t launch_step_adjoint(hipStream_t s, const Geo& g, const Rows& r, const
 
 template <int D>
 hipError_t launch_moving_pass_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, double unit, bool second,
-                                      const double* qbar, const MovingBars& b) {
+                                      const double* qbar, const MovingBars& b, bool order0) {
     if (g.N == 0) return hipSuccess;
     if (g.F > 0) {
         if (second)
-            hipLaunchKernelGGL((k_mpass_face<D, true>), grid_of(g.F), dim3(BLOCK), 0, s, g, r, gas, riemann, dt, unit, qbar, b.row, b.Ab, b.Xfb);
+            hipLaunchKernelGGL((k_mpass_face<D, true>), grid_of(g.F), dim3(BLOCK), 0, s, g, r, gas, riemann, dt, unit, qbar, b.row, b.Ab, b.Xfb, 0);
         else
-            hipLaunchKernelGGL((k_mpass_face<D, false>), grid_of(g.F), dim3(BLOCK), 0, s, g, r, gas, riemann, dt, unit, qbar, b.row, b.Ab, b.Xfb);
+            hipLaunchKernelGGL((k_mpass_face<D, false>), grid_of(g.F), dim3(BLOCK), 0, s, g, r, gas, riemann, dt, unit, qbar, b.row, b.Ab, b.Xfb,
+                               order0 ? 1 : 0);
     }
     if (second)
         hipLaunchKernelGGL((k_mpass_gather<D, true>), grid_of(g.N), dim3(BLOCK), 0, s, g, dt, b.row, b.Wbar, b.gbar, b.whbar, b.c0bar, b.cbar, b.xbar);
@@ -1324,9 +1327,9 @@ hipError_t launch_moving_cell_adjoint(hipStream_t s, const Geo& g, const Rows& r
 }
 
 template hipError_t launch_moving_pass_adjoint<2>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, double, bool, const double*,
-                                                  const MovingBars&);
+                                                  const MovingBars&, bool);
 template hipError_t launch_moving_pass_adjoint<3>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, double, bool, const double*,
-                                                  const MovingBars&);
+                                                  const MovingBars&, bool);
 template hipError_t launch_moving_cell_adjoint<2>(hipStream_t, const Geo&, const Rows&, const Gas&, bool, bool, const double*, const MovingBars&);
 template hipError_t launch_moving_cell_adjoint<3>(hipStream_t, const Geo&, const Rows&, const Gas&, bool, bool, const double*, const MovingBars&);
 template hipError_t launch_step_tangent<2>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&);

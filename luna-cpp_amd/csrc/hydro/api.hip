@@ -737,8 +737,7 @@ int luna_fv_step_adjoint(luna_fv* fv, double dt, double* qbar, int device) {
 int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, const luna_geometry* mesh1, luna_geometry_adjoint_fn adjoint0,
                            const void* ctx0, luna_geometry_adjoint_fn adjoint1, const void* ctx1, int steps_fixed, double* qbar, double* xbar) {
     if (!fv || !mesh0 || !mesh1 || !adjoint0 || !adjoint1 || !qbar || !xbar) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_adjoint: null argument");
-    if (fv->opt.motion != LUNA_MOTION_MOVING || fv->opt.order != 1)
-        return fail(LUNA_ERR_LOGIC, "hydro::FiniteVolume::step_moving_adjoint: a moving mesh, order 1");
+    if (fv->opt.motion != LUNA_MOTION_MOVING) return fail(LUNA_ERR_LOGIC, "hydro::FiniteVolume::step_moving_adjoint: a moving mesh");
     if (fv->opt.limiter == LUNA_LIMITER_LEARNED)
         return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: derivatives of the face-centroid limiter or none (not the learned one)");
     if (fv->opt.precision != LUNA_PRECISION_DOUBLE) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: derivatives of the double policy");
@@ -755,7 +754,7 @@ int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, c
     if (int rc = luna_fv_set_geometry(fv, mesh0)) return rc;  // and x^n
     if (int rc = primitives(fv)) return rc;
     if (int rc = sweep(fv, 1.0)) return rc;
-    {
+    if (fv->opt.order == 1) {
         const luna::Geo g = fv->geo();
         const luna::Rows r = fv->rows();
         LUNA_HIP(D == 3 ? luna::launch_keep_centroids<3>(s, g, r, fv->grid, dt) : luna::launch_keep_centroids<2>(s, g, r, fv->grid, dt));
@@ -789,6 +788,37 @@ int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, c
                                 fv->m_extreme.p, fv->m_fmax.p};
     };
     const luna::Gas gas = fv->gas();
+    if (fv->opt.order == 0) {
+        // step_moving_adjoint at order 0 (finite_volume.hpp:2490-2598): the one pass on t^n (sides W, the point velocities),
+        // x^n+1 = x^n + dt w (w's bar takes dt of x^n+1's, x^n's is x^n+1's), then the point velocity's and the sound
+        // speed's adjoints and W = primitive(Q / V) per cell (zero gradient bars: the fit adds nothing).
+        LUNA_HIP(fv->m_entry.reserve(static_cast<size_t>(fv->slots > 0 ? fv->slots : 1) * (2 * D + 1)));
+        const luna::Geo g = fv->geo();
+        const luna::Rows r = fv->rows();
+        luna::MovingBars b = bars();
+        b.entry = fv->m_entry.p;
+        // w's bar from x^n+1's first (m_xbar holds the caller's bar), then the pass adds the separations' bars to x^n's
+        LUNA_HIP(luna::launch_drift_adjoint(s, static_cast<long long>(ND), dt, fv->m_xbar.p, fv->m_whbar.p));
+        LUNA_HIP(D == 3 ? luna::launch_moving_pass_adjoint<3>(s, g, r, gas, fv->opt.riemann, dt, unit, false, fv->m_qbar.p, b, true)
+                        : luna::launch_moving_pass_adjoint<2>(s, g, r, gas, fv->opt.riemann, dt, unit, false, fv->m_qbar.p, b, true));
+        LUNA_HIP(D == 3 ? luna::launch_moving_cell_adjoint<3>(s, g, r, gas, false, false, fv->m_qbar.p, b)
+                        : luna::launch_moving_cell_adjoint<2>(s, g, r, gas, false, false, fv->m_qbar.p, b));
+        const size_t F0 = static_cast<size_t>(mesh0->faces);
+        std::vector<double> hV(N, 0.0), hc(ND), hA(Fmax * D), hxf(Fmax * D), hx(ND);
+        LUNA_HIP(hipMemcpyAsync(hV.data(), fv->m_Vbar.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+        LUNA_HIP(hipMemcpyAsync(hc.data(), fv->m_cbar.p, sizeof(double) * ND, hipMemcpyDeviceToHost, s));
+        LUNA_HIP(hipMemcpyAsync(hA.data(), fv->m_Ab.p, sizeof(double) * F0 * D, hipMemcpyDeviceToHost, s));
+        LUNA_HIP(hipMemcpyAsync(hxf.data(), fv->m_Xfb.p, sizeof(double) * F0 * D, hipMemcpyDeviceToHost, s));
+        LUNA_HIP(hipMemcpyAsync(hx.data(), fv->m_xbar.p, sizeof(double) * ND, hipMemcpyDeviceToHost, s));
+        LUNA_HIP(hipMemcpyAsync(qbar, fv->m_out.p, sizeof(double) * NV, hipMemcpyDeviceToHost, s));
+        LUNA_HIP(hipStreamSynchronize(s));
+        if (adjoint0(ctx0, hV.data(), hc.data(), hA.data(), hxf.data(), hx.data()) != 0)
+            return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_adjoint: the geometry adjoint of the mesh of t^n failed");
+        std::memcpy(xbar, hx.data(), sizeof(double) * ND);
+        fv->prims_ready = false;
+        fv->swept = false;
+        return LUNA_OK;
+    }
     // Pass 2 on t^n+1 (the rows W^n, g^n, c^n and the steps stay), then x^n+1's bar with its geometry's adjoint.
     fv->begun = true;  // keeps W, g across the upload
     int rc = upload(fv, mesh1);

@@ -146,10 +146,11 @@ struct MovingBars {
     int *extreme, *fmax_of;
 };
 // One pass on the mesh the Geo holds (second: t^n+1): the faces' rows and A, x_f bars (written), then the cells' gathers
-// (added to W, g, w-hat, c^n, c and x bars).
+// (added to W, g, w-hat, c^n, c and x bars). order0: the order-0 moving step's single pass (sides W, the sweep's w for
+// w-hat, Q moved by the whole dt; no gradient bars).
 template <int D>
 hipError_t launch_moving_pass_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, double unit, bool second,
-                                      const double* qbar, const MovingBars& b);
+                                      const double* qbar, const MovingBars& b, bool order0 = false);
 // w-hat bar += dt x bar (n entries).
 hipError_t launch_drift_adjoint(hipStream_t s, long long n, double dt, const double* xbar, double* whbar);
 // On t^n: the point velocity's and the fit's adjoints per cell, the neighbours' gather with the primitive adjoint (out, V

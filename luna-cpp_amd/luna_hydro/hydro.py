@@ -210,7 +210,8 @@ class FiniteVolume:
         return self._derivative(lib().luna_fv_step_adjoint, dt, qbar, "step_adjoint")
 
     def moving_adjoint(self, dt, mesh0, mesh1, qbar, xbar, steps_fixed=False):
-        """FiniteVolume::step_moving_adjoint() at order 1 (luna_fv_moving_adjoint): the handle's Q is the state at t^n, mesh0
+        """FiniteVolume::step_moving_adjoint() (luna_fv_moving_adjoint; order 1, or order 0, where mesh1 is not read): the
+        handle's Q is the state at t^n, mesh0
         the mesh of t^n, mesh1 the mesh the forward step reached; qbar (N x V) and xbar (N x D), the bars after the step,
         -> those before it, with this build's mesh library's geometry derivatives of both meshes. Leaves the handle on
         mesh0."""

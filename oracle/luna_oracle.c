@@ -3120,9 +3120,70 @@ static void moving_fit_adjoint(const orc_fv* fv, const orc_geometry* g, int i, c
     }
 }
 
+/* finite_volume.hpp:2490-2598 step_moving_adjoint (order 0): Q^n+1 = Q^n - dt sum F(W_o, W_j, A, w_f) on the mesh of t^n,
+ * x^n+1 = x^n + dt w (wrapped: the identity's derivative), w the regularised point velocity. The pass is the first pass's
+ * adjoint with zero gradients (sides W), w in place of w-hat and dt for dt/2; then w's bar takes dt of x^n+1's, the point
+ * velocity's and the sound speed's adjoints follow, and W = primitive(Q / V) gives Q's and V's bars. */
+static int moving_adjoint_order0(orc_fv* fv, const orc_geometry* g0, double dt, double* qbar, double* xbar, orc_geometry_adjoint adjoint0,
+                                 void* ctx0) {
+    const int D = fv->D, V = D + 2, N = fv->N, PW = V + 2;
+    orc_primitives(fv, g0);
+    for (int i = 0; i < N; ++i) sweep_cell(fv, g0, i);
+    memset(fv->grad, 0, sizeof(double) * (size_t)N * V * D);
+    double* Wbar = (double*)calloc((size_t)N * V, sizeof(double));
+    double* gbar = (double*)calloc((size_t)N * V * D, sizeof(double));
+    double* wbar = (double*)calloc((size_t)N * D, sizeof(double));
+    double* c0bar = (double*)calloc((size_t)N * D, sizeof(double));
+    const int F0 = g0->F;
+    geo_bars b0 = {(double*)calloc(N, sizeof(double)), (double*)calloc((size_t)N * D, sizeof(double)), (double*)calloc((size_t)N * D, sizeof(double)),
+                   (double*)calloc((size_t)(F0 > 0 ? F0 : 1) * D, sizeof(double)), (double*)calloc((size_t)(F0 > 0 ? F0 : 1) * D, sizeof(double))};
+    moving_pass_adjoint(fv, g0, 0, 2 * dt, fv->w, qbar, Wbar, gbar, wbar, c0bar, &b0);
+    for (size_t k = 0; k < (size_t)N * D; ++k) {
+        b0.x[k] += xbar[k];
+        wbar[k] += dt * xbar[k];
+    }
+    for (int i = 0; i < N; ++i) {
+        const double* Wi = fv->prims + (size_t)i * PW;
+        double* Wb = Wbar + (size_t)i * V;
+        double csbar = 0;
+        regularised_adjoint(fv, g0, i, wbar + (size_t)i * D, Wb, &csbar, &b0);
+        if (Wi[V] > 0) {
+            const double c2bar = csbar / (2 * Wi[V]);
+            Wb[D + 1] += c2bar * fv->gas.gamma / Wi[0];
+            Wb[0] -= c2bar * fv->gas.gamma * Wi[D + 1] / (Wi[0] * Wi[0]);
+        }
+    }
+    for (int j = 0; j < N; ++j) {
+        const double inv = 1.0 / g0->volume[j];
+        double Uj[MAXV], Ub[MAXV];
+        for (int v = 0; v < V; ++v) Uj[v] = fv->q[(size_t)j * V + v] * inv;
+        orc_primitive_adjoint(D, &fv->gas, Uj, Wbar + (size_t)j * V, Ub);
+        double vb = 0;
+        for (int v = 0; v < V; ++v) {
+            qbar[(size_t)j * V + v] += Ub[v] * inv;
+            vb -= Ub[v] * Uj[v] * inv;
+        }
+        b0.V[j] += vb;
+    }
+    adjoint0(ctx0, b0.V, b0.c, b0.A, b0.xf, b0.x);
+    memcpy(xbar, b0.x, sizeof(double) * (size_t)N * D);
+    free(Wbar);
+    free(gbar);
+    free(wbar);
+    free(c0bar);
+    free(b0.V);
+    free(b0.c);
+    free(b0.x);
+    free(b0.A);
+    free(b0.xf);
+    return 0;
+}
+
 int orc_moving_adjoint(orc_fv* fv, const orc_geometry* g0, const orc_geometry* g1, double dt, int steps_fixed, double* qbar, double* xbar,
                        orc_geometry_adjoint adjoint0, void* ctx0, orc_geometry_adjoint adjoint1, void* ctx1) {
-    if (!fv->moving || fv->order != 1 || fv->D < 2 || fv->precision != 0 || fv->hll_compression > 0) return -1;
+    if (!fv->moving || fv->D < 2 || fv->precision != 0 || fv->hll_compression > 0) return -1;
+    if (fv->order == 0) return moving_adjoint_order0(fv, g0, dt, qbar, xbar, adjoint0, ctx0);
+    if (fv->order != 1) return -1;
     const int D = fv->D, V = D + 2, U = D + 1, N = fv->N, PW = V + 2;
     /* t^n: W, the limited gradients and the point velocities (the sweep), c^n, the grid steps and w-hat. */
     orc_primitives(fv, g0);

@@ -183,3 +183,29 @@ def test_moving_rollout_central_differences(steps_n):
     print("moving rollout (HLL, %d steps, steps held): |FD - adjoint| / |FD| %.2e" % (steps_n, worst), flush=True)
     assert worst < 1e-6
     fv.close()
+
+
+@pytest.mark.parametrize("D,n,riemann", [(2, 12, "hllc"), (3, 6, "hllc"), (3, 6, "hll")])
+def test_order0_moving_adjoint_matches_oracle(D, n, riemann):
+    """The order-0 moving step's reverse (finite_volume.hpp:2490-2598) against the oracle's, which central differences pin."""
+    print("order-0 moving adjoint D=%d n=%d %s" % (D, n, riemann), flush=True)
+    rng = np.random.default_rng(5)
+    m0 = mesh.build(problems.jittered_lattice(n, D, 0.25, 5), 1.0)
+    N, V = m0["cells"], D + 2
+    q = np.abs(rng.normal(size=(N, V))) * 0.1
+    q[:, 0] = (1 + 0.2 * rng.uniform(-1, 1, N)) * m0["volume"]
+    q[:, V - 1] = (2 + 0.2 * rng.uniform(-1, 1, N)) * m0["volume"]
+    fv = hydro.FiniteVolume(m0, GAMMA, order=0, riemann=riemann, motion="moving")
+    fv.set_state(q)
+    dt = fv.time_step(0.3)
+    qbar, xbar = rng.normal(size=(N, V)), rng.normal(size=(N, D))
+    qg, xg = fv.moving_adjoint(dt, m0, m0, qbar, xbar)
+    fvo = oracle.FiniteVolume(m0, GAMMA, order=0, moving=True, riemann=riemann)
+    fvo.set_state(q)
+    g0 = mesh.GeometryDerivative(m0)
+    qo, xo = fvo.moving_adjoint(dt, qbar, xbar, m0, lambda *a: g0.adjoint(*a), lambda *a: None)
+    for got, want, name in ((qg, qo, "qbar"), (xg, xo, "xbar")):
+        err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+        print("  %s: max |GPU - oracle| / max |oracle| = %.2e" % (name, err), flush=True)
+        assert err < 1e-10, (name, err)
+    fv.close()
