@@ -213,7 +213,7 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; the step's sharding is described in DESIGN.md (no multi-GPU path)",
+        print("bench.py: one GPU only; the step has no multi-GPU path (--gpus 1)",
               file=sys.stderr)
         sys.exit(2)
     n = args.n

@@ -26,7 +26,7 @@ struct Gas {
     double gamma = 5.0 / 3.0, smallr = 1e-10, smallc = 1e-10;
 };
 
-// learned: this build's learned limiter (no counterpart in the reference; 2D/3D, order 1; luna_hydro.h).
+// learned: this library's learned limiter (no counterpart in the reference; 2D/3D, order 1; luna_hydro.h).
 enum class Limiter { face_centroid = LUNA_LIMITER_FACE_CENTROID, none = LUNA_LIMITER_NONE, minmod = LUNA_LIMITER_MINMOD,
                      van_leer = LUNA_LIMITER_VAN_LEER, learned = LUNA_LIMITER_LEARNED };
 enum class Motion { static_mesh = LUNA_MOTION_STATIC, moving = LUNA_MOTION_MOVING };

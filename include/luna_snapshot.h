@@ -23,7 +23,7 @@ typedef struct {
     const double* positions;     /* N x 3 generators */
     const double* volume;        /* N */
     const double* q;             /* N x 5 extensive conserved state (FiniteVolume::state()); U = Q / V */
-    const char* commit;          /* provenance: this build's commit (or "") */
+    const char* commit;          /* provenance: the library's commit (or "") */
     int dirty;
 } luna_snapshot;
 

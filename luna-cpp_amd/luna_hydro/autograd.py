@@ -15,7 +15,7 @@ Any object with set_state(tensor), step(dt) -> None, state() -> tensor and adjoi
 state last set) can be the solver; Solver adapts a hydro.FiniteVolume.
 
 On a moving mesh the state is (Q, x) (moving_rollout, MovingSolver): each step rebuilds the mesh of the new positions with
-this build's producer, and the backward pass runs FiniteVolume::step_moving_adjoint (luna_fv_moving_adjoint) with both
+the project's Voronoi producer, and the backward pass runs FiniteVolume::step_moving_adjoint (luna_fv_moving_adjoint) with both
 meshes' geometry derivatives; checkpoints keep (Q, x):
 
     solver = autograd.MovingSolver(hydro.FiniteVolume(mesh, 5 / 3, motion="moving"))

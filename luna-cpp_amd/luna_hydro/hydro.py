@@ -213,7 +213,7 @@ class FiniteVolume:
         """FiniteVolume::step_moving_adjoint() (luna_fv_moving_adjoint; order 1, or order 0, where mesh1 is not read): the
         handle's Q is the state at t^n, mesh0
         the mesh of t^n, mesh1 the mesh the forward step reached; qbar (N x V) and xbar (N x D), the bars after the step,
-        -> those before it, with this build's mesh library's geometry derivatives of both meshes. Leaves the handle on
+        -> those before it, with the project's mesh library's geometry derivatives of both meshes. Leaves the handle on
         mesh0."""
         from . import mesh as mesh_lib
 

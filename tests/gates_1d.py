@@ -1,5 +1,5 @@
 """The 1D ladder gates of luna-cpp (src/test/luna_1d_gates.hpp, kept as tests/golden/luna_1d_gates.json) as a check of
-this build's static line: each problem on N uniform cells to its t_end under LUNA's rule at CFL 0.8 (the reference's
+the static line of libluna_hydro: each problem on N uniform cells to its t_end under LUNA's rule at CFL 0.8 (the reference's
 hydro1d_cases_test.cpp:46-110 simulate()), then the L1 errors of rho, v, P with the rules of LUNA's tests
 (:164-221: exact cell averages of the Riemann solution, Noh's exact profile away from the wall, the linear waves from
 their +eps and -eps runs, the simple wave from its characteristics, three problems against stored fine-run densities).

@@ -1,6 +1,6 @@
 """The oracle's reverse of the moving step (order 1, fixed topology; finite_volume.hpp:2233-3155 of the reference), with the
 mesh library's geometry derivative for both meshes. What pins it (the reference's own checks, hydro_derivatives_test.cpp:
-1470-1706, with this build's meshes):
+1470-1706, with the project's own meshes):
   - central differences of the oracle's forward with the whole grid steps held, along directions of Q alone, of the
     positions alone (whole grid steps, so no snapping error; the topology unchanged) and of both: 2D and 3D, HLLC and HLL;
   - with the steps free (the quantisation taken as the identity, x^n+1 = x^n + dt w), central differences of the
