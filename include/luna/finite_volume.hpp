@@ -96,6 +96,15 @@ public:
         check(luna_fv_time_step(fv_, courant, &dt), "time_step");
         return dt;
     }
+    // set_step_rule (:181), release_mesh (:134-139: the handle holds its own copy; refused on a moving mesh), and of
+    // mesh_stats (:160-163) the mesh updates taken.
+    void set_step_rule(StepRule rule) { check(luna_fv_set_step_rule(fv_, static_cast<int>(rule)), "set_step_rule"); }
+    void release_mesh() { check(luna_fv_release_mesh(fv_), "release_mesh"); }
+    long long mesh_updates() const {
+        long long n = 0;
+        check(luna_fv_mesh_stats(fv_, &n), "mesh_stats");
+        return n;
+    }
     // Derivatives of one static order-1 step at fixed dt (:1941-2231): step_tangent advances Q and dq (dq becomes J dq);
     // step_adjoint maps qbar after the step to J^T qbar before it (Q stays); time_step_tangent is time_step() for them.
     void step_tangent(double dt, std::vector<double>& dq) { check(luna_fv_step_tangent(fv_, dt, dq.data(), 0), "step_tangent"); }

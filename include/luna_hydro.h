@@ -131,6 +131,17 @@ int luna_fv_primitives(luna_fv* fv, double* rows);
 /* faces() (finite_volume.hpp:141). */
 int luna_fv_faces(luna_fv* fv, int* faces);
 
+/* set_step_rule(StepRule) (finite_volume.hpp:181): the rule of the next time steps; LUNA_STEP_LUNA in 2D only
+ * (LUNA_ERR_ARGUMENT otherwise). */
+int luna_fv_set_step_rule(luna_fv* fv, int rule);
+/* release_mesh() (finite_volume.hpp:134-139): the reference stops reading the caller's mesh on a static mesh and refuses on a
+ * moving one (LUNA_ERR_LOGIC). The handle keeps its own copy of every mesh it takes, so on a static mesh there is
+ * nothing to release and the call only checks the motion. */
+int luna_fv_release_mesh(luna_fv* fv);
+/* mesh_stats() (finite_volume.hpp:160-163): of the reference's counters, the meshes this handle took after its first one
+ * (set_geometry and moving_finish with a mesh); the rest are the mesh library's own (voronoi-cpp's update report). */
+int luna_fv_mesh_stats(luna_fv* fv, long long* updates);
+
 /* Kernel timing with HIP events on the handle's stream (on: 1, off: 0; turning it on clears the sums). read: up to max
  * entries of (name, launches, total milliseconds), for the launches since it was turned on. */
 typedef struct {

@@ -167,6 +167,7 @@ struct luna_fv {
     double side[3] = {0, 0, 0};
     luna::Grid grid{};
     int parity = 0;
+    long long uploads = 0;  // meshes taken, the first one included
     bool prims_ready = false, swept = false, begun = false, new_mesh = false;
     Profiler prof;
     // pinned staging of the packed mesh records (reused across uploads)
@@ -397,6 +398,7 @@ int upload(luna_fv* fv, const luna_geometry* m) {
     fv->F = F;
     fv->E = E;
     fv->slots = static_cast<int>(slots);
+    ++fv->uploads;
     for (int a = 0; a < 3; ++a) fv->side[a] = a < D ? m->side[a] : 0.0;
     fv->grid.step = m->step;
     fv->grid.per_step = m->step > 0 ? 1.0 / m->step : 0.0;
@@ -749,6 +751,11 @@ int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, c
     const size_t ND = static_cast<size_t>(N) * D, NV = static_cast<size_t>(N) * V;
     hipStream_t s = fv->stream;
     const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID;
+    struct KeepCount {  // the reverse's own uploads are not mesh updates
+        long long& count;
+        long long value;
+        ~KeepCount() { count = value; }
+    } keep{fv->uploads, fv->uploads};
     const int row_width = D == 3 ? 2 * (5 + 15 + 3) + 9 : 2 * (4 + 8 + 2) + 6;
     // t^n: its mesh and positions, W, the limited gradients and point velocities (the sweep keeps c^n), the grid steps.
     if (int rc = luna_fv_set_geometry(fv, mesh0)) return rc;  // and x^n
@@ -1036,6 +1043,27 @@ int luna_fv_primitives(luna_fv* fv, double* rows) {
 int luna_fv_faces(luna_fv* fv, int* faces) {
     if (!fv || !faces) return fail(LUNA_ERR_ARGUMENT, "luna_fv_faces: null argument");
     *faces = fv->F;
+    return LUNA_OK;
+}
+
+int luna_fv_set_step_rule(luna_fv* fv, int rule) {
+    if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv_set_step_rule: null argument");
+    if (rule != LUNA_STEP_AREPO && rule != LUNA_STEP_LUNA) return fail(LUNA_ERR_ARGUMENT, "set_step_rule: AREPO or LUNA");
+    if (rule == LUNA_STEP_LUNA && fv->D != 2) return fail(LUNA_ERR_ARGUMENT, "StepRule::luna: 2D");
+    fv->opt.step_rule = rule;
+    fv->swept = false;  // the next step sweeps again with the rule's dt terms
+    return LUNA_OK;
+}
+
+int luna_fv_release_mesh(luna_fv* fv) {
+    if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv_release_mesh: null argument");
+    if (fv->opt.motion == LUNA_MOTION_MOVING) return fail(LUNA_ERR_LOGIC, "hydro::FiniteVolume::release_mesh: a moving mesh updates the mesh");
+    return LUNA_OK;
+}
+
+int luna_fv_mesh_stats(luna_fv* fv, long long* updates) {
+    if (!fv || !updates) return fail(LUNA_ERR_ARGUMENT, "luna_fv_mesh_stats: null argument");
+    *updates = fv->uploads > 0 ? fv->uploads - 1 : 0;
     return LUNA_OK;
 }
 

@@ -121,7 +121,7 @@ def exported_symbols():
              "luna_fv1_mesh_stats",
              "luna_last_error",
              "luna_fv_set_geometry", "luna_synchronize", "luna_fv_step_tangent", "luna_fv_step_adjoint", "luna_fv_moving_adjoint",
-             "luna_fv_time_step_tangent",
+             "luna_fv_time_step_tangent", "luna_fv_set_step_rule", "luna_fv_release_mesh", "luna_fv_mesh_stats",
              "luna_fv_set_limiter_network", "luna_fv_limiter_features", "luna_mlp_create", "luna_mlp_destroy", "luna_mlp_set_params",
              "luna_mlp_get_params", "luna_mlp_forward", "luna_mlp_gradient", "luna_mlp_train_step", "luna_mlp_synchronize"]
     return {n: hasattr(lib(), n) for n in names}
@@ -190,6 +190,20 @@ class FiniteVolume:
 
     def step(self, dt):
         _check(lib().luna_fv_step(self._h, ctypes.c_double(dt)), "step")
+
+    def set_step_rule(self, rule):
+        """set_step_rule(StepRule) (finite_volume.hpp:181): "arepo", or "luna" in 2D."""
+        _check(lib().luna_fv_set_step_rule(self._h, RULES[rule]), "set_step_rule")
+
+    def release_mesh(self):
+        """release_mesh() (finite_volume.hpp:134-139): nothing to release (the handle copies each mesh); refused on a moving mesh."""
+        _check(lib().luna_fv_release_mesh(self._h), "release_mesh")
+
+    def mesh_updates(self):
+        """mesh_stats().updates: the meshes taken after the first."""
+        n = ctypes.c_longlong()
+        _check(lib().luna_fv_mesh_stats(self._h, ctypes.byref(n)), "mesh_stats")
+        return n.value
 
     def time_step_tangent(self, courant, dq):
         """FiniteVolume::time_step_tangent(courant, dq): dt as time_step(); the next step_tangent(dt, dq) takes the sweep's

@@ -213,3 +213,34 @@ def test_sedov_128_first_step_matches_oracle():
     after = [math.fsum(qg[:, v]) for v in range(5)]
     assert abs(after[0] - before[0]) / before[0] < 1e-14
     assert abs(after[4] - before[4]) / before[4] < 1e-14
+
+
+def test_set_step_rule_release_mesh_and_mesh_stats():
+    """set_step_rule (finite_volume.hpp:181) gives the dt of a handle made with that rule; release_mesh refuses a moving mesh
+    (:134-139); mesh_stats counts the meshes a moving run takes after its first."""
+    m = cases.voronoi(24, 2)
+    q = cases.rough_state(m)
+    a = hydro.FiniteVolume(m, 5.0 / 3.0)
+    b = hydro.FiniteVolume(m, 5.0 / 3.0, step_rule="luna")
+    c = hydro.FiniteVolume(m, 5.0 / 3.0, motion="moving")
+    try:
+        for fv in (a, b, c):
+            fv.set_state(q)
+        a.set_step_rule("luna")
+        assert a.time_step(0.8) == b.time_step(0.8)
+        with pytest.raises(ValueError):
+            hydro.FiniteVolume(cases.voronoi(6, 3), 5.0 / 3.0).set_step_rule("luna")
+        a.release_mesh()
+        with pytest.raises(RuntimeError):
+            c.release_mesh()
+        assert c.mesh_updates() == 0
+        mm = m
+        for _ in range(2):
+            dt = c.time_step(0.3)
+            mm = mesh.build(c.moving_begin(dt), 1.0)
+            c.moving_finish(mm, dt)
+        assert c.mesh_updates() == 2
+    finally:
+        a.close()
+        b.close()
+        c.close()
