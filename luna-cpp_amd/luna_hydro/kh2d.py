@@ -6,7 +6,7 @@ so runs compare line by line.
 
     python -m luna_hydro.kh2d 128 [--moving] [--ic smooth|sharp] [--dt luna|arepo] [--courant C] [--t-end T]
                                [--limiter face|none|learned] [--jitter J] [--seed S] [--steps K] [--out PREFIX]
-                               [--precision default|double|grad32|grad16]
+                               [--precision default|double|grad32|grad16] [--order 0|1] [--riemann hllc|hll]
 
 Defaults: luna-cpp's smooth McNally layer and LUNA's rule at 0.8 (AREPO's at 0.3), t_end 1.5, --precision default
 (DefaultPrecision<2>, grad32, as hydro_kh2d.cpp:330-337; double with the learned limiter). Mesh: this project's
@@ -46,7 +46,7 @@ def mode_amplitude(q, positions, centroid, volume):
 
 
 def run(n=128, moving=False, ic="smooth", t_end=1.5, rule="luna", courant=None, limiter="face", params=None, jitter=0.25,
-        seed=1, steps=0, count_fallbacks=False, out="", log=print, precision="double"):
+        seed=1, steps=0, count_fallbacks=False, out="", log=print, precision="double", order=1, riemann="hllc"):
     """One run; returns its report (series of (t, M, K), hash, totals, steps, wall) after printing the lines."""
     courant = courant if courant is not None else (0.3 if rule == "arepo" else 0.8)
     t0 = time.perf_counter()
@@ -55,8 +55,8 @@ def run(n=128, moving=False, ic="smooth", t_end=1.5, rule="luna", courant=None, 
     state = problems.kelvin_helmholtz_smooth_state if ic == "smooth" else problems.kelvin_helmholtz_state
     q = state(m, GAMMA)
     N = m["cells"]
-    fv = hydro.FiniteVolume(m, GAMMA, limiter=limiter, motion="moving" if moving else "static", step_rule=rule,
-                            precision=hydro.policy(precision, 2))
+    fv = hydro.FiniteVolume(m, GAMMA, order=order, limiter=limiter, riemann=riemann, motion="moving" if moving else "static",
+                            step_rule=rule, precision=hydro.policy(precision, 2) if order == 1 else "double")
     try:
         if limiter == "learned":
             from . import learned
@@ -99,9 +99,10 @@ def run(n=128, moving=False, ic="smooth", t_end=1.5, rule="luna", courant=None, 
     if count_fallbacks:
         log("fallbacks: total=%d steps=%d" % (fallbacks, count))
     log("run: wall_s=%.4f steps=%d M_cell_updates_per_s=%.1f hash=%016x" % (wall, count, rate, h))
-    log("ours: N=%d mesh=%s order=1 solver=hllc limiter=%s dt=%s courant=%g jitter=%g ic=%s precision=%s steps=%d wall_s=%.4f "
-        "build_ms=%.2f M_cell_updates_per_s=%.1f M_final=%.8e" % (n, "moving" if moving else "static", limiter, rule, courant, jitter,
-                                                                   ic, precision, count, wall, build_ms, rate, series[-1][1]))
+    log("ours: N=%d mesh=%s order=%d solver=%s limiter=%s dt=%s courant=%g jitter=%g ic=%s precision=%s steps=%d wall_s=%.4f "
+        "build_ms=%.2f M_cell_updates_per_s=%.1f M_final=%.8e" % (n, "moving" if moving else "static", order, riemann, limiter, rule,
+                                                                   courant, jitter, ic, precision, count, wall, build_ms, rate,
+                                                                   series[-1][1]))
     if out:
         with open(out + ".txt", "w") as f:
             for row in series:
@@ -125,11 +126,13 @@ def main(argv=None):
     ap.add_argument("--out", default="")
     ap.add_argument("--precision", choices=["default", "double", "grad32", "grad16"], default="default",
                     help="default: the reference's DefaultPrecision<2> (grad32); the learned limiter needs double")
+    ap.add_argument("--order", type=int, choices=[0, 1], default=1)
+    ap.add_argument("--riemann", choices=["hllc", "hll"], default="hllc")
     a = ap.parse_args(argv)
     if a.limiter == "learned" and a.precision == "default":
         a.precision = "double"
     run(a.n, a.moving, a.ic, a.t_end, a.dt, a.courant, a.limiter, None, a.jitter, a.seed, a.steps, a.count_fallbacks, a.out,
-        precision=a.precision)
+        precision=a.precision, order=a.order, riemann=a.riemann)
     return 0
 
 
