@@ -4,6 +4,7 @@ Bar: every conserved field after every step within 1e-12 relative of the oracle 
 contraction on both sides, so most cases agree to the bit; the test reports the bits), dt and fallback counts equal.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -244,3 +245,38 @@ def test_set_step_rule_release_mesh_and_mesh_stats():
         a.close()
         b.close()
         c.close()
+
+
+@pytest.mark.skipif(os.environ.get("LUNA_LARGE") != "1", reason="256^3 (16.8 M cells): about two minutes with the oracle; LUNA_LARGE=1")
+@pytest.mark.parametrize("moving", [False, True])
+def test_sedov_256_first_steps_match_oracle(moving):
+    """The 256^3 Sedov configuration (static and moving, the HLL fallback on for the moving one as that configuration
+    asks): the first step against the oracle, to the bit, and conservation."""
+    say("building 256^3 mesh")
+    m = cases.voronoi(256, 3, 0.25, 1)
+    q, deposited = problems.sedov_state(m)
+    say("sedov 256: %d cells, %d faces, blast in %d cells" % (m["cells"], m["faces"], deposited))
+    kappa = 1.0 if moving else 0.0
+    gpu = hydro.FiniteVolume(m, 5.0 / 3.0, motion="moving" if moving else "static", hll_compression=kappa)
+    orc = oracle.FiniteVolume(m, 5.0 / 3.0, moving=moving, hll_compression=kappa)
+    gpu.set_state(q)
+    orc.set_state(q)
+    try:
+        dt = orc.time_step(0.3)
+        assert gpu.time_step(0.3) == dt
+        if moving:
+            xg = gpu.moving_begin(dt)
+            assert np.array_equal(xg, orc.moving_begin(dt))
+            m1 = mesh.build(xg, 1.0)
+            gpu.moving_finish(m1, dt)
+            orc.moving_finish(m1, dt)
+        else:
+            gpu.step(dt)
+            orc.step(dt)
+        qg = gpu.state()
+        assert np.array_equal(qg, orc.q)
+        for v in (0, 4):
+            assert abs(math.fsum(qg[:, v]) - math.fsum(q[:, v])) <= 1e-14 * math.fsum(q[:, v])
+        say("  sedov 256 %s: dt=%.6g bits equal" % ("moving" if moving else "static", dt))
+    finally:
+        gpu.close()
