@@ -38,20 +38,23 @@ def test_sedov_driver_checksum_matches_oracle(tmp_path, precision):
     assert np.array_equal(snap["U"], (q / m["volume"][:, None]).T)
 
 
-@pytest.mark.parametrize("ic,precision", [("smooth", "double"), ("sharp", "double"), ("smooth", "default")])
-def test_kh_driver_hash_and_modes_match_oracle(ic, precision):
+@pytest.mark.parametrize("ic,precision,order,riemann", [("smooth", "double", 1, "hllc"), ("sharp", "double", 1, "hllc"),
+                                                        ("smooth", "default", 1, "hllc"), ("smooth", "double", 1, "hll"),
+                                                        ("smooth", "double", 0, "hllc")])
+def test_kh_driver_hash_and_modes_match_oracle(ic, precision, order, riemann):
     """The KH driver (hydro_kh2d's loop: outputs every 0.02, dt cut to land on them) on 24^2 to t = 0.06: its run line's
     hash is the FNV-1a of the oracle's final state after the same steps, and its mode series is the oracle's. --precision
     default is the reference's 2D default, grad32."""
     from luna_hydro import hydro, kh2d
 
     lines = []
-    res = kh2d.run(n=24, ic=ic, t_end=0.06, log=lines.append, precision=precision)
+    res = kh2d.run(n=24, ic=ic, t_end=0.06, log=lines.append, precision=precision, order=order, riemann=riemann)
     print("\n".join(lines), flush=True)
     assert [l.split(":")[0] for l in lines] == ["totals", "run", "ours"]
     m = mesh.build(problems.jittered_lattice(24, 2, 0.25, 1), 1.0)
     state = problems.kelvin_helmholtz_smooth_state if ic == "smooth" else problems.kelvin_helmholtz_state
-    orc = oracle.FiniteVolume(m, 5.0 / 3.0, step_rule="luna", precision=hydro.policy(precision, 2))
+    orc = oracle.FiniteVolume(m, 5.0 / 3.0, step_rule="luna", order=order, riemann=riemann,
+                              precision=hydro.policy(precision, 2) if order == 1 else "double")
     orc.set_state(state(m, 5.0 / 3.0))
     t, series = 0.0, []
     for k in range(1, 4):
