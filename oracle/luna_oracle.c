@@ -424,9 +424,10 @@ static float dotf(int D, const float* a, const float* b) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-/* finite_volume.hpp:526-537 (constants :74): the regularised point velocity of cell i. */
-static void regularised_velocity(orc_fv* fv, int i, const double* Wi, const double* centroid, double max_angle) {
-    const int D = fv->D, V = D + 2;
+/* finite_volume.hpp:526-537 (constants :74): the regularised point velocity of a cell with primitive row Wi (W, c_s),
+ * centroid relative to its generator and largest face angle max_angle. */
+void orc_regularised_point_velocity(int D, const double* Wi, const double* centroid, double max_angle, double* w) {
+    const int V = D + 2;
     const double shaping = 0.5, factor = 2.25;
     double fraction = 0;
     if (max_angle > 0.75 * factor) fraction = max_angle > factor ? shaping : shaping * (max_angle - 0.75 * factor) / (0.25 * factor);
@@ -437,7 +438,11 @@ static void regularised_velocity(orc_fv* fv, int i, const double* Wi, const doub
         const double s = fraction * Wi[V] / cd;
         for (int a = 0; a < D; ++a) wi[a] = wi[a] + centroid[a] * s;
     }
-    for (int a = 0; a < D; ++a) fv->w[(size_t)i * D + a] = wi[a];
+    for (int a = 0; a < D; ++a) w[a] = wi[a];
+}
+
+static void regularised_velocity(orc_fv* fv, int i, const double* Wi, const double* centroid, double max_angle) {
+    orc_regularised_point_velocity(fv->D, Wi, centroid, max_angle, fv->w + (size_t)i * fv->D);
 }
 
 /* finite_volume.hpp:463-569 Sweep::operator() under Grad32 / Grad16 (G = float): the fit's sums in float from the double
@@ -772,19 +777,24 @@ static void pass_side_first(const orc_fv* fv, int i, const double* r, double* Wf
     for (int v = 0; v < V; ++v) Wf[v] = W[v];
 }
 
-/* finite_volume.hpp:1380-1397 FaceFlux::face_velocity() */
-static void face_velocity(const orc_geometry* geo, int f, int o, int j, const double* wL, const double* wR, double* wf) {
-    const int D = geo->D;
-    double d[MAXD], c[MAXD], dw[MAXD];
-    separation(geo, o, j, d);
+/* finite_volume.hpp:1380-1397 FaceFlux::face_velocity(): d the separation owner -> neighbour, xf the face centroid relative
+ * to the owner's generator, wL / wR the two point velocities. */
+void orc_face_velocity(int D, const double* d, const double* xf, const double* wL, const double* wR, double* wf) {
+    double c[MAXD], dw[MAXD];
     const double dist = sqrt(dot(D, d, d));
     const double inv_dist = 1.0 / dist;
-    for (int a = 0; a < D; ++a) c[a] = geo->face_centroid[(size_t)f * D + a] - d[a] * 0.5;
+    for (int a = 0; a < D; ++a) c[a] = xf[a] - d[a] * 0.5;
     for (int a = 0; a < D; ++a) dw[a] = wL[a] - wR[a];
     double facv = dot(D, c, dw) * inv_dist;
     const double limit = 0.9 * dist, c2 = dot(D, c, c);
     if (c2 > limit * limit) facv *= limit / sqrt(c2);
     for (int a = 0; a < D; ++a) wf[a] = (wL[a] + wR[a]) * 0.5 + d[a] * (facv * inv_dist);
+}
+
+static void face_velocity(const orc_geometry* geo, int f, int o, int j, const double* wL, const double* wR, double* wf) {
+    double d[MAXD];
+    separation(geo, o, j, d);
+    orc_face_velocity(geo->D, d, geo->face_centroid + (size_t)f * geo->D, wL, wR, wf);
 }
 
 /* The moving step's point velocity from its grid steps (finite_volume.hpp:1446-1448). */
@@ -794,14 +804,15 @@ static void grid_velocity(const orc_fv* fv, int i, double unit, double* w) {
 
 /* One face's flux times |A| (finite_volume.hpp:1415-1434 FaceFlux::operator(), :1466-1490 MovingFlux::operator()).
  * pass: -1 static, 0 a moving step's first pass (mesh t^n), 1 its second (mesh t^n+1). */
-static void face_flux(const orc_fv* fv, const orc_geometry* geo, int f, double dt, int pass, double unit, double* F, int* fallbacks) {
-    const int D = fv->D, V = D + 2;
+static double face_states(const orc_fv* fv, const orc_geometry* geo, int f, double dt, int pass, double unit, double* L, double* R,
+                          double* n, double* wf, int* fallbacks) {
+    const int D = fv->D;
     const int o = geo->owner[f], j = geo->neighbour[f];
     const double* a = geo->area + (size_t)f * D;
     const double magnitude = sqrt(dot(D, a, a));
-    for (int v = 0; v < V; ++v) F[v] = 0;
-    if (magnitude == 0) return;
-    double L[MAXV], R[MAXV], n[MAXD], wf[MAXD] = {0, 0, 0}, ro[MAXD], rj[MAXD];
+    for (int k = 0; k < D; ++k) wf[k] = 0;
+    if (magnitude == 0) return 0;
+    double ro[MAXD], rj[MAXD];
     if (pass < 0) {
         const double inv = 1.0 / magnitude;
         for (int k = 0; k < D; ++k) n[k] = a[k] * inv;
@@ -855,10 +866,30 @@ static void face_flux(const orc_fv* fv, const orc_geometry* geo, int f, double d
             }
         }
     }
+    return magnitude;
+}
+
+static void face_flux(const orc_fv* fv, const orc_geometry* geo, int f, double dt, int pass, double unit, double* F, int* fallbacks) {
+    const int D = fv->D, V = D + 2;
+    const int o = geo->owner[f], j = geo->neighbour[f];
+    double L[MAXV], R[MAXV], n[MAXD], wf[MAXD];
+    for (int v = 0; v < V; ++v) F[v] = 0;
+    const double magnitude = face_states(fv, geo, f, dt, pass, unit, L, R, n, wf, fallbacks);
+    if (magnitude == 0) return;
     /* the HLL fallback: HLLC takes HLL where a side is flagged as extremely compressed */
     const int solver = fv->riemann == 0 && fv->hll_compression > 0 && (fv->hll[o] || fv->hll[j]) ? 1 : fv->riemann;
     riemann(solver, D, &fv->gas, L, R, n, wf, F);
     for (int v = 0; v < V; ++v) F[v] *= magnitude;
+}
+
+/* The two face states of face f as face_flux() hands them to the Riemann solver (for the tests that pin them): L, R [V],
+ * the unit normal n and the face velocity wf [D]; returns |A| (0: a zero-area face, nothing written but wf = 0) and the
+ * positivity fallbacks taken in *fallbacks. pass and unit as face_flux(): -1 static (dt the step's), 0 / 1 a moving
+ * step's first / second pass (dt as apply() passes it: 0 / the step's dt; unit = grid step / dt), 2 order 0 moving. */
+double orc_face_sides(const orc_fv* fv, const orc_geometry* geo, int f, double dt, int pass, double unit, double* L, double* R, double* n,
+                      double* wf, int* fallbacks) {
+    *fallbacks = 0;
+    return face_states(fv, geo, f, dt, pass, unit, L, R, n, wf, fallbacks);
 }
 
 /* finite_volume.hpp:1539-1631 apply(), per-face loop: the flux rows, then Q -= dt sum_k sign_k F in list order.

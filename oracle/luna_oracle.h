@@ -12,8 +12,18 @@
  *   orc_time_step / orc_step_static (D = 2, 3): closed forms (linear field reproduced by the unlimited fit, uniform state
  *                                   stays uniform, conservation), brute-force lstsq of the fit, limiter range invariant,
  *                                   dt closed form on a lattice
- *   orc_moving_begin / orc_moving_finish: uniform bulk flow stays uniform, conservation; the whole moving step against
- *                                   the reference's own build is "parity unpinned" (voronoi-cpp and Kokkos are absent).
+ *   orc_half_step (2D, 3D) ........ the primitive-variable Euler Jacobians, the Euler equations (complex-step derivatives
+ *                                   of the exact flux), rotation covariance
+ *   orc_face_velocity ............. the normal speed of the moving bisector plane at the face centroid (central
+ *                                   differences of the geometry), both sides of the 0.9 |d| limit
+ *   orc_regularised_point_velocity  each branch of the face-angle ramp (closed forms), and in the sweep on a mesh where
+ *                                   all three fire
+ *   the face-centroid limiter ..... maximality: limited = min(1, min over faces of the needed factor) x unlimited
+ *   orc_face_sides ................ static W - dt/2 A g + g r, moving pass 1 W + g (x_f^n - c^n), pass 2
+ *                                   W - dt A g + g (x_f^n+1 - c^n), against the Jacobians and the meshes' lab positions
+ *   orc_moving_begin / orc_moving_finish: uniform bulk flow stays uniform, conservation, Galilean invariance of a blast
+ *                                   (hydro_test.cpp:474-510, the reference's bounds); the whole moving step against the
+ *                                   reference's own build is "parity unpinned" (voronoi-cpp and Kokkos are absent).
  */
 #ifndef LUNA_ORACLE_H
 #define LUNA_ORACLE_H
@@ -88,6 +98,13 @@ double orc_time_step(orc_fv* fv, const orc_geometry* geo, double courant);
 void orc_step_static(orc_fv* fv, const orc_geometry* geo, double dt);
 void orc_moving_begin(orc_fv* fv, const orc_geometry* geo, double dt);
 void orc_moving_finish(orc_fv* fv, const orc_geometry* geo, double dt);
+/* Pieces of the step the tests pin on their own: the face velocity (finite_volume.hpp:1380-1397; d owner -> neighbour,
+ * xf relative to the owner's generator), the regularised point velocity (:526-537) and a face's two states as the flux
+ * solves them (pass -1 static, 0 / 1 a moving step's first / second pass, 2 order 0 moving; returns |A|). */
+void orc_face_velocity(int D, const double* d, const double* xf, const double* wL, const double* wR, double* wf);
+void orc_regularised_point_velocity(int D, const double* Wi, const double* centroid, double max_angle, double* w);
+double orc_face_sides(const orc_fv* fv, const orc_geometry* geo, int f, double dt, int pass, double unit, double* L, double* R, double* n,
+                      double* wf, int* fallbacks);
 /* Derivatives of the static order-1 step (HLLC or HLL) at fixed dt (finite_volume.hpp:1670-2231): the tangent advances q and dq (J dq)
  * in place; the adjoint maps qbar after the step to J^T qbar before it (q, the state before the step, stays). Return -1
  * for a moving mesh, order 0 or D = 1 (see orc_line_*). */

@@ -203,6 +203,23 @@ def hllc_adjoint(L, R, n, w, Fbar, gamma, riemann="hllc"):
     return Lbar, Rbar
 
 
+def face_velocity(d, xf, wL, wR):
+    """FaceFlux::face_velocity (finite_volume.hpp:1380-1397): d owner -> neighbour, xf relative to the owner's generator."""
+    d, xf, wL, wR = _vec(d), _vec(xf), _vec(wL), _vec(wR)
+    wf = np.empty_like(d)
+    lib().orc_face_velocity(len(d), _ptr(d), _ptr(xf), _ptr(wL), _ptr(wR), _ptr(wf))
+    return wf
+
+
+def regularised_point_velocity(W_row, centroid, max_angle):
+    """The sweep's regularised point velocity (finite_volume.hpp:526-537): W_row = (W, c_s), centroid relative to the
+    generator, max_angle the cell's largest face angle."""
+    W_row, centroid = _vec(W_row), _vec(centroid)
+    w = np.empty_like(centroid)
+    lib().orc_regularised_point_velocity(len(centroid), _ptr(W_row), _ptr(centroid), ctypes.c_double(max_angle), _ptr(w))
+    return w
+
+
 def snap(x, step, period):
     return lib().orc_snap(ctypes.c_double(x), ctypes.c_double(step), ctypes.c_longlong(period))
 
@@ -306,6 +323,18 @@ class FiniteVolume:
         if lib().orc_step_adjoint(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(dt), _ptr(qbar)) != 0:
             raise ValueError("step_adjoint: a static mesh, order 1")
         return qbar
+
+    def face_sides(self, f, dt, pass_=-1, unit=0.0):
+        """Face f's two states as the flux solves them: (L, R, n, wf, |A|, fallbacks). pass_: -1 static (dt the step's),
+        0 / 1 a moving step's first / second pass (dt 0 / the step's, unit = grid step / dt), 2 order 0 moving."""
+        D, V = self.D, self.V
+        L, R, n, wf = np.zeros(V), np.zeros(V), np.zeros(D), np.zeros(D)
+        fb = ctypes.c_int(0)
+        L_ = lib()
+        L_.orc_face_sides.restype = ctypes.c_double
+        mag = L_.orc_face_sides(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_int(f), ctypes.c_double(dt), ctypes.c_int(pass_),
+                                ctypes.c_double(unit), _ptr(L), _ptr(R), _ptr(n), _ptr(wf), ctypes.byref(fb))
+        return L, R, n, wf, mag, fb.value
 
     def moving_begin(self, dt):
         lib().orc_moving_begin(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.c_double(dt))

@@ -4,7 +4,6 @@ Bar: every conserved field after every step within 1e-12 relative of the oracle 
 contraction on both sides, so most cases agree to the bit; the test reports the bits), dt and fallback counts equal.
 """
 import math
-import os
 
 import numpy as np
 import pytest
@@ -247,7 +246,6 @@ def test_set_step_rule_release_mesh_and_mesh_stats():
         c.close()
 
 
-@pytest.mark.skipif(os.environ.get("LUNA_LARGE") != "1", reason="256^3 (16.8 M cells): about two minutes with the oracle; LUNA_LARGE=1")
 @pytest.mark.parametrize("moving", [False, True])
 def test_sedov_256_first_steps_match_oracle(moving):
     """The 256^3 Sedov configuration (static and moving, the HLL fallback on for the moving one as that configuration
