@@ -15,6 +15,8 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "luna-cpp_amd"))
+# The CPU baseline's thread placement, read by OpenMP when it starts (before any library here loads it).
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 import numpy as np  # noqa: E402
 
@@ -93,68 +95,155 @@ def run_static(m, q, steps, warmup, precision="double", hll=0.0):
 
 
 def run_moving(m, q, steps, warmup, precision="double", hll=0.0):
-    fv = hydro.FiniteVolume(m, GAMMA, motion="moving", precision=precision, hll_compression=hll)
+    """Moving steps; the hydro part is timed as the reference's: time_step, the first pass and drift, the new mesh's
+    geometry pass (luna_fv_set_geometry_device on the producer's arrays in device memory, as the reference reads
+    voronoi-cpp's device views in its sweep and primitives), the second pass. The mesh producer (host) and the copies
+    that stand in for a GPU producer's output (positions to the host, the new mesh's arrays to the device) are excluded
+    and reported apart."""
+    fv = hydro.FiniteVolume(hydro.DeviceMesh(m), GAMMA, motion="moving", precision=precision, hll_compression=hll)
     fv.set_state(q)
-    mesh_s, hydro_s, walls = [], [], []
-    current = m
+    mesh_s, hydro_s, geo_s = [], [], []
 
-    def one(timed_kernels):
-        nonlocal current
+    def one():
         a = time.perf_counter()
         dt = fv.time_step(COURANT)
         fv.moving_begin(dt, positions=False)
         b = time.perf_counter()
-        x = fv.positions()
-        nm = mesh.build(x, 1.0)
-        fv.set_geometry(nm)
+        nm = mesh.build(fv.positions(), 1.0)
+        dm = hydro.DeviceMesh(nm)
         c = time.perf_counter()
+        fv.set_geometry(dm)
+        e = time.perf_counter()
         fv.moving_finish(None, dt)
         fv.fallbacks()  # fences
         d = time.perf_counter()
-        current = nm
-        return (b - a) + (d - c), c - b
+        return (b - a) + (d - c), c - b, e - c
 
     for _ in range(warmup):
-        one(False)
+        one()
     hydro.synchronize()
     fv.profile(True)
     t0 = time.perf_counter()
     for k in range(steps):
-        h, mm = one(True)
+        h, mm, g = one()
         hydro_s.append(h)
         mesh_s.append(mm)
-        log("  moving step %d: hydro %.3f ms, mesh build + upload %.2f s" % (k, 1e3 * h, mm))
+        geo_s.append(g)
+        log("  moving step %d: hydro %.3f ms (geometry pass %.3f ms), mesh producer + copies %.2f s" % (k, 1e3 * h, 1e3 * g, mm))
     hydro.synchronize()
     total = time.perf_counter() - t0
     times = fv.kernel_times()
     fv.close()
-    return total, hydro_s, mesh_s, times
+    return total, hydro_s, mesh_s, times, geo_s
 
 
-def cpu_baseline(m, q, steps=2):
-    """The oracle (CPU restatement of luna-cpp's FP64 algorithm, C + OpenMP) on the same mesh and state: the moving step's
-    hydro part (mesh producer excluded), on this host's cores."""
+def cpu_baseline(m, q, static_steps=40, moving_steps=10, warmup=2):
+    """The oracle (CPU restatement of luna-cpp's FP64 algorithm, C + OpenMP) on the same mesh and state, on this host's
+    cores with OMP_PROC_BIND=close (set before OpenMP starts): the static step's median wall over `static_steps` after
+    `warmup`, as hydro_sedov's run line; the moving step's hydro part (mesh producer excluded) over `moving_steps`; and
+    one round of each on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    threads = os.cpu_count() or 1
-    fv = oracle.FiniteVolume(m, GAMMA, moving=True)
-    fv.set_state(q)
-    hydro_s = []
-    for _ in range(steps):
-        a = time.perf_counter()
-        dt = fv.time_step(COURANT)
-        x = fv.moving_begin(dt)
-        b = time.perf_counter()
-        nm = mesh.build(x, 1.0)
-        c = time.perf_counter()
-        fv.moving_finish(nm, dt)
-        d = time.perf_counter()
-        hydro_s.append((b - a) + (d - c))
-    return {"value": m["cells"] / float(np.median(hydro_s)), "unit": "cell updates/s", "cores": threads,
-            "kind": "port",
-            "sample": "%d moving steps of the same 128^3 Sedov mesh and state (hydro part, median), oracle/ C+OpenMP "
-                      "restatement of luna-cpp's algorithm, OMP threads = cores" % steps}
+    threads = oracle.max_threads()
+
+    def static_walls(count, warm, budget):
+        fv = oracle.FiniteVolume(m, GAMMA)
+        fv.set_state(q)
+        walls = []
+        t0 = time.perf_counter()
+        for k in range(warm + count):
+            a = time.perf_counter()
+            fv.step(fv.time_step(COURANT))
+            if k >= warm:
+                walls.append(time.perf_counter() - a)
+            if (k + 1) % 10 == 0 or k + 1 == warm + count:
+                log("  cpu static step %d: %.1f ms" % (k, 1e3 * (time.perf_counter() - a)))
+            if time.perf_counter() - t0 > budget and walls:
+                break
+        return walls
+
+    def moving_walls(count, budget):
+        fv = oracle.FiniteVolume(m, GAMMA, moving=True)
+        fv.set_state(q)
+        walls = []
+        t0 = time.perf_counter()
+        for k in range(count):
+            a = time.perf_counter()
+            dt = fv.time_step(COURANT)
+            x = fv.moving_begin(dt)
+            b = time.perf_counter()
+            nm = mesh.build(x, 1.0)
+            c = time.perf_counter()
+            fv.moving_finish(nm, dt)
+            walls.append((b - a) + (time.perf_counter() - c))
+            log("  cpu moving step %d: hydro %.1f ms" % (k, 1e3 * walls[-1]))
+            if time.perf_counter() - t0 > budget:
+                break
+        return walls
+
+    log("  cpu: %d threads, OMP_PROC_BIND=%s" % (threads, os.environ.get("OMP_PROC_BIND")))
+    s_walls = static_walls(static_steps, warmup, 30.0)
+    m_walls = moving_walls(moving_steps, 40.0)
+    oracle.set_threads(1)
+    s1 = static_walls(1, 0, 0.0)
+    m1 = moving_walls(1, 0.0)
+    oracle.set_threads(threads)
+    N = m["cells"]
+    return {"value": N / float(np.median(m_walls)), "unit": "cell updates/s", "cores": threads, "kind": "port",
+            "static": N / float(np.median(s_walls)), "static_1_thread": N / s1[0], "moving_1_thread": N / m1[0],
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "sample": "oracle/ (C + OpenMP restatement of luna-cpp's FP64 algorithm) on the same %d^3 Sedov mesh and state: "
+                      "value = moving hydro part (mesh producer excluded), median of %d steps; static = median of %d steps "
+                      "after %d warm-ups (each leg capped at about 30-40 s of wall); one step each on 1 thread"
+                      % (round(N ** (1 / 3)), len(m_walls), len(s_walls), warmup)}
+
+
+def derivative_costs(m, q, repeats=5):
+    """Wall per call of the static step's derivatives against the forward step on the same 128^3 state (device rows):
+    forward = time_step + step; tangent = time_step_tangent + step_tangent; adjoint = step_adjoint (its own sweep of Q)."""
+    import torch
+
+    fv = hydro.FiniteVolume(m, GAMMA)
+    rng = np.random.Generator(np.random.MT19937(3))
+    dq = torch.as_tensor(rng.standard_normal(q.shape) * 1e-3 * np.abs(q)).cuda()
+    out = {}
+    for name in ("forward", "tangent", "adjoint"):
+        walls = []
+        for k in range(repeats + 1):
+            fv.set_state(q)
+            hydro.synchronize()
+            a = time.perf_counter()
+            if name == "forward":
+                fv.step(fv.time_step(COURANT))
+                fv.fallbacks()
+            elif name == "tangent":
+                fv.step_tangent(fv.time_step_tangent(COURANT, dq), dq)
+            else:
+                fv.step_adjoint(2e-6, dq)
+            hydro.synchronize()
+            if k:
+                walls.append(time.perf_counter() - a)
+        out[name + "_ms"] = 1e3 * float(np.median(walls))
+    fv.close()
+    out["tangent_over_forward"] = out["tangent_ms"] / out["forward_ms"]
+    out["adjoint_over_forward"] = out["adjoint_ms"] / out["forward_ms"]
+    out["reference_a100"] = {"forward_ms": 13.0, "tangent_over_forward": 1.86, "adjoint_over_forward": 2.46,
+                             "source": "research_notes/autodiff_l2_2026-11-20.md:36-45"}
+    return out
+
+
+def sod_1d():
+    """The 1D Sod tube of luna-cpp's hydro_1d (N = 1024, minmod, HLLC, CFL 0.4, LUNA's rule, fused step, t = 0.2):
+    steps and microseconds per step on the GPU (launch-latency bound at 72 KB of state)."""
+    from luna_hydro import hydro1d
+
+    best = None
+    for _ in range(3):
+        steps, t, wall, h, *_ = hydro1d.simulate("sod", 1024, limiter="minmod", courant=0.4, t_end=0.2)
+        best = wall if best is None else min(best, wall)
+    return {"workload": "1D Sod, 1024 cells, minmod, HLLC, CFL 0.4, LUNA rule, fused step, t = 0.2", "steps": steps,
+            "us_per_step": 1e6 * best / steps, "steps_per_s": steps / best, "hash": "%016x" % h}
 
 
 def learned_vs_plain(log):
@@ -204,6 +293,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-limiter comparison")
     ap.add_argument("--no-precision", action="store_true", help="skip the grad32/grad16 policy runs")
+    ap.add_argument("--no-derivatives", action="store_true", help="skip the derivative cost runs")
+    ap.add_argument("--no-sod", action="store_true", help="skip the 1D Sod run")
     ap.add_argument("--hll-compression", type=float, default=0.0,
                     help="the HLL fallback at extreme compression in the FP64 runs (threshold; 0: off, the reference's scheme)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -232,7 +323,7 @@ def main():
                                                                     {k: round(v[1] / v[0], 4) for k, v in s_times.items()}))
 
     log("moving: %d warm-up + %d timed steps" % (args.warmup, args.steps))
-    m_total, hydro_s, mesh_s, m_times = run_moving(m, q, args.steps, args.warmup, hll=args.hll_compression)
+    m_total, hydro_s, mesh_s, m_times, geo_s = run_moving(m, q, args.steps, args.warmup, hll=args.hll_compression)
     hydro_total = float(np.sum(hydro_s))
     value = N * args.steps / hydro_total
     log("moving: hydro %.3f ms/step, %.3g cell updates/s; kernels %s" % (1e3 * hydro_total / args.steps, value,
@@ -257,13 +348,24 @@ def main():
         policies = {}
         for p in ("grad32", "grad16"):
             t_s, w_s, k_s = run_static(m, q, args.static_steps, args.warmup, p)
-            t_m, h_m, _, k_m = run_moving(m, q, max(2, args.steps // 2), args.warmup, p)
+            t_m, h_m, _, k_m, _ = run_moving(m, q, max(2, args.steps // 2), args.warmup, p)
             h_total = float(np.sum(h_m))
             policies[p] = {"static": N * args.static_steps / t_s, "static_ms_per_step": 1e3 * t_s / args.static_steps,
                            "moving": N * len(h_m) / h_total, "moving_hydro_ms_per_step": 1e3 * h_total / len(h_m),
                            "static_kernels_ms_avg": {k: v[1] / v[0] for k, v in k_s.items()},
                            "moving_kernels_ms_avg": {k: v[1] / v[0] for k, v in k_m.items()}}
             log("%s: static %.3g, moving %.3g cell updates/s" % (p, policies[p]["static"], policies[p]["moving"]))
+
+    derivs = None
+    if not args.no_derivatives:
+        log("derivative costs of the static step (forward, tangent, adjoint)")
+        derivs = derivative_costs(m, q)
+        log("derivatives: %s" % {k: v for k, v in derivs.items() if k != "reference_a100"})
+
+    sod = None
+    if not args.no_sod:
+        sod = sod_1d()
+        log("1D Sod: %s" % sod)
 
     learned_cmp = None
     if not args.no_learned:
@@ -272,7 +374,7 @@ def main():
 
     step_bytes = sum(ALGO_BYTES["moving"].values())
     out = {
-        "metric": "cell updates/s at 128^3 moving mesh (hydro part, mesh producer excluded)",
+        "metric": "cell updates/s at 128^3 moving mesh (hydro part with the per-step geometry pass, mesh producer excluded)",
         "value": value,
         "unit": "cell updates/s",
         "n_gpus": 1,
@@ -296,8 +398,13 @@ def main():
         "step_roofline": {"algorithmic_bytes_per_cell": step_bytes,
                           "achieved_GBs": step_bytes * N * args.steps / hydro_total / 1e9,
                           "frac": step_bytes * N * args.steps / hydro_total / 1e9 / HBM_PEAK_GBS},
-        "mesh": {"build_s_first": build_s, "build_and_upload_s_per_step": float(np.mean(mesh_s)),
-                 "note": "the Voronoi producer (C++/OpenMP, host) and the geometry upload, excluded from value"},
+        "mesh": {"build_s_first": build_s, "producer_and_copies_s_per_step": float(np.mean(mesh_s)),
+                 "geometry_pass_ms_per_step": 1e3 * float(np.mean(geo_s)),
+                 "note": "value includes the geometry pass on the device (luna_fv_set_geometry_device: cell rows, face "
+                         "records, slot lists with |A| and face offsets, radii); excluded: the Voronoi producer (C++/OpenMP, "
+                         "host) and the copies standing in for a GPU producer (positions to the host, its arrays to the device)"},
+        "derivatives": derivs,
+        "sod_1d": sod,
         "wall_s_moving_loop": m_total,
         "kernels_ms_avg": {k: v[1] / v[0] for k, v in m_times.items()},
         "static": {"value": static_rate, "unit": "cell updates/s", "ms_per_step": 1e3 * s_total / args.static_steps,

@@ -149,6 +149,17 @@ public:
     }
     // mesh_moved() (:152-156): the caller changed the mesh itself.
     void mesh_moved(const luna_geometry& mesh) { check(luna_fv_set_geometry(fv_, &mesh), "mesh_moved"); }
+    // A moving step with a mesh producer that leaves its arrays on the GPU (voronoi-cpp's device views, read in place as
+    // :257-327 reads them): mesh_of(positions, host copy) -> luna_geometry of device pointers.
+    template <class DeviceMeshOf>
+    void step_device(double dt, DeviceMeshOf&& mesh_of) {
+        std::vector<double> x(static_cast<size_t>(n_) * D);
+        check(luna_fv_moving_begin(fv_, dt, x.data()), "step (moving: begin)");
+        const luna_geometry& next = mesh_of(x);
+        check(luna_fv_set_geometry_device(fv_, &next), "step (moving: the new mesh)");
+        check(luna_fv_moving_finish(fv_, nullptr, dt), "step (moving: finish)");
+    }
+    void mesh_moved_device(const luna_geometry& mesh) { check(luna_fv_set_geometry_device(fv_, &mesh), "mesh_moved"); }
     luna_fv* handle() { return fv_; }
 
 private:

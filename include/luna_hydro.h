@@ -107,10 +107,21 @@ int luna_fv_moving_finish(luna_fv* fv, const luna_geometry* mesh, double dt);
  * the mesh of t^n, with face velocities from the sweep's point velocities, and moves the points off the grid
  * (x + dt w wrapped into the box); finish takes the new mesh and writes its primitives (Q is unchanged). */
 
-/* Uploads a mesh. Between moving_begin and moving_finish: the mesh of the new positions. Otherwise: a caller moved the
+/* Takes a mesh. Between moving_begin and moving_finish: the mesh of the new positions. Otherwise: a caller moved the
  * mesh itself (mesh_moved(), finite_volume.hpp:152-156), and the cached rows are dropped; a moving solver's positions
- * become the mesh's (a restart from a checkpoint of Q and x). */
+ * become the mesh's (a restart from a checkpoint of Q and x). The handle derives what its kernels read (the cell rows,
+ * face records, lists in 64-cell slices with |A| and the face offsets x_f - c_i, the cell radii) in one pass on the GPU
+ * and keeps its own copy, so the caller's arrays may change once the call returns. A malformed mesh (CSR offsets not
+ * running from 0 to 2 faces, a cell or face index out of range) gives LUNA_ERR_ARGUMENT, and the handle then refuses to
+ * step (LUNA_ERR_LOGIC) until a valid mesh is set. */
 int luna_fv_set_geometry(luna_fv* fv, const luna_geometry* mesh);
+
+/* set_geometry and the constructor with the mesh arrays in device memory (pointers from hipMalloc on the handle's
+ * device, or torch tensors' data): voronoi-cpp's device views, which the reference reads in place
+ * (finite_volume.hpp:257-327, 353-364). The side, step and period fields are values as for a host mesh. The arrays must
+ * be ready when the call is made (the handle's stream does not wait on another stream) and are read only during it. */
+int luna_fv_set_geometry_device(luna_fv* fv, const luna_geometry* mesh);
+int luna_fv_create_device(const luna_fv_options* options, const luna_geometry* mesh, luna_fv** out);
 
 /* fallbacks() (finite_volume.hpp:230-238): face sides of the last step that took the cell's state. Fences. */
 int luna_fv_fallbacks(luna_fv* fv, long long* count);
@@ -138,8 +149,9 @@ int luna_fv_set_step_rule(luna_fv* fv, int rule);
  * moving one (LUNA_ERR_LOGIC). The handle keeps its own copy of every mesh it takes, so on a static mesh there is
  * nothing to release and the call only checks the motion. */
 int luna_fv_release_mesh(luna_fv* fv);
-/* mesh_stats() (finite_volume.hpp:160-163): of the reference's counters, the meshes this handle took after its first one
- * (set_geometry and moving_finish with a mesh); the rest are the mesh library's own (voronoi-cpp's update report). */
+/* mesh_stats() (finite_volume.hpp:160-163): of the reference's counters, `updates`, the mesh updates of moving steps
+ * (:1237; here the meshes a moving step took between begin and finish); the rest are the mesh library's own
+ * (voronoi-cpp's update report). A mesh set outside a step (mesh_moved()) is not an update. */
 int luna_fv_mesh_stats(luna_fv* fv, long long* updates);
 
 /* Kernel timing with HIP events on the handle's stream (on: 1, off: 0; turning it on clears the sums). read: up to max
@@ -254,6 +266,21 @@ typedef int (*luna_geometry_adjoint_fn)(const void* ctx, const double* Vbar, con
  * limiter, a reduced precision policy or the HLL fallback. */
 int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, const luna_geometry* mesh1, luna_geometry_adjoint_fn adjoint0,
                            const void* ctx0, luna_geometry_adjoint_fn adjoint1, const void* ctx1, int steps_fixed, double* qbar, double* xbar);
+
+/* The geometry's tangent the moving step's tangent reads (hydro::GeometryDerivative::tangent, mesh_derivative.hpp:26-27): from dx
+ * (cells x dim), write dV (cells), dc (cells x dim), dA and dxf (faces x dim); host arrays; 0 on success. The mesh library's
+ * luna_mesh_derivative_tangent (include/luna_mesh.h) has this shape, with its handle as ctx. */
+typedef int (*luna_geometry_tangent_fn)(const void* ctx, const double* dx, double* dV, double* dc, double* dA, double* dxf);
+
+/* FiniteVolume::step_moving_tangent(dt, dq, dx, geometry, steps_fixed) (finite_volume.hpp:2428-2486 order 0, 2764-2870 order 1),
+ * at fixed topology: the forward mode of the moving step at the state the handle holds (Q at t^n, as set; mesh0 the mesh of t^n,
+ * its positions x^n; mesh1 the mesh the forward step reached from it, not read at order 0). dq (cells x (dim + 2)) and dx
+ * (cells x dim), host, the tangents of Q and x at t^n, become those at t^n+1, in place. tangent0 / tangent1: the two meshes'
+ * geometry tangents (called on the host). steps_fixed as luna_fv_moving_adjoint. The reference's member also takes the
+ * forward step; here the caller's forward (moving_begin / finish) takes it, and the handle is left on mesh0 with Q unchanged,
+ * as luna_fv_moving_adjoint leaves it. Blocks. LUNA_ERR_LOGIC where luna_fv_moving_adjoint refuses. */
+int luna_fv_moving_tangent(luna_fv* fv, double dt, const luna_geometry* mesh0, const luna_geometry* mesh1, luna_geometry_tangent_fn tangent0,
+                           const void* ctx0, luna_geometry_tangent_fn tangent1, const void* ctx1, int steps_fixed, double* dq, double* dx);
 
 /* ------------------------------------------------------------ the learned slope limiter (no counterpart in luna-cpp)
  *

@@ -25,6 +25,11 @@ typedef struct {
     const double* q;             /* N x 5 extensive conserved state (FiniteVolume::state()); U = Q / V */
     const char* commit;          /* provenance: the library's commit (or "") */
     int dirty;
+    /* the mesh producer's provenance, written as voronoi_cpp_commit, voronoi_cpp_dirty and voronoi_cpp_origin (the
+     * reference's build_info of its mesh library); NULL strings are written empty */
+    const char* mesh_commit;
+    int mesh_dirty;
+    const char* mesh_origin;
 } luna_snapshot;
 
 /* Writes the snapshot; name (capacity cap) receives the file's path. 0 on success, -1 with luna_snapshot_error(). */

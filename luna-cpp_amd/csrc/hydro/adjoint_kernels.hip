@@ -1267,6 +1267,405 @@ __global__ __launch_bounds__(BLOCK) void k_mfaces_bar(Geo g, const double* entry
     }
 }
 
+
+// ---- the moving step's tangent (forward mode, fixed topology; finite_volume.hpp:2428-2486, 2764-2870) ----
+// Device rows of the tangent: the geometry's tangents of the pass's mesh (dV, dc per cell; dA, dxf per face), the positions'
+// (dx), c^n's (dc0, the first mesh's dc), the W and c_s tangents (dW: N x V, dcs: N), the point velocities' (dw) and the
+// limited gradients' (dg: N x V D); the faces' dG rows.
+
+// FaceVelocity::tangent (finite_volume.hpp:2322-2330), at the forward's branch of the 0.9 |d| limit.
+template <int D>
+__device__ __forceinline__ void face_velocity_tangent(const double* d, const double* xf, const double* wL, const double* wR, const double* dd,
+                                                      const double* dxf, const double* dwL, const double* dwR, double* dwf) {
+    double c[D], u[D], dc[D], du[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        c[a] = xf[a] - d[a] * 0.5;
+        u[a] = wL[a] - wR[a];
+        dc[a] = dxf[a] - dd[a] * 0.5;
+        du[a] = dwL[a] - dwR[a];
+    }
+    const double dist = sqrt(dot<D>(d, d)), ddist = dot<D>(d, dd) / dist;
+    const double f0 = dot<D>(c, u) / dist, df0 = (dot<D>(dc, u) + dot<D>(c, du)) / dist - f0 * ddist / dist;
+    const double limit = 0.9 * dist, c2 = dot<D>(c, c);
+    double facv = f0, dfacv = df0;
+    if (c2 > limit * limit) {
+        const double cc = sqrt(c2), sc = limit / cc, dsc = 0.9 * ddist / cc - sc * dot<D>(c, dc) / c2;
+        facv = f0 * sc;
+        dfacv = df0 * sc + f0 * dsc;
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) dwf[a] = (dwL[a] + dwR[a]) * 0.5 + dd[a] * (facv / dist) + d[a] * (dfacv / dist - facv * ddist / (dist * dist));
+}
+
+// MovingFace::tangent (euler.hpp:861-891): dG of G = |A| F(L, R, A / |A|, w). F = to_lab(Phi(L - w, R - w, n), w); Phi's change
+// with n from its rotation covariance (Omega = dn n^T - n dn^T): one Riemann tangent at (L, R) with dL - dw - Omega L_w and
+// dR - dw - Omega R_w, plus Omega Phi and to_lab's own change with w.
+template <int D>
+__device__ void moving_face_tangent(int riemann_kind, const Gas& gas, const double* L, const double* R, const double* A, const double* w,
+                                    const double* dL, const double* dR, const double* dA, const double* dw, double* dG) {
+    constexpr int V = D + 2, P = D + 1;
+    const double s = sqrt(dot<D>(A, A));
+    double n[D], dn[D], mw[D], lw[D], rw[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        n[k] = A[k] * (1.0 / s);
+        mw[k] = -w[k];
+        lw[k] = L[1 + k] - w[k];
+        rw[k] = R[1 + k] - w[k];
+    }
+    const double ds = dot<D>(n, dA);
+#pragma unroll
+    for (int k = 0; k < D; ++k) dn[k] = (dA[k] - n[k] * ds) / s;
+    double eL[V], eR[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        eL[v] = dL[v];
+        eR[v] = dR[v];
+    }
+    const double nl = dot<D>(n, lw), dnl = dot<D>(dn, lw), nr = dot<D>(n, rw), dnr = dot<D>(dn, rw);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        eL[1 + k] -= dw[k] + (dn[k] * nl - n[k] * dnl);
+        eR[1 + k] -= dw[k] + (dn[k] * nr - n[k] * dnr);
+    }
+    double F[V], dF[V], Phi[V], X[V];
+    riemann_tangent<D>(riemann_kind, gas, L, R, eL, eR, n, w, F, dF);
+#pragma unroll
+    for (int v = 0; v < V; ++v) Phi[v] = F[v];
+    to_lab<D>(Phi, mw);
+    const double nP = dot<D>(n, Phi + 1), dnP = dot<D>(dn, Phi + 1);
+    X[0] = 0;
+    X[P] = 0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) X[1 + k] = dn[k] * nP - n[k] * dnP;
+    to_lab<D>(X, w);
+#pragma unroll
+    for (int v = 0; v < V; ++v) dF[v] += X[v];
+    dF[P] += dot<D>(dw, Phi + 1) + dot<D>(w, dw) * Phi[0];
+#pragma unroll
+    for (int k = 0; k < D; ++k) dF[1 + k] += dw[k] * Phi[0];
+#pragma unroll
+    for (int v = 0; v < V; ++v) dG[v] = ds * F[v] + s * dF[v];
+}
+
+// The tangent of face_offset(f, i): dx_f - dc_i, plus the owner's separation's when i does not own f.
+template <int D>
+__device__ __forceinline__ void offset_dot(const double* dxf, bool owned, const double* dxi, const double* dxo, const double* dci, double* dr) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) dr[a] = dxf[a] - dci[a] + (owned ? 0.0 : dxo[a] - dxi[a]);
+}
+
+// W, c_s tangents per cell (finite_volume.hpp:2408-2426 moving_primitives): dU = (dq - U dV) / V.
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_mprim_tangent(Geo g, Rows r, Gas gas, const double* dq, const double* dV, double* dW, double* dcs) {
+    constexpr int V = D + 2;
+    using C = CellAt<D>;
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= g.N) return;
+    const double* row = g.cell + (size_t)i * CELL_ROW;
+    const double inv = 1.0 / row[C::VOL];
+    double U[V], dU[V], Wt[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) U[v] = r.q[(size_t)i * V + v] * inv;
+#pragma unroll
+    for (int v = 0; v < V; ++v) dU[v] = (dq[(size_t)i * V + v] - U[v] * dV[i]) * inv;
+    primitive_tangent<D>(gas, U, dU, Wt);
+    const double rho = row[C::W], P = row[C::W + D + 1], cs = row[C::CS];
+    double* out = dW + (size_t)i * V;
+#pragma unroll
+    for (int v = 0; v < V; ++v) out[v] = Wt[v];
+    dcs[i] = cs > 0 ? 0.5 * gas.gamma * (Wt[D + 1] / rho - P * Wt[0] / (rho * rho)) / cs : 0.0;
+}
+
+// Per cell on t^n: the point velocity's tangent (finite_volume.hpp:2251-2306 Regularised::tangent) and, at order 1, the limited
+// fit's with its geometry (:2687-2762 moving_gradient_tangent: x_v = M^-1 b_v moves by M^-1 (db_v - dM x_v); after the last
+// clip g = g0 (bound - W_i) / (g0 . o), o that face's offset, which moves too).
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_mcell_tangent(Geo g, Rows r, int limit, int fit, const double* dW, const double* dcs, const double* dx,
+                                                         const double* dc, const double* dA, const double* dxf, double* dw, double* dg) {
+    constexpr int V = D + 2, U = D + 1;
+    using C = CellAt<D>;
+    using FA = FaceAt<D>;
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= g.N) return;
+    const double* mine = g.cell + (size_t)i * CELL_ROW;
+    double Wi[V], xi[D], ci[D], dxi[D], dci[D], dWi[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        Wi[v] = mine[C::W + v];
+        dWi[v] = dW[(size_t)i * V + v];
+    }
+    load_vec<D>(mine + C::X, xi);
+    load_vec<D>(mine + C::C, ci);
+    load_vec<D>(dx + (size_t)i * D, dxi);
+    load_vec<D>(dc + (size_t)i * D, dci);
+    const int count = g.count[i], base = first_slot(g, i);
+    // the point velocity w = v + c (fraction c_s / |c|), fraction from the largest face angle
+    {
+        double amax = 0, dmax[D];
+        int fmax = -1, other = -1;
+#pragma unroll
+        for (int a = 0; a < D; ++a) dmax[a] = 0;
+        for (int k = 0; k < count; ++k) {
+            const int slot = base + k * SLICE;
+            const int e = g.list[slot], f = e >= 0 ? e : ~e, j = g.other[slot];
+            double xj[D], d[D];
+            load_vec<D>(g.cell + (size_t)j * CELL_ROW + C::X, xj);
+            separation<D>(g, xi, xj, d);
+            const double ang = face_angle<D>(g.face[(size_t)f * FACE_ROW + FA::MAG]) / (0.5 * sqrt(dot<D>(d, d)));
+            if (ang > amax) {
+                amax = ang;
+                fmax = f;
+                other = j;
+#pragma unroll
+                for (int a = 0; a < D; ++a) dmax[a] = d[a];
+            }
+        }
+        const double Fc = 2.25, shaping = 0.5;
+        double fraction = 0, slope = 0;
+        if (amax > 0.75 * Fc) {
+            fraction = amax > Fc ? shaping : shaping * (amax - 0.75 * Fc) / (0.25 * Fc);
+            slope = amax > Fc ? 0.0 : shaping / (0.25 * Fc);
+        }
+        double t[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) t[a] = dWi[1 + a];
+        const double cd = sqrt(dot<D>(ci, ci)), cs = mine[C::CS];
+        if (cd > 0 && fraction > 0) {
+            const double kk = fraction * cs / cd;
+            double dfraction = 0;
+            if (slope != 0) {
+                const double* A = g.face + (size_t)fmax * FACE_ROW + FA::A;
+                double dd[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) dd[a] = dx[(size_t)other * D + a] - dxi[a];
+                const double mag = sqrt(dot<D>(A, A)), dist = sqrt(dot<D>(dmax, dmax));
+                const double dmag = dot<D>(A, dA + (size_t)fmax * D) / mag, ddist = dot<D>(dmax, dd) / dist;
+                dfraction = slope * (face_angle_slope<D>(mag) * dmag / (0.5 * dist) - amax * ddist / dist);
+            }
+            const double dcd = dot<D>(ci, dci) / cd;
+            const double dk = (dfraction * cs + fraction * dcs[i]) / cd - kk * dcd / cd;
+#pragma unroll
+            for (int a = 0; a < D; ++a) t[a] += dci[a] * kk + ci[a] * dk;
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a) dw[(size_t)i * D + a] = t[a];
+    }
+    if (!fit) return;
+    // the limited fit: the forward's values and pattern (the bound's tangent from dW), then the geometry's parts
+    double gl[V * D], g0[V * D], dg0[V * D], Lc[U][U], M0[U], icpt[V];
+    Pattern<D> pat;
+    fit_cell<D, true>(g, i, dW, limit != 0, gl, g0, dg0, Lc, M0, pat, icpt);
+    double dM[U][U], db[V][U];
+#pragma unroll
+    for (int p = 0; p < U; ++p)
+#pragma unroll
+        for (int q = 0; q < U; ++q) dM[p][q] = 0;
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+#pragma unroll
+        for (int p = 0; p < U; ++p) db[v][p] = 0;
+    for (int k = 0; k < count; ++k) {
+        const int slot = base + k * SLICE;
+        const int e = g.list[slot], f = e >= 0 ? e : ~e, j = g.other[slot];
+        const double* row = g.cell + (size_t)j * CELL_ROW;
+        double xj[D], d[D], X[U], dX[U];
+        load_vec<D>(row + C::X, xj);
+        separation<D>(g, xi, xj, d);
+        const double* face = g.face + (size_t)f * FACE_ROW;
+        const double w = face[FA::MAG];
+        double Af[D], dAf[D];
+        load_vec<D>(face + FA::A, Af);
+        load_vec<D>(dA + (size_t)f * D, dAf);
+        const double dwt = w > 0 ? dot<D>(Af, dAf) / w : 0.0;
+        X[0] = 1;
+        dX[0] = 0;
+#pragma unroll
+        for (int p = 1; p < U; ++p) {
+            X[p] = d[p - 1] + row[C::C + p - 1] - ci[p - 1];
+            dX[p] = (dx[(size_t)j * D + p - 1] - dxi[p - 1]) + dc[(size_t)j * D + p - 1] - dci[p - 1];
+        }
+#pragma unroll
+        for (int p = 0; p < U; ++p)
+#pragma unroll
+            for (int q = 0; q < U; ++q) dM[p][q] += dwt * X[p] * X[q] + w * (dX[p] * X[q] + X[p] * dX[q]);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const double delta = row[C::W + v] - Wi[v], ddelta = dW[(size_t)j * V + v] - dWi[v];
+#pragma unroll
+            for (int p = 0; p < U; ++p) db[v][p] += dwt * X[p] * delta + w * dX[p] * delta + w * X[p] * ddelta;
+        }
+    }
+    double* out = dg + (size_t)i * V * D;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        double z[U], xs[U], rhs[U];
+#pragma unroll
+        for (int p = 0; p < U; ++p) {
+            double mx = dM[p][0] * icpt[v];
+#pragma unroll
+            for (int a = 0; a < D; ++a) mx += dM[p][1 + a] * g0[v * D + a];
+            rhs[p] = db[v][p] - mx;
+        }
+#pragma unroll
+        for (int p = 0; p < U; ++p) {
+            double sum = rhs[p];
+#pragma unroll
+            for (int m = 0; m < p; ++m) sum -= Lc[p][m] * z[m];
+            z[p] = sum / Lc[p][p];
+        }
+#pragma unroll
+        for (int p = U - 1; p >= 0; --p) {
+            double sum = z[p];
+#pragma unroll
+            for (int m = p + 1; m < U; ++m) sum -= Lc[m][p] * xs[m];
+            xs[p] = sum / Lc[p][p];
+        }
+        const double* gv = g0 + v * D;
+        if (pat.last[v] < 0) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) out[v * D + a] = xs[1 + a];
+        } else if (pat.zero[v]) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) out[v * D + a] = 0.0;
+        } else {
+            const int slot = base + pat.last[v] * SLICE;
+            const int e = g.list[slot], f = e >= 0 ? e : ~e;
+            double dor[D];
+            offset_dot<D>(dxf + (size_t)f * D, e >= 0, dxi, dx + (size_t)g.other[slot] * D, dci, dor);
+            const double* o = pat.offset[v];
+            double dg0v[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) dg0v[a] = xs[1 + a];
+            const double den = dot<D>(gv, o);
+            const double sc = (pat.bound[v] - Wi[v]) / den;
+            const double dsc = (pat.dbound[v] - dWi[v] - sc * (dot<D>(dg0v, o) + dot<D>(gv, dor))) / den;
+#pragma unroll
+            for (int a = 0; a < D; ++a) out[v * D + a] = dg0v[a] * sc + gv[a] * dsc;
+        }
+    }
+}
+
+// One moving pass's face tangent (finite_volume.hpp:2829-2870 moving_pass_tangent): dG of G = |A| F(side_o, side_j, A / |A|,
+// face_velocity(d, x_f, w-hat_o, w-hat_j)); first pass (order0: the order-0 step's one pass, w-hat = the sweep's w, no
+// gradient) sides W + g.r, second half_step(W, g, 2 dt) + g.(r + c - c^n + dt w-hat).
+template <int D, bool Second>
+__global__ __launch_bounds__(BLOCK) void k_mpass_face_tangent(Geo g, Rows r, Gas gas, int riemann_kind, double dt, double unit, int order0,
+                                                              const double* dW, const double* dg, const double* dw, const double* dx,
+                                                              const double* dc, const double* dA, const double* dxf, const double* dc0,
+                                                              double* dG) {
+    constexpr int V = D + 2, GR = grad_row<D>();
+    using C = CellAt<D>;
+    using FA = FaceAt<D>;
+    const int f = blockIdx.x * BLOCK + threadIdx.x;
+    if (f >= g.F) return;
+    double* out = dG + (size_t)f * V;
+    const double* face = g.face + (size_t)f * FACE_ROW;
+    if (face[FA::MAG] == 0) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) out[v] = 0;
+        return;
+    }
+    const int2 pair = *reinterpret_cast<const int2*>(face + FA::PAIR);
+    const int cells[2] = {pair.x, pair.y};
+    double A[D], fc[D], xs[2][D], d[D], back[D], wh[2][D], dwh[2][D], dxs[2][D], wf[D], dwf[D], dd[D];
+    load_vec<D>(face + FA::A, A);
+    load_vec<D>(face + FA::FC, fc);
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2) {
+        const int i = cells[s2];
+        load_vec<D>(g.cell + (size_t)i * CELL_ROW + C::X, xs[s2]);
+        load_vec<D>(dx + (size_t)i * D, dxs[s2]);
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            wh[s2][a] = order0 ? r.w[(size_t)i * D + a] : (double)r.steps[(size_t)i * D + a] * unit;
+            dwh[s2][a] = dw[(size_t)i * D + a];
+        }
+    }
+    separation<D>(g, xs[0], xs[1], d);     // owner -> neighbour
+    separation<D>(g, xs[1], xs[0], back);  // neighbour -> owner
+#pragma unroll
+    for (int a = 0; a < D; ++a) dd[a] = dxs[1][a] - dxs[0][a];
+    face_velocity<D>(d, fc, wh[0], wh[1], wf);
+    face_velocity_tangent<D>(d, fc, wh[0], wh[1], dd, dxf + (size_t)f * D, dwh[0], dwh[1], dwf);
+    double sides[2][V], dsides[2][V];
+#pragma unroll
+    for (int s2 = 0; s2 < 2; ++s2) {
+        const int i = cells[s2];
+        const double* cr = g.cell + (size_t)i * CELL_ROW;
+        double W[V], dWi[V], gr[V * D], dgr[V * D], rr[D], dr[D];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            W[v] = cr[C::W + v];
+            dWi[v] = dW[(size_t)i * V + v];
+        }
+#pragma unroll
+        for (int k = 0; k < V * D; ++k) {
+            gr[k] = order0 ? 0.0 : r.grad[(size_t)i * GR + k];
+            dgr[k] = order0 ? 0.0 : dg[(size_t)i * V * D + k];
+        }
+        const double* dci = dc + (size_t)i * D;
+        offset_dot<D>(dxf + (size_t)f * D, s2 == 0, dxs[s2], dxs[0], dci, dr);
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            rr[a] = (s2 == 0 ? fc[a] : fc[a] + back[a]) - cr[C::C + a];
+            if constexpr (Second) {
+                rr[a] = rr[a] + cr[C::C + a] - r.old_centroid[(size_t)i * D + a] + wh[s2][a] * (0.5 * (2 * dt));
+                dr[a] = dr[a] + dci[a] - dc0[(size_t)i * D + a] + dwh[s2][a] * dt;
+            }
+        }
+        if constexpr (Second) {
+            half_step<D>(gas, W, gr, 2 * dt, sides[s2]);
+            half_step_tangent<D>(gas, W, dWi, gr, dgr, 2 * dt, dsides[s2]);
+        } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                sides[s2][v] = W[v];
+                dsides[s2][v] = dWi[v];
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) sides[s2][v] += dot<D>(gr + v * D, rr);
+        if (sides[s2][0] > 0 && sides[s2][D + 1] > 0) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) dsides[s2][v] += dot<D>(dgr + v * D, rr) + dot<D>(gr + v * D, dr);
+        } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                sides[s2][v] = W[v];
+                dsides[s2][v] = dWi[v];
+            }
+        }
+    }
+    moving_face_tangent<D>(riemann_kind, gas, sides[0], sides[1], A, wf, dsides[0], dsides[1], dA + (size_t)f * D, dwf, out);
+}
+
+// dq -= scale sum_k sign_k dG_k per cell in list order (scale: dt / 2 per pass, dt at order 0).
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_mpass_gather_tangent(Geo g, double scale, const double* dG, double* dq) {
+    constexpr int V = D + 2;
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= g.N) return;
+    double sum[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) sum[v] = 0;
+    const int count = g.count[i], base = first_slot(g, i);
+    for (int k = 0; k < count; ++k) {
+        const int e = g.list[base + k * SLICE], f = e >= 0 ? e : ~e;
+        const double sign = e >= 0 ? 1.0 : -1.0;
+#pragma unroll
+        for (int v = 0; v < V; ++v) sum[v] += sign * dG[(size_t)f * V + v];
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) dq[(size_t)i * V + v] -= scale * sum[v];
+}
+
+// out = x + dt y (n entries).
+__global__ __launch_bounds__(BLOCK) void k_axpy(long long n, double dt, const double* x, const double* y, double* out) {
+    const long long k = (long long)blockIdx.x * BLOCK + threadIdx.x;
+    if (k < n) out[k] = x[k] + dt * y[k];
+}
 }  // namespace
 
 template <int D>
@@ -1326,6 +1725,41 @@ hipError_t launch_moving_cell_adjoint(hipStream_t s, const Geo& g, const Rows& r
     return hipGetLastError();
 }
 
+template <int D>
+hipError_t launch_moving_tangent_cells(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, bool limit, bool fit, const double* dq,
+                                       const MovingDots& t) {
+    hipLaunchKernelGGL(k_mprim_tangent<D>, grid_of(g.N), dim3(BLOCK), 0, s, g, r, gas, dq, t.dV, t.dW, t.dcs);
+    hipLaunchKernelGGL(k_mcell_tangent<D>, grid_of(g.N), dim3(BLOCK), 0, s, g, r, limit ? 1 : 0, fit ? 1 : 0, t.dW, t.dcs, t.dx, t.dc, t.dA, t.dxf,
+                       t.dw, t.dg);
+    return hipGetLastError();
+}
+
+template <int D>
+hipError_t launch_moving_pass_tangent(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, double unit, bool second,
+                                      bool order0, const MovingDots& t, double* dq) {
+    if (g.F > 0) {
+        if (second)
+            hipLaunchKernelGGL((k_mpass_face_tangent<D, true>), grid_of(g.F), dim3(BLOCK), 0, s, g, r, gas, riemann, dt, unit, 0, t.dW, t.dg, t.dw,
+                               t.dx, t.dc, t.dA, t.dxf, t.dc0, t.dG);
+        else
+            hipLaunchKernelGGL((k_mpass_face_tangent<D, false>), grid_of(g.F), dim3(BLOCK), 0, s, g, r, gas, riemann, dt, unit, order0 ? 1 : 0, t.dW,
+                               t.dg, t.dw, t.dx, t.dc, t.dA, t.dxf, t.dc0, t.dG);
+    }
+    hipLaunchKernelGGL(k_mpass_gather_tangent<D>, grid_of(g.N), dim3(BLOCK), 0, s, g, order0 ? dt : 0.5 * dt, t.dG, dq);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy(hipStream_t s, long long n, double dt, const double* x, const double* y, double* out) {
+    if (n > 0) hipLaunchKernelGGL(k_axpy, grid_of(n), dim3(BLOCK), 0, s, n, dt, x, y, out);
+    return hipGetLastError();
+}
+
+template hipError_t launch_moving_tangent_cells<2>(hipStream_t, const Geo&, const Rows&, const Gas&, bool, bool, const double*, const MovingDots&);
+template hipError_t launch_moving_tangent_cells<3>(hipStream_t, const Geo&, const Rows&, const Gas&, bool, bool, const double*, const MovingDots&);
+template hipError_t launch_moving_pass_tangent<2>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, double, bool, bool, const MovingDots&,
+                                                  double*);
+template hipError_t launch_moving_pass_tangent<3>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, double, bool, bool, const MovingDots&,
+                                                  double*);
 template hipError_t launch_moving_pass_adjoint<2>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, double, bool, const double*,
                                                   const MovingBars&, bool);
 template hipError_t launch_moving_pass_adjoint<3>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, double, bool, const double*,

@@ -159,6 +159,8 @@ struct luna_fv {
     // the moving step's reverse (first use)
     Buf<double> m_row, m_Ab, m_Xfb, m_Wbar, m_gbar, m_whbar, m_c0bar, m_cbar, m_xbar, m_beta, m_boundbar, m_entry, m_angle, m_Vbar, m_out, m_qbar;
     Buf<int> m_extreme, m_fmax;
+    // the moving step's tangent (first use)
+    Buf<double> t_dx, t_dx1, t_dV, t_dc, t_dc0, t_dA, t_dxf, t_dW, t_dcs, t_dw, t_dg, t_dG, t_dq;
     // the learned limiter: its features and troubled flags (recorded by each sweep) and the network copy
     Buf<unsigned short> features;
     Buf<unsigned char> troubled, net;
@@ -167,17 +169,17 @@ struct luna_fv {
     double side[3] = {0, 0, 0};
     luna::Grid grid{};
     int parity = 0;
-    long long uploads = 0;  // meshes taken, the first one included
+    long long updates = 0;  // mesh updates of moving steps (MeshStats::updates, finite_volume.hpp:1237)
     bool prims_ready = false, swept = false, begun = false, new_mesh = false;
     Profiler prof;
-    // pinned staging of the packed mesh records (reused across uploads)
-    double* stage_cell = nullptr;
-    double* stage_face = nullptr;
-    int* stage_list = nullptr;
-    int* stage_other = nullptr;
-    int* stage_count = nullptr;
-    double* stage_slot = nullptr;
-    size_t stage_cell_cap = 0, stage_face_cap = 0, stage_list_cap = 0, stage_other_cap = 0, stage_count_cap = 0, stage_slot_cap = 0;
+    // a host mesh's arrays in device memory (the geometry pass reads them as it reads a device mesh); the centroids,
+    // positions and volumes share src_cells
+    Buf<int> src_owner, src_neighbour, src_start, src_list;
+    Buf<double> src_area, src_fc, src_cells;
+    // the geometry pass's total slot count and its checks' flags
+    Buf<long long> geo_info;
+    Buf<unsigned int> geo_flags;
+    bool mesh_ok = false;  // the last mesh given was accepted
 
     luna::Geo geo() const {
         luna::Geo g{};
@@ -221,190 +223,137 @@ int check_geometry(const luna_geometry* m, int D, int N) {
     if (m->faces < 0 || !m->owner || !m->neighbour || !m->start || !m->list || !m->area || !m->face_centroid || !m->centroid ||
         !m->volume || !m->positions)
         return fail(LUNA_ERR_ARGUMENT, "luna: mesh arrays missing");
-    if (m->start[0] != 0) return fail(LUNA_ERR_ARGUMENT, "luna: CSR start[0] must be 0");
     for (int a = 0; a < D; ++a)
         if (!(m->side[a] > 0)) return fail(LUNA_ERR_ARGUMENT, "luna: box sides must be positive");
     return LUNA_OK;
 }
 
-// Runs body(lo, hi) over [0, n) on the host's cores.
-template <class Body>
-void parallel_for(long long n, Body&& body) {
-    const unsigned hw = std::thread::hardware_concurrency();
-    const long long threads = std::max(1LL, std::min<long long>(hw ? hw : 1, n / 65536 + 1));
-    if (threads == 1) {
-        body(0LL, n);
-        return;
-    }
-    std::vector<std::thread> pool;
-    for (long long t = 0; t < threads; ++t) pool.emplace_back([&, t] { body(n * t / threads, n * (t + 1) / threads); });
-    for (auto& th : pool) th.join();
-}
-
-template <class T>
-hipError_t grow_pinned(T*& p, size_t& cap, size_t n) {
-    if (n <= cap) return hipSuccess;
-    if (p) {
-        hipError_t e = hipHostFree(p);
-        if (e != hipSuccess) return e;
-        p = nullptr;
-        cap = 0;
-    }
-    const size_t want = n + n / 8 + 1;
-    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(T) * want, hipHostMallocDefault);
-    if (e == hipSuccess) cap = want;
-    return e;
-}
-
-// Uploads a mesh as the kernels read it (kernels.hpp): the geometry half of the cell rows [x, c, V, r], face records
-// [A, fc, |A|, (o, j)], and the lists in 64-cell slices with, per entry, the cell on the other side. The packing runs on
-// the host, beside the mesh producer, as voronoi-cpp's Layout lets the mesh write the rows the hydro reads fastest
-// (finite_volume.hpp:119-128). The radius (finite_volume.hpp:241-246) is taken with the C library's cbrt (sqrt in 2D),
-// the function the reference's host build calls (the device's cbrt rounds some volumes one ulp apart); |A| is the
-// correctly rounded sqrt of A.A as on the device. Host checks: the index ranges the kernels rely on.
-int upload(luna_fv* fv, const luna_geometry* m) {
+// The mesh a handle steps on, from the mesh producer's arrays (device pointers in `in`) to the rows the kernels read: the
+// geometry pass of geometry_kernels.hip on the handle's stream. One read-back sizes the list slots after the scan, one
+// at the end collects the pass's checks; the caller's arrays are no longer read when this returns. A malformed mesh
+// (CSR offsets, cell or face indices out of range, more than 2^31 slots) is refused, and the handle takes no step until
+// a valid mesh is set.
+int derive(luna_fv* fv, const luna_geometry* m, luna::MeshIn in) {
     const int D = fv->D, N = fv->N;
-    if (int rc = check_geometry(m, D, N)) return rc;
+    fv->mesh_ok = false;
     if (fv->opt.motion == LUNA_MOTION_MOVING) {
         if (!(m->step > 0)) return fail(LUNA_ERR_ARGUMENT, "luna: a moving mesh needs its snapping grid step");
         for (int a = 0; a < D; ++a)
             if (m->period[a] <= 0) return fail(LUNA_ERR_ARGUMENT, "luna: a moving mesh needs the grid's periods");
     }
-    const int F = m->faces, E = m->start[N];
-    if (E != 2 * F) return fail(LUNA_ERR_ARGUMENT, "luna: every face must appear in two lists");
-    for (int i = 0; i < N; ++i)
-        if (m->start[i + 1] < m->start[i]) return fail(LUNA_ERR_ARGUMENT, "luna: CSR start must not decrease");
-    for (int f = 0; f < F; ++f)
-        if (m->owner[f] < 0 || m->owner[f] >= N || m->neighbour[f] < 0 || m->neighbour[f] >= N)
-            return fail(LUNA_ERR_ARGUMENT, "luna: face cell index out of range");
-    for (int k = 0; k < E; ++k) {
-        const int e = m->list[k], f = e >= 0 ? e : ~e;
-        if (f >= F) return fail(LUNA_ERR_ARGUMENT, "luna: list entry out of range");
-    }
-    // slices of 64 cells, each as long as its longest list
+    const int F = m->faces;
     const int S = (N + luna::SLICE - 1) / luna::SLICE;
-    std::vector<int> slice(S);
-    long long slots = 0;
-    for (int g = 0; g < S; ++g) {
-        int longest = 0;
-        for (int i = g * luna::SLICE; i < std::min(N, (g + 1) * luna::SLICE); ++i) longest = std::max(longest, m->start[i + 1] - m->start[i]);
-        if (slots + static_cast<long long>(longest) * luna::SLICE > INT_MAX) return fail(LUNA_ERR_ARGUMENT, "luna: mesh lists exceed 2^31 slots");
-        slice[g] = static_cast<int>(slots);
-        slots += static_cast<long long>(longest) * luna::SLICE;
+    for (int a = 0; a < 3; ++a) {
+        in.side[a] = a < D ? m->side[a] : 0.0;
+        in.inverse[a] = in.side[a] > 0 ? 1.0 / in.side[a] : 0.0;
     }
+    hipStream_t s = fv->stream;
     LUNA_HIP(fv->slice.reserve(S));
+    LUNA_HIP(fv->facerec.reserve(static_cast<size_t>(F) * luna::FACE_ROW));
+    LUNA_HIP(fv->flux.reserve(static_cast<size_t>(F) * fv->V));
+    LUNA_HIP(hipMemsetAsync(fv->geo_flags.p, 0, sizeof(unsigned int), s));
+    LUNA_HIP(timed(fv->prof, s, "geometry_cells", [&] {
+        return D == 3 ? luna::launch_geometry_cells<3>(s, in, fv->cell.p, fv->count.p, fv->slice.p, fv->geo_flags.p)
+                      : luna::launch_geometry_cells<2>(s, in, fv->cell.p, fv->count.p, fv->slice.p, fv->geo_flags.p);
+    }));
+    LUNA_HIP(timed(fv->prof, s, "geometry_scan", [&] { return luna::launch_geometry_scan(s, fv->slice.p, S, fv->geo_info.p, fv->geo_flags.p); }));
+    unsigned long long* back = fv->host + 4;
+    LUNA_HIP(hipMemcpyAsync(back, fv->geo_info.p, sizeof(long long), hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipMemcpyAsync(back + 1, fv->geo_flags.p, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipStreamSynchronize(s));
+    unsigned int flags = static_cast<unsigned int>(back[1]);
+    if (flags & luna::GEO_BAD_START) return fail(LUNA_ERR_ARGUMENT, "luna: CSR start must run from 0 to 2 x faces without decreasing");
+    if (flags & luna::GEO_TOO_MANY_SLOTS) return fail(LUNA_ERR_ARGUMENT, "luna: mesh lists exceed 2^31 slots");
+    const long long slots = static_cast<long long>(back[0]);
     LUNA_HIP(fv->list.reserve(slots > 0 ? slots : 1));
     LUNA_HIP(fv->other.reserve(slots > 0 ? slots : 1));
     LUNA_HIP(fv->slotgeo.reserve(static_cast<size_t>(D + 1) * (slots > 0 ? slots : 1)));
-    LUNA_HIP(fv->facerec.reserve(static_cast<size_t>(F) * luna::FACE_ROW));
-    LUNA_HIP(fv->flux.reserve(static_cast<size_t>(F) * fv->V));
-    LUNA_HIP(grow_pinned(fv->stage_cell, fv->stage_cell_cap, static_cast<size_t>(N) * luna::GEO_ROW));
-    LUNA_HIP(grow_pinned(fv->stage_face, fv->stage_face_cap, static_cast<size_t>(F) * luna::FACE_ROW));
-    LUNA_HIP(grow_pinned(fv->stage_list, fv->stage_list_cap, static_cast<size_t>(slots)));
-    LUNA_HIP(grow_pinned(fv->stage_other, fv->stage_other_cap, static_cast<size_t>(slots)));
-    LUNA_HIP(grow_pinned(fv->stage_count, fv->stage_count_cap, static_cast<size_t>(N)));
-    LUNA_HIP(grow_pinned(fv->stage_slot, fv->stage_slot_cap, static_cast<size_t>(D + 1) * slots));
-    hipStream_t s = fv->stream;
-    double* cell = fv->stage_cell;
-    double* face = fv->stage_face;
-    int* list = fv->stage_list;
-    int* other = fv->stage_other;
-    int* count = fv->stage_count;
-    double* slotgeo = fv->stage_slot;
-    double side[3], inverse[3];
-    for (int a = 0; a < 3; ++a) {
-        side[a] = a < D ? m->side[a] : 0.0;
-        inverse[a] = side[a] > 0 ? 1.0 / side[a] : 0.0;
-    }
-    // The sweep's per-slot geometry: |A| as the face record holds it, and the face centroid relative to this cell's
-    // centroid, Geometry::face_offset (finite_volume.hpp:322-326): the stored centroid (relative to the owner's
-    // generator), moved to this cell's generator by the nearest-image separation when the cell does not own the face.
-    auto slot_geometry = [&](long long slot, long long i, int e) {
-        const int f = e >= 0 ? e : ~e;
-        const double* a = m->area + static_cast<size_t>(f) * D;
-        slotgeo[slot] = std::sqrt(D == 3 ? a[0] * a[0] + a[1] * a[1] + a[2] * a[2] : a[0] * a[0] + a[1] * a[1]);
-        for (int k = 0; k < D; ++k) {
-            double rel = m->face_centroid[static_cast<size_t>(f) * D + k];
-            if (e < 0) {
-                const double d = m->positions[static_cast<size_t>(m->owner[f]) * D + k] - m->positions[i * D + k];
-                rel = rel + (d - side[k] * std::round(d * inverse[k]));
-            }
-            slotgeo[(k + 1) * slots + slot] = rel - m->centroid[i * D + k];
-        }
-    };
-    auto slot_padding = [&](long long slot) {
-        for (int k = 0; k <= D; ++k) slotgeo[k * slots + slot] = 0.0;
-    };
-    parallel_for(N, [&](long long lo, long long hi) {
-        for (long long i = lo; i < hi; ++i) {
-            double* row = cell + i * luna::GEO_ROW;
-            for (int k = 0; k < luna::GEO_ROW; ++k) row[k] = 0;
-            for (int a = 0; a < D; ++a) {
-                row[a] = m->positions[i * D + a];
-                row[D + a] = m->centroid[i * D + a];
-            }
-            row[2 * D] = m->volume[i];
-            row[2 * D + 1] = D == 3 ? std::cbrt(m->volume[i] * (3.0 / (4.0 * M_PI))) : std::sqrt(m->volume[i] * (1.0 / M_PI));
-            const int first = m->start[i], n = m->start[i + 1] - first;
-            count[i] = n;
-            const long long g = i / luna::SLICE, lane = i % luna::SLICE;
-            const long long longest = (g + 1 < S ? slice[g + 1] : slots) - slice[g];
-            for (long long t = 0; t < longest / luna::SLICE; ++t) {
-                const long long slot = slice[g] + t * luna::SLICE + lane;
-                if (t < n) {
-                    const int e = m->list[first + t], f = e >= 0 ? e : ~e;
-                    list[slot] = e;
-                    other[slot] = e >= 0 ? m->neighbour[f] : m->owner[f];
-                    slot_geometry(slot, i, e);
-                } else {  // padding: a valid index that no kernel reads
-                    list[slot] = 0;
-                    other[slot] = 0;
-                    slot_padding(slot);
-                }
-            }
-        }
-    });
-    // padding lanes of the last slice (cells past N)
-    for (long long i = N; i < static_cast<long long>(S) * luna::SLICE; ++i) {
-        const long long g = i / luna::SLICE, lane = i % luna::SLICE;
-        for (long long slot = slice[g] + lane; slot < slots; slot += luna::SLICE) {
-            list[slot] = 0;
-            other[slot] = 0;
-            slot_padding(slot);
-        }
-    }
-    parallel_for(F, [&](long long lo, long long hi) {
-        for (long long f = lo; f < hi; ++f) {
-            double* row = face + f * luna::FACE_ROW;
-            for (int k = 0; k < luna::FACE_ROW; ++k) row[k] = 0;
-            const double* a = m->area + f * D;
-            for (int k = 0; k < D; ++k) {
-                row[k] = a[k];
-                row[D + k] = m->face_centroid[f * D + k];
-            }
-            row[2 * D] = std::sqrt(D == 3 ? a[0] * a[0] + a[1] * a[1] + a[2] * a[2] : a[0] * a[0] + a[1] * a[1]);
-            const int pair[2] = {m->owner[f], m->neighbour[f]};
-            std::memcpy(row + 7, pair, sizeof pair);
-        }
-    });
-    LUNA_HIP(hipMemcpyAsync(fv->count.p, count, sizeof(int) * N, hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpyAsync(fv->slice.p, slice.data(), sizeof(int) * S, hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpyAsync(fv->list.p, list, sizeof(int) * slots, hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpyAsync(fv->other.p, other, sizeof(int) * slots, hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpyAsync(fv->slotgeo.p, slotgeo, sizeof(double) * (D + 1) * slots, hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpy2DAsync(fv->cell.p, sizeof(double) * luna::CELL_ROW, cell, sizeof(double) * luna::GEO_ROW, sizeof(double) * luna::GEO_ROW,
-                              N, hipMemcpyHostToDevice, s));
-    LUNA_HIP(hipMemcpyAsync(fv->facerec.p, face, sizeof(double) * static_cast<size_t>(F) * luna::FACE_ROW, hipMemcpyHostToDevice, s));
+    LUNA_HIP(timed(fv->prof, s, "geometry_faces", [&] {
+        return D == 3 ? luna::launch_geometry_faces<3>(s, in, fv->facerec.p, fv->geo_flags.p)
+                      : luna::launch_geometry_faces<2>(s, in, fv->facerec.p, fv->geo_flags.p);
+    }));
+    LUNA_HIP(timed(fv->prof, s, "geometry_slots", [&] {
+        return D == 3 ? luna::launch_geometry_slots<3>(s, in, fv->count.p, fv->slice.p, fv->geo_info.p, fv->list.p, fv->other.p, fv->slotgeo.p,
+                                                       fv->geo_flags.p)
+                      : luna::launch_geometry_slots<2>(s, in, fv->count.p, fv->slice.p, fv->geo_info.p, fv->list.p, fv->other.p, fv->slotgeo.p,
+                                                       fv->geo_flags.p);
+    }));
+    LUNA_HIP(hipMemcpyAsync(back + 1, fv->geo_flags.p, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipStreamSynchronize(s));
+    flags = static_cast<unsigned int>(back[1]);
+    if (flags & luna::GEO_BAD_CELL) return fail(LUNA_ERR_ARGUMENT, "luna: face cell index out of range");
+    if (flags & luna::GEO_BAD_FACE) return fail(LUNA_ERR_ARGUMENT, "luna: list entry out of range");
     fv->F = F;
-    fv->E = E;
+    fv->E = 2 * F;
     fv->slots = static_cast<int>(slots);
-    ++fv->uploads;
+    fv->mesh_ok = true;
     for (int a = 0; a < 3; ++a) fv->side[a] = a < D ? m->side[a] : 0.0;
     fv->grid.step = m->step;
     fv->grid.per_step = m->step > 0 ? 1.0 / m->step : 0.0;
     for (int a = 0; a < 3; ++a) fv->grid.period[a] = a < D ? m->period[a] : 1;
-    LUNA_HIP(hipStreamSynchronize(s));  // the caller's arrays (and the staging) may be reused after this call
     return LUNA_OK;
+}
+
+luna::MeshIn mesh_in(const luna_geometry* m) {
+    luna::MeshIn in{};
+    in.N = m->cells;
+    in.F = m->faces;
+    in.owner = m->owner;
+    in.neighbour = m->neighbour;
+    in.start = m->start;
+    in.list = m->list;
+    in.area = m->area;
+    in.face_centroid = m->face_centroid;
+    in.centroid = m->centroid;
+    in.volume = m->volume;
+    in.positions = m->positions;
+    return in;
+}
+
+// A mesh in device memory (the arrays the producer left on the GPU), read in place.
+int upload_device(luna_fv* fv, const luna_geometry* m) {
+    if (int rc = check_geometry(m, fv->D, fv->N)) return rc;
+    return derive(fv, m, mesh_in(m));
+}
+
+// A mesh in host memory: its arrays are copied to the handle's device buffers, then derived as a device mesh. Host
+// checks first, which the copies' sizes rely on: the CSR offsets' range and the face count.
+int upload(luna_fv* fv, const luna_geometry* m) {
+    const int D = fv->D, N = fv->N;
+    if (int rc = check_geometry(m, D, N)) return rc;
+    const size_t F = m->faces, E = static_cast<size_t>(m->start[N]);
+    if (E != 2 * F) return fail(LUNA_ERR_ARGUMENT, "luna: every face must appear in two lists");
+    hipStream_t s = fv->stream;
+    LUNA_HIP(fv->src_owner.reserve(F > 0 ? F : 1));
+    LUNA_HIP(fv->src_neighbour.reserve(F > 0 ? F : 1));
+    LUNA_HIP(fv->src_start.reserve(static_cast<size_t>(N) + 1));
+    LUNA_HIP(fv->src_list.reserve(E > 0 ? E : 1));
+    LUNA_HIP(fv->src_area.reserve(F * D > 0 ? F * D : 1));
+    LUNA_HIP(fv->src_fc.reserve(F * D > 0 ? F * D : 1));
+    LUNA_HIP(fv->src_cells.reserve(static_cast<size_t>(N) * (2 * D + 1)));
+    double* centroid = fv->src_cells.p;
+    double* positions = centroid + static_cast<size_t>(N) * D;
+    double* volume = positions + static_cast<size_t>(N) * D;
+    LUNA_HIP(hipMemcpyAsync(fv->src_owner.p, m->owner, sizeof(int) * F, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->src_neighbour.p, m->neighbour, sizeof(int) * F, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->src_start.p, m->start, sizeof(int) * (N + 1), hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->src_list.p, m->list, sizeof(int) * E, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->src_area.p, m->area, sizeof(double) * F * D, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->src_fc.p, m->face_centroid, sizeof(double) * F * D, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(centroid, m->centroid, sizeof(double) * N * D, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(positions, m->positions, sizeof(double) * N * D, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(volume, m->volume, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    luna::MeshIn in = mesh_in(m);
+    in.owner = fv->src_owner.p;
+    in.neighbour = fv->src_neighbour.p;
+    in.start = fv->src_start.p;
+    in.list = fv->src_list.p;
+    in.area = fv->src_area.p;
+    in.face_centroid = fv->src_fc.p;
+    in.centroid = centroid;
+    in.volume = volume;
+    in.positions = positions;
+    return derive(fv, m, in);
 }
 
 int primitives(luna_fv* fv) {
@@ -474,6 +423,7 @@ int sweep(luna_fv* fv, double courant) {
 }
 
 int ensure_swept(luna_fv* fv) {
+    if (!fv->mesh_ok) return fail(LUNA_ERR_LOGIC, "luna: the last mesh given was refused; set a valid mesh first");
     if (fv->swept) return LUNA_OK;
     if (!fv->prims_ready)
         if (int rc = primitives(fv)) return rc;
@@ -506,7 +456,11 @@ extern "C" {
 
 const char* luna_last_error(void) { return g_error.c_str(); }
 
-int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_fv** out) {
+}  // extern "C"
+
+namespace {
+
+int create(const luna_fv_options* opt, const luna_geometry* mesh, luna_fv** out, bool device) {
     if (!opt || !mesh || !out) return fail(LUNA_ERR_ARGUMENT, "luna_fv_create: null argument");
     *out = nullptr;
     if (opt->dim != 2 && opt->dim != 3) return fail(LUNA_ERR_ARGUMENT, "hydro::FiniteVolume: D = 2 or 3 (1D: luna_fv1_create)");
@@ -544,7 +498,8 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
     if ((e = fv->q.reserve(N * V)) != hipSuccess || (e = fv->count.reserve(N)) != hipSuccess ||
         (e = fv->grad.reserve(N * fv->grad_row())) != hipSuccess || (e = fv->w.reserve(N * D)) != hipSuccess ||
         (e = fv->cell.reserve(N * luna::CELL_ROW)) != hipSuccess ||
-        (e = fv->counters.reserve(4)) != hipSuccess)
+        (e = fv->counters.reserve(4)) != hipSuccess || (e = fv->geo_info.reserve(1)) != hipSuccess ||
+        (e = fv->geo_flags.reserve(1)) != hipSuccess)
         return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
     if (opt->limiter == LUNA_LIMITER_LEARNED) {
         if ((e = fv->features.reserve(N * V * luna::FEATURES)) != hipSuccess || (e = fv->troubled.reserve(N)) != hipSuccess ||
@@ -560,7 +515,7 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
             (e = fv->steps.reserve(N * D)) != hipSuccess)
             return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
     }
-    if (hipHostMalloc(reinterpret_cast<void**>(&fv->host), 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc(reinterpret_cast<void**>(&fv->host), 8 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(LUNA_ERR_MEMORY, "luna: pinned host allocation failed"));
     const unsigned long long init[4] = {luna::INF_BITS, luna::INF_BITS, 0, 0};
     if (hipMemcpy(fv->counters.p, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess)
@@ -568,9 +523,9 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
     std::vector<double> zero(N * V, 0.0);
     if (hipMemcpy(fv->q.p, zero.data(), sizeof(double) * N * V, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(LUNA_ERR_DEVICE, "luna: state initialisation failed"));
-    if (int rc = upload(fv, mesh)) return cleanup(rc);
+    if (int rc = device ? upload_device(fv, mesh) : upload(fv, mesh)) return cleanup(rc);
     if (opt->motion == LUNA_MOTION_MOVING) {  // copy_positions (finite_volume.hpp:342-349)
-        if (hipMemcpy(fv->x.p, mesh->positions, sizeof(double) * N * D, hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemcpy(fv->x.p, mesh->positions, sizeof(double) * N * D, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(LUNA_ERR_DEVICE, "luna: position upload failed"));
         std::vector<double> zw(N * D, 0.0);
         if (hipMemcpy(fv->w.p, zw.data(), sizeof(double) * N * D, hipMemcpyHostToDevice) != hipSuccess)
@@ -579,6 +534,14 @@ int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_f
     *out = fv;
     return LUNA_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+int luna_fv_create(const luna_fv_options* opt, const luna_geometry* mesh, luna_fv** out) { return create(opt, mesh, out, false); }
+
+int luna_fv_create_device(const luna_fv_options* opt, const luna_geometry* mesh, luna_fv** out) { return create(opt, mesh, out, true); }
 
 int luna_fv_destroy(luna_fv* fv) {
     if (!fv) return LUNA_OK;
@@ -589,9 +552,18 @@ int luna_fv_destroy(luna_fv* fv) {
     fv->exponents.release();
     fv->hll.release();
     for (auto* b : {&fv->d_dq, &fv->d_dW, &fv->d_dgrad, &fv->d_dflux, &fv->d_sidebar, &fv->d_bbar, &fv->d_boundbar, &fv->d_out}) b->release();
-    for (void* p : {static_cast<void*>(fv->stage_cell), static_cast<void*>(fv->stage_face), static_cast<void*>(fv->stage_list),
-                    static_cast<void*>(fv->stage_other), static_cast<void*>(fv->stage_count), static_cast<void*>(fv->stage_slot)})
-        if (p) (void)hipHostFree(p);
+    for (auto* b : {&fv->m_row, &fv->m_Ab, &fv->m_Xfb, &fv->m_Wbar, &fv->m_gbar, &fv->m_whbar, &fv->m_c0bar, &fv->m_cbar, &fv->m_xbar,
+                    &fv->m_beta, &fv->m_boundbar, &fv->m_entry, &fv->m_angle, &fv->m_Vbar, &fv->m_out, &fv->m_qbar})
+        b->release();
+    fv->m_extreme.release();
+    fv->m_fmax.release();
+    for (auto* b : {&fv->t_dx, &fv->t_dx1, &fv->t_dV, &fv->t_dc, &fv->t_dc0, &fv->t_dA, &fv->t_dxf, &fv->t_dW, &fv->t_dcs, &fv->t_dw, &fv->t_dg,
+                    &fv->t_dG, &fv->t_dq})
+        b->release();
+    for (auto* b : {&fv->src_owner, &fv->src_neighbour, &fv->src_start, &fv->src_list}) b->release();
+    for (auto* b : {&fv->src_area, &fv->src_fc, &fv->src_cells}) b->release();
+    fv->geo_info.release();
+    fv->geo_flags.release();
     fv->counters.release();
     fv->features.release();
     fv->troubled.release();
@@ -629,6 +601,7 @@ int luna_fv_state_device(luna_fv* fv, double** q) {
 int luna_fv_time_step(luna_fv* fv, double courant, double* dt) {
     if (!fv || !dt) return fail(LUNA_ERR_ARGUMENT, "luna_fv_time_step: null argument");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_time_step: a moving step is between begin and finish");
+    if (!fv->mesh_ok) return fail(LUNA_ERR_LOGIC, "luna: the last mesh given was refused; set a valid mesh first");
     if (!fv->prims_ready)
         if (int rc = primitives(fv)) return rc;
     fv->prims_ready = false;
@@ -663,6 +636,7 @@ int derivatives_ready(luna_fv* fv, const char* what) {
     if (fv->opt.precision != LUNA_PRECISION_DOUBLE) return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of the double policy");
     if (fv->opt.hll_compression > 0) return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives without the HLL fallback");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, std::string(what) + ": a moving step is between begin and finish");
+    if (!fv->mesh_ok) return fail(LUNA_ERR_LOGIC, "luna: the last mesh given was refused; set a valid mesh first");
     const size_t N = fv->N, V = fv->V, F = fv->F;
     LUNA_HIP(fv->d_dq.reserve(N * V));
     LUNA_HIP(fv->d_dW.reserve(N * V));
@@ -751,11 +725,6 @@ int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, c
     const size_t ND = static_cast<size_t>(N) * D, NV = static_cast<size_t>(N) * V;
     hipStream_t s = fv->stream;
     const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID;
-    struct KeepCount {  // the reverse's own uploads are not mesh updates
-        long long& count;
-        long long value;
-        ~KeepCount() { count = value; }
-    } keep{fv->uploads, fv->uploads};
     const int row_width = D == 3 ? 2 * (5 + 15 + 3) + 9 : 2 * (4 + 8 + 2) + 6;
     // t^n: its mesh and positions, W, the limited gradients and point velocities (the sweep keeps c^n), the grid steps.
     if (int rc = luna_fv_set_geometry(fv, mesh0)) return rc;  // and x^n
@@ -881,6 +850,102 @@ int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, c
     return LUNA_OK;
 }
 
+int luna_fv_moving_tangent(luna_fv* fv, double dt, const luna_geometry* mesh0, const luna_geometry* mesh1, luna_geometry_tangent_fn tangent0,
+                           const void* ctx0, luna_geometry_tangent_fn tangent1, const void* ctx1, int steps_fixed, double* dq, double* dx) {
+    if (!fv || !mesh0 || !mesh1 || !tangent0 || !tangent1 || !dq || !dx) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_tangent: null argument");
+    if (fv->opt.motion != LUNA_MOTION_MOVING) return fail(LUNA_ERR_LOGIC, "hydro::FiniteVolume::step_moving_tangent: a moving mesh");
+    if (fv->opt.limiter == LUNA_LIMITER_LEARNED)
+        return fail(LUNA_ERR_LOGIC, "luna_fv_moving_tangent: derivatives of the face-centroid limiter or none (not the learned one)");
+    if (fv->opt.precision != LUNA_PRECISION_DOUBLE) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_tangent: derivatives of the double policy");
+    if (fv->opt.hll_compression > 0) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_tangent: derivatives without the HLL fallback");
+    if (fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_tangent: a moving step is between begin and finish");
+    if (!(dt > 0)) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_tangent: dt must be positive");
+    if (mesh0->cells != fv->N || mesh1->cells != fv->N) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_tangent: both meshes must have the handle's cells");
+    const int D = fv->D, V = fv->V, N = fv->N;
+    const size_t ND = static_cast<size_t>(N) * D, NV = static_cast<size_t>(N) * V;
+    hipStream_t s = fv->stream;
+    const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID, order0 = fv->opt.order == 0;
+    // t^n: its mesh and positions, W, the limited gradients and point velocities, c^n and the grid steps
+    if (int rc = luna_fv_set_geometry(fv, mesh0)) return rc;
+    if (int rc = primitives(fv)) return rc;
+    if (int rc = sweep(fv, 1.0)) return rc;
+    if (!order0) {
+        const luna::Geo g = fv->geo();
+        const luna::Rows r = fv->rows();
+        LUNA_HIP(D == 3 ? luna::launch_keep_centroids<3>(s, g, r, fv->grid, dt) : luna::launch_keep_centroids<2>(s, g, r, fv->grid, dt));
+    }
+    const double unit = fv->grid.step / dt;
+    const size_t Fmax = static_cast<size_t>(std::max(std::max(mesh0->faces, mesh1->faces), 1));
+    for (Buf<double>* b : {&fv->t_dx, &fv->t_dx1, &fv->t_dc, &fv->t_dc0, &fv->t_dw}) LUNA_HIP(b->reserve(ND));
+    LUNA_HIP(fv->t_dV.reserve(N));
+    LUNA_HIP(fv->t_dcs.reserve(N));
+    LUNA_HIP(fv->t_dA.reserve(Fmax * D));
+    LUNA_HIP(fv->t_dxf.reserve(Fmax * D));
+    LUNA_HIP(fv->t_dW.reserve(NV));
+    LUNA_HIP(fv->t_dg.reserve(NV * D));
+    LUNA_HIP(fv->t_dG.reserve(Fmax * V));
+    LUNA_HIP(fv->t_dq.reserve(NV));
+    // the geometry's tangent of the mesh of t^n (the mesh library's, on the host)
+    const size_t F0 = static_cast<size_t>(mesh0->faces), F1 = static_cast<size_t>(mesh1->faces);
+    std::vector<double> hV(N), hc(ND), hA(Fmax * D), hxf(Fmax * D), hx1(ND);
+    if (tangent0(ctx0, dx, hV.data(), hc.data(), hA.data(), hxf.data()) != 0)
+        return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_tangent: the geometry tangent of the mesh of t^n failed");
+    LUNA_HIP(hipMemcpyAsync(fv->t_dq.p, dq, sizeof(double) * NV, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->t_dx.p, dx, sizeof(double) * ND, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->t_dV.p, hV.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->t_dc.p, hc.data(), sizeof(double) * ND, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->t_dA.p, hA.data(), sizeof(double) * F0 * D, hipMemcpyHostToDevice, s));
+    LUNA_HIP(hipMemcpyAsync(fv->t_dxf.p, hxf.data(), sizeof(double) * F0 * D, hipMemcpyHostToDevice, s));
+    luna::MovingDots t{fv->t_dx.p, fv->t_dV.p, fv->t_dc.p, fv->t_dc0.p, fv->t_dA.p, fv->t_dxf.p, fv->t_dW.p, fv->t_dcs.p, fv->t_dw.p, fv->t_dg.p,
+                       fv->t_dG.p};
+    const luna::Gas gas = fv->gas();
+    {
+        const luna::Geo g = fv->geo();
+        const luna::Rows r = fv->rows();
+        LUNA_HIP(D == 3 ? luna::launch_moving_tangent_cells<3>(s, g, r, gas, limit, !order0, fv->t_dq.p, t)
+                        : luna::launch_moving_tangent_cells<2>(s, g, r, gas, limit, !order0, fv->t_dq.p, t));
+        if (!order0 && steps_fixed) LUNA_HIP(hipMemsetAsync(fv->t_dw.p, 0, sizeof(double) * ND, s));  // no derivative through w-hat
+        // pass 1 on t^n (order 0: the one pass over dt with the sweep's w)
+        LUNA_HIP(D == 3 ? luna::launch_moving_pass_tangent<3>(s, g, r, gas, fv->opt.riemann, dt, unit, false, order0, t, fv->t_dq.p)
+                        : luna::launch_moving_pass_tangent<2>(s, g, r, gas, fv->opt.riemann, dt, unit, false, order0, t, fv->t_dq.p));
+    }
+    // x^n+1 = x^n + dt w-hat (order 0: + dt w, off the grid)
+    LUNA_HIP(luna::launch_axpy(s, static_cast<long long>(ND), dt, fv->t_dx.p, fv->t_dw.p, fv->t_dx1.p));
+    LUNA_HIP(hipMemcpyAsync(hx1.data(), fv->t_dx1.p, sizeof(double) * ND, hipMemcpyDeviceToHost, s));
+    if (!order0) {
+        LUNA_HIP(hipMemcpyAsync(fv->t_dc0.p, fv->t_dc.p, sizeof(double) * ND, hipMemcpyDeviceToDevice, s));
+        LUNA_HIP(hipStreamSynchronize(s));
+        // pass 2 on t^n+1 with its geometry's tangent along dx^n+1 (W^n, g^n, c^n and the steps stay)
+        if (tangent1(ctx1, hx1.data(), hV.data(), hc.data(), hA.data(), hxf.data()) != 0)
+            return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_tangent: the geometry tangent of the mesh of t^n+1 failed");
+        LUNA_HIP(hipMemcpyAsync(fv->t_dV.p, hV.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+        LUNA_HIP(hipMemcpyAsync(fv->t_dc.p, hc.data(), sizeof(double) * ND, hipMemcpyHostToDevice, s));
+        LUNA_HIP(hipMemcpyAsync(fv->t_dA.p, hA.data(), sizeof(double) * F1 * D, hipMemcpyHostToDevice, s));
+        LUNA_HIP(hipMemcpyAsync(fv->t_dxf.p, hxf.data(), sizeof(double) * F1 * D, hipMemcpyHostToDevice, s));
+        fv->begun = true;  // keeps W, g across the upload
+        int rc = upload(fv, mesh1);
+        fv->begun = false;
+        if (rc != LUNA_OK) return rc;
+        t.dx = fv->t_dx1.p;
+        const luna::Geo g = fv->geo();
+        const luna::Rows r = fv->rows();
+        LUNA_HIP(D == 3 ? luna::launch_moving_pass_tangent<3>(s, g, r, gas, fv->opt.riemann, dt, unit, true, false, t, fv->t_dq.p)
+                        : luna::launch_moving_pass_tangent<2>(s, g, r, gas, fv->opt.riemann, dt, unit, true, false, t, fv->t_dq.p));
+    }
+    LUNA_HIP(hipMemcpyAsync(dq, fv->t_dq.p, sizeof(double) * NV, hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipStreamSynchronize(s));
+    std::memcpy(dx, hx1.data(), sizeof(double) * ND);
+    if (!order0) {  // back on the mesh of t^n
+        fv->begun = true;
+        int rc = upload(fv, mesh0);
+        fv->begun = false;
+        if (rc != LUNA_OK) return rc;
+    }
+    fv->prims_ready = false;
+    fv->swept = false;
+    return LUNA_OK;
+}
+
 int luna_fv_moving_begin(luna_fv* fv, double dt, double* positions) {
     if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_begin: null argument");
     if (fv->opt.motion != LUNA_MOTION_MOVING) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_begin: a static mesh steps with luna_fv_step");
@@ -915,27 +980,44 @@ int luna_fv_moving_begin(luna_fv* fv, double dt, double* positions) {
     return LUNA_OK;
 }
 
-int luna_fv_set_geometry(luna_fv* fv, const luna_geometry* mesh) {
+}  // extern "C"
+
+namespace {
+
+int set_geometry(luna_fv* fv, const luna_geometry* mesh, bool device) {
     if (!fv || !mesh) return fail(LUNA_ERR_ARGUMENT, "luna_fv_set_geometry: null argument");
-    if (int rc = upload(fv, mesh)) return rc;
+    if (fv->begun) fv->new_mesh = false;
+    if (int rc = device ? upload_device(fv, mesh) : upload(fv, mesh)) return rc;
     if (fv->begun) {
         fv->new_mesh = true;
+        ++fv->updates;
     } else {  // mesh_moved() (finite_volume.hpp:152-156): the cached rows go; a moving solver takes the mesh's positions
         fv->prims_ready = false;
         fv->swept = false;
         if (fv->opt.motion == LUNA_MOTION_MOVING) {
-            LUNA_HIP(hipMemcpyAsync(fv->x.p, mesh->positions, sizeof(double) * fv->N * fv->D, hipMemcpyHostToDevice, fv->stream));
+            LUNA_HIP(hipMemcpyAsync(fv->x.p, mesh->positions, sizeof(double) * fv->N * fv->D,
+                                    device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, fv->stream));
             LUNA_HIP(hipStreamSynchronize(fv->stream));
         }
     }
     return LUNA_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int luna_fv_set_geometry(luna_fv* fv, const luna_geometry* mesh) { return set_geometry(fv, mesh, false); }
+
+int luna_fv_set_geometry_device(luna_fv* fv, const luna_geometry* mesh) { return set_geometry(fv, mesh, true); }
+
 int luna_fv_moving_finish(luna_fv* fv, const luna_geometry* mesh, double dt) {
     if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_finish: null argument");
     if (!fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_finish: no step was begun");
     if (mesh) {
+        fv->new_mesh = false;
         if (int rc = upload(fv, mesh)) return rc;
+        ++fv->updates;
     } else if (!fv->new_mesh) {
         return fail(LUNA_ERR_LOGIC, "luna_fv_moving_finish: no mesh of the new positions (pass one or call luna_fv_set_geometry)");
     }
@@ -1063,7 +1145,7 @@ int luna_fv_release_mesh(luna_fv* fv) {
 
 int luna_fv_mesh_stats(luna_fv* fv, long long* updates) {
     if (!fv || !updates) return fail(LUNA_ERR_ARGUMENT, "luna_fv_mesh_stats: null argument");
-    *updates = fv->uploads > 0 ? fv->uploads - 1 : 0;
+    *updates = fv->updates;
     return LUNA_OK;
 }
 
@@ -1346,7 +1428,7 @@ int luna_fv1_create(const luna_fv1_options* opt, const luna_line* line, luna_fv1
     if (fv->keys.reserve(n) != hipSuccess || fv->q.reserve(3 * n) != hipSuccess || fv->next.reserve(3 * n) != hipSuccess ||
         fv->counters.reserve(4) != hipSuccess)
         return cleanup(fail(LUNA_ERR_MEMORY, "luna: device allocation failed"));
-    if (hipHostMalloc(reinterpret_cast<void**>(&fv->host), 4 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc(reinterpret_cast<void**>(&fv->host), 8 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(LUNA_ERR_MEMORY, "luna: pinned host allocation failed"));
     const unsigned long long init[4] = {luna::INF_BITS, luna::INF_BITS, 0, 0};
     std::vector<double> zero(3 * n, 0.0);

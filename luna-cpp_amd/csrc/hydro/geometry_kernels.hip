@@ -1,0 +1,230 @@
+// The mesh, from the mesh producer's arrays to the rows the step's kernels read (kernels.hpp), on the device. The input
+// is the voronoi-cpp output contract as finite_volume.hpp:257-327 and :353-364 read it, in device memory: owner and
+// neighbour per face, area vectors and face centroids (relative to the owner's generator) per face, the CSR lists with
+// the owner bit, and per cell the centroid (relative to its generator), volume and position. The reference reads those
+// views in place inside its timed kernels (the face offset in its sweep, :322-326; the radius in primitives(), :413);
+// here one pass per mesh derives what they derive and packs it:
+//   k_geo_cells   per cell: the list length and the geometry half of the cell row [x, c, V, r]; per 64-cell slice (one
+//                 work-group) the longest list; checks of the CSR offsets;
+//   k_geo_scan    the slices' first slots (exclusive sum of 64 x longest), one work-group;
+//   k_geo_faces   per face: the record [A, fc, |A|, (owner, neighbour)]; checks of the cell indices;
+//   k_geo_slots   per cell and list entry, in the slices' layout: the entry, the other cell, |A| and the face offset
+//                 x_f - c_i; padding slots zero; checks of the face indices.
+// Every read is guarded by the checks before it, so a malformed mesh sets a flag (read by the host) and reads nothing out
+// of range. The arithmetic is that of the host restatement (the same products, sums and rounding), so the rows are equal
+// to the bit whichever side derives them.
+#include "kernels.hpp"
+
+namespace luna {
+namespace {
+
+constexpr int BLOCK = 256;
+
+__device__ __forceinline__ void flag(unsigned int* flags, unsigned int bit) { atomicOr(flags, bit); }
+
+template <int D>
+__global__ __launch_bounds__(SLICE) void k_geo_cells(MeshIn m, double* cell, int* count, int* longest, unsigned int* flags) {
+    using C = CellAt<D>;
+    const int i = blockIdx.x * SLICE + threadIdx.x;
+    const long long E = 2LL * m.F;
+    int n = 0;
+    if (i < m.N) {
+        const int first = m.start[i], last = m.start[i + 1];
+        if (first < 0 || last < first || last > E || (i == 0 && first != 0) || (i == m.N - 1 && last != E)) {
+            flag(flags, GEO_BAD_START);
+        } else {
+            n = last - first;
+        }
+        count[i] = n;
+        double row[GEO_ROW];
+#pragma unroll
+        for (int k = 0; k < GEO_ROW; ++k) row[k] = 0.0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            row[C::X + a] = m.positions[(size_t)i * D + a];
+            row[C::C + a] = m.centroid[(size_t)i * D + a];
+        }
+        const double v = m.volume[i];
+        row[C::VOL] = v;
+        row[C::RAD] = cell_radius<D>(v);
+        double* out = cell + (size_t)i * CELL_ROW;
+#pragma unroll
+        for (int k = 0; k < GEO_ROW; ++k) out[k] = row[k];
+    }
+    // the slice's longest list (one wavefront per work-group)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) n = max(n, __shfl_xor(n, o));
+    if (threadIdx.x == 0) longest[blockIdx.x] = n;
+}
+
+// In place: slice[g] = 64 sum_{h < g} longest[h]; info[0] = the total slots. One work-group of SCAN threads, each taking a
+// contiguous run of slices.
+constexpr int SCAN = 1024;
+__global__ __launch_bounds__(SCAN) void k_geo_scan(int* slice, int S, long long* info, unsigned int* flags) {
+    __shared__ long long part[SCAN];
+    const int t = threadIdx.x;
+    const int per = (S + SCAN - 1) / SCAN;
+    const int lo = min(S, t * per), hi = min(S, lo + per);
+    long long sum = 0;
+    for (int g = lo; g < hi; ++g) sum += (long long)slice[g] * SLICE;
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < SCAN; o <<= 1) {  // inclusive scan of the runs' sums (Hillis-Steele)
+        const long long add = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    long long run = part[t] - sum;
+    for (int g = lo; g < hi; ++g) {
+        const long long here = (long long)slice[g] * SLICE;
+        slice[g] = run > 0x7fffffffLL ? 0 : (int)run;
+        run += here;
+    }
+    if (t == SCAN - 1) {
+        info[0] = part[t];
+        if (part[t] > 0x7fffffffLL) flag(flags, GEO_TOO_MANY_SLOTS);
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(BLOCK) void k_geo_faces(MeshIn m, double* face, unsigned int* flags) {
+    using Fa = FaceAt<D>;
+    const int f = blockIdx.x * BLOCK + threadIdx.x;
+    if (f >= m.F) return;
+    double row[FACE_ROW];
+#pragma unroll
+    for (int k = 0; k < FACE_ROW; ++k) row[k] = 0.0;
+    double a[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        a[k] = m.area[(size_t)f * D + k];
+        row[Fa::A + k] = a[k];
+        row[Fa::FC + k] = m.face_centroid[(size_t)f * D + k];
+    }
+    row[Fa::MAG] = sqrt(dot<D>(a, a));
+    const int o = m.owner[f], j = m.neighbour[f];
+    if (o < 0 || o >= m.N || j < 0 || j >= m.N) flag(flags, GEO_BAD_CELL);
+    int pair[2] = {o, j};
+    __builtin_memcpy(&row[Fa::PAIR], pair, sizeof pair);
+    double* out = face + (size_t)f * FACE_ROW;
+#pragma unroll
+    for (int k = 0; k < FACE_ROW; ++k) out[k] = row[k];
+}
+
+// One work-group of SLOT_WAVES wavefronts per slice: lane = cell i mod 64, wavefront w takes the entries t = w, w + SLOT_WAVES,
+// ... of the slice's cells, so each work-item walks a few entries and the loads of different entries are in flight
+// together; entry t of cell i goes to slot slice[g] + 64 t + lane (consecutive lanes: consecutive slots).
+constexpr int SLOT_WAVES = 4;
+template <int D>
+__global__ __launch_bounds__(SLICE* SLOT_WAVES) void k_geo_slots(MeshIn m, const int* __restrict__ count, const int* __restrict__ slice,
+                                                                  const long long* __restrict__ info, int S, int* __restrict__ list,
+                                                                  int* __restrict__ other, double* __restrict__ slot, unsigned int* flags) {
+    const int g = blockIdx.x, lane = threadIdx.x & (SLICE - 1), wave = threadIdx.x / SLICE;
+    const int i = g * SLICE + lane;
+    const long long slots = info[0];
+    const long long base = slice[g];
+    const long long end = g + 1 < S ? (long long)slice[g + 1] : slots;
+    const int depth = (int)((end - base) / SLICE);
+    const int n = i < m.N ? count[i] : 0;
+    const int first = n > 0 ? m.start[i] : 0;
+    const int* __restrict__ in_list = m.list;
+    const int* __restrict__ owner = m.owner;
+    const int* __restrict__ neighbour = m.neighbour;
+    const double* __restrict__ area = m.area;
+    const double* __restrict__ fc = m.face_centroid;
+    const double* __restrict__ pos = m.positions;
+    double xi[D], ci[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        xi[a] = i < m.N ? pos[(size_t)i * D + a] : 0.0;
+        ci[a] = i < m.N ? m.centroid[(size_t)i * D + a] : 0.0;
+    }
+    unsigned int bad = 0;
+    for (int t = wave; t < depth; t += SLOT_WAVES) {
+        const long long s = base + (long long)t * SLICE + lane;
+        int e = 0, cell = 0;
+        double mag = 0.0, off[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) off[a] = 0.0;
+        if (t < n) {
+            e = in_list[first + t];
+            const int f = e >= 0 ? e : ~e;
+            if (f >= m.F) {
+                bad |= GEO_BAD_FACE;
+                e = 0;
+            } else {
+                const int o = owner[f], j = neighbour[f];
+                if (o < 0 || o >= m.N || j < 0 || j >= m.N) {
+                    bad |= GEO_BAD_CELL;
+                } else {
+                    cell = e >= 0 ? j : o;
+                    double A[D], rel[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) {
+                        A[a] = area[(size_t)f * D + a];
+                        rel[a] = fc[(size_t)f * D + a];
+                    }
+                    mag = sqrt(dot<D>(A, A));
+                    // Geometry::face_offset (finite_volume.hpp:322-326): the centroid moved to this cell's generator by
+                    // the nearest-image separation when the cell does not own the face, then relative to its centroid
+                    if (e < 0) {
+#pragma unroll
+                        for (int a = 0; a < D; ++a) {
+                            const double d = pos[(size_t)o * D + a] - xi[a];
+                            rel[a] = rel[a] + (d - m.side[a] * round(d * m.inverse[a]));
+                        }
+                    }
+#pragma unroll
+                    for (int a = 0; a < D; ++a) off[a] = rel[a] - ci[a];
+                }
+            }
+        }
+        list[s] = e;
+        other[s] = cell;
+        slot[s] = mag;
+#pragma unroll
+        for (int a = 0; a < D; ++a) slot[(a + 1) * slots + s] = off[a];
+    }
+    if (bad) flag(flags, bad);
+}
+
+}  // namespace
+
+template <int D>
+hipError_t launch_geometry_cells(hipStream_t s, const MeshIn& m, double* cell, int* count, int* slice, unsigned int* flags) {
+    const int S = (m.N + SLICE - 1) / SLICE;
+    if (S == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_geo_cells<D>, dim3(S), dim3(SLICE), 0, s, m, cell, count, slice, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_geometry_scan(hipStream_t s, int* slice, int S, long long* info, unsigned int* flags) {
+    hipLaunchKernelGGL(k_geo_scan, dim3(1), dim3(SCAN), 0, s, slice, S, info, flags);
+    return hipGetLastError();
+}
+
+template <int D>
+hipError_t launch_geometry_faces(hipStream_t s, const MeshIn& m, double* face, unsigned int* flags) {
+    if (m.F == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_geo_faces<D>, dim3((m.F + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, m, face, flags);
+    return hipGetLastError();
+}
+
+template <int D>
+hipError_t launch_geometry_slots(hipStream_t s, const MeshIn& m, const int* count, const int* slice, const long long* info, int* list, int* other,
+                                 double* slot, unsigned int* flags) {
+    const int S = (m.N + SLICE - 1) / SLICE;
+    if (S == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_geo_slots<D>, dim3(S), dim3(SLICE * SLOT_WAVES), 0, s, m, count, slice, info, S, list, other, slot, flags);
+    return hipGetLastError();
+}
+
+template hipError_t launch_geometry_cells<2>(hipStream_t, const MeshIn&, double*, int*, int*, unsigned int*);
+template hipError_t launch_geometry_cells<3>(hipStream_t, const MeshIn&, double*, int*, int*, unsigned int*);
+template hipError_t launch_geometry_faces<2>(hipStream_t, const MeshIn&, double*, unsigned int*);
+template hipError_t launch_geometry_faces<3>(hipStream_t, const MeshIn&, double*, unsigned int*);
+template hipError_t launch_geometry_slots<2>(hipStream_t, const MeshIn&, const int*, const int*, const long long*, int*, int*, double*, unsigned int*);
+template hipError_t launch_geometry_slots<3>(hipStream_t, const MeshIn&, const int*, const int*, const long long*, int*, int*, double*, unsigned int*);
+
+}  // namespace luna

@@ -51,6 +51,24 @@ struct Geo {
     double side[3], inverse[3];  // the periodic box and 1 / side
 };
 
+// The mesh producer's arrays as the reference reads them (device pointers; the voronoi-cpp output contract,
+// finite_volume.hpp:257-327): the input of the geometry pass that derives Geo's rows (geometry_kernels.hip).
+struct MeshIn {
+    int N, F;
+    const int* owner;
+    const int* neighbour;
+    const int* start;          // N + 1
+    const int* list;           // 2 F entries, f or ~f
+    const double* area;        // F x D
+    const double* face_centroid;  // F x D, relative to the owner's generator
+    const double* centroid;    // N x D, relative to the cell's generator
+    const double* volume;      // N
+    const double* positions;   // N x D
+    double side[3], inverse[3];
+};
+// Flags the geometry pass raises on a malformed mesh (it then reads nothing out of range and the host refuses the mesh).
+constexpr unsigned int GEO_BAD_START = 1, GEO_BAD_CELL = 2, GEO_BAD_FACE = 4, GEO_TOO_MANY_SLOTS = 8;
+
 // Per-cell rows of the step (device pointers).
 struct Rows {
     double* q;             // N x V, extensive
@@ -111,6 +129,18 @@ hipError_t launch_keep_centroids(hipStream_t s, const Geo& g, const Rows& r, con
 template <int D>
 hipError_t launch_drift(hipStream_t s, const Geo& g, const Rows& r, double dt);
 
+// The geometry pass (geometry_kernels.hip), in this order on one stream: cells (the geometry half of the cell rows, the
+// list lengths, each slice's longest list into slice), scan (slice's first slots, the total into info[0]); the host then
+// checks the flags and sizes the slot arrays; faces (records) and slots (lists, other cells, |A| and face offsets).
+template <int D>
+hipError_t launch_geometry_cells(hipStream_t s, const MeshIn& m, double* cell, int* count, int* slice, unsigned int* flags);
+hipError_t launch_geometry_scan(hipStream_t s, int* slice, int S, long long* info, unsigned int* flags);
+template <int D>
+hipError_t launch_geometry_faces(hipStream_t s, const MeshIn& m, double* face, unsigned int* flags);
+template <int D>
+hipError_t launch_geometry_slots(hipStream_t s, const MeshIn& m, const int* count, const int* slice, const long long* info, int* list, int* other,
+                                 double* slot, unsigned int* flags);
+
 // Derivatives of the static order-1 step (HLLC or HLL) at fixed dt (adjoint_kernels.hip). Device rows:
 //   dq       N x V   in: the tangent of Q (tangent) or Q bar after the step (adjoint); the tangent advances it to J dq
 //   dW       N x V   W tangents (tangent) or W bars (adjoint)
@@ -158,6 +188,24 @@ hipError_t launch_drift_adjoint(hipStream_t s, long long n, double dt, const dou
 template <int D>
 hipError_t launch_moving_cell_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, bool limit, bool steps_fixed, const double* qbar,
                                       const MovingBars& b);
+
+// The moving step's tangent (order 0 and 1, fixed topology): device rows. Per cell: dx (positions), dV, dc (the pass's mesh's
+// geometry tangents), dc0 (c^n's: the first mesh's dc), dW (N x V), dcs (N), dw (point velocities, zero with the steps held),
+// dg (N x V D limited gradients); per face of the pass's mesh: dA, dxf, and the dG rows.
+struct MovingDots {
+    double *dx, *dV, *dc, *dc0, *dA, *dxf, *dW, *dcs, *dw, *dg, *dG;
+};
+// On t^n: W and c_s tangents from dq and dV, the point velocities' and (fit) the limited gradients' tangents.
+template <int D>
+hipError_t launch_moving_tangent_cells(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, bool limit, bool fit, const double* dq,
+                                       const MovingDots& t);
+// One pass on the mesh the Geo holds (second: t^n+1, with t.dc0): the faces' dG, then dq -= dt/2 sum +-dG per cell in list
+// order (order0: the order-0 step's one pass with the sweep's w and no gradients, dq -= dt sum).
+template <int D>
+hipError_t launch_moving_pass_tangent(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, double unit, bool second,
+                                      bool order0, const MovingDots& t, double* dq);
+// out = x + dt y (n entries).
+hipError_t launch_axpy(hipStream_t s, long long n, double dt, const double* x, const double* y, double* out);
 
 // D = 1: the sorted line. Slots in key order; Q (and on a moving line the positions) by cell. Walls: 0 reflective,
 // 1 outflow, 2 inflow (a moving line's open wall holding the state inflow[side]).

@@ -59,6 +59,43 @@ __device__ __forceinline__ double face_angle(double area) {
         return 0.5 * area;
 }
 
+// The cube root of v > 0 rounded to the nearest double. The library's cbrt is within an ulp of it; the result is then
+// corrected by the sign of (y +- half an ulp)^3 - v: y^2 and y^3 as double-double products (fma gives a product's
+// rounding error exactly), y^3 - v exact by Sterbenz, the half-ulp terms 3 y^2 h + 3 y h^2 + h^3 added after. A tie
+// cannot occur (a 54-bit midpoint cubed is no double); only |(y + h)^3 - v| below about 2^-100 v could be misjudged.
+// The same function, written the same way in C, gives the oracle's radius, so both sides take the same r.
+__device__ __forceinline__ double cube_excess(double y, double h, double v) {
+    const double p2 = y * y, e2 = fma(y, y, -p2);
+    const double p3 = p2 * y, e3 = fma(p2, y, -p3);
+    const double r0 = p3 - v;
+    const double big = r0 + 3.0 * (p2 * h);
+    return big + (e3 + e2 * y) + 3.0 * (e2 * h) + 3.0 * (y * h * h) + h * h * h;
+}
+__device__ __forceinline__ double cbrt_rn(double v) {
+    double y = cbrt(v);
+    if (!(v > 0) || !(v < INFINITY)) return y;
+    for (int k = 0; k < 4; ++k) {
+        const double up = nextafter(y, INFINITY), down = nextafter(y, 0.0);
+        if (cube_excess(y, 0.5 * (up - y), v) < 0) {
+            y = up;
+        } else if (cube_excess(y, -0.5 * (y - down), v) > 0) {
+            y = down;
+        } else {
+            break;
+        }
+    }
+    return y;
+}
+
+// finite_volume.hpp:241-246 radius(): cbrt(3V / 4 pi) in 3D, sqrt(V / pi) in 2D.
+template <int D>
+__device__ __forceinline__ double cell_radius(double v) {
+    if constexpr (D == 3)
+        return cbrt_rn(v * (3.0 / (4.0 * M_PI)));
+    else
+        return sqrt(v * (1.0 / M_PI));
+}
+
 // euler.hpp:186-214 half_step_rate() and half_step(): W - dt/2 sum_d A_d(W) dW/dx_d. g: [V][D].
 template <int D>
 __device__ __forceinline__ void half_step(const Gas& gas, const double* W, const double* g, double dt, double* out) {

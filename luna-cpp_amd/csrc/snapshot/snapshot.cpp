@@ -122,7 +122,10 @@ int luna_snapshot_write(const char* dir, const luna_snapshot* s, char* name, siz
              attribute(cells.id, "Ly", s->sides[1]) && attribute(cells.id, "Lz", s->sides[2]) &&
              attribute(cells.id, "mesh", std::string("voronoi")) && attribute(cells.id, "N_cells", static_cast<long long>(n)) &&
              attribute(cells.id, "luna_cpp_commit", std::string(s->commit ? s->commit : "")) &&
-             attribute(cells.id, "luna_cpp_dirty", static_cast<long long>(s->dirty ? 1 : 0));
+             attribute(cells.id, "luna_cpp_dirty", static_cast<long long>(s->dirty ? 1 : 0)) &&
+             attribute(cells.id, "voronoi_cpp_commit", std::string(s->mesh_commit ? s->mesh_commit : "")) &&
+             attribute(cells.id, "voronoi_cpp_dirty", static_cast<long long>(s->mesh_dirty ? 1 : 0)) &&
+             attribute(cells.id, "voronoi_cpp_origin", std::string(s->mesh_origin ? s->mesh_origin : ""));
         ok = ok && dataset(cells.id, "x", x) && dataset(cells.id, "y", y) && dataset(cells.id, "z", z) && dataset(cells.id, "volume", vol) &&
              dataset(cells.id, "density", rho) && dataset(cells.id, "velocity_x", vx) && dataset(cells.id, "velocity_y", vy) &&
              dataset(cells.id, "velocity_z", vz) && dataset(cells.id, "internal_energy", e_int) && dataset(cells.id, "pressure", P) &&

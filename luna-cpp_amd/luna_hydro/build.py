@@ -6,6 +6,7 @@
 
 Run `python luna-cpp_amd/luna_hydro/build.py`, or __graft_entry__.build().
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -16,7 +17,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["hydro/mesh_kernels.hip", "hydro/line_kernels.hip", "hydro/adjoint_kernels.hip", "hydro/limiter_kernels.hip",
+HIP_SOURCES = ["hydro/mesh_kernels.hip", "hydro/geometry_kernels.hip", "hydro/line_kernels.hip", "hydro/adjoint_kernels.hip",
+               "hydro/limiter_kernels.hip",
                "hydro/api.hip"]
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall"]
@@ -34,10 +36,21 @@ def _run(cmd):
 
 
 def build_hydro():
+    """Each HIP source compiled on its own (in parallel, objects under _build/), then linked into libluna_hydro.so."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     out = os.path.join(HERE, "libluna_hydro.so")
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    _run([hipcc] + HIPCC_FLAGS + ["-I" + INCLUDE, "-o", out + ".tmp"] + srcs)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    jobs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(objdir, os.path.basename(src).replace(".hip", ".o"))
+        jobs.append((obj, [hipcc] + compile_flags + ["-I" + INCLUDE, "-c", "-o", obj, os.path.join(CSRC, src)]))
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for f in [pool.submit(_run, cmd) for _, cmd in jobs]:
+            f.result()
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out + ".tmp"] + [obj for obj, _ in jobs])
     os.replace(out + ".tmp", out)
     return out
 

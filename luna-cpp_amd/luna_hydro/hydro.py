@@ -120,8 +120,8 @@ def exported_symbols():
              "luna_fv1_create_moving", "luna_fv1_cells", "luna_fv1_positions", "luna_fv1_line", "luna_fv1_set_inflow_spacing",
              "luna_fv1_mesh_stats",
              "luna_last_error",
-             "luna_fv_set_geometry", "luna_synchronize", "luna_fv_step_tangent", "luna_fv_step_adjoint", "luna_fv_moving_adjoint",
-             "luna_fv_time_step_tangent", "luna_fv_set_step_rule", "luna_fv_release_mesh", "luna_fv_mesh_stats",
+             "luna_fv_set_geometry", "luna_fv_set_geometry_device", "luna_fv_create_device", "luna_synchronize", "luna_fv_step_tangent", "luna_fv_step_adjoint", "luna_fv_moving_adjoint",
+             "luna_fv_time_step_tangent", "luna_fv_moving_tangent", "luna_fv_set_step_rule", "luna_fv_release_mesh", "luna_fv_mesh_stats",
              "luna_fv_set_limiter_network", "luna_fv_limiter_features", "luna_mlp_create", "luna_mlp_destroy", "luna_mlp_set_params",
              "luna_mlp_get_params", "luna_mlp_forward", "luna_mlp_gradient", "luna_mlp_train_step", "luna_mlp_synchronize"]
     return {n: hasattr(lib(), n) for n in names}
@@ -147,8 +147,40 @@ def geometry_record(mesh):
     return g, m
 
 
+class DeviceMesh:
+    """A mesh's arrays in GPU memory (torch tensors), as a GPU mesh producer leaves them (voronoi-cpp's device views): the
+    step reads them in place with set_geometry_device / FiniteVolume(DeviceMesh(...)). Built from a mesh dict by copying
+    its arrays to the device once; `record` is the luna_geometry of device pointers."""
+
+    def __init__(self, mesh, device="cuda"):
+        import torch
+
+        self.dim, self.cells, self.faces = mesh["dim"], mesh["cells"], mesh["faces"]
+        self.tensors = {}
+        for k in ("owner", "neighbour", "start", "list"):
+            self.tensors[k] = torch.as_tensor(np.ascontiguousarray(mesh[k], dtype=np.int32)).to(device)
+        for k in ("area", "face_centroid", "centroid", "volume", "positions"):
+            self.tensors[k] = torch.as_tensor(np.ascontiguousarray(mesh[k], dtype=np.float64)).to(device)
+        torch.cuda.synchronize()
+        g = Geometry()
+        g.dim, g.cells, g.faces = self.dim, self.cells, self.faces
+        for k in ("owner", "neighbour", "start", "list"):
+            setattr(g, k, ctypes.cast(ctypes.c_void_p(self.tensors[k].data_ptr()), _ip))
+        for k in ("area", "face_centroid", "centroid", "volume", "positions"):
+            setattr(g, k, ctypes.cast(ctypes.c_void_p(self.tensors[k].data_ptr()), _dp))
+        for a in range(3):
+            g.side[a] = mesh["side"][a] if a < len(mesh["side"]) else 0.0
+            g.period[a] = int(mesh["period"][a]) if a < len(mesh["period"]) else 0
+        g.step = mesh["step"]
+        self.record = g
+
+    def __getitem__(self, key):
+        return {"dim": self.dim, "cells": self.cells, "faces": self.faces}[key]
+
+
 class FiniteVolume:
-    """hydro::FiniteVolume<D> (D = 2, 3) on the GPU: FiniteVolume(exec, mesh, gas, order, per_face, limiter, motion)."""
+    """hydro::FiniteVolume<D> (D = 2, 3) on the GPU: FiniteVolume(exec, mesh, gas, order, per_face, limiter, motion).
+    mesh: a mesh dict (host arrays) or a DeviceMesh (read in place on the GPU)."""
 
     def __init__(self, mesh, gamma, order=1, limiter="face", riemann="hllc", motion="static", step_rule="arepo",
                  smallr=1e-10, smallc=1e-10, precision="double", hll_compression=0.0):
@@ -156,9 +188,13 @@ class FiniteVolume:
         self.V = self.D + 2
         opt = FVOptions(self.D, order, LIMITERS[limiter], RIEMANN[riemann], MOTION[motion], RULES[step_rule],
                         gamma, smallr, smallc, PRECISIONS[precision], hll_compression)
-        g, self._mesh_arrays = geometry_record(mesh)
         h = ctypes.c_void_p()
-        _check(lib().luna_fv_create(ctypes.byref(opt), ctypes.byref(g), ctypes.byref(h)), "FiniteVolume")
+        if isinstance(mesh, DeviceMesh):
+            self._mesh_arrays = mesh
+            _check(lib().luna_fv_create_device(ctypes.byref(opt), ctypes.byref(mesh.record), ctypes.byref(h)), "FiniteVolume")
+        else:
+            g, self._mesh_arrays = geometry_record(mesh)
+            _check(lib().luna_fv_create(ctypes.byref(opt), ctypes.byref(g), ctypes.byref(h)), "FiniteVolume")
         self._h = h
         self.motion = motion
 
@@ -200,7 +236,7 @@ class FiniteVolume:
         _check(lib().luna_fv_release_mesh(self._h), "release_mesh")
 
     def mesh_updates(self):
-        """mesh_stats().updates: the meshes taken after the first."""
+        """mesh_stats().updates: the meshes moving steps took (between moving_begin and moving_finish)."""
         n = ctypes.c_longlong()
         _check(lib().luna_fv_mesh_stats(self._h, ctypes.byref(n)), "mesh_stats")
         return n.value
@@ -246,6 +282,28 @@ class FiniteVolume:
         self._mesh_arrays = a0
         return qb, xb
 
+    def moving_tangent(self, dt, mesh0, mesh1, dq, dx, steps_fixed=False):
+        """FiniteVolume::step_moving_tangent() (luna_fv_moving_tangent; order 1, or order 0, where mesh1 is not read): the
+        handle's Q is the state at t^n, mesh0 the mesh of t^n, mesh1 the mesh the forward step reached; dq (N x V) and dx
+        (N x D), the tangents at t^n, -> those at t^n+1, with the project's mesh library's geometry tangents of both meshes.
+        Leaves the handle on mesh0 with Q unchanged."""
+        from . import mesh as mesh_lib
+
+        g0, a0 = geometry_record(mesh0)
+        g1, a1 = geometry_record(mesh1)
+        d0, d1 = mesh_lib.GeometryDerivative(mesh0), mesh_lib.GeometryDerivative(mesh1)
+        fn = ctypes.cast(mesh_lib.lib().luna_mesh_derivative_tangent, ctypes.c_void_p)
+        tq = np.ascontiguousarray(np.array(dq, dtype=np.float64).reshape(self.N, self.V))
+        tx = np.ascontiguousarray(np.array(dx, dtype=np.float64).reshape(self.N, self.D))
+        try:
+            _check(lib().luna_fv_moving_tangent(self._h, ctypes.c_double(dt), ctypes.byref(g0), ctypes.byref(g1), fn, d0._h, fn, d1._h,
+                                                int(steps_fixed), tq.ctypes.data_as(_dp), tx.ctypes.data_as(_dp)), "moving_tangent")
+        finally:
+            d0.close()
+            d1.close()
+        self._mesh_arrays = a0
+        return tq, tx
+
     def _derivative(self, fn, dt, rows, what):
         if hasattr(rows, "data_ptr"):
             if not rows.is_contiguous() or rows.numel() != self.N * self.V:
@@ -284,6 +342,10 @@ class FiniteVolume:
         return x
 
     def set_geometry(self, mesh):
+        """set_geometry with a mesh dict (host arrays) or a DeviceMesh (luna_fv_set_geometry_device)."""
+        if isinstance(mesh, DeviceMesh):
+            _check(lib().luna_fv_set_geometry_device(self._h, ctypes.byref(mesh.record)), "set_geometry_device")
+            return
         g, arrays = geometry_record(mesh)
         _check(lib().luna_fv_set_geometry(self._h, ctypes.byref(g)), "set_geometry")
         self._mesh_arrays = arrays

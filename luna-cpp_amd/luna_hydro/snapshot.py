@@ -15,7 +15,12 @@ class Snapshot(ctypes.Structure):
                 ("gamma", ctypes.c_double), ("smallr", ctypes.c_double), ("smallc", ctypes.c_double),
                 ("length_cm", ctypes.c_double), ("mass_g", ctypes.c_double), ("velocity_cm_s", ctypes.c_double),
                 ("cells", ctypes.c_longlong), ("positions", _dp), ("volume", _dp), ("q", _dp), ("commit", ctypes.c_char_p),
-                ("dirty", ctypes.c_int)]
+                ("dirty", ctypes.c_int), ("mesh_commit", ctypes.c_char_p), ("mesh_dirty", ctypes.c_int),
+                ("mesh_origin", ctypes.c_char_p)]
+
+# The mesh producer whose meshes the snapshots hold (written as voronoi_cpp_origin, the attribute the reference fills
+# with its mesh library's origin).
+MESH_ORIGIN = "luna-cpp_amd/csrc/mesh/voronoi.cpp (periodic Voronoi producer of this project, voronoi-cpp's output contract)"
 
 
 def lib():
@@ -35,7 +40,7 @@ def _check(rc, what):
 
 
 def write(directory, index, time, positions, volume, q, gamma, sides=(1.0, 1.0, 1.0), order=1, units=None, commit="", dirty=False,
-          smallr=1e-10, smallc=1e-10):
+          smallr=1e-10, smallc=1e-10, mesh_commit="", mesh_dirty=False, mesh_origin=MESH_ORIGIN):
     """Writes one snapshot of a 3D state (Q extensive, N x 5); returns the file's path. units: (length_cm, mass_g,
     velocity_cm_s) or None."""
     x = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
@@ -49,6 +54,7 @@ def write(directory, index, time, positions, volume, q, gamma, sides=(1.0, 1.0, 
     s.cells = x.shape[0]
     s.positions, s.volume, s.q = x.ctypes.data_as(_dp), vol.ctypes.data_as(_dp), q.ctypes.data_as(_dp)
     s.commit, s.dirty = commit.encode(), int(dirty)
+    s.mesh_commit, s.mesh_dirty, s.mesh_origin = mesh_commit.encode(), int(mesh_dirty), mesh_origin.encode()
     name = ctypes.create_string_buffer(4096)
     _check(lib().luna_snapshot_write(os.fsencode(directory), ctypes.byref(s), name, 4096), "write")
     return name.value.decode()
@@ -65,12 +71,16 @@ def read(path, names=("x", "y", "z", "volume", "density", "velocity_x", "velocit
         _check(lib().luna_snapshot_read_dataset(os.fsencode(path), name.encode(), a.ctypes.data_as(_dp), a.size), name)
         out[name] = a.reshape(5, N) if name == "U" else a
     attrs = {}
-    for name in ("time", "step", "gamma", "spatial_dimension", "order", "Lx", "Ly", "Lz", "N_cells", "luna_cpp_dirty"):
+    for name in ("time", "step", "gamma", "spatial_dimension", "order", "Lx", "Ly", "Lz", "N_cells", "luna_cpp_dirty",
+                 "voronoi_cpp_dirty"):
         v = ctypes.c_double()
         _check(lib().luna_snapshot_read_attribute(os.fsencode(path), b"cells", name.encode(), ctypes.byref(v)), name)
         attrs[name] = v.value
     buf = ctypes.create_string_buffer(256)
     _check(lib().luna_snapshot_read_string(os.fsencode(path), b"cells", b"mesh", buf, 256), "mesh")
     attrs["mesh"] = buf.value.decode()
+    for name in ("luna_cpp_commit", "voronoi_cpp_commit", "voronoi_cpp_origin"):
+        _check(lib().luna_snapshot_read_string(os.fsencode(path), b"cells", name.encode(), buf, 256), name)
+        attrs[name] = buf.value.decode()
     out["attributes"] = attrs
     return out

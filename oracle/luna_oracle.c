@@ -7,6 +7,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -22,6 +23,10 @@ static double dot(int D, const double* a, const double* b) {
 }
 
 /* euler.hpp:91-93 */
+/* The OpenMP threads the oracle's loops use (the CPU baseline's single-thread round). */
+void orc_set_threads(int n) { omp_set_num_threads(n); }
+int orc_max_threads(void) { return omp_get_max_threads(); }
+
 void orc_gas_init(orc_gas* gas, double gamma) {
     gas->gamma = gamma;
     gas->smallr = 1e-10;
@@ -236,9 +241,35 @@ static int list_face(int e) { return e >= 0 ? e : ~e; }
 
 static double magnitude_of(const orc_geometry* g, int f) { return sqrt(dot(g->D, g->area + f * g->D, g->area + f * g->D)); }
 
+/* The cube root of v > 0 rounded to the nearest double (the reference calls its backend's cbrt, whose last bit differs
+ * between glibc and the GPUs' libraries; the correctly rounded one is the same on both sides of the parity tests):
+ * the library's cbrt, within an ulp, then corrected by the sign of (y +- half an ulp)^3 - v from double-double products
+ * (fma gives each product's rounding error exactly; y^3 - v is exact by Sterbenz). No tie is possible. */
+static double cube_excess(double y, double h, double v) {
+    const double p2 = y * y, e2 = fma(y, y, -p2);
+    const double p3 = p2 * y, e3 = fma(p2, y, -p3);
+    const double r0 = p3 - v;
+    const double big = r0 + 3.0 * (p2 * h);
+    return big + (e3 + e2 * y) + 3.0 * (e2 * h) + 3.0 * (y * h * h) + h * h * h;
+}
+double orc_cbrt_rn(double v) {
+    double y = cbrt(v);
+    if (!(v > 0) || !(v < INFINITY)) return y;
+    for (int k = 0; k < 4; ++k) {
+        const double up = nextafter(y, INFINITY), down = nextafter(y, 0.0);
+        if (cube_excess(y, 0.5 * (up - y), v) < 0)
+            y = up;
+        else if (cube_excess(y, -0.5 * (y - down), v) > 0)
+            y = down;
+        else
+            break;
+    }
+    return y;
+}
+
 /* finite_volume.hpp:241-246 radius() */
 static double radius(int D, double v) {
-    if (D == 3) return cbrt(v * (3.0 / (4.0 * M_PI)));
+    if (D == 3) return orc_cbrt_rn(v * (3.0 / (4.0 * M_PI)));
     return sqrt(v * (1.0 / M_PI));
 }
 /* finite_volume.hpp:248-253 face_angle() */
@@ -3445,4 +3476,345 @@ static void riemann_adjoint(int kind, int D, const orc_gas* gas, const double* L
         orc_hll_adjoint(D, gas, L, R, n, w, Fbar, Lbar, Rbar);
     else
         orc_hllc_adjoint(D, gas, L, R, n, w, Fbar, Lbar, Rbar);
+}
+
+/* ---- The moving step's tangent (forward mode), fixed topology (finite_volume.hpp:2428-2486, 2764-2870) ----
+ * The forward-mode counterpart of orc_moving_adjoint: from the state held (Q, x at t^n on g0) and the mesh g1 the forward step
+ * reaches, (dq, dx) at t^n -> at t^n+1, in place; Q, x stay. The geometry's tangents come from the mesh library's derivative
+ * (orc_geometry_tangent: dx -> dV, dc per cell, dA, dx_f per face). */
+
+/* The tangents of the positions and of one mesh's geometry. */
+typedef struct {
+    const double *x, *V, *c, *A, *xf;
+} geo_dots;
+
+/* The tangent of face_offset(f, i): dx_f - dc_i, plus the owner's separation's when i does not own f. */
+static void offset_dot(const orc_geometry* g, const geo_dots* t, int f, int i, double* dr) {
+    const int D = g->D, o = g->owner[f];
+    for (int a = 0; a < D; ++a) dr[a] = t->xf[(size_t)f * D + a] - t->c[(size_t)i * D + a];
+    if (o != i)
+        for (int a = 0; a < D; ++a) dr[a] += t->x[(size_t)o * D + a] - t->x[(size_t)i * D + a];
+}
+
+/* finite_volume.hpp:2322-2330 FaceVelocity::tangent(): of face_velocity(d, x_f, w_L, w_R) at the forward's branch of the
+ * 0.9 |d| limit. */
+void orc_face_velocity_tangent(int D, const double* d, const double* xf, const double* wL, const double* wR, const double* dd, const double* dxf,
+                               const double* dwL, const double* dwR, double* dwf) {
+    double c[MAXD], u[MAXD], dc[MAXD], du[MAXD];
+    for (int a = 0; a < D; ++a) {
+        c[a] = xf[a] - d[a] * 0.5;
+        u[a] = wL[a] - wR[a];
+        dc[a] = dxf[a] - dd[a] * 0.5;
+        du[a] = dwL[a] - dwR[a];
+    }
+    const double dist = sqrt(dot(D, d, d)), ddist = dot(D, d, dd) / dist;
+    const double f0 = dot(D, c, u) / dist, df0 = (dot(D, dc, u) + dot(D, c, du)) / dist - f0 * ddist / dist;
+    const double limit = 0.9 * dist, c2 = dot(D, c, c);
+    double facv = f0, dfacv = df0;
+    if (c2 > limit * limit) {
+        const double cc = sqrt(c2), sc = limit / cc, dsc = 0.9 * ddist / cc - sc * dot(D, c, dc) / c2;
+        facv = f0 * sc;
+        dfacv = df0 * sc + f0 * dsc;
+    }
+    for (int a = 0; a < D; ++a) dwf[a] = (dwL[a] + dwR[a]) * 0.5 + dd[a] * (facv / dist) + d[a] * (dfacv / dist - facv * ddist / (dist * dist));
+}
+
+/* euler.hpp:861-891 MovingFace::tangent(): dG of G = |A| F(L, R, A / |A|, w). The flux is the face frame's Phi(L - w, R - w, n)
+ * taken to the lab (to_lab, linear in Phi); Phi's change with n follows from its rotation covariance: for the rotation
+ * Omega = dn n^T - n dn^T (Omega n = dn), Phi_n dn = Omega Phi - Phi_L (Omega L_w) - Phi_R (Omega R_w). So one tangent of the
+ * Riemann solver at (L, R) with dL - dw - Omega L_w, dR - dw - Omega R_w, plus Omega Phi and to_lab's own change with w. */
+void orc_moving_face_tangent(int kind, int D, const orc_gas* gas, const double* L, const double* R, const double* A, const double* w,
+                             const double* dL, const double* dR, const double* dA, const double* dw, double* dG) {
+    const int V = D + 2, P = D + 1;
+    const double s = sqrt(dot(D, A, A));
+    double n[MAXD], dn[MAXD], mw[MAXD], lw[MAXD], rw[MAXD];
+    for (int k = 0; k < D; ++k) {
+        n[k] = A[k] * (1.0 / s);
+        mw[k] = -w[k];
+        lw[k] = L[1 + k] - w[k];
+        rw[k] = R[1 + k] - w[k];
+    }
+    const double ds = dot(D, n, dA);
+    for (int k = 0; k < D; ++k) dn[k] = (dA[k] - n[k] * ds) / s;
+    double eL[MAXV], eR[MAXV];
+    memcpy(eL, dL, sizeof(double) * V);
+    memcpy(eR, dR, sizeof(double) * V);
+    const double nl = dot(D, n, lw), dnl = dot(D, dn, lw), nr = dot(D, n, rw), dnr = dot(D, dn, rw);
+    for (int k = 0; k < D; ++k) {
+        eL[1 + k] -= dw[k] + (dn[k] * nl - n[k] * dnl);
+        eR[1 + k] -= dw[k] + (dn[k] * nr - n[k] * dnr);
+    }
+    double F[MAXV], dF[MAXV], Phi[MAXV], X[MAXV];
+    riemann_tangent(kind, D, gas, L, R, eL, eR, n, w, F, dF);
+    memcpy(Phi, F, sizeof(double) * V);
+    to_lab(D, Phi, mw);
+    const double nP = dot(D, n, Phi + 1), dnP = dot(D, dn, Phi + 1);
+    X[0] = 0;
+    X[P] = 0;
+    for (int k = 0; k < D; ++k) X[1 + k] = dn[k] * nP - n[k] * dnP;
+    to_lab(D, X, w);
+    for (int v = 0; v < V; ++v) dF[v] += X[v];
+    dF[P] += dot(D, dw, Phi + 1) + dot(D, w, dw) * Phi[0];
+    for (int k = 0; k < D; ++k) dF[1 + k] += dw[k] * Phi[0];
+    for (int v = 0; v < V; ++v) dG[v] = ds * F[v] + s * dF[v];
+}
+
+/* W, c_s tangents of cell i (finite_volume.hpp:2408-2426 moving_primitives): dU = (dq - U dV) / V. */
+static void moving_prim_tangent(const orc_fv* fv, const orc_geometry* g, const geo_dots* t, const double* dq, int i, double* dW, double* dcs) {
+    const int D = fv->D, V = D + 2, PW = V + 2;
+    const double* Wi = fv->prims + (size_t)i * PW;
+    const double inv = 1.0 / g->volume[i];
+    double U[MAXV], dU[MAXV];
+    for (int v = 0; v < V; ++v) U[v] = fv->q[(size_t)i * V + v] * inv;
+    for (int v = 0; v < V; ++v) dU[v] = (dq[(size_t)i * V + v] - U[v] * t->V[i]) * inv;
+    orc_primitive_tangent(D, &fv->gas, U, dU, dW);
+    const double cs = Wi[V];
+    *dcs = cs > 0 ? 0.5 * fv->gas.gamma * (dW[D + 1] / Wi[0] - Wi[D + 1] * dW[0] / (Wi[0] * Wi[0])) / cs : 0.0;
+}
+
+/* finite_volume.hpp:2251-2306 Regularised::tangent(): of the point velocity w = v + c (fraction c_s / |c|), at the forward's
+ * branches (fraction from the largest face angle, its slope zero outside the ramp). */
+static void regularised_tangent(const orc_fv* fv, const orc_geometry* g, const geo_dots* t, int i, const double* dWi, double dcs, double* dw) {
+    const int D = fv->D, V = D + 2, PW = V + 2;
+    const double* Wi = fv->prims + (size_t)i * PW;
+    double amax = 0;
+    int fmax = -1, other = -1;
+    for (int k = g->start[i]; k < g->start[i + 1]; ++k) {
+        const int e = g->list[k], f = list_face(e), j = e >= 0 ? g->neighbour[f] : g->owner[f];
+        double d[MAXD];
+        separation(g, i, j, d);
+        const double angle = face_angle(D, magnitude_of(g, f)) / (0.5 * sqrt(dot(D, d, d)));
+        if (angle > amax) {
+            amax = angle;
+            fmax = f;
+            other = j;
+        }
+    }
+    const double F = 2.25, shaping = 0.5;
+    double fraction = 0, slope = 0;
+    if (amax > 0.75 * F) {
+        fraction = amax > F ? shaping : shaping * (amax - 0.75 * F) / (0.25 * F);
+        slope = amax > F ? 0.0 : shaping / (0.25 * F);
+    }
+    for (int a = 0; a < D; ++a) dw[a] = dWi[1 + a];
+    const double* c = g->centroid + (size_t)i * D;
+    const double* dc = t->c + (size_t)i * D;
+    const double cd = sqrt(dot(D, c, c)), cs = Wi[V];
+    if (!(cd > 0 && fraction > 0)) return;
+    const double k = fraction * cs / cd;
+    double dfraction = 0;
+    if (slope != 0) {
+        const double* A = g->area + (size_t)fmax * D;
+        double d[MAXD], dd[MAXD];
+        separation(g, i, other, d);
+        for (int a = 0; a < D; ++a) dd[a] = t->x[(size_t)other * D + a] - t->x[(size_t)i * D + a];
+        const double mag = sqrt(dot(D, A, A)), dist = sqrt(dot(D, d, d));
+        const double dmag = dot(D, A, t->A + (size_t)fmax * D) / mag, ddist = dot(D, d, dd) / dist;
+        dfraction = slope * (face_angle_slope(D, mag) * dmag / (0.5 * dist) - amax * ddist / dist);
+    }
+    const double dcd = dot(D, c, dc) / cd;
+    const double dk = (dfraction * cs + fraction * dcs) / cd - k * dcd / cd;
+    for (int a = 0; a < D; ++a) dw[a] += dc[a] * k + c[a] * dk;
+}
+
+/* finite_volume.hpp:2687-2762 moving_gradient_tangent(): the limited fit's tangent in W and in the geometry it reads. The fit
+ * x_v = M^-1 b_v (X_k = [1, d_k + c_j - c_i], weights |A_k|) moves by M^-1 (db_v - dM x_v); after the last clip
+ * g = g0 (bound - W_i) / (g0 . o), o that face's offset (which moves with the geometry too). dW: all cells' W tangents. */
+static void moving_fit_tangent(const orc_fv* fv, const orc_geometry* g, const geo_dots* t, int i, const double* dW, double* dg) {
+    const int D = fv->D, V = D + 2, U = D + 1, PW = V + 2;
+    const double* Wi = fv->prims + (size_t)i * PW;
+    const double* dWi = dW + (size_t)i * V;
+    double gl[MAXV * MAXD], g0[MAXV * MAXD], dg_unused[MAXV * MAXD], Lc[MAXD + 1][MAXD + 1], M0[MAXD + 1], icpt[MAXV];
+    limit_pattern pat;
+    fit_cell_x(fv, g, i, dW, gl, g0, dg_unused, Lc, M0, &pat, icpt);
+    double dM[MAXD + 1][MAXD + 1], db[MAXV][MAXD + 1];
+    memset(dM, 0, sizeof dM);
+    memset(db, 0, sizeof db);
+    const double* ci = g->centroid + (size_t)i * D;
+    const int first = g->start[i];
+    for (int k = first; k < g->start[i + 1]; ++k) {
+        const int e = g->list[k], f = list_face(e), j = e >= 0 ? g->neighbour[f] : g->owner[f];
+        const double* Wj = fv->prims + (size_t)j * PW;
+        double d[MAXD], X[MAXD + 1], dX[MAXD + 1];
+        separation(g, i, j, d);
+        const double w = magnitude_of(g, f);
+        const double dwt = w > 0 ? dot(D, g->area + (size_t)f * D, t->A + (size_t)f * D) / w : 0.0;
+        X[0] = 1;
+        dX[0] = 0;
+        for (int p = 1; p < U; ++p) {
+            X[p] = d[p - 1] + g->centroid[(size_t)j * D + p - 1] - ci[p - 1];
+            dX[p] = (t->x[(size_t)j * D + p - 1] - t->x[(size_t)i * D + p - 1]) + t->c[(size_t)j * D + p - 1] - t->c[(size_t)i * D + p - 1];
+        }
+        for (int p = 0; p < U; ++p)
+            for (int q = 0; q < U; ++q) dM[p][q] += dwt * X[p] * X[q] + w * (dX[p] * X[q] + X[p] * dX[q]);
+        for (int v = 0; v < V; ++v) {
+            const double delta = Wj[v] - Wi[v], ddelta = dW[(size_t)j * V + v] - dWi[v];
+            for (int p = 0; p < U; ++p) db[v][p] += dwt * X[p] * delta + w * dX[p] * delta + w * X[p] * ddelta;
+        }
+    }
+    double rhs[MAXV][MAXD + 1], dx[MAXV][MAXD + 1];
+    for (int v = 0; v < V; ++v)
+        for (int p = 0; p < U; ++p) {
+            double mx = dM[p][0] * icpt[v];
+            for (int a = 0; a < D; ++a) mx += dM[p][1 + a] * g0[v * D + a];
+            rhs[v][p] = db[v][p] - mx;
+        }
+    solve_factored(D, Lc, rhs, dx);
+    for (int v = 0; v < V; ++v) {
+        const double* gv = g0 + v * D;
+        double dg0[MAXD];
+        for (int a = 0; a < D; ++a) dg0[a] = dx[v][1 + a];
+        if (pat.last[v] < 0) {
+            for (int a = 0; a < D; ++a) dg[v * D + a] = dg0[a];
+        } else if (pat.zero[v]) {
+            for (int a = 0; a < D; ++a) dg[v * D + a] = 0.0;
+        } else {
+            const double* o = pat.offset[v];
+            double dor[MAXD];
+            offset_dot(g, t, list_face(g->list[first + pat.last[v]]), i, dor);
+            const double den = dot(D, gv, o);
+            const double s = (pat.bound[v] - Wi[v]) / den;
+            const double ds = (pat.dbound[v] - dWi[v] - s * (dot(D, dg0, o) + dot(D, gv, dor))) / den;
+            for (int a = 0; a < D; ++a) dg[v * D + a] = dg0[a] * s + gv[a] * ds;
+        }
+    }
+}
+
+/* One moving pass's tangent (finite_volume.hpp:2829-2870 moving_pass_tangent): dq -= dt/2 sum_k sign_k dG_k per cell in list
+ * order, G = |A| F(side_o, side_j, A / |A|, face_velocity(d, x_f, w-hat_o, w-hat_j)). First pass (mesh t^n): sides W + g.r;
+ * second (mesh t^n+1): half_step(W, g, 2 dt) + g.(r + c^n+1 - c^n + dt w-hat), with c^n's tangent dc0. */
+static void moving_pass_tangent(const orc_fv* fv, const orc_geometry* g, const geo_dots* t, int second, double dt, const double* wh, const double* dwh,
+                                const double* dW, const double* dg, const double* dc0, double* dq) {
+    const int D = fv->D, V = D + 2, PW = V + 2;
+    double* dG = (double*)calloc((size_t)(g->F > 0 ? g->F : 1) * V, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int f = 0; f < g->F; ++f) {
+        const int o = g->owner[f], j = g->neighbour[f];
+        const double* A = g->area + (size_t)f * D;
+        if (sqrt(dot(D, A, A)) == 0) continue;
+        double d[MAXD], dd[MAXD], wf[MAXD], dwf[MAXD], sides[2][MAXV], dsides[2][MAXV];
+        separation(g, o, j, d);
+        for (int a = 0; a < D; ++a) dd[a] = t->x[(size_t)j * D + a] - t->x[(size_t)o * D + a];
+        face_velocity(g, f, o, j, wh + (size_t)o * D, wh + (size_t)j * D, wf);
+        orc_face_velocity_tangent(D, d, g->face_centroid + (size_t)f * D, wh + (size_t)o * D, wh + (size_t)j * D, dd, t->xf + (size_t)f * D,
+                                  dwh + (size_t)o * D, dwh + (size_t)j * D, dwf);
+        const int cells[2] = {o, j};
+        for (int s2 = 0; s2 < 2; ++s2) {
+            const int i = cells[s2];
+            const double* W = fv->prims + (size_t)i * PW;
+            const double* gi = fv->grad + (size_t)i * V * D;
+            const double* dWi = dW + (size_t)i * V;
+            const double* dgi = dg + (size_t)i * V * D;
+            double r[MAXD], dr[MAXD];
+            face_offset(g, f, i, r);
+            offset_dot(g, t, f, i, dr);
+            if (second) {
+                for (int a = 0; a < D; ++a) {
+                    r[a] = r[a] + g->centroid[(size_t)i * D + a] - fv->old_centroid[(size_t)i * D + a] + wh[(size_t)i * D + a] * (0.5 * (2 * dt));
+                    dr[a] = dr[a] + t->c[(size_t)i * D + a] - dc0[(size_t)i * D + a] + dwh[(size_t)i * D + a] * dt;
+                }
+                orc_half_step(D, &fv->gas, W, gi, 2 * dt, sides[s2]);
+                orc_half_step_tangent(D, &fv->gas, W, dWi, gi, dgi, 2 * dt, dsides[s2]);
+            } else {
+                for (int v = 0; v < V; ++v) {
+                    sides[s2][v] = W[v];
+                    dsides[s2][v] = dWi[v];
+                }
+            }
+            for (int v = 0; v < V; ++v) sides[s2][v] += dot(D, gi + v * D, r);
+            if (sides[s2][0] > 0 && sides[s2][D + 1] > 0) {
+                for (int v = 0; v < V; ++v) dsides[s2][v] += dot(D, dgi + v * D, r) + dot(D, gi + v * D, dr);
+            } else {
+                for (int v = 0; v < V; ++v) {
+                    sides[s2][v] = W[v];
+                    dsides[s2][v] = dWi[v];
+                }
+            }
+        }
+        orc_moving_face_tangent(fv->riemann, D, &fv->gas, sides[0], sides[1], A, wf, dsides[0], dsides[1], t->A + (size_t)f * D, dwf,
+                                dG + (size_t)f * V);
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < fv->N; ++i) {
+        double sum[MAXV] = {0, 0, 0, 0, 0};
+        for (int k = g->start[i]; k < g->start[i + 1]; ++k) {
+            const int e = g->list[k], f = list_face(e);
+            const double sign = e >= 0 ? 1.0 : -1.0;
+            for (int v = 0; v < V; ++v) sum[v] += sign * dG[(size_t)f * V + v];
+        }
+        for (int v = 0; v < V; ++v) dq[(size_t)i * V + v] -= 0.5 * dt * sum[v];
+    }
+    free(dG);
+}
+
+static double* dots_alloc(size_t n) { return (double*)calloc(n > 0 ? n : 1, sizeof(double)); }
+
+int orc_moving_tangent(orc_fv* fv, const orc_geometry* g0, const orc_geometry* g1, double dt, int steps_fixed, double* dq, double* dx,
+                       orc_geometry_tangent tangent0, void* ctx0, orc_geometry_tangent tangent1, void* ctx1) {
+    if (!fv->moving || fv->D < 2 || fv->precision != 0 || fv->hll_compression > 0 || fv->limiter == 2) return -1;
+    if (fv->order != 0 && fv->order != 1) return -1;
+    const int D = fv->D, V = D + 2, N = fv->N;
+    const size_t ND = (size_t)N * D;
+    /* t^n: W, the limited gradients and the point velocities (the sweep), c^n, the grid steps and w-hat */
+    orc_primitives(fv, g0);
+    for (int i = 0; i < N; ++i) sweep_cell(fv, g0, i);
+    if (fv->order == 0) memset(fv->grad, 0, sizeof(double) * ND * V);
+    double* wh = dots_alloc(ND);
+    if (fv->order == 0) {
+        memcpy(wh, fv->w, sizeof(double) * ND);
+    } else {
+        const double unit = g0->step / dt;
+        for (int i = 0; i < N; ++i)
+            for (int a = 0; a < D; ++a) {
+                const size_t at = (size_t)i * D + a;
+                fv->old_centroid[at] = g0->centroid[at];
+                if (!fv->hold_steps) {
+                    const long long k0 = orc_snap(fv->x[at], g0->step, g0->period[a]);
+                    const long long k1 = orc_snap(fv->x[at] + dt * fv->w[at], g0->step, g0->period[a]);
+                    fv->steps[at] = orc_delta(k0, k1, g0->period[a]);
+                }
+                wh[at] = (double)fv->steps[at] * unit;
+            }
+    }
+    const int F0 = g0->F, F1 = g1->F;
+    double *dV0 = dots_alloc(N), *dc0 = dots_alloc(ND), *dA0 = dots_alloc((size_t)F0 * D), *dxf0 = dots_alloc((size_t)F0 * D);
+    tangent0(ctx0, dx, dV0, dc0, dA0, dxf0);
+    const geo_dots t0 = {dx, dV0, dc0, dA0, dxf0};
+    double *dW = dots_alloc((size_t)N * V), *dcs = dots_alloc(N), *dw = dots_alloc(ND), *dg = dots_alloc((size_t)N * V * D);
+    for (int i = 0; i < N; ++i) moving_prim_tangent(fv, g0, &t0, dq, i, dW + (size_t)i * V, dcs + i);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        regularised_tangent(fv, g0, &t0, i, dW + (size_t)i * V, dcs[i], dw + (size_t)i * D);
+        if (fv->order == 1) moving_fit_tangent(fv, g0, &t0, i, dW, dg + (size_t)i * V * D);
+    }
+    if (fv->order == 1 && steps_fixed) memset(dw, 0, sizeof(double) * ND); /* the grid steps held: no derivative through w-hat */
+    double* dx1 = dots_alloc(ND);
+    if (fv->order == 0) {
+        /* finite_volume.hpp:2428-2486: one pass over dt on t^n with the point velocities (sides W), then x += dt w */
+        moving_pass_tangent(fv, g0, &t0, 0, 2 * dt, wh, dw, dW, dg, NULL, dq);
+        for (size_t k = 0; k < ND; ++k) dx[k] = dx[k] + dt * dw[k];
+    } else {
+        moving_pass_tangent(fv, g0, &t0, 0, dt, wh, dw, dW, dg, NULL, dq);
+        for (size_t k = 0; k < ND; ++k) dx1[k] = dx[k] + dt * dw[k];
+        double *dV1 = dots_alloc(N), *dc1 = dots_alloc(ND), *dA1 = dots_alloc((size_t)F1 * D), *dxf1 = dots_alloc((size_t)F1 * D);
+        tangent1(ctx1, dx1, dV1, dc1, dA1, dxf1);
+        const geo_dots t1 = {dx1, dV1, dc1, dA1, dxf1};
+        moving_pass_tangent(fv, g1, &t1, 1, dt, wh, dw, dW, dg, dc0, dq);
+        memcpy(dx, dx1, sizeof(double) * ND);
+        free(dV1);
+        free(dc1);
+        free(dA1);
+        free(dxf1);
+    }
+    free(dx1);
+    free(wh);
+    free(dV0);
+    free(dc0);
+    free(dA0);
+    free(dxf0);
+    free(dW);
+    free(dcs);
+    free(dw);
+    free(dg);
+    return 0;
 }

@@ -38,10 +38,13 @@ typedef struct {
 } orc_gas;
 
 void orc_gas_init(orc_gas* gas, double gamma);
+void orc_set_threads(int n);
+int orc_max_threads(void);
 void orc_primitive(int D, const orc_gas* gas, const double* U, double* W);
 void orc_conserved(int D, const orc_gas* gas, const double* W, double* U);
 double orc_sound_speed(int D, const orc_gas* gas, const double* W);
 void orc_half_step(int D, const orc_gas* gas, const double* W, const double* g, double dt, double* out);
+double orc_cbrt_rn(double v); /* the cube root rounded to nearest (the cell radius) */
 void orc_hll(int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, double* F);
 void orc_hllc(int D, const orc_gas* gas, const double* L, const double* R, const double* n, const double* w, double* F);
 
@@ -120,6 +123,19 @@ int orc_step_adjoint(orc_fv* fv, const orc_geometry* geo, double dt, double* qba
 typedef void (*orc_geometry_adjoint)(void* ctx, const double* Vbar, const double* cbar, const double* Abar, const double* xfbar, double* xbar);
 int orc_moving_adjoint(orc_fv* fv, const orc_geometry* g0, const orc_geometry* g1, double dt, int steps_fixed, double* qbar, double* xbar,
                        orc_geometry_adjoint adjoint0, void* ctx0, orc_geometry_adjoint adjoint1, void* ctx1);
+/* The moving step's tangent (finite_volume.hpp:2428-2486 order 0, :2764-2870 order 1), fixed topology: from the state held
+ * (Q, x at t^n on g0; g1 the mesh the forward step reaches), dq (N x V) and dx (N x D) at t^n become their tangents at
+ * t^n+1, in place; Q and x stay. tangent0 / tangent1: the two meshes' geometry tangents (dx -> dV, dc, dA, dx_f, the
+ * mesh library's GeometryDerivative::tangent). steps_fixed as orc_moving_adjoint. -1 where the adjoint refuses too. */
+typedef void (*orc_geometry_tangent)(void* ctx, const double* dx, double* dV, double* dc, double* dA, double* dxf);
+int orc_moving_tangent(orc_fv* fv, const orc_geometry* g0, const orc_geometry* g1, double dt, int steps_fixed, double* dq, double* dx,
+                       orc_geometry_tangent tangent0, void* ctx0, orc_geometry_tangent tangent1, void* ctx1);
+/* Its pieces, pinned on their own: the face velocity's tangent (:2322-2330) and the moving face's (euler.hpp:861-891). */
+void orc_face_velocity_tangent(int D, const double* d, const double* xf, const double* wL, const double* wR, const double* dd, const double* dxf,
+                               const double* dwL, const double* dwR, double* dwf);
+void orc_moving_face_tangent(int kind, int D, const orc_gas* gas, const double* L, const double* R, const double* A, const double* w,
+                             const double* dL, const double* dR, const double* dA, const double* dw, double* dG);
+
 long long orc_snap(double x, double step, long long period);
 /* precision.hpp:55-130, half::from_float / to_float (IEEE binary16, round to nearest even), group_exponent,
  * power_of_two. */

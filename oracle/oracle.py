@@ -39,6 +39,7 @@ class _FV(ctypes.Structure):
 
 
 _GEO_ADJOINT = ctypes.CFUNCTYPE(None, ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp)
+_GEO_TANGENT = ctypes.CFUNCTYPE(None, ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp)
 
 
 class _Line(ctypes.Structure):
@@ -88,6 +89,15 @@ def lib():
         L.orc_mline_step_fused.restype = ctypes.c_double
         _LIB = L
     return _LIB
+
+
+def set_threads(n):
+    """The OpenMP threads of the oracle's loops."""
+    lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().orc_max_threads())
 
 
 def _ptr(a, kind=_dp):
@@ -369,6 +379,55 @@ class FiniteVolume:
         if rc != 0:
             raise ValueError("moving_adjoint: a moving mesh, order 1")
         return qb, xb
+
+
+    def moving_tangent(self, dt, dq, dx, mesh1, tangent0, tangent1, steps_fixed=True):
+        """The moving step's tangent (orders 0 and 1, fixed topology) at the state held (q, x at t^n on the current mesh):
+        (dq, dx) at t^n -> at t^n+1; q, x stay. mesh1: the mesh the forward step reached (order 1). tangent0/1: the two
+        meshes' geometry tangents, f(dx) -> (dV, dc, dA, dxf) (mesh.GeometryDerivative.tangent)."""
+        D, N, V = self.D, self.N, self.V
+        geo1 = Geometry(mesh1)
+        F0, F1 = self.geo.c.F, geo1.c.F
+
+        def wrap(fn, F):
+            def cb(_ctx, dxp, dVp, dcp, dAp, dxfp):
+                as_ = np.ctypeslib.as_array
+                dV, dc, dA, dxf = fn(as_(dxp, (N, D)).copy())
+                as_(dVp, (N,))[...] = dV
+                as_(dcp, (N, D))[...] = dc
+                if F:
+                    as_(dAp, (F, D))[...] = dA
+                    as_(dxfp, (F, D))[...] = dxf
+            return _GEO_TANGENT(cb)
+
+        c0, c1 = wrap(tangent0, F0), wrap(tangent1, F1)
+        tq = np.ascontiguousarray(np.array(dq, dtype=np.float64).reshape(N, V))
+        tx = np.ascontiguousarray(np.array(dx, dtype=np.float64).reshape(N, D))
+        L = lib()
+        L.orc_moving_tangent.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                         _GEO_TANGENT, ctypes.c_void_p, _GEO_TANGENT, ctypes.c_void_p]
+        rc = L.orc_moving_tangent(ctypes.byref(self.c), ctypes.byref(self.geo.c), ctypes.byref(geo1.c), ctypes.c_double(dt),
+                                  int(steps_fixed), _ptr(tq), _ptr(tx), c0, None, c1, None)
+        if rc != 0:
+            raise ValueError("moving_tangent: a moving mesh, order 0 or 1, the double policy")
+        return tq, tx
+
+
+def face_velocity_tangent(d, xf, wL, wR, dd, dxf, dwL, dwR):
+    """FaceVelocity::tangent (finite_volume.hpp:2322-2330)."""
+    args = [_vec(a) for a in (d, xf, wL, wR, dd, dxf, dwL, dwR)]
+    out = np.empty_like(args[0])
+    lib().orc_face_velocity_tangent(len(args[0]), *(_ptr(a) for a in args), _ptr(out))
+    return out
+
+
+def moving_face_tangent(L, R, A, w, dL, dR, dA, dw, gamma, riemann="hllc"):
+    """MovingFace::tangent (euler.hpp:861-891): dG of G = |A| F(L, R, A / |A|, w)."""
+    args = [_vec(a) for a in (L, R, A, w, dL, dR, dA, dw)]
+    out = np.empty_like(args[0])
+    g = gas(gamma)
+    lib().orc_moving_face_tangent({"hllc": 0, "hll": 1}[riemann], len(args[2]), ctypes.byref(g), *(_ptr(a) for a in args), _ptr(out))
+    return out
 
 
 WALLS = {"reflective": 0, "outflow": 1, "inflow": 2}

@@ -29,6 +29,9 @@ def test_snapshot_round_trip(tmp_path):
     a = s["attributes"]
     assert a["time"] == 0.0125 and a["step"] == 7 and a["spatial_dimension"] == 3 and a["N_cells"] == m["cells"]
     assert a["mesh"] == "voronoi" and a["gamma"] == 5.0 / 3.0 and a["luna_cpp_dirty"] == 1
+    # the provenance attributes of the reference's writer (snapshot.hpp:101-106), the mesh producer's included
+    assert a["luna_cpp_commit"] == "abc123" and a["voronoi_cpp_commit"] == "" and a["voronoi_cpp_dirty"] == 0
+    assert a["voronoi_cpp_origin"] == snapshot.MESH_ORIGIN
 
 
 def test_fnv1a_known_answers():
