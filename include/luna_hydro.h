@@ -244,6 +244,15 @@ int luna_fv1_step_adjoint(luna_fv1* fv, double dt, double* qbar);
  * of the step) stays. Both block until done. LUNA_ERR_LOGIC on a moving mesh or order 0. */
 int luna_fv_step_tangent(luna_fv* fv, double dt, double* dq, int device);
 int luna_fv_step_adjoint(luna_fv* fv, double dt, double* qbar, int device);
+/* No counterpart in the reference: luna_fv_step_adjoint (device = 1) for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It allocates nothing: the adjoint's work
+ * rows (the faces' side bars, the cells' W bars, fit bars, bound bars and extremes, the output) are carved from the
+ * library's own scratch, a file-scope device array of 64 MiB per GPU (held from the library's load until the process
+ * ends), which bounds the mesh: about 70,000 cells in 3D and 160,000 in 2D; a larger mesh gives LUNA_ERR_ARGUMENT
+ * (use luna_fv_step_adjoint). qbar: a device array (cells x (dim + 2)), read and then overwritten with J^T qbar. Every
+ * call on a GPU uses that GPU's one scratch array, so calls that overlap on one GPU (two threads, or two streams in
+ * flight) race; each call writes every row it reads before reading it. Blocks, as luna_fv_step_adjoint. */
+int luna_fv_step_adjoint_noalloc(luna_fv* fv, double dt, double* qbar);
 /* FiniteVolume::time_step_tangent(courant, dq) (:1941-1964): dt, as luna_fv_time_step; the reference also keeps the sweep's
  * tangent for the next step_tangent with the same dq, which here takes it itself (the same numbers), so dq is not read.
  * The derivatives' conditions apply (LUNA_ERR_LOGIC otherwise). */

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <thread>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -704,6 +705,93 @@ int luna_fv_step_adjoint(luna_fv* fv, double dt, double* qbar, int device) {
                           : luna::launch_step_adjoint<2>(s, g, r, gas, fv->opt.riemann, dt, limit, w);
     }));
     LUNA_HIP(hipMemcpyAsync(qbar, fv->d_out.p, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipStreamSynchronize(s));
+    fv->prims_ready = false;
+    fv->swept = false;
+    return LUNA_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The work rows of luna_fv_step_adjoint_noalloc: a file-scope device array (each GPU holds its own copy from the library's
+// load until the process ends), carved per call into the adjoint's rows. Its device address comes from
+// hipGetSymbolAddress once per device (the array's name on the host is only a stand-in, never a device address).
+constexpr size_t SCRATCH_BYTES = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_lock;
+void* g_scratch[MAX_DEVICES] = {};
+
+// The scratch array's address on the current device (looked up on the first call there).
+int scratch_address(void** out) {
+    int device = 0;
+    LUNA_HIP(hipGetDevice(&device));
+    if (device < 0 || device >= MAX_DEVICES) return fail(LUNA_ERR_DEVICE, "luna: device index beyond the scratch table");
+    std::lock_guard<std::mutex> hold(g_scratch_lock);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(LUNA_ERR_DEVICE, std::string("luna: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch[device] = p;
+    }
+    *out = g_scratch[device];
+    return LUNA_OK;
+}
+
+// The adjoint's rows carved from [base, base + SCRATCH_BYTES), each 256-byte aligned; false if they do not fit.
+bool carve_adjoint(const luna_fv* fv, unsigned char* base, luna::Derivs& w, size_t& need) {
+    const size_t N = fv->N, V = fv->V, F = fv->F, U = fv->D + 1;
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        const size_t here = at;
+        at += (bytes + 255) / 256 * 256;
+        return here;
+    };
+    const size_t o_sidebar = take(sizeof(double) * 2 * F * V), o_dW = take(sizeof(double) * N * V), o_bbar = take(sizeof(double) * N * V * U),
+                 o_bound = take(sizeof(double) * N * V), o_ext = take(sizeof(int) * N * V), o_out = take(sizeof(double) * N * V);
+    need = at;
+    if (need > SCRATCH_BYTES) return false;
+    w = luna::Derivs{nullptr, reinterpret_cast<double*>(base + o_dW), nullptr, nullptr, reinterpret_cast<double*>(base + o_sidebar),
+                     reinterpret_cast<double*>(base + o_bbar), reinterpret_cast<double*>(base + o_bound), reinterpret_cast<int*>(base + o_ext),
+                     reinterpret_cast<double*>(base + o_out)};
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int luna_fv_step_adjoint_noalloc(luna_fv* fv, double dt, double* qbar) {
+    if (!fv || !qbar) return fail(LUNA_ERR_ARGUMENT, "luna_fv_step_adjoint_noalloc: null argument");
+    if (fv->opt.motion != LUNA_MOTION_STATIC || fv->opt.order != 1)
+        return fail(LUNA_ERR_LOGIC, "luna_fv_step_adjoint_noalloc: derivatives of a static order-1 step");
+    if (fv->opt.limiter == LUNA_LIMITER_LEARNED || fv->opt.precision != LUNA_PRECISION_DOUBLE || fv->opt.hll_compression > 0)
+        return fail(LUNA_ERR_LOGIC, "luna_fv_step_adjoint_noalloc: the double policy, the face-centroid limiter or none, no HLL fallback");
+    if (fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_step_adjoint_noalloc: a moving step is between begin and finish");
+    if (!fv->mesh_ok) return fail(LUNA_ERR_LOGIC, "luna: the last mesh given was refused; set a valid mesh first");
+    void* base = nullptr;
+    if (int rc = scratch_address(&base)) return rc;
+    luna::Derivs w{};
+    size_t need = 0;
+    if (!carve_adjoint(fv, static_cast<unsigned char*>(base), w, need))
+        return fail(LUNA_ERR_ARGUMENT, "luna_fv_step_adjoint_noalloc: this mesh needs " + std::to_string(need) + " bytes of work rows, beyond the " +
+                                           std::to_string(SCRATCH_BYTES) + " of the library's scratch (use luna_fv_step_adjoint)");
+    w.dq = qbar;  // read only: the cotangent after the step
+    hipStream_t s = fv->stream;
+    if (int rc = primitives(fv)) return rc;  // the forward's rows at Q: primitives and the limited gradients
+    if (int rc = sweep(fv, 1.0)) return rc;
+    const luna::Geo g = fv->geo();
+    const luna::Rows r = fv->rows();
+    const luna::Gas gas = fv->gas();
+    const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID;
+    LUNA_HIP(timed(fv->prof, s, "step_adjoint", [&] {
+        return fv->D == 3 ? luna::launch_step_adjoint<3>(s, g, r, gas, fv->opt.riemann, dt, limit, w)
+                          : luna::launch_step_adjoint<2>(s, g, r, gas, fv->opt.riemann, dt, limit, w);
+    }));
+    LUNA_HIP(hipMemcpyAsync(qbar, w.out, sizeof(double) * fv->N * fv->V, hipMemcpyDeviceToDevice, s));
     LUNA_HIP(hipStreamSynchronize(s));
     fv->prims_ready = false;
     fv->swept = false;

@@ -120,7 +120,7 @@ def exported_symbols():
              "luna_fv1_create_moving", "luna_fv1_cells", "luna_fv1_positions", "luna_fv1_line", "luna_fv1_set_inflow_spacing",
              "luna_fv1_mesh_stats",
              "luna_last_error",
-             "luna_fv_set_geometry", "luna_fv_set_geometry_device", "luna_fv_create_device", "luna_synchronize", "luna_fv_step_tangent", "luna_fv_step_adjoint", "luna_fv_moving_adjoint",
+             "luna_fv_set_geometry", "luna_fv_set_geometry_device", "luna_fv_create_device", "luna_synchronize", "luna_fv_step_tangent", "luna_fv_step_adjoint", "luna_fv_step_adjoint_noalloc", "luna_fv_moving_adjoint",
              "luna_fv_time_step_tangent", "luna_fv_moving_tangent", "luna_fv_set_step_rule", "luna_fv_release_mesh", "luna_fv_mesh_stats",
              "luna_fv_set_limiter_network", "luna_fv_limiter_features", "luna_mlp_create", "luna_mlp_destroy", "luna_mlp_set_params",
              "luna_mlp_get_params", "luna_mlp_forward", "luna_mlp_gradient", "luna_mlp_train_step", "luna_mlp_synchronize"]
@@ -258,6 +258,14 @@ class FiniteVolume:
     def step_adjoint(self, dt, qbar):
         """FiniteVolume::step_adjoint(): J^T qbar of one static step from the current Q (which stays); qbar as step_tangent's dq."""
         return self._derivative(lib().luna_fv_step_adjoint, dt, qbar, "step_adjoint")
+
+    def step_adjoint_noalloc(self, dt, qbar):
+        """luna_fv_step_adjoint_noalloc: step_adjoint on a device array (an object with data_ptr(), e.g. a torch tensor on the
+        GPU, N x V contiguous, overwritten with J^T qbar) with the library's scratch for its work rows: no allocation."""
+        if not hasattr(qbar, "data_ptr") or not qbar.is_contiguous() or qbar.numel() != self.N * self.V:
+            raise ValueError("step_adjoint_noalloc: a contiguous N x V device array")
+        _check(lib().luna_fv_step_adjoint_noalloc(self._h, ctypes.c_double(dt), ctypes.c_void_p(qbar.data_ptr())), "step_adjoint_noalloc")
+        return qbar
 
     def moving_adjoint(self, dt, mesh0, mesh1, qbar, xbar, steps_fixed=False):
         """FiniteVolume::step_moving_adjoint() (luna_fv_moving_adjoint; order 1, or order 0, where mesh1 is not read): the
