@@ -15,8 +15,6 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "luna-cpp_amd"))
-# The CPU baseline's thread placement, read by OpenMP when it starts (before any library here loads it).
-os.environ.setdefault("OMP_PROC_BIND", "close")
 
 import numpy as np  # noqa: E402
 
@@ -139,9 +137,10 @@ def run_moving(m, q, steps, warmup, precision="double", hll=0.0):
 
 def cpu_baseline(m, q, static_steps=40, moving_steps=10, warmup=2):
     """The oracle (CPU restatement of luna-cpp's FP64 algorithm, C + OpenMP) on the same mesh and state, on this host's
-    cores with OMP_PROC_BIND=close (set before OpenMP starts): the static step's median wall over `static_steps` after
-    `warmup`, as hydro_sedov's run line; the moving step's hydro part (mesh producer excluded) over `moving_steps`; and
-    one round of each on one thread."""
+    cores (OpenMP's own placement: OMP_PROC_BIND=close put all threads on one core of the GPU box, whose affinity mask is
+    wider than its CPU share, so it is not set here): the static step's median wall over `static_steps` after `warmup`, as
+    hydro_sedov's run line; the moving step's hydro part (mesh producer excluded) over `moving_steps`; and one round of
+    each on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 

@@ -34,6 +34,8 @@ enum class Riemann { hllc = LUNA_RIEMANN_HLLC, hll = LUNA_RIEMANN_HLL };
 enum class StepRule { arepo = LUNA_STEP_AREPO, luna = LUNA_STEP_LUNA };
 // The gradient slot of hydro::Precision (precision.hpp:41-54): Precision<>, Grad32, Grad16. A run-time choice here.
 enum class Precision { double_ = LUNA_PRECISION_DOUBLE, grad32 = LUNA_PRECISION_GRAD32, grad16 = LUNA_PRECISION_GRAD16 };
+// hydro::Update (finite_volume.hpp:25): the per-face loop or the per-cell one (fluxes_and_update).
+enum class Update { per_face = LUNA_UPDATE_PER_FACE, per_cell = LUNA_UPDATE_PER_CELL };
 
 // hydro::FiniteVolume<D> for D = 2, 3 (Update::per_face; the precision policy a constructor argument, double by default).
 template <int D>
@@ -46,7 +48,8 @@ public:
     // output contract as luna_geometry.
     FiniteVolume(const luna_geometry& mesh, const Gas& gas, int order, Limiter limiter = Limiter::face_centroid,
                  Motion motion = Motion::static_mesh, Riemann riemann = Riemann::hllc, Precision precision = Precision::double_,
-                 double hll_compression = 0)  // the HLL fallback at extreme compression (luna_hydro.h; none in the reference)
+                 double hll_compression = 0,  // the HLL fallback at extreme compression (luna_hydro.h; none in the reference)
+                 Update update = Update::per_face)
         : n_(mesh.cells) {
         luna_fv_options o{};
         o.dim = D;
@@ -60,6 +63,7 @@ public:
         o.smallc = gas.smallc;
         o.precision = static_cast<int>(precision);
         o.hll_compression = hll_compression;
+        o.update = static_cast<int>(update);
         options_ = o;
         check(luna_fv_create(&o, &mesh, &fv_), "hydro::FiniteVolume");
     }

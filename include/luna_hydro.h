@@ -33,9 +33,14 @@ enum { LUNA_WALL_REFLECTIVE = 0, LUNA_WALL_OUTFLOW = 1, LUNA_WALL_INFLOW = 2 }; 
  * power-of-two exponent per group rho, v, P). Q, W, volumes, centroids, the Riemann solve and the flux sums stay double
  * in every policy. */
 enum { LUNA_PRECISION_DOUBLE = 0, LUNA_PRECISION_GRAD32 = 1, LUNA_PRECISION_GRAD16 = 2 };
+/* hydro::Update (finite_volume.hpp:25, apply() :1539-1631): the per-face loop (flux rows, then a per-cell gather in list
+ * order) or the per-cell one (fluxes_and_update, :1544-1566: each cell computes its faces' fluxes itself, each face twice;
+ * the same bits, no flux rows). */
+enum { LUNA_UPDATE_PER_FACE = 0, LUNA_UPDATE_PER_CELL = 1 };
 
 /* hydro::Gas (euler.hpp:82-101) and the constructor's options (finite_volume.hpp:79-80). The update loop is the
- * reference's per-face one (Update::per_face): flux rows, then a deterministic per-cell gather in list order. */
+ * reference's per-face one (Update::per_face: flux rows, then a deterministic per-cell gather in list order) unless
+ * `update` asks for the per-cell one. */
 typedef struct {
     int dim;       /* 2 or 3 */
     int order;     /* 0 or 1 */
@@ -53,6 +58,8 @@ typedef struct {
                                div v = sum_a dv_a/dx_a of the limited gradient and r the cell radius. The HLLC solver then
                                takes HLL at every face with a flagged side. 0 (a zero-initialised struct): off. Needs
                                order 1, the double policy and the face-centroid limiter or none. The derivatives refuse it. */
+    int update;             /* LUNA_UPDATE_PER_FACE (a zero-initialised struct) or LUNA_UPDATE_PER_CELL (order 1; the
+                               derivatives and the adjoint's checkpoints use the per-face rows either way) */
 } luna_fv_options;
 
 /* The mesh arrays the step reads, host pointers: voronoi-cpp's output contract as finite_volume.hpp:257-327 and :353-364

@@ -432,7 +432,9 @@ int ensure_swept(luna_fv* fv) {
     return sweep(fv, 1.0);
 }
 
-int flux_and_update(luna_fv* fv, int pass, double flux_dt, double apply_dt, double unit, bool next_prims) {
+// One flux pass and its update; returns through *prims whether the update wrote the next primitives (the per-face loop
+// with next_prims), else the next time_step computes them.
+int flux_and_update(luna_fv* fv, int pass, double flux_dt, double apply_dt, double unit, bool next_prims, bool* prims = nullptr) {
     const luna::Grid* drift = pass == 0 ? &fv->grid : nullptr;
     const luna::Geo g = fv->geo();
     const luna::Rows r = fv->rows();
@@ -440,6 +442,14 @@ int flux_and_update(luna_fv* fv, int pass, double flux_dt, double apply_dt, doub
     hipStream_t s = fv->stream;
     unsigned long long* count = fv->counters.p + (pass == 1 ? 3 : 2);
     const int riemann = fv->riemann(), order = fv->opt.order;
+    if (prims) *prims = next_prims && fv->opt.update != LUNA_UPDATE_PER_CELL;
+    if (fv->opt.update == LUNA_UPDATE_PER_CELL) {
+        LUNA_HIP(timed(fv->prof, s, pass < 0 || pass == 2 ? "fluxes_and_update" : pass == 0 ? "fluxes_and_update_pass1" : "fluxes_and_update_pass2", [&] {
+            return fv->D == 3 ? luna::launch_fluxes_and_update<3>(s, g, r, gas, riemann, order, pass, flux_dt, apply_dt, unit, drift, count)
+                              : luna::launch_fluxes_and_update<2>(s, g, r, gas, riemann, order, pass, flux_dt, apply_dt, unit, drift, count);
+        }));
+        return LUNA_OK;
+    }
     LUNA_HIP(timed(fv->prof, s, pass < 0 || pass == 2 ? "flux" : pass == 0 ? "flux_pass1" : "flux_pass2", [&] {
         return fv->D == 3 ? luna::launch_flux<3>(s, g, r, gas, riemann, order, pass, flux_dt, unit, count)
                           : luna::launch_flux<2>(s, g, r, gas, riemann, order, pass, flux_dt, unit, count);
@@ -483,6 +493,10 @@ int create(const luna_fv_options* opt, const luna_geometry* mesh, luna_fv** out,
         (opt->order != 1 || opt->precision != LUNA_PRECISION_DOUBLE || opt->limiter == LUNA_LIMITER_LEARNED))
         return fail(LUNA_ERR_ARGUMENT, "luna: the HLL fallback needs order 1, the double policy and the face-centroid limiter or none");
     if (!(opt->gamma > 1)) return fail(LUNA_ERR_ARGUMENT, "luna: gamma must exceed 1");
+    if (opt->update != LUNA_UPDATE_PER_FACE && opt->update != LUNA_UPDATE_PER_CELL)
+        return fail(LUNA_ERR_ARGUMENT, "hydro::Update: per_face or per_cell");
+    if (opt->update == LUNA_UPDATE_PER_CELL && opt->order != 1)
+        return fail(LUNA_ERR_ARGUMENT, "luna: the per-cell update here steps order 1 (the order-0 fluxes read Q, which it updates in place)");
     if (mesh->cells <= 0) return fail(LUNA_ERR_ARGUMENT, "luna: no cells");
     luna_fv* fv = new luna_fv;
     fv->opt = *opt;
@@ -622,8 +636,9 @@ int luna_fv_step(luna_fv* fv, double dt) {
         return fail(LUNA_ERR_LOGIC, "luna_fv_step: a moving mesh steps with luna_fv_moving_begin / luna_fv_moving_finish");
     if (int rc = ensure_swept(fv)) return rc;
     fv->swept = false;
-    if (int rc = flux_and_update(fv, -1, dt, dt, 0.0, true)) return rc;
-    fv->prims_ready = true;
+    bool prims = false;
+    if (int rc = flux_and_update(fv, -1, dt, dt, 0.0, true, &prims)) return rc;
+    fv->prims_ready = prims;
     return LUNA_OK;
 }
 
@@ -1123,8 +1138,9 @@ int luna_fv_moving_finish(luna_fv* fv, const luna_geometry* mesh, double dt) {
         return LUNA_OK;
     }
     const double unit = fv->grid.step / dt;
-    if (int rc = flux_and_update(fv, 1, dt, 0.5 * dt, unit, true)) return rc;
-    fv->prims_ready = true;
+    bool prims = false;
+    if (int rc = flux_and_update(fv, 1, dt, 0.5 * dt, unit, true, &prims)) return rc;
+    fv->prims_ready = prims;
     fv->begun = false;
     fv->new_mesh = false;
     return LUNA_OK;

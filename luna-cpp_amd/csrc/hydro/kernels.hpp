@@ -123,6 +123,12 @@ hipError_t launch_flux(hipStream_t s, const Geo& g, const Rows& r, const Gas& ga
 // drift: a moving step's first pass, which also drifts the positions on the grid (finite_volume.hpp:1219-1231).
 template <int D>
 hipError_t launch_update(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double dt, bool next_prims, const Grid* drift);
+// Update::per_cell (finite_volume.hpp:1544-1566): the fluxes and the update in one per-cell kernel (each face's flux
+// computed by both its cells, the same bits as the per-face loop); pass, order and flux_dt as launch_flux, dt the update's,
+// drift as launch_update. It leaves the next primitives to the next time_step.
+template <int D>
+hipError_t launch_fluxes_and_update(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, int order, int pass, double flux_dt,
+                                    double dt, double unit, const Grid* drift, unsigned long long* fallbacks);
 template <int D>
 hipError_t launch_keep_centroids(hipStream_t s, const Geo& g, const Rows& r, const Grid& grid, double dt);
 // move() off the grid (order 0 on a moving mesh): x += dt w, wrapped into the box.

@@ -654,14 +654,16 @@ __device__ __forceinline__ void load_side(const Geo& g, const Rows& r, int i, Si
     load_vec<D>(row + C::C, s.c);
 }
 
-// One face's flux row: FaceFlux::operator() (:1415-1434) statically, MovingFlux::operator() (:1466-1490) on a moving step.
-template <int D, int Pass, int Riemann, int Prec = 0>
-__global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, Gas gas, int order, double dt, double unit, unsigned long long* fallbacks) {
+__device__ __forceinline__ long long wrap_key(long long k, long long p);
+
+// One face's flux times |A|: FaceFlux::operator() (:1415-1434) statically, MovingFlux::operator() (:1466-1490) on a moving
+// step; the positivity fallbacks of the owner's and the neighbour's side added to count_o and count_j.
+template <int D, int Pass, int Riemann, int Prec>
+__device__ __forceinline__ void flux_of_face(const Geo& g, const Rows& r, const Gas& gas, int order, double dt, double unit, int f, double* F,
+                                             int& count_o, int& count_j) {
     constexpr int V = D + 2;
     using FA = FaceAt<D>;
-    const int f = xcd_block() * BLOCK + threadIdx.x;
-    int count = 0;
-    if (f < g.F) {
+    {
         const double* face = g.face + (size_t)f * FACE_ROW;
         double rec[FACE_ROW - 1];
 #pragma unroll
@@ -669,7 +671,6 @@ __global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, 
         const int2 pair = *reinterpret_cast<const int2*>(face + FA::PAIR);
         const int o = pair.x, j = pair.y;
         const double magnitude = rec[FA::MAG];
-        double F[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) F[v] = 0;
         if (magnitude != 0) {
@@ -697,8 +698,8 @@ __global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, 
                     ro[k] = ro[k] + wf[k] * half;
                     rj[k] = rj[k] + wf[k] * half;
                 }
-                face_side_p<D, Prec>(gas, r, o, so.p, so.g, ro, dt, order, L, count);
-                face_side_p<D, Prec>(gas, r, j, sj.p, sj.g, rj, dt, order, R, count);
+                face_side_p<D, Prec>(gas, r, o, so.p, so.g, ro, dt, order, L, count_o);
+                face_side_p<D, Prec>(gas, r, j, sj.p, sj.g, rj, dt, order, R, count_j);
             } else {
                 double wo[D], wj[D];
 #pragma unroll
@@ -726,11 +727,11 @@ __global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, 
 #pragma unroll
                 for (int k = 0; k < D; ++k) wf[k] = (wo[k] + wj[k]) * 0.5 + sep[k] * (facv * inv_dist);
                 if constexpr (Pass == 2) {
-                    face_side_p<D, Prec>(gas, r, o, so.p, so.g, ro, dt, 0, L, count);
-                    face_side_p<D, Prec>(gas, r, j, sj.p, sj.g, rj, dt, 0, R, count);
+                    face_side_p<D, Prec>(gas, r, o, so.p, so.g, ro, dt, 0, L, count_o);
+                    face_side_p<D, Prec>(gas, r, j, sj.p, sj.g, rj, dt, 0, R, count_j);
                 } else if constexpr (Pass == 0) {
-                    pass_side_first_p<D, Prec>(r, o, so.p, so.g, ro, L, count);
-                    pass_side_first_p<D, Prec>(r, j, sj.p, sj.g, rj, R, count);
+                    pass_side_first_p<D, Prec>(r, o, so.p, so.g, ro, L, count_o);
+                    pass_side_first_p<D, Prec>(r, j, sj.p, sj.g, rj, R, count_j);
                 } else {
                     // side(i, r + c^n+1 - c^n + w dt) with the FaceFlux dt = 2 dt (:1452-1454)
                     const double step_dt = 2 * dt;
@@ -740,13 +741,13 @@ __global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, 
                         const double moved = wo[k] * (0.5 * step_dt);
                         r2[k] = ro[k] + so.c[k] - r.old_centroid[(size_t)o * D + k] + moved;
                     }
-                    face_side_p<D, Prec>(gas, r, o, so.p, so.g, r2, step_dt, 1, L, count);
+                    face_side_p<D, Prec>(gas, r, o, so.p, so.g, r2, step_dt, 1, L, count_o);
 #pragma unroll
                     for (int k = 0; k < D; ++k) {
                         const double moved = wj[k] * (0.5 * step_dt);
                         r2[k] = rj[k] + sj.c[k] - r.old_centroid[(size_t)j * D + k] + moved;
                     }
-                    face_side_p<D, Prec>(gas, r, j, sj.p, sj.g, r2, step_dt, 1, R, count);
+                    face_side_p<D, Prec>(gas, r, j, sj.p, sj.g, r2, step_dt, 1, R, count_j);
                 }
             }
             if constexpr (Riemann == 1)
@@ -761,10 +762,63 @@ __global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, 
 #pragma unroll
             for (int v = 0; v < V; ++v) F[v] *= magnitude;
         }
+    }
+}
+
+// One face's flux row per work-item (the per-face loop, Update::per_face).
+template <int D, int Pass, int Riemann, int Prec = 0>
+__global__ __launch_bounds__(BLOCK, LUNA_FLUX_WAVES) void k_flux(Geo g, Rows r, Gas gas, int order, double dt, double unit, unsigned long long* fallbacks) {
+    constexpr int V = D + 2;
+    const int f = xcd_block() * BLOCK + threadIdx.x;
+    int count = 0;
+    if (f < g.F) {
+        double F[V];
+        int co = 0, cj = 0;
+        flux_of_face<D, Pass, Riemann, Prec>(g, r, gas, order, dt, unit, f, F, co, cj);
+        count = co + cj;
 #pragma unroll
         for (int v = 0; v < V; ++v) r.flux[(size_t)f * V + v] = F[v];
     }
     block_count<BLOCK>(count, fallbacks);
+}
+
+// finite_volume.hpp:1544-1566 fluxes_and_update (Update::per_cell): each cell computes the flux of every face in its list
+// (each face twice, once from each side, with the same arithmetic, so the same bits) and Q -= dt sum_k sign_k F in list order,
+// in place (the fluxes read only the primitive and gradient rows); each side's fallbacks are counted by its own cell. No
+// flux rows; the next primitives are left to the next time_step (the fluxes of the other cells still read this cell's).
+template <int D, int Pass, int Riemann, int Prec, bool Drift>
+__global__ __launch_bounds__(CELL_BLOCK) void k_fluxes_and_update(Geo g, Rows r, Gas gas, int order, double flux_dt, double dt, double unit,
+                                                                  Grid grid, unsigned long long* fallbacks) {
+    constexpr int V = D + 2;
+    const int i = xcd_block() * CELL_BLOCK + threadIdx.x;
+    int count = 0;
+    if (i < g.N) {
+        double sum[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) sum[v] = 0;
+        const int n = g.count[i], base = first_slot(g, i);
+        for (int t = 0; t < n; ++t) {
+            const int e = g.list[base + t * SLICE], f = e >= 0 ? e : ~e;
+            double F[V];
+            int co = 0, cj = 0;
+            flux_of_face<D, Pass, Riemann, Prec>(g, r, gas, order, flux_dt, unit, f, F, co, cj);
+            count += e >= 0 ? co : cj;
+            const double sign = e >= 0 ? 1.0 : -1.0;
+#pragma unroll
+            for (int v = 0; v < V; ++v) sum[v] += sign * F[v];
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) r.q[(size_t)i * V + v] = r.q[(size_t)i * V + v] - dt * sum[v];
+        if constexpr (Drift) {  // finite_volume.hpp:1219-1231, as k_update
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                const size_t at = (size_t)i * D + a;
+                const long long k = (long long)round(r.x[at] * grid.per_step) + r.steps[at];
+                r.x[at] = (double)wrap_key(k, grid.period[a]) * grid.step;
+            }
+        }
+    }
+    block_count<CELL_BLOCK>(count, fallbacks);
 }
 
 // k mod p in [0, p): one conditional step for a key within a period of the box (every position and move here), the
@@ -953,6 +1007,44 @@ hipError_t launch_flux(hipStream_t s, const Geo& g, const Rows& r, const Gas& ga
     return flux_pass<D, 1>(s, g, r, gas, riemann, order, dt, unit, fallbacks);
 }
 
+template <int D, int Pass, int Riemann, int Prec>
+static hipError_t per_cell_drift(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int order, double flux_dt, double dt, double unit,
+                                 const Grid* drift, unsigned long long* fallbacks) {
+    const dim3 grid = blocks(g.N > 0 ? g.N : 1, CELL_BLOCK);
+    const Grid gr = drift ? *drift : Grid{};
+    if (drift)
+        hipLaunchKernelGGL((k_fluxes_and_update<D, Pass, Riemann, Prec, true>), grid, dim3(CELL_BLOCK), 0, s, g, r, gas, order, flux_dt, dt, unit,
+                           gr, fallbacks);
+    else
+        hipLaunchKernelGGL((k_fluxes_and_update<D, Pass, Riemann, Prec, false>), grid, dim3(CELL_BLOCK), 0, s, g, r, gas, order, flux_dt, dt, unit,
+                           gr, fallbacks);
+    return hipGetLastError();
+}
+template <int D, int Pass, int Prec>
+static hipError_t per_cell_riemann(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, int order, double flux_dt, double dt,
+                                   double unit, const Grid* drift, unsigned long long* fallbacks) {
+    if (riemann == 1) return per_cell_drift<D, Pass, 1, Prec>(s, g, r, gas, order, flux_dt, dt, unit, drift, fallbacks);
+    if (Prec == 0 && riemann == 2) return per_cell_drift<D, Pass, 2, 0>(s, g, r, gas, order, flux_dt, dt, unit, drift, fallbacks);
+    return per_cell_drift<D, Pass, 0, Prec>(s, g, r, gas, order, flux_dt, dt, unit, drift, fallbacks);
+}
+template <int D, int Pass>
+static hipError_t per_cell_prec(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, int order, double flux_dt, double dt,
+                                double unit, const Grid* drift, unsigned long long* fallbacks) {
+    if (order == 1 && r.precision == 1) return per_cell_riemann<D, Pass, 1>(s, g, r, gas, riemann, order, flux_dt, dt, unit, drift, fallbacks);
+    if (order == 1 && r.precision == 2) return per_cell_riemann<D, Pass, 2>(s, g, r, gas, riemann, order, flux_dt, dt, unit, drift, fallbacks);
+    return per_cell_riemann<D, Pass, 0>(s, g, r, gas, riemann, order, flux_dt, dt, unit, drift, fallbacks);
+}
+
+template <int D>
+hipError_t launch_fluxes_and_update(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, int order, int pass, double flux_dt,
+                                    double dt, double unit, const Grid* drift, unsigned long long* fallbacks) {
+    if (g.N == 0) return hipSuccess;
+    if (pass < 0) return per_cell_prec<D, -1>(s, g, r, gas, riemann, order, flux_dt, dt, unit, nullptr, fallbacks);
+    if (pass == 0) return per_cell_prec<D, 0>(s, g, r, gas, riemann, order, flux_dt, dt, unit, drift, fallbacks);
+    if (pass == 2) return per_cell_prec<D, 2>(s, g, r, gas, riemann, 0, flux_dt, dt, unit, nullptr, fallbacks);
+    return per_cell_prec<D, 1>(s, g, r, gas, riemann, order, flux_dt, dt, unit, nullptr, fallbacks);
+}
+
 template <int D>
 hipError_t launch_update(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double dt, bool next_prims, const Grid* drift) {
     if (g.N == 0) return hipSuccess;
@@ -980,6 +1072,8 @@ hipError_t launch_keep_centroids(hipStream_t s, const Geo& g, const Rows& r, con
     template hipError_t launch_flux<D>(hipStream_t, const Geo&, const Rows&, const Gas&, int, int, int, double, double,              \
                                        unsigned long long*);                                                                        \
     template hipError_t launch_update<D>(hipStream_t, const Geo&, const Rows&, const Gas&, double, bool, const Grid*);              \
+    template hipError_t launch_fluxes_and_update<D>(hipStream_t, const Geo&, const Rows&, const Gas&, int, int, int, double, double, double, \
+                                                    const Grid*, unsigned long long*);                                            \
     template hipError_t launch_keep_centroids<D>(hipStream_t, const Geo&, const Rows&, const Grid&, double);                      \
     template hipError_t launch_drift<D>(hipStream_t, const Geo&, const Rows&, double);
 

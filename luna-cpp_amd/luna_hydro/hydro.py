@@ -26,6 +26,7 @@ MOTION = {"static": 0, "moving": 1}
 RULES = {"arepo": 0, "luna": 1}
 WALLS = {"reflective": 0, "outflow": 1, "inflow": 2}
 PRECISIONS = {"double": 0, "grad32": 1, "grad16": 2}  # hydro::Precision's gradient slot (precision.hpp:41-54)
+UPDATES = {"per_face": 0, "per_cell": 1}  # hydro::Update (finite_volume.hpp:25)
 
 
 def policy(name, dim):
@@ -42,7 +43,7 @@ class FVOptions(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("limiter", ctypes.c_int), ("riemann", ctypes.c_int),
                 ("motion", ctypes.c_int), ("step_rule", ctypes.c_int), ("gamma", ctypes.c_double),
                 ("smallr", ctypes.c_double), ("smallc", ctypes.c_double), ("precision", ctypes.c_int),
-                ("hll_compression", ctypes.c_double)]
+                ("hll_compression", ctypes.c_double), ("update", ctypes.c_int)]
 
 
 class Geometry(ctypes.Structure):
@@ -183,11 +184,11 @@ class FiniteVolume:
     mesh: a mesh dict (host arrays) or a DeviceMesh (read in place on the GPU)."""
 
     def __init__(self, mesh, gamma, order=1, limiter="face", riemann="hllc", motion="static", step_rule="arepo",
-                 smallr=1e-10, smallc=1e-10, precision="double", hll_compression=0.0):
+                 smallr=1e-10, smallc=1e-10, precision="double", hll_compression=0.0, update="per_face"):
         self.D, self.N = mesh["dim"], mesh["cells"]
         self.V = self.D + 2
         opt = FVOptions(self.D, order, LIMITERS[limiter], RIEMANN[riemann], MOTION[motion], RULES[step_rule],
-                        gamma, smallr, smallc, PRECISIONS[precision], hll_compression)
+                        gamma, smallr, smallc, PRECISIONS[precision], hll_compression, UPDATES[update])
         h = ctypes.c_void_p()
         if isinstance(mesh, DeviceMesh):
             self._mesh_arrays = mesh

@@ -1030,6 +1030,10 @@ void orc_moving_finish(orc_fv* fv, const orc_geometry* geo, double dt) {
 
 /* ------------------------------------------------------------------ D = 1 on a sorted line ---------------------------------- */
 
+/* A line this short runs on one thread: its loops are microseconds, below OpenMP's fork cost (the results do not depend
+ * on the thread count: each slot is computed alone, and the reductions are a minimum and an integer count). */
+#define LINE_PARALLEL 16384
+
 /* sorted_mesh::Line as finite_volume_1d.hpp reads it: gaps in grid steps between neighbouring generators, a wall's gap
  * twice the distance to it (the mirror ghost, :152-155, :636-653). */
 static long long line_gap(const orc_line* l, int s) {
@@ -1341,7 +1345,7 @@ static double line_body_moving(const orc_line* l, int s, double dt, double coura
  * is |w - v| of the point velocity, from the cell's density gradient. */
 double orc_line_time_step(orc_line* l, double courant, int luna_rule) {
     double smallest = INFINITY;
-#pragma omp parallel for reduction(min : smallest)
+#pragma omp parallel for reduction(min : smallest) if (l->n >= LINE_PARALLEL)
     for (int s = 0; s < l->n; ++s) {
         double Wi[3];
         line_primitive(l, s, Wi);
@@ -1385,7 +1389,7 @@ static void line_swap(orc_line* l) {
 
 /* finite_volume_1d.hpp:389-407 step() (static): one kernel over the slots, then Q and next swap. */
 void orc_line_step(orc_line* l, double dt) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (l->n >= LINE_PARALLEL)
     for (int s = 0; s < l->n; ++s) line_body(l, s, dt, 0.0, 0);
     line_swap(l);
 }
@@ -1393,7 +1397,7 @@ void orc_line_step(orc_line* l, double dt) {
 /* finite_volume_1d.hpp:412-417, 679-708 step_and_time_step() (static): the step and the next dt by LUNA's rule. */
 double orc_line_step_fused(orc_line* l, double dt, double courant) {
     double smallest = INFINITY;
-#pragma omp parallel for reduction(min : smallest) schedule(static)
+#pragma omp parallel for reduction(min : smallest) schedule(static) if (l->n >= LINE_PARALLEL)
     for (int s = 0; s < l->n; ++s) smallest = fmin(smallest, line_body(l, s, dt, courant, 1));
     line_swap(l);
     return smallest;
@@ -1638,7 +1642,7 @@ static int mline_kernel(orc_mline* m, double dt, double courant, int fused, doub
     orc_line* l = &m->l;
     int descents = 0;
     double least = INFINITY;
-#pragma omp parallel for reduction(min : least) reduction(+ : descents) schedule(static)
+#pragma omp parallel for reduction(min : least) reduction(+ : descents) schedule(static) if (l->n >= LINE_PARALLEL)
     for (int s = 0; s < l->n; ++s) {
         int d = 0;
         least = fmin(least, line_body_moving(l, s, dt, courant, fused, m->x, m->x_next, m->keys_next, &d));
@@ -2370,7 +2374,7 @@ static void line_load_tangent(const orc_line* l, const double* dq, int s, double
  * advanced in place (dq: a tangent of the extensive state). */
 void orc_line_step_tangent(orc_line* l, double dt, double* dq) {
     double* dnext = (double*)malloc(sizeof(double) * 3 * (size_t)l->n);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (l->n >= LINE_PARALLEL)
     for (int s = 0; s < l->n; ++s) {
         line_cell cell[5];
         double gap[4], dW[5][3], F0[3], F1[3], dF0[3], dF1[3];
@@ -2394,7 +2398,7 @@ void orc_line_step_tangent(orc_line* l, double dt, double* dq) {
 void orc_line_step_adjoint(orc_line* l, double dt, double* qbar) {
     const int n = l->n;
     double* out = (double*)malloc(sizeof(double) * 3 * (size_t)n);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (n >= LINE_PARALLEL)
     for (int j = 0; j < n; ++j) {
         double Wj_bar[3] = {0, 0, 0};
         for (int d = -1; d <= 3; ++d) {
