@@ -112,79 +112,121 @@ __global__ __launch_bounds__(BLOCK) void k_geo_faces(MeshIn m, double* face, uns
     for (int k = 0; k < FACE_ROW; ++k) out[k] = row[k];
 }
 
-// One work-group of SLOT_WAVES wavefronts per slice: lane = cell i mod 64, wavefront w takes the entries t = w, w + SLOT_WAVES,
-// ... of the slice's cells, so each work-item walks a few entries and the loads of different entries are in flight
-// together; entry t of cell i goes to slot slice[g] + 64 t + lane (consecutive lanes: consecutive slots).
-constexpr int SLOT_WAVES = 4;
+// One work-group per quarter slice: 16 cells, whose list entries are one contiguous run of the CSR list. The work-items
+// take consecutive entries (so the list and the faces a cell owns, numbered consecutively, are read in runs); each finds
+// its cell among the 16 offsets and computes its slot's values into LDS, laid out as the slots are; then the work-group
+// writes each (entry t, array) as one whole run of 16 slots (one 128-byte line for the doubles). Padding slots (t at or
+// past the cell's list length) get zeros. A quarter deeper than QUARTER_DEPTH entries writes the rest straight to memory.
+// The CSR offsets were checked by k_geo_cells (the host runs this only when they passed).
+constexpr int SLOT_BLOCK = 256, QUARTER = 16, QUARTER_DEPTH = 48;
 template <int D>
-__global__ __launch_bounds__(SLICE* SLOT_WAVES) void k_geo_slots(MeshIn m, const int* __restrict__ count, const int* __restrict__ slice,
-                                                                  const long long* __restrict__ info, int S, int* __restrict__ list,
-                                                                  int* __restrict__ other, double* __restrict__ slot, unsigned int* flags) {
-    const int g = blockIdx.x, lane = threadIdx.x & (SLICE - 1), wave = threadIdx.x / SLICE;
-    const int i = g * SLICE + lane;
+__global__ __launch_bounds__(SLOT_BLOCK) void k_geo_slots(MeshIn m, const int* __restrict__ count, const int* __restrict__ slice,
+                                                          const long long* __restrict__ info, int S, int* __restrict__ list,
+                                                          int* __restrict__ other, double* __restrict__ slot, unsigned int* flags) {
+    __shared__ int offs[QUARTER + 1];
+    __shared__ int s_list[QUARTER_DEPTH * QUARTER], s_other[QUARTER_DEPTH * QUARTER];
+    __shared__ double s_geo[D + 1][QUARTER_DEPTH * QUARTER];
+    const int g = blockIdx.x >> 2, quarter = blockIdx.x & 3;
+    const int c0 = g * SLICE + quarter * QUARTER, cells = max(0, min(QUARTER, m.N - c0));
+    const int tid = threadIdx.x;
+    if (cells > 0)
+        for (int l = tid; l <= cells; l += SLOT_BLOCK) offs[l] = m.start[c0 + l];
     const long long slots = info[0];
     const long long base = slice[g];
     const long long end = g + 1 < S ? (long long)slice[g + 1] : slots;
     const int depth = (int)((end - base) / SLICE);
-    const int n = i < m.N ? count[i] : 0;
-    const int first = n > 0 ? m.start[i] : 0;
+    const int staged = min(depth, QUARTER_DEPTH);
+    for (int p = tid; p < staged * QUARTER; p += SLOT_BLOCK) {
+        s_list[p] = 0;
+        s_other[p] = 0;
+#pragma unroll
+        for (int a = 0; a <= D; ++a) s_geo[a][p] = 0.0;
+    }
+    __syncthreads();
+    const int k0 = cells > 0 ? offs[0] : 0, k1 = cells > 0 ? offs[cells] : 0;
     const int* __restrict__ in_list = m.list;
     const int* __restrict__ owner = m.owner;
     const int* __restrict__ neighbour = m.neighbour;
     const double* __restrict__ area = m.area;
     const double* __restrict__ fc = m.face_centroid;
     const double* __restrict__ pos = m.positions;
-    double xi[D], ci[D];
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-        xi[a] = i < m.N ? pos[(size_t)i * D + a] : 0.0;
-        ci[a] = i < m.N ? m.centroid[(size_t)i * D + a] : 0.0;
-    }
+    const double* __restrict__ cen = m.centroid;
     unsigned int bad = 0;
-    for (int t = wave; t < depth; t += SLOT_WAVES) {
-        const long long s = base + (long long)t * SLICE + lane;
-        int e = 0, cell = 0;
+    for (int k = k0 + tid; k < k1; k += SLOT_BLOCK) {
+        int lane = 0;  // the last l with offs[l] <= k
+#pragma unroll
+        for (int step = QUARTER / 2; step > 0; step >>= 1)
+            if (lane + step < cells && offs[lane + step] <= k) lane += step;
+        const int i = c0 + lane, t = k - offs[lane];
+        int e = in_list[k];
+        const int f = e >= 0 ? e : ~e;
+        int cell = 0;
         double mag = 0.0, off[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) off[a] = 0.0;
-        if (t < n) {
-            e = in_list[first + t];
-            const int f = e >= 0 ? e : ~e;
-            if (f >= m.F) {
-                bad |= GEO_BAD_FACE;
-                e = 0;
+        if (f >= m.F) {
+            bad |= GEO_BAD_FACE;
+            e = 0;
+        } else {
+            const int o = owner[f], j = neighbour[f];
+            if (o < 0 || o >= m.N || j < 0 || j >= m.N) {
+                bad |= GEO_BAD_CELL;
             } else {
-                const int o = owner[f], j = neighbour[f];
-                if (o < 0 || o >= m.N || j < 0 || j >= m.N) {
-                    bad |= GEO_BAD_CELL;
-                } else {
-                    cell = e >= 0 ? j : o;
-                    double A[D], rel[D];
+                cell = e >= 0 ? j : o;
+                double A[D], rel[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    A[a] = area[(size_t)f * D + a];
+                    rel[a] = fc[(size_t)f * D + a];
+                }
+                mag = sqrt(dot<D>(A, A));
+                // Geometry::face_offset (finite_volume.hpp:322-326): the centroid moved to this cell's generator by the
+                // nearest-image separation when the cell does not own the face, then relative to its centroid
+                if (e < 0) {
 #pragma unroll
                     for (int a = 0; a < D; ++a) {
-                        A[a] = area[(size_t)f * D + a];
-                        rel[a] = fc[(size_t)f * D + a];
+                        const double d = pos[(size_t)o * D + a] - pos[(size_t)i * D + a];
+                        rel[a] = rel[a] + (d - m.side[a] * round(d * m.inverse[a]));
                     }
-                    mag = sqrt(dot<D>(A, A));
-                    // Geometry::face_offset (finite_volume.hpp:322-326): the centroid moved to this cell's generator by
-                    // the nearest-image separation when the cell does not own the face, then relative to its centroid
-                    if (e < 0) {
-#pragma unroll
-                        for (int a = 0; a < D; ++a) {
-                            const double d = pos[(size_t)o * D + a] - xi[a];
-                            rel[a] = rel[a] + (d - m.side[a] * round(d * m.inverse[a]));
-                        }
-                    }
-#pragma unroll
-                    for (int a = 0; a < D; ++a) off[a] = rel[a] - ci[a];
                 }
+#pragma unroll
+                for (int a = 0; a < D; ++a) off[a] = rel[a] - cen[(size_t)i * D + a];
             }
         }
-        list[s] = e;
-        other[s] = cell;
-        slot[s] = mag;
+        if (t < QUARTER_DEPTH) {
+            const int p = t * QUARTER + lane;
+            s_list[p] = e;
+            s_other[p] = cell;
+            s_geo[0][p] = mag;
 #pragma unroll
-        for (int a = 0; a < D; ++a) slot[(a + 1) * slots + s] = off[a];
+            for (int a = 0; a < D; ++a) s_geo[1 + a][p] = off[a];
+        } else {  // beyond the staging: straight to memory
+            const long long sl = base + (long long)t * SLICE + quarter * QUARTER + lane;
+            list[sl] = e;
+            other[sl] = cell;
+            slot[sl] = mag;
+#pragma unroll
+            for (int a = 0; a < D; ++a) slot[(a + 1) * slots + sl] = off[a];
+        }
+    }
+    __syncthreads();
+    // the staged slots, a run of 16 per entry and array; then the padding past the staging
+    for (int p = tid; p < staged * QUARTER; p += SLOT_BLOCK) {
+        const int t = p / QUARTER, l = p % QUARTER;
+        const long long sl = base + (long long)t * SLICE + quarter * QUARTER + l;
+        list[sl] = s_list[p];
+        other[sl] = s_other[p];
+        slot[sl] = s_geo[0][p];
+#pragma unroll
+        for (int a = 0; a < D; ++a) slot[(a + 1) * slots + sl] = s_geo[1 + a][p];
+    }
+    for (int p = staged * QUARTER + tid; p < depth * QUARTER; p += SLOT_BLOCK) {
+        const int t = p / QUARTER, l = p % QUARTER;
+        const int n = l < cells ? offs[l + 1] - offs[l] : 0;
+        if (t < n) continue;
+        const long long sl = base + (long long)t * SLICE + quarter * QUARTER + l;
+        list[sl] = 0;
+        other[sl] = 0;
     }
     if (bad) flag(flags, bad);
 }
@@ -216,7 +258,7 @@ hipError_t launch_geometry_slots(hipStream_t s, const MeshIn& m, const int* coun
                                  double* slot, unsigned int* flags) {
     const int S = (m.N + SLICE - 1) / SLICE;
     if (S == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_geo_slots<D>, dim3(S), dim3(SLICE * SLOT_WAVES), 0, s, m, count, slice, info, S, list, other, slot, flags);
+    hipLaunchKernelGGL(k_geo_slots<D>, dim3(S * (SLICE / QUARTER)), dim3(SLOT_BLOCK), 0, s, m, count, slice, info, S, list, other, slot, flags);
     return hipGetLastError();
 }
 

@@ -409,3 +409,31 @@ def test_galilean_blast_on_the_oracle():
     # mesh's profile error over 1e3 times the moving one's
     assert r["cells"] < 1e-6, r
     assert r["moving"] < 1e-10 and r["static"] > 1e3 * r["moving"], r
+
+
+# ----------------------------------------------------------------------------------------------- the cell radius
+
+
+def test_cbrt_is_correctly_rounded():
+    """The radius' cube root (finite_volume.hpp:241-246), correctly rounded on both sides: equal to the cube root computed
+    with 60 significant digits (Python's decimal module) and rounded to the nearest double, on volumes across the range a
+    mesh gives and at exact cubes; the C library's cbrt misses it on some of them."""
+    import ctypes
+    from decimal import Decimal, getcontext
+
+    getcontext().prec = 60
+    fn = oracle.lib().orc_cbrt_rn
+    fn.restype = ctypes.c_double
+    fn.argtypes = [ctypes.c_double]
+    rng = rng_of(71)
+    values = np.concatenate([10.0 ** rng.uniform(-12, 2, 4000), rng.uniform(0.5, 1.0, 2000) * 2.0 ** -21,
+                             np.array([1.0, 8.0, 27.0, 0.125, 2.0 ** -30, 3.0 * 2.0 ** -21 / (4 * np.pi)])])
+    third = Decimal(1) / Decimal(3)
+    library_misses = 0
+    for v in values:
+        exact = float(Decimal(float(v)) ** third)
+        got = fn(float(v))
+        assert got == exact, (v, got, exact)
+        library_misses += np.cbrt(v) != exact
+    assert fn(0.0) == 0.0 and fn(8.0) == 2.0
+    print("cbrt_rn: %d values correctly rounded; numpy's cbrt misses %d of them" % (len(values), library_misses))
