@@ -38,6 +38,13 @@ ALGO_BYTES = {
     "moving": {"sweep": 816.8, "keep_centroids": 84.0, "flux_pass1": 983.2, "update_pass1": 525.6, "flux_pass2": 1007.2,
                "update_pass2": 465.6},
 }
+# The geometry pass of a moving step (the new mesh's derived layouts from the producer's arrays; F/N = 7.77 and E/N =
+# 15.55 on the jittered 128^3 mesh):
+#   cells:  the producer's V, c, x and list start in (60), the 64 B geometry half of the cell row and the count out -> 128
+#   faces:  owner, neighbour, A, face centroid in (56 a face), the 64 B face record out -> 932
+#   slots:  the producer's list (62), the used fields of each face record once (40 a face, 311), the cells' x and c once
+#           (48), 40 B per list entry out (the face id with its owner bit, the other cell, |A|, the face offset; 622) -> 1043
+GEO_BYTES = {"geometry_cells": 128.0, "geometry_faces": 932.0, "geometry_slots": 1043.0}
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_sedov128.md")
 
@@ -69,8 +76,8 @@ def dominant(times, table):
     return best, launches, total / launches
 
 
-def run_static(m, q, steps, warmup, precision="double", hll=0.0):
-    fv = hydro.FiniteVolume(m, GAMMA, precision=precision, hll_compression=hll)
+def run_static(m, q, steps, warmup, precision="double", hll=0.0, update="per_face"):
+    fv = hydro.FiniteVolume(m, GAMMA, precision=precision, hll_compression=hll, update=update)
     fv.set_state(q)
     for _ in range(warmup):
         fv.step(fv.time_step(COURANT))
@@ -92,13 +99,14 @@ def run_static(m, q, steps, warmup, precision="double", hll=0.0):
     return total, walls, times
 
 
-def run_moving(m, q, steps, warmup, precision="double", hll=0.0):
+def run_moving(m, q, steps, warmup, precision="double", hll=0.0, update="per_face"):
     """Moving steps; the hydro part is timed as the reference's: time_step, the first pass and drift, the new mesh's
     geometry pass (luna_fv_set_geometry_device on the producer's arrays in device memory, as the reference reads
     voronoi-cpp's device views in its sweep and primitives), the second pass. The mesh producer (host) and the copies
     that stand in for a GPU producer's output (positions to the host, the new mesh's arrays to the device) are excluded
     and reported apart."""
-    fv = hydro.FiniteVolume(hydro.DeviceMesh(m), GAMMA, motion="moving", precision=precision, hll_compression=hll)
+    fv = hydro.FiniteVolume(hydro.DeviceMesh(m), GAMMA, motion="moving", precision=precision, hll_compression=hll,
+                            update=update)
     fv.set_state(q)
     mesh_s, hydro_s, geo_s = [], [], []
 
@@ -294,6 +302,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true", help="skip the grad32/grad16 policy runs")
     ap.add_argument("--no-derivatives", action="store_true", help="skip the derivative cost runs")
     ap.add_argument("--no-sod", action="store_true", help="skip the 1D Sod run")
+    ap.add_argument("--no-update-modes", action="store_true", help="skip the per-cell update (Update::per_cell) runs")
     ap.add_argument("--hll-compression", type=float, default=0.0,
                     help="the HLL fallback at extreme compression in the FP64 runs (threshold; 0: off, the reference's scheme)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -355,6 +364,19 @@ def main():
                            "moving_kernels_ms_avg": {k: v[1] / v[0] for k, v in k_m.items()}}
             log("%s: static %.3g, moving %.3g cell updates/s" % (p, policies[p]["static"], policies[p]["moving"]))
 
+    modes = None
+    if not args.no_update_modes:
+        # The reference's other update schedule (Update::per_cell, hydro::fluxes_and_update): each cell computes the
+        # fluxes of all its faces and updates only itself; the same bits as per_face.
+        t_s, _, k_s = run_static(m, q, args.static_steps, args.warmup, update="per_cell")
+        t_m, h_m, _, k_m, _ = run_moving(m, q, max(2, args.steps // 2), args.warmup, update="per_cell")
+        h_total = float(np.sum(h_m))
+        modes = {"per_cell": {"static": N * args.static_steps / t_s, "static_ms_per_step": 1e3 * t_s / args.static_steps,
+                              "moving": N * len(h_m) / h_total, "moving_hydro_ms_per_step": 1e3 * h_total / len(h_m),
+                              "static_kernels_ms_avg": {k: v[1] / v[0] for k, v in k_s.items()},
+                              "moving_kernels_ms_avg": {k: v[1] / v[0] for k, v in k_m.items()}}}
+        log("per_cell update: static %.3g, moving %.3g cell updates/s" % (modes["per_cell"]["static"], modes["per_cell"]["moving"]))
+
     derivs = None
     if not args.no_derivatives:
         log("derivative costs of the static step (forward, tangent, adjoint)")
@@ -394,11 +416,16 @@ def main():
         "cpu_baseline": cpu,
         "learned_vs_plain": learned_cmp,
         "precision_policies": policies,
+        "update_modes": modes,
         "step_roofline": {"algorithmic_bytes_per_cell": step_bytes,
                           "achieved_GBs": step_bytes * N * args.steps / hydro_total / 1e9,
                           "frac": step_bytes * N * args.steps / hydro_total / 1e9 / HBM_PEAK_GBS},
         "mesh": {"build_s_first": build_s, "producer_and_copies_s_per_step": float(np.mean(mesh_s)),
                  "geometry_pass_ms_per_step": 1e3 * float(np.mean(geo_s)),
+                 "geometry_roofline": {k: {"algorithmic_bytes_per_cell": b, "avg_ms": m_times[k][1] / m_times[k][0],
+                                           "achieved_GBs": b * N / (m_times[k][1] / m_times[k][0] * 1e-3) / 1e9,
+                                           "frac": b * N / (m_times[k][1] / m_times[k][0] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                                       for k, b in GEO_BYTES.items() if k in m_times},
                  "note": "value includes the geometry pass on the device (luna_fv_set_geometry_device: cell rows, face "
                          "records, slot lists with |A| and face offsets, radii); excluded: the Voronoi producer (C++/OpenMP, "
                          "host) and the copies standing in for a GPU producer (positions to the host, its arrays to the device)"},

@@ -432,3 +432,75 @@ def speedup_at_equal_error(rows):
                 break
         out[r["n"]] = val
     return out
+
+
+# ------------------------------------------------------------------------------------------------ command line
+
+PROBLEMS = {"kh": KH, "kh-smooth": KH_SMOOTH, "sedov": SEDOV}
+
+
+def main(argv=None):
+    """python -m luna_hydro.learned train [--out FILE.npy] [--steps K] [--quick]
+       python -m luna_hydro.learned curve --problem kh|kh-smooth|sedov --ns 32,64 --t-end T --ref-n R [--moving] [--params FILE.npy]
+
+    train: a network on the shipped one's problem mix (coarse/fine pairs of the smooth KH layer at 64^2 x 4 seeds and
+    128^2 x 2, the sharp one at 64^2 x 2, Sedov at 32^3 x 3, each against a plain run at 4x the resolution, states at
+    t = 0, 1, 2 (KH) and 0, 0.02, 0.05 (Sedov); 8000 Adam steps, batch 65536, lr 1e-3, seed 0); --quick: one 64^2
+    smooth KH seed and 500 steps. curve: plain and learned runs at each n
+    against a plain static run at ref_n, with the GPU-seconds and the speed-up at equal error. Prints one JSON line."""
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser(description=main.__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("what", choices=["train", "curve"])
+    ap.add_argument("--out", default="")
+    ap.add_argument("--steps", type=int, default=8000)
+    ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--problem", choices=sorted(PROBLEMS), default="kh-smooth")
+    ap.add_argument("--ns", default="32,64")
+    ap.add_argument("--t-end", type=float, default=1.0)
+    ap.add_argument("--ref-n", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=99)
+    ap.add_argument("--moving", action="store_true")
+    ap.add_argument("--params", default="")
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    if a.what == "train":
+        t0 = time.perf_counter()
+        if a.quick:
+            sets = [(KH_SMOOTH, 64, (1,), (0.5, 1.0))]
+            steps = min(a.steps, 500)
+        else:
+            sets = [(KH_SMOOTH, 64, (1, 2, 3, 4), (1.0, 2.0)), (KH_SMOOTH, 128, (5, 6), (1.0, 2.0)),
+                    (KH, 64, (7, 8), (1.0, 2.0)), (SEDOV, 32, (9, 10, 11), (0.02, 0.05))]
+            steps = a.steps
+        xs, ts, ws = [], [], []
+        for problem, n, seeds, marks in sets:
+            x, t, w = pairs(problem, n, seeds, marks, log=log)
+            xs.append(x)
+            ts.append(t)
+            ws.append(w)
+        x, t, w = np.concatenate(xs), np.concatenate(ts), np.concatenate(ws)
+        t1 = time.perf_counter()
+        log("%d rows in %.1f s; training %d steps" % (len(t), t1 - t0, steps))
+        params, history = train(x, t, w, steps=steps, log=log)
+        t2 = time.perf_counter()
+        if a.out:
+            np.save(a.out, params)
+        print(json.dumps({"rows": int(len(t)), "steps": steps, "pairs_s": t1 - t0, "train_s": t2 - t1,
+                          "loss": [[int(k), float(v)] for k, v in history]}), flush=True)
+        return 0
+    params = np.load(a.params, allow_pickle=False) if a.params else default_params()
+    rows, ref = curve(PROBLEMS[a.problem], [int(s) for s in a.ns.split(",")], a.seed, a.t_end, params, a.ref_n,
+                      motion="moving" if a.moving else "static", log=log)
+    print(json.dumps({"problem": a.problem, "t_end": a.t_end, "motion": "moving" if a.moving else "static",
+                      "reference": ref, "rows": rows, "speedup_at_equal_error": speedup_at_equal_error(rows)}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -37,6 +37,34 @@ def totals(q):
             "energy": math.fsum(q[:, 4])}
 
 
+# The Sedov-Taylor radius R = xi0 (E t^2 / rho)^(1/5) of a point blast, xi0 = 1.15167 for gamma = 5/3 (Sedov 1959;
+# the constant of Kamm & Timmes's solution for a spherical blast in a uniform medium).
+XI0 = 1.15167
+
+
+def shock_radius(q, x, volume, h, t, energy=1.0, rho=1.0):
+    """(the radius of the peak of the shell-averaged density, the radius beyond it where the average falls to half way
+    between the peak and the ambient density, the Sedov-Taylor radius) of the blast about (1/2, 1/2, 1/2): shells of
+    width h, volume-weighted averages of rho = Q_0 / V at the generators' periodic distances."""
+    d = x - 0.5
+    d -= np.round(d)
+    r = np.sqrt(np.sum(d * d, axis=1))
+    shell = np.minimum((r / h).astype(int), int(0.5 / h))
+    mass = np.bincount(shell, weights=q[:, 0])
+    vol = np.bincount(shell, weights=volume)
+    avg = np.where(vol > 0, mass / np.maximum(vol, 1e-300), rho)
+    k = int(np.argmax(avg))
+    half = rho + 0.5 * (avg[k] - rho)
+    j = k
+    while j + 1 < len(avg) and avg[j + 1] > half:
+        j += 1
+    # the half-way crossing between shells j and j + 1, linearly
+    front = (j + 0.5) * h
+    if j + 1 < len(avg) and avg[j] != avg[j + 1]:
+        front = (j + 0.5 + (avg[j] - half) / (avg[j] - avg[j + 1])) * h
+    return (k + 0.5) * h, front, XI0 * (energy * t * t / rho) ** 0.2
+
+
 def commit():
     try:
         here = os.path.dirname(os.path.abspath(__file__))
@@ -126,6 +154,8 @@ def run(n=64, moving=False, t_end=0.05, courant=0.3, snapshots=10, out="", max_s
         after["px"] - before["px"], after["py"] - before["py"], after["pz"] - before["pz"], scale))
     x = fv.positions() if moving else np.asarray(current["positions"]).reshape(-1, 3)
     log("checksum: steps=%d Q=%x x=%x volume=%x" % (steps, bits_hash(qf), bits_hash(x), bits_hash(current["volume"])))
+    peak, front, analytic = shock_radius(qf, x, current["volume"], 1.0 / n, t)
+    log("shock: t=%.6g peak_radius=%.5g half_max_radius=%.5g sedov_taylor_radius=%.5g" % (t, peak, front, analytic))
     fv.close()
     return {"steps": steps, "t": t, "q": qf, "x": x, "volume": current["volume"]}
 

@@ -54,3 +54,22 @@ def test_fnv1a_known_answers():
     for b in a.tobytes():
         h = ((h ^ b) * 1099511628211) % (1 << 64)
     assert sedov.bits_hash(a) == h
+
+
+def test_sedov_shock_radius_of_a_known_shell():
+    """The driver's shock report on a synthetic state: density 4 in the shell 0.20 <= r < 0.25 about the centre, 1
+    elsewhere, on a 40^3 lattice: the peak shell holds the dense cells, the half-max front lies at the shell's outer edge
+    (within a shell width), and the Sedov-Taylor radius is xi0 (E t^2 / rho)^(1/5)."""
+    from luna_hydro import sedov
+
+    n = 40
+    g = (np.arange(n) + 0.5) / n
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    r = np.sqrt(np.sum((x - 0.5) ** 2, axis=1))
+    vol = np.full(len(x), 1.0 / n ** 3)
+    q = np.zeros((len(x), 5))
+    q[:, 0] = np.where((r >= 0.20) & (r < 0.25), 4.0, 1.0) * vol
+    peak, front, analytic = sedov.shock_radius(q, x, vol, 1.0 / n, 0.05)
+    assert 0.20 <= peak <= 0.25
+    assert abs(front - 0.25) <= 1.0 / n
+    assert abs(analytic - 1.15167 * 0.05 ** 0.4) < 1e-15
