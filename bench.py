@@ -76,8 +76,8 @@ def dominant(times, table):
     return best, launches, total / launches
 
 
-def run_static(m, q, steps, warmup, precision="double", hll=0.0, update="per_face"):
-    fv = hydro.FiniteVolume(m, GAMMA, precision=precision, hll_compression=hll, update=update)
+def run_static(m, q, steps, warmup, precision="double", hll=0.0, update="per_face", lanes=1):
+    fv = hydro.FiniteVolume(m, GAMMA, precision=precision, hll_compression=hll, update=update, sweep_lanes=lanes)
     fv.set_state(q)
     for _ in range(warmup):
         fv.step(fv.time_step(COURANT))
@@ -99,14 +99,14 @@ def run_static(m, q, steps, warmup, precision="double", hll=0.0, update="per_fac
     return total, walls, times
 
 
-def run_moving(m, q, steps, warmup, precision="double", hll=0.0, update="per_face"):
+def run_moving(m, q, steps, warmup, precision="double", hll=0.0, update="per_face", lanes=1):
     """Moving steps; the hydro part is timed as the reference's: time_step, the first pass and drift, the new mesh's
     geometry pass (luna_fv_set_geometry_device on the producer's arrays in device memory, as the reference reads
     voronoi-cpp's device views in its sweep and primitives), the second pass. The mesh producer (host) and the copies
     that stand in for a GPU producer's output (positions to the host, the new mesh's arrays to the device) are excluded
     and reported apart."""
     fv = hydro.FiniteVolume(hydro.DeviceMesh(m), GAMMA, motion="moving", precision=precision, hll_compression=hll,
-                            update=update)
+                            update=update, sweep_lanes=lanes)
     fv.set_state(q)
     mesh_s, hydro_s, geo_s = [], [], []
 
@@ -303,6 +303,7 @@ def main():
     ap.add_argument("--no-derivatives", action="store_true", help="skip the derivative cost runs")
     ap.add_argument("--no-sod", action="store_true", help="skip the 1D Sod run")
     ap.add_argument("--no-update-modes", action="store_true", help="skip the per-cell update (Update::per_cell) runs")
+    ap.add_argument("--no-lanes", action="store_true", help="skip the LaneSweep (2, 4, 8 lanes per cell) runs")
     ap.add_argument("--hll-compression", type=float, default=0.0,
                     help="the HLL fallback at extreme compression in the FP64 runs (threshold; 0: off, the reference's scheme)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -377,6 +378,19 @@ def main():
                               "moving_kernels_ms_avg": {k: v[1] / v[0] for k, v in k_m.items()}}}
         log("per_cell update: static %.3g, moving %.3g cell updates/s" % (modes["per_cell"]["static"], modes["per_cell"]["moving"]))
 
+    lanes = None
+    if not args.no_lanes:
+        # The reference's LaneSweep (LUNA_SWEEP_LANES): L lanes per cell, other bits than the one-lane sweep's.
+        lanes = {}
+        for L in (2, 4, 8):
+            t_s, _, k_s = run_static(m, q, args.static_steps, args.warmup, lanes=L)
+            t_m, h_m, _, k_m, _ = run_moving(m, q, max(2, args.steps // 2), args.warmup, lanes=L)
+            h_total = float(np.sum(h_m))
+            lanes[L] = {"static": N * args.static_steps / t_s, "static_ms_per_step": 1e3 * t_s / args.static_steps,
+                        "moving": N * len(h_m) / h_total, "moving_hydro_ms_per_step": 1e3 * h_total / len(h_m),
+                        "static_sweep_ms": k_s["sweep"][1] / k_s["sweep"][0], "moving_sweep_ms": k_m["sweep"][1] / k_m["sweep"][0]}
+            log("LaneSweep %d lanes: static sweep %.3f ms, moving sweep %.3f ms" % (L, lanes[L]["static_sweep_ms"], lanes[L]["moving_sweep_ms"]))
+
     derivs = None
     if not args.no_derivatives:
         log("derivative costs of the static step (forward, tangent, adjoint)")
@@ -417,6 +431,7 @@ def main():
         "learned_vs_plain": learned_cmp,
         "precision_policies": policies,
         "update_modes": modes,
+        "sweep_lanes": lanes,
         "step_roofline": {"algorithmic_bytes_per_cell": step_bytes,
                           "achieved_GBs": step_bytes * N * args.steps / hydro_total / 1e9,
                           "frac": step_bytes * N * args.steps / hydro_total / 1e9 / HBM_PEAK_GBS},

@@ -49,7 +49,8 @@ public:
     FiniteVolume(const luna_geometry& mesh, const Gas& gas, int order, Limiter limiter = Limiter::face_centroid,
                  Motion motion = Motion::static_mesh, Riemann riemann = Riemann::hllc, Precision precision = Precision::double_,
                  double hll_compression = 0,  // the HLL fallback at extreme compression (luna_hydro.h; none in the reference)
-                 Update update = Update::per_face)
+                 Update update = Update::per_face,
+                 int sweep_lanes = 1)  // LUNA_SWEEP_LANES (LaneSweep, :880-1037): 1, 2, 4 or 8
         : n_(mesh.cells) {
         luna_fv_options o{};
         o.dim = D;
@@ -64,6 +65,7 @@ public:
         o.precision = static_cast<int>(precision);
         o.hll_compression = hll_compression;
         o.update = static_cast<int>(update);
+        o.sweep_lanes = sweep_lanes;
         options_ = o;
         check(luna_fv_create(&o, &mesh, &fv_), "hydro::FiniteVolume");
     }

@@ -60,6 +60,11 @@ typedef struct {
                                order 1, the double policy and the face-centroid limiter or none. The derivatives refuse it. */
     int update;             /* LUNA_UPDATE_PER_FACE (a zero-initialised struct) or LUNA_UPDATE_PER_CELL (order 1; the
                                derivatives and the adjoint's checkpoints use the per-face rows either way) */
+    int sweep_lanes;        /* the reference's LUNA_SWEEP_LANES (finite_volume.hpp:880-1037 LaneSweep): 0 or 1 (a
+                               zero-initialised struct) the one-lane sweep; 2, 4 or 8 lanes per cell, each taking every
+                               L-th list entry, their sums combined by a fixed tree and the limiter by the smallest factor
+                               over the faces: the same bits run to run, other bits than the one-lane sweep's. Needs order
+                               1, the double policy and the face-centroid limiter or none; the derivatives refuse it. */
 } luna_fv_options;
 
 /* The mesh arrays the step reads, host pointers: voronoi-cpp's output contract as finite_volume.hpp:257-327 and :353-364

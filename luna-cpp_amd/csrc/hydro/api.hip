@@ -203,7 +203,7 @@ struct luna_fv {
     int grad_row() const { return D == 3 ? luna::grad_row<3>() : luna::grad_row<2>(); }
     luna::Rows rows() const {
         return luna::Rows{q.p, grad.p, w.p, x.p, old_centroid.p, steps.p, flux.p, features.p, troubled.p, opt.precision, exponents.p,
-                          opt.hll_compression > 0 ? hll.p : nullptr, opt.hll_compression};
+                          opt.hll_compression > 0 ? hll.p : nullptr, opt.hll_compression, opt.sweep_lanes};
     }
     // the kernels' Riemann selector: 2 = HLLC with the HLL fallback at flagged cells
     int riemann() const {
@@ -497,6 +497,10 @@ int create(const luna_fv_options* opt, const luna_geometry* mesh, luna_fv** out,
         return fail(LUNA_ERR_ARGUMENT, "hydro::Update: per_face or per_cell");
     if (opt->update == LUNA_UPDATE_PER_CELL && opt->order != 1)
         return fail(LUNA_ERR_ARGUMENT, "luna: the per-cell update here steps order 1 (the order-0 fluxes read Q, which it updates in place)");
+    if (opt->sweep_lanes != 0 && opt->sweep_lanes != 1 && opt->sweep_lanes != 2 && opt->sweep_lanes != 4 && opt->sweep_lanes != 8)
+        return fail(LUNA_ERR_ARGUMENT, "LUNA_SWEEP_LANES: 1 (the one-lane sweep), 2, 4 or 8");
+    if (opt->sweep_lanes > 1 && (opt->order != 1 || opt->precision != LUNA_PRECISION_DOUBLE || opt->limiter == LUNA_LIMITER_LEARNED))
+        return fail(LUNA_ERR_ARGUMENT, "LaneSweep: order 1, the double policy and the face-centroid limiter or none");
     if (mesh->cells <= 0) return fail(LUNA_ERR_ARGUMENT, "luna: no cells");
     luna_fv* fv = new luna_fv;
     fv->opt = *opt;
@@ -651,6 +655,7 @@ int derivatives_ready(luna_fv* fv, const char* what) {
         return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of the face-centroid limiter or none (not the learned one)");
     if (fv->opt.precision != LUNA_PRECISION_DOUBLE) return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of the double policy");
     if (fv->opt.hll_compression > 0) return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives without the HLL fallback");
+    if (fv->opt.sweep_lanes > 1) return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of the one-lane sweep");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, std::string(what) + ": a moving step is between begin and finish");
     if (!fv->mesh_ok) return fail(LUNA_ERR_LOGIC, "luna: the last mesh given was refused; set a valid mesh first");
     const size_t N = fv->N, V = fv->V, F = fv->F;
@@ -783,8 +788,10 @@ int luna_fv_step_adjoint_noalloc(luna_fv* fv, double dt, double* qbar) {
     if (!fv || !qbar) return fail(LUNA_ERR_ARGUMENT, "luna_fv_step_adjoint_noalloc: null argument");
     if (fv->opt.motion != LUNA_MOTION_STATIC || fv->opt.order != 1)
         return fail(LUNA_ERR_LOGIC, "luna_fv_step_adjoint_noalloc: derivatives of a static order-1 step");
-    if (fv->opt.limiter == LUNA_LIMITER_LEARNED || fv->opt.precision != LUNA_PRECISION_DOUBLE || fv->opt.hll_compression > 0)
-        return fail(LUNA_ERR_LOGIC, "luna_fv_step_adjoint_noalloc: the double policy, the face-centroid limiter or none, no HLL fallback");
+    if (fv->opt.limiter == LUNA_LIMITER_LEARNED || fv->opt.precision != LUNA_PRECISION_DOUBLE || fv->opt.hll_compression > 0 ||
+        fv->opt.sweep_lanes > 1)
+        return fail(LUNA_ERR_LOGIC,
+                    "luna_fv_step_adjoint_noalloc: the double policy, the face-centroid limiter or none, no HLL fallback, the one-lane sweep");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_step_adjoint_noalloc: a moving step is between begin and finish");
     if (!fv->mesh_ok) return fail(LUNA_ERR_LOGIC, "luna: the last mesh given was refused; set a valid mesh first");
     void* base = nullptr;
@@ -821,6 +828,7 @@ int luna_fv_moving_adjoint(luna_fv* fv, double dt, const luna_geometry* mesh0, c
         return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: derivatives of the face-centroid limiter or none (not the learned one)");
     if (fv->opt.precision != LUNA_PRECISION_DOUBLE) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: derivatives of the double policy");
     if (fv->opt.hll_compression > 0) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: derivatives without the HLL fallback");
+    if (fv->opt.sweep_lanes > 1) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: derivatives of the one-lane sweep");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_adjoint: a moving step is between begin and finish");
     if (!(dt > 0)) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_adjoint: dt must be positive");
     if (mesh0->cells != fv->N || mesh1->cells != fv->N) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_adjoint: both meshes must have the handle's cells");
@@ -961,6 +969,7 @@ int luna_fv_moving_tangent(luna_fv* fv, double dt, const luna_geometry* mesh0, c
         return fail(LUNA_ERR_LOGIC, "luna_fv_moving_tangent: derivatives of the face-centroid limiter or none (not the learned one)");
     if (fv->opt.precision != LUNA_PRECISION_DOUBLE) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_tangent: derivatives of the double policy");
     if (fv->opt.hll_compression > 0) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_tangent: derivatives without the HLL fallback");
+    if (fv->opt.sweep_lanes > 1) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_tangent: derivatives of the one-lane sweep");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, "luna_fv_moving_tangent: a moving step is between begin and finish");
     if (!(dt > 0)) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_tangent: dt must be positive");
     if (mesh0->cells != fv->N || mesh1->cells != fv->N) return fail(LUNA_ERR_ARGUMENT, "luna_fv_moving_tangent: both meshes must have the handle's cells");

@@ -91,6 +91,9 @@ struct Rows {
     // instead of HLLC. Null when off.
     unsigned char* hll;
     double hll_threshold;
+    // LaneSweep (finite_volume.hpp:880-1037): 0 or 1 the one-lane sweep; 2, 4 or 8 lanes per cell (double policy, order 1,
+    // the face-centroid limiter or none).
+    int sweep_lanes;
 };
 template <int D>
 constexpr int lp_row() {  // row_width<float>(V D): whole 32-byte sectors

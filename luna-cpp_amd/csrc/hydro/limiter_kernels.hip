@@ -288,13 +288,26 @@ __global__ __launch_bounds__(64) void k_mlp_partials(Network net, Batch b, float
     }
 }
 
-// Sums the partial rows in wave order (so a batch always gives the same bits), then either writes the sums (gradient
+// Sums the partial rows in a fixed order (so a batch always gives the same bits), then either writes the sums (gradient
 // and loss) or takes one Adam step of the FP32 parameters and refreshes the network copy (loss into out[0] if out).
-__global__ __launch_bounds__(256) void k_mlp_reduce(const float* partials, int waves, float* out, Adam a, int adam) {
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p > PARAMS) return;
+// A work-group takes 64 parameters; its 16 wavefronts each sum one sixteenth of the rows in row order, and the first
+// wavefront adds the 16 sums in wavefront order (one long chain of dependent loads per parameter took 175 us).
+constexpr int REDUCE_SPLIT = 16;
+__global__ __launch_bounds__(64 * REDUCE_SPLIT) void k_mlp_reduce(const float* partials, int waves, float* out, Adam a, int adam) {
+    __shared__ float part[REDUCE_SPLIT][64];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
+    const int per = (waves + REDUCE_SPLIT - 1) / REDUCE_SPLIT, w0 = q * per, w1 = min(waves, w0 + per);
     float s = 0.0f;
-    for (int w = 0; w < waves; ++w) s += partials[(size_t)w * PARTIAL_ROW + p];
+    if (p <= PARAMS) {
+#pragma unroll 8
+        for (int w = w0; w < w1; ++w) s += partials[(size_t)w * PARTIAL_ROW + p];
+    }
+    part[q][lane] = s;
+    __syncthreads();
+    if (q != 0 || p > PARAMS) return;
+    s = part[0][lane];
+    for (int k = 1; k < REDUCE_SPLIT; ++k) s += part[k][lane];
     if (!adam) {
         out[p] = s;
         return;
@@ -343,7 +356,7 @@ hipError_t launch_mlp_partials(hipStream_t s, const Network& net, const Batch& b
 
 hipError_t launch_mlp_reduce(hipStream_t s, const float* partials, int waves, float* out, const Adam* adam) {
     const Adam a = adam ? *adam : Adam{};
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((PARAMS + 1 + 255) / 256), dim3(256), 0, s, partials, waves, out, a, adam ? 1 : 0);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((PARAMS + 1 + 63) / 64), dim3(64 * REDUCE_SPLIT), 0, s, partials, waves, out, a, adam ? 1 : 0);
     return hipGetLastError();
 }
 

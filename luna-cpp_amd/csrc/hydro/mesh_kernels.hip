@@ -13,8 +13,8 @@
 #include "kernels.hpp"
 
 // Tuning knobs (compile-time; the defaults are the measured best): entries of the sweep's first walk and of its limiter
-// walk loaded together, flux rows of the update loaded together, and the occupancy floor (waves per SIMD) of the sweep
-// and flux kernels.
+// walk loaded together, flux rows of the update loaded together, and the occupancy floor (waves per SIMD) of the sweep,
+// flux and LaneSweep kernels.
 #ifndef LUNA_SWEEP_UNROLL
 #define LUNA_SWEEP_UNROLL 1
 #endif
@@ -29,6 +29,9 @@
 #endif
 #ifndef LUNA_FLUX_WAVES
 #define LUNA_FLUX_WAVES 1
+#endif
+#ifndef LUNA_LANE_WAVES  // LaneSweep's occupancy floor: 2 and 8 lanes measured best at 2 waves (no spill), 4 lanes at 3
+#define LUNA_LANE_WAVES(L) ((L) == 4 ? 3 : 2)
 #endif
 // 1: every sweep takes the neighbours' range (min, max of W) in a loop of its own after the fit, re-reading the
 // neighbours' W (cache-resident), instead of carrying it through the first walk (20 registers fewer there); 0: only the
@@ -534,6 +537,211 @@ __global__ __launch_bounds__(CELL_BLOCK, Limit == 2 ? 1 : LUNA_SWEEP_WAVES) void
     block_min<CELL_BLOCK>(smallest, c.dt);
 }
 
+// finite_volume.hpp:880-1037 LaneSweep: L lanes per cell (L = 2, 4 or 8; 64 / L cells per wavefront). Lane l takes the
+// list entries l, l + L, ... in list order with the one-lane sweep's arithmetic per entry (Sums::add, the time step's
+// terms included); the lanes' sums are combined by a butterfly of shuffles (stage o = L/2, ..., 1: each lane adds lane
+// l^o's value; addition commutes, so every lane of the cell holds the same bits after each stage), the minima and maxima
+// likewise. The limiter takes, per variable, the smallest factor over the faces, each from the unlimited gradient, and
+// scales the gradient once. Bits: the same run to run, other bits than the one-lane sweep's (its sums in list order, its
+// clips one after the other). Double policy, order 1, the face-centroid limiter (Limit 1) or none (Limit 0).
+template <int L>
+__device__ __forceinline__ double lanes_sum(double v) {
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) v = v + __shfl_xor(v, o);
+    return v;
+}
+template <int L>
+__device__ __forceinline__ double lanes_min(double v) {
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    return v;
+}
+template <int L>
+__device__ __forceinline__ double lanes_max(double v) {
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+template <int D, bool Moving, int Limit, int Rule, int L>
+__global__ __launch_bounds__(CELL_BLOCK, LUNA_LANE_WAVES(L)) void k_sweep_lanes(Geo g, Rows r, Gas gas, double courant, Counters c) {
+    constexpr int V = D + 2, U = D + 1, CELLS = CELL_BLOCK / L;
+    using C = CellAt<D>;
+    const int lane = threadIdx.x & (L - 1);
+    const int i = xcd_block() * CELLS + threadIdx.x / L;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *c.dt_next = INF_BITS;
+        c.fallbacks[0] = 0;
+        c.fallbacks[1] = 0;
+    }
+    double smallest = INFINITY;
+    if (i < g.N) {  // the same for the L lanes of a cell, so their shuffles stay among active lanes
+        const double* mine = g.cell + (size_t)i * CELL_ROW;
+        double Wi[V], xi[D], centroid[D];
+#pragma unroll
+        for (int v = 0; v < V; ++v) Wi[v] = mine[C::W + v];
+        const double ci = mine[C::CS], ri = mine[C::RAD];
+        load_vec<D>(mine + C::X, xi);
+        load_vec<D>(mine + C::C, centroid);
+        Sums<D, Moving, true, Rule, double, false> s;
+        s.init();
+        const int count = g.count[i], base = first_slot(g, i);
+        for (int t = lane; t < count; t += L) {
+            Near<D> n;
+            load_near<D>(g, base + t * SLICE, g.other[base + t * SLICE], n);
+            s.add(g, n, xi, centroid, Wi, ci, ri, courant);
+        }
+        s.surface = lanes_sum<L>(s.surface);
+        s.smallest_face = lanes_min<L>(s.smallest_face);
+        if constexpr (Moving) s.max_angle = lanes_max<L>(s.max_angle);
+        smallest = lanes_min<L>(s.smallest);
+#pragma unroll
+        for (int p = 0; p < U; ++p)
+#pragma unroll
+            for (int q = p; q < U; ++q) s.M[p][q] = lanes_sum<L>(s.M[p][q]);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+#pragma unroll
+            for (int p = 0; p < U; ++p) s.b[v][p] = lanes_sum<L>(s.b[v][p]);
+            s.lo[v] = lanes_min<L>(s.lo[v]);
+            s.hi[v] = lanes_max<L>(s.hi[v]);
+        }
+        const double tiny = 1e-6 * s.surface;
+        const bool any_tiny = s.smallest_face < tiny;
+        if (any_tiny) {  // rare: the range again without the tiny faces, each lane walking the whole list
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                s.lo[v] = finite_max<double>();
+                s.hi[v] = -finite_max<double>();
+            }
+            for (int k = 0; k < count; ++k) {
+                if (g.slot[base + k * SLICE] < tiny) continue;
+                const double* row = g.cell + (size_t)g.other[base + k * SLICE] * CELL_ROW + C::W;
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    s.lo[v] = fmin(s.lo[v], row[v]);
+                    s.hi[v] = fmax(s.hi[v], row[v]);
+                }
+            }
+        }
+        double grad[V][D];
+        {  // Sweep::solve (:590-615), as the one-lane sweep
+            double Lc[U][U];
+#pragma unroll
+            for (int p = 0; p < U; ++p)
+#pragma unroll
+                for (int q = 0; q < U; ++q) Lc[p][q] = 0;
+#pragma unroll
+            for (int p = 0; p < U; ++p)
+#pragma unroll
+                for (int q = 0; q <= p; ++q) {
+                    double sum = s.M[q][p];
+#pragma unroll
+                    for (int m = 0; m < q; ++m) sum -= Lc[p][m] * Lc[q][m];
+                    Lc[p][q] = p == q ? sqrt(sum) : sum / Lc[q][q];
+                }
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                double z[U], x[U];
+#pragma unroll
+                for (int p = 0; p < U; ++p) {
+                    double sum = s.b[v][p];
+#pragma unroll
+                    for (int m = 0; m < p; ++m) sum -= Lc[p][m] * z[m];
+                    z[p] = sum / Lc[p][p];
+                }
+#pragma unroll
+                for (int p = U - 1; p >= 0; --p) {
+                    double sum = z[p];
+#pragma unroll
+                    for (int m = p + 1; m < U; ++m) sum -= Lc[m][p] * x[m];
+                    x[p] = sum / Lc[p][p];
+                }
+#pragma unroll
+                for (int a = 0; a < D; ++a) grad[v][a] = x[1 + a];
+            }
+        }
+        double wi[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) wi[a] = Wi[1 + a];
+        if constexpr (Moving) {  // the regularised point velocity (:526-537), as the one-lane sweep
+            const double shaping = 0.5, factor = 2.25;
+            double fraction = 0;
+            if (s.max_angle > 0.75 * factor)
+                fraction = s.max_angle > factor ? shaping : shaping * (s.max_angle - 0.75 * factor) / (0.25 * factor);
+            const double cd = sqrt(dot<D>(centroid, centroid));
+            if (cd > 0 && fraction > 0) {
+                const double sc = fraction * ci / cd;
+#pragma unroll
+                for (int a = 0; a < D; ++a) wi[a] = wi[a] + centroid[a] * sc;
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    r.w[(size_t)i * D + a] = wi[a];
+                    r.old_centroid[(size_t)i * D + a] = centroid[a];
+                }
+            }
+        }
+        if constexpr (Rule == 0) {  // CellStep (:1046-1051)
+            double rel[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) rel[a] = Moving ? Wi[1 + a] - wi[a] : Wi[1 + a];
+            const double step = ri / (ci + sqrt(dot<D>(rel, rel)));
+            smallest = fmin(smallest, courant * step);
+        } else {  // LunaStep (:1061-1070), 2D
+            const double dx = ri * sqrt(M_PI);
+            double rate = 0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                double relative = Wi[1 + a];
+                if constexpr (Moving) relative -= wi[a];
+                rate += (fabs(relative) + ci) / dx;
+            }
+            smallest = fmin(smallest, courant / rate);
+        }
+        if constexpr (Limit > 0) {  // LaneSweep's limiter (:1006-1034): the smallest factor per variable
+            double f[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) f[v] = 1.0;
+            for (int t = lane; t < count; t += L) {
+                const long long slot = base + t * SLICE;
+                if (any_tiny && g.slot[slot] < tiny) continue;
+                double offset[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) offset[a] = g.slot[(a + 1) * g.slots + slot];
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const double dp = dot<D>(grad[v], offset);
+                    double factor = 1.0;
+                    if (dp > 0 && Wi[v] + dp > s.hi[v])
+                        factor = s.hi[v] > Wi[v] ? (s.hi[v] - Wi[v]) / dp : 0.0;
+                    else if (dp < 0 && Wi[v] + dp < s.lo[v])
+                        factor = s.lo[v] < Wi[v] ? (s.lo[v] - Wi[v]) / dp : 0.0;
+                    f[v] = fmin(f[v], factor);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                f[v] = lanes_min<L>(f[v]);
+                if (f[v] < 1.0)
+#pragma unroll
+                    for (int a = 0; a < D; ++a) grad[v][a] = grad[v][a] * f[v];
+            }
+        }
+        if (lane == 0) {
+            store_gradient<D, 0>(r, i, grad);
+            if (r.hll) {  // the compression indicator of the HLL fallback
+                double div = 0;
+#pragma unroll
+                for (int a = 0; a < D; ++a) div += grad[1 + a][a];
+                r.hll[i] = div * ri / ci < -r.hll_threshold ? 1 : 0;
+            }
+        }
+    }
+    block_min<CELL_BLOCK>(smallest, c.dt);
+}
+
 // finite_volume.hpp:1306-1329 FaceFlux::side() at order 1: half step (over step_dt) + g.r, positivity fallback. prow,
 // grow: the cell's primitive half and gradient row, read here so that only one side's rows are live at a time.
 template <int D>
@@ -956,9 +1164,21 @@ static hipError_t sweep_prec(hipStream_t s, const Geo& g, const Rows& r, const G
         hipLaunchKernelGGL((k_sweep<D, Moving, Grad, Limit, 0, Prec>), grid, dim3(CELL_BLOCK), 0, s, g, r, gas, courant, c);
     return hipGetLastError();
 }
+template <int D, bool Moving, int Limit, int L>
+static hipError_t sweep_lanes(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double courant, int rule, const Counters& c) {
+    const dim3 grid = blocks(g.N > 0 ? (long long)g.N * L : 1, CELL_BLOCK);
+    if (rule == 1)
+        hipLaunchKernelGGL((k_sweep_lanes<D, Moving, Limit, 1, L>), grid, dim3(CELL_BLOCK), 0, s, g, r, gas, courant, c);
+    else
+        hipLaunchKernelGGL((k_sweep_lanes<D, Moving, Limit, 0, L>), grid, dim3(CELL_BLOCK), 0, s, g, r, gas, courant, c);
+    return hipGetLastError();
+}
 template <int D, bool Moving, bool Grad, int Limit>
 static hipError_t sweep_rule(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, double courant, int rule, const Counters& c) {
     if constexpr (Grad && Limit < 2) {
+        if (r.precision == 0 && r.sweep_lanes == 2) return sweep_lanes<D, Moving, Limit, 2>(s, g, r, gas, courant, rule, c);
+        if (r.precision == 0 && r.sweep_lanes == 4) return sweep_lanes<D, Moving, Limit, 4>(s, g, r, gas, courant, rule, c);
+        if (r.precision == 0 && r.sweep_lanes == 8) return sweep_lanes<D, Moving, Limit, 8>(s, g, r, gas, courant, rule, c);
         if (r.precision == 1) return sweep_prec<D, Moving, Grad, Limit, 1>(s, g, r, gas, courant, rule, c);
         if (r.precision == 2) return sweep_prec<D, Moving, Grad, Limit, 2>(s, g, r, gas, courant, rule, c);
     }

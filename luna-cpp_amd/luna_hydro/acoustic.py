@@ -139,7 +139,7 @@ def main(argv=None):
     loss.backward()
     f2 = time.perf_counter()
     g_rho, g_vel = rho_t.grad.numpy().copy(), vel_t.grad.numpy().copy()
-    L0 = float(loss)
+    L0 = float(loss.detach())
     log("gradient: forward %.1f s, reverse %.1f s; L = %.6e, |dL/drho| %.3e, |dL/dv| %.3e"
         % (f1 - f0, f2 - f1, L0, np.linalg.norm(g_rho), np.linalg.norm(g_vel)))
     if args.save:

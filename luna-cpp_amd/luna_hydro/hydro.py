@@ -43,7 +43,7 @@ class FVOptions(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int), ("order", ctypes.c_int), ("limiter", ctypes.c_int), ("riemann", ctypes.c_int),
                 ("motion", ctypes.c_int), ("step_rule", ctypes.c_int), ("gamma", ctypes.c_double),
                 ("smallr", ctypes.c_double), ("smallc", ctypes.c_double), ("precision", ctypes.c_int),
-                ("hll_compression", ctypes.c_double), ("update", ctypes.c_int)]
+                ("hll_compression", ctypes.c_double), ("update", ctypes.c_int), ("sweep_lanes", ctypes.c_int)]
 
 
 class Geometry(ctypes.Structure):
@@ -181,14 +181,15 @@ class DeviceMesh:
 
 class FiniteVolume:
     """hydro::FiniteVolume<D> (D = 2, 3) on the GPU: FiniteVolume(exec, mesh, gas, order, per_face, limiter, motion).
-    mesh: a mesh dict (host arrays) or a DeviceMesh (read in place on the GPU)."""
+    mesh: a mesh dict (host arrays) or a DeviceMesh (read in place on the GPU). sweep_lanes: the reference's
+    LUNA_SWEEP_LANES (LaneSweep with 2, 4 or 8 lanes per cell; 1 the one-lane sweep)."""
 
     def __init__(self, mesh, gamma, order=1, limiter="face", riemann="hllc", motion="static", step_rule="arepo",
-                 smallr=1e-10, smallc=1e-10, precision="double", hll_compression=0.0, update="per_face"):
+                 smallr=1e-10, smallc=1e-10, precision="double", hll_compression=0.0, update="per_face", sweep_lanes=1):
         self.D, self.N = mesh["dim"], mesh["cells"]
         self.V = self.D + 2
         opt = FVOptions(self.D, order, LIMITERS[limiter], RIEMANN[riemann], MOTION[motion], RULES[step_rule],
-                        gamma, smallr, smallc, PRECISIONS[precision], hll_compression, UPDATES[update])
+                        gamma, smallr, smallc, PRECISIONS[precision], hll_compression, UPDATES[update], sweep_lanes)
         h = ctypes.c_void_p()
         if isinstance(mesh, DeviceMesh):
             self._mesh_arrays = mesh

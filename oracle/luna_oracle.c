@@ -598,9 +598,144 @@ static void sweep_cell_lp(orc_fv* fv, const orc_geometry* geo, int i) {
 
 /* finite_volume.hpp:463-569 Sweep::operator() (double policy, the face-centroid limiter or none), with the moving mesh's
  * point velocity (:526-537, constants :74) and neighbour_range (:572-588). */
+/* finite_volume.hpp:880-1037 LaneSweep<Moving> (order 1, double, face-centroid limiter or none): L lanes per cell, lane l
+ * taking the list entries l, l + L, ... in list order with the one-lane sweep's arithmetic per entry. The lanes' sums
+ * combine by a butterfly (stage o = L/2, ..., 1: lane l adds lane l^o's value; addition commutes, so every lane holds the
+ * same bits after each stage, and the total is lane 0's), the minima and maxima likewise. The reference combines them with
+ * Kokkos's vector reduction, a fixed tree of its own backend; this restates the tree the GPU kernel uses. The limiter
+ * takes, per variable, the smallest factor over the faces, each from the unlimited gradient, and scales once. */
+static void lanes_combine(int L, double* a, int op) { /* op 0 sum, 1 min, 2 max; a[0] gets the result */
+    double t[8];
+    for (int o = L / 2; o > 0; o /= 2) {
+        for (int l = 0; l < L; ++l) t[l] = op == 0 ? a[l] + a[l ^ o] : op == 1 ? fmin(a[l], a[l ^ o]) : fmax(a[l], a[l ^ o]);
+        for (int l = 0; l < L; ++l) a[l] = t[l];
+    }
+}
+
+static void sweep_cell_lanes(orc_fv* fv, const orc_geometry* geo, int i, int L) {
+    const int D = fv->D, V = D + 2, U = D + 1, PW = V + 2;
+    const double* Wi = fv->prims + (size_t)i * PW;
+    const double* centroid = geo->centroid + (size_t)i * D;
+    const int first = geo->start[i], end = geo->start[i + 1];
+    double surface[8], smallest[8], angle[8], lo[MAXV][8], hi[MAXV][8], M[MAXD + 1][MAXD + 1][8], b[MAXV][MAXD + 1][8];
+    for (int l = 0; l < L; ++l) {
+        surface[l] = 0;
+        smallest[l] = INFINITY;
+        angle[l] = 0;
+        for (int v = 0; v < V; ++v) {
+            lo[v][l] = DBL_MAX;
+            hi[v][l] = -DBL_MAX;
+            for (int p = 0; p < U; ++p) b[v][p][l] = 0;
+        }
+        for (int p = 0; p < U; ++p)
+            for (int s = 0; s < U; ++s) M[p][s][l] = 0;
+    }
+    for (int k = first; k < end; ++k) {
+        const int l = (k - first) % L;
+        const int e = geo->list[k], f = list_face(e);
+        const int j = e >= 0 ? geo->neighbour[f] : geo->owner[f];
+        const double* Wj = fv->prims + (size_t)j * PW;
+        double d[MAXD];
+        separation(geo, i, j, d);
+        const double magnitude = magnitude_of(geo, f);
+        surface[l] += magnitude;
+        smallest[l] = fmin(smallest[l], magnitude);
+        if (fv->moving) angle[l] = fmax(angle[l], face_angle(D, magnitude) / (0.5 * sqrt(dot(D, d, d))));
+        double X[MAXD + 1];
+        X[0] = 1;
+        for (int p = 1; p < U; ++p) X[p] = d[p - 1] + geo->centroid[(size_t)j * D + p - 1] - centroid[p - 1];
+        const double w = magnitude;
+        for (int p = 0; p < U; ++p)
+            for (int s = p; s < U; ++s) M[p][s][l] += w * X[p] * X[s];
+        for (int v = 0; v < V; ++v) {
+            const double dW = Wj[v] - Wi[v];
+            for (int p = 0; p < U; ++p) b[v][p][l] += w * X[p] * dW;
+            lo[v][l] = fmin(lo[v][l], Wj[v]);
+            hi[v][l] = fmax(hi[v][l], Wj[v]);
+        }
+    }
+    lanes_combine(L, surface, 0);
+    lanes_combine(L, smallest, 1);
+    lanes_combine(L, angle, 2);
+    double Mt[MAXD + 1][MAXD + 1], bt[MAXV][MAXD + 1], lot[MAXV], hit[MAXV];
+    memset(Mt, 0, sizeof Mt);
+    for (int p = 0; p < U; ++p)
+        for (int s = p; s < U; ++s) {
+            lanes_combine(L, M[p][s], 0);
+            Mt[p][s] = M[p][s][0];
+        }
+    for (int v = 0; v < V; ++v) {
+        for (int p = 0; p < U; ++p) {
+            lanes_combine(L, b[v][p], 0);
+            bt[v][p] = b[v][p][0];
+        }
+        lanes_combine(L, lo[v], 1);
+        lanes_combine(L, hi[v], 2);
+        lot[v] = lo[v][0];
+        hit[v] = hi[v][0];
+    }
+    /* Rare: faces below 1e-6 of the surface leave the range and the limiter (the range taken again, one walk). */
+    const double tiny = 1e-6 * surface[0];
+    const int any_tiny = smallest[0] < tiny;
+    if (any_tiny) {
+        for (int v = 0; v < V; ++v) {
+            lot[v] = DBL_MAX;
+            hit[v] = -DBL_MAX;
+        }
+        for (int k = first; k < end; ++k) {
+            const int e = geo->list[k], f = list_face(e);
+            const int j = e >= 0 ? geo->neighbour[f] : geo->owner[f];
+            if (magnitude_of(geo, f) < tiny) continue;
+            const double* Wj = fv->prims + (size_t)j * PW;
+            for (int v = 0; v < V; ++v) {
+                lot[v] = fmin(lot[v], Wj[v]);
+                hit[v] = fmax(hit[v], Wj[v]);
+            }
+        }
+    }
+    double grad[MAXV * MAXD], intercept[MAXV];
+    solve_fit(D, Mt, bt, grad, intercept);
+    if (fv->moving) regularised_velocity(fv, i, Wi, centroid, angle[0]);
+    if (fv->limiter == 0) {
+        double factor[MAXV][8];
+        for (int v = 0; v < V; ++v)
+            for (int l = 0; l < L; ++l) factor[v][l] = 1.0;
+        for (int k = first; k < end; ++k) {
+            const int l = (k - first) % L, f = list_face(geo->list[k]);
+            if (any_tiny && magnitude_of(geo, f) < tiny) continue;
+            double offset[MAXD];
+            face_offset(geo, f, i, offset);
+            for (int v = 0; v < V; ++v) {
+                const double dp = dot(D, grad + v * D, offset);
+                double s = 1.0;
+                if (dp > 0 && Wi[v] + dp > hit[v])
+                    s = hit[v] > Wi[v] ? (hit[v] - Wi[v]) / dp : 0.0;
+                else if (dp < 0 && Wi[v] + dp < lot[v])
+                    s = lot[v] < Wi[v] ? (lot[v] - Wi[v]) / dp : 0.0;
+                factor[v][l] = fmin(factor[v][l], s);
+            }
+        }
+        for (int v = 0; v < V; ++v) {
+            lanes_combine(L, factor[v], 1);
+            if (factor[v][0] < 1.0)
+                for (int a = 0; a < D; ++a) grad[v * D + a] = grad[v * D + a] * factor[v][0];
+        }
+    }
+    memcpy(fv->grad + (size_t)i * V * D, grad, sizeof(double) * V * D);
+    if (fv->hll_compression > 0) {
+        double div = 0;
+        for (int a = 0; a < D; ++a) div += grad[(1 + a) * D + a];
+        fv->hll[i] = div * Wi[V + 1] / Wi[V] < -fv->hll_compression ? 1 : 0;
+    }
+}
+
 static void sweep_cell(orc_fv* fv, const orc_geometry* geo, int i) {
     if (fv->precision != 0 && fv->order == 1) {
         sweep_cell_lp(fv, geo, i);
+        return;
+    }
+    if (fv->sweep_lanes > 1 && fv->order == 1) {
+        sweep_cell_lanes(fv, geo, i, fv->sweep_lanes);
         return;
     }
     const int D = fv->D, V = D + 2, U = D + 1, PW = V + 2;

@@ -94,6 +94,8 @@ typedef struct {
                                  sets hll[i] when r_i (div v)_i / c_s,i < -hll_compression (div v of the limited gradient);
                                  HLLC takes HLL at faces with a flagged side */
     unsigned char* hll;       /* N, written by the sweep when hll_compression > 0 */
+    int sweep_lanes;          /* 0 or 1: the sweep (finite_volume.hpp:463-569); 2, 4, 8: LaneSweep (:880-1037), order 1,
+                                 double, face-centroid limiter or none */
 } orc_fv;
 
 void orc_primitives(orc_fv* fv, const orc_geometry* geo);

@@ -35,7 +35,7 @@ class _FV(ctypes.Structure):
                 ("steps", _ip), ("flux", _dp), ("fallbacks", ctypes.c_longlong * 2), ("prims_ready", ctypes.c_int),
                 ("features", ctypes.POINTER(ctypes.c_ushort)), ("troubled", ctypes.POINTER(ctypes.c_ubyte)),
                 ("hold_steps", ctypes.c_int), ("precision", ctypes.c_int), ("hll_compression", ctypes.c_double),
-                ("hll", ctypes.POINTER(ctypes.c_ubyte))]
+                ("hll", ctypes.POINTER(ctypes.c_ubyte)), ("sweep_lanes", ctypes.c_int)]
 
 
 _GEO_ADJOINT = ctypes.CFUNCTYPE(None, ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp)
@@ -257,10 +257,12 @@ class FiniteVolume:
     """The oracle's FiniteVolume<D> (D = 2, 3): state, time_step, step, and the moving step split at the mesh update."""
 
     def __init__(self, mesh, gamma, order=1, limiter="face", riemann="hllc", moving=False, step_rule="arepo", precision="double",
-                 hll_compression=0.0):
+                 hll_compression=0.0, sweep_lanes=1):
         D, N = mesh["dim"], mesh["cells"]
         if precision != "double" and limiter == "learned":
             raise ValueError("the learned limiter's features need the double policy")
+        if sweep_lanes not in (0, 1, 2, 4, 8) or (sweep_lanes > 1 and (order != 1 or precision != "double" or limiter == "learned")):
+            raise ValueError("LaneSweep: 2, 4 or 8 lanes, order 1, the double policy, the face-centroid limiter or none")
         V = D + 2
         self.D, self.N, self.V = D, N, V
         self.q = np.zeros((N, V))
@@ -282,6 +284,7 @@ class FiniteVolume:
         self.hll = np.zeros(N, dtype=np.uint8)
         c.hll_compression = hll_compression
         c.hll = self.hll.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
+        c.sweep_lanes = sweep_lanes
         c.gas = gas(gamma)
         c.q, c.prims, c.grad, c.w = _ptr(self.q), _ptr(self.prims), _ptr(self.grad), _ptr(self.w)
         c.x, c.old_centroid, c.steps, c.flux = _ptr(self.x), _ptr(self.old_centroid), _ptr(self.steps, _ip), _ptr(self.flux)
