@@ -24,27 +24,28 @@ GAMMA = 5.0 / 3.0
 COURANT = 0.3
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per cell of each kernel: every array a phase needs read once and each output written once
-# (unique bytes; 3D, F/N = 7.95 and E/N = 15.9 as luna-cpp measures them; FP64 values, int32 indices).
-#   sweep+dt: Q-derived rows W (40), cell geometry (V 8, c 24, x 24, CSR 4), list entries 63.6, face records 445.2,
-#             writes g 120 and W 40 -> 768.8 static; moving also writes w (24) and keeps c^n (24) -> 816.8
+# Algorithmic bytes per cell of each kernel: every array a phase needs read once and each output written once (unique
+# bytes; FP64 values, int32 indices), as functions of the mesh's faces and list entries per cell (f = F/N, e = E/N = 2f;
+# luna-cpp quotes its figures at f = 7.95, the jittered 128^3 mesh here has 7.775).
+#   sweep+dt: Q-derived rows W (40), cell geometry (V 8, c 24, x 24, CSR 4), list entries 4e, face records 56f, writes
+#             g 120 and W 40 -> 260 + 4e + 56f static (768.8 at f = 7.95); moving also writes w (24) and keeps c^n (24)
 #   keep_centroids (moving): x read and snapped (48), w (24), grid steps written (12) -> 84
-#   flux:     faces 445.2, W + g + c + x of the cells 208, writes the flux rows 318 -> 971.2; moving pass 1 also reads
-#             the grid steps (int32, 12) -> 983.2; pass 2 the steps and the old centroids (24) -> 1007.2
-#   update:   CSR 4, list 63.6, flux rows 318, Q read and written 80 -> 465.6; moving pass 1 with the drift (x r/w 48,
-#             grid steps 12) -> 525.6
-ALGO_BYTES = {
-    "static": {"sweep": 768.8, "flux": 971.2, "update": 465.6},
-    "moving": {"sweep": 816.8, "keep_centroids": 84.0, "flux_pass1": 983.2, "update_pass1": 525.6, "flux_pass2": 1007.2,
-               "update_pass2": 465.6},
-}
-# The geometry pass of a moving step (the new mesh's derived layouts from the producer's arrays; F/N = 7.77 and E/N =
-# 15.55 on the jittered 128^3 mesh):
+#   flux:     faces 56f, W + g + c + x of the cells 208, writes the flux rows 40f -> 208 + 96f; moving pass 1 also reads
+#             the grid steps (int32, 12); pass 2 the steps and the old centroids (24)
+#   update:   CSR 4, list 4e, flux rows 40f, Q read and written 80 -> 84 + 4e + 40f; moving pass 1 with the drift (x r/w
+#             48, grid steps 12)
+# and of the geometry pass of a moving step (the new mesh's derived layouts from the producer's arrays):
 #   cells:  the producer's V, c, x and list start in (60), the 64 B geometry half of the cell row and the count out -> 128
-#   faces:  owner, neighbour, A, face centroid in (56 a face), the 64 B face record out -> 932
-#   slots:  the producer's list (62), the used fields of each face record once (40 a face, 311), the cells' x and c once
-#           (48), 40 B per list entry out (the face id with its owner bit, the other cell, |A|, the face offset; 622) -> 1043
-GEO_BYTES = {"geometry_cells": 128.0, "geometry_faces": 932.0, "geometry_slots": 1043.0}
+#   faces:  owner, neighbour, A, face centroid in (56 a face), the 64 B face record out -> 120f
+#   slots:  the producer's list (4e), the used fields of each face record once (40f), the cells' x and c once (48), 40 B
+#           per list entry out (the face id with its owner bit, the other cell, |A|, the face offset) -> 48 + 44e + 40f
+def algo_bytes(f, e):
+    return {
+        "static": {"sweep": 260 + 4 * e + 56 * f, "flux": 208 + 96 * f, "update": 84 + 4 * e + 40 * f},
+        "moving": {"sweep": 308 + 4 * e + 56 * f, "keep_centroids": 84.0, "flux_pass1": 220 + 96 * f,
+                   "update_pass1": 144 + 4 * e + 40 * f, "flux_pass2": 244 + 96 * f, "update_pass2": 84 + 4 * e + 40 * f},
+        "geometry": {"geometry_cells": 128.0, "geometry_faces": 120 * f, "geometry_slots": 48 + 44 * e + 40 * f},
+    }
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_sedov128.md")
 
@@ -338,11 +339,12 @@ def main():
     log("moving: hydro %.3f ms/step, %.3g cell updates/s; kernels %s" % (1e3 * hydro_total / args.steps, value,
                                                                           {k: round(v[1] / v[0], 4) for k, v in m_times.items()}))
 
-    name, launches, avg_ms = dominant(m_times, ALGO_BYTES["moving"])
-    per_cell = ALGO_BYTES["moving"][name]
+    ALGO = algo_bytes(F / N, 2.0 * F / N)
+    name, launches, avg_ms = dominant(m_times, ALGO["moving"])
+    per_cell = ALGO["moving"][name]
     achieved = per_cell * N / (avg_ms * 1e-3) / 1e9
-    sname, _, savg = dominant(s_times, ALGO_BYTES["static"])
-    s_ach = ALGO_BYTES["static"][sname] * N / (savg * 1e-3) / 1e9
+    sname, _, savg = dominant(s_times, ALGO["static"])
+    s_ach = ALGO["static"][sname] * N / (savg * 1e-3) / 1e9
 
     cpu = None
     if not args.no_cpu:
@@ -407,7 +409,7 @@ def main():
         log("learned limiter vs plain (smooth KH 64^2, reference 256^2)")
         learned_cmp = learned_vs_plain(log)
 
-    step_bytes = sum(ALGO_BYTES["moving"].values())
+    step_bytes = sum(ALGO["moving"].values())
     out = {
         "metric": "cell updates/s at 128^3 moving mesh (hydro part with the per-step geometry pass, mesh producer excluded)",
         "value": value,
@@ -440,7 +442,7 @@ def main():
                  "geometry_roofline": {k: {"algorithmic_bytes_per_cell": b, "avg_ms": m_times[k][1] / m_times[k][0],
                                            "achieved_GBs": b * N / (m_times[k][1] / m_times[k][0] * 1e-3) / 1e9,
                                            "frac": b * N / (m_times[k][1] / m_times[k][0] * 1e-3) / 1e9 / HBM_PEAK_GBS}
-                                       for k, b in GEO_BYTES.items() if k in m_times},
+                                       for k, b in ALGO["geometry"].items() if k in m_times},
                  "note": "value includes the geometry pass on the device (luna_fv_set_geometry_device: cell rows, face "
                          "records, slot lists with |A| and face offsets, radii); excluded: the Voronoi producer (C++/OpenMP, "
                          "host) and the copies standing in for a GPU producer (positions to the host, its arrays to the device)"},
@@ -450,7 +452,7 @@ def main():
         "kernels_ms_avg": {k: v[1] / v[0] for k, v in m_times.items()},
         "static": {"value": static_rate, "unit": "cell updates/s", "ms_per_step": 1e3 * s_total / args.static_steps,
                    "step_median_ms": 1e3 * float(np.median(s_walls)), "kernel": sname, "achieved_GBs": s_ach,
-                   "frac": s_ach / HBM_PEAK_GBS, "step_frac": sum(ALGO_BYTES["static"].values()) * static_rate / 1e9 / HBM_PEAK_GBS,
+                   "frac": s_ach / HBM_PEAK_GBS, "step_frac": sum(ALGO["static"].values()) * static_rate / 1e9 / HBM_PEAK_GBS,
                    "kernels_ms_avg": {k: v[1] / v[0] for k, v in s_times.items()}},
     }
     print(json.dumps(out), flush=True)

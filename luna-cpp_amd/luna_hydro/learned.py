@@ -366,7 +366,11 @@ def run_timed(m, q0, t_end, gamma, courant, limiter="face", params=None, motion=
             fv.set_limiter_network(params)
         fv.set_state(q0)
         t, steps, busy, current = 0.0, 0, 0.0, m
+        said = time.perf_counter()
         while t < t_end * (1 - 1e-12):
+            if log and time.perf_counter() - said > 20.0:  # progress every 20 s of wall (moving runs wait on the host)
+                said = time.perf_counter()
+                log("    progress: steps=%d t=%.6g" % (steps, t))
             a = time.perf_counter()
             dt = min(fv.time_step(courant), t_end - t)
             if motion == "static":
@@ -394,12 +398,12 @@ def density_l1(q, m, q_ref):
     return float(np.sum(np.abs(q[:, 0] - q_ref[:, 0])) / np.sum(vol))
 
 
-def curve(problem, ns, seed, t_end, params, ref_n, gamma=5.0 / 3.0, motion="static", log=None):
+def curve(problem, ns, seed, t_end, params, ref_n, gamma=5.0 / 3.0, motion="static", log=None, limiters=("plain", "learned")):
     """Error against cost: plain and learned runs at each n in ns (meshes of `seed`) against one plain static reference
     run at ref_n (another seed's mesh), restricted to each run's final mesh. Returns rows {"n", "limiter", "l1",
     "seconds", "steps"} and the reference's cost."""
     fine = problem.mesh(ref_n, seed + 1000)
-    fq, fm, fine_s, fine_steps = run_timed(fine, problem.state(fine, ref_n, gamma), t_end, gamma, problem.courant)
+    fq, fm, fine_s, fine_steps = run_timed(fine, problem.state(fine, ref_n, gamma), t_end, gamma, problem.courant, log=log)
     if log:
         log("  %s reference n=%d t=%g: %.2f GPU-s, %d steps" % (problem.name, ref_n, t_end, fine_s, fine_steps))
     rows = []
@@ -407,7 +411,9 @@ def curve(problem, ns, seed, t_end, params, ref_n, gamma=5.0 / 3.0, motion="stat
         coarse = problem.mesh(n, seed)
         q0 = problem.state(coarse, n, gamma)
         for name, limiter, p in (("plain", "face", None), ("learned", "learned", params)):
-            q, cm, secs, steps = run_timed(coarse, q0, t_end, gamma, problem.courant, limiter, p, motion)
+            if name not in limiters:
+                continue
+            q, cm, secs, steps = run_timed(coarse, q0, t_end, gamma, problem.courant, limiter, p, motion, log)
             l1 = density_l1(q, cm, restrict(fq, fm, cm, owners(cm, fm)))
             rows.append({"n": n, "limiter": name, "l1": l1, "seconds": secs, "steps": steps})
             if log:
@@ -464,6 +470,7 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=99)
     ap.add_argument("--moving", action="store_true")
     ap.add_argument("--params", default="")
+    ap.add_argument("--limiters", default="plain,learned")
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -496,7 +503,7 @@ def main(argv=None):
         return 0
     params = np.load(a.params, allow_pickle=False) if a.params else default_params()
     rows, ref = curve(PROBLEMS[a.problem], [int(s) for s in a.ns.split(",")], a.seed, a.t_end, params, a.ref_n,
-                      motion="moving" if a.moving else "static", log=log)
+                      motion="moving" if a.moving else "static", log=log, limiters=a.limiters.split(","))
     print(json.dumps({"problem": a.problem, "t_end": a.t_end, "motion": "moving" if a.moving else "static",
                       "reference": ref, "rows": rows, "speedup_at_equal_error": speedup_at_equal_error(rows)}), flush=True)
     return 0

@@ -115,7 +115,11 @@ def run(n=64, moving=False, t_end=0.05, courant=0.3, snapshots=10, out="", max_s
     walls, steps, fallbacks, most, with_fallbacks, t, nxt = [], 0, 0, 0, 0, 0.0, 1
     fv.profile(True)
     start = time.perf_counter()
+    said = start
     while t < t_end and (max_steps == 0 or steps < max_steps):
+        if time.perf_counter() - said > 20.0:  # a progress line every 20 s of wall (moving runs wait on the host producer)
+            said = time.perf_counter()
+            print("progress: steps=%d t=%.6g" % (steps, t), file=sys.stderr, flush=True)
         a = time.perf_counter()
         dt = fv.time_step(courant)
         t_next = t_end * nxt / snapshots

@@ -209,3 +209,57 @@ def test_order0_moving_adjoint_matches_oracle(D, n, riemann):
         print("  %s: max |GPU - oracle| / max |oracle| = %.2e" % (name, err), flush=True)
         assert err < 1e-10, (name, err)
     fv.close()
+
+
+def test_acoustic_driver_runs_and_repeats(tmp_path):
+    """The acoustic-wave adjoint driver (python -m luna_hydro.acoustic) on a small case: its JSON line, and the same
+    gradient bits from a second run (the checkpointed rollout and its reverse are deterministic)."""
+    import json
+
+    from luna_hydro import acoustic
+
+    outs = []
+    for k in range(2):
+        out, npz = tmp_path / ("a%d.json" % k), tmp_path / ("a%d.npz" % k)
+        acoustic.main(["--n", "6", "--steps", "6", "--every", "4", "--directions", "0:1", "--out", str(out), "--save", str(npz)])
+        outs.append((json.loads(out.read_text()), np.load(npz)))
+    (j0, g0), (j1, g1) = outs
+    assert j0["cells"] == 216 and len(j0["directions"]) == 1 and j0["loss"] == j1["loss"]
+    assert np.array_equal(g0["g_rho"], g1["g_rho"]) and np.array_equal(g0["g_vel"], g1["g_vel"])
+    assert np.isfinite(j0["directions"][0]["fd"]) and np.isfinite(j0["directions"][0]["adjoint"])
+
+
+def test_handles_release_their_device_memory():
+    """A moving handle that ran the step, the moving reverse and the moving tangent, then closed, leaves the device's free
+    memory where it found it (hipMemGetInfo through torch, after one warm-up round for the runtime's own first-use
+    allocations); likewise a static handle after the static derivatives."""
+    torch = pytest.importorskip("torch")
+
+    def round_trip():
+        m0 = mesh.build(problems.jittered_lattice(10, 3, 0.25, 4), 1.0)
+        q = problems.acoustic_state(m0)
+        fv = hydro.FiniteVolume(m0, GAMMA, motion="moving")
+        fv.set_state(q)
+        dt = fv.time_step(0.3)
+        m1 = mesh.build(fv.moving_begin(dt), 1.0)
+        fv.moving_finish(m1, dt)
+        fv.set_geometry(m0)
+        fv.set_state(q)
+        N = m0["cells"]
+        fv.moving_adjoint(dt, m0, m1, np.ones((N, 5)), np.zeros((N, 3)))
+        fv.moving_tangent(dt, m0, m1, np.ones((N, 5)), np.zeros((N, 3)))
+        fv.close()
+        st = hydro.FiniteVolume(m0, GAMMA)
+        st.set_state(q)
+        dt = st.time_step(0.3)
+        st.step_tangent(dt, np.ones((N, 5)))
+        st.step_adjoint(dt, np.ones((N, 5)))
+        st.close()
+        hydro.synchronize()
+
+    round_trip()
+    before = torch.cuda.mem_get_info()[0]
+    round_trip()
+    after = torch.cuda.mem_get_info()[0]
+    print("device free memory before %d, after %d" % (before, after), flush=True)
+    assert after == before
