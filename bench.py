@@ -67,6 +67,28 @@ def measured_traffic(phase, n):
     return None
 
 
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_counters_sedov128.md")
+SIMDS, CLOCK_HZ = 1024, 2.4e9  # 256 CUs x 4 SIMDs; a wave64 VALU instruction issues in 4 cycles of one SIMD
+
+
+def valu_floor_ms(kernel_prefix, n, cells):
+    """The time a kernel would take issuing only its VALU instructions, every SIMD busy (instructions per wave from the
+    committed rocprofv3 SQ_INSTS_VALU summary's VALU-busy table, at n = 128): the bound of the FP64 arithmetic the
+    kernel must do. None when the summary is absent or the workload is another size."""
+    if n != 128 or not os.path.exists(VALU_FILE):
+        return None
+    with open(VALU_FILE) as f:
+        for line in f:
+            cols = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+            if len(cols) == 5 and cols[0].startswith(kernel_prefix):
+                try:
+                    per_wave = float(cols[2])
+                except ValueError:
+                    return None
+                return per_wave * (cells / 64.0) * 4 / SIMDS / CLOCK_HZ * 1e3
+    return None
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -428,7 +450,11 @@ def main():
                    "parallelism": "single GPU", "hll_compression": args.hll_compression},
         "roofline": {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("moving " + name, n),
-                     "algorithmic_bytes_per_cell": per_cell, "avg_ms": avg_ms, "launches": launches},
+                     "algorithmic_bytes_per_cell": per_cell, "avg_ms": avg_ms, "launches": launches,
+                     "valu_floor_ms": valu_floor_ms("k_sweep<3, true, true, 1, 0, 0>", n, N) if name == "sweep" else None,
+                     "note": "the FP64 sweep's arithmetic (7430-8396 VALU instructions a cell, 46 IEEE divisions in its solve) "
+                             "bounds it at its VALU floor before HBM: at the floor it would move its algorithmic bytes at "
+                             "about half of 8 TB/s"},
         "cpu_baseline": cpu,
         "learned_vs_plain": learned_cmp,
         "precision_policies": policies,

@@ -69,8 +69,10 @@ typedef struct {
 
 /* The mesh arrays the step reads, host pointers: voronoi-cpp's output contract as finite_volume.hpp:257-327 and :353-364
  * read it (luna_mesh_record of include/luna_mesh.h starts with the same fields). Row-major [rows][dim] FP64, int32
- * indices; list entries f (the cell owns face f) or ~f (it does not); area A = |A| n points owner -> neighbour; face
- * centroids relative to the owner's generator, cell centroids relative to their own. */
+ * indices; list entries f (the cell owns face f) or ~f (it does not) — or plain face ids: the owner bit is set from
+ * owner[] as the reference's owner_layout does (finite_volume.hpp:78-87), so a list with or without it steps the same;
+ * area A = |A| n points owner -> neighbour; face centroids relative to the owner's generator, cell centroids relative to
+ * their own. */
 typedef struct {
     int dim, cells, faces;
     const int* owner;

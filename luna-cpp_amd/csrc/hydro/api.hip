@@ -274,16 +274,16 @@ int derive(luna_fv* fv, const luna_geometry* m, luna::MeshIn in) {
                       : luna::launch_geometry_faces<2>(s, in, fv->facerec.p, fv->geo_flags.p);
     }));
     LUNA_HIP(timed(fv->prof, s, "geometry_slots", [&] {
-        return D == 3 ? luna::launch_geometry_slots<3>(s, in, fv->count.p, fv->slice.p, fv->geo_info.p, fv->list.p, fv->other.p, fv->slotgeo.p,
+        return D == 3 ? luna::launch_geometry_slots<3>(s, in, fv->facerec.p, fv->count.p, fv->slice.p, fv->geo_info.p, fv->list.p, fv->other.p, fv->slotgeo.p,
                                                        fv->geo_flags.p)
-                      : luna::launch_geometry_slots<2>(s, in, fv->count.p, fv->slice.p, fv->geo_info.p, fv->list.p, fv->other.p, fv->slotgeo.p,
+                      : luna::launch_geometry_slots<2>(s, in, fv->facerec.p, fv->count.p, fv->slice.p, fv->geo_info.p, fv->list.p, fv->other.p, fv->slotgeo.p,
                                                        fv->geo_flags.p);
     }));
     LUNA_HIP(hipMemcpyAsync(back + 1, fv->geo_flags.p, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
     LUNA_HIP(hipStreamSynchronize(s));
     flags = static_cast<unsigned int>(back[1]);
     if (flags & luna::GEO_BAD_CELL) return fail(LUNA_ERR_ARGUMENT, "luna: face cell index out of range");
-    if (flags & luna::GEO_BAD_FACE) return fail(LUNA_ERR_ARGUMENT, "luna: list entry out of range");
+    if (flags & luna::GEO_BAD_FACE) return fail(LUNA_ERR_ARGUMENT, "luna: list entry out of range or not a face of its cell");
     fv->F = F;
     fv->E = 2 * F;
     fv->slots = static_cast<int>(slots);

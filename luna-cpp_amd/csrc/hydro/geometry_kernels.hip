@@ -8,7 +8,8 @@
 //                 work-group) the longest list; checks of the CSR offsets;
 //   k_geo_scan    the slices' first slots (exclusive sum of 64 x longest), one work-group;
 //   k_geo_faces   per face: the record [A, fc, |A|, (owner, neighbour)]; checks of the cell indices;
-//   k_geo_slots   per cell and list entry, in the slices' layout: the entry, the other cell, |A| and the face offset
+//   k_geo_slots   per cell and list entry, in the slices' layout: the entry, the other cell, |A| and the face offset (from
+//                 the face records k_geo_faces wrote)
 //                 x_f - c_i; padding slots zero; checks of the face indices.
 // Every read is guarded by the checks before it, so a malformed mesh sets a flag (read by the host) and reads nothing out
 // of range. The arithmetic is that of the host restatement (the same products, sums and rounding), so the rows are equal
@@ -120,9 +121,11 @@ __global__ __launch_bounds__(BLOCK) void k_geo_faces(MeshIn m, double* face, uns
 // The CSR offsets were checked by k_geo_cells (the host runs this only when they passed).
 constexpr int SLOT_BLOCK = 256, QUARTER = 16, QUARTER_DEPTH = 48;
 template <int D>
-__global__ __launch_bounds__(SLOT_BLOCK) void k_geo_slots(MeshIn m, const int* __restrict__ count, const int* __restrict__ slice,
-                                                          const long long* __restrict__ info, int S, int* __restrict__ list,
-                                                          int* __restrict__ other, double* __restrict__ slot, unsigned int* flags) {
+__global__ __launch_bounds__(SLOT_BLOCK) void k_geo_slots(MeshIn m, const double* __restrict__ face, const int* __restrict__ count,
+                                                          const int* __restrict__ slice, const long long* __restrict__ info, int S,
+                                                          int* __restrict__ list, int* __restrict__ other, double* __restrict__ slot,
+                                                          unsigned int* flags) {
+    using Fa = FaceAt<D>;
     __shared__ int offs[QUARTER + 1];
     __shared__ int s_list[QUARTER_DEPTH * QUARTER], s_other[QUARTER_DEPTH * QUARTER];
     __shared__ double s_geo[D + 1][QUARTER_DEPTH * QUARTER];
@@ -145,10 +148,6 @@ __global__ __launch_bounds__(SLOT_BLOCK) void k_geo_slots(MeshIn m, const int* _
     __syncthreads();
     const int k0 = cells > 0 ? offs[0] : 0, k1 = cells > 0 ? offs[cells] : 0;
     const int* __restrict__ in_list = m.list;
-    const int* __restrict__ owner = m.owner;
-    const int* __restrict__ neighbour = m.neighbour;
-    const double* __restrict__ area = m.area;
-    const double* __restrict__ fc = m.face_centroid;
     const double* __restrict__ pos = m.positions;
     const double* __restrict__ cen = m.centroid;
     unsigned int bad = 0;
@@ -168,18 +167,25 @@ __global__ __launch_bounds__(SLOT_BLOCK) void k_geo_slots(MeshIn m, const int* _
             bad |= GEO_BAD_FACE;
             e = 0;
         } else {
-            const int o = owner[f], j = neighbour[f];
+            // the face's record, which k_geo_faces wrote from the same arrays (one 64 B record instead of the producer's
+            // four arrays; |A| as it computed it, the same operations)
+            const double* rec = face + (size_t)f * FACE_ROW;
+            int pair[2];
+            __builtin_memcpy(pair, &rec[Fa::PAIR], sizeof pair);
+            const int o = pair[0], j = pair[1];
             if (o < 0 || o >= m.N || j < 0 || j >= m.N) {
                 bad |= GEO_BAD_CELL;
+            } else if (o != i && j != i) {
+                bad |= GEO_BAD_FACE;  // a face of other cells in this cell's list
             } else {
+                // The owner bit from the face's owner (finite_volume.hpp:78-87 owner_layout: set when the mesh's lists
+                // lack it; a list that carries it keeps its bits).
+                e = o == i ? f : ~f;
                 cell = e >= 0 ? j : o;
-                double A[D], rel[D];
+                double rel[D];
 #pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    A[a] = area[(size_t)f * D + a];
-                    rel[a] = fc[(size_t)f * D + a];
-                }
-                mag = sqrt(dot<D>(A, A));
+                for (int a = 0; a < D; ++a) rel[a] = rec[Fa::FC + a];
+                mag = rec[Fa::MAG];
                 // Geometry::face_offset (finite_volume.hpp:322-326): the centroid moved to this cell's generator by the
                 // nearest-image separation when the cell does not own the face, then relative to its centroid
                 if (e < 0) {
@@ -254,11 +260,12 @@ hipError_t launch_geometry_faces(hipStream_t s, const MeshIn& m, double* face, u
 }
 
 template <int D>
-hipError_t launch_geometry_slots(hipStream_t s, const MeshIn& m, const int* count, const int* slice, const long long* info, int* list, int* other,
-                                 double* slot, unsigned int* flags) {
+hipError_t launch_geometry_slots(hipStream_t s, const MeshIn& m, const double* face, const int* count, const int* slice, const long long* info,
+                                 int* list, int* other, double* slot, unsigned int* flags) {
     const int S = (m.N + SLICE - 1) / SLICE;
     if (S == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_geo_slots<D>, dim3(S * (SLICE / QUARTER)), dim3(SLOT_BLOCK), 0, s, m, count, slice, info, S, list, other, slot, flags);
+    hipLaunchKernelGGL(k_geo_slots<D>, dim3(S * (SLICE / QUARTER)), dim3(SLOT_BLOCK), 0, s, m, face, count, slice, info, S, list, other, slot,
+                       flags);
     return hipGetLastError();
 }
 
@@ -266,7 +273,9 @@ template hipError_t launch_geometry_cells<2>(hipStream_t, const MeshIn&, double*
 template hipError_t launch_geometry_cells<3>(hipStream_t, const MeshIn&, double*, int*, int*, unsigned int*);
 template hipError_t launch_geometry_faces<2>(hipStream_t, const MeshIn&, double*, unsigned int*);
 template hipError_t launch_geometry_faces<3>(hipStream_t, const MeshIn&, double*, unsigned int*);
-template hipError_t launch_geometry_slots<2>(hipStream_t, const MeshIn&, const int*, const int*, const long long*, int*, int*, double*, unsigned int*);
-template hipError_t launch_geometry_slots<3>(hipStream_t, const MeshIn&, const int*, const int*, const long long*, int*, int*, double*, unsigned int*);
+template hipError_t launch_geometry_slots<2>(hipStream_t, const MeshIn&, const double*, const int*, const int*, const long long*, int*, int*, double*,
+                                             unsigned int*);
+template hipError_t launch_geometry_slots<3>(hipStream_t, const MeshIn&, const double*, const int*, const int*, const long long*, int*, int*, double*,
+                                             unsigned int*);
 
 }  // namespace luna

@@ -147,8 +147,8 @@ hipError_t launch_geometry_scan(hipStream_t s, int* slice, int S, long long* inf
 template <int D>
 hipError_t launch_geometry_faces(hipStream_t s, const MeshIn& m, double* face, unsigned int* flags);
 template <int D>
-hipError_t launch_geometry_slots(hipStream_t s, const MeshIn& m, const int* count, const int* slice, const long long* info, int* list, int* other,
-                                 double* slot, unsigned int* flags);
+hipError_t launch_geometry_slots(hipStream_t s, const MeshIn& m, const double* face, const int* count, const int* slice, const long long* info,
+                                 int* list, int* other, double* slot, unsigned int* flags);
 
 // Derivatives of the static order-1 step (HLLC or HLL) at fixed dt (adjoint_kernels.hip). Device rows:
 //   dq       N x V   in: the tangent of Q (tangent) or Q bar after the step (adjoint); the tangent advances it to J dq

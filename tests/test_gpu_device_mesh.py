@@ -128,3 +128,34 @@ def test_mesh_stats_counts_moving_updates_only():
     finally:
         a.close()
         c.close()
+
+
+def test_lists_without_the_owner_bit_step_the_same():
+    """The reference's owner_layout (finite_volume.hpp:78-87): a mesh whose lists carry plain face ids gets the owner bit
+    from owner[], with the same bits as the mesh that carries it; a list entry naming a face of other cells is refused."""
+    import cases
+
+    m = cases.voronoi(8, 3)
+    q = cases.rough_state(m)
+    plain = dict(m)
+    lst = np.asarray(m["list"]).copy()
+    plain["list"] = np.where(lst < 0, ~lst, lst).astype(lst.dtype)
+    assert np.any(lst < 0) and not np.any(plain["list"] < 0)
+    runs = []
+    for mm in (m, plain):
+        fv = hydro.FiniteVolume(mm, GAMMA)
+        fv.set_state(q)
+        dt = fv.time_step(0.3)
+        fv.step(dt)
+        runs.append((dt, fv.gradients(), fv.state()))
+        fv.close()
+    assert runs[0][0] == runs[1][0]
+    assert np.array_equal(runs[0][1], runs[1][1]) and np.array_equal(runs[0][2], runs[1][2])
+    bad = dict(m)
+    bl = lst.copy()
+    start = np.asarray(m["start"])
+    far = next(f for f in range(m["faces"]) if m["owner"][f] != 0 and m["neighbour"][f] != 0)
+    bl[start[0]] = far
+    bad["list"] = bl
+    with pytest.raises(ValueError):
+        hydro.FiniteVolume(bad, GAMMA)
