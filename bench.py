@@ -319,7 +319,7 @@ def main():
     ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--static-steps", type=int, default=20)
+    ap.add_argument("--static-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-limiter comparison")
     ap.add_argument("--no-precision", action="store_true", help="skip the grad32/grad16 policy runs")

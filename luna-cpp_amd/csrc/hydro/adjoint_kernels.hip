@@ -13,6 +13,21 @@ constexpr int BLOCK = 64;  // one wavefront per work-group: the per-cell walks d
 
 inline dim3 grid_of(long long n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 
+// The heaviest per-cell kernels (the fit's tangent, the cell adjoint) need every register a lane can hold; at 2 waves per
+// SIMD they spill a few bytes and run faster than at 1 (the compiler's choice without a floor).
+#ifndef LUNA_DERIV_WAVES
+#define LUNA_DERIV_WAVES 2
+#endif
+
+// The logical block of this work-group, as the forward kernels map them: work-groups go to the 8 XCDs in turn (block b
+// to XCD b mod 8), so XCD x takes the contiguous run [x q + min(x, r), ...) of q + (x < r) blocks (q = n / 8, r = n mod
+// 8), and the neighbours a cell's walk gathers sit in that XCD's L2.
+__device__ __forceinline__ int xcd_block() {
+    const int b = blockIdx.x, n = gridDim.x;
+    const int q = n >> 3, rem = n & 7, x = b & 7, k = b >> 3;
+    return x * q + min(x, rem) + k;
+}
+
 __device__ __forceinline__ int first_slot(const Geo& g, int i) { return g.slice[i >> 6] + (i & (SLICE - 1)); }
 
 template <int D>
@@ -289,7 +304,7 @@ template <int D>
 __global__ __launch_bounds__(BLOCK) void k_prim_tangent(Geo g, Rows r, Gas gas, const double* dq, double* dW) {
     constexpr int V = D + 2;
     using C = CellAt<D>;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
     if (i >= g.N) return;
     double* row = g.cell + (size_t)i * CELL_ROW;
     const double inv = 1.0 / row[C::VOL];
@@ -309,10 +324,10 @@ __global__ __launch_bounds__(BLOCK) void k_prim_tangent(Geo g, Rows r, Gas gas, 
 // The limited gradient (into r.grad) and its tangent: after the last clip g = g0 (bound - W_i) / (g0 . o), whatever clips
 // came before (finite_volume.hpp:809-831).
 template <int D>
-__global__ __launch_bounds__(BLOCK) void k_fit_tangent(Geo g, Rows r, int limit, const double* dW, double* dgrad) {
+__global__ __launch_bounds__(BLOCK, LUNA_DERIV_WAVES) void k_fit_tangent(Geo g, Rows r, int limit, const double* dW, double* dgrad) {
     constexpr int V = D + 2, U = D + 1, GR = grad_row<D>();
     using C = CellAt<D>;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
     if (i >= g.N) return;
     double gl[V * D], g0[V * D], dg0[V * D], L[U][U], M0[U];
     Pattern<D> pat;
@@ -351,8 +366,9 @@ __global__ __launch_bounds__(BLOCK) void k_flux_tangent(Geo g, Rows r, Gas gas, 
     using C = CellAt<D>;
     const int f = blockIdx.x * BLOCK + threadIdx.x;
     if (f >= g.F) return;
-    double* Fv = r.flux + (size_t)f * V;
-    double* dF = dflux + (size_t)f * V;
+    // the face's flux and its tangent side by side (2V doubles a face), so that the update gathers one row per entry
+    double* Fv = dflux + (size_t)f * 2 * V;
+    double* dF = Fv + V;
     const double* face = g.face + (size_t)f * FACE_ROW;
     if (face[FaceAt<D>::MAG] == 0) {
 #pragma unroll
@@ -403,19 +419,46 @@ __global__ __launch_bounds__(BLOCK) void k_flux_tangent(Geo g, Rows r, Gas gas, 
 template <int D>
 __global__ __launch_bounds__(BLOCK) void k_update_tangent(Geo g, Rows r, double dt, const double* dflux, double* dq) {
     constexpr int V = D + 2;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
     if (i >= g.N) return;
     double sum[V], dsum[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) sum[v] = dsum[v] = 0;
     const int count = g.count[i], base = first_slot(g, i);
-    for (int k = 0; k < count; ++k) {
+    constexpr int UN = 4;  // the rows of UN entries in flight, summed in list order
+    int k = 0;
+    for (; k + UN - 1 < count; k += UN) {
+        int e[UN];
+#pragma unroll
+        for (int u = 0; u < UN; ++u) e[u] = g.list[base + (k + u) * SLICE];
+        double F[UN][V], dF[UN][V];
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            const double* row = dflux + (size_t)(e[u] >= 0 ? e[u] : ~e[u]) * 2 * V;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                F[u][v] = row[v];
+                dF[u][v] = row[V + v];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            const double sign = e[u] >= 0 ? 1.0 : -1.0;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                sum[v] += sign * F[u][v];
+                dsum[v] += sign * dF[u][v];
+            }
+        }
+    }
+    for (; k < count; ++k) {
         const int e = g.list[base + k * SLICE], f = e >= 0 ? e : ~e;
         const double sign = e >= 0 ? 1.0 : -1.0;
+        const double* row = dflux + (size_t)f * 2 * V;
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            sum[v] += sign * r.flux[(size_t)f * V + v];
-            dsum[v] += sign * dflux[(size_t)f * V + v];
+            sum[v] += sign * row[v];
+            dsum[v] += sign * row[V + v];
         }
     }
 #pragma unroll
@@ -459,11 +502,11 @@ __global__ __launch_bounds__(BLOCK) void k_face_adjoint(Geo g, Rows r, Gas gas, 
 // (finite_volume.hpp:832-864): the bound's bar (for the neighbour holding it), b bar = M^-1 x bar (for the neighbours),
 // and the cell's own share.
 template <int D>
-__global__ __launch_bounds__(BLOCK) void k_cell_adjoint(Geo g, Rows r, Gas gas, double dt, int limit, const double* sidebar, double* Wbar,
+__global__ __launch_bounds__(BLOCK, LUNA_DERIV_WAVES) void k_cell_adjoint(Geo g, Rows r, Gas gas, double dt, int limit, const double* sidebar, double* Wbar,
                                                         double* bbar, double* boundbar, int* extreme) {
     constexpr int V = D + 2, U = D + 1, GR = grad_row<D>();
     using C = CellAt<D>;
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int i = xcd_block() * BLOCK + threadIdx.x;
     if (i >= g.N) return;
     double Wi[V], gi[V * D];
 #pragma unroll
@@ -475,24 +518,51 @@ __global__ __launch_bounds__(BLOCK) void k_cell_adjoint(Geo g, Rows r, Gas gas, 
     for (int v = 0; v < V; ++v) Wb[v] = 0;
 #pragma unroll
     for (int k = 0; k < V * D; ++k) gb[k] = 0;
+    // This cell's side of each of its faces, as face_sides computes it for the face kernel: the half step is the cell's
+    // own (the same for all its faces), plus g . r with r the face centroid in this cell's frame minus its centroid; only
+    // the fallback flag and r are needed here, so the other side's rows are not read.
+    double hs[V], ci[D], xi[D];
+    half_step<D>(gas, Wi, gi, dt, hs);
+    load_vec<D>(g.cell + (size_t)i * CELL_ROW + C::C, ci);
+    load_vec<D>(g.cell + (size_t)i * CELL_ROW + C::X, xi);
+    const double half = 0.5 * dt, wf = 0.0;
     const int count = g.count[i], base = first_slot(g, i);
     for (int k = 0; k < count; ++k) {
         const int e = g.list[base + k * SLICE], f = e >= 0 ? e : ~e;
-        if (g.face[(size_t)f * FACE_ROW + FaceAt<D>::MAG] == 0) continue;
-        double Ls[V], Rs[V], ro[D], rj[D], n[D], magnitude;
-        bool fallback[2];
-        face_sides<D>(g, r, gas, f, dt, Ls, Rs, fallback, ro, rj, n, magnitude);
+        const double* face = g.face + (size_t)f * FACE_ROW;
+        if (face[FaceAt<D>::MAG] == 0) continue;
         const int s = e >= 0 ? 0 : 1;
+        double rr[D];
+        if (s == 0) {
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                rr[c] = face[FaceAt<D>::FC + c] - ci[c];
+                rr[c] = rr[c] + wf * half;
+            }
+        } else {
+            const int2 pair = *reinterpret_cast<const int2*>(face + FaceAt<D>::PAIR);
+            double xo[D], d[D];
+            load_vec<D>(g.cell + (size_t)pair.x * CELL_ROW + C::X, xo);
+            separation<D>(g, xi, xo, d);  // the owner seen from this cell, the neighbour
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                rr[c] = face[FaceAt<D>::FC + c] + d[c] - ci[c];
+                rr[c] = rr[c] + wf * half;
+            }
+        }
+        double Wf[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) Wf[v] = hs[v] + dot<D>(gi + v * D, rr);
+        const bool fallback = !(Wf[0] > 0 && Wf[D + 1] > 0);
         const double* side = sidebar + (size_t)f * 2 * V + s * V;
         double sb[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) sb[v] = side[v];
-        if (fallback[s]) {
+        if (fallback) {
 #pragma unroll
             for (int v = 0; v < V; ++v) Wb[v] += sb[v];
             continue;
         }
-        const double* rr = s == 0 ? ro : rj;
         double hb[V * D];
 #pragma unroll
         for (int v = 0; v < V; ++v)
@@ -567,7 +637,7 @@ __global__ __launch_bounds__(BLOCK) void k_gather_adjoint(Geo g, Rows r, Gas gas
                                                           const double* boundbar, const int* extreme, double* out) {
     constexpr int V = D + 2, U = D + 1;
     using C = CellAt<D>;
-    const int j = blockIdx.x * BLOCK + threadIdx.x;
+    const int j = xcd_block() * BLOCK + threadIdx.x;
     if (j >= g.N) return;
     const double* mine = g.cell + (size_t)j * CELL_ROW;
     double xj[D], cj[D], Wb[V];
@@ -583,7 +653,7 @@ __global__ __launch_bounds__(BLOCK) void k_gather_adjoint(Geo g, Rows r, Gas gas
         double xi[D], d[D], X[U];
         load_vec<D>(row + C::X, xi);
         separation<D>(g, xi, xj, d);  // j seen from i, as i's fit saw it
-        const double w = g.face[(size_t)f * FACE_ROW + FaceAt<D>::MAG];
+        const double w = g.slot[slot];  // |A|, the face record's value, read coalesced from the entry's slot
         X[0] = 1;
 #pragma unroll
         for (int p = 1; p < U; ++p) X[p] = d[p - 1] + cj[p - 1] - row[C::C + p - 1];

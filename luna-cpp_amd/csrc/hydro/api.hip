@@ -662,7 +662,7 @@ int derivatives_ready(luna_fv* fv, const char* what) {
     LUNA_HIP(fv->d_dq.reserve(N * V));
     LUNA_HIP(fv->d_dW.reserve(N * V));
     LUNA_HIP(fv->d_dgrad.reserve(N * fv->grad_row()));
-    LUNA_HIP(fv->d_dflux.reserve(F * V > 0 ? F * V : 1));
+    LUNA_HIP(fv->d_dflux.reserve(2 * F * V > 0 ? 2 * F * V : 1));  // each face's flux and its tangent
     LUNA_HIP(fv->d_sidebar.reserve(2 * F * V > 0 ? 2 * F * V : 1));
     LUNA_HIP(fv->d_bbar.reserve(N * V * (fv->D + 1)));
     LUNA_HIP(fv->d_boundbar.reserve(N * V));

@@ -154,7 +154,7 @@ hipError_t launch_geometry_slots(hipStream_t s, const MeshIn& m, const double* f
 //   dq       N x V   in: the tangent of Q (tangent) or Q bar after the step (adjoint); the tangent advances it to J dq
 //   dW       N x V   W tangents (tangent) or W bars (adjoint)
 //   dgrad    N x grad_row<D>()   gradient tangents
-//   dflux    F x V   flux-row tangents
+//   dflux    F x 2V  each face's flux row and its tangent, side by side
 //   sidebar  F x 2V  the bars of each face's two sides
 //   bbar     N x V (D + 1), boundbar N x V, extreme N x V: each cell's fit adjoint for its neighbours
 //   out      N x V   J^T Q bar (adjoint)

@@ -241,15 +241,26 @@ def owners(coarse, fine):
 
 def restrict(q_fine, fine, coarse, owner):
     """The fine state on the coarse cells: conserved densities averaged over the fine cells each coarse cell holds,
-    times the coarse volume (so the coarse densities are the fine averages)."""
+    times the coarse volume (so the coarse densities are the fine averages). A coarse cell that holds no fine generator
+    (a small cell of a distorted moving mesh, against a reference only twice as fine) takes the densities of the fine
+    cell nearest to its generator."""
     Nc, V = coarse["cells"], q_fine.shape[1]
     vol_f = np.asarray(fine["volume"])
     mass = np.zeros((Nc, V))
     for v in range(V):
         mass[:, v] = np.bincount(owner, weights=q_fine[:, v], minlength=Nc)
     held = np.bincount(owner, weights=vol_f, minlength=Nc)
-    if np.any(held <= 0):
-        raise ValueError("a coarse cell holds no fine generator")
+    empty = np.flatnonzero(held <= 0)
+    if len(empty):
+        from scipy.spatial import cKDTree
+
+        D = coarse["dim"]
+        side = np.asarray(coarse["side"][:D], dtype=np.float64)
+        xf = np.asarray(fine["positions"]).reshape(-1, D) % side
+        xc = np.asarray(coarse["positions"]).reshape(-1, D)[empty] % side
+        near = cKDTree(xf, boxsize=side).query(xc)[1]
+        mass[empty] = q_fine[near]
+        held[empty] = vol_f[near]
     return mass / held[:, None] * np.asarray(coarse["volume"])[:, None]
 
 

@@ -191,3 +191,22 @@ def test_shipped_network_file():
     assert p.dtype == np.float32 and p.shape == (mlp.PARAMS,) and np.all(np.isfinite(p))
     f = mlp.forward(p, mlp.bf16_bits(np.zeros((4, 16))))["fraction"]
     assert np.all((f > 0) & (f < 1))
+
+
+def test_restrict_fills_a_coarse_cell_without_fine_generators():
+    """learned.restrict: volume-weighted averages of the fine cells each coarse cell holds; a coarse cell that holds none
+    takes the densities of the fine cell nearest to its generator (periodic box)."""
+    from luna_hydro import learned
+
+    coarse = {"dim": 2, "cells": 3, "side": [1.0, 1.0, 1.0], "positions": np.array([[0.1, 0.1], [0.6, 0.6], [0.95, 0.1]]),
+              "volume": np.array([0.5, 0.4, 0.1])}
+    fine = {"dim": 2, "cells": 4, "side": [1.0, 1.0, 1.0],
+            "positions": np.array([[0.1, 0.2], [0.2, 0.1], [0.6, 0.5], [0.7, 0.7]]), "volume": np.array([0.25, 0.25, 0.25, 0.25])}
+    rho = np.array([1.0, 3.0, 5.0, 7.0])
+    q_fine = np.column_stack([rho, 2 * rho]) * fine["volume"][:, None]
+    owner = np.array([0, 0, 1, 1])  # coarse cell 2 holds no fine generator
+    q = learned.restrict(q_fine, fine, coarse, owner)
+    dens = q / coarse["volume"][:, None]
+    assert np.allclose(dens[0], [2.0, 4.0]) and np.allclose(dens[1], [6.0, 12.0])
+    # across the periodic x boundary (0.1, 0.2) is 0.18 from (0.95, 0.1) and (0.2, 0.1) 0.25: rho 1
+    assert np.allclose(dens[2], [1.0, 2.0])
