@@ -168,8 +168,8 @@ def run_moving(m, q, steps, warmup, precision="double", hll=0.0, update="per_fac
 
 def cpu_baseline(m, q, static_steps=40, moving_steps=10, warmup=2):
     """The oracle (CPU restatement of luna-cpp's FP64 algorithm, C + OpenMP) on the same mesh and state, on this host's
-    cores (OpenMP's own placement: OMP_PROC_BIND=close put all threads on one core of the GPU box, whose affinity mask is
-    wider than its CPU share, so it is not set here): the static step's median wall over `static_steps` after `warmup`, as
+    cores (OpenMP's own placement: on the MI355X's host, whose affinity mask is wider than the process's CPU share,
+    OMP_PROC_BIND=close put all threads on one core, so it is not set here): the static step's median wall over `static_steps` after `warmup`, as
     hydro_sedov's run line; the moving step's hydro part (mesh producer excluded) over `moving_steps`; and one round of
     each on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -18,7 +18,8 @@ def test_sedov_driver_checksum_matches_oracle(tmp_path, precision):
     res = sedov.run(n=10, moving=False, t_end=0.05, courant=0.3, snapshots=10, out=str(tmp_path), max_steps=4, log=lines.append,
                     precision=precision)
     print("\n".join(lines), flush=True)
-    assert [l.split(":")[0] for l in lines] == ["run", "fallbacks", "phases_ms_median", "conservation", "checksum"]
+    # the reference's report lines, then this build's shock-radius line
+    assert [l.split(":")[0] for l in lines] == ["run", "fallbacks", "phases_ms_median", "conservation", "checksum", "shock"]
     m = mesh.build(problems.jittered_lattice(10, 3, 0.25, 1), 1.0)
     q, _ = problems.sedov_state(m)
     orc = oracle.FiniteVolume(m, 5.0 / 3.0, precision=hydro.policy(precision, 3))
@@ -33,7 +34,7 @@ def test_sedov_driver_checksum_matches_oracle(tmp_path, precision):
             t += dt
         orc.step(dt)
     want = "checksum: steps=%d Q=%x" % (res["steps"], sedov.bits_hash(orc.q))
-    assert lines[-1].startswith(want), (lines[-1], want)
+    assert lines[4].startswith(want), (lines[4], want)
     snap = snapshot.read(str(tmp_path / "snapdir_000" / "snapshot_000.0.hdf5"))
     assert np.array_equal(snap["U"], (q / m["volume"][:, None]).T)
 
