@@ -112,7 +112,8 @@ public:
         return n;
     }
     // Derivatives of one static order-1 step at fixed dt (:1941-2231): step_tangent advances Q and dq (dq becomes J dq);
-    // step_adjoint maps qbar after the step to J^T qbar before it (Q stays); time_step_tangent is time_step() for them.
+    // step_adjoint maps qbar after the step to J^T qbar before it (Q stays); time_step_tangent is time_step() with the
+    // fit's tangent for dq, which the next step_tangent with the same dq reuses.
     void step_tangent(double dt, std::vector<double>& dq) { check(luna_fv_step_tangent(fv_, dt, dq.data(), 0), "step_tangent"); }
     void step_adjoint(double dt, std::vector<double>& qbar) { check(luna_fv_step_adjoint(fv_, dt, qbar.data(), 0), "step_adjoint"); }
     double time_step_tangent(double courant, const std::vector<double>& dq) {

@@ -267,9 +267,11 @@ int luna_fv_step_adjoint(luna_fv* fv, double dt, double* qbar, int device);
  * call on a GPU uses that GPU's one scratch array, so calls that overlap on one GPU (two threads, or two streams in
  * flight) race; each call writes every row it reads before reading it. Blocks, as luna_fv_step_adjoint. */
 int luna_fv_step_adjoint_noalloc(luna_fv* fv, double dt, double* qbar);
-/* FiniteVolume::time_step_tangent(courant, dq) (:1941-1964): dt, as luna_fv_time_step; the reference also keeps the sweep's
- * tangent for the next step_tangent with the same dq, which here takes it itself (the same numbers), so dq is not read.
- * The derivatives' conditions apply (LUNA_ERR_LOGIC otherwise). */
+/* FiniteVolume::time_step_tangent(courant, dq) (:1941-1964): dt, as luna_fv_time_step (the same terms, the same bits),
+ * with the primitives' and the limited fit's tangents for dq (N x V, host or device as `device` says) kept for the next
+ * luna_fv_step_tangent, which must be called with the same dq (the reference's contract); any other call in between
+ * (set_state, time_step, step, ...) drops them and step_tangent takes them itself. The derivatives' conditions apply
+ * (LUNA_ERR_LOGIC otherwise). */
 int luna_fv_time_step_tangent(luna_fv* fv, double courant, const double* dq, int device, double* dt);
 
 /* The geometry's derivative the moving step's reverse reads (hydro::GeometryDerivative::adjoint, mesh_derivative.hpp:28-29): from

@@ -1739,10 +1739,19 @@ __global__ __launch_bounds__(BLOCK) void k_axpy(long long n, double dt, const do
 }  // namespace
 
 template <int D>
-hipError_t launch_step_tangent(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, bool limit, const Derivs& w) {
+hipError_t launch_sweep_tangent(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, bool limit, const Derivs& w) {
     if (g.N == 0) return hipSuccess;
     hipLaunchKernelGGL(k_prim_tangent<D>, grid_of(g.N), dim3(BLOCK), 0, s, g, r, gas, w.dq, w.dW);
     hipLaunchKernelGGL(k_fit_tangent<D>, grid_of(g.N), dim3(BLOCK), 0, s, g, r, limit ? 1 : 0, w.dW, w.dgrad);
+    return hipGetLastError();
+}
+
+template <int D>
+hipError_t launch_step_tangent(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, bool limit, const Derivs& w,
+                               bool fitted) {
+    if (g.N == 0) return hipSuccess;
+    if (!fitted)
+        if (hipError_t e = launch_sweep_tangent<D>(s, g, r, gas, limit, w)) return e;
     if (g.F > 0) hipLaunchKernelGGL(k_flux_tangent<D>, grid_of(g.F), dim3(BLOCK), 0, s, g, r, gas, riemann, dt, w.dW, w.dgrad, w.dflux);
     hipLaunchKernelGGL(k_update_tangent<D>, grid_of(g.N), dim3(BLOCK), 0, s, g, r, dt, w.dflux, w.dq);
     return hipGetLastError();
@@ -1836,8 +1845,10 @@ template hipError_t launch_moving_pass_adjoint<3>(hipStream_t, const Geo&, const
                                                   const MovingBars&, bool);
 template hipError_t launch_moving_cell_adjoint<2>(hipStream_t, const Geo&, const Rows&, const Gas&, bool, bool, const double*, const MovingBars&);
 template hipError_t launch_moving_cell_adjoint<3>(hipStream_t, const Geo&, const Rows&, const Gas&, bool, bool, const double*, const MovingBars&);
-template hipError_t launch_step_tangent<2>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&);
-template hipError_t launch_step_tangent<3>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&);
+template hipError_t launch_sweep_tangent<2>(hipStream_t, const Geo&, const Rows&, const Gas&, bool, const Derivs&);
+template hipError_t launch_sweep_tangent<3>(hipStream_t, const Geo&, const Rows&, const Gas&, bool, const Derivs&);
+template hipError_t launch_step_tangent<2>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&, bool);
+template hipError_t launch_step_tangent<3>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&, bool);
 template hipError_t launch_step_adjoint<2>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&);
 template hipError_t launch_step_adjoint<3>(hipStream_t, const Geo&, const Rows&, const Gas&, int, double, bool, const Derivs&);
 

@@ -172,6 +172,7 @@ struct luna_fv {
     int parity = 0;
     long long updates = 0;  // mesh updates of moving steps (MeshStats::updates, finite_volume.hpp:1237)
     bool prims_ready = false, swept = false, begun = false, new_mesh = false;
+    bool tangent_swept = false;  // time_step_tangent wrote the fit's tangent rows for the next step_tangent (same dq)
     Profiler prof;
     // a host mesh's arrays in device memory (the geometry pass reads them as it reads a device mesh); the centroids,
     // positions and volumes share src_cells
@@ -396,12 +397,12 @@ std::vector<unsigned char> network_bytes(const float* params) {
     return out;
 }
 
-int sweep(luna_fv* fv, double courant) {
+int sweep(luna_fv* fv, double courant, bool dt_only = false) {
     const luna::Geo g = fv->geo();
     const luna::Rows r = fv->rows();
     const luna::Gas gas = fv->gas();
     const luna::Counters c = fv->counters_now();
-    const bool moving = fv->opt.motion == LUNA_MOTION_MOVING, grad = fv->opt.order == 1;
+    const bool moving = fv->opt.motion == LUNA_MOTION_MOVING, grad = fv->opt.order == 1 && !dt_only;
     const bool learned = fv->opt.limiter == LUNA_LIMITER_LEARNED;
     const int limit = learned ? 2 : fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID ? 1 : 0;
     hipStream_t s = fv->stream;
@@ -631,6 +632,7 @@ int luna_fv_time_step(luna_fv* fv, double courant, double* dt) {
     LUNA_HIP(fv->prof.collect());
     *dt = bits_to_double(fv->host[0]);
     fv->swept = true;
+    fv->tangent_swept = false;
     return LUNA_OK;
 }
 
@@ -689,10 +691,12 @@ int luna_fv_step_tangent(luna_fv* fv, double dt, double* dq, int device) {
     const luna::Gas gas = fv->gas();
     const luna::Derivs w = derivs_of(fv);
     const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID;
+    const bool fitted = fv->swept && fv->tangent_swept;  // time_step_tangent's rows (the reference's contract: the same dq)
     LUNA_HIP(timed(fv->prof, s, "step_tangent", [&] {
-        return fv->D == 3 ? luna::launch_step_tangent<3>(s, g, r, gas, fv->opt.riemann, dt, limit, w)
-                          : luna::launch_step_tangent<2>(s, g, r, gas, fv->opt.riemann, dt, limit, w);
+        return fv->D == 3 ? luna::launch_step_tangent<3>(s, g, r, gas, fv->opt.riemann, dt, limit, w, fitted)
+                          : luna::launch_step_tangent<2>(s, g, r, gas, fv->opt.riemann, dt, limit, w, fitted);
     }));
+    fv->tangent_swept = false;
     LUNA_HIP(hipMemcpyAsync(dq, fv->d_dq.p, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     LUNA_HIP(hipStreamSynchronize(s));
     fv->prims_ready = false;
@@ -701,10 +705,34 @@ int luna_fv_step_tangent(luna_fv* fv, double dt, double* dq, int device) {
 }
 
 int luna_fv_time_step_tangent(luna_fv* fv, double courant, const double* dq, int device, double* dt) {
-    (void)device;
     if (!fv || !dq || !dt) return fail(LUNA_ERR_ARGUMENT, "luna_fv_time_step_tangent: null argument");
     if (int rc = derivatives_ready(fv, "luna_fv_time_step_tangent")) return rc;
-    return luna_fv_time_step(fv, courant, dt);
+    // As the reference's (its sweep with the tangent hook): the primitives and their tangents, the limited fit and its
+    // tangent (k_fit_tangent writes the gradients the step uses and their tangents), and dt from a sweep of the time
+    // step's terms alone (the same terms, so the same dt as time_step's). The next step_tangent with the same dq reuses
+    // the rows.
+    hipStream_t s = fv->stream;
+    LUNA_HIP(hipMemcpyAsync(fv->d_dq.p, dq, sizeof(double) * fv->N * fv->V, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    if (!fv->prims_ready)
+        if (int rc = primitives(fv)) return rc;
+    fv->prims_ready = false;
+    const int slot = fv->parity;
+    if (int rc = sweep(fv, courant, true)) return rc;
+    const luna::Geo g = fv->geo();
+    const luna::Rows r = fv->rows();
+    const luna::Gas gas = fv->gas();
+    const luna::Derivs w = derivs_of(fv);
+    const bool limit = fv->opt.limiter == LUNA_LIMITER_FACE_CENTROID;
+    LUNA_HIP(timed(fv->prof, s, "sweep_tangent", [&] {
+        return fv->D == 3 ? luna::launch_sweep_tangent<3>(s, g, r, gas, limit, w) : luna::launch_sweep_tangent<2>(s, g, r, gas, limit, w);
+    }));
+    LUNA_HIP(hipMemcpyAsync(fv->host, fv->counters.p + slot, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    LUNA_HIP(hipStreamSynchronize(s));
+    LUNA_HIP(fv->prof.collect());
+    *dt = bits_to_double(fv->host[0]);
+    fv->swept = true;
+    fv->tangent_swept = true;
+    return LUNA_OK;
 }
 
 int luna_fv_step_adjoint(luna_fv* fv, double dt, double* qbar, int device) {

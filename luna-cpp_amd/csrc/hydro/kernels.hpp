@@ -169,10 +169,15 @@ struct Derivs {
     int* extreme;
     double* out;
 };
-// The tangent writes Q' into r.q, its flux rows into r.flux and the limited gradients into r.grad; the adjoint needs the
-// primitive rows and the limited gradients of Q (the sweep) and leaves Q as it is.
+// The tangent writes Q' into r.q, each face's flux and its tangent into dflux and the limited gradients into r.grad
+// (fitted: time_step_tangent has written the gradients, dW and dgrad already); the adjoint needs the primitive rows and the
+// limited gradients of Q (the sweep) and leaves Q as it is.
 template <int D>
-hipError_t launch_step_tangent(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, bool limit, const Derivs& w);
+hipError_t launch_step_tangent(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, bool limit, const Derivs& w,
+                               bool fitted = false);
+// The primitives' and the limited fit's tangents (and the forward gradients), as time_step_tangent takes them.
+template <int D>
+hipError_t launch_sweep_tangent(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, bool limit, const Derivs& w);
 template <int D>
 hipError_t launch_step_adjoint(hipStream_t s, const Geo& g, const Rows& r, const Gas& gas, int riemann, double dt, bool limit, const Derivs& w);
 

@@ -244,12 +244,20 @@ class FiniteVolume:
         return n.value
 
     def time_step_tangent(self, courant, dq):
-        """FiniteVolume::time_step_tangent(courant, dq): dt as time_step(); the next step_tangent(dt, dq) takes the sweep's
-        tangent itself."""
+        """FiniteVolume::time_step_tangent(courant, dq): dt as time_step(), with the primitives' and the limited fit's
+        tangents for dq, which the next step_tangent(dt, dq) with the same dq reuses. dq: N x V, host (numpy) or device
+        (an object with data_ptr())."""
         dt = ctypes.c_double()
-        probe = np.zeros(1)
-        _check(lib().luna_fv_time_step_tangent(self._h, ctypes.c_double(courant), probe.ctypes.data_as(ctypes.c_void_p), 0,
-                                               ctypes.byref(dt)), "time_step_tangent")
+        if hasattr(dq, "data_ptr"):
+            if not dq.is_contiguous() or dq.numel() != self.N * self.V:
+                raise ValueError("time_step_tangent: a contiguous N x V device array")
+            ptr, device = ctypes.c_void_p(dq.data_ptr()), 1
+        else:
+            rows = np.ascontiguousarray(np.array(dq, dtype=np.float64))
+            if rows.size != self.N * self.V:
+                raise ValueError("time_step_tangent: N x V rows")
+            ptr, device = rows.ctypes.data_as(ctypes.c_void_p), 0
+        _check(lib().luna_fv_time_step_tangent(self._h, ctypes.c_double(courant), ptr, device, ctypes.byref(dt)), "time_step_tangent")
         return dt.value
 
     def step_tangent(self, dt, dq):

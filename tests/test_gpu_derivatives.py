@@ -241,6 +241,21 @@ def test_time_step_tangent_is_time_step_for_the_tangent():
         c.set_state(q)
         dt = a.time_step(0.3)
         assert b.time_step_tangent(0.3, dq) == dt
+        assert np.array_equal(b.gradients(), a.gradients())  # the fit's tangent pass wrote the step's gradients
+        assert np.array_equal(a.step_tangent(dt, dq.copy()), b.step_tangent(dt, dq.copy()))
+        assert np.array_equal(a.state(), b.state())
+        # a plain step after time_step_tangent steps as after time_step
+        a.set_state(q)
+        b.set_state(q)
+        a.step(a.time_step(0.3))
+        b.time_step_tangent(0.3, dq)
+        b.step(dt)
+        assert np.array_equal(a.state(), b.state())
+        # a state set in between drops the kept rows: step_tangent takes the fit's tangent itself
+        a.set_state(q)
+        b.time_step_tangent(0.3, 2 * dq)
+        b.set_state(q)
+        assert a.time_step(0.3) == b.time_step(0.3)
         assert np.array_equal(a.step_tangent(dt, dq.copy()), b.step_tangent(dt, dq.copy()))
         with pytest.raises(RuntimeError):
             c.time_step_tangent(0.3, dq)
