@@ -149,7 +149,7 @@ struct luna_fv {
     int D = 3, V = 5, N = 0, F = 0, E = 0, slots = 0;
     hipStream_t stream = nullptr;
     Buf<int> count, slice, list, other, steps;
-    Buf<double> cell, facerec, slotgeo;
+    Buf<double> cell, volume, facerec, slotgeo;
     Buf<double> q, grad, w, x, old_centroid, flux;
     Buf<unsigned int> exponents;  // grad16: the per-cell group exponents (int8 x 3)
     Buf<unsigned char> hll;       // the HLL fallback's per-cell compression flags
@@ -192,6 +192,7 @@ struct luna_fv {
         g.list = list.p;
         g.other = other.p;
         g.cell = cell.p;
+        g.volume = volume.p;
         g.face = facerec.p;
         g.slot = slotgeo.p;
         g.slots = slots;
@@ -255,8 +256,8 @@ int derive(luna_fv* fv, const luna_geometry* m, luna::MeshIn in) {
     LUNA_HIP(fv->flux.reserve(static_cast<size_t>(F) * fv->V));
     LUNA_HIP(hipMemsetAsync(fv->geo_flags.p, 0, sizeof(unsigned int), s));
     LUNA_HIP(timed(fv->prof, s, "geometry_cells", [&] {
-        return D == 3 ? luna::launch_geometry_cells<3>(s, in, fv->cell.p, fv->count.p, fv->slice.p, fv->geo_flags.p)
-                      : luna::launch_geometry_cells<2>(s, in, fv->cell.p, fv->count.p, fv->slice.p, fv->geo_flags.p);
+        return D == 3 ? luna::launch_geometry_cells<3>(s, in, fv->cell.p, fv->volume.p, fv->count.p, fv->slice.p, fv->geo_flags.p)
+                      : luna::launch_geometry_cells<2>(s, in, fv->cell.p, fv->volume.p, fv->count.p, fv->slice.p, fv->geo_flags.p);
     }));
     LUNA_HIP(timed(fv->prof, s, "geometry_scan", [&] { return luna::launch_geometry_scan(s, fv->slice.p, S, fv->geo_info.p, fv->geo_flags.p); }));
     unsigned long long* back = fv->host + 4;
@@ -517,7 +518,7 @@ int create(const luna_fv_options* opt, const luna_geometry* mesh, luna_fv** out,
     hipError_t e = hipSuccess;
     if ((e = fv->q.reserve(N * V)) != hipSuccess || (e = fv->count.reserve(N)) != hipSuccess ||
         (e = fv->grad.reserve(N * fv->grad_row())) != hipSuccess || (e = fv->w.reserve(N * D)) != hipSuccess ||
-        (e = fv->cell.reserve(N * luna::CELL_ROW)) != hipSuccess ||
+        (e = fv->cell.reserve(N * luna::CELL_ROW)) != hipSuccess || (e = fv->volume.reserve(N)) != hipSuccess ||
         (e = fv->counters.reserve(4)) != hipSuccess || (e = fv->geo_info.reserve(1)) != hipSuccess ||
         (e = fv->geo_flags.reserve(1)) != hipSuccess)
         return cleanup(fail(LUNA_ERR_MEMORY, std::string("luna: device allocation: ") + hipGetErrorString(e)));
@@ -567,7 +568,7 @@ int luna_fv_destroy(luna_fv* fv) {
     if (!fv) return LUNA_OK;
     if (fv->stream) (void)hipStreamSynchronize(fv->stream);
     fv->prof.destroy();
-    for (auto* b : {&fv->cell, &fv->facerec, &fv->slotgeo, &fv->q, &fv->grad, &fv->w, &fv->x, &fv->old_centroid, &fv->flux}) b->release();
+    for (auto* b : {&fv->cell, &fv->volume, &fv->facerec, &fv->slotgeo, &fv->q, &fv->grad, &fv->w, &fv->x, &fv->old_centroid, &fv->flux}) b->release();
     for (auto* b : {&fv->count, &fv->slice, &fv->list, &fv->other, &fv->steps, &fv->d_extreme}) b->release();
     fv->exponents.release();
     fv->hll.release();

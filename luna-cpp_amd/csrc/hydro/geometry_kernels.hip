@@ -24,7 +24,7 @@ constexpr int BLOCK = 256;
 __device__ __forceinline__ void flag(unsigned int* flags, unsigned int bit) { atomicOr(flags, bit); }
 
 template <int D>
-__global__ __launch_bounds__(SLICE) void k_geo_cells(MeshIn m, double* cell, int* count, int* longest, unsigned int* flags) {
+__global__ __launch_bounds__(SLICE) void k_geo_cells(MeshIn m, double* cell, double* volume, int* count, int* longest, unsigned int* flags) {
     using C = CellAt<D>;
     const int i = blockIdx.x * SLICE + threadIdx.x;
     const long long E = 2LL * m.F;
@@ -47,6 +47,7 @@ __global__ __launch_bounds__(SLICE) void k_geo_cells(MeshIn m, double* cell, int
         }
         const double v = m.volume[i];
         row[C::VOL] = v;
+        volume[i] = v;
         row[C::RAD] = cell_radius<D>(v);
         double* out = cell + (size_t)i * CELL_ROW;
 #pragma unroll
@@ -240,10 +241,10 @@ __global__ __launch_bounds__(SLOT_BLOCK) void k_geo_slots(MeshIn m, const double
 }  // namespace
 
 template <int D>
-hipError_t launch_geometry_cells(hipStream_t s, const MeshIn& m, double* cell, int* count, int* slice, unsigned int* flags) {
+hipError_t launch_geometry_cells(hipStream_t s, const MeshIn& m, double* cell, double* volume, int* count, int* slice, unsigned int* flags) {
     const int S = (m.N + SLICE - 1) / SLICE;
     if (S == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_geo_cells<D>, dim3(S), dim3(SLICE), 0, s, m, cell, count, slice, flags);
+    hipLaunchKernelGGL(k_geo_cells<D>, dim3(S), dim3(SLICE), 0, s, m, cell, volume, count, slice, flags);
     return hipGetLastError();
 }
 
@@ -269,8 +270,8 @@ hipError_t launch_geometry_slots(hipStream_t s, const MeshIn& m, const double* f
     return hipGetLastError();
 }
 
-template hipError_t launch_geometry_cells<2>(hipStream_t, const MeshIn&, double*, int*, int*, unsigned int*);
-template hipError_t launch_geometry_cells<3>(hipStream_t, const MeshIn&, double*, int*, int*, unsigned int*);
+template hipError_t launch_geometry_cells<2>(hipStream_t, const MeshIn&, double*, double*, int*, int*, unsigned int*);
+template hipError_t launch_geometry_cells<3>(hipStream_t, const MeshIn&, double*, double*, int*, int*, unsigned int*);
 template hipError_t launch_geometry_faces<2>(hipStream_t, const MeshIn&, double*, unsigned int*);
 template hipError_t launch_geometry_faces<3>(hipStream_t, const MeshIn&, double*, unsigned int*);
 template hipError_t launch_geometry_slots<2>(hipStream_t, const MeshIn&, const double*, const int*, const int*, const long long*, int*, int*, double*,

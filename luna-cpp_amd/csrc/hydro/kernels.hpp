@@ -43,6 +43,7 @@ struct Geo {
     const int* list;    // slots: f if the cell owns face f, ~f otherwise
     const int* other;   // slots: the cell on the other side of the entry's face
     double* cell;       // N x CELL_ROW (the kernels write only the primitive half)
+    const double* volume;  // N: the cell rows' volumes again, densely, for the kernels that read no other geometry
     const double* face; // F x FACE_ROW
     // per slot, in the slices' layout, D + 1 planes of `slots` doubles: the face's |A|, then the face centroid relative
     // to this cell's centroid (Geometry::face_offset, finite_volume.hpp:322-326), one plane per axis
@@ -142,7 +143,7 @@ hipError_t launch_drift(hipStream_t s, const Geo& g, const Rows& r, double dt);
 // list lengths, each slice's longest list into slice), scan (slice's first slots, the total into info[0]); the host then
 // checks the flags and sizes the slot arrays; faces (records) and slots (lists, other cells, |A| and face offsets).
 template <int D>
-hipError_t launch_geometry_cells(hipStream_t s, const MeshIn& m, double* cell, int* count, int* slice, unsigned int* flags);
+hipError_t launch_geometry_cells(hipStream_t s, const MeshIn& m, double* cell, double* volume, int* count, int* slice, unsigned int* flags);
 hipError_t launch_geometry_scan(hipStream_t s, int* slice, int S, long long* info, unsigned int* flags);
 template <int D>
 hipError_t launch_geometry_faces(hipStream_t s, const MeshIn& m, double* face, unsigned int* flags);

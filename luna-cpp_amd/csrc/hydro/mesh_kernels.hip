@@ -120,7 +120,7 @@ __global__ __launch_bounds__(BLOCK) void k_primitives(Geo g, Rows r, Gas gas) {
     const int i = xcd_block() * BLOCK + threadIdx.x;
     if (i >= g.N) return;
     double* row = g.cell + (size_t)i * CELL_ROW;
-    const double inv = 1.0 / row[C::VOL];
+    const double inv = 1.0 / g.volume[i];
     double U[V], W[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) U[v] = r.q[(size_t)i * V + v] * inv;
@@ -1081,7 +1081,7 @@ __global__ __launch_bounds__(CELL_BLOCK) void k_update(Geo g, Rows r, Gas gas, d
     for (int v = 0; v < V; ++v) r.q[(size_t)i * V + v] = next[v] = r.q[(size_t)i * V + v] - dt * sum[v];
     if constexpr (NextPrims) {
         double* row = g.cell + (size_t)i * CELL_ROW;
-        const double inv = 1.0 / row[C::VOL];
+        const double inv = 1.0 / g.volume[i];  // the dense copy: this kernel then only writes the row
         double U[V], W[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) U[v] = next[v] * inv;
