@@ -167,6 +167,7 @@ struct luna_fv {
     Buf<unsigned char> troubled, net;
     bool has_net = false, features_ready = false;
     unsigned long long* host = nullptr;  // pinned: read-backs
+    hipEvent_t sized = nullptr;          // the geometry pass's slot count and checks are on the host
     double side[3] = {0, 0, 0};
     luna::Grid grid{};
     int parity = 0;
@@ -263,7 +264,13 @@ int derive(luna_fv* fv, const luna_geometry* m, luna::MeshIn in) {
     unsigned long long* back = fv->host + 4;
     LUNA_HIP(hipMemcpyAsync(back, fv->geo_info.p, sizeof(long long), hipMemcpyDeviceToHost, s));
     LUNA_HIP(hipMemcpyAsync(back + 1, fv->geo_flags.p, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
-    LUNA_HIP(hipStreamSynchronize(s));
+    LUNA_HIP(hipEventRecord(fv->sized, s));
+    // The face records need neither the lists nor their size: the GPU derives them while the host reads the slot count.
+    LUNA_HIP(timed(fv->prof, s, "geometry_faces", [&] {
+        return D == 3 ? luna::launch_geometry_faces<3>(s, in, fv->facerec.p, fv->geo_flags.p)
+                      : luna::launch_geometry_faces<2>(s, in, fv->facerec.p, fv->geo_flags.p);
+    }));
+    LUNA_HIP(hipEventSynchronize(fv->sized));
     unsigned int flags = static_cast<unsigned int>(back[1]);
     if (flags & luna::GEO_BAD_START) return fail(LUNA_ERR_ARGUMENT, "luna: CSR start must run from 0 to 2 x faces without decreasing");
     if (flags & luna::GEO_TOO_MANY_SLOTS) return fail(LUNA_ERR_ARGUMENT, "luna: mesh lists exceed 2^31 slots");
@@ -271,10 +278,6 @@ int derive(luna_fv* fv, const luna_geometry* m, luna::MeshIn in) {
     LUNA_HIP(fv->list.reserve(slots > 0 ? slots : 1));
     LUNA_HIP(fv->other.reserve(slots > 0 ? slots : 1));
     LUNA_HIP(fv->slotgeo.reserve(static_cast<size_t>(D + 1) * (slots > 0 ? slots : 1)));
-    LUNA_HIP(timed(fv->prof, s, "geometry_faces", [&] {
-        return D == 3 ? luna::launch_geometry_faces<3>(s, in, fv->facerec.p, fv->geo_flags.p)
-                      : luna::launch_geometry_faces<2>(s, in, fv->facerec.p, fv->geo_flags.p);
-    }));
     LUNA_HIP(timed(fv->prof, s, "geometry_slots", [&] {
         return D == 3 ? luna::launch_geometry_slots<3>(s, in, fv->facerec.p, fv->count.p, fv->slice.p, fv->geo_info.p, fv->list.p, fv->other.p, fv->slotgeo.p,
                                                        fv->geo_flags.p)
@@ -514,6 +517,8 @@ int create(const luna_fv_options* opt, const luna_geometry* mesh, luna_fv** out,
         return rc;
     };
     if (hipStreamCreate(&fv->stream) != hipSuccess) return cleanup(fail(LUNA_ERR_DEVICE, "luna: hipStreamCreate failed"));
+    if (hipEventCreateWithFlags(&fv->sized, hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(LUNA_ERR_DEVICE, "luna: hipEventCreate failed"));
     const size_t N = fv->N, D = fv->D, V = fv->V;
     hipError_t e = hipSuccess;
     if ((e = fv->q.reserve(N * V)) != hipSuccess || (e = fv->count.reserve(N)) != hipSuccess ||
@@ -590,6 +595,7 @@ int luna_fv_destroy(luna_fv* fv) {
     fv->troubled.release();
     fv->net.release();
     if (fv->host) (void)hipHostFree(fv->host);
+    if (fv->sized) (void)hipEventDestroy(fv->sized);
     if (fv->stream) (void)hipStreamDestroy(fv->stream);
     delete fv;
     return LUNA_OK;

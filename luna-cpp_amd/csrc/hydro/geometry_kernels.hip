@@ -59,33 +59,54 @@ __global__ __launch_bounds__(SLICE) void k_geo_cells(MeshIn m, double* cell, dou
     if (threadIdx.x == 0) longest[blockIdx.x] = n;
 }
 
-// In place: slice[g] = 64 sum_{h < g} longest[h]; info[0] = the total slots. One work-group of SCAN threads, each taking a
-// contiguous run of slices.
-constexpr int SCAN = 1024;
+// In place: slice[g] = 64 sum_{h < g} longest[h]; info[0] = the total slots. One work-group of SCAN threads, in tiles of
+// SCAN x PER slices: each thread loads PER consecutive slices (the tile's loads coalesced); the threads' sums are scanned
+// within each wavefront by shuffles and across the 16 wavefronts through LDS; a carry runs from tile to tile.
+constexpr int SCAN = 1024, PER = 16, TILE = SCAN * PER, SCAN_WAVES = SCAN / 64;
+__device__ __forceinline__ long long wave_inclusive_sum(long long x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
 __global__ __launch_bounds__(SCAN) void k_geo_scan(int* slice, int S, long long* info, unsigned int* flags) {
-    __shared__ long long part[SCAN];
-    const int t = threadIdx.x;
-    const int per = (S + SCAN - 1) / SCAN;
-    const int lo = min(S, t * per), hi = min(S, lo + per);
-    long long sum = 0;
-    for (int g = lo; g < hi; ++g) sum += (long long)slice[g] * SLICE;
-    part[t] = sum;
-    __syncthreads();
-    for (int o = 1; o < SCAN; o <<= 1) {  // inclusive scan of the runs' sums (Hillis-Steele)
-        const long long add = t >= o ? part[t - o] : 0;
+    __shared__ long long wave_total[SCAN_WAVES];
+    const int t = threadIdx.x, w = t >> 6;
+    long long carry = 0;
+    for (int base = 0; base < S; base += TILE) {
+        const int lo = base + t * PER;
+        long long v[PER], sum = 0;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            v[k] = lo + k < S ? (long long)slice[lo + k] * SLICE : 0;
+            sum += v[k];
+        }
+        const long long in_wave = wave_inclusive_sum(sum);
+        if ((t & 63) == 63) wave_total[w] = in_wave;
         __syncthreads();
-        part[t] += add;
-        __syncthreads();
+        long long before = 0, tile = 0;  // the earlier wavefronts' totals, and the tile's
+#pragma unroll
+        for (int k = 0; k < SCAN_WAVES; ++k) {
+            const long long x = wave_total[k];
+            before += k < w ? x : 0;
+            tile += x;
+        }
+        long long run = carry + before + in_wave - sum;
+#pragma unroll
+        for (int k = 0; k < PER; ++k)
+            if (lo + k < S) {
+                slice[lo + k] = run > 0x7fffffffLL ? 0 : (int)run;
+                run += v[k];
+            }
+        carry += tile;
+        __syncthreads();  // wave_total is rewritten by the next tile
     }
-    long long run = part[t] - sum;
-    for (int g = lo; g < hi; ++g) {
-        const long long here = (long long)slice[g] * SLICE;
-        slice[g] = run > 0x7fffffffLL ? 0 : (int)run;
-        run += here;
-    }
-    if (t == SCAN - 1) {
-        info[0] = part[t];
-        if (part[t] > 0x7fffffffLL) flag(flags, GEO_TOO_MANY_SLOTS);
+    if (t == 0) {
+        info[0] = carry;
+        if (carry > 0x7fffffffLL) flag(flags, GEO_TOO_MANY_SLOTS);
     }
 }
 
