@@ -35,7 +35,8 @@ HBM_PEAK_GBS = 8000.0
 #   update:   CSR 4, list 4e, flux rows 40f, Q read and written 80 -> 84 + 4e + 40f; moving pass 1 with the drift (x r/w
 #             48, grid steps 12)
 # and of the geometry pass of a moving step (the new mesh's derived layouts from the producer's arrays):
-#   cells:  the producer's V, c, x and list start in (60), the 64 B geometry half of the cell row and the count out -> 128
+#   cells:  the producer's V, c, x and list start in (60), the 64 B geometry half of the cell row, the count and a dense
+#           copy of V out -> 136
 #   faces:  owner, neighbour, A, face centroid in (56 a face), the 64 B face record out -> 120f
 #   slots:  the producer's list (4e), the used fields of each face record once (40f), the cells' x and c once (48), 40 B
 #           per list entry out (the face id with its owner bit, the other cell, |A|, the face offset) -> 48 + 44e + 40f
@@ -44,7 +45,7 @@ def algo_bytes(f, e):
         "static": {"sweep": 260 + 4 * e + 56 * f, "flux": 208 + 96 * f, "update": 84 + 4 * e + 40 * f},
         "moving": {"sweep": 308 + 4 * e + 56 * f, "keep_centroids": 84.0, "flux_pass1": 220 + 96 * f,
                    "update_pass1": 144 + 4 * e + 40 * f, "flux_pass2": 244 + 96 * f, "update_pass2": 84 + 4 * e + 40 * f},
-        "geometry": {"geometry_cells": 128.0, "geometry_faces": 120 * f, "geometry_slots": 48 + 44 * e + 40 * f},
+        "geometry": {"geometry_cells": 136.0, "geometry_faces": 120 * f, "geometry_slots": 48 + 44 * e + 40 * f},
     }
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_sedov128.md")
