@@ -37,15 +37,16 @@ HBM_PEAK_GBS = 8000.0
 # and of the geometry pass of a moving step (the new mesh's derived layouts from the producer's arrays):
 #   cells:  the producer's V, c, x and list start in (60), the 64 B geometry half of the cell row, the count and a dense
 #           copy of V out -> 136
-#   faces:  owner, neighbour, A, face centroid in (56 a face), the 64 B face record out -> 120f
-#   slots:  the producer's list (4e), the used fields of each face record once (40f), the cells' x and c once (48), 40 B
-#           per list entry out (the face id with its owner bit, the other cell, |A|, the face offset) -> 48 + 44e + 40f
+#   pairs:  owner and neighbour checked (8 a face) -> 8f (the flux kernels read the face arrays in place)
+#   slots:  the producer's list (4e), each face's owner, neighbour, A and centroid once (56f), the cells' x and c once
+#           (48), 40 B per list entry out (the face id with its owner bit, the other cell, |A|, the face offset)
+#           -> 48 + 44e + 56f
 def algo_bytes(f, e):
     return {
         "static": {"sweep": 260 + 4 * e + 56 * f, "flux": 208 + 96 * f, "update": 84 + 4 * e + 40 * f},
         "moving": {"sweep": 308 + 4 * e + 56 * f, "keep_centroids": 84.0, "flux_pass1": 220 + 96 * f,
                    "update_pass1": 144 + 4 * e + 40 * f, "flux_pass2": 244 + 96 * f, "update_pass2": 84 + 4 * e + 40 * f},
-        "geometry": {"geometry_cells": 136.0, "geometry_faces": 120 * f, "geometry_slots": 48 + 44 * e + 40 * f},
+        "geometry": {"geometry_cells": 136.0, "geometry_pairs": 8 * f, "geometry_slots": 48 + 44 * e + 56 * f},
     }
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_sedov128.md")
@@ -470,8 +471,8 @@ def main():
                                            "achieved_GBs": b * N / (m_times[k][1] / m_times[k][0] * 1e-3) / 1e9,
                                            "frac": b * N / (m_times[k][1] / m_times[k][0] * 1e-3) / 1e9 / HBM_PEAK_GBS}
                                        for k, b in ALGO["geometry"].items() if k in m_times},
-                 "note": "value includes the geometry pass on the device (luna_fv_set_geometry_device: cell rows, face "
-                         "records, slot lists with |A| and face offsets, radii); excluded: the Voronoi producer (C++/OpenMP, "
+                 "note": "value includes the geometry pass on the device (luna_fv_set_geometry_device: cell rows, "
+                         "slot lists with |A| and face offsets, radii; the face arrays read in place); excluded: the Voronoi producer (C++/OpenMP, "
                          "host) and the copies standing in for a GPU producer (positions to the host, its arrays to the device)"},
         "derivatives": derivs,
         "sod_1d": sod,

@@ -102,7 +102,8 @@ public:
         check(luna_fv_time_step(fv_, courant, &dt), "time_step");
         return dt;
     }
-    // set_step_rule (:181), release_mesh (:134-139: the handle holds its own copy; refused on a moving mesh), and of
+    // set_step_rule (:181), release_mesh (:134-139: the handle copies a device mesh's face arrays, which it reads in place,
+    // and lets the mesh go; refused on a moving mesh), and of
     // mesh_stats (:160-163) the mesh updates taken.
     void set_step_rule(StepRule rule) { check(luna_fv_set_step_rule(fv_, static_cast<int>(rule)), "set_step_rule"); }
     void release_mesh() { check(luna_fv_release_mesh(fv_), "release_mesh"); }
@@ -157,7 +158,8 @@ public:
     // mesh_moved() (:152-156): the caller changed the mesh itself.
     void mesh_moved(const luna_geometry& mesh) { check(luna_fv_set_geometry(fv_, &mesh), "mesh_moved"); }
     // A moving step with a mesh producer that leaves its arrays on the GPU (voronoi-cpp's device views, read in place as
-    // :257-327 reads them): mesh_of(positions, host copy) -> luna_geometry of device pointers.
+    // :257-327 reads them): mesh_of(positions, host copy) -> luna_geometry of device pointers. The new mesh's face arrays
+    // are read by the steps that follow, so the producer keeps them until it hands over the next mesh.
     template <class DeviceMeshOf>
     void step_device(double dt, DeviceMeshOf&& mesh_of) {
         std::vector<double> x(static_cast<size_t>(n_) * D);

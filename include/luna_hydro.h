@@ -124,8 +124,8 @@ int luna_fv_moving_finish(luna_fv* fv, const luna_geometry* mesh, double dt);
 /* Takes a mesh. Between moving_begin and moving_finish: the mesh of the new positions. Otherwise: a caller moved the
  * mesh itself (mesh_moved(), finite_volume.hpp:152-156), and the cached rows are dropped; a moving solver's positions
  * become the mesh's (a restart from a checkpoint of Q and x). The handle derives what its kernels read (the cell rows,
- * face records, lists in 64-cell slices with |A| and the face offsets x_f - c_i, the cell radii) in one pass on the GPU
- * and keeps its own copy, so the caller's arrays may change once the call returns. A malformed mesh (CSR offsets not
+ * lists in 64-cell slices with |A| and the face offsets x_f - c_i, the cell radii) in one pass on the GPU; a host mesh's
+ * arrays are copied into the handle first, so the caller's may change once the call returns. A malformed mesh (CSR offsets not
  * running from 0 to 2 faces, a cell or face index out of range) gives LUNA_ERR_ARGUMENT, and the handle then refuses to
  * step (LUNA_ERR_LOGIC) until a valid mesh is set. */
 int luna_fv_set_geometry(luna_fv* fv, const luna_geometry* mesh);
@@ -133,7 +133,10 @@ int luna_fv_set_geometry(luna_fv* fv, const luna_geometry* mesh);
 /* set_geometry and the constructor with the mesh arrays in device memory (pointers from hipMalloc on the handle's
  * device, or torch tensors' data): voronoi-cpp's device views, which the reference reads in place
  * (finite_volume.hpp:257-327, 353-364). The side, step and period fields are values as for a host mesh. The arrays must
- * be ready when the call is made (the handle's stream does not wait on another stream) and are read only during it. */
+ * be ready when the call is made (the handle's stream does not wait on another stream). The cell arrays and the lists
+ * are read only during the call; the face arrays (owner, neighbour, area, face_centroid) are read in place by the steps
+ * that follow, as the reference reads the mesh it holds, so they must stay valid and unchanged until the next mesh is
+ * set, luna_fv_release_mesh (static) copies them, or the handle is destroyed. */
 int luna_fv_set_geometry_device(luna_fv* fv, const luna_geometry* mesh);
 int luna_fv_create_device(const luna_fv_options* options, const luna_geometry* mesh, luna_fv** out);
 
@@ -160,8 +163,8 @@ int luna_fv_faces(luna_fv* fv, int* faces);
  * (LUNA_ERR_ARGUMENT otherwise). */
 int luna_fv_set_step_rule(luna_fv* fv, int rule);
 /* release_mesh() (finite_volume.hpp:134-139): the reference stops reading the caller's mesh on a static mesh and refuses on a
- * moving one (LUNA_ERR_LOGIC). The handle keeps its own copy of every mesh it takes, so on a static mesh there is
- * nothing to release and the call only checks the motion. */
+ * moving one (LUNA_ERR_LOGIC). On a static mesh the handle copies the face arrays of a device mesh, which it reads in
+ * place, into its own memory (a host mesh's are its copies already); the caller may then free the mesh. */
 int luna_fv_release_mesh(luna_fv* fv);
 /* mesh_stats() (finite_volume.hpp:160-163): of the reference's counters, `updates`, the mesh updates of moving steps
  * (:1237; here the meshes a moving step took between begin and finish); the rest are the mesh library's own

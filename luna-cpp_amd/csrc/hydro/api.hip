@@ -183,6 +183,10 @@ struct luna_fv {
     Buf<long long> geo_info;
     Buf<unsigned int> geo_flags;
     bool mesh_ok = false;  // the last mesh given was accepted
+    // The accepted mesh's arrays: the step reads its face arrays in place (a device mesh's, or the copies of a host mesh in
+    // the src_ buffers); `records` once the derivatives' face records are built for it.
+    luna::MeshIn mesh{};
+    bool records = false;
 
     luna::Geo geo() const {
         luna::Geo g{};
@@ -195,6 +199,10 @@ struct luna_fv {
         g.cell = cell.p;
         g.volume = volume.p;
         g.face = facerec.p;
+        g.owner = mesh.owner;
+        g.neighbour = mesh.neighbour;
+        g.area = mesh.area;
+        g.fcen = mesh.face_centroid;
         g.slot = slotgeo.p;
         g.slots = slots;
         for (int a = 0; a < 3; ++a) {
@@ -234,10 +242,13 @@ int check_geometry(const luna_geometry* m, int D, int N) {
 
 // The mesh a handle steps on, from the mesh producer's arrays (device pointers in `in`) to the rows the kernels read: the
 // geometry pass of geometry_kernels.hip on the handle's stream. One read-back sizes the list slots after the scan, one
-// at the end collects the pass's checks; the caller's arrays are no longer read when this returns. A malformed mesh
-// (CSR offsets, cell or face indices out of range, more than 2^31 slots) is refused, and the handle takes no step until
-// a valid mesh is set.
-int derive(luna_fv* fv, const luna_geometry* m, luna::MeshIn in) {
+// at the end collects the pass's checks. The face arrays (owner, neighbour, area, face centroid) are read in place by
+// the steps that follow, until the next mesh; the cell arrays and lists are not read after this returns. A malformed
+// mesh (CSR offsets, cell or face indices out of range, more than 2^31 slots) is refused, and the handle takes no step
+// until a valid mesh is set.
+int records(luna_fv* fv);
+
+int derive(luna_fv* fv, const luna_geometry* m, luna::MeshIn in, bool eager_records) {
     const int D = fv->D, N = fv->N;
     fv->mesh_ok = false;
     if (fv->opt.motion == LUNA_MOTION_MOVING) {
@@ -265,11 +276,8 @@ int derive(luna_fv* fv, const luna_geometry* m, luna::MeshIn in) {
     LUNA_HIP(hipMemcpyAsync(back, fv->geo_info.p, sizeof(long long), hipMemcpyDeviceToHost, s));
     LUNA_HIP(hipMemcpyAsync(back + 1, fv->geo_flags.p, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
     LUNA_HIP(hipEventRecord(fv->sized, s));
-    // The face records need neither the lists nor their size: the GPU derives them while the host reads the slot count.
-    LUNA_HIP(timed(fv->prof, s, "geometry_faces", [&] {
-        return D == 3 ? luna::launch_geometry_faces<3>(s, in, fv->facerec.p, fv->geo_flags.p)
-                      : luna::launch_geometry_faces<2>(s, in, fv->facerec.p, fv->geo_flags.p);
-    }));
+    // The faces' cell indices need neither the lists nor their size: the GPU checks them while the host reads the count.
+    LUNA_HIP(timed(fv->prof, s, "geometry_pairs", [&] { return luna::launch_geometry_pairs(s, in, fv->geo_flags.p); }));
     LUNA_HIP(hipEventSynchronize(fv->sized));
     unsigned int flags = static_cast<unsigned int>(back[1]);
     if (flags & luna::GEO_BAD_START) return fail(LUNA_ERR_ARGUMENT, "luna: CSR start must run from 0 to 2 x faces without decreasing");
@@ -279,9 +287,9 @@ int derive(luna_fv* fv, const luna_geometry* m, luna::MeshIn in) {
     LUNA_HIP(fv->other.reserve(slots > 0 ? slots : 1));
     LUNA_HIP(fv->slotgeo.reserve(static_cast<size_t>(D + 1) * (slots > 0 ? slots : 1)));
     LUNA_HIP(timed(fv->prof, s, "geometry_slots", [&] {
-        return D == 3 ? luna::launch_geometry_slots<3>(s, in, fv->facerec.p, fv->count.p, fv->slice.p, fv->geo_info.p, fv->list.p, fv->other.p, fv->slotgeo.p,
+        return D == 3 ? luna::launch_geometry_slots<3>(s, in, fv->count.p, fv->slice.p, fv->geo_info.p, fv->list.p, fv->other.p, fv->slotgeo.p,
                                                        fv->geo_flags.p)
-                      : luna::launch_geometry_slots<2>(s, in, fv->facerec.p, fv->count.p, fv->slice.p, fv->geo_info.p, fv->list.p, fv->other.p, fv->slotgeo.p,
+                      : luna::launch_geometry_slots<2>(s, in, fv->count.p, fv->slice.p, fv->geo_info.p, fv->list.p, fv->other.p, fv->slotgeo.p,
                                                        fv->geo_flags.p);
     }));
     LUNA_HIP(hipMemcpyAsync(back + 1, fv->geo_flags.p, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
@@ -292,7 +300,13 @@ int derive(luna_fv* fv, const luna_geometry* m, luna::MeshIn in) {
     fv->F = F;
     fv->E = 2 * F;
     fv->slots = static_cast<int>(slots);
+    fv->mesh = in;  // the face arrays only are read from here on
+    fv->mesh.start = fv->mesh.list = nullptr;
+    fv->mesh.centroid = fv->mesh.volume = fv->mesh.positions = nullptr;
+    fv->records = false;
     fv->mesh_ok = true;
+    if (eager_records)
+        if (int rc = records(fv)) return rc;
     for (int a = 0; a < 3; ++a) fv->side[a] = a < D ? m->side[a] : 0.0;
     fv->grid.step = m->step;
     fv->grid.per_step = m->step > 0 ? 1.0 / m->step : 0.0;
@@ -319,7 +333,7 @@ luna::MeshIn mesh_in(const luna_geometry* m) {
 // A mesh in device memory (the arrays the producer left on the GPU), read in place.
 int upload_device(luna_fv* fv, const luna_geometry* m) {
     if (int rc = check_geometry(m, fv->D, fv->N)) return rc;
-    return derive(fv, m, mesh_in(m));
+    return derive(fv, m, mesh_in(m), false);
 }
 
 // A mesh in host memory: its arrays are copied to the handle's device buffers, then derived as a device mesh. Host
@@ -359,7 +373,20 @@ int upload(luna_fv* fv, const luna_geometry* m) {
     in.centroid = centroid;
     in.volume = volume;
     in.positions = positions;
-    return derive(fv, m, in);
+    return derive(fv, m, in, true);
+}
+
+// The derivatives read each face as one 64 B record (k_geo_faces, from the face arrays the step reads in place; the pass
+// has checked their cell indices). Built on their first use of a mesh, stream-ordered; a host mesh's at once.
+int records(luna_fv* fv) {
+    if (fv->records || !fv->mesh_ok) return LUNA_OK;
+    hipStream_t s = fv->stream;
+    LUNA_HIP(timed(fv->prof, s, "geometry_faces", [&] {
+        return fv->D == 3 ? luna::launch_geometry_faces<3>(s, fv->mesh, fv->facerec.p, fv->geo_flags.p)
+                          : luna::launch_geometry_faces<2>(s, fv->mesh, fv->facerec.p, fv->geo_flags.p);
+    }));
+    fv->records = true;
+    return LUNA_OK;
 }
 
 int primitives(luna_fv* fv) {
@@ -667,6 +694,7 @@ int derivatives_ready(luna_fv* fv, const char* what) {
     if (fv->opt.sweep_lanes > 1) return fail(LUNA_ERR_LOGIC, std::string(what) + ": derivatives of the one-lane sweep");
     if (fv->begun) return fail(LUNA_ERR_LOGIC, std::string(what) + ": a moving step is between begin and finish");
     if (!fv->mesh_ok) return fail(LUNA_ERR_LOGIC, "luna: the last mesh given was refused; set a valid mesh first");
+    if (int rc = records(fv)) return rc;
     const size_t N = fv->N, V = fv->V, F = fv->F;
     LUNA_HIP(fv->d_dq.reserve(N * V));
     LUNA_HIP(fv->d_dW.reserve(N * V));
@@ -838,6 +866,7 @@ int luna_fv_step_adjoint_noalloc(luna_fv* fv, double dt, double* qbar) {
                                            std::to_string(SCRATCH_BYTES) + " of the library's scratch (use luna_fv_step_adjoint)");
     w.dq = qbar;  // read only: the cotangent after the step
     hipStream_t s = fv->stream;
+    if (int rc = records(fv)) return rc;  // launches only: the records' memory is the handle's, reserved with the mesh
     if (int rc = primitives(fv)) return rc;  // the forward's rows at Q: primitives and the limited gradients
     if (int rc = sweep(fv, 1.0)) return rc;
     const luna::Geo g = fv->geo();
@@ -1288,6 +1317,25 @@ int luna_fv_set_step_rule(luna_fv* fv, int rule) {
 int luna_fv_release_mesh(luna_fv* fv) {
     if (!fv) return fail(LUNA_ERR_ARGUMENT, "luna_fv_release_mesh: null argument");
     if (fv->opt.motion == LUNA_MOTION_MOVING) return fail(LUNA_ERR_LOGIC, "hydro::FiniteVolume::release_mesh: a moving mesh updates the mesh");
+    // A device mesh's face arrays are read in place: the handle copies them into its own memory, so the caller may free
+    // the mesh (a host mesh's are the handle's copies already).
+    if (fv->mesh_ok && fv->mesh.owner != fv->src_owner.p) {
+        const size_t F = fv->F, FD = F * fv->D;
+        hipStream_t s = fv->stream;
+        LUNA_HIP(fv->src_owner.reserve(F > 0 ? F : 1));
+        LUNA_HIP(fv->src_neighbour.reserve(F > 0 ? F : 1));
+        LUNA_HIP(fv->src_area.reserve(FD > 0 ? FD : 1));
+        LUNA_HIP(fv->src_fc.reserve(FD > 0 ? FD : 1));
+        LUNA_HIP(hipMemcpyAsync(fv->src_owner.p, fv->mesh.owner, sizeof(int) * F, hipMemcpyDeviceToDevice, s));
+        LUNA_HIP(hipMemcpyAsync(fv->src_neighbour.p, fv->mesh.neighbour, sizeof(int) * F, hipMemcpyDeviceToDevice, s));
+        LUNA_HIP(hipMemcpyAsync(fv->src_area.p, fv->mesh.area, sizeof(double) * FD, hipMemcpyDeviceToDevice, s));
+        LUNA_HIP(hipMemcpyAsync(fv->src_fc.p, fv->mesh.face_centroid, sizeof(double) * FD, hipMemcpyDeviceToDevice, s));
+        LUNA_HIP(hipStreamSynchronize(s));
+        fv->mesh.owner = fv->src_owner.p;
+        fv->mesh.neighbour = fv->src_neighbour.p;
+        fv->mesh.area = fv->src_area.p;
+        fv->mesh.face_centroid = fv->src_fc.p;
+    }
     return LUNA_OK;
 }
 

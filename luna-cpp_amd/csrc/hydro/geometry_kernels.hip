@@ -7,10 +7,11 @@
 //   k_geo_cells   per cell: the list length and the geometry half of the cell row [x, c, V, r]; per 64-cell slice (one
 //                 work-group) the longest list; checks of the CSR offsets;
 //   k_geo_scan    the slices' first slots (exclusive sum of 64 x longest), one work-group;
-//   k_geo_faces   per face: the record [A, fc, |A|, (owner, neighbour)]; checks of the cell indices;
-//   k_geo_slots   per cell and list entry, in the slices' layout: the entry, the other cell, |A| and the face offset (from
-//                 the face records k_geo_faces wrote)
-//                 x_f - c_i; padding slots zero; checks of the face indices.
+//   k_geo_pairs   per face: checks of the cell indices (the step reads the face arrays in place);
+//   k_geo_slots   per cell and list entry, in the slices' layout: the entry, the other cell, |A| and the face offset
+//                 x_f - c_i; padding slots zero; checks of the face indices;
+//   k_geo_faces   per face: the record [A, fc, |A|, (owner, neighbour)] the derivatives read (on their first use of a
+//                 mesh, not in the pass).
 // Every read is guarded by the checks before it, so a malformed mesh sets a flag (read by the host) and reads nothing out
 // of range. The arithmetic is that of the host restatement (the same products, sums and rounding), so the rows are equal
 // to the bit whichever side derives them.
@@ -135,6 +136,14 @@ __global__ __launch_bounds__(BLOCK) void k_geo_faces(MeshIn m, double* face, uns
     for (int k = 0; k < FACE_ROW; ++k) out[k] = row[k];
 }
 
+// The faces' cell indices in range (the flux kernels read owner and neighbour of every face in place).
+__global__ __launch_bounds__(BLOCK) void k_geo_pairs(MeshIn m, unsigned int* flags) {
+    const int f = blockIdx.x * BLOCK + threadIdx.x;
+    if (f >= m.F) return;
+    const int o = m.owner[f], j = m.neighbour[f];
+    if (o < 0 || o >= m.N || j < 0 || j >= m.N) flag(flags, GEO_BAD_CELL);
+}
+
 // One work-group per quarter slice: 16 cells, whose list entries are one contiguous run of the CSR list. The work-items
 // take consecutive entries (so the list and the faces a cell owns, numbered consecutively, are read in runs); each finds
 // its cell among the 16 offsets and computes its slot's values into LDS, laid out as the slots are; then the work-group
@@ -143,11 +152,9 @@ __global__ __launch_bounds__(BLOCK) void k_geo_faces(MeshIn m, double* face, uns
 // The CSR offsets were checked by k_geo_cells (the host runs this only when they passed).
 constexpr int SLOT_BLOCK = 256, QUARTER = 16, QUARTER_DEPTH = 48;
 template <int D>
-__global__ __launch_bounds__(SLOT_BLOCK) void k_geo_slots(MeshIn m, const double* __restrict__ face, const int* __restrict__ count,
-                                                          const int* __restrict__ slice, const long long* __restrict__ info, int S,
-                                                          int* __restrict__ list, int* __restrict__ other, double* __restrict__ slot,
-                                                          unsigned int* flags) {
-    using Fa = FaceAt<D>;
+__global__ __launch_bounds__(SLOT_BLOCK) void k_geo_slots(MeshIn m, const int* __restrict__ count, const int* __restrict__ slice,
+                                                          const long long* __restrict__ info, int S, int* __restrict__ list,
+                                                          int* __restrict__ other, double* __restrict__ slot, unsigned int* flags) {
     __shared__ int offs[QUARTER + 1];
     __shared__ int s_list[QUARTER_DEPTH * QUARTER], s_other[QUARTER_DEPTH * QUARTER];
     __shared__ double s_geo[D + 1][QUARTER_DEPTH * QUARTER];
@@ -189,12 +196,7 @@ __global__ __launch_bounds__(SLOT_BLOCK) void k_geo_slots(MeshIn m, const double
             bad |= GEO_BAD_FACE;
             e = 0;
         } else {
-            // the face's record, which k_geo_faces wrote from the same arrays (one 64 B record instead of the producer's
-            // four arrays; |A| as it computed it, the same operations)
-            const double* rec = face + (size_t)f * FACE_ROW;
-            int pair[2];
-            __builtin_memcpy(pair, &rec[Fa::PAIR], sizeof pair);
-            const int o = pair[0], j = pair[1];
+            const int o = m.owner[f], j = m.neighbour[f];
             if (o < 0 || o >= m.N || j < 0 || j >= m.N) {
                 bad |= GEO_BAD_CELL;
             } else if (o != i && j != i) {
@@ -204,10 +206,13 @@ __global__ __launch_bounds__(SLOT_BLOCK) void k_geo_slots(MeshIn m, const double
                 // lack it; a list that carries it keeps its bits).
                 e = o == i ? f : ~f;
                 cell = e >= 0 ? j : o;
-                double rel[D];
+                double A[D], rel[D];
 #pragma unroll
-                for (int a = 0; a < D; ++a) rel[a] = rec[Fa::FC + a];
-                mag = rec[Fa::MAG];
+                for (int a = 0; a < D; ++a) {
+                    A[a] = m.area[(size_t)f * D + a];
+                    rel[a] = m.face_centroid[(size_t)f * D + a];
+                }
+                mag = sqrt(dot<D>(A, A));  // as k_geo_faces and the flux kernels take it
                 // Geometry::face_offset (finite_volume.hpp:322-326): the centroid moved to this cell's generator by the
                 // nearest-image separation when the cell does not own the face, then relative to its centroid
                 if (e < 0) {
@@ -281,13 +286,18 @@ hipError_t launch_geometry_faces(hipStream_t s, const MeshIn& m, double* face, u
     return hipGetLastError();
 }
 
+hipError_t launch_geometry_pairs(hipStream_t s, const MeshIn& m, unsigned int* flags) {
+    if (m.F == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_geo_pairs, dim3((m.F + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, m, flags);
+    return hipGetLastError();
+}
+
 template <int D>
-hipError_t launch_geometry_slots(hipStream_t s, const MeshIn& m, const double* face, const int* count, const int* slice, const long long* info,
-                                 int* list, int* other, double* slot, unsigned int* flags) {
+hipError_t launch_geometry_slots(hipStream_t s, const MeshIn& m, const int* count, const int* slice, const long long* info, int* list,
+                                 int* other, double* slot, unsigned int* flags) {
     const int S = (m.N + SLICE - 1) / SLICE;
     if (S == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_geo_slots<D>, dim3(S * (SLICE / QUARTER)), dim3(SLOT_BLOCK), 0, s, m, face, count, slice, info, S, list, other, slot,
-                       flags);
+    hipLaunchKernelGGL(k_geo_slots<D>, dim3(S * (SLICE / QUARTER)), dim3(SLOT_BLOCK), 0, s, m, count, slice, info, S, list, other, slot, flags);
     return hipGetLastError();
 }
 
@@ -295,9 +305,7 @@ template hipError_t launch_geometry_cells<2>(hipStream_t, const MeshIn&, double*
 template hipError_t launch_geometry_cells<3>(hipStream_t, const MeshIn&, double*, double*, int*, int*, unsigned int*);
 template hipError_t launch_geometry_faces<2>(hipStream_t, const MeshIn&, double*, unsigned int*);
 template hipError_t launch_geometry_faces<3>(hipStream_t, const MeshIn&, double*, unsigned int*);
-template hipError_t launch_geometry_slots<2>(hipStream_t, const MeshIn&, const double*, const int*, const int*, const long long*, int*, int*, double*,
-                                             unsigned int*);
-template hipError_t launch_geometry_slots<3>(hipStream_t, const MeshIn&, const double*, const int*, const int*, const long long*, int*, int*, double*,
-                                             unsigned int*);
+template hipError_t launch_geometry_slots<2>(hipStream_t, const MeshIn&, const int*, const int*, const long long*, int*, int*, double*, unsigned int*);
+template hipError_t launch_geometry_slots<3>(hipStream_t, const MeshIn&, const int*, const int*, const long long*, int*, int*, double*, unsigned int*);
 
 }  // namespace luna

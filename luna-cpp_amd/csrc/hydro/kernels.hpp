@@ -44,7 +44,13 @@ struct Geo {
     const int* other;   // slots: the cell on the other side of the entry's face
     double* cell;       // N x CELL_ROW (the kernels write only the primitive half)
     const double* volume;  // N: the cell rows' volumes again, densely, for the kernels that read no other geometry
-    const double* face; // F x FACE_ROW
+    const double* face; // F x FACE_ROW records, for the derivatives (built on their first use of a mesh)
+    // The mesh's face arrays, read in place as the reference reads the mesh's views (finite_volume.hpp:257-327): owner
+    // and neighbour (F), area vectors and face centroids relative to the owner's generator (F x D)
+    const int* owner;
+    const int* neighbour;
+    const double* area;
+    const double* fcen;
     // per slot, in the slices' layout, D + 1 planes of `slots` doubles: the face's |A|, then the face centroid relative
     // to this cell's centroid (Geometry::face_offset, finite_volume.hpp:322-326), one plane per axis
     const double* slot;
@@ -140,16 +146,18 @@ template <int D>
 hipError_t launch_drift(hipStream_t s, const Geo& g, const Rows& r, double dt);
 
 // The geometry pass (geometry_kernels.hip), in this order on one stream: cells (the geometry half of the cell rows, the
-// list lengths, each slice's longest list into slice), scan (slice's first slots, the total into info[0]); the host then
-// checks the flags and sizes the slot arrays; faces (records) and slots (lists, other cells, |A| and face offsets).
+// list lengths, each slice's longest list into slice), scan (slice's first slots, the total into info[0]), pairs (the
+// faces' owner and neighbour checked); the host then checks the flags and sizes the slot arrays; slots (lists, other
+// cells, |A| and face offsets). faces (the F x FACE_ROW records) runs on a derivative's first use of a mesh.
 template <int D>
 hipError_t launch_geometry_cells(hipStream_t s, const MeshIn& m, double* cell, double* volume, int* count, int* slice, unsigned int* flags);
 hipError_t launch_geometry_scan(hipStream_t s, int* slice, int S, long long* info, unsigned int* flags);
 template <int D>
 hipError_t launch_geometry_faces(hipStream_t s, const MeshIn& m, double* face, unsigned int* flags);
+hipError_t launch_geometry_pairs(hipStream_t s, const MeshIn& m, unsigned int* flags);
 template <int D>
-hipError_t launch_geometry_slots(hipStream_t s, const MeshIn& m, const double* face, const int* count, const int* slice, const long long* info,
-                                 int* list, int* other, double* slot, unsigned int* flags);
+hipError_t launch_geometry_slots(hipStream_t s, const MeshIn& m, const int* count, const int* slice, const long long* info, int* list,
+                                 int* other, double* slot, unsigned int* flags);
 
 // Derivatives of the static order-1 step (HLLC or HLL) at fixed dt (adjoint_kernels.hip). Device rows:
 //   dq       N x V   in: the tangent of Q (tangent) or Q bar after the step (adjoint); the tangent advances it to J dq

@@ -872,13 +872,15 @@ __device__ __forceinline__ void flux_of_face(const Geo& g, const Rows& r, const 
     constexpr int V = D + 2;
     using FA = FaceAt<D>;
     {
-        const double* face = g.face + (size_t)f * FACE_ROW;
+        // the mesh's face arrays in place: A, the face centroid (relative to the owner's generator), owner, neighbour
         double rec[FACE_ROW - 1];
 #pragma unroll
-        for (int k = 0; k < FACE_ROW - 1; ++k) rec[k] = face[k];
-        const int2 pair = *reinterpret_cast<const int2*>(face + FA::PAIR);
-        const int o = pair.x, j = pair.y;
-        const double magnitude = rec[FA::MAG];
+        for (int k = 0; k < D; ++k) {
+            rec[FA::A + k] = g.area[(size_t)f * D + k];
+            rec[FA::FC + k] = g.fcen[(size_t)f * D + k];
+        }
+        const int o = g.owner[f], j = g.neighbour[f];
+        const double magnitude = sqrt(dot<D>(rec + FA::A, rec + FA::A));  // |A| (as k_geo_faces writes it)
 #pragma unroll
         for (int v = 0; v < V; ++v) F[v] = 0;
         if (magnitude != 0) {

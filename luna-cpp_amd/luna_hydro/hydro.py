@@ -234,8 +234,11 @@ class FiniteVolume:
         _check(lib().luna_fv_set_step_rule(self._h, RULES[rule]), "set_step_rule")
 
     def release_mesh(self):
-        """release_mesh() (finite_volume.hpp:134-139): nothing to release (the handle copies each mesh); refused on a moving mesh."""
+        """release_mesh() (finite_volume.hpp:134-139): the handle copies a device mesh's face arrays, which it read in place,
+        and lets the mesh go; refused on a moving mesh."""
         _check(lib().luna_fv_release_mesh(self._h), "release_mesh")
+        if isinstance(self._mesh_arrays, DeviceMesh):
+            self._mesh_arrays = None
 
     def mesh_updates(self):
         """mesh_stats().updates: the meshes moving steps took (between moving_begin and moving_finish)."""
@@ -360,9 +363,11 @@ class FiniteVolume:
         return x
 
     def set_geometry(self, mesh):
-        """set_geometry with a mesh dict (host arrays) or a DeviceMesh (luna_fv_set_geometry_device)."""
+        """set_geometry with a mesh dict (host arrays) or a DeviceMesh (luna_fv_set_geometry_device; the step reads its face
+        arrays in place until the next mesh, so the handle keeps a reference to it)."""
         if isinstance(mesh, DeviceMesh):
             _check(lib().luna_fv_set_geometry_device(self._h, ctypes.byref(mesh.record)), "set_geometry_device")
+            self._mesh_arrays = mesh
             return
         g, arrays = geometry_record(mesh)
         _check(lib().luna_fv_set_geometry(self._h, ctypes.byref(g)), "set_geometry")

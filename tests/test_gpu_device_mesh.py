@@ -159,3 +159,54 @@ def test_lists_without_the_owner_bit_step_the_same():
     bad["list"] = bl
     with pytest.raises(ValueError):
         hydro.FiniteVolume(bad, GAMMA)
+
+
+def test_release_mesh_copies_a_device_meshs_face_arrays():
+    """The step reads a device mesh's face arrays in place (as finite_volume.hpp:257-327 reads the mesh it holds);
+    release_mesh() copies them into the handle, after which the caller's arrays are not read: overwritten with zeros
+    (in range, so nothing is read out of bounds if they were), the steps still equal the oracle's to the bit."""
+    say("release_mesh on a static device mesh")
+    m = cases.voronoi(10, 3)
+    q = cases.rough_state(m)
+    dm = hydro.DeviceMesh(m)
+    dev = hydro.FiniteVolume(dm, GAMMA)
+    orc = oracle.FiniteVolume(m, GAMMA)
+    try:
+        dev.set_state(q)
+        orc.set_state(q)
+        dev.release_mesh()
+        for t in dm.tensors.values():
+            t.zero_()
+        hydro.synchronize()
+        del dm
+        for s in range(2):
+            dt = orc.time_step(0.3)
+            assert dev.time_step(0.3) == dt
+            dev.step(dt)
+            orc.step(dt)
+            assert np.array_equal(dev.state(), orc.q), s
+    finally:
+        dev.close()
+
+
+def test_derivatives_on_a_device_mesh_equal_the_host_meshs():
+    """On a device mesh the derivatives' face records are built on their first use (from the face arrays read in place):
+    the tangent and the adjoint equal those of the same mesh given as host arrays, to the bit."""
+    say("derivatives on a device mesh")
+    m = cases.voronoi(8, 3)
+    q = cases.rough_state(m)
+    dev = hydro.FiniteVolume(hydro.DeviceMesh(m), GAMMA)
+    host = hydro.FiniteVolume(m, GAMMA)
+    try:
+        for fv in (dev, host):
+            fv.set_state(q)
+        dt = host.time_step(0.3)
+        assert dev.time_step(0.3) == dt
+        rng = np.random.default_rng(5)
+        qbar = rng.standard_normal(np.shape(q))
+        assert np.array_equal(dev.step_adjoint(dt, qbar), host.step_adjoint(dt, qbar))
+        dq = rng.standard_normal(np.shape(q))
+        assert np.array_equal(dev.step_tangent(dt, dq), host.step_tangent(dt, dq))
+    finally:
+        dev.close()
+        host.close()
