@@ -476,6 +476,7 @@ int flux_and_update(luna_fv* fv, int pass, double flux_dt, double apply_dt, doub
     const int riemann = fv->riemann(), order = fv->opt.order;
     if (prims) *prims = next_prims && fv->opt.update != LUNA_UPDATE_PER_CELL;
     if (fv->opt.update == LUNA_UPDATE_PER_CELL) {
+        if (int rc = records(fv)) return rc;  // the per-cell loop reads each face's record
         LUNA_HIP(timed(fv->prof, s, pass < 0 || pass == 2 ? "fluxes_and_update" : pass == 0 ? "fluxes_and_update_pass1" : "fluxes_and_update_pass2", [&] {
             return fv->D == 3 ? luna::launch_fluxes_and_update<3>(s, g, r, gas, riemann, order, pass, flux_dt, apply_dt, unit, drift, count)
                               : luna::launch_fluxes_and_update<2>(s, g, r, gas, riemann, order, pass, flux_dt, apply_dt, unit, drift, count);

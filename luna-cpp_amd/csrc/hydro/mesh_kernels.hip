@@ -865,22 +865,37 @@ __device__ __forceinline__ void load_side(const Geo& g, const Rows& r, int i, Si
 __device__ __forceinline__ long long wrap_key(long long k, long long p);
 
 // One face's flux times |A|: FaceFlux::operator() (:1415-1434) statically, MovingFlux::operator() (:1466-1490) on a moving
-// step; the positivity fallbacks of the owner's and the neighbour's side added to count_o and count_j.
-template <int D, int Pass, int Riemann, int Prec>
+// step; the positivity fallbacks of the owner's and the neighbour's side added to count_o and count_j. Record: read the
+// face from its 64 B record (the per-cell loop, which gathers faces through the lists: one line each), else from the
+// mesh's face arrays in place (the per-face loop, which streams them).
+template <int D, int Pass, int Riemann, int Prec, bool Record = false>
 __device__ __forceinline__ void flux_of_face(const Geo& g, const Rows& r, const Gas& gas, int order, double dt, double unit, int f, double* F,
                                              int& count_o, int& count_j) {
     constexpr int V = D + 2;
     using FA = FaceAt<D>;
     {
-        // the mesh's face arrays in place: A, the face centroid (relative to the owner's generator), owner, neighbour
+        // A, the face centroid (relative to the owner's generator), owner, neighbour, |A|
         double rec[FACE_ROW - 1];
+        int o, j;
+        double magnitude;
+        if constexpr (Record) {
+            const double* face = g.face + (size_t)f * FACE_ROW;
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            rec[FA::A + k] = g.area[(size_t)f * D + k];
-            rec[FA::FC + k] = g.fcen[(size_t)f * D + k];
+            for (int k = 0; k < FACE_ROW - 1; ++k) rec[k] = face[k];
+            const int2 pair = *reinterpret_cast<const int2*>(face + FA::PAIR);
+            o = pair.x;
+            j = pair.y;
+            magnitude = rec[FA::MAG];
+        } else {
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                rec[FA::A + k] = g.area[(size_t)f * D + k];
+                rec[FA::FC + k] = g.fcen[(size_t)f * D + k];
+            }
+            o = g.owner[f];
+            j = g.neighbour[f];
+            magnitude = sqrt(dot<D>(rec + FA::A, rec + FA::A));  // |A| as k_geo_faces writes it
         }
-        const int o = g.owner[f], j = g.neighbour[f];
-        const double magnitude = sqrt(dot<D>(rec + FA::A, rec + FA::A));  // |A| (as k_geo_faces writes it)
 #pragma unroll
         for (int v = 0; v < V; ++v) F[v] = 0;
         if (magnitude != 0) {
@@ -1011,7 +1026,7 @@ __global__ __launch_bounds__(CELL_BLOCK) void k_fluxes_and_update(Geo g, Rows r,
             const int e = g.list[base + t * SLICE], f = e >= 0 ? e : ~e;
             double F[V];
             int co = 0, cj = 0;
-            flux_of_face<D, Pass, Riemann, Prec>(g, r, gas, order, flux_dt, unit, f, F, co, cj);
+            flux_of_face<D, Pass, Riemann, Prec, true>(g, r, gas, order, flux_dt, unit, f, F, co, cj);
             count += e >= 0 ? co : cj;
             const double sign = e >= 0 ? 1.0 : -1.0;
 #pragma unroll
