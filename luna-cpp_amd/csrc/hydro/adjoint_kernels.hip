@@ -247,20 +247,36 @@ __device__ void fit_cell(const Geo& g, int i, const double* dW, bool limit, doub
     }
 }
 
+// A face from the mesh's arrays, read in place: A, the face centroid, owner and neighbour, and |A| as k_geo_faces takes
+// it. The per-face kernels stream these (the rows' gathers then wait on the 8 B of the cell pair, not on a 64 B record).
+template <int D>
+struct FaceIn {
+    double A[D], fc[D], mag;
+    int2 pair;
+};
+template <int D>
+__device__ __forceinline__ void face_in(const Geo& g, int f, FaceIn<D>& x) {
+    x.pair = make_int2(g.owner[f], g.neighbour[f]);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        x.A[k] = g.area[(size_t)f * D + k];
+        x.fc[k] = g.fcen[(size_t)f * D + k];
+    }
+    x.mag = sqrt(dot<D>(x.A, x.A));
+}
+
 // One face's two sides as the static flux computes them (half step + g.r, positivity fallback), with their fallback
 // flags, offsets, the unit normal and |A|.
 template <int D>
-__device__ __forceinline__ void face_sides(const Geo& g, const Rows& r, const Gas& gas, int f, double dt, double* Ls, double* Rs,
+__device__ __forceinline__ void face_sides(const Geo& g, const Rows& r, const Gas& gas, const FaceIn<D>& face, double dt, double* Ls, double* Rs,
                                            bool* fallback, double* ro, double* rj, double* n, double& magnitude) {
     constexpr int V = D + 2;
     using C = CellAt<D>;
-    using FA = FaceAt<D>;
-    const double* face = g.face + (size_t)f * FACE_ROW;
-    const int2 pair = *reinterpret_cast<const int2*>(face + FA::PAIR);
-    magnitude = face[FA::MAG];
+    const int2 pair = face.pair;
+    magnitude = face.mag;
     const double inv = 1.0 / magnitude;
 #pragma unroll
-    for (int k = 0; k < D; ++k) n[k] = face[FA::A + k] * inv;
+    for (int k = 0; k < D; ++k) n[k] = face.A[k] * inv;
     const double* ro_row = g.cell + (size_t)pair.x * CELL_ROW;
     const double* rj_row = g.cell + (size_t)pair.y * CELL_ROW;
     double xo[D], xj[D], d[D];
@@ -270,8 +286,8 @@ __device__ __forceinline__ void face_sides(const Geo& g, const Rows& r, const Ga
     const double half = 0.5 * dt, wf = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        ro[k] = face[FA::FC + k] - ro_row[C::C + k];
-        rj[k] = face[FA::FC + k] + d[k] - rj_row[C::C + k];
+        ro[k] = face.fc[k] - ro_row[C::C + k];
+        rj[k] = face.fc[k] + d[k] - rj_row[C::C + k];
         ro[k] = ro[k] + wf * half;
         rj[k] = rj[k] + wf * half;
     }
@@ -369,16 +385,17 @@ __global__ __launch_bounds__(BLOCK) void k_flux_tangent(Geo g, Rows r, Gas gas, 
     // the face's flux and its tangent side by side (2V doubles a face), so that the update gathers one row per entry
     double* Fv = dflux + (size_t)f * 2 * V;
     double* dF = Fv + V;
-    const double* face = g.face + (size_t)f * FACE_ROW;
-    if (face[FaceAt<D>::MAG] == 0) {
+    FaceIn<D> face;
+    face_in<D>(g, f, face);
+    if (face.mag == 0) {
 #pragma unroll
         for (int v = 0; v < V; ++v) Fv[v] = dF[v] = 0;
         return;
     }
     double Ls[V], Rs[V], ro[D], rj[D], n[D], magnitude, dLs[V], dRs[V];
     bool fallback[2];
-    face_sides<D>(g, r, gas, f, dt, Ls, Rs, fallback, ro, rj, n, magnitude);
-    const int2 pair = *reinterpret_cast<const int2*>(face + FaceAt<D>::PAIR);
+    face_sides<D>(g, r, gas, face, dt, Ls, Rs, fallback, ro, rj, n, magnitude);
+    const int2 pair = face.pair;
     const int cells[2] = {pair.x, pair.y};
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
@@ -479,12 +496,13 @@ __global__ __launch_bounds__(BLOCK) void k_face_adjoint(Geo g, Rows r, Gas gas, 
     double Lbar[V], Rbar[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) Lbar[v] = Rbar[v] = 0;
-    const double* face = g.face + (size_t)f * FACE_ROW;
-    if (face[FaceAt<D>::MAG] != 0) {
+    FaceIn<D> face;
+    face_in<D>(g, f, face);
+    if (face.mag != 0) {
         double Ls[V], Rs[V], ro[D], rj[D], n[D], magnitude, Fbar[V];
         bool fallback[2];
-        face_sides<D>(g, r, gas, f, dt, Ls, Rs, fallback, ro, rj, n, magnitude);
-        const int2 pair = *reinterpret_cast<const int2*>(face + FaceAt<D>::PAIR);
+        face_sides<D>(g, r, gas, face, dt, Ls, Rs, fallback, ro, rj, n, magnitude);
+        const int2 pair = face.pair;
 #pragma unroll
         for (int v = 0; v < V; ++v) Fbar[v] = (dt * qbar[(size_t)pair.y * V + v] - dt * qbar[(size_t)pair.x * V + v]) * magnitude;
         const double w0[D] = {};
